@@ -1,0 +1,134 @@
+/*
+ * tclbx.h -- C ABI of the MI355X lattice Boltzmann node-kernel engine.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes
+ * (TCLBX_OK = 0, TCLBX_ERR = -1; tclbx_last_error() gives the message of the
+ * calling thread's last failure).  No call exits the process (the reference
+ * exits on any device error, src/cross.cu:24-32).  Calls on one lattice are
+ * not re-entrant; every call is synchronous on return, like the reference's
+ * Lattice methods.  Arrays are x-fastest (x + nx*(y + ny*z)); vector
+ * quantities are 3 doubles per node, interleaved.
+ *
+ * Each entry point names the reference interface it replaces.
+ */
+#ifndef TCLBX_H
+#define TCLBX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TCLBX_OK 0
+#define TCLBX_ERR (-1)
+#define TCLBX_ITERATION_STOP 1 /* src/CommonHandler.h:10 */
+
+/* iteration types, src/Lattice.h.Rt:19-31 */
+#define TCLBX_ITER_NORM 0x001
+#define TCLBX_ITER_GLOBS 0x010
+#define TCLBX_ITER_LASTGLOB 0x080
+
+typedef struct tclbx_lattice tclbx_lattice;
+
+/* Message of the calling thread's last failed call ("" if none). */
+const char* tclbx_last_error(void);
+int tclbx_version(void);
+
+/* Models compiled in: "d2q9_SRT", "d2q9_SRT_drag", "d3q19".
+ * Replaces the per-model binaries CLB/<model>/main (src/makefile.main.Rt:331-349). */
+int tclbx_model_count(void);
+const char* tclbx_model_name(int i);
+
+/* Lattice(lbRegion, MPIInfo, ns) -- src/Lattice.cu.Rt:87-151, called from
+ * Solver::InitAll (src/Solver.cpp.Rt:378-402).  storage: "f64", "f32",
+ * "f32s", "f16s", "bf16s" (the reference's --with-storage, fixed at build
+ * time there, src/configure.ac:213-233).  One subdomain, periodic on every
+ * axis unless NodeTypes say otherwise; nx is not padded. */
+int tclbx_lattice_create(const char* model, int nx, int ny, int nz, const char* storage,
+                         tclbx_lattice** out);
+int tclbx_lattice_destroy(tclbx_lattice* h);
+
+/* Model_m registry lookups by model name, src/Lists.h:13-165 and
+ * Things::by_name (src/Things.h:15-23): the id, or -1 if either name is
+ * unknown.  They need no GPU. */
+int tclbx_setting_id(const char* model, const char* name);
+int tclbx_zone_setting_id(const char* model, const char* name);
+int tclbx_global_id(const char* model, const char* name);
+int tclbx_quantity_id(const char* model, const char* name, int* is_vector);
+/* counts: Q, Globals, Settings, zonal Settings, Quantities */
+int tclbx_model_info(const char* model, int* Q, int* nglobals, int* nsettings, int* nzonal,
+                     int* nquantities);
+/* name of entry `id` (NULL if out of range) */
+const char* tclbx_global_name(const char* model, int id);
+const char* tclbx_setting_name(const char* model, int id);
+const char* tclbx_zone_setting_name(const char* model, int id);
+const char* tclbx_quantity_name(const char* model, int id);
+/* default value of setting `id` (the reference applies defaults of the
+ * non-deriving settings only, src/Solver.cpp.Rt:277-280) */
+int tclbx_setting_default(const char* model, int id, double* value);
+/* NodeType flag value and group mask (NODE_<name>, src/conf.R:600-699).
+ * Group names: "BOUNDARY", "COLLISION", "ADDITIONALS", "SETTINGZONE", "ALL". */
+int tclbx_nodetype(const char* model, const char* name, uint16_t* value, uint16_t* mask);
+int tclbx_nodetype_group(const char* model, const char* group, uint16_t* mask, int* shift);
+
+/* Lattice::SetSetting (src/Lattice.cu.Rt:1350-1358): a derived setting
+ * also sets its target (nu -> omega, InletPressure -> InletDensity). */
+int tclbx_set_setting(tclbx_lattice* h, int id, double value);
+int tclbx_get_setting(const tclbx_lattice* h, int id, double* value);
+/* ZoneSettings::set (src/ZoneSettings.h:61-80); zone = -1: every zone. */
+int tclbx_set_zone_setting(tclbx_lattice* h, int id, int zone, double value);
+
+/* Lattice::FlagOverwrite / GetFlags (src/Lattice.cu.Rt:1040-1089) over the
+ * box [dx,dx+nx) x [dy,dy+ny) x [dz,dz+nz); the caller owns the buffer. */
+int tclbx_flag_overwrite(tclbx_lattice* h, const uint16_t* flags, int dx, int dy, int dz, int nx, int ny,
+                         int nz);
+int tclbx_get_flags(tclbx_lattice* h, int dx, int dy, int dz, int nx, int ny, int nz, uint16_t* out);
+
+/* Lattice::Init (src/Lattice.cu.Rt:799-821): Init action, Iter = 0,
+ * Globals cleared. */
+int tclbx_init(tclbx_lattice* h);
+/* Lattice::Iterate(niter, iter_type) (src/Lattice.cu.Rt:900-989): niter
+ * steps; with TCLBX_ITER_LASTGLOB the Globals of the last step are
+ * computed and become tclbx_get_globals(); TCLBX_ITER_GLOBS accumulates
+ * Globals over every step. */
+int tclbx_iterate(tclbx_lattice* h, int niter, int iter_type);
+/* Lattice::globals[] after calcGlobals (src/Lattice.cu.Rt:1299-1315), in
+ * the model's Globals order (SUM..., Objective, MAX...). */
+int tclbx_get_globals(const tclbx_lattice* h, double* out, int n);
+int tclbx_get_iteration(const tclbx_lattice* h, int* iter);
+
+/* Lattice::GetQuantity(q, region, tab, scale) (src/Lattice.cu.Rt:1184-1222):
+ * evaluated on the pulled populations of the current snapshot. */
+int tclbx_get_quantity(tclbx_lattice* h, int q, int dx, int dy, int dz, int nx, int ny, int nz,
+                       double* out, double scale);
+/* Get_<f> / Set_<f> (src/Lattice.cu.Rt:1133-1172): density q of the whole
+ * lattice, as real values (storage shift removed / applied). */
+int tclbx_get_density(tclbx_lattice* h, int q, double* out);
+int tclbx_set_density(tclbx_lattice* h, int q, const double* in);
+/* saveSolution / loadSolution (src/Lattice.cu.Rt:708-769): raw densities
+ * of the current snapshot in the storage format; -1 if the file cannot be
+ * opened or is short. */
+int tclbx_save(tclbx_lattice* h, const char* path);
+int tclbx_load(tclbx_lattice* h, const char* path);
+
+/* Event timing of the last tclbx_iterate and the running total, in ms --
+ * what MainCallback measures for MLBUps (src/main.cpp:65-157). */
+int tclbx_get_timing(const tclbx_lattice* h, double* last_ms, double* total_ms);
+/* Algorithmic HBM bytes per node and step: 2*Q*sizeof(storage) + 2 (flag),
+ * the reference's GB/s formula with the storage width (src/main.cpp:127). */
+int tclbx_bytes_per_node(const tclbx_lattice* h, double* bytes);
+/* Launch shape: threads per block along x (64, 128 or 256); 0 = default. */
+int tclbx_set_block(tclbx_lattice* h, int block);
+/* Steps per captured hipGraph replay in tclbx_iterate (0 = no graph). */
+int tclbx_set_graph_steps(tclbx_lattice* h, int steps);
+
+/* Measured device copy bandwidth: `reps` copies of `bytes` (rounded down
+ * to 16) between two device buffers by a plain vector-copy kernel;
+ * *gbps = 2*bytes*reps / time (read + write), *ms = time of one copy. */
+int tclbx_copy_bandwidth(long long bytes, int reps, double* gbps, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

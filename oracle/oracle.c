@@ -1,0 +1,404 @@
+/*
+ * oracle.c -- CPU restatement of TCLB's node kernel (see oracle.h).
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP path, never the product.
+ *
+ * Lattice traversal follows the reference CPU build: every node of the
+ * nx*ny*nz box, x fastest (src/LatticeContainer.inc.cpp.Rt:206-222); pull
+ * streaming f_i(x) = in_i(x - e_i) with periodic wrap on all three axes
+ * (src/LatticeAccess.inc.cpp.Rt:319-335 with self-aliased margins,
+ * src/Lattice.cu.Rt:445-453); A/B snapshots swapped every step
+ * (src/Lattice.cu.Rt:958-964).
+ */
+#include "oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------ storage conversions --- */
+static uint32_t f2u(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static float u2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+
+/* IEEE binary32 -> binary16, round to nearest even (as __float2half_rn). */
+uint16_t orc_f32_to_f16(float v)
+{
+    uint32_t x = f2u(v);
+    uint32_t sign = (x >> 16) & 0x8000u;
+    uint32_t ax = x & 0x7fffffffu;
+    if (ax >= 0x7f800000u) /* inf or nan */
+        return (uint16_t)(sign | 0x7c00u | (ax > 0x7f800000u ? 0x200u : 0));
+    if (ax >= 0x477ff000u) /* rounds to >= 65520 -> inf */
+        return (uint16_t)(sign | 0x7c00u);
+    if (ax < 0x38800000u) { /* below 2^-14: subnormal half or zero */
+        if (ax < 0x33000000u) return (uint16_t)sign; /* < 2^-25 rounds to 0 */
+        {
+            uint32_t mant = (ax & 0x7fffffu) | 0x800000u;
+            int e = (int)(ax >> 23);           /* biased exponent 102..112 */
+            int shift = 126 - e;               /* 14..24 */
+            uint32_t q = mant >> shift;
+            uint32_t rem = mant & ((1u << shift) - 1);
+            uint32_t half = 1u << (shift - 1);
+            if (rem > half || (rem == half && (q & 1))) q++;
+            return (uint16_t)(sign | q);
+        }
+    }
+    {
+        uint32_t e = (ax >> 23) - 112;         /* rebias 127 -> 15 */
+        uint32_t mant = ax & 0x7fffffu;
+        uint32_t q = (e << 10) | (mant >> 13);
+        uint32_t rem = mant & 0x1fffu;
+        if (rem > 0x1000u || (rem == 0x1000u && (q & 1))) q++;
+        return (uint16_t)(sign | q);
+    }
+}
+
+float orc_f16_to_f32(uint16_t h)
+{
+    uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+    uint32_t e = (h >> 10) & 0x1f, m = h & 0x3ff;
+    if (e == 0) {
+        /* zero or subnormal: m * 2^-24 is exact in float */
+        float v = (float)m * 5.9604644775390625e-08f;
+        return sign ? -v : v;
+    }
+    if (e == 31) return u2f(sign | 0x7f800000u | (m << 13));
+    return u2f(sign | ((e + 112) << 23) | (m << 13));
+}
+
+/* binary32 -> bfloat16, round to nearest even (as __float2bfloat16_rn). */
+uint16_t orc_f32_to_bf16(float v)
+{
+    uint32_t x = f2u(v);
+    if ((x & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((x >> 16) | 0x40u);
+    x += 0x7fffu + ((x >> 16) & 1u);
+    return (uint16_t)(x >> 16);
+}
+
+float orc_bf16_to_f32(uint16_t h) { return u2f((uint32_t)h << 16); }
+
+/* ----------------------------------------------------- node arithmetic --- */
+#define REAL double
+#define SFX(n) n##_d
+#include "oracle_core.inc"
+#undef REAL
+#undef SFX
+#define REAL float
+#define SFX(n) n##_f
+#include "oracle_core.inc"
+#undef REAL
+#undef SFX
+
+/* ------------------------------------------------------------ lattice --- */
+static const int d2q9_e[9][3] = {{0,0,0},{1,0,0},{0,1,0},{-1,0,0},{0,-1,0},
+                                 {1,1,0},{-1,1,0},{-1,-1,0},{1,-1,0}};
+static const double d2q9_w[9] = {4./9, 1./9, 1./9, 1./9, 1./9,
+                                 1./36, 1./36, 1./36, 1./36};
+static const double d3q19_w[19] = {1./3, 1./18, 1./18, 1./18, 1./18, 1./18, 1./18,
+                                   1./36, 1./36, 1./36, 1./36, 1./36, 1./36,
+                                   1./36, 1./36, 1./36, 1./36, 1./36, 1./36};
+
+struct orc_lattice {
+    int model, nx, ny, nz, storage, Q, nglob;
+    size_t N;
+    void* buf[2];               /* A/B snapshots, storage format */
+    int cur;                    /* snapshot holding the current state */
+    uint16_t* flags;
+    double params[16];
+    double* zone;               /* [nzones][ncols] */
+    int nzones, ncols;
+    int e[19][3];
+    double w[19];
+};
+
+static size_t esize(int storage)
+{
+    switch (storage) {
+    case ORC_F64: return 8;
+    case ORC_F32: case ORC_F32S: return 4;
+    default: return 2;
+    }
+}
+
+static int shifted(int storage) { return storage == ORC_F32S || storage == ORC_F16S || storage == ORC_BF16S; }
+
+/* storage -> real (as double; exact for every format) */
+static double ld(const orc_lattice* L, const void* b, size_t i, int q)
+{
+    switch (L->storage) {
+    case ORC_F64: return ((const double*)b)[i];
+    case ORC_F32: return (double)((const float*)b)[i];
+    case ORC_F32S: return (double)(((const float*)b)[i] + (float)L->w[q]);
+    case ORC_F16S: return (double)(orc_f16_to_f32(((const uint16_t*)b)[i]) + (float)L->w[q]);
+    default: return (double)(orc_bf16_to_f32(((const uint16_t*)b)[i]) + (float)L->w[q]);
+    }
+}
+
+/* real -> storage; for reduced formats v is a float value */
+static void st(const orc_lattice* L, void* b, size_t i, int q, double v)
+{
+    switch (L->storage) {
+    case ORC_F64: ((double*)b)[i] = v; break;
+    case ORC_F32: ((float*)b)[i] = (float)v; break;
+    case ORC_F32S: ((float*)b)[i] = (float)v - (float)L->w[q]; break;
+    case ORC_F16S: ((uint16_t*)b)[i] = orc_f32_to_f16((float)v - (float)L->w[q]); break;
+    default: ((uint16_t*)b)[i] = orc_f32_to_bf16((float)v - (float)L->w[q]); break;
+    }
+}
+
+orc_lattice* orc_create(int model, int nx, int ny, int nz, int storage)
+{
+    orc_lattice* L = (orc_lattice*)calloc(1, sizeof(orc_lattice));
+    int q, a;
+    if (!L) return NULL;
+    d3q19_sparse_init_d();
+    d3q19_sparse_init_f();
+    L->model = model; L->nx = nx; L->ny = ny; L->nz = nz; L->storage = storage;
+    L->N = (size_t)nx * ny * nz;
+    if (model == ORC_D3Q19) {
+        L->Q = 19; L->nglob = 26;
+        for (q = 0; q < 19; q++) {
+            L->e[q][0] = M_d[3][q]; L->e[q][1] = M_d[5][q]; L->e[q][2] = M_d[7][q];
+            L->w[q] = d3q19_w[q];
+        }
+    } else {
+        L->Q = 9; L->nglob = (model == ORC_D2Q9_SRT_DRAG) ? 3 : 1;
+        for (q = 0; q < 9; q++) {
+            for (a = 0; a < 3; a++) L->e[q][a] = d2q9_e[q][a];
+            L->w[q] = d2q9_w[q];
+        }
+    }
+    L->buf[0] = calloc(L->N * L->Q, esize(storage));
+    L->buf[1] = calloc(L->N * L->Q, esize(storage));
+    L->flags = (uint16_t*)calloc(L->N, sizeof(uint16_t));
+    L->nzones = 1; L->ncols = 32;
+    L->zone = (double*)calloc(32, sizeof(double));
+    return L;
+}
+
+void orc_destroy(orc_lattice* L)
+{
+    if (!L) return;
+    free(L->buf[0]); free(L->buf[1]); free(L->flags); free(L->zone); free(L);
+}
+
+int orc_q(const orc_lattice* L) { return L->Q; }
+int orc_nglobals(const orc_lattice* L) { return L->nglob; }
+
+void orc_set_params(orc_lattice* L, const double* p, int n)
+{
+    int i;
+    for (i = 0; i < 16; i++) L->params[i] = i < n ? p[i] : 0.0;
+}
+
+void orc_set_zone_table(orc_lattice* L, const double* table, int nzones, int ncols)
+{
+    int z, c;
+    free(L->zone);
+    /* pad the columns to 32 so every model can index its InObj weights */
+    L->ncols = ncols < 32 ? 32 : ncols;
+    L->nzones = nzones;
+    L->zone = (double*)calloc((size_t)nzones * L->ncols, sizeof(double));
+    for (z = 0; z < nzones; z++)
+        for (c = 0; c < ncols; c++) L->zone[(size_t)z * L->ncols + c] = table[(size_t)z * ncols + c];
+}
+
+void orc_set_flags(orc_lattice* L, const uint16_t* flags)
+{
+    memcpy(L->flags, flags, L->N * sizeof(uint16_t));
+}
+
+static int zone_shift(const orc_lattice* L) { return L->model == ORC_D3Q19 ? 6 : 4; }
+
+static const double* zone_row(const orc_lattice* L, uint16_t nt)
+{
+    int z = nt >> zone_shift(L);
+    if (z >= L->nzones) z = 0;
+    return L->zone + (size_t)z * L->ncols;
+}
+
+void orc_init(orc_lattice* L)
+{
+    size_t n;
+    int q;
+    for (n = 0; n < L->N; n++) {
+        const double* zr = zone_row(L, L->flags[n]);
+        if (L->model == ORC_D3Q19) {
+            /* Init(): SetEquilibrum(1.0, InletVelocity, 0, 0), Dynamics.c.Rt:219-221 */
+            if (L->storage == ORC_F64) {
+                double f[19];
+                d3q19_feq_d(f, 1.0, L->params[1], 0., 0.);
+                for (q = 0; q < 19; q++) st(L, L->buf[0], (size_t)q * L->N + n, q, f[q]);
+            } else {
+                float f[19];
+                d3q19_feq_f(f, 1.0f, (float)L->params[1], 0.f, 0.f);
+                for (q = 0; q < 19; q++) st(L, L->buf[0], (size_t)q * L->N + n, q, f[q]);
+            }
+        } else {
+            /* Init(): SetEquilibrium(Density, Velocity_x, Velocity_y), Dynamics.c:26-38 */
+            if (L->storage == ORC_F64) {
+                double f[9];
+                d2q9_feq_d(f, L->params[1], zr[1], zr[2]);
+                for (q = 0; q < 9; q++) st(L, L->buf[0], (size_t)q * L->N + n, q, f[q]);
+            } else {
+                float f[9];
+                d2q9_feq_f(f, (float)L->params[1], (float)zr[1], (float)zr[2]);
+                for (q = 0; q < 9; q++) st(L, L->buf[0], (size_t)q * L->N + n, q, f[q]);
+            }
+        }
+    }
+    L->cur = 0;
+}
+
+static int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
+/* pull the Q populations of node (x,y,z) from snapshot b */
+static void pull(const orc_lattice* L, const void* b, int x, int y, int z, double* f)
+{
+    int q;
+    for (q = 0; q < L->Q; q++) {
+        int xs = wrap(x - L->e[q][0], L->nx);
+        int ys = wrap(y - L->e[q][1], L->ny);
+        int zs = wrap(z - L->e[q][2], L->nz);
+        size_t i = (size_t)q * L->N + (size_t)xs + (size_t)L->nx * ((size_t)ys + (size_t)L->ny * zs);
+        f[q] = L->storage == ORC_F64 ? ((const double*)b)[i] : ld(L, b, i, q);
+    }
+}
+
+/* z-planes [z0, z1) of one step */
+static void step_planes(orc_lattice* L, long double* glob, int z0, int z1)
+{
+    const void* in = L->buf[L->cur];
+    void* out = L->buf[1 - L->cur];
+    int x, y, z, q;
+    int drag = L->model == ORC_D2Q9_SRT_DRAG;
+    for (z = z0; z < z1; z++)
+        for (y = 0; y < L->ny; y++)
+            for (x = 0; x < L->nx; x++) {
+                size_t n = (size_t)x + (size_t)L->nx * ((size_t)y + (size_t)L->ny * z);
+                uint16_t nt = L->flags[n];
+                const double* zr = zone_row(L, nt);
+                double fd[19];
+                pull(L, in, x, y, z, fd);
+                if (L->storage == ORC_F64) {
+                    if (L->model == ORC_D3Q19) d3q19_node_d(fd, nt, L->params, zr, glob);
+                    else d2q9_node_d(fd, nt, L->params[0], L->params[1], zr, drag, glob);
+                    for (q = 0; q < L->Q; q++) st(L, out, (size_t)q * L->N + n, q, fd[q]);
+                } else {
+                    float ff[19];
+                    for (q = 0; q < L->Q; q++) ff[q] = (float)fd[q];
+                    if (L->model == ORC_D3Q19) d3q19_node_f(ff, nt, L->params, zr, glob);
+                    else d2q9_node_f(ff, nt, (float)L->params[0], (float)L->params[1], zr, drag, glob);
+                    for (q = 0; q < L->Q; q++) st(L, out, (size_t)q * L->N + n, q, ff[q]);
+                }
+            }
+}
+
+static int g_threads = 1;
+void orc_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
+
+/* One step.  With several threads the z-planes are split into contiguous
+ * chunks (OpenMP, as the reference's --with-openmp CPU build runs its
+ * kernel loops); the Globals of the chunks are added in chunk order. */
+static void step(orc_lattice* L, long double* glob)
+{
+    int nt = g_threads < L->nz ? g_threads : L->nz;
+    if (nt <= 1) {
+        step_planes(L, glob, 0, L->nz);
+    } else {
+        long double part[64][32];
+        int t, g;
+        if (nt > 64) nt = 64;
+        #pragma omp parallel for num_threads(nt) schedule(static, 1)
+        for (t = 0; t < nt; t++) {
+            int z0 = (int)((long long)L->nz * t / nt), z1 = (int)((long long)L->nz * (t + 1) / nt);
+            for (g = 0; g < 32; g++) part[t][g] = 0.0L;
+            step_planes(L, glob ? part[t] : NULL, z0, z1);
+        }
+        if (glob)
+            for (t = 0; t < nt; t++)
+                for (g = 0; g < 32; g++) {
+                    if (L->model == ORC_D3Q19 && g == 25) { if (part[t][g] > glob[g]) glob[g] = part[t][g]; }
+                    else glob[g] += part[t][g];
+                }
+    }
+    L->cur = 1 - L->cur;
+}
+
+void orc_iterate(orc_lattice* L, int niter, int glob_last, double* globals_out)
+{
+    int i, g;
+    long double glob[32];
+    for (i = 0; i < niter; i++) {
+        int last = glob_last && i == niter - 1;
+        if (last) for (g = 0; g < 32; g++) glob[g] = 0.0L;
+        step(L, last ? glob : NULL);
+    }
+    if (glob_last && niter > 0 && globals_out)
+        for (g = 0; g < L->nglob; g++) globals_out[g] = (double)glob[g];
+}
+
+void orc_get_f(const orc_lattice* L, double* out)
+{
+    size_t i;
+    int q;
+    for (q = 0; q < L->Q; q++)
+        for (i = 0; i < L->N; i++) out[(size_t)q * L->N + i] = ld(L, L->buf[L->cur], (size_t)q * L->N + i, q);
+}
+
+void orc_set_f(orc_lattice* L, const double* in)
+{
+    size_t i;
+    int q;
+    for (q = 0; q < L->Q; q++)
+        for (i = 0; i < L->N; i++) {
+            double v = in[(size_t)q * L->N + i];
+            if (L->storage != ORC_F64) v = (double)(float)v;
+            st(L, L->buf[L->cur], (size_t)q * L->N + i, q, v);
+        }
+}
+
+void orc_get_quantity(const orc_lattice* L, int which, double* out)
+{
+    int x, y, z, q;
+    for (z = 0; z < L->nz; z++)
+        for (y = 0; y < L->ny; y++)
+            for (x = 0; x < L->nx; x++) {
+                size_t n = (size_t)x + (size_t)L->nx * ((size_t)y + (size_t)L->ny * z);
+                uint16_t nt = L->flags[n];
+                double f[19], rho = 0, jx = 0, jy = 0, jz = 0;
+                pull(L, L->buf[L->cur], x, y, z, f);
+                if (L->model == ORC_D3Q19) {
+                    /* getRho / getU / getP, Dynamics.c.Rt:23-40 */
+                    for (q = 0; q < 19; q++) {
+                        rho += f[q];
+                        jx += f[q] * L->e[q][0]; jy += f[q] * L->e[q][1]; jz += f[q] * L->e[q][2];
+                    }
+                    if (which == 0) out[n] = rho;
+                    else if (which == 2) out[n] = (rho - 1.0) / 3.0;
+                    else {
+                        double ux = jx / rho, uy = jy / rho, uz = jz / rho;
+                        if ((nt & 0x1C) == 0x0C) ux = uy = uz = 0;
+                        out[3*n] = ux; out[3*n+1] = uy; out[3*n+2] = uz;
+                    }
+                } else {
+                    /* getRho / getU, Dynamics.c:102-116 */
+                    const double* zr = zone_row(L, nt);
+                    double d = f[8]+f[7]+f[6]+f[5]+f[4]+f[3]+f[2]+f[1]+f[0];
+                    if (which == 0) out[n] = d;
+                    else if (which == 1) {
+                        out[3*n]   = (( f[8]-f[7]-f[6]+f[5]-f[3]+f[1] )/d + zr[3]*0.5 );
+                        out[3*n+1] = ((-f[8]-f[7]+f[6]+f[5]-f[4]+f[2] )/d + zr[4]*0.5 );
+                        out[3*n+2] = 0;
+                    } else out[n] = 0;
+                }
+            }
+}
+
+/* ---------------------------------------------- single-node exports --- */
+void orc_d2q9_feq(double feq[9], double d, double ux, double uy) { d2q9_feq_d(feq, d, ux, uy); }
+void orc_d3q19_feq(double feq[19], double rho, double jx, double jy, double jz) { d3q19_feq_d(feq, rho, jx, jy, jz); }
+void orc_d3q19_collide(double f[19], const double* params)
+{
+    static const double zone[32] = {0};
+    d3q19_mrt_d(f, 0x20, params, zone, NULL);
+}

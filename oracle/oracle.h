@@ -1,0 +1,83 @@
+/*
+ * oracle.h -- CPU restatement of TCLB's per-time-step node kernel.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it.  The product (libtclbx.so) never links or calls it.
+ *
+ * It restates, in plain C, what the reference CPU build (CROSS_CPU, double)
+ * computes for the models in scope:
+ *   d2q9_SRT       models/flow/d2q9_SRT/Dynamics.c:26-176, Dynamics.R:6-37
+ *   d2q9_SRT_drag  the same, plus Ladd momentum-exchange drag on Solid nodes
+ *                  in the form of models/flow/d3q27/Dynamics.c.Rt:68-82
+ *   d3q19          models/flow/d3q19/Dynamics.c.Rt:23-290, Dynamics.R:1-95,
+ *                  src/lib/d3q19.R:24-96
+ * with pull streaming on a periodic torus (src/LatticeAccess.inc.cpp.Rt:
+ * 319-347, src/Lattice.cu.Rt:445-453), the Run() node order of
+ * src/cuda.cu.Rt:322-327 and Globals on the last step of a segment
+ * (src/Lattice.cu.Rt:900-989).
+ *
+ * Storage emulation (src/LatticeAccess.inc.cpp.Rt:14-35): F64 keeps double
+ * arithmetic; the reduced formats compute in float and store (f - w_i) in
+ * float / IEEE half / bfloat16 with round-to-nearest-even.
+ *
+ * Globals are accumulated in long double (x87 80-bit) so the oracle's sums
+ * are at least as accurate as any summation order of the device.
+ */
+#ifndef TCLBX_ORACLE_H
+#define TCLBX_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_D2Q9_SRT = 0, ORC_D2Q9_SRT_DRAG = 1, ORC_D3Q19 = 2 };
+enum { ORC_F64 = 0, ORC_F32 = 1, ORC_F32S = 2, ORC_F16S = 3, ORC_BF16S = 4 };
+
+/* d2q9 parameter vector:   [omega, Density]
+ * d2q9 zone columns:       [Velocity, Velocity_x, Velocity_y, GravitationX,
+ *                           GravitationY, (drag model only) DragInObj, LiftInObj]
+ * d3q19 parameter vector:  [omega, InletVelocity, InletDensity, ForceX, ForceY,
+ *                           ForceZ, S_e, S_eps, S_q, S_pi, S_m]   (S_* <= 0: the
+ *                           reference's fixed rates)
+ * d3q19 zone columns:      24 <Global>InObj weights in Globals order.        */
+
+typedef struct orc_lattice orc_lattice;
+
+orc_lattice* orc_create(int model, int nx, int ny, int nz, int storage);
+void orc_destroy(orc_lattice* L);
+int  orc_q(const orc_lattice* L);
+int  orc_nglobals(const orc_lattice* L);
+void orc_set_params(orc_lattice* L, const double* p, int n);
+void orc_set_zone_table(orc_lattice* L, const double* table, int nzones, int ncols);
+void orc_set_flags(orc_lattice* L, const uint16_t* flags);
+void orc_init(orc_lattice* L);
+/* niter steps; if glob_last, Globals of the last step go to globals_out. */
+void orc_iterate(orc_lattice* L, int niter, int glob_last, double* globals_out);
+/* f as real values (storage -> real + shift), layout [q][z][y][x]. */
+void orc_get_f(const orc_lattice* L, double* out);
+void orc_set_f(orc_lattice* L, const double* in);
+/* quantity: 0 = Rho, 1 = U (3 doubles per node, interleaved), 2 = P.
+ * Evaluated like the reference get<Q> kernels: on pulled (streamed)
+ * populations of the current snapshot, without boundary or collision. */
+void orc_get_quantity(const orc_lattice* L, int which, double* out);
+
+/* threads of the step loop (OpenMP over z-planes); 1 by default */
+void orc_set_threads(int n);
+
+/* storage conversions, exported so tests can pin them */
+uint16_t orc_f32_to_f16(float v);
+float    orc_f16_to_f32(uint16_t h);
+uint16_t orc_f32_to_bf16(float v);
+float    orc_bf16_to_f32(uint16_t h);
+
+/* single-node kernels exported for closed-form tests (double arithmetic) */
+void orc_d2q9_feq(double feq[9], double d, double ux, double uy);
+void orc_d3q19_feq(double feq[19], double rho, double jx, double jy, double jz);
+void orc_d3q19_collide(double f[19], const double* params);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,201 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY.  The oracle restates the reference CPU build's node
+kernel (see oracle.h for the reference files it follows) and is the checker
+of the HIP path.  Only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg import this module; the product never does.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+D2Q9_SRT, D2Q9_SRT_DRAG, D3Q19 = 0, 1, 2
+F64, F32, F32S, F16S, BF16S = 0, 1, 2, 3, 4
+MODELS = {"d2q9_SRT": D2Q9_SRT, "d2q9_SRT_drag": D2Q9_SRT_DRAG, "d3q19": D3Q19}
+STORAGE = {"f64": F64, "f32": F32, "f32s": F32S, "f16s": F16S, "bf16s": BF16S}
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-B", "-C", HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        vp, ip, dp = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)
+        L.orc_create.restype = vp
+        L.orc_create.argtypes = [ip, ip, ip, ip, ip]
+        L.orc_destroy.argtypes = [vp]
+        L.orc_q.argtypes = [vp]
+        L.orc_nglobals.argtypes = [vp]
+        L.orc_set_params.argtypes = [vp, dp, ip]
+        L.orc_set_zone_table.argtypes = [vp, dp, ip, ip]
+        L.orc_set_flags.argtypes = [vp, ctypes.POINTER(ctypes.c_uint16)]
+        L.orc_init.argtypes = [vp]
+        L.orc_iterate.argtypes = [vp, ip, ip, dp]
+        L.orc_get_f.argtypes = [vp, dp]
+        L.orc_set_f.argtypes = [vp, dp]
+        L.orc_get_quantity.argtypes = [vp, ip, dp]
+        L.orc_f32_to_f16.restype = ctypes.c_uint16
+        L.orc_f32_to_f16.argtypes = [ctypes.c_float]
+        L.orc_f16_to_f32.restype = ctypes.c_float
+        L.orc_f16_to_f32.argtypes = [ctypes.c_uint16]
+        L.orc_f32_to_bf16.restype = ctypes.c_uint16
+        L.orc_f32_to_bf16.argtypes = [ctypes.c_float]
+        L.orc_bf16_to_f32.restype = ctypes.c_float
+        L.orc_bf16_to_f32.argtypes = [ctypes.c_uint16]
+        L.orc_d2q9_feq.argtypes = [dp, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        L.orc_d3q19_feq.argtypes = [dp] + [ctypes.c_double] * 4
+        L.orc_d3q19_collide.argtypes = [dp, dp]
+        L.orc_set_threads.argtypes = [ip]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class OracleLattice:
+    """One lattice of the oracle; layout of f is [q][z][y][x]."""
+
+    def __init__(self, model, nx, ny, nz=1, storage="f64"):
+        self.model = MODELS[model] if isinstance(model, str) else model
+        self.storage = STORAGE[storage] if isinstance(storage, str) else storage
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.h = lib().orc_create(self.model, nx, ny, nz, self.storage)
+        self.Q = lib().orc_q(self.h)
+        self.nglob = lib().orc_nglobals(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    @property
+    def shape(self):
+        return (self.nz, self.ny, self.nx)
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        lib().orc_set_params(self.h, _dp(p), len(p))
+
+    def set_zone_table(self, table):
+        t = np.ascontiguousarray(np.atleast_2d(table), dtype=np.float64)
+        lib().orc_set_zone_table(self.h, _dp(t), t.shape[0], t.shape[1])
+
+    def set_flags(self, flags):
+        f = np.ascontiguousarray(flags, dtype=np.uint16).reshape(-1)
+        assert f.size == self.nx * self.ny * self.nz
+        lib().orc_set_flags(self.h, f.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)))
+
+    def init(self):
+        lib().orc_init(self.h)
+
+    def iterate(self, n, globals_last=True):
+        g = np.zeros(max(self.nglob, 1), dtype=np.float64)
+        lib().orc_iterate(self.h, n, 1 if globals_last else 0, _dp(g))
+        return g
+
+    def get_f(self):
+        out = np.empty((self.Q,) + self.shape, dtype=np.float64)
+        lib().orc_get_f(self.h, _dp(out))
+        return out
+
+    def set_f(self, f):
+        a = np.ascontiguousarray(f, dtype=np.float64)
+        assert a.size == self.Q * self.nx * self.ny * self.nz
+        lib().orc_set_f(self.h, _dp(a))
+
+    def quantity(self, which):
+        w = {"Rho": 0, "U": 1, "P": 2}[which] if isinstance(which, str) else which
+        n = self.nx * self.ny * self.nz
+        out = np.empty(n * (3 if w == 1 else 1), dtype=np.float64)
+        lib().orc_get_quantity(self.h, w, _dp(out))
+        return out.reshape(self.shape + ((3,) if w == 1 else ()))
+
+
+def set_threads(n):
+    """Threads of the oracle's step loop (OpenMP over z-planes)."""
+    lib().orc_set_threads(int(n))
+
+
+def d2q9_feq(d, ux, uy):
+    out = np.empty(9)
+    lib().orc_d2q9_feq(_dp(out), d, ux, uy)
+    return out
+
+
+def d3q19_feq(rho, jx, jy, jz):
+    out = np.empty(19)
+    lib().orc_d3q19_feq(_dp(out), rho, jx, jy, jz)
+    return out
+
+
+def d3q19_collide(f, params):
+    a = np.array(f, dtype=np.float64)
+    p = np.zeros(16)
+    p[: len(params)] = params
+    lib().orc_d3q19_collide(_dp(a), _dp(p))
+    return a
+
+
+D3Q19_PARAMS = ["omega", "InletVelocity", "InletDensity", "ForceX", "ForceY", "ForceZ",
+                "MRT_Se", "MRT_Seps", "MRT_Sq", "MRT_Spi", "MRT_Sm"]
+
+
+def resolved_settings(model, settings):
+    """Setting values after applying `settings` in order the way
+    Lattice::SetSetting does (src/Lattice.cu.Rt:1350-1358): defaults of the
+    non-deriving settings, then each value and its derived target."""
+    from tclb_amd import Model
+    m = Model(model)
+    vals = {n: 0.0 for n in m.settings}
+    for n in m.settings:
+        if n not in ("nu", "InletPressure"):
+            vals[n] = m.setting_default(n)
+    for n, v in settings:
+        vals[n] = v
+        if n == "nu":
+            vals["omega"] = 1.0 / (3 * v + 0.5)
+        if n == "InletPressure":
+            vals["InletDensity"] = 1.0 + v * 3
+    return vals
+
+
+def lattice_for_case(case, storage="f64"):
+    """OracleLattice holding the same case as case.build(storage)."""
+    from tclb_amd import Model
+    m = Model(case.model)
+    L = OracleLattice(case.model, case.nx, case.ny, case.nz, storage)
+    vals = resolved_settings(case.model, case.settings)
+    if case.model == "d3q19":
+        L.set_params([vals[n] for n in D3Q19_PARAMS])
+    else:
+        L.set_params([vals["omega"], vals["Density"]])
+    zshift = m.group("SETTINGZONE")[1]
+    used = int((np.asarray(case.flags) >> zshift).max())
+    rows = max(used, max([z for _, z, _ in case.zonal], default=0)) + 1
+    table = np.zeros((rows, m.nzonal))
+    for n, z, v in case.zonal:
+        c = m.zonal_settings.index(n)
+        if z < 0:
+            table[:, c] = v
+        else:
+            table[z, c] = v
+    L.set_zone_table(table)
+    L.set_flags(case.flags)
+    L.init()
+    if case.init_f is not None:
+        L.set_f(case.init_f)
+    return L

@@ -1,0 +1,320 @@
+"""tclb_amd -- MI355X-native lattice Boltzmann node-kernel engine.
+
+Python mirror of the C ABI in include/tclbx.h (libtclbx.so, HIP kernels for
+gfx950).  The Lattice class follows the reference's Lattice interface
+(src/Lattice.h.Rt:39-191): SetSetting, FlagOverwrite, Init, Iterate,
+globals, GetQuantity, Get/Set densities, save/load.
+
+The library is loaded from this directory; there is no fallback: if it is
+missing or fails to load, every call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libtclbx.so")
+
+ITER_NORM = 0x001
+ITER_GLOBS = 0x010
+ITER_LASTGLOB = 0x080
+
+# exported symbols declared in include/tclbx.h
+SYMBOLS = [
+    "tclbx_last_error", "tclbx_version", "tclbx_model_count", "tclbx_model_name",
+    "tclbx_lattice_create", "tclbx_lattice_destroy", "tclbx_model_info", "tclbx_setting_default",
+    "tclbx_setting_id", "tclbx_zone_setting_id", "tclbx_global_id", "tclbx_global_name",
+    "tclbx_setting_name", "tclbx_zone_setting_name", "tclbx_quantity_name", "tclbx_quantity_id",
+    "tclbx_nodetype", "tclbx_nodetype_group", "tclbx_set_setting", "tclbx_get_setting",
+    "tclbx_set_zone_setting", "tclbx_flag_overwrite", "tclbx_get_flags", "tclbx_init",
+    "tclbx_iterate", "tclbx_get_globals", "tclbx_get_iteration", "tclbx_get_quantity",
+    "tclbx_get_density", "tclbx_set_density", "tclbx_save", "tclbx_load", "tclbx_get_timing",
+    "tclbx_bytes_per_node", "tclbx_set_block", "tclbx_set_graph_steps", "tclbx_copy_bandwidth",
+]
+
+
+class TclbxError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libtclbx.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise TclbxError(f"{LIBPATH} is not built; run __graft_entry__.build() or make -C tclb_amd/csrc")
+    L = ctypes.CDLL(LIBPATH)
+    c_int, c_double, c_char_p, vp = ctypes.c_int, ctypes.c_double, ctypes.c_char_p, ctypes.c_void_p
+    dp = ctypes.POINTER(ctypes.c_double)
+    u16p = ctypes.POINTER(ctypes.c_uint16)
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.tclbx_last_error.restype = c_char_p
+    L.tclbx_model_name.restype = c_char_p
+    L.tclbx_model_name.argtypes = [c_int]
+    L.tclbx_lattice_create.argtypes = [c_char_p, c_int, c_int, c_int, c_char_p, ctypes.POINTER(vp)]
+    L.tclbx_lattice_destroy.argtypes = [vp]
+    L.tclbx_model_info.argtypes = [c_char_p, ip, ip, ip, ip, ip]
+    for n in ["tclbx_setting_id", "tclbx_zone_setting_id", "tclbx_global_id"]:
+        getattr(L, n).argtypes = [c_char_p, c_char_p]
+    for n in ["tclbx_global_name", "tclbx_setting_name", "tclbx_zone_setting_name", "tclbx_quantity_name"]:
+        getattr(L, n).argtypes = [c_char_p, c_int]
+        getattr(L, n).restype = c_char_p
+    L.tclbx_setting_default.argtypes = [c_char_p, c_int, dp]
+    L.tclbx_quantity_id.argtypes = [c_char_p, c_char_p, ip]
+    L.tclbx_nodetype.argtypes = [c_char_p, c_char_p, u16p, u16p]
+    L.tclbx_nodetype_group.argtypes = [c_char_p, c_char_p, u16p, ip]
+    L.tclbx_set_setting.argtypes = [vp, c_int, c_double]
+    L.tclbx_get_setting.argtypes = [vp, c_int, dp]
+    L.tclbx_set_zone_setting.argtypes = [vp, c_int, c_int, c_double]
+    L.tclbx_flag_overwrite.argtypes = [vp, u16p] + [c_int] * 6
+    L.tclbx_get_flags.argtypes = [vp] + [c_int] * 6 + [u16p]
+    L.tclbx_init.argtypes = [vp]
+    L.tclbx_iterate.argtypes = [vp, c_int, c_int]
+    L.tclbx_get_globals.argtypes = [vp, dp, c_int]
+    L.tclbx_get_iteration.argtypes = [vp, ip]
+    L.tclbx_get_quantity.argtypes = [vp, c_int] + [c_int] * 6 + [dp, c_double]
+    L.tclbx_get_density.argtypes = [vp, c_int, dp]
+    L.tclbx_set_density.argtypes = [vp, c_int, dp]
+    L.tclbx_save.argtypes = [vp, c_char_p]
+    L.tclbx_load.argtypes = [vp, c_char_p]
+    L.tclbx_get_timing.argtypes = [vp, dp, dp]
+    L.tclbx_bytes_per_node.argtypes = [vp, dp]
+    L.tclbx_set_block.argtypes = [vp, c_int]
+    L.tclbx_set_graph_steps.argtypes = [vp, c_int]
+    L.tclbx_copy_bandwidth.argtypes = [ctypes.c_longlong, c_int, dp, dp]
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise TclbxError(lib().tclbx_last_error().decode())
+    return rc
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def copy_bandwidth(nbytes=1 << 30, reps=10):
+    """(GB/s counting read + write, ms per copy) of a plain device copy."""
+    g, t = ctypes.c_double(), ctypes.c_double()
+    _check(lib().tclbx_copy_bandwidth(int(nbytes), int(reps), ctypes.byref(g), ctypes.byref(t)))
+    return g.value, t.value
+
+
+def models():
+    L = lib()
+    return [L.tclbx_model_name(i).decode() for i in range(L.tclbx_model_count())]
+
+
+class Model:
+    """The model contract (Model_m, src/Lists.h:13-165) of one compiled-in
+    model: settings, zonal settings, Globals, Quantities, NodeType flags.
+    Needs no GPU."""
+
+    def __init__(self, name):
+        L = lib()
+        self.name = name
+        b = name.encode()
+        vals = [ctypes.c_int() for _ in range(5)]
+        _check(L.tclbx_model_info(b, *[ctypes.byref(v) for v in vals]))
+        self.Q, self.nglobals, self.nsettings, self.nzonal, self.nquantities = [v.value for v in vals]
+        self.settings = [L.tclbx_setting_name(b, i).decode() for i in range(self.nsettings)]
+        self.zonal_settings = [L.tclbx_zone_setting_name(b, i).decode() for i in range(self.nzonal)]
+        self.global_names = [L.tclbx_global_name(b, i).decode() for i in range(self.nglobals)]
+        self.quantities = [L.tclbx_quantity_name(b, i).decode() for i in range(self.nquantities)]
+
+    def setting_id(self, name):
+        return lib().tclbx_setting_id(self.name.encode(), name.encode())
+
+    def zone_setting_id(self, name):
+        return lib().tclbx_zone_setting_id(self.name.encode(), name.encode())
+
+    def global_id(self, name):
+        return lib().tclbx_global_id(self.name.encode(), name.encode())
+
+    def quantity_id(self, name):
+        isv = ctypes.c_int()
+        q = lib().tclbx_quantity_id(self.name.encode(), name.encode(), ctypes.byref(isv))
+        return q, bool(isv.value)
+
+    def setting_default(self, name):
+        v = ctypes.c_double()
+        _check(lib().tclbx_setting_default(self.name.encode(), self.setting_id(name), ctypes.byref(v)))
+        return v.value
+
+    def nodetype(self, name):
+        """(value, group mask) of NODE_<name>."""
+        v, m = ctypes.c_uint16(), ctypes.c_uint16()
+        _check(lib().tclbx_nodetype(self.name.encode(), name.encode(), ctypes.byref(v), ctypes.byref(m)))
+        return v.value, m.value
+
+    def group(self, name):
+        """(mask, shift) of NodeType group `name`."""
+        m, s = ctypes.c_uint16(), ctypes.c_int()
+        _check(lib().tclbx_nodetype_group(self.name.encode(), name.encode(), ctypes.byref(m), ctypes.byref(s)))
+        return m.value, s.value
+
+    def flag(self, *names, zone=0):
+        """NodeType value of the OR of `names` plus a zone number."""
+        v = 0
+        for n in names:
+            v |= self.nodetype(n)[0]
+        return v | (zone << self.group("SETTINGZONE")[1])
+
+
+class Lattice:
+    """One lattice on the GPU (one subdomain, periodic unless NodeTypes say
+    otherwise), mirroring the reference Lattice (src/Lattice.h.Rt:39-191)."""
+
+    def __init__(self, model, nx, ny, nz=1, storage="f64"):
+        L = lib()
+        h = ctypes.c_void_p()
+        _check(L.tclbx_lattice_create(model.encode(), nx, ny, nz, storage.encode(), ctypes.byref(h)))
+        self._h = h
+        self.model, self.storage = model, storage
+        self.nx, self.ny, self.nz = nx, ny, nz
+        m = Model(model)
+        self.spec = m
+        self.Q, self.nglobals = m.Q, m.nglobals
+        self.settings, self.zonal_settings = m.settings, m.zonal_settings
+        self.global_names, self.quantities = m.global_names, m.quantities
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().tclbx_lattice_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def shape(self):
+        return (self.nz, self.ny, self.nx)
+
+    # --- model registry ---------------------------------------------------
+    def nodetype(self, name):
+        return self.spec.nodetype(name)
+
+    def group(self, name):
+        return self.spec.group(name)
+
+    # --- settings ---------------------------------------------------------
+    def set_setting(self, name, value):
+        i = self.spec.setting_id(name)
+        if i < 0:
+            raise TclbxError(f"unknown setting {name}")
+        _check(lib().tclbx_set_setting(self._h, i, float(value)))
+
+    def get_setting(self, name):
+        i = self.spec.setting_id(name)
+        if i < 0:
+            raise TclbxError(f"unknown setting {name}")
+        v = ctypes.c_double()
+        _check(lib().tclbx_get_setting(self._h, i, ctypes.byref(v)))
+        return v.value
+
+    def set_zone_setting(self, name, value, zone=-1):
+        i = self.spec.zone_setting_id(name)
+        if i < 0:
+            raise TclbxError(f"unknown zonal setting {name}")
+        _check(lib().tclbx_set_zone_setting(self._h, i, int(zone), float(value)))
+
+    # --- flags ------------------------------------------------------------
+    def set_flags(self, flags, region=None):
+        f = np.ascontiguousarray(flags, dtype=np.uint16)
+        if region is None:
+            region = (0, 0, 0, self.nx, self.ny, self.nz)
+        dx, dy, dz, nx, ny, nz = region
+        assert f.size == nx * ny * nz
+        _check(lib().tclbx_flag_overwrite(self._h, f.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)),
+                                          dx, dy, dz, nx, ny, nz))
+
+    def get_flags(self, region=None):
+        if region is None:
+            region = (0, 0, 0, self.nx, self.ny, self.nz)
+        dx, dy, dz, nx, ny, nz = region
+        out = np.empty((nz, ny, nx), dtype=np.uint16)
+        _check(lib().tclbx_get_flags(self._h, dx, dy, dz, nx, ny, nz,
+                                     out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16))))
+        return out
+
+    # --- run --------------------------------------------------------------
+    def init(self):
+        _check(lib().tclbx_init(self._h))
+
+    def iterate(self, n, iter_type=ITER_NORM | ITER_LASTGLOB):
+        _check(lib().tclbx_iterate(self._h, int(n), int(iter_type)))
+
+    @property
+    def globals(self):
+        out = np.zeros(self.nglobals)
+        _check(lib().tclbx_get_globals(self._h, _dp(out), self.nglobals))
+        return out
+
+    def global_dict(self):
+        return dict(zip(self.global_names, self.globals))
+
+    @property
+    def iteration(self):
+        v = ctypes.c_int()
+        _check(lib().tclbx_get_iteration(self._h, ctypes.byref(v)))
+        return v.value
+
+    def quantity(self, name, region=None, scale=1.0):
+        q, isv = self.spec.quantity_id(name)
+        if q < 0:
+            raise TclbxError(f"unknown quantity {name}")
+        if region is None:
+            region = (0, 0, 0, self.nx, self.ny, self.nz)
+        dx, dy, dz, nx, ny, nz = region
+        shape = (nz, ny, nx) + ((3,) if isv else ())
+        out = np.empty(shape, dtype=np.float64)
+        _check(lib().tclbx_get_quantity(self._h, q, dx, dy, dz, nx, ny, nz, _dp(out), float(scale)))
+        return out
+
+    def get_f(self):
+        out = np.empty((self.Q,) + self.shape, dtype=np.float64)
+        for q in range(self.Q):
+            _check(lib().tclbx_get_density(self._h, q, _dp(out[q])))
+        return out
+
+    def set_f(self, f):
+        a = np.ascontiguousarray(f, dtype=np.float64).reshape((self.Q,) + self.shape)
+        for q in range(self.Q):
+            plane = np.ascontiguousarray(a[q])
+            _check(lib().tclbx_set_density(self._h, q, _dp(plane)))
+
+    def save(self, path):
+        _check(lib().tclbx_save(self._h, str(path).encode()))
+
+    def load(self, path):
+        _check(lib().tclbx_load(self._h, str(path).encode()))
+
+    def timing(self):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        _check(lib().tclbx_get_timing(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def bytes_per_node(self):
+        v = ctypes.c_double()
+        _check(lib().tclbx_bytes_per_node(self._h, ctypes.byref(v)))
+        return v.value
+
+    def set_block(self, block):
+        _check(lib().tclbx_set_block(self._h, int(block)))
+
+    def set_graph_steps(self, steps):
+        _check(lib().tclbx_set_graph_steps(self._h, int(steps)))

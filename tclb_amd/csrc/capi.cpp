@@ -1,0 +1,288 @@
+// capi.cpp -- the extern "C" boundary (include/tclbx.h) over tclbx::Lattice.
+#include "../../include/tclbx.h"
+
+#include <exception>
+#include <string>
+
+#include "lattice.h"
+#include "model.h"
+
+using namespace tclbx;
+
+struct tclbx_lattice {
+    Lattice* lat;
+};
+
+namespace {
+thread_local std::string g_err;
+
+int fail(const std::string& msg) {
+    g_err = msg;
+    return TCLBX_ERR;
+}
+
+template <class F>
+int guard(F&& f) {
+    try {
+        g_err.clear();
+        return f();
+    } catch (const std::exception& e) {
+        return fail(e.what());
+    } catch (...) {
+        return fail("unknown error");
+    }
+}
+
+#define CHECK_H(h) \
+    if (!(h) || !(h)->lat) return fail("null lattice handle")
+}  // namespace
+
+extern "C" {
+
+const char* tclbx_last_error(void) { return g_err.c_str(); }
+int tclbx_version(void) { return 1; }
+
+int tclbx_model_count(void) { return MODEL_COUNT; }
+const char* tclbx_model_name(int i) {
+    const ModelDef* m = model_by_id(i);
+    return m ? m->name.c_str() : nullptr;
+}
+
+int tclbx_lattice_create(const char* model, int nx, int ny, int nz, const char* storage, tclbx_lattice** out) {
+    return guard([&]() {
+        if (!out) return fail("null output pointer");
+        *out = nullptr;
+        const ModelDef* m = model ? model_by_name(model) : nullptr;
+        if (!m) return fail(std::string("unknown model: ") + (model ? model : "(null)"));
+        int st = storage_by_name(storage ? storage : "f64");
+        if (st < 0) return fail(std::string("unknown storage: ") + storage);
+        Lattice* L = new Lattice(m, nx, ny, nz, st);
+        *out = new tclbx_lattice{L};
+        return TCLBX_OK;
+    });
+}
+
+int tclbx_lattice_destroy(tclbx_lattice* h) {
+    if (!h) return TCLBX_OK;
+    delete h->lat;
+    delete h;
+    return TCLBX_OK;
+}
+
+int tclbx_model_info(const char* model, int* Q, int* nglobals, int* nsettings, int* nzonal, int* nquantities) {
+    const ModelDef* m = model ? model_by_name(model) : nullptr;
+    if (!m) return fail(std::string("unknown model: ") + (model ? model : "(null)"));
+    if (Q) *Q = m->Q;
+    if (nglobals) *nglobals = (int)m->globals.size();
+    if (nsettings) *nsettings = (int)m->settings.size();
+    if (nzonal) *nzonal = (int)m->zonal.size();
+    if (nquantities) *nquantities = (int)m->quantities.size();
+    return TCLBX_OK;
+}
+
+namespace {
+const ModelDef* M(const char* model) { return model ? model_by_name(model) : nullptr; }
+}  // namespace
+
+int tclbx_setting_id(const char* model, const char* name) {
+    const ModelDef* m = M(model);
+    return (m && name) ? m->setting_id(name) : -1;
+}
+int tclbx_zone_setting_id(const char* model, const char* name) {
+    const ModelDef* m = M(model);
+    return (m && name) ? m->zonal_id(name) : -1;
+}
+int tclbx_global_id(const char* model, const char* name) {
+    const ModelDef* m = M(model);
+    return (m && name) ? m->global_id(name) : -1;
+}
+int tclbx_quantity_id(const char* model, const char* name, int* is_vector) {
+    const ModelDef* m = M(model);
+    if (!m || !name) return -1;
+    int q = m->quantity_id(name);
+    if (q >= 0 && is_vector) *is_vector = m->quantities[q].vector ? 1 : 0;
+    return q;
+}
+const char* tclbx_global_name(const char* model, int id) {
+    const ModelDef* m = M(model);
+    return (m && id >= 0 && id < (int)m->globals.size()) ? m->globals[id].name.c_str() : nullptr;
+}
+const char* tclbx_setting_name(const char* model, int id) {
+    const ModelDef* m = M(model);
+    return (m && id >= 0 && id < (int)m->settings.size()) ? m->settings[id].name.c_str() : nullptr;
+}
+const char* tclbx_zone_setting_name(const char* model, int id) {
+    const ModelDef* m = M(model);
+    return (m && id >= 0 && id < (int)m->zonal.size()) ? m->zonal[id].name.c_str() : nullptr;
+}
+const char* tclbx_quantity_name(const char* model, int id) {
+    const ModelDef* m = M(model);
+    return (m && id >= 0 && id < (int)m->quantities.size()) ? m->quantities[id].name.c_str() : nullptr;
+}
+int tclbx_setting_default(const char* model, int id, double* value) {
+    const ModelDef* m = M(model);
+    if (!m) return fail("unknown model");
+    if (id < 0 || id >= (int)m->settings.size()) return fail("setting id out of range");
+    if (value) *value = m->settings[id].def;
+    return TCLBX_OK;
+}
+int tclbx_nodetype(const char* model, const char* name, uint16_t* value, uint16_t* mask) {
+    const ModelDef* m = M(model);
+    if (!m) return fail("unknown model");
+    const NodeTypeDef* t = name ? m->nodetype(name) : nullptr;
+    if (!t) return fail(std::string("unknown node type: ") + (name ? name : "(null)"));
+    if (value) *value = (uint16_t)t->value;
+    if (mask) *mask = (uint16_t)t->mask;
+    return TCLBX_OK;
+}
+int tclbx_nodetype_group(const char* model, const char* group, uint16_t* mask, int* shift) {
+    const ModelDef* m = M(model);
+    if (!m) return fail("unknown model");
+    const GroupDef* g = group ? m->group(group) : nullptr;
+    if (!g) return fail(std::string("unknown node type group: ") + (group ? group : "(null)"));
+    if (mask) *mask = (uint16_t)g->mask;
+    if (shift) *shift = g->shift;
+    return TCLBX_OK;
+}
+
+int tclbx_set_setting(tclbx_lattice* h, int id, double value) {
+    CHECK_H(h);
+    return guard([&]() { h->lat->SetSetting(id, value); return TCLBX_OK; });
+}
+int tclbx_get_setting(const tclbx_lattice* h, int id, double* value) {
+    CHECK_H(h);
+    if (id < 0 || id >= (int)h->lat->model->settings.size()) return fail("setting id out of range");
+    if (value) *value = h->lat->GetSetting(id);
+    return TCLBX_OK;
+}
+int tclbx_set_zone_setting(tclbx_lattice* h, int id, int zone, double value) {
+    CHECK_H(h);
+    return guard([&]() { h->lat->SetZoneSetting(id, zone, value); return TCLBX_OK; });
+}
+
+int tclbx_flag_overwrite(tclbx_lattice* h, const uint16_t* flags, int dx, int dy, int dz, int nx, int ny, int nz) {
+    CHECK_H(h);
+    if (!flags) return fail("null flags");
+    return guard([&]() { h->lat->FlagOverwrite(flags, Region{dx, dy, dz, nx, ny, nz}); return TCLBX_OK; });
+}
+int tclbx_get_flags(tclbx_lattice* h, int dx, int dy, int dz, int nx, int ny, int nz, uint16_t* out) {
+    CHECK_H(h);
+    if (!out) return fail("null output");
+    return guard([&]() { h->lat->GetFlags(Region{dx, dy, dz, nx, ny, nz}, out); return TCLBX_OK; });
+}
+
+int tclbx_init(tclbx_lattice* h) {
+    CHECK_H(h);
+    return guard([&]() { h->lat->Init(); return TCLBX_OK; });
+}
+int tclbx_iterate(tclbx_lattice* h, int niter, int iter_type) {
+    CHECK_H(h);
+    return guard([&]() { h->lat->Iterate(niter, iter_type); return TCLBX_OK; });
+}
+int tclbx_get_globals(const tclbx_lattice* h, double* out, int n) {
+    CHECK_H(h);
+    const auto& g = h->lat->globals();
+    if (!out || n < (int)g.size()) return fail("globals buffer too small");
+    for (size_t i = 0; i < g.size(); i++) out[i] = g[i];
+    return TCLBX_OK;
+}
+int tclbx_get_iteration(const tclbx_lattice* h, int* iter) {
+    CHECK_H(h);
+    if (iter) *iter = h->lat->iter();
+    return TCLBX_OK;
+}
+
+int tclbx_get_quantity(tclbx_lattice* h, int q, int dx, int dy, int dz, int nx, int ny, int nz, double* out,
+                       double scale) {
+    CHECK_H(h);
+    if (!out) return fail("null output");
+    return guard([&]() { h->lat->GetQuantity(q, Region{dx, dy, dz, nx, ny, nz}, out, scale); return TCLBX_OK; });
+}
+int tclbx_get_density(tclbx_lattice* h, int q, double* out) {
+    CHECK_H(h);
+    if (!out) return fail("null output");
+    return guard([&]() { h->lat->GetDensity(q, out); return TCLBX_OK; });
+}
+int tclbx_set_density(tclbx_lattice* h, int q, const double* in) {
+    CHECK_H(h);
+    if (!in) return fail("null input");
+    return guard([&]() { h->lat->SetDensity(q, in); return TCLBX_OK; });
+}
+int tclbx_save(tclbx_lattice* h, const char* path) {
+    CHECK_H(h);
+    if (!path) return fail("null path");
+    return guard([&]() { return h->lat->Save(path) == 0 ? TCLBX_OK : fail(std::string("cannot write ") + path); });
+}
+int tclbx_load(tclbx_lattice* h, const char* path) {
+    CHECK_H(h);
+    if (!path) return fail("null path");
+    return guard([&]() { return h->lat->Load(path) == 0 ? TCLBX_OK : fail(std::string("cannot read ") + path); });
+}
+
+int tclbx_get_timing(const tclbx_lattice* h, double* last_ms, double* total_ms) {
+    CHECK_H(h);
+    if (last_ms) *last_ms = h->lat->last_iterate_ms();
+    if (total_ms) *total_ms = h->lat->kernel_ms_total();
+    return TCLBX_OK;
+}
+int tclbx_bytes_per_node(const tclbx_lattice* h, double* bytes) {
+    CHECK_H(h);
+    if (bytes) *bytes = (double)h->lat->bytes_per_node_step();
+    return TCLBX_OK;
+}
+int tclbx_set_block(tclbx_lattice* h, int block) {
+    CHECK_H(h);
+    return guard([&]() {
+        int b = block;
+        if (b == 0) {
+            b = 256;
+            long long best = -1;
+            for (int c : {256, 128, 64}) {
+                long long w = (long long)((h->lat->nx + c - 1) / c) * c - h->lat->nx;
+                if (best < 0 || w < best) { best = w; b = c; }
+            }
+        }
+        h->lat->set_block(b);
+        return TCLBX_OK;
+    });
+}
+int tclbx_set_graph_steps(tclbx_lattice* h, int steps) {
+    CHECK_H(h);
+    if (steps < 0 || (steps % 2) != 0) return fail("graph steps must be a non-negative even number");
+    h->lat->set_graph_steps(steps);
+    return TCLBX_OK;
+}
+
+int tclbx_copy_bandwidth(long long bytes, int reps, double* gbps, double* ms) {
+    return guard([&]() {
+        if (bytes < 16 || reps < 1) return fail("copy size and repetitions must be positive");
+        const size_t b = (size_t)bytes / 16 * 16;
+        void *src = nullptr, *dst = nullptr;
+        hipStream_t s = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        hipError_t e = hipMalloc(&src, b);
+        if (e == hipSuccess) e = hipMalloc(&dst, b);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreate(&e0);
+        if (e == hipSuccess) e = hipEventCreate(&e1);
+        if (e == hipSuccess) e = hipMemsetAsync(src, 0, b, s);
+        if (e == hipSuccess) e = launch_copy(src, dst, b, s);  // warm up
+        if (e == hipSuccess) e = hipEventRecord(e0, s);
+        for (int r = 0; r < reps && e == hipSuccess; r++) e = launch_copy(src, dst, b, s);
+        if (e == hipSuccess) e = hipEventRecord(e1, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        float t = 0;
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+        if (src) (void)hipFree(src);
+        if (dst) (void)hipFree(dst);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (s) (void)hipStreamDestroy(s);
+        hip_check(e, "copy bandwidth");
+        if (gbps) *gbps = 2.0 * (double)b * reps / (t * 1e-3) / 1e9;
+        if (ms) *ms = t / reps;
+        return TCLBX_OK;
+    });
+}
+
+}  // extern "C"

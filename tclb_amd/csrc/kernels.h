@@ -1,0 +1,54 @@
+// kernels.h -- host-side interface of the node kernels (kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace tclbx {
+
+// Model parameters passed by value in every launch (no per-step constant
+// upload).  d2q9: [omega, Density].  d3q19: [omega, InletVelocity,
+// InletDensity, ForceX, ForceY, ForceZ, MRT_Se, MRT_Seps, MRT_Sq, MRT_Spi,
+// MRT_Sm].
+constexpr int MAX_PAR = 12;
+constexpr int MAX_GLOBALS = 32;
+
+struct StepArgs {
+    const void* fin;       // current snapshot: Q planes of `pitch` elements
+    void* fout;            // next snapshot
+    const uint16_t* flags; // NodeType per node, x fastest
+    const double* zone;    // zonal settings [zone][zcols]
+    double* gpart;         // per-block partial Globals [nblocks][nglob]
+    long long pitch;       // elements per density plane
+    int nx, ny, nz;
+    int bpr;               // blocks per (y,z) row
+    int zcols, nzones;
+    double par[MAX_PAR];
+};
+
+// One time step: pull streaming + NodeType dispatch + collision (+ Globals).
+hipError_t launch_step(int model, int storage, bool globals, const StepArgs& a, int block,
+                       hipStream_t s);
+// Init(): equilibrium from the settings, written to a.fout.
+hipError_t launch_init(int model, int storage, const StepArgs& a, int block, hipStream_t s);
+// Deterministic reduction of the per-block partials into out[nglob]
+// (fixed order: independent of timing, identical run to run).
+hipError_t launch_reduce_globals(const double* gpart, int nblocks, int nglob, int sum_globals,
+                                 double* out, hipStream_t s);
+// Quantity `q` of the model over the box region [x0,x0+rx)x[y0..)x[z0..),
+// pulled populations of a.fin, times scale; vector quantities write 3
+// doubles per node.
+hipError_t launch_quantity(int model, int storage, int q, const StepArgs& a, int x0, int y0, int z0,
+                           int rx, int ry, int rz, double scale, double* out, hipStream_t s);
+// Density plane q of snapshot `buf` <-> double array over the whole lattice.
+hipError_t launch_get_density(int model, int storage, const void* buf, long long pitch, long long n,
+                              int q, double* out, hipStream_t s);
+hipError_t launch_set_density(int model, int storage, void* buf, long long pitch, long long n,
+                              int q, const double* in, hipStream_t s);
+
+// plain copy of `bytes` (a multiple of 16) on the device
+hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
+
+int kernel_nglobals(int model);
+
+}  // namespace tclbx

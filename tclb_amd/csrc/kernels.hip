@@ -1,0 +1,790 @@
+// kernels.hip -- the per-time-step node kernel for gfx950.
+//
+// One thread = one node.  A launch processes the whole lattice for one step:
+// pull streaming from snapshot A (f_i(x) = A_i(x - e_i), periodic wrap on
+// every axis, as the reference's self-aliased margins give on one rank:
+// src/LatticeAccess.inc.cpp.Rt:319-335, src/Lattice.cu.Rt:445-453), the
+// NodeType boundary switch, the collision, and a store to snapshot B at x
+// (push, src/LatticeAccess.inc.cpp.Rt:337-347).  Node order inside Run()
+// follows src/cuda.cu.Rt:322-327 (Load -> Run -> Save -> Glob).
+//
+// Layout in HBM: SoA, density plane q at base + q*pitch, node n = x + nx*(y +
+// ny*z) inside a plane (the reference's block14[f*N + x + nx*(y+ny*z)],
+// src/Lattice.cu.Rt:1138); pitch is N rounded up so every plane starts on a
+// 256-byte boundary.  Blocks cover x-segments of one (y,z) row, so a
+// wavefront's 64 stores fill whole 128-byte lines and its pulls are 64
+// consecutive elements shifted by at most one.
+//
+// Globals: per-thread FP64 accumulators, reduced by wavefront shuffles, then
+// across the block's wavefronts through LDS, one partial per block; a second
+// launch sums the partials in a fixed order (deterministic, unlike the
+// reference's per-warp atomicAdd, src/cross.hpp:202-218).
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+#include "model.h"
+#include "storage.h"
+
+namespace tclbx {
+
+// ------------------------------------------------------------ lattices ---
+template <int MODEL> struct Lat;
+template <> struct Lat<MODEL_D2Q9_SRT> {
+    static constexpr int Q = 9, G = 1, ZSHIFT = 4;
+    __host__ __device__ static constexpr int ex(int i) { return i == 1 || i == 5 || i == 8 ? 1 : (i == 3 || i == 6 || i == 7 ? -1 : 0); }
+    __host__ __device__ static constexpr int ey(int i) { return i == 2 || i == 5 || i == 6 ? 1 : (i == 4 || i == 7 || i == 8 ? -1 : 0); }
+    __host__ __device__ static constexpr int ez(int) { return 0; }
+    __host__ __device__ static constexpr double w(int i) { return i == 0 ? 4. / 9 : (i < 5 ? 1. / 9 : 1. / 36); }
+};
+template <> struct Lat<MODEL_D2Q9_SRT_DRAG> : Lat<MODEL_D2Q9_SRT> {
+    static constexpr int G = 3;
+};
+template <> struct Lat<MODEL_D3Q19> {
+    static constexpr int Q = 19, G = 26, ZSHIFT = 6;
+    __host__ __device__ static constexpr int ex(int i) {
+        return (i == 1 || i == 7 || i == 9 || i == 11 || i == 13) ? 1
+             : (i == 2 || i == 8 || i == 10 || i == 12 || i == 14) ? -1 : 0;
+    }
+    __host__ __device__ static constexpr int ey(int i) {
+        return (i == 3 || i == 7 || i == 8 || i == 15 || i == 17) ? 1
+             : (i == 4 || i == 9 || i == 10 || i == 16 || i == 18) ? -1 : 0;
+    }
+    __host__ __device__ static constexpr int ez(int i) {
+        return (i == 5 || i == 11 || i == 12 || i == 15 || i == 16) ? 1
+             : (i == 6 || i == 13 || i == 14 || i == 17 || i == 18) ? -1 : 0;
+    }
+    __host__ __device__ static constexpr double w(int i) { return i == 0 ? 1. / 3 : (i < 7 ? 1. / 18 : 1. / 36); }
+};
+
+int kernel_nglobals(int model) {
+    switch (model) {
+    case MODEL_D2Q9_SRT: return Lat<MODEL_D2Q9_SRT>::G;
+    case MODEL_D2Q9_SRT_DRAG: return Lat<MODEL_D2Q9_SRT_DRAG>::G;
+    case MODEL_D3Q19: return Lat<MODEL_D3Q19>::G;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------- d2q9 ---
+// models/flow/d2q9_SRT/Dynamics.c, same operations in the same order.
+template <class R>
+__device__ __forceinline__ void d2q9_feq(R* f_eq, R d, R ux, R uy) {
+    f_eq[0] = ( R(2.) + ( -uy*uy - ux*ux )*R(3.) )*d*R(2.)/R(9.);
+    f_eq[1] = ( R(2.) + ( -uy*uy + ( 1 + ux )*ux*R(2.) )*R(3.) )*d/R(18.);
+    f_eq[2] = ( R(2.) + ( -ux*ux + ( 1 + uy )*uy*R(2.) )*R(3.) )*d/R(18.);
+    f_eq[3] = ( R(2.) + ( -uy*uy + ( -1 + ux )*ux*R(2.) )*R(3.) )*d/R(18.);
+    f_eq[4] = ( R(2.) + ( -ux*ux + ( -1 + uy )*uy*R(2.) )*R(3.) )*d/R(18.);
+    f_eq[5] = ( R(1.) + ( ( 1 + uy )*uy + ( 1 + ux + uy*R(3.) )*ux )*R(3.) )*d/R(36.);
+    f_eq[6] = ( R(1.) + ( ( 1 + uy )*uy + ( -1 + ux - uy*R(3.) )*ux )*R(3.) )*d/R(36.);
+    f_eq[7] = ( R(1.) + ( ( -1 + uy )*uy + ( -1 + ux + uy*R(3.) )*ux )*R(3.) )*d/R(36.);
+    f_eq[8] = ( R(1.) + ( ( -1 + uy )*uy + ( 1 + ux - uy*R(3.) )*ux )*R(3.) )*d/R(36.);
+}
+
+// Run(), Dynamics.c:40-67, plus (DRAG) Ladd momentum exchange on Solid nodes.
+// zone columns: Velocity, Velocity_x, Velocity_y, GravitationX,
+// GravitationY, [DragInObj, LiftInObj].
+template <class R, bool DRAG, bool GLOB>
+__device__ __forceinline__ void d2q9_node(R* f, unsigned nt, R omega, R Density, const double* zr,
+                                          double* g) {
+    R uf;
+    switch (nt & 0x7u) {
+    case 5:
+    case 6:
+        if (DRAG && GLOB && (nt & 0x7u) == 5) {
+            R fx = R(2.0) * (f[1] - f[3] + f[5] - f[6] - f[7] + f[8]);
+            R fy = R(2.0) * (f[2] - f[4] + f[5] + f[6] - f[7] - f[8]);
+            g[0] += (double)fx; g[2] += zr[5];
+            g[1] += (double)fy; g[2] += zr[6];
+        }
+        uf = f[3]; f[3] = f[1]; f[1] = uf;
+        uf = f[4]; f[4] = f[2]; f[2] = uf;
+        uf = f[7]; f[7] = f[5]; f[5] = uf;
+        uf = f[8]; f[8] = f[6]; f[6] = uf;
+        break;
+    case 4: {  // EVelocity
+        R ux0 = (R)zr[1];
+        R rho = ( f[0] + f[2] + f[4] + R(2.)*(f[1] + f[5] + f[8]) ) / (R(1.) + ux0);
+        R ru = rho * ux0;
+        f[3] = f[1] - (R(2.)/R(3.)) * ru;
+        f[7] = f[5] - (R(1.)/R(6.)) * ru + (R(1.)/R(2.))*(f[2] - f[4]);
+        f[6] = f[8] - (R(1.)/R(6.)) * ru + (R(1.)/R(2.))*(f[4] - f[2]);
+        break; }
+    case 2: {  // WPressure
+        R rho = Density;
+        R ux0 = R(-1.) + ( f[0] + f[2] + f[4] + R(2.)*(f[3] + f[7] + f[6]) ) / rho;
+        R ru = rho * ux0;
+        f[1] = f[3] - (R(2.)/R(3.)) * ru;
+        f[5] = f[7] - (R(1.)/R(6.)) * ru + (R(1.)/R(2.))*(f[4] - f[2]);
+        f[8] = f[6] - (R(1.)/R(6.)) * ru + (R(1.)/R(2.))*(f[2] - f[4]);
+        break; }
+    case 3: {  // WVelocity
+        R u0 = (R)zr[1];
+        R rho = ( f[0] + f[2] + f[4] + R(2.)*(f[3] + f[7] + f[6]) ) / (R(1.) - u0);
+        R ru = rho * u0;
+        f[1] = f[3] + (R(2.)/R(3.)) * ru;
+        f[5] = f[7] + (R(1.)/R(6.)) * ru + (R(1.)/R(2.))*(f[4] - f[2]);
+        f[8] = f[6] + (R(1.)/R(6.)) * ru + (R(1.)/R(2.))*(f[2] - f[4]);
+        break; }
+    case 1: {  // EPressure
+        R rho = Density;
+        R ux0 = R(-1.) + ( f[0] + f[2] + f[4] + R(2.)*(f[1] + f[5] + f[8]) ) / rho;
+        R ru = rho * ux0;
+        f[3] = f[1] - (R(2.)/R(3.)) * ru;
+        f[7] = f[5] - (R(1.)/R(6.)) * ru + (R(1.)/R(2.))*(f[2] - f[4]);
+        f[6] = f[8] - (R(1.)/R(6.)) * ru + (R(1.)/R(2.))*(f[4] - f[2]);
+        break; }
+    default: break;
+    }
+    if (nt & 0x8u) {  // BGK
+        R d = f[8]+f[7]+f[6]+f[5]+f[4]+f[3]+f[2]+f[1]+f[0];
+        R ux = (( f[8]-f[7]-f[6]+f[5]-f[3]+f[1] )/d + (R)zr[3]/omega );
+        R uy = ((-f[8]-f[7]+f[6]+f[5]-f[4]+f[2] )/d + (R)zr[4]/omega );
+        R f_eq[9];
+        d2q9_feq(f_eq, d, ux, uy);
+#pragma unroll
+        for (int i = 0; i < 9; i++) f[i] = f[i] + omega*(f_eq[i]-f[i]);
+    }
+}
+
+// --------------------------------------------------------------- d3q19 ---
+// Equilibrium moments (src/lib/d3q19.R:24-62, product form truncated to
+// O(J^2)); entries 0,3,5,7 are the conserved rho, J.
+template <class R>
+__device__ __forceinline__ void d3q19_req(R* q, R rho, R Jx, R Jy, R Jz) {
+    R xx = Jx*Jx/rho, yy = Jy*Jy/rho, zz = Jz*Jz/rho;
+    R j2 = xx + yy + zz;
+    q[1] = R(-11.)*rho + R(19.)*j2;
+    q[2] = R(3.)*rho - R(2.)*j2;
+    q[4] = R(-2.)*Jx/R(3.);
+    q[6] = R(-2.)*Jy/R(3.);
+    q[8] = R(-2.)*Jz/R(3.);
+    q[9] = R(2.)*xx - yy - zz;
+    q[10] = -q[9];
+    q[11] = yy - zz;
+    q[12] = -q[11];
+    q[13] = Jx*Jy/rho;
+    q[14] = Jy*Jz/rho;
+    q[15] = Jx*Jz/rho;
+}
+
+// f = sum_k r_k M[k][i] for the d3q19 moment basis of
+// models/flow/d3q19/Dynamics.R:1-21, factored by symmetry groups.
+template <class R>
+__device__ __forceinline__ void d3q19_from_moments(R* f, const R* r) {
+    R bf = r[0] - R(11.)*r[1] - R(4.)*r[2];
+    R be = r[0] + R(8.)*r[1] + r[2];
+    R d1x = r[3] - R(4.)*r[4], d1y = r[5] - R(4.)*r[6], d1z = r[7] - R(4.)*r[8];
+    R t3 = r[3] + r[4], t5 = r[5] + r[6], t7 = r[7] + r[8];
+    R p9 = R(2.)*r[9] - R(4.)*r[10];
+    R q9 = R(2.)*r[10] - r[9];
+    R e9 = r[9] + r[10];
+    R w11 = r[11] - R(2.)*r[12];
+    R e11 = r[11] + r[12];
+    f[0] = r[0] - R(30.)*r[1] + R(12.)*r[2];
+    f[1] = bf + d1x + p9;
+    f[2] = bf - d1x + p9;
+    f[3] = bf + d1y + q9 + w11;
+    f[4] = bf - d1y + q9 + w11;
+    f[5] = bf + d1z + q9 - w11;
+    f[6] = bf - d1z + q9 - w11;
+    R E0 = be + e9 + e11;
+    f[7]  = E0 + t3 + t5 + r[13] + r[16] - r[17];
+    f[8]  = E0 - t3 + t5 - r[13] - r[16] - r[17];
+    f[9]  = E0 + t3 - t5 - r[13] + r[16] + r[17];
+    f[10] = E0 - t3 - t5 + r[13] - r[16] + r[17];
+    R F0 = be + e9 - e11;
+    f[11] = F0 + t3 + t7 + r[15] - r[16] + r[18];
+    f[12] = F0 - t3 + t7 - r[15] + r[16] + r[18];
+    f[13] = F0 + t3 - t7 - r[15] - r[16] - r[18];
+    f[14] = F0 - t3 - t7 + r[15] + r[16] - r[18];
+    R G0 = be - R(2.)*e9;
+    f[15] = G0 + t5 + t7 + r[14] + r[17] - r[18];
+    f[16] = G0 - t5 + t7 - r[14] - r[17] - r[18];
+    f[17] = G0 + t5 - t7 - r[14] + r[17] + r[18];
+    f[18] = G0 - t5 - t7 + r[14] - r[17] + r[18];
+}
+
+// SetEquilibrum, Dynamics.c.Rt:212-217: f = Req %*% diag(1/Mw) %*% t(M)
+template <class R>
+__device__ __forceinline__ void d3q19_feq(R* f, R rho, R Jx, R Jy, R Jz) {
+    R q[19];
+    d3q19_req(q, rho, Jx, Jy, Jz);
+    q[0] = rho; q[3] = Jx; q[5] = Jy; q[7] = Jz;
+    q[16] = 0; q[17] = 0; q[18] = 0;
+    constexpr double iv[19] = {1./19, 1./2394, 1./252, 1./10, 1./40, 1./10, 1./40, 1./10, 1./40,
+                               1./36, 1./72, 1./12, 1./24, 1./4, 1./4, 1./4, 1./8, 1./8, 1./8};
+#pragma unroll
+    for (int k = 0; k < 19; k++) q[k] = q[k] * R(iv[k]);
+    d3q19_from_moments(f, q);
+}
+
+template <bool GLOB>
+__device__ __forceinline__ void gadd(double* g, int i, double x, const double* zr) {
+    if (GLOB) { g[i] += x; g[24] += zr[i]; }
+}
+
+// CollisionMRT, Dynamics.c.Rt:223-290.  Rates by moment group (0-based):
+// e {1}, eps {2}, q {4,6,8}, nu {9,11,13,14,15}, pi {10,12}, m {16,17,18};
+// the reference uses gamma1 = 1-omega for e/eps/nu/pi and
+// gamma2 = 1-8(2-omega)/(8-omega) for q/m; a positive MRT_S* setting
+// replaces the rate of its group by 1 - S.
+template <class R, bool GLOB>
+__device__ __forceinline__ void d3q19_mrt(R* f, unsigned nt, const double* par, const double* zr,
+                                          double* g) {
+    const R omega = (R)par[0];
+    const R gamma1 = 1 - omega;
+    const R gamma2 = 1 - R(8.0) * ( R(2.0) - omega )/( R(8.0) - omega );
+    const R ge = par[6] > 0 ? 1 - (R)par[6] : gamma1;
+    const R geps = par[7] > 0 ? 1 - (R)par[7] : gamma1;
+    const R gq = par[8] > 0 ? 1 - (R)par[8] : gamma2;
+    const R gpi = par[9] > 0 ? 1 - (R)par[9] : gamma1;
+    const R gm = par[10] > 0 ? 1 - (R)par[10] : gamma2;
+
+    // moments R = f M (Dynamics.R:1-21), by symmetry groups
+    R A = f[1] + f[2] + f[3] + f[4] + f[5] + f[6];
+    R c1 = f[1] + f[2], c2 = f[3] + f[4], c3 = f[5] + f[6];
+    R E = f[7] + f[8] + f[9] + f[10];
+    R F = f[11] + f[12] + f[13] + f[14];
+    R Gs = f[15] + f[16] + f[17] + f[18];
+    R B = E + F + Gs;
+    R m[19];
+    m[0] = f[0] + A + B;
+    m[1] = R(-30.)*f[0] - R(11.)*A + R(8.)*B;
+    m[2] = R(12.)*f[0] - R(4.)*A + B;
+    R ax = f[1] - f[2];
+    R bx = (f[7] - f[8]) + (f[9] - f[10]) + (f[11] - f[12]) + (f[13] - f[14]);
+    R ay = f[3] - f[4];
+    R by = (f[7] + f[8] - f[9] - f[10]) + (f[15] - f[16] + f[17] - f[18]);
+    R az = f[5] - f[6];
+    R bz = (f[11] + f[12] - f[13] - f[14]) + (f[15] + f[16] - f[17] - f[18]);
+    m[3] = ax + bx; m[4] = R(-4.)*ax + bx;
+    m[5] = ay + by; m[6] = R(-4.)*ay + by;
+    m[7] = az + bz; m[8] = R(-4.)*az + bz;
+    R EF = E + F;
+    m[9] = R(2.)*c1 - c2 - c3 + EF - R(2.)*Gs;
+    m[10] = R(-4.)*c1 + R(2.)*(c2 + c3) + EF - R(2.)*Gs;
+    m[11] = c2 - c3 + E - F;
+    m[12] = R(-2.)*c2 + R(2.)*c3 + E - F;
+    m[13] = f[7] - f[8] - f[9] + f[10];
+    m[14] = f[15] - f[16] - f[17] + f[18];
+    m[15] = f[11] - f[12] - f[13] + f[14];
+    m[16] = f[7] - f[8] + f[9] - f[10] - f[11] + f[12] - f[13] + f[14];
+    m[17] = -f[7] - f[8] + f[9] + f[10] + f[15] - f[16] + f[17] - f[18];
+    m[18] = f[11] + f[12] - f[13] - f[14] - f[15] - f[16] + f[17] + f[18];
+
+    const R rho = m[0];
+    R q[19];
+    d3q19_req(q, rho, m[3], m[5], m[7]);
+    m[1] = ge * (m[1] - q[1]);
+    m[2] = geps * (m[2] - q[2]);
+    m[4] = gq * (m[4] - q[4]);
+    m[6] = gq * (m[6] - q[6]);
+    m[8] = gq * (m[8] - q[8]);
+    m[9] = gamma1 * (m[9] - q[9]);
+    m[10] = gpi * (m[10] - q[10]);
+    m[11] = gamma1 * (m[11] - q[11]);
+    m[12] = gpi * (m[12] - q[12]);
+    m[13] = gamma1 * (m[13] - q[13]);
+    m[14] = gamma1 * (m[14] - q[14]);
+    m[15] = gamma1 * (m[15] - q[15]);
+    m[16] = gm * m[16];
+    m[17] = gm * m[17];
+    m[18] = gm * m[18];
+    // forcing after relaxation (Dynamics.c.Rt:256-258)
+    m[3] += rho * (R)par[3];
+    m[5] += rho * (R)par[4];
+    m[7] += rho * (R)par[5];
+    d3q19_req(q, rho, m[3], m[5], m[7]);
+    m[1] += q[1]; m[2] += q[2]; m[4] += q[4]; m[6] += q[6]; m[8] += q[8];
+#pragma unroll
+    for (int k = 9; k < 16; k++) m[k] += q[k];
+    constexpr double iv[19] = {1./19, 1./2394, 1./252, 1./10, 1./40, 1./10, 1./40, 1./10, 1./40,
+                               1./36, 1./72, 1./12, 1./24, 1./4, 1./4, 1./4, 1./8, 1./8, 1./8};
+#pragma unroll
+    for (int k = 0; k < 19; k++) m[k] = m[k] * R(iv[k]);
+    d3q19_from_moments(f, m);
+
+    if (GLOB) {
+        // Dynamics.c.Rt:266-289 evaluates these after "R /= Mw", i.e. on
+        // rho/19 and J/10 -- kept as the reference computes them
+        const R rs = m[0], jx = m[3], jy = m[5], jz = m[7];
+        const R ux = jx / rs, uy = jy / rs, uz = jz / rs;
+        const unsigned add = nt & 0x3u;
+        if (add) {
+            const int base = add == 1 ? 1 : (add == 2 ? 6 : 11);
+            gadd<GLOB>(g, base + 0, ux, zr);
+            gadd<GLOB>(g, base + 1, uy, zr);
+            gadd<GLOB>(g, base + 2, uz, zr);
+            gadd<GLOB>(g, base + 3, rs, zr);
+            gadd<GLOB>(g, base + 4, 1.0, zr);
+        }
+        gadd<GLOB>(g, 16, ux, zr);
+        gadd<GLOB>(g, 17, uy, zr);
+        gadd<GLOB>(g, 18, uz, zr);
+        gadd<GLOB>(g, 19, jx, zr);
+        gadd<GLOB>(g, 20, jy, zr);
+        gadd<GLOB>(g, 21, jz, zr);
+        gadd<GLOB>(g, 22, rs, zr);
+        gadd<GLOB>(g, 23, 1.0, zr);
+        const double v = (double)sqrt(ux*ux + uy*uy + uz*uz);
+        g[25] = fmax(g[25], v);
+    }
+}
+
+// Run(), Dynamics.c.Rt:181-210, with the Zou-He conditions of :98-179.
+// Direction sets: X<0 {2,8,10,12,14}, X==0 {0,3,4,5,6,15,16,17,18},
+// X>0 {1,7,9,11,13}; summed in ascending index order.
+template <class R>
+__device__ __forceinline__ void d3q19_inlet(R* f, R rho, R ux) {
+    R Nxy = ((f[3] + f[15] + f[17]) - (f[4] + f[16] + f[18]))/R(2.);
+    R Nxz = ((f[5] + f[15] + f[16]) - (f[6] + f[17] + f[18]))/R(2.);
+    f[1]  = f[2]  + rho * ux / R(3.);
+    f[9]  = f[8]  + rho * ux / R(6.) + Nxy;
+    f[7]  = f[10] + rho * ux / R(6.) - Nxy;
+    f[11] = f[14] + rho * ux / R(6.) - Nxz;
+    f[13] = f[12] + rho * ux / R(6.) + Nxz;
+}
+
+template <class R>
+__device__ __forceinline__ R sum_xm(const R* f) { return f[2] + f[8] + f[10] + f[12] + f[14]; }
+template <class R>
+__device__ __forceinline__ R sum_xp(const R* f) { return f[1] + f[7] + f[9] + f[11] + f[13]; }
+template <class R>
+__device__ __forceinline__ R sum_x0(const R* f) {
+    return f[0] + f[3] + f[4] + f[5] + f[6] + f[15] + f[16] + f[17] + f[18];
+}
+
+template <class R, bool GLOB>
+__device__ __forceinline__ void d3q19_node(R* f, unsigned nt, const double* par, const double* zr,
+                                           double* g) {
+    const R InletVelocity = (R)par[1], InletDensity = (R)par[2];
+    switch (nt & 0x1Cu) {
+    case 0x14: {  // WPressureL
+        R SF = sum_xm(f);
+        SF = sum_x0(f) + 2 * SF;
+        R rho = InletDensity;
+        R ux = R(1.) - SF / rho;
+        if (ux > InletVelocity) {
+            ux = InletVelocity;
+            rho = SF / (R(1.) - ux);
+        }
+        d3q19_inlet(f, rho, ux);
+        break; }
+    case 0x10: {  // WPressure
+        R rho = InletDensity;
+        R ux = sum_xm(f);
+        ux = sum_x0(f) + 2 * ux;
+        ux = R(1.) - ux / rho;
+        d3q19_inlet(f, rho, ux);
+        break; }
+    case 0x18: {  // WVelocity
+        R ux = InletVelocity;
+        R rho = sum_xm(f);
+        rho = sum_x0(f) + 2 * rho;
+        rho = rho / (R(1.) - ux);
+        d3q19_inlet(f, rho, ux);
+        break; }
+    case 0x04: {  // EPressure
+        R rho = R(1.0);
+        R ux = sum_xp(f);
+        ux = sum_x0(f) + 2 * ux;
+        ux = R(-1.) + ux / rho;
+        R Nxy = ((f[3] + f[15] + f[17]) - (f[4] + f[16] + f[18]))/2;
+        R Nxz = ((f[5] + f[15] + f[16]) - (f[6] + f[17] + f[18]))/2;
+        f[2]  = f[1]  - rho * ux / R(3.0);
+        f[8]  = f[9]  - rho * ux / R(6.0) - Nxy;
+        f[10] = f[7]  - rho * ux / R(6.0) + Nxy;
+        f[14] = f[11] - rho * ux / R(6.0) + Nxz;
+        f[12] = f[13] - rho * ux / R(6.0) - Nxz;
+        break; }
+    case 0x0C:    // Wall
+    case 0x08: {  // Solid: swap every opposite pair (Dynamics.c.Rt:54-75)
+        R t;
+        t = f[1]; f[1] = f[2]; f[2] = t;
+        t = f[3]; f[3] = f[4]; f[4] = t;
+        t = f[5]; f[5] = f[6]; f[6] = t;
+        t = f[7]; f[7] = f[10]; f[10] = t;
+        t = f[8]; f[8] = f[9]; f[9] = t;
+        t = f[11]; f[11] = f[14]; f[14] = t;
+        t = f[12]; f[12] = f[13]; f[13] = t;
+        t = f[15]; f[15] = f[18]; f[18] = t;
+        t = f[16]; f[16] = f[17]; f[17] = t;
+        break; }
+    default: break;
+    }
+    if ((nt & 0x20u) == 0x20u) d3q19_mrt<R, GLOB>(f, nt, par, zr, g);
+}
+
+// ---------------------------------------------------------- reductions ---
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// per-block partials: wave shuffle, then LDS across the block's waves
+template <int G>
+__device__ __forceinline__ void block_globals(const double* g, int sum_globals, double* gpart) {
+    __shared__ double lds[16][G];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int i = 0; i < G; i++) {
+        double v = i < sum_globals ? wave_sum(g[i]) : wave_max(g[i]);
+        if (lane == 0) lds[wave][i] = v;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < G) {
+        const int i = threadIdx.x;
+        double v = lds[0][i];
+        for (int w = 1; w < nw; w++) v = i < sum_globals ? v + lds[w][i] : fmax(v, lds[w][i]);
+        gpart[(size_t)blockIdx.x * G + i] = v;
+    }
+}
+
+// ---------------------------------------------------------- step kernel ---
+template <int MODEL, int ST, bool GLOB>
+__global__ void __launch_bounds__(256) k_step(StepArgs a) {
+    using L = Lat<MODEL>;
+    using S = Store<ST>;
+    using T = typename S::T;
+    using R = typename S::R;
+    constexpr int Q = L::Q;
+
+    // block -> (x segment, row); uniform scalar arithmetic
+    const int row = blockIdx.x / a.bpr;
+    const int xb = blockIdx.x - row * a.bpr;
+    const int y = row % a.ny;
+    const int z = row / a.ny;
+    const int x = xb * blockDim.x + threadIdx.x;
+    const bool active = x < a.nx;
+    const int xc = active ? x : a.nx - 1;
+
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int xm = xc == 0 ? nx - 1 : xc - 1, xp = xc == nx - 1 ? 0 : xc + 1;
+    const int ym = y == 0 ? ny - 1 : y - 1, yp = y == ny - 1 ? 0 : y + 1;
+    const int zm = z == 0 ? nz - 1 : z - 1, zp = z == nz - 1 ? 0 : z + 1;
+
+    const unsigned n = (unsigned)xc + (unsigned)nx * ((unsigned)y + (unsigned)ny * (unsigned)z);
+    const unsigned nt = a.flags[n];
+    int zone = (int)(nt >> L::ZSHIFT);
+    if (zone >= a.nzones) zone = 0;
+    const double* zr = a.zone + (size_t)zone * a.zcols;
+
+    const T* fin = (const T*)a.fin;
+    R f[Q];
+#pragma unroll
+    for (int i = 0; i < Q; i++) {
+        // source node x - e_i with periodic wrap
+        const int sx = L::ex(i) > 0 ? xm : (L::ex(i) < 0 ? xp : xc);
+        const int sy = L::ey(i) > 0 ? ym : (L::ey(i) < 0 ? yp : y);
+        const int sz = L::ez(i) > 0 ? zm : (L::ez(i) < 0 ? zp : z);
+        const unsigned src = (unsigned)sx + (unsigned)nx * ((unsigned)sy + (unsigned)ny * (unsigned)sz);
+        f[i] = S::get(fin[(size_t)i * a.pitch + src], (R)L::w(i));
+    }
+
+    double g[L::G > 0 ? L::G : 1];
+    if (GLOB) {
+#pragma unroll
+        for (int i = 0; i < L::G; i++) g[i] = 0.0;
+    }
+    if (active) {
+        if constexpr (MODEL == MODEL_D3Q19) {
+            d3q19_node<R, GLOB>(f, nt, a.par, zr, g);
+        } else {
+            d2q9_node<R, MODEL == MODEL_D2Q9_SRT_DRAG, GLOB>(f, nt, (R)a.par[0], (R)a.par[1], zr, g);
+        }
+        T* fout = (T*)a.fout;
+#pragma unroll
+        for (int i = 0; i < Q; i++) fout[(size_t)i * a.pitch + n] = S::put(f[i], (R)L::w(i));
+    }
+    if (GLOB) {
+        constexpr int SUMG = (MODEL == MODEL_D3Q19) ? 25 : L::G;
+        block_globals<L::G>(g, SUMG, a.gpart);
+    }
+}
+
+// Init(): d2q9 SetEquilibrium(Density, Velocity_x, Velocity_y) per zone
+// (Dynamics.c:26-38); d3q19 SetEquilibrum(1, InletVelocity, 0, 0)
+// (Dynamics.c.Rt:219-221).
+template <int MODEL, int ST>
+__global__ void __launch_bounds__(256) k_init(StepArgs a) {
+    using L = Lat<MODEL>;
+    using S = Store<ST>;
+    using T = typename S::T;
+    using R = typename S::R;
+    constexpr int Q = L::Q;
+    const int row = blockIdx.x / a.bpr;
+    const int xb = blockIdx.x - row * a.bpr;
+    const int y = row % a.ny;
+    const int z = row / a.ny;
+    const int x = xb * blockDim.x + threadIdx.x;
+    if (x >= a.nx) return;
+    const unsigned n = (unsigned)x + (unsigned)a.nx * ((unsigned)y + (unsigned)a.ny * (unsigned)z);
+    const unsigned nt = a.flags[n];
+    int zone = (int)(nt >> L::ZSHIFT);
+    if (zone >= a.nzones) zone = 0;
+    const double* zr = a.zone + (size_t)zone * a.zcols;
+    R f[Q];
+    if constexpr (MODEL == MODEL_D3Q19) {
+        d3q19_feq<R>(f, R(1.0), (R)a.par[1], R(0.), R(0.));
+    } else {
+        d2q9_feq<R>(f, (R)a.par[1], (R)zr[1], (R)zr[2]);
+    }
+    T* fout = (T*)a.fout;
+#pragma unroll
+    for (int i = 0; i < Q; i++) fout[(size_t)i * a.pitch + n] = S::put(f[i], (R)L::w(i));
+}
+
+// Quantities on pulled populations (src/LatticeContainer.inc.cpp.Rt:437-473):
+// d2q9: U (J/rho + G/2) = 0, Rho = 1; d3q19: P = 0, U (J/rho, 0 on Wall) = 1.
+template <int MODEL, int ST>
+__global__ void __launch_bounds__(256) k_quantity(StepArgs a, int qid, int x0, int y0, int z0, int rx,
+                                                  int ry, int rz, double scale, double* out) {
+    using L = Lat<MODEL>;
+    using S = Store<ST>;
+    using T = typename S::T;
+    constexpr int Q = L::Q;
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)rx * ry * rz;
+    if (tid >= total) return;
+    const int lx = (int)(tid % rx);
+    const int ly = (int)((tid / rx) % ry);
+    const int lz = (int)(tid / ((long long)rx * ry));
+    const int x = x0 + lx, y = y0 + ly, z = z0 + lz;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const unsigned n = (unsigned)x + (unsigned)nx * ((unsigned)y + (unsigned)ny * (unsigned)z);
+    const unsigned nt = a.flags[n];
+    const T* fin = (const T*)a.fin;
+    double f[Q];
+#pragma unroll
+    for (int i = 0; i < Q; i++) {
+        const int sx = (x - L::ex(i) + nx) % nx;
+        const int sy = (y - L::ey(i) + ny) % ny;
+        const int sz = (z - L::ez(i) + nz) % nz;
+        const unsigned src = (unsigned)sx + (unsigned)nx * ((unsigned)sy + (unsigned)ny * (unsigned)sz);
+        f[i] = (double)S::get(fin[(size_t)i * a.pitch + src], (typename S::R)L::w(i));
+    }
+    if constexpr (MODEL == MODEL_D3Q19) {
+        double rho = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+        for (int i = 0; i < Q; i++) {
+            rho += f[i]; jx += f[i] * L::ex(i); jy += f[i] * L::ey(i); jz += f[i] * L::ez(i);
+        }
+        if (qid == 0) {
+            out[tid] = (rho - 1.0) / 3.0 * scale;
+        } else {
+            double ux = jx / rho, uy = jy / rho, uz = jz / rho;
+            if ((nt & 0x1Cu) == 0x0Cu) ux = uy = uz = 0;
+            out[3 * tid] = ux * scale; out[3 * tid + 1] = uy * scale; out[3 * tid + 2] = uz * scale;
+        }
+    } else {
+        int zone = (int)(nt >> L::ZSHIFT);
+        if (zone >= a.nzones) zone = 0;
+        const double* zr = a.zone + (size_t)zone * a.zcols;
+        double d = f[8]+f[7]+f[6]+f[5]+f[4]+f[3]+f[2]+f[1]+f[0];
+        if (qid == 1) {
+            out[tid] = d * scale;
+        } else {
+            out[3 * tid] = (( f[8]-f[7]-f[6]+f[5]-f[3]+f[1] )/d + zr[3]*0.5 ) * scale;
+            out[3 * tid + 1] = ((-f[8]-f[7]+f[6]+f[5]-f[4]+f[2] )/d + zr[4]*0.5 ) * scale;
+            out[3 * tid + 2] = 0.0;
+        }
+    }
+}
+
+template <int ST>
+__global__ void k_get_density(const void* buf, long long pitch, long long n, int q, double w, double* out) {
+    using S = Store<ST>;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = (double)S::get(((const typename S::T*)buf)[q * pitch + i], (typename S::R)w);
+}
+
+template <int ST>
+__global__ void k_set_density(void* buf, long long pitch, long long n, int q, double w, const double* in) {
+    using S = Store<ST>;
+    using R = typename S::R;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    ((typename S::T*)buf)[q * pitch + i] = S::put((R)in[i], (R)w);
+}
+
+// deterministic second pass: one block per Global, fixed-order tree
+__global__ void __launch_bounds__(256) k_reduce_globals(const double* gpart, int nblocks, int nglob,
+                                                        int sum_globals, double* out) {
+    const int gi = blockIdx.x;
+    const bool is_sum = gi < sum_globals;
+    double v = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+        const double x = gpart[(size_t)b * nglob + gi];
+        v = is_sum ? v + x : fmax(v, x);
+    }
+    __shared__ double lds[4];
+    v = is_sum ? wave_sum(v) : wave_max(v);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) lds[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = lds[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) r = is_sum ? r + lds[w] : fmax(r, lds[w]);
+        // accumulate into the device Globals (cleared before a Globals step)
+        out[gi] = is_sum ? out[gi] + r : fmax(out[gi], r);
+    }
+}
+
+// plain device copy (16-byte vectors, grid-stride): the bandwidth ceiling
+// the node kernel is compared with, and a kernel of known traffic for
+// calibrating the HBM byte counters
+__global__ void __launch_bounds__(256) k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s) {
+    const size_t n = bytes / sizeof(uint4);
+    hipLaunchKernelGGL(k_copy, dim3(256 * 64), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, n);
+    return hipGetLastError();
+}
+
+// --------------------------------------------------------------- launch ---
+namespace {
+int nblocks_of(const StepArgs& a, int block) { return a.bpr * a.ny * a.nz; }
+
+template <int MODEL, int ST>
+hipError_t step_ms(bool glob, const StepArgs& a, int block, hipStream_t s) {
+    dim3 grid(nblocks_of(a, block)), blk(block);
+    if (glob) hipLaunchKernelGGL((k_step<MODEL, ST, true>), grid, blk, 0, s, a);
+    else hipLaunchKernelGGL((k_step<MODEL, ST, false>), grid, blk, 0, s, a);
+    return hipGetLastError();
+}
+
+template <int MODEL>
+hipError_t step_m(int st, bool glob, const StepArgs& a, int block, hipStream_t s) {
+    switch (st) {
+    case ST_F64: return step_ms<MODEL, ST_F64>(glob, a, block, s);
+    case ST_F32: return step_ms<MODEL, ST_F32>(glob, a, block, s);
+    case ST_F32S: return step_ms<MODEL, ST_F32S>(glob, a, block, s);
+    case ST_F16S: return step_ms<MODEL, ST_F16S>(glob, a, block, s);
+    case ST_BF16S: return step_ms<MODEL, ST_BF16S>(glob, a, block, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+template <int MODEL, int ST>
+hipError_t init_ms(const StepArgs& a, int block, hipStream_t s) {
+    hipLaunchKernelGGL((k_init<MODEL, ST>), dim3(nblocks_of(a, block)), dim3(block), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int MODEL>
+hipError_t init_m(int st, const StepArgs& a, int block, hipStream_t s) {
+    switch (st) {
+    case ST_F64: return init_ms<MODEL, ST_F64>(a, block, s);
+    case ST_F32: return init_ms<MODEL, ST_F32>(a, block, s);
+    case ST_F32S: return init_ms<MODEL, ST_F32S>(a, block, s);
+    case ST_F16S: return init_ms<MODEL, ST_F16S>(a, block, s);
+    case ST_BF16S: return init_ms<MODEL, ST_BF16S>(a, block, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+template <int MODEL, int ST>
+hipError_t quantity_ms(int q, const StepArgs& a, int x0, int y0, int z0, int rx, int ry, int rz,
+                       double scale, double* out, hipStream_t s) {
+    const long long total = (long long)rx * ry * rz;
+    if (total == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_quantity<MODEL, ST>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+                       a, q, x0, y0, z0, rx, ry, rz, scale, out);
+    return hipGetLastError();
+}
+
+template <int MODEL>
+hipError_t quantity_m(int st, int q, const StepArgs& a, int x0, int y0, int z0, int rx, int ry, int rz,
+                      double scale, double* out, hipStream_t s) {
+    switch (st) {
+    case ST_F64: return quantity_ms<MODEL, ST_F64>(q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
+    case ST_F32: return quantity_ms<MODEL, ST_F32>(q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
+    case ST_F32S: return quantity_ms<MODEL, ST_F32S>(q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
+    case ST_F16S: return quantity_ms<MODEL, ST_F16S>(q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
+    case ST_BF16S: return quantity_ms<MODEL, ST_BF16S>(q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+double weight(int model, int q) {
+    return model == MODEL_D3Q19 ? Lat<MODEL_D3Q19>::w(q) : Lat<MODEL_D2Q9_SRT>::w(q);
+}
+}  // namespace
+
+hipError_t launch_step(int model, int storage, bool globals, const StepArgs& a, int block, hipStream_t s) {
+    switch (model) {
+    case MODEL_D2Q9_SRT: return step_m<MODEL_D2Q9_SRT>(storage, globals, a, block, s);
+    case MODEL_D2Q9_SRT_DRAG: return step_m<MODEL_D2Q9_SRT_DRAG>(storage, globals, a, block, s);
+    case MODEL_D3Q19: return step_m<MODEL_D3Q19>(storage, globals, a, block, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_init(int model, int storage, const StepArgs& a, int block, hipStream_t s) {
+    switch (model) {
+    case MODEL_D2Q9_SRT: return init_m<MODEL_D2Q9_SRT>(storage, a, block, s);
+    case MODEL_D2Q9_SRT_DRAG: return init_m<MODEL_D2Q9_SRT_DRAG>(storage, a, block, s);
+    case MODEL_D3Q19: return init_m<MODEL_D3Q19>(storage, a, block, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_reduce_globals(const double* gpart, int nblocks, int nglob, int sum_globals, double* out,
+                                 hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce_globals, dim3(nglob), dim3(256), 0, s, gpart, nblocks, nglob, sum_globals, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_quantity(int model, int storage, int q, const StepArgs& a, int x0, int y0, int z0, int rx,
+                           int ry, int rz, double scale, double* out, hipStream_t s) {
+    switch (model) {
+    case MODEL_D2Q9_SRT: return quantity_m<MODEL_D2Q9_SRT>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
+    case MODEL_D2Q9_SRT_DRAG:
+        return quantity_m<MODEL_D2Q9_SRT_DRAG>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
+    case MODEL_D3Q19: return quantity_m<MODEL_D3Q19>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_get_density(int model, int storage, const void* buf, long long pitch, long long n, int q,
+                              double* out, hipStream_t s) {
+    const dim3 grid((unsigned)((n + 255) / 256)), blk(256);
+    const double w = weight(model, q);
+    switch (storage) {
+    case ST_F64: hipLaunchKernelGGL(k_get_density<ST_F64>, grid, blk, 0, s, buf, pitch, n, q, w, out); break;
+    case ST_F32: hipLaunchKernelGGL(k_get_density<ST_F32>, grid, blk, 0, s, buf, pitch, n, q, w, out); break;
+    case ST_F32S: hipLaunchKernelGGL(k_get_density<ST_F32S>, grid, blk, 0, s, buf, pitch, n, q, w, out); break;
+    case ST_F16S: hipLaunchKernelGGL(k_get_density<ST_F16S>, grid, blk, 0, s, buf, pitch, n, q, w, out); break;
+    case ST_BF16S: hipLaunchKernelGGL(k_get_density<ST_BF16S>, grid, blk, 0, s, buf, pitch, n, q, w, out); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_set_density(int model, int storage, void* buf, long long pitch, long long n, int q,
+                              const double* in, hipStream_t s) {
+    const dim3 grid((unsigned)((n + 255) / 256)), blk(256);
+    const double w = weight(model, q);
+    switch (storage) {
+    case ST_F64: hipLaunchKernelGGL(k_set_density<ST_F64>, grid, blk, 0, s, buf, pitch, n, q, w, in); break;
+    case ST_F32: hipLaunchKernelGGL(k_set_density<ST_F32>, grid, blk, 0, s, buf, pitch, n, q, w, in); break;
+    case ST_F32S: hipLaunchKernelGGL(k_set_density<ST_F32S>, grid, blk, 0, s, buf, pitch, n, q, w, in); break;
+    case ST_F16S: hipLaunchKernelGGL(k_set_density<ST_F16S>, grid, blk, 0, s, buf, pitch, n, q, w, in); break;
+    case ST_BF16S: hipLaunchKernelGGL(k_set_density<ST_BF16S>, grid, blk, 0, s, buf, pitch, n, q, w, in); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace tclbx

@@ -1,0 +1,379 @@
+// lattice.cpp -- see lattice.h.
+#include "lattice.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+
+namespace tclbx {
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) {
+        throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+    }
+}
+
+namespace {
+// x-segment width: the wavefront multiple (64..256) that wastes the fewest
+// lanes on a row of nx nodes, preferring wider blocks
+int pick_block(int nx) {
+    int best = 256;
+    long long best_waste = -1;
+    for (int b : {256, 128, 64}) {
+        long long waste = (long long)((nx + b - 1) / b) * b - nx;
+        if (best_waste < 0 || waste < best_waste) { best = b; best_waste = waste; }
+    }
+    return best;
+}
+}  // namespace
+
+Lattice::Lattice(const ModelDef* m, int nx_, int ny_, int nz_, int storage_)
+    : model(m), nx(nx_), ny(ny_), nz(nz_), storage(storage_), N((size_t)nx_ * ny_ * nz_) {
+    if (!m) throw std::invalid_argument("unknown model");
+    if (nx <= 0 || ny <= 0 || nz <= 0) throw std::invalid_argument("lattice size must be positive");
+    if (m->dim == 2 && nz != 1) throw std::invalid_argument("2D model needs nz = 1");
+    if (storage < 0 || storage >= ST_COUNT) throw std::invalid_argument("unknown storage format");
+    if (N >= (1ull << 32)) throw std::invalid_argument("lattice exceeds 2^32 nodes");
+    const int S = storage_bytes(storage);
+    const size_t align = 256 / S;
+    pitch = (N + align - 1) / align * align;
+
+    settings_.resize(m->settings.size());
+    // defaults of the non-deriving settings (src/Solver.cpp.Rt:277-280)
+    for (size_t i = 0; i < m->settings.size(); i++)
+        settings_[i] = m->settings[i].derived < 0 ? m->settings[i].def : 0.0;
+    zcols_ = (int)m->zonal.size();
+    nzones_ = m->zone_max + 1;
+    zone_.assign((size_t)nzones_ * zcols_, 0.0);
+    for (int z = 0; z < nzones_; z++)
+        for (int c = 0; c < zcols_; c++) zone_[(size_t)z * zcols_ + c] = m->zonal[c].def;
+
+    hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipEventCreate(&ev0_), "hipEventCreate");
+    hip_check(hipEventCreate(&ev1_), "hipEventCreate");
+    const size_t bytes = (size_t)m->Q * pitch * S;
+    for (int i = 0; i < 2; i++) {
+        hip_check(hipMalloc(&snaps_[i], bytes), "hipMalloc(snapshot)");
+        hip_check(hipMemsetAsync(snaps_[i], 0, bytes, stream_), "hipMemset(snapshot)");
+    }
+    hip_check(hipMalloc((void**)&d_flags_, N * sizeof(uint16_t)), "hipMalloc(flags)");
+    hip_check(hipMemsetAsync(d_flags_, 0, N * sizeof(uint16_t), stream_), "hipMemset(flags)");
+    hip_check(hipMalloc((void**)&d_zone_, std::max<size_t>(1, zone_.size()) * sizeof(double)), "hipMalloc(zone)");
+    set_block(pick_block(nx));
+    const int G = kernel_nglobals(m->id);
+    hip_check(hipMalloc((void**)&d_glob_, MAX_GLOBALS * sizeof(double)), "hipMalloc(globals)");
+    hip_check(hipMemsetAsync(d_glob_, 0, MAX_GLOBALS * sizeof(double), stream_), "hipMemset(globals)");
+    globals_.assign(G, 0.0);
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+}
+
+Lattice::~Lattice() {
+    drop_graph();
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (void* p : snaps_) if (p) (void)hipFree(p);
+    if (d_flags_) (void)hipFree(d_flags_);
+    if (d_zone_) (void)hipFree(d_zone_);
+    if (d_gpart_) (void)hipFree(d_gpart_);
+    if (d_glob_) (void)hipFree(d_glob_);
+    if (ev0_) (void)hipEventDestroy(ev0_);
+    if (ev1_) (void)hipEventDestroy(ev1_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Lattice::set_block(int b) {
+    if (b != 64 && b != 128 && b != 256) throw std::invalid_argument("block must be 64, 128 or 256");
+    block_ = b;
+    bpr_ = (nx + b - 1) / b;
+    nblocks_ = bpr_ * ny * nz;
+    if (d_gpart_) (void)hipFree(d_gpart_);
+    d_gpart_ = nullptr;
+    hip_check(hipMalloc((void**)&d_gpart_, (size_t)nblocks_ * kernel_nglobals(model->id) * sizeof(double)),
+              "hipMalloc(partials)");
+    drop_graph();
+}
+
+size_t Lattice::bytes_per_node_step() const {
+    return (size_t)2 * model->Q * storage_bytes(storage) + sizeof(uint16_t);
+}
+
+void Lattice::SetSetting(int id, double v) {
+    if (id < 0 || id >= (int)settings_.size()) throw std::out_of_range("setting id");
+    settings_[id] = v;
+    params_version_++;
+    const SettingDef& s = model->settings[id];
+    if (s.derived >= 0) {
+        double d = v;
+        if (s.derive_kind == 1) d = 1.0 / (3 * v + 0.5);       // omega = 1/(3 nu + 0.5)
+        else if (s.derive_kind == 2) d = 1.0 + v * 3;          // InletDensity = 1 + 3 p
+        SetSetting(s.derived, d);
+    }
+}
+
+void Lattice::SetZoneSetting(int id, int zone, double v) {
+    if (id < 0 || id >= zcols_) throw std::out_of_range("zonal setting id");
+    if (zone < -1 || zone >= nzones_) throw std::out_of_range("zone");
+    if (zone == -1) {
+        for (int z = 0; z < nzones_; z++) zone_[(size_t)z * zcols_ + id] = v;
+    } else {
+        zone_[(size_t)zone * zcols_ + id] = v;
+    }
+    zone_dirty_ = true;
+}
+
+double Lattice::GetZoneSetting(int id, int zone) const {
+    if (id < 0 || id >= zcols_ || zone < 0 || zone >= nzones_) throw std::out_of_range("zonal setting");
+    return zone_[(size_t)zone * zcols_ + id];
+}
+
+void Lattice::upload_zone() {
+    if (!zone_dirty_) return;
+    if (!zone_.empty())
+        hip_check(hipMemcpyAsync(d_zone_, zone_.data(), zone_.size() * sizeof(double), hipMemcpyHostToDevice,
+                                 stream_), "upload zone table");
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    zone_dirty_ = false;
+}
+
+static bool region_in(const Region& r, int nx, int ny, int nz) {
+    return r.dx >= 0 && r.dy >= 0 && r.dz >= 0 && r.nx >= 0 && r.ny >= 0 && r.nz >= 0 &&
+           r.dx + r.nx <= nx && r.dy + r.ny <= ny && r.dz + r.nz <= nz;
+}
+
+void Lattice::FlagOverwrite(const uint16_t* flags, const Region& over) {
+    if (!region_in(over, nx, ny, nz)) throw std::out_of_range("flag region outside the lattice");
+    if (over.dx == 0 && over.dy == 0 && over.dz == 0 && over.nx == nx && over.ny == ny && over.nz == nz) {
+        hip_check(hipMemcpy(d_flags_, flags, N * sizeof(uint16_t), hipMemcpyHostToDevice), "FlagOverwrite");
+        return;
+    }
+    for (int z = 0; z < over.nz; z++)
+        for (int y = 0; y < over.ny; y++) {
+            const size_t dst = (size_t)over.dx + (size_t)nx * ((size_t)(over.dy + y) + (size_t)ny * (over.dz + z));
+            const size_t src = (size_t)over.nx * ((size_t)y + (size_t)over.ny * z);
+            hip_check(hipMemcpy(d_flags_ + dst, flags + src, over.nx * sizeof(uint16_t), hipMemcpyHostToDevice),
+                      "FlagOverwrite");
+        }
+}
+
+void Lattice::GetFlags(const Region& over, uint16_t* out) {
+    if (!region_in(over, nx, ny, nz)) throw std::out_of_range("flag region outside the lattice");
+    for (int z = 0; z < over.nz; z++)
+        for (int y = 0; y < over.ny; y++) {
+            const size_t src = (size_t)over.dx + (size_t)nx * ((size_t)(over.dy + y) + (size_t)ny * (over.dz + z));
+            const size_t dst = (size_t)over.nx * ((size_t)y + (size_t)over.ny * z);
+            hip_check(hipMemcpy(out + dst, d_flags_ + src, over.nx * sizeof(uint16_t), hipMemcpyDeviceToHost),
+                      "GetFlags");
+        }
+}
+
+StepArgs Lattice::args(int from, int to, bool glob) const {
+    StepArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.fin = snaps_[from];
+    a.fout = snaps_[to];
+    a.flags = d_flags_;
+    a.zone = d_zone_;
+    a.gpart = glob ? d_gpart_ : nullptr;
+    a.pitch = (long long)pitch;
+    a.nx = nx; a.ny = ny; a.nz = nz;
+    a.bpr = bpr_;
+    a.zcols = zcols_;
+    a.nzones = nzones_;
+    auto S = [&](const char* n) { int i = model->setting_id(n); return i >= 0 ? settings_[i] : 0.0; };
+    if (model->id == MODEL_D3Q19) {
+        const char* names[] = {"omega", "InletVelocity", "InletDensity", "ForceX", "ForceY", "ForceZ",
+                               "MRT_Se", "MRT_Seps", "MRT_Sq", "MRT_Spi", "MRT_Sm"};
+        for (int i = 0; i < 11; i++) a.par[i] = S(names[i]);
+    } else {
+        a.par[0] = S("omega");
+        a.par[1] = S("Density");
+    }
+    return a;
+}
+
+void Lattice::launch(int from, int to, bool glob) {
+    StepArgs a = args(from, to, glob);
+    hip_check(launch_step(model->id, storage, glob, a, block_, stream_), "step kernel");
+    if (glob) {
+        hip_check(launch_reduce_globals(d_gpart_, nblocks_, kernel_nglobals(model->id), model->sum_globals,
+                                        d_glob_, stream_), "globals reduction");
+    }
+}
+
+void Lattice::drop_graph() {
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    graph_exec_ = nullptr;
+    graph_ = nullptr;
+    graph_version_ = ~0u;
+}
+
+// replay a captured loop of graph_steps_ plain steps (0->1->0 ...); the
+// kernel arguments are baked in, so a settings change re-captures it
+void Lattice::run_graph_steps(int nsteps) {
+    if (graph_exec_ == nullptr || graph_version_ != params_version_) {
+        drop_graph();
+        hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture");
+        for (int i = 0; i < graph_steps_; i++) {
+            StepArgs a = args(i % 2, (i + 1) % 2, false);
+            hipError_t e = launch_step(model->id, storage, false, a, block_, stream_);
+            if (e != hipSuccess) {
+                hipGraph_t g;
+                (void)hipStreamEndCapture(stream_, &g);
+                if (g) (void)hipGraphDestroy(g);
+                hip_check(e, "step kernel (capture)");
+            }
+        }
+        hip_check(hipStreamEndCapture(stream_, &graph_), "end capture");
+        hip_check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "graph instantiate");
+        graph_version_ = params_version_;
+    }
+    for (int done = 0; done < nsteps; done += graph_steps_)
+        hip_check(hipGraphLaunch(graph_exec_, stream_), "graph launch");
+}
+
+void Lattice::Init() {
+    upload_zone();
+    iter_ = 0;
+    snap_ = 0;
+    StepArgs a = args(1, 0, false);
+    hip_check(launch_init(model->id, storage, a, block_, stream_), "init kernel");
+    hip_check(hipStreamSynchronize(stream_), "Init");
+    clearGlobals();
+}
+
+void Lattice::Iterate(int niter, int iter_type) {
+    if (niter < 0) throw std::invalid_argument("negative iteration count");
+    upload_zone();
+    int last_glob = iter_type & ITER_LASTGLOB;
+    if (iter_type & ITER_INTEG) last_glob = 0;
+    const bool every_glob = (iter_type & ITER_GLOBS) != 0;
+    hip_check(hipEventRecord(ev0_, stream_), "event record");
+    int i = 0;
+    // leading plain steps until snapshot 0 is current, then whole graph
+    // replays, then the tail (the Globals step is always launched directly)
+    const int plain_end = niter - (last_glob ? 1 : 0);
+    if (!every_glob && graph_steps_ >= 2 && plain_end - i >= 2 * graph_steps_) {
+        if (snap_ != 0) {
+            launch(snap_, 1 - snap_, false);
+            snap_ = 1 - snap_;
+            i++;
+        }
+        const int reps = (plain_end - i) / graph_steps_;
+        if (reps > 0) {
+            run_graph_steps(reps * graph_steps_);
+            i += reps * graph_steps_;
+        }
+    }
+    for (; i < niter; i++) {
+        const bool glob = every_glob || (last_glob && i == niter - 1);
+        if (last_glob && i == niter - 1)
+            hip_check(hipMemsetAsync(d_glob_, 0, MAX_GLOBALS * sizeof(double), stream_), "clear globals");
+        launch(snap_, 1 - snap_, glob);
+        snap_ = 1 - snap_;
+    }
+    hip_check(hipEventRecord(ev1_, stream_), "event record");
+    hip_check(hipStreamSynchronize(stream_), "Iterate");
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, ev0_, ev1_), "event elapsed");
+    last_ms_ = ms;
+    kernel_ms_ += ms;
+    iter_ += niter;
+    if (last_glob) {
+        std::fill(globals_.begin(), globals_.end(), 0.0);
+        calcGlobals();
+    } else if (iter_type & ITER_INTEG) {
+        calcGlobals();
+    }
+}
+
+void Lattice::clearGlobals() { std::fill(globals_.begin(), globals_.end(), 0.0); }
+
+// src/Lattice.cu.Rt:1299-1315: globals += device values, device cleared.
+// (The non-zonal InObj objective term there is empty for these models:
+// every InObj weight is zonal and already added on the device.)
+void Lattice::calcGlobals() {
+    std::vector<double> tab(globals_.size());
+    hip_check(hipMemcpyAsync(tab.data(), d_glob_, tab.size() * sizeof(double), hipMemcpyDeviceToHost, stream_),
+              "read globals");
+    hip_check(hipMemsetAsync(d_glob_, 0, MAX_GLOBALS * sizeof(double), stream_), "clear globals");
+    hip_check(hipStreamSynchronize(stream_), "calcGlobals");
+    for (size_t i = 0; i < tab.size(); i++) globals_[i] += tab[i];
+}
+
+void Lattice::GetQuantity(int q, const Region& over, double* out, double scale) {
+    if (q < 0 || q >= (int)model->quantities.size()) throw std::out_of_range("quantity id");
+    if (!region_in(over, nx, ny, nz)) throw std::out_of_range("quantity region outside the lattice");
+    upload_zone();
+    const size_t n = (size_t)over.nx * over.ny * over.nz;
+    if (n == 0) return;
+    const size_t bytes = n * (model->quantities[q].vector ? 3 : 1) * sizeof(double);
+    double* d_out = nullptr;
+    hip_check(hipMalloc((void**)&d_out, bytes), "hipMalloc(quantity)");
+    StepArgs a = args(snap_, 1 - snap_, false);
+    hipError_t e = launch_quantity(model->id, storage, q, a, over.dx, over.dy, over.dz, over.nx, over.ny,
+                                   over.nz, scale, d_out, stream_);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, bytes, hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+    (void)hipFree(d_out);
+    hip_check(e, "GetQuantity");
+}
+
+void Lattice::GetDensity(int q, double* out) {
+    if (q < 0 || q >= model->Q) throw std::out_of_range("density id");
+    double* d = nullptr;
+    hip_check(hipMalloc((void**)&d, N * sizeof(double)), "hipMalloc(density)");
+    hipError_t e = launch_get_density(model->id, storage, snaps_[snap_], (long long)pitch, (long long)N, q, d,
+                                      stream_);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d, N * sizeof(double), hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+    (void)hipFree(d);
+    hip_check(e, "GetDensity");
+}
+
+void Lattice::SetDensity(int q, const double* in) {
+    if (q < 0 || q >= model->Q) throw std::out_of_range("density id");
+    double* d = nullptr;
+    hip_check(hipMalloc((void**)&d, N * sizeof(double)), "hipMalloc(density)");
+    hipError_t e = hipMemcpyAsync(d, in, N * sizeof(double), hipMemcpyHostToDevice, stream_);
+    if (e == hipSuccess)
+        e = launch_set_density(model->id, storage, snaps_[snap_], (long long)pitch, (long long)N, q, d, stream_);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+    (void)hipFree(d);
+    hip_check(e, "SetDensity");
+}
+
+// raw dump of the current snapshot, density planes in order, in the storage
+// format (the reference's saveSolution writes its FTabs raw, no header:
+// src/Lattice.cu.Rt:708-737)
+int Lattice::Save(const std::string& path) {
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) return -1;
+    const size_t S = storage_bytes(storage);
+    std::vector<char> host(N * S);
+    int rc = 0;
+    for (int q = 0; q < model->Q && rc == 0; q++) {
+        hip_check(hipMemcpy(host.data(), (char*)snaps_[snap_] + (size_t)q * pitch * S, N * S, hipMemcpyDeviceToHost),
+                  "Save");
+        if (std::fwrite(host.data(), 1, host.size(), f) != host.size()) rc = -1;
+    }
+    std::fclose(f);
+    return rc;
+}
+
+int Lattice::Load(const std::string& path) {
+    FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) return -1;
+    const size_t S = storage_bytes(storage);
+    std::vector<char> host(N * S);
+    int rc = 0;
+    for (int q = 0; q < model->Q && rc == 0; q++) {
+        if (std::fread(host.data(), 1, host.size(), f) != host.size()) { rc = -1; break; }
+        hip_check(hipMemcpy((char*)snaps_[snap_] + (size_t)q * pitch * S, host.data(), N * S, hipMemcpyHostToDevice),
+                  "Load");
+    }
+    std::fclose(f);
+    return rc;
+}
+
+}  // namespace tclbx

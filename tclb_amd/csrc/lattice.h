@@ -1,0 +1,100 @@
+// lattice.h -- one lattice (one subdomain on one GPU): device memory, the
+// step loop and the Globals, with the operations of TCLB's Lattice class
+// (src/Lattice.h.Rt:39-191) that the node-kernel path uses.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+#include "model.h"
+
+namespace tclbx {
+
+// iteration types, src/Lattice.h.Rt:19-31
+constexpr int ITER_STREAM = 0x000, ITER_NORM = 0x001, ITER_TYPE = 0x003, ITER_NO = 0x000,
+              ITER_GLOBS = 0x010, ITER_OBJ = 0x020, ITER_INTEG = 0x070, ITER_LASTGLOB = 0x080;
+
+struct Region {
+    int dx = 0, dy = 0, dz = 0, nx = 0, ny = 0, nz = 0;
+};
+
+class Lattice {
+public:
+    Lattice(const ModelDef* m, int nx, int ny, int nz, int storage);
+    ~Lattice();
+    Lattice(const Lattice&) = delete;
+    Lattice& operator=(const Lattice&) = delete;
+
+    // settings (src/Lattice.cu.Rt:1344-1358): a derived setting also sets
+    // the setting it derives (nu -> omega, InletPressure -> InletDensity)
+    void SetSetting(int id, double v);
+    double GetSetting(int id) const { return settings_[id]; }
+    // zonal settings (src/ZoneSettings.h:61-80): zone -1 sets every zone
+    void SetZoneSetting(int id, int zone, double v);
+    double GetZoneSetting(int id, int zone) const;
+
+    void FlagOverwrite(const uint16_t* flags, const Region& over);
+    void GetFlags(const Region& over, uint16_t* out);
+
+    void Init();                              // src/Lattice.cu.Rt:799-821
+    void Iterate(int niter, int iter_type);   // src/Lattice.cu.Rt:900-989
+    void clearGlobals();
+    void calcGlobals();
+    const std::vector<double>& globals() const { return globals_; }
+
+    void GetQuantity(int q, const Region& over, double* out, double scale);
+    void GetDensity(int q, double* out);      // Get_<f> of the current snapshot
+    void SetDensity(int q, const double* in); // Set_<f>
+    int Save(const std::string& path);
+    int Load(const std::string& path);
+
+    double last_iterate_ms() const { return last_ms_; }
+    double kernel_ms_total() const { return kernel_ms_; }
+    int iter() const { return iter_; }
+    hipStream_t stream() const { return stream_; }
+    size_t bytes_per_node_step() const;       // algorithmic HBM bytes
+    void set_block(int b);
+    int block() const { return block_; }
+    void set_graph_steps(int k) { graph_steps_ = k; drop_graph(); }
+
+    const ModelDef* model;
+    const int nx, ny, nz, storage;
+    const size_t N;
+    size_t pitch;  // elements per density plane
+
+private:
+    StepArgs args(int from, int to, bool glob) const;
+    void upload_zone();
+    void launch(int from, int to, bool glob);
+    void drop_graph();
+    void run_graph_steps(int nsteps);
+
+    std::vector<double> settings_;
+    std::vector<double> zone_;  // [zone][zcols]
+    int zcols_, nzones_;
+    bool zone_dirty_ = true;
+    void* snaps_[2] = {nullptr, nullptr};
+    int snap_ = 0, iter_ = 0;
+    uint16_t* d_flags_ = nullptr;
+    double* d_zone_ = nullptr;
+    double* d_gpart_ = nullptr;
+    double* d_glob_ = nullptr;
+    std::vector<double> globals_;
+    int nblocks_ = 0, block_ = 256, bpr_ = 1;
+    hipStream_t stream_ = nullptr;
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    double last_ms_ = 0, kernel_ms_ = 0;
+    // captured step loop: graph_steps_ plain steps starting at snapshot 0
+    hipGraphExec_t graph_exec_ = nullptr;
+    hipGraph_t graph_ = nullptr;
+    int graph_steps_ = 32;
+    unsigned params_version_ = 0, graph_version_ = ~0u;
+};
+
+// hipError_t -> exception with a message naming the call
+void hip_check(hipError_t e, const char* what);
+
+}  // namespace tclbx

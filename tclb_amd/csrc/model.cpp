@@ -1,0 +1,198 @@
+// model.cpp -- model tables (see model.h for the contract they follow).
+#include "model.h"
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+
+namespace tclbx {
+
+int ModelDef::setting_id(const std::string& n) const {
+    for (size_t i = 0; i < settings.size(); i++) if (settings[i].name == n) return (int)i;
+    return -1;
+}
+int ModelDef::zonal_id(const std::string& n) const {
+    for (size_t i = 0; i < zonal.size(); i++) if (zonal[i].name == n) return (int)i;
+    return -1;
+}
+int ModelDef::global_id(const std::string& n) const {
+    for (size_t i = 0; i < globals.size(); i++) if (globals[i].name == n) return (int)i;
+    return -1;
+}
+int ModelDef::quantity_id(const std::string& n) const {
+    for (size_t i = 0; i < quantities.size(); i++) if (quantities[i].name == n) return (int)i;
+    return -1;
+}
+const NodeTypeDef* ModelDef::nodetype(const std::string& n) const {
+    for (auto& t : nodetypes) if (t.name == n) return &t;
+    return nullptr;
+}
+const GroupDef* ModelDef::group(const std::string& n) const {
+    for (auto& g : groups) if (g.name == n) return &g;
+    return nullptr;
+}
+
+namespace {
+
+// NodeType packing of src/conf.R:600-699: groups sorted by name, each gets
+// ceil(log2(n+1)) bits and values 1..n shifted; the zone takes the remaining
+// bits of the 16-bit flag; None (0/0) and Clear (0/ALL) are appended.
+void pack_nodetypes(ModelDef& m, const std::vector<std::pair<std::string, std::string>>& types) {
+    std::map<std::string, std::vector<std::string>> by_group;  // sorted by group name
+    for (auto& t : types) by_group[t.second].push_back(t.first);
+    uint32_t shift_val = 1;
+    int shift_num = 0;
+    for (auto& g : by_group) {
+        int n = (int)g.second.size();
+        int l = (int)std::ceil(std::log2((double)n + 1.0));
+        GroupDef gd;
+        gd.name = g.first;
+        gd.mask = shift_val * ((1u << l) - 1);
+        gd.shift = shift_num;
+        gd.bits = l;
+        gd.max = n;
+        m.groups.push_back(gd);
+        for (int i = 0; i < n; i++) {
+            NodeTypeDef t;
+            t.name = g.second[i];
+            t.group = g.first;
+            t.value = shift_val * (uint32_t)(i + 1);
+            t.mask = gd.mask;
+            m.nodetypes.push_back(t);
+        }
+        shift_val <<= l;
+        shift_num += l;
+    }
+    m.flag_bits = 16;
+    m.zone_shift = shift_num;
+    m.zone_max = (1 << (16 - shift_num)) - 1;
+    GroupDef zg;
+    zg.name = "SETTINGZONE";
+    zg.mask = (uint32_t)m.zone_max << shift_num;
+    zg.shift = shift_num;
+    zg.bits = 16 - shift_num;
+    zg.max = m.zone_max;
+    m.groups.push_back(zg);
+    NodeTypeDef dz;
+    dz.name = "DefaultZone"; dz.group = "SETTINGZONE"; dz.value = 0; dz.mask = zg.mask;
+    m.nodetypes.push_back(dz);
+    GroupDef none; none.name = "NONE";
+    m.groups.push_back(none);
+    NodeTypeDef nn; nn.name = "None"; nn.group = "NONE";
+    m.nodetypes.push_back(nn);
+    GroupDef all; all.name = "ALL"; all.mask = 0xFFFF; all.bits = 16; all.max = 0xFFFF;
+    m.groups.push_back(all);
+    NodeTypeDef cl; cl.name = "Clear"; cl.group = "ALL"; cl.value = 0; cl.mask = 0xFFFF;
+    m.nodetypes.push_back(cl);
+}
+
+// Globals of src/conf.R:740-761: SUM globals, Objective, MAX globals; one
+// zonal <G>InObj weight per SUM global other than Objective.
+void finish_globals(ModelDef& m, const std::vector<GlobalDef>& declared) {
+    for (auto& g : declared) if (!g.is_max) m.globals.push_back(g);
+    m.objective = (int)m.globals.size();
+    m.globals.push_back({"Objective", false});
+    m.sum_globals = (int)m.globals.size();
+    for (auto& g : declared) if (g.is_max) m.globals.push_back(g);
+    for (auto& g : declared)
+        if (!g.is_max) m.zonal.push_back({g.name + "InObj", 0.0, -1, 0});
+}
+
+ModelDef make_d2q9(bool drag) {
+    ModelDef m;
+    m.id = drag ? MODEL_D2Q9_SRT_DRAG : MODEL_D2Q9_SRT;
+    m.name = drag ? "d2q9_SRT_drag" : "d2q9_SRT";
+    m.Q = 9; m.dim = 2;
+    // models/flow/d2q9_SRT/Dynamics.R:6-14
+    m.ex = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+    m.ey = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+    m.ez = std::vector<int>(9, 0);
+    m.w = {4. / 9, 1. / 9, 1. / 9, 1. / 9, 1. / 9, 1. / 36, 1. / 36, 1. / 36, 1. / 36};
+    // Dynamics.R:22-29; Threshold from src/conf.R:769
+    m.settings = {{"omega", 0.0, -1, 0}, {"nu", 0.16666666, 0, 1}, {"Density", 1.0, -1, 0},
+                  {"Threshold", 0.5, -1, 0}};
+    m.zonal = {{"Velocity", 0, -1, 0}, {"Velocity_x", 0, -1, 0}, {"Velocity_y", 0, -1, 0},
+               {"GravitationX", 0, -1, 0}, {"GravitationY", 0, -1, 0}};
+    std::vector<GlobalDef> g;
+    if (drag) g = {{"Drag", false}, {"Lift", false}};
+    finish_globals(m, g);
+    m.quantities = {{"U", true}, {"Rho", false}};  // Dynamics.R:18-19
+    pack_nodetypes(m, {{"EPressure", "BOUNDARY"}, {"WPressure", "BOUNDARY"}, {"WVelocity", "BOUNDARY"},
+                       {"EVelocity", "BOUNDARY"}, {"Solid", "BOUNDARY"}, {"Wall", "BOUNDARY"},
+                       {"BGK", "COLLISION"}});
+    return m;
+}
+
+ModelDef make_d3q19() {
+    ModelDef m;
+    m.id = MODEL_D3Q19;
+    m.name = "d3q19";
+    m.Q = 19; m.dim = 3;
+    // columns 4/6/8 of MRTMAT, models/flow/d3q19/Dynamics.R:1-35
+    m.ex = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+    m.ey = {0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1};
+    m.ez = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1};
+    m.w.assign(19, 1. / 36);
+    m.w[0] = 1. / 3;
+    for (int i = 1; i <= 6; i++) m.w[i] = 1. / 18;
+    // Dynamics.R:40-48, then the free MRT rates of this engine (0 = the
+    // reference's fixed rate of that moment group), then Threshold
+    m.settings = {{"omega", 0.0, -1, 0},        {"nu", 0.16666666, 0, 1},
+                  {"InletVelocity", 0.0, -1, 0}, {"InletPressure", 0.0, 4, 2},
+                  {"InletDensity", 1.0, -1, 0},  {"ForceX", 0.0, -1, 0},
+                  {"ForceY", 0.0, -1, 0},        {"ForceZ", 0.0, -1, 0},
+                  {"MRT_Se", 0.0, -1, 0},        {"MRT_Seps", 0.0, -1, 0},
+                  {"MRT_Sq", 0.0, -1, 0},        {"MRT_Spi", 0.0, -1, 0},
+                  {"MRT_Sm", 0.0, -1, 0},        {"Threshold", 0.5, -1, 0}};
+    std::vector<GlobalDef> g = {{"Flux", false}};
+    for (const char* p : {"XY", "XZ", "YZ"})
+        for (const char* s : {"vx", "vy", "vz", "rho", "area"}) g.push_back({std::string(p) + s, false});
+    for (const char* s : {"vx", "vy", "vz", "px", "py", "pz", "rho", "volume"})
+        g.push_back({std::string("VOL") + s, false});
+    g.push_back({"MaxV", true});
+    finish_globals(m, g);
+    m.quantities = {{"P", false}, {"U", true}};  // Dynamics.R:37-38
+    pack_nodetypes(m, {{"XYslice", "ADDITIONALS"}, {"XZslice", "ADDITIONALS"}, {"YZslice", "ADDITIONALS"},
+                       {"EPressure", "BOUNDARY"}, {"Solid", "BOUNDARY"}, {"Wall", "BOUNDARY"},
+                       {"WPressure", "BOUNDARY"}, {"WPressureL", "BOUNDARY"}, {"WVelocity", "BOUNDARY"},
+                       {"MRT", "COLLISION"}});
+    return m;
+}
+
+const std::vector<ModelDef>& registry() {
+    static std::vector<ModelDef> r = {make_d2q9(false), make_d2q9(true), make_d3q19()};
+    return r;
+}
+
+}  // namespace
+
+const ModelDef* model_by_name(const std::string& name) {
+    for (auto& m : registry()) if (m.name == name) return &m;
+    return nullptr;
+}
+
+const ModelDef* model_by_id(int id) {
+    if (id < 0 || id >= (int)registry().size()) return nullptr;
+    return &registry()[id];
+}
+
+int storage_bytes(int s) {
+    switch (s) {
+    case ST_F64: return 8;
+    case ST_F32: case ST_F32S: return 4;
+    case ST_F16S: case ST_BF16S: return 2;
+    default: return 0;
+    }
+}
+
+const char* storage_name(int s) {
+    static const char* n[] = {"f64", "f32", "f32s", "f16s", "bf16s"};
+    return (s >= 0 && s < ST_COUNT) ? n[s] : "?";
+}
+
+int storage_by_name(const std::string& name) {
+    for (int s = 0; s < ST_COUNT; s++) if (name == storage_name(s)) return s;
+    return -1;
+}
+
+}  // namespace tclbx

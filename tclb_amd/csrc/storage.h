@@ -1,0 +1,67 @@
+// storage.h -- distribution storage formats in HBM and their conversions.
+//
+// A stored value is (f_i - shift_i) in the storage type; reading adds the
+// shift back (src/LatticeAccess.inc.cpp.Rt:14-35, which TCLB applies only to
+// models that declare shift=w_i; here every reduced format is shifted by the
+// lattice weight w_i, i.e. f_i - w_i*rho0 with rho0 = 1).  F64 keeps double
+// arithmetic; the reduced formats compute in float.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "model.h"
+
+namespace tclbx {
+
+template <int ST> struct Store;
+
+template <> struct Store<ST_F64> {
+    using T = double;
+    using R = double;
+    static constexpr bool shifted = false;
+    __device__ __forceinline__ static R get(T v, R) { return v; }
+    __device__ __forceinline__ static T put(R v, R) { return v; }
+};
+
+template <> struct Store<ST_F32> {
+    using T = float;
+    using R = float;
+    static constexpr bool shifted = false;
+    __device__ __forceinline__ static R get(T v, R) { return v; }
+    __device__ __forceinline__ static T put(R v, R) { return v; }
+};
+
+template <> struct Store<ST_F32S> {
+    using T = float;
+    using R = float;
+    static constexpr bool shifted = true;
+    __device__ __forceinline__ static R get(T v, R s) { return v + s; }
+    __device__ __forceinline__ static T put(R v, R s) { return v - s; }
+};
+
+template <> struct Store<ST_F16S> {
+    using T = _Float16;
+    using R = float;
+    static constexpr bool shifted = true;
+    __device__ __forceinline__ static R get(T v, R s) { return (float)v + s; }
+    // float -> half conversion rounds to nearest even (v_cvt_f16_f32)
+    __device__ __forceinline__ static T put(R v, R s) { return (_Float16)(v - s); }
+};
+
+template <> struct Store<ST_BF16S> {
+    using T = uint16_t;  // bfloat16 bits
+    using R = float;
+    static constexpr bool shifted = true;
+    __device__ __forceinline__ static R get(T v, R s) {
+        return __uint_as_float((uint32_t)v << 16) + s;
+    }
+    // round to nearest even on the 16 dropped bits; NaN stays NaN
+    __device__ __forceinline__ static T put(R v, R s) {
+        uint32_t x = __float_as_uint(v - s);
+        if ((x & 0x7fffffffu) > 0x7f800000u) return (T)((x >> 16) | 0x40u);
+        x += 0x7fffu + ((x >> 16) & 1u);
+        return (T)(x >> 16);
+    }
+};
+
+}  // namespace tclbx

@@ -1,0 +1,83 @@
+"""The C-ABI library without a GPU: it loads, exports every entry point
+include/tclbx.h declares, and its model registry holds the reference's
+NodeType bit layout (src/conf.R:600-699), Globals order (src/conf.R:740-761)
+and settings."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from conftest import ROOT, gpu_available
+
+import tclb_amd
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "tclbx.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(tclbx_\w+)\s*\(", text, re.M)))
+
+
+def test_every_declared_symbol_is_exported():
+    lib = tclb_amd.lib()
+    syms = declared_symbols()
+    assert len(syms) >= 30
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert sorted(tclb_amd.SYMBOLS) == syms
+
+
+def test_models():
+    assert tclb_amd.models() == ["d2q9_SRT", "d2q9_SRT_drag", "d3q19"]
+
+
+def test_d2q9_srt_contract():
+    m = tclb_amd.Model("d2q9_SRT")
+    # NodeType values (BOUNDARY group, 3 bits; COLLISION 1 bit; zone from bit 4)
+    want = {"EPressure": 1, "WPressure": 2, "WVelocity": 3, "EVelocity": 4, "Solid": 5, "Wall": 6}
+    for n, v in want.items():
+        assert m.nodetype(n) == (v, 0x7)
+    assert m.nodetype("BGK") == (0x8, 0x8)
+    assert m.group("SETTINGZONE") == (0xFFF0, 4)
+    assert m.nodetype("None") == (0, 0)
+    assert m.nodetype("Clear") == (0, 0xFFFF)
+    assert m.global_names == ["Objective"]
+    assert m.quantities == ["U", "Rho"]
+    assert m.settings[:3] == ["omega", "nu", "Density"]
+    assert m.zonal_settings == ["Velocity", "Velocity_x", "Velocity_y", "GravitationX", "GravitationY"]
+    assert m.setting_default("Density") == 1.0
+    assert m.setting_id("nonexistent") == -1
+
+
+def test_d3q19_contract():
+    m = tclb_amd.Model("d3q19")
+    assert [m.nodetype(n)[0] for n in ("XYslice", "XZslice", "YZslice")] == [1, 2, 3]
+    assert m.group("ADDITIONALS") == (0x3, 0)
+    want = {"EPressure": 0x4, "Solid": 0x8, "Wall": 0xC, "WPressure": 0x10, "WPressureL": 0x14, "WVelocity": 0x18}
+    for n, v in want.items():
+        assert m.nodetype(n) == (v, 0x1C)
+    assert m.nodetype("MRT") == (0x20, 0x20)
+    assert m.group("SETTINGZONE") == (0xFFC0, 6)
+    g = m.global_names
+    assert len(g) == 26 and g[0] == "Flux" and g[24] == "Objective" and g[25] == "MaxV"
+    assert g[16:24] == ["VOLvx", "VOLvy", "VOLvz", "VOLpx", "VOLpy", "VOLpz", "VOLrho", "VOLvolume"]
+    assert len(m.zonal_settings) == 24 and m.zonal_settings[0] == "FluxInObj"
+    assert m.quantities == ["P", "U"]
+    assert m.quantity_id("U") == (1, True)
+
+
+def test_drag_model_globals():
+    m = tclb_amd.Model("d2q9_SRT_drag")
+    assert m.global_names == ["Drag", "Lift", "Objective"]
+    assert m.zonal_settings[-2:] == ["DragInObj", "LiftInObj"]
+
+
+@pytest.mark.skipif(gpu_available(), reason="checks the no-GPU error path")
+def test_lattice_creation_fails_loudly_without_gpu():
+    with pytest.raises(tclb_amd.TclbxError):
+        tclb_amd.Lattice("d3q19", 8, 8, 8)
+
+
+def test_unknown_model_is_an_error():
+    with pytest.raises(tclb_amd.TclbxError):
+        tclb_amd.Lattice("d3q27", 8, 8, 8)

@@ -1,0 +1,279 @@
+"""Parity of the HIP path (libtclbx.so, called through its C ABI) with the CPU
+oracle on the same seeded inputs.
+
+Tolerances (written in each test):
+  * FP64 storage: distributions, rho and u within 1e-12 relative max-norm,
+    Globals within 1e-10 relative (the bars of the reference's own harness,
+    tools/csvdiff:7 and tools/tests.sh:200), NodeType maps bit-exact;
+  * reduced storage (FP32/FP16/BF16, FP32 collision): the oracle emulates the
+    same format; values agree to a few units in the last place of the
+    stored format (the GPU and the oracle differ in the order of float
+    operations, so an occasional rounding flip of one ulp is expected).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import oracle_for_case
+
+import tclb_amd
+from tclb_amd import cases
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_max(a, b):
+    scale = max(np.max(np.abs(b)), 1e-300)
+    return float(np.max(np.abs(a - b)) / scale)
+
+
+def glob_close(g_gpu, g_orc, tol):
+    for i, (a, b) in enumerate(zip(g_gpu, g_orc)):
+        scale = max(abs(a), abs(b))
+        if scale == 0:
+            continue
+        assert abs(a - b) <= tol * scale + 1e-300, (i, a, b)
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+# ------------------------------------------------------- random cases ---
+def random_case(model, nx, ny, nz, seed, steps=10):
+    """Every node type of the model on random nodes, a few zones with random
+    zonal settings, random near-equilibrium initial populations."""
+    rng = np.random.default_rng(seed)
+    m = tclb_amd.Model(model)
+    zs = m.group("SETTINGZONE")[1]
+    if model.startswith("d2q9"):
+        BGK = m.flag("BGK")
+        kinds = [BGK] * 12 + [m.flag("Wall"), m.flag("Solid"), m.flag("Wall", "BGK"), m.flag("Solid", "BGK"),
+                             m.flag("WVelocity", "BGK"), m.flag("EVelocity", "BGK"), m.flag("WPressure", "BGK"),
+                             m.flag("EPressure", "BGK"), m.flag("WVelocity"), 0]
+        w = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+        settings = [("omega", 1.0 / 0.56), ("Density", 1.01)]
+        zonal = []
+        for z in range(4):
+            zonal += [("Velocity_x", z, 0.02 * rng.standard_normal()), ("Velocity_y", z, 0.01 * rng.standard_normal()),
+                      ("GravitationX", z, 1e-4 * rng.standard_normal()), ("GravitationY", z, 1e-4 * rng.standard_normal())]
+        if model.endswith("drag"):
+            zonal += [("DragInObj", 1, 0.5), ("LiftInObj", 2, -0.25)]
+    else:
+        MRT = m.flag("MRT")
+        kinds = [MRT] * 12 + [m.flag("Wall"), m.flag("Solid"), m.flag("Solid", "MRT"), m.flag("Wall", "MRT"),
+                             m.flag("WVelocity", "MRT"), m.flag("WPressure", "MRT"), m.flag("WPressureL", "MRT"),
+                             m.flag("EPressure", "MRT"), m.flag("EPressure"), 0]
+        w = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+        settings = [("nu", 0.05), ("InletVelocity", 0.02), ("InletPressure", 0.003),
+                    ("ForceX", 1e-5), ("ForceY", -2e-6), ("ForceZ", 3e-6)]
+        zonal = [("VOLvxInObj", 1, 0.7), ("XYrhoInObj", 2, 0.1), ("VOLvolumeInObj", -1, 0.01)]
+    n = nx * ny * nz
+    flags = np.array(kinds, dtype=np.uint16)[rng.integers(0, len(kinds), n)]
+    if model == "d3q19":
+        flags |= (rng.integers(0, 4, n) * ((flags & 0x20) > 0)).astype(np.uint16)  # slices on MRT nodes
+    flags |= (rng.integers(0, 4, n) << zs).astype(np.uint16)
+    flags = flags.reshape(nz, ny, nx)
+    Q = len(w)
+    f = w[:, None, None, None] * (1 + 0.02 * rng.standard_normal((Q, nz, ny, nx)))
+    return cases.Case("random", model, nx, ny, nz, flags, settings, zonal, init_f=f, steps=steps)
+
+
+MODELS = [("d2q9_SRT", 24, 10, 1), ("d2q9_SRT_drag", 37, 9, 1), ("d3q19", 13, 7, 6)]
+
+
+@pytest.mark.parametrize("model,nx,ny,nz", MODELS)
+def test_random_case_f64(oracle_mod, model, nx, ny, nz):
+    say(f"random case {model} {nx}x{ny}x{nz} f64")
+    case = random_case(model, nx, ny, nz, seed={"d2q9_SRT": 11, "d2q9_SRT_drag": 12, "d3q19": 13}[model])
+    lat = case.build("f64")
+    orc = oracle_for_case(oracle_mod, case, "f64")
+    for n in (1, 6, 13):
+        lat.iterate(n)
+        g_orc = orc.iterate(n)
+        assert rel_max(lat.get_f(), orc.get_f()) <= 1e-12, n
+        glob_close(lat.globals, g_orc[: lat.nglobals], 1e-10)
+    assert lat.iteration == 20
+
+
+@pytest.mark.parametrize("model,nx,ny,nz", MODELS)
+def test_init_and_quantities(oracle_mod, model, nx, ny, nz):
+    say(f"init/quantities {model}")
+    case = random_case(model, nx, ny, nz, seed=5)
+    case.init_f = None
+    lat = case.build("f64")
+    orc = oracle_for_case(oracle_mod, case, "f64")
+    assert rel_max(lat.get_f(), orc.get_f()) <= 1e-15
+    lat.iterate(3)
+    orc.iterate(3)
+    for q in lat.quantities:
+        a = lat.quantity(q)
+        b = orc.quantity({"U": "U", "Rho": "Rho", "P": "P"}[q])
+        assert rel_max(a, b) <= 1e-12, q
+    # a sub-region and a scale
+    q = lat.quantities[0]
+    sub = lat.quantity(q, region=(1, 2, 0, 5, 3, nz), scale=2.5)
+    full = lat.quantity(q)
+    assert np.allclose(sub, 2.5 * full[0:nz, 2:5, 1:6], rtol=1e-15, atol=0)
+
+
+@pytest.mark.parametrize("model,nx,ny,nz", MODELS)
+def test_flags_roundtrip(model, nx, ny, nz):
+    case = random_case(model, nx, ny, nz, seed=9)
+    lat = tclb_amd.Lattice(model, nx, ny, nz)
+    lat.set_flags(case.flags)
+    assert np.array_equal(lat.get_flags(), case.flags)
+    patch = np.full((nz, 2, 3), 0x1234 & 0xFFFF, np.uint16)
+    lat.set_flags(patch, region=(2, 1, 0, 3, 2, nz))
+    want = case.flags.copy()
+    want[:, 1:3, 2:5] = patch
+    assert np.array_equal(lat.get_flags(), want)
+    assert np.array_equal(lat.get_flags(region=(2, 1, 0, 3, 2, nz)), patch)
+
+
+STORAGE_TOL = {"f32": 4 * 2.0 ** -23, "f32s": 4 * 2.0 ** -23, "f16s": 2 * 2.0 ** -10, "bf16s": 2 * 2.0 ** -7}
+
+
+@pytest.mark.parametrize("storage", ["f32", "f32s", "f16s", "bf16s"])
+@pytest.mark.parametrize("model,nx,ny,nz", MODELS)
+def test_reduced_storage(oracle_mod, model, nx, ny, nz, storage):
+    """One step at a time against the oracle emulating the same format: the
+    difference is bounded by a few units in the last place of the stored
+    value (f - w for shifted formats)."""
+    say(f"reduced storage {model} {storage}")
+    case = random_case(model, nx, ny, nz, seed=17)
+    lat = case.build(storage)
+    orc = oracle_for_case(oracle_mod, case, storage)
+    shifted = storage.endswith("s")
+    w = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4) if lat.Q == 9 else \
+        np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+    for step in range(3):
+        a, b = lat.get_f(), orc.get_f()
+        stored = b - (w[:, None, None, None] if shifted else 0)
+        bound = STORAGE_TOL[storage] * np.maximum(np.abs(stored), 1e-3) + 1e-7
+        assert np.all(np.abs(a - b) <= bound), (step, np.max(np.abs(a - b) / bound))
+        # restart both from the oracle's state so errors do not compound
+        lat.set_f(b)
+        lat.iterate(1)
+        orc.iterate(1)
+
+
+def test_graph_replay_is_bit_identical():
+    say("graph replay vs plain launches")
+    case = random_case("d3q19", 16, 8, 8, seed=21)
+    a = case.build("f64")
+    b = case.build("f64")
+    a.set_graph_steps(8)
+    b.set_graph_steps(0)
+    for n in (40, 3, 17, 64):
+        a.iterate(n)
+        b.iterate(n)
+        assert np.array_equal(a.get_f(), b.get_f())
+        assert np.array_equal(a.globals, b.globals)
+    # a settings change re-captures the graph
+    a.set_setting("ForceX", 2e-5)
+    b.set_setting("ForceX", 2e-5)
+    a.iterate(40)
+    b.iterate(40)
+    assert np.array_equal(a.get_f(), b.get_f())
+
+
+def test_globals_every_step_accumulate(oracle_mod):
+    say("ITER_GLOBS accumulation")
+    case = random_case("d3q19", 9, 6, 5, seed=23)
+    lat = case.build("f64")
+    orc = oracle_for_case(oracle_mod, case, "f64")
+    lat.iterate(4, tclb_amd.ITER_NORM | tclb_amd.ITER_GLOBS)
+    want = np.zeros(26)
+    for _ in range(4):
+        g = orc.iterate(1)
+        want[:25] += g[:25]
+        want[25] = max(want[25], g[25])
+    glob_close(lat.globals, want, 1e-10)
+
+
+def test_save_load_roundtrip(tmp_path):
+    case = random_case("d2q9_SRT", 20, 8, 1, seed=3)
+    for storage in ("f64", "f16s"):
+        a = case.build(storage)
+        a.iterate(5)
+        p = tmp_path / f"state_{storage}.bin"
+        a.save(p)
+        b = case.build(storage)
+        b.load(p)
+        assert np.array_equal(a.get_f(), b.get_f())
+        a.iterate(3)
+        b.iterate(3)
+        assert np.array_equal(a.get_f(), b.get_f())
+    with pytest.raises(tclb_amd.TclbxError):
+        b.load(tmp_path / "missing.bin")
+
+
+def test_bad_arguments_raise():
+    lat = tclb_amd.Lattice("d3q19", 8, 4, 4)
+    with pytest.raises(tclb_amd.TclbxError):
+        lat.set_flags(np.zeros((4, 4, 8), np.uint16), region=(4, 0, 0, 8, 4, 4))
+    with pytest.raises(tclb_amd.TclbxError):
+        lat.set_zone_setting("VOLvxInObj", 1.0, zone=5000)
+    with pytest.raises(tclb_amd.TclbxError):
+        lat.iterate(-1)
+    with pytest.raises(tclb_amd.TclbxError):
+        tclb_amd.Lattice("d2q9_SRT", 8, 4, 2)
+
+
+# ---------------------------------------------------- the workloads ---
+def test_cylinder_full_size_parity(oracle_mod):
+    """The 1024x128 cylinder case: rho and u after 2000 steps within 1e-12
+    relative max-norm, per-step drag/lift within 1e-10."""
+    say("cylinder 1024x128, 2000 steps")
+    case = cases.d2q9_cylinder(steps=2000)
+    lat = case.build("f64")
+    orc = oracle_for_case(oracle_mod, case, "f64")
+    oracle_mod.set_threads(8)
+    try:
+        for n in (1, 999, 1000):
+            lat.iterate(n)
+            g = orc.iterate(n)
+            glob_close(lat.globals, g[:3], 1e-10)
+    finally:
+        oracle_mod.set_threads(1)
+    assert rel_max(lat.quantity("Rho"), orc.quantity("Rho")) <= 1e-12
+    assert rel_max(lat.quantity("U"), orc.quantity("U")) <= 1e-12
+    assert np.array_equal(lat.get_flags(), case.flags)
+
+
+def test_porous_small_parity(oracle_mod):
+    """The porous-channel generator at 40^3 with its free MRT rates:
+    200 steps against the oracle."""
+    say("porous 40^3, 200 steps")
+    case = cases.d3q19_porous(n=40, steps=200)
+    lat = case.build("f64")
+    orc = oracle_for_case(oracle_mod, case, "f64")
+    oracle_mod.set_threads(8)
+    try:
+        lat.iterate(200)
+        g = orc.iterate(200)
+    finally:
+        oracle_mod.set_threads(1)
+    assert rel_max(lat.quantity("U"), orc.quantity("U")) <= 1e-12
+    assert rel_max(lat.quantity("P"), orc.quantity("P")) <= 1e-12
+    glob_close(lat.globals, g, 1e-10)
+
+
+def test_porous_full_size_invariants():
+    """256^3 porous channel (too large for the oracle): total mass is
+    conserved by streaming, bounce-back and MRT, the body force accelerates
+    the flow along +x only on average, and Globals count the fluid volume."""
+    say("porous 256^3 invariants")
+    case = cases.d3q19_porous(n=256, steps=100)
+    fluid = int(np.count_nonzero(case.flags == 0x20))
+    lat = case.build("f64")
+    m0 = 1 + 3 * lat.quantity("P").sum() / lat.nx ** 3
+    lat.iterate(100)
+    g = lat.global_dict()
+    m1 = 1 + 3 * lat.quantity("P").sum() / lat.nx ** 3
+    assert abs(m1 - m0) < 1e-12
+    assert g["VOLvolume"] == fluid
+    assert g["VOLvx"] > 0 and abs(g["VOLvy"]) < 0.2 * g["VOLvx"]

@@ -1,0 +1,468 @@
+"""Pins of the CPU oracle (oracle/) against facts that do not come from the
+oracle itself.
+
+The reference's own golden data for these models lives in the TCLB_tests
+submodule (tests/external, .gitmodules:1-3), which is absent, so the oracle
+is pinned here by:
+  * a sympy derivation of the d3q19 equilibrium moments following the
+    algorithm of src/lib/d3q19.R:24-62 (product-form feq, truncation to
+    second order in J) on the MRTMAT data of models/flow/d3q19/Dynamics.R;
+  * what the mathematics fixes: the BGK fixed point, conservation of mass and
+    momentum, the momentum gain of a body force, MRT with equal rates being
+    BGK, the relaxation factor of every moment group, Zou-He invariants,
+    bounce-back reversal, one-hot streaming with periodic wrap, and the exact
+    Poiseuille solution that halfway bounce-back gives with the reference's
+    q-moment rate 8(2-w)/(8-w);
+  * numpy's IEEE float16 rounding and a brute-force bfloat16 rounding.
+Each check is chosen so that a plausible slip (a wrong sign, index, rate
+group, missing term or transposed matrix) fails it.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+@pytest.fixture(scope="module")
+def O(oracle_mod):
+    return oracle_mod
+
+
+def mrtmat():
+    rows = json.load(open(os.path.join(GOLDEN, "d3q19_mrtmat.json")))["rows"]
+    return np.array(rows, dtype=np.int64)  # [moment k][direction i]
+
+
+# ------------------------------------------------------------- storage ---
+def test_f16_matches_numpy_rounding(O):
+    rng = np.random.default_rng(0)
+    vals = np.concatenate([
+        rng.standard_normal(4000).astype(np.float32) * np.float32(1e-2),
+        rng.uniform(-70000, 70000, 2000).astype(np.float32),
+        (rng.standard_normal(2000) * 1e-6).astype(np.float32),   # subnormal halves
+        np.float32([0.0, -0.0, 65504.0, 65519.0, 65520.0, 6.1035156e-05, 5.9604645e-08, 2.9802322e-08,
+                    np.inf, -np.inf]),
+    ])
+    # exact ties: halfway between two halves
+    h = rng.integers(0, 0x7bff, 500).astype(np.uint16)
+    a = h.view(np.float16).astype(np.float32)
+    b = (h + 1).view(np.float16).astype(np.float32)
+    vals = np.concatenate([vals, (a + (b - a) / 2).astype(np.float32)])
+    L = O.lib()
+    got = np.array([L.orc_f32_to_f16(float(v)) for v in vals], dtype=np.uint16)
+    with np.errstate(over="ignore"):
+        want = vals.astype(np.float16).view(np.uint16)
+    assert np.array_equal(got, want)
+
+
+def test_f16_to_f32_every_pattern(O):
+    L = O.lib()
+    pats = np.arange(0, 0x10000, dtype=np.uint32).astype(np.uint16)
+    f = pats.view(np.float16).astype(np.float32)
+    finite = np.isfinite(f)
+    got = np.array([L.orc_f16_to_f32(int(p)) for p in pats[finite]], dtype=np.float32)
+    assert np.array_equal(got.view(np.uint32), f[finite].view(np.uint32))
+
+
+def test_bf16_round_to_nearest_even(O):
+    rng = np.random.default_rng(1)
+    vals = np.concatenate([rng.standard_normal(3000) * 1e-3, rng.standard_normal(1000) * 10]).astype(np.float32)
+    hi = (rng.integers(0, 0x7f00, 300).astype(np.uint32) << 16)
+    ties = (hi | 0x8000).view(np.float32)
+    vals = np.concatenate([vals, ties])
+    L = O.lib()
+    for v in vals:
+        b = L.orc_f32_to_bf16(float(v))
+        got = np.float64(np.uint32(b << 16).view(np.float32))
+        u = np.float32(v).view(np.uint32)
+        lo = np.float64((u & np.uint32(0xFFFF0000)).view(np.float32))
+        up = np.float64(((u & np.uint32(0xFFFF0000)) + np.uint32(0x10000)).view(np.float32))
+        x = np.float64(v)
+        dlo, dup = abs(x - lo), abs(up - x)
+        if dlo < dup:
+            want = lo
+        elif dup < dlo:
+            want = up
+        else:
+            want = lo if ((u >> np.uint32(16)) & np.uint32(1)) == 0 else up
+        assert got == want, (v, got, want)
+
+
+# ------------------------------------------------------ d3q19 equilibrium ---
+@pytest.fixture(scope="module")
+def req_sympy():
+    """Req(rho, J) by the algorithm of src/lib/d3q19.R:24-62."""
+    sp = pytest.importorskip("sympy")
+    Mk = mrtmat()
+    M = sp.Matrix(19, 19, lambda i, k: int(Mk[k][i]))  # MRTMAT[i, k]
+    U = M[:, [3, 5, 7]]
+    p = [[2 if U[j, a] < 0 else int(U[j, a]) for a in range(3)] for j in range(19)]
+    W = sp.Matrix(19, 19, lambda b, i: sp.prod([sp.Integer(U[b, a]) ** p[i][a] for a in range(3)]))
+    rho, Jx, Jy, Jz, sig = sp.symbols("rho Jx Jy Jz sigma")
+    J = [Jx, Jy, Jz]
+    H = []
+    for j in range(19):
+        h = rho
+        for a in range(3):
+            if p[j][a] == 1:
+                h = h * J[a] / rho
+            if p[j][a] == 2:
+                h = h * (J[a] ** 2 / rho ** 2 + sig)
+        H.append(h)
+    feq = sp.Matrix(1, 19, H) * W.inv()
+    Req = feq.subs(sig, sp.Rational(1, 3)) * M
+    out = []
+    for k in range(19):
+        e = sp.expand(Req[0, k])
+        keep = [t for t in sp.Add.make_args(e) if e != 0 and sp.Poly(t * rho ** 10, Jx, Jy, Jz).total_degree() < 3]
+        out.append(sp.Add(*keep))
+    v = [int(sum(Mk[k][i] ** 2 for i in range(19))) for k in range(19)]
+    Minv = sp.diag(*[sp.Rational(1, vk) for vk in v]) * M.T
+    feq_trunc = sp.Matrix(1, 19, out) * Minv
+    return {
+        "req": sp.lambdify((rho, Jx, Jy, Jz), out, "numpy"),
+        "feq": sp.lambdify((rho, Jx, Jy, Jz), list(feq_trunc), "numpy"),
+        "v": np.array(v, dtype=float),
+    }
+
+
+def test_mrtmat_is_orthogonal():
+    M = mrtmat().astype(float)
+    G = M @ M.T
+    assert np.array_equal(G, np.diag(np.diag(G)))
+    assert list(np.diag(G)) == [19, 2394, 252, 10, 40, 10, 40, 10, 40, 36, 72, 12, 24, 4, 4, 4, 8, 8, 8]
+
+
+def test_d3q19_feq_matches_product_form_derivation(O, req_sympy):
+    rng = np.random.default_rng(2)
+    for _ in range(20):
+        rho = 1 + 0.1 * rng.standard_normal()
+        J = 0.05 * rng.standard_normal(3)
+        got = O.d3q19_feq(rho, *J)
+        want = np.array(req_sympy["feq"](rho, *J), dtype=float)
+        assert np.allclose(got, want, rtol=0, atol=1e-15)
+
+
+def test_d3q19_feq_weights_at_rest(O):
+    f = O.d3q19_feq(1.0, 0, 0, 0)
+    assert np.allclose(f, [1/3] + [1/18] * 6 + [1/36] * 12, rtol=0, atol=1e-16)
+
+
+def _random_f(rng, Q=19, amp=0.01):
+    f = np.array([1/3] + [1/18] * 6 + [1/36] * 12) if Q == 19 else np.array([4/9] + [1/9] * 4 + [1/36] * 4)
+    return f * (1 + amp * rng.standard_normal(Q))
+
+
+def test_d3q19_mrt_equal_rates_is_bgk(O):
+    """With every rate equal to omega, MRT reduces to BGK with the same
+    (product-form, truncated) equilibrium."""
+    rng = np.random.default_rng(3)
+    M = mrtmat()
+    for w in (0.7, 1.3, 1.9):
+        f = _random_f(rng)
+        params = [w, 0, 1, 0, 0, 0, w, w, w, w, w]
+        out = O.d3q19_collide(f, params)
+        rho, J = f.sum(), M[[3, 5, 7]] @ f
+        want = f + w * (O.d3q19_feq(rho, *J) - f)
+        assert np.allclose(out, want, rtol=0, atol=2e-16 * 10)
+
+
+def test_d3q19_moment_groups_relax_at_reference_rates(O, req_sympy):
+    """R'_k - Req_k = gamma_k (R_k - Req_k) with gamma1 = 1-w on e, eps, nu,
+    pi moments and gamma2 = 1 - 8(2-w)/(8-w) on q and m moments
+    (models/flow/d3q19/Dynamics.c.Rt:246-252)."""
+    rng = np.random.default_rng(4)
+    M = mrtmat().astype(float)
+    w = 1.37
+    g1, g2 = 1 - w, 1 - 8 * (2 - w) / (8 - w)
+    gam = np.array([0, g1, g1, 0, g2, 0, g2, 0, g2, g1, g1, g1, g1, g1, g1, g1, g2, g2, g2])
+    f = _random_f(rng, amp=0.05)
+    out = O.d3q19_collide(f, [w, 0, 1, 0, 0, 0] + [0] * 5)
+    R, R2 = M @ f, M @ out
+    rho, J = R[0], R[[3, 5, 7]]
+    req = np.array(req_sympy["req"](rho, *J), dtype=float)
+    for k in [1, 2, 4, 6, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18]:
+        assert abs((R2[k] - req[k]) - gam[k] * (R[k] - req[k])) < 1e-14, k
+    assert abs(R2[0] - R[0]) < 1e-15
+    assert np.allclose(R2[[3, 5, 7]], J, atol=1e-16)
+
+
+def test_d3q19_free_rates_map_to_groups(O, req_sympy):
+    rng = np.random.default_rng(5)
+    M = mrtmat().astype(float)
+    w, se, seps, sq, spi, sm = 1.1, 1.19, 1.4, 1.2, 1.05, 1.6
+    f = _random_f(rng, amp=0.05)
+    out = O.d3q19_collide(f, [w, 0, 1, 0, 0, 0, se, seps, sq, spi, sm])
+    R, R2 = M @ f, M @ out
+    req = np.array(req_sympy["req"](R[0], *R[[3, 5, 7]]), dtype=float)
+    rate = {1: se, 2: seps, 4: sq, 6: sq, 8: sq, 9: w, 11: w, 13: w, 14: w, 15: w, 10: spi, 12: spi,
+            16: sm, 17: sm, 18: sm}
+    for k, s in rate.items():
+        assert abs((R2[k] - req[k]) - (1 - s) * (R[k] - req[k])) < 1e-14, k
+
+
+def test_d3q19_force_adds_rho_F(O):
+    rng = np.random.default_rng(6)
+    M = mrtmat().astype(float)
+    f = _random_f(rng)
+    F = np.array([1e-4, -2e-4, 3e-5])
+    out = O.d3q19_collide(f, [1.2, 0, 1, *F] + [0] * 5)
+    assert abs(out.sum() - f.sum()) < 1e-15
+    assert np.allclose(M[[3, 5, 7]] @ out, M[[3, 5, 7]] @ f + f.sum() * F, rtol=0, atol=1e-16)
+
+
+# ------------------------------------------------------------ lattices ---
+def test_one_hot_streaming_d2q9(O):
+    ex = [0, 1, 0, -1, 0, 1, -1, -1, 1]
+    ey = [0, 0, 1, 0, -1, 1, 1, -1, -1]
+    nx, ny = 5, 4
+    for (x, y) in [(0, 0), (4, 3), (2, 1)]:
+        L = O.OracleLattice("d2q9_SRT", nx, ny, 1)
+        L.set_params([1.0, 1.0])
+        L.set_flags(np.zeros(nx * ny, np.uint16))
+        f = np.zeros((9, 1, ny, nx))
+        for q in range(9):
+            f[q, 0, y, x] = q + 1
+        L.set_f(f)
+        L.iterate(1)
+        g = L.get_f()
+        for q in range(9):
+            want = np.zeros((1, ny, nx))
+            want[0, (y + ey[q]) % ny, (x + ex[q]) % nx] = q + 1
+            assert np.array_equal(g[q], want), q
+
+
+def test_one_hot_streaming_d3q19(O):
+    M = mrtmat()
+    e = M[[3, 5, 7]].T
+    n = (4, 3, 5)  # nx, ny, nz
+    for (x, y, z) in [(0, 0, 0), (3, 2, 4), (1, 1, 2)]:
+        L = O.OracleLattice("d3q19", *n)
+        L.set_params([1.0, 0, 1] + [0] * 8)
+        L.set_flags(np.zeros(np.prod(n), np.uint16))
+        f = np.zeros((19, n[2], n[1], n[0]))
+        for q in range(19):
+            f[q, z, y, x] = q + 1
+        L.set_f(f)
+        L.iterate(1)
+        g = L.get_f()
+        for q in range(19):
+            want = np.zeros((n[2], n[1], n[0]))
+            want[(z + e[q][2]) % n[2], (y + e[q][1]) % n[1], (x + e[q][0]) % n[0]] = q + 1
+            assert np.array_equal(g[q], want), q
+
+
+def _d2q9_periodic(O, f, omega=1.3, grav=(0.0, 0.0), steps=1, storage="f64"):
+    ny, nx = f.shape[2], f.shape[3]
+    L = O.OracleLattice("d2q9_SRT", nx, ny, 1, storage)
+    L.set_params([omega, 1.0])
+    L.set_zone_table(np.array([[0, 0, 0, grav[0], grav[1]]]))
+    L.set_flags(np.full(nx * ny, 8, np.uint16))
+    L.set_f(f)
+    L.iterate(steps)
+    return L
+
+
+def test_d2q9_bgk_fixed_point(O):
+    feq = O.d2q9_feq(1.1, 0.03, -0.02)
+    f = np.broadcast_to(feq[:, None, None, None], (9, 1, 4, 6)).copy()
+    L = _d2q9_periodic(O, f, steps=5)
+    assert np.allclose(L.get_f(), f, rtol=0, atol=4e-16)
+
+
+def test_d2q9_conservation_and_body_force(O):
+    rng = np.random.default_rng(7)
+    ex = np.array([0, 1, 0, -1, 0, 1, -1, -1, 1])
+    ey = np.array([0, 0, 1, 0, -1, 1, 1, -1, -1])
+    f = _random_f(rng, 9)[:, None, None, None] * (1 + 0.01 * rng.standard_normal((9, 1, 6, 7)))
+    mass = f.sum()
+    J = np.array([(f * ex[:, None, None, None]).sum(), (f * ey[:, None, None, None]).sum()])
+    L = _d2q9_periodic(O, f, steps=3)
+    g = L.get_f()
+    assert abs(g.sum() - mass) < 1e-13
+    assert np.allclose([(g * ex[:, None, None, None]).sum(), (g * ey[:, None, None, None]).sum()], J, atol=1e-14)
+    G = (2e-4, -1e-4)
+    L = _d2q9_periodic(O, f, grav=G, steps=1)
+    g = L.get_f()
+    Jn = np.array([(g * ex[:, None, None, None]).sum(), (g * ey[:, None, None, None]).sum()])
+    assert np.allclose(Jn, J + mass * np.array(G), rtol=0, atol=1e-14)
+
+
+def _node_after(O, model, flag, f_node, params, zone_row=None):
+    """Post-boundary state of one node with NodeType `flag` on a 3x3(x3)
+    lattice whose other nodes have flag 0 (stream only)."""
+    if model == "d3q19":
+        L = O.OracleLattice("d3q19", 3, 3, 3)
+        n = 27
+        shape = (19, 3, 3, 3)
+        idx = (slice(None), 1, 1, 1)
+    else:
+        L = O.OracleLattice(model, 3, 3, 1)
+        n = 9
+        shape = (9, 1, 3, 3)
+        idx = (slice(None), 0, 1, 1)
+    L.set_params(params)
+    if zone_row is not None:
+        L.set_zone_table(np.array([zone_row]))
+    flags = np.zeros(n, np.uint16)
+    flags[n // 2] = flag
+    L.set_flags(flags)
+    f = np.zeros(shape)
+    # node (1,1,1) pulls f_i from (1,1,1) - e_i: fill every node with the
+    # same populations so the pulled values are f_node
+    f[:] = np.asarray(f_node).reshape((-1,) + (1,) * (len(shape) - 1))
+    L.set_f(f)
+    L.iterate(1)
+    return L.get_f()[idx]
+
+
+D2_EX = np.array([0, 1, 0, -1, 0, 1, -1, -1, 1])
+D2_EY = np.array([0, 0, 1, 0, -1, 1, 1, -1, -1])
+
+
+@pytest.mark.parametrize("kind", ["WVelocity", "EVelocity"])
+def test_d2q9_zou_he_velocity(O, kind):
+    rng = np.random.default_rng(8)
+    flag = {"WVelocity": 3, "EVelocity": 4}[kind]
+    u0 = 0.037
+    f = _random_f(rng, 9, 0.05)
+    g = _node_after(O, "d2q9_SRT", flag, f, [1.0, 1.0], [0, u0, 0, 0, 0])
+    rho = g.sum()
+    assert abs((g * D2_EX).sum() - rho * u0) < 1e-15
+    assert abs((g * D2_EY).sum()) < 1e-15
+
+
+@pytest.mark.parametrize("kind", ["WPressure", "EPressure"])
+def test_d2q9_zou_he_pressure(O, kind):
+    rng = np.random.default_rng(9)
+    flag = {"WPressure": 2, "EPressure": 1}[kind]
+    f = _random_f(rng, 9, 0.05)
+    g = _node_after(O, "d2q9_SRT", flag, f, [1.0, 1.03])
+    assert abs(g.sum() - 1.03) < 1e-15
+    assert abs((g * D2_EY).sum()) < 1e-15
+
+
+def test_d2q9_bounce_back_reverses(O):
+    rng = np.random.default_rng(10)
+    f = _random_f(rng, 9, 0.05)
+    g = _node_after(O, "d2q9_SRT", 6, f, [1.0, 1.0])
+    opp = [0, 3, 4, 1, 2, 7, 8, 5, 6]
+    assert np.array_equal(g, f[opp])
+
+
+def _d3e():
+    M = mrtmat()
+    return M[3], M[5], M[7]
+
+
+@pytest.mark.parametrize("kind,flag", [("WVelocity", 0x18), ("WPressure", 0x10), ("EPressure", 0x04),
+                                       ("WPressureL", 0x14)])
+def test_d3q19_zou_he(O, kind, flag):
+    rng = np.random.default_rng(11)
+    ex, ey, ez = _d3e()
+    f = _random_f(rng, 19, 0.05)
+    uin, rin = 0.031, 1.02
+    g = _node_after(O, "d3q19", flag, f, [1.0, uin, rin] + [0] * 8)
+    rho = g.sum()
+    jx, jy, jz = (g * ex).sum(), (g * ey).sum(), (g * ez).sum()
+    assert abs(jy) < 1e-15 and abs(jz) < 1e-15
+    if kind == "WVelocity":
+        assert abs(jx - rho * uin) < 1e-15
+    elif kind == "WPressure":
+        assert abs(rho - rin) < 1e-15
+    elif kind == "EPressure":
+        assert abs(rho - 1.0) < 1e-15
+    else:  # limited: pressure inlet unless the velocity would exceed InletVelocity
+        assert abs(rho - rin) < 1e-15 or abs(jx - rho * uin) < 1e-15
+
+
+def test_d3q19_bounce_back_reverses(O):
+    rng = np.random.default_rng(12)
+    ex, ey, ez = _d3e()
+    f = _random_f(rng, 19, 0.05)
+    g = _node_after(O, "d3q19", 0x0C, f, [1.0, 0, 1] + [0] * 8)
+    opp = [next(j for j in range(19) if ex[j] == -ex[i] and ey[j] == -ey[i] and ez[j] == -ez[i]) for i in range(19)]
+    assert np.array_equal(g, f[opp])
+
+
+def test_d3q19_globals_on_a_periodic_box(O):
+    """All-MRT periodic box, one step with Globals: the reference evaluates
+    them after `R /= Mw`, so VOLrho = sum(rho)/19, VOLpx = sum(Jx + rho Fx)/10
+    and the velocities are (J/10)/(rho/19) (Dynamics.c.Rt:262-289)."""
+    rng = np.random.default_rng(13)
+    nx, ny, nz = 4, 5, 3
+    ex, ey, ez = _d3e()
+    base = np.array([1/3] + [1/18] * 6 + [1/36] * 12)
+    f = base[:, None, None, None] * (1 + 0.02 * rng.standard_normal((19, nz, ny, nx)))
+    F = (1e-4, 0, -5e-5)
+    L = O.OracleLattice("d3q19", nx, ny, nz)
+    L.set_params([1.3, 0, 1, *F] + [0] * 5)
+    flags = np.full((nz, ny, nx), 0x20, np.uint16)
+    flags[:, 2, :] |= 1  # XYslice on one row of nodes
+    L.set_flags(flags)
+    L.set_f(f)
+    g = L.iterate(1)
+    names = ["Flux"] + [p + s for p in ("XY", "XZ", "YZ") for s in ("vx", "vy", "vz", "rho", "area")] + \
+            ["VOL" + s for s in ("vx", "vy", "vz", "px", "py", "pz", "rho", "volume")] + ["Objective", "MaxV"]
+    G = dict(zip(names, g))
+    mass = f.sum()
+    # pulled rho/J per node: streaming permutes populations
+    rho = np.zeros((nz, ny, nx))
+    J = np.zeros((3, nz, ny, nx))
+    for q in range(19):
+        s = np.roll(f[q], shift=(ez[q], ey[q], ex[q]), axis=(0, 1, 2))
+        rho += s
+        J[0] += s * ex[q]; J[1] += s * ey[q]; J[2] += s * ez[q]
+    Jn = J + rho[None] * np.array(F)[:, None, None, None]
+    u = (Jn / 10) / (rho / 19)
+    assert G["VOLvolume"] == nx * ny * nz
+    assert abs(G["VOLrho"] - mass / 19) < 1e-13
+    assert np.allclose([G["VOLpx"], G["VOLpy"], G["VOLpz"]], Jn.sum(axis=(1, 2, 3)) / 10, rtol=0, atol=1e-15)
+    assert np.allclose([G["VOLvx"], G["VOLvy"], G["VOLvz"]], u.sum(axis=(1, 2, 3)), rtol=1e-12, atol=1e-15)
+    assert abs(G["MaxV"] - np.sqrt((u ** 2).sum(axis=0)).max()) < 1e-15
+    assert G["XYarea"] == nx * nz
+    assert abs(G["XYrho"] - rho[:, 2, :].sum() / 19) < 1e-14
+    assert abs(G["XYvx"] - u[0][:, 2, :].sum()) < 1e-14
+    assert G["XZarea"] == 0 and G["Flux"] == 0 and G["Objective"] == 0
+
+
+def test_d3q19_objective_adds_zone_weights(O):
+    nx, ny, nz = 3, 3, 2
+    L = O.OracleLattice("d3q19", nx, ny, nz)
+    L.set_params([1.0, 0, 1] + [0] * 8)
+    flags = np.full((nz, ny, nx), 0x20 | (1 << 6), np.uint16)  # zone 1
+    L.set_flags(flags)
+    table = np.zeros((2, 24))
+    table[1, 23] = 0.5   # VOLvolumeInObj
+    table[1, 16] = 2.0   # VOLvxInObj
+    L.set_zone_table(table)
+    L.init()
+    g = L.iterate(1)
+    assert abs(g[24] - (0.5 + 2.0) * nx * ny * nz) < 1e-14
+
+
+def test_d3q19_poiseuille_halfway_bounce_back(O):
+    """Force-driven channel between bounce-back walls on rows 0 and ny-1:
+    with the reference's q rate 8(2-w)/(8-w) the steady profile is the exact
+    parabola u(y) = g/(2 nu) (y - 1/2)(ny - 3/2 - y) (walls half-way between
+    nodes); u taken as J/rho + F/2 (mean of pre- and post-force momentum)."""
+    ny, g, tau = 12, 1e-6, 0.8
+    nu = (tau - 0.5) / 3
+    L = O.OracleLattice("d3q19", 1, ny, 1)
+    L.set_params([1 / tau, 0, 1, g, 0, 0] + [0] * 5)
+    flags = np.full((1, ny, 1), 0x20, np.uint16)
+    flags[0, 0, 0] = flags[0, ny - 1, 0] = 0x0C
+    L.set_flags(flags)
+    L.init()
+    L.iterate(6000)
+    U = L.quantity("U")[0, :, 0, 0]
+    rho = 1 + 3 * L.quantity("P")[0, :, 0]
+    y = np.arange(1, ny - 1)
+    u = U[1:-1] + g / 2 / rho[1:-1]
+    want = g / (2 * nu) * (y - 0.5) * (ny - 1.5 - y)
+    assert np.max(np.abs(u - want)) / want.max() < 1e-9
