@@ -62,7 +62,7 @@ def cpu_baseline(n, seconds, threads):
         "cores": threads,
         "kind": "port",
         "sample": f"same generator at {n}^3 (seed 2, 10% solid), {steps} steps, FP64, "
-                  f"OpenMP over z-planes, {threads} threads, MPI ranks = 1, {dt:.1f} s",
+                  f"OpenMP over (y,z) rows, {threads} threads, MPI ranks = 1, {dt:.1f} s",
     }
 
 

@@ -264,16 +264,16 @@ static void pull(const orc_lattice* L, const void* b, int x, int y, int z, doubl
     }
 }
 
-/* z-planes [z0, z1) of one step */
-static void step_planes(orc_lattice* L, long double* glob, int z0, int z1)
+/* rows r0 <= y + ny*z < r1 of one step */
+static void step_rows(orc_lattice* L, long double* glob, long r0, long r1)
 {
     const void* in = L->buf[L->cur];
     void* out = L->buf[1 - L->cur];
     int x, y, z, q;
+    long r;
     int drag = L->model == ORC_D2Q9_SRT_DRAG;
-    for (z = z0; z < z1; z++)
-        for (y = 0; y < L->ny; y++)
-            for (x = 0; x < L->nx; x++) {
+    for (r = r0; r < r1; r++)
+        for (z = (int)(r / L->ny), y = (int)(r % L->ny), x = 0; x < L->nx; x++) {
                 size_t n = (size_t)x + (size_t)L->nx * ((size_t)y + (size_t)L->ny * z);
                 uint16_t nt = L->flags[n];
                 const double* zr = zone_row(L, nt);
@@ -290,29 +290,30 @@ static void step_planes(orc_lattice* L, long double* glob, int z0, int z1)
                     else d2q9_node_f(ff, nt, (float)L->params[0], (float)L->params[1], zr, drag, glob);
                     for (q = 0; q < L->Q; q++) st(L, out, (size_t)q * L->N + n, q, ff[q]);
                 }
-            }
+        }
 }
 
 static int g_threads = 1;
 void orc_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
 
-/* One step.  With several threads the z-planes are split into contiguous
+/* One step.  With several threads the (y,z) rows are split into contiguous
  * chunks (OpenMP, as the reference's --with-openmp CPU build runs its
  * kernel loops); the Globals of the chunks are added in chunk order. */
 static void step(orc_lattice* L, long double* glob)
 {
-    int nt = g_threads < L->nz ? g_threads : L->nz;
+    long rows = (long)L->ny * L->nz;
+    int nt = g_threads < rows ? g_threads : (int)rows;
     if (nt <= 1) {
-        step_planes(L, glob, 0, L->nz);
+        step_rows(L, glob, 0, rows);
     } else {
         long double part[64][32];
         int t, g;
         if (nt > 64) nt = 64;
-        #pragma omp parallel for num_threads(nt) schedule(static, 1)
+        #pragma omp parallel for num_threads(nt) schedule(static, 1) private(g)
         for (t = 0; t < nt; t++) {
-            int z0 = (int)((long long)L->nz * t / nt), z1 = (int)((long long)L->nz * (t + 1) / nt);
+            long r0 = rows * t / nt, r1 = rows * (t + 1) / nt;
             for (g = 0; g < 32; g++) part[t][g] = 0.0L;
-            step_planes(L, glob ? part[t] : NULL, z0, z1);
+            step_rows(L, glob ? part[t] : NULL, r0, r1);
         }
         if (glob)
             for (t = 0; t < nt; t++)

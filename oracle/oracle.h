@@ -63,7 +63,7 @@ void orc_set_f(orc_lattice* L, const double* in);
  * populations of the current snapshot, without boundary or collision. */
 void orc_get_quantity(const orc_lattice* L, int which, double* out);
 
-/* threads of the step loop (OpenMP over z-planes); 1 by default */
+/* threads of the step loop (OpenMP over (y,z) rows); 1 by default */
 void orc_set_threads(int n);
 
 /* storage conversions, exported so tests can pin them */

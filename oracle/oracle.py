@@ -126,7 +126,7 @@ class OracleLattice:
 
 
 def set_threads(n):
-    """Threads of the oracle's step loop (OpenMP over z-planes)."""
+    """Threads of the oracle's step loop (OpenMP over (y,z) rows)."""
     lib().orc_set_threads(int(n))
 
 
