@@ -14,7 +14,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPATH = os.path.join(HERE, "libtclbx.so")
+# TCLBX_LIB selects another build of the library (an A/B comparison of two
+# builds); by default the in-tree build is loaded
+LIBPATH = os.environ.get("TCLBX_LIB") or os.path.join(HERE, "libtclbx.so")
 
 ITER_NORM = 0x001
 ITER_GLOBS = 0x010

@@ -8,8 +8,8 @@ namespace tclbx {
 
 // Model parameters passed by value in every launch (no per-step constant
 // upload).  d2q9: [omega, Density].  d3q19: [omega, InletVelocity,
-// InletDensity, ForceX, ForceY, ForceZ, MRT_Se, MRT_Seps, MRT_Sq, MRT_Spi,
-// MRT_Sm].
+// InletDensity, ForceX, ForceY, ForceZ, then the relaxation factors of the
+// e, eps, q, pi and m moment groups, then gamma1 = 1 - omega (nu group)].
 constexpr int MAX_PAR = 12;
 constexpr int MAX_GLOBALS = 32;
 
@@ -18,7 +18,7 @@ struct StepArgs {
     void* fout;            // next snapshot
     const uint16_t* flags; // NodeType per node, x fastest
     const double* zone;    // zonal settings [zone][zcols]
-    double* gpart;         // per-block partial Globals [nblocks][nglob]
+    double* gpart;         // per-block partial Globals [nblocks][nglob][2] (sum, compensation)
     long long pitch;       // elements per density plane
     int nx, ny, nz;
     int bpr;               // blocks per (y,z) row

@@ -80,6 +80,22 @@ __device__ __forceinline__ void d2q9_feq(R* f_eq, R d, R ux, R uy) {
     f_eq[8] = ( R(1.) + ( ( -1 + uy )*uy + ( 1 + ux - uy*R(3.) )*ux )*R(3.) )*d/R(36.);
 }
 
+// SetEquilibrium for the float path: the same polynomial with the divisions
+// by 9, 18 and 36 as products (held to a float tolerance, not to identical
+// roundings)
+__device__ __forceinline__ void d2q9_feq_fast(float* f_eq, float d, float ux, float uy) {
+    const float d9 = d * (2.f/9.f), d18 = d * (1.f/18.f), d36 = d * (1.f/36.f);
+    f_eq[0] = ( 2.f + ( -uy*uy - ux*ux )*3.f )*d9;
+    f_eq[1] = ( 2.f + ( -uy*uy + ( 1 + ux )*ux*2.f )*3.f )*d18;
+    f_eq[2] = ( 2.f + ( -ux*ux + ( 1 + uy )*uy*2.f )*3.f )*d18;
+    f_eq[3] = ( 2.f + ( -uy*uy + ( -1 + ux )*ux*2.f )*3.f )*d18;
+    f_eq[4] = ( 2.f + ( -ux*ux + ( -1 + uy )*uy*2.f )*3.f )*d18;
+    f_eq[5] = ( 1.f + ( ( 1 + uy )*uy + ( 1 + ux + uy*3.f )*ux )*3.f )*d36;
+    f_eq[6] = ( 1.f + ( ( 1 + uy )*uy + ( -1 + ux - uy*3.f )*ux )*3.f )*d36;
+    f_eq[7] = ( 1.f + ( ( -1 + uy )*uy + ( -1 + ux + uy*3.f )*ux )*3.f )*d36;
+    f_eq[8] = ( 1.f + ( ( -1 + uy )*uy + ( 1 + ux - uy*3.f )*ux )*3.f )*d36;
+}
+
 // Run(), Dynamics.c:40-67, plus (DRAG) Ladd momentum exchange on Solid nodes.
 // zone columns: Velocity, Velocity_x, Velocity_y, GravitationX,
 // GravitationY, [DragInObj, LiftInObj].
@@ -137,10 +153,17 @@ __device__ __forceinline__ void d2q9_node(R* f, unsigned nt, R omega, R Density,
     }
     if (nt & 0x8u) {  // BGK
         R d = f[8]+f[7]+f[6]+f[5]+f[4]+f[3]+f[2]+f[1]+f[0];
-        R ux = (( f[8]-f[7]-f[6]+f[5]-f[3]+f[1] )/d + (R)zr[3]/omega );
-        R uy = ((-f[8]-f[7]+f[6]+f[5]-f[4]+f[2] )/d + (R)zr[4]/omega );
         R f_eq[9];
-        d2q9_feq(f_eq, d, ux, uy);
+        if constexpr (sizeof(R) == 8) {
+            R ux = (( f[8]-f[7]-f[6]+f[5]-f[3]+f[1] )/d + (R)zr[3]/omega );
+            R uy = ((-f[8]-f[7]+f[6]+f[5]-f[4]+f[2] )/d + (R)zr[4]/omega );
+            d2q9_feq(f_eq, d, ux, uy);
+        } else {
+            const float id = __builtin_amdgcn_rcpf(d), iw = __builtin_amdgcn_rcpf(omega);
+            R ux = ( f[8]-f[7]-f[6]+f[5]-f[3]+f[1] )*id + (R)zr[3]*iw;
+            R uy = (-f[8]-f[7]+f[6]+f[5]-f[4]+f[2] )*id + (R)zr[4]*iw;
+            d2q9_feq_fast(f_eq, d, ux, uy);
+        }
 #pragma unroll
         for (int i = 0; i < 9; i++) f[i] = f[i] + omega*(f_eq[i]-f[i]);
     }
@@ -165,6 +188,27 @@ __device__ __forceinline__ void d3q19_req(R* q, R rho, R Jx, R Jy, R Jz) {
     q[13] = Jx*Jy/rho;
     q[14] = Jy*Jz/rho;
     q[15] = Jx*Jz/rho;
+}
+
+// The same moments for the float path: one reciprocal of rho and constant
+// products instead of the nine IEEE divisions (the float formats are held to
+// a tolerance of a few float ulps, not to identical roundings).
+__device__ __forceinline__ void d3q19_req_fast(float* q, float rho, float Jx, float Jy, float Jz) {
+    const float ir = __builtin_amdgcn_rcpf(rho);
+    const float xx = Jx*Jx*ir, yy = Jy*Jy*ir, zz = Jz*Jz*ir;
+    const float j2 = xx + yy + zz;
+    q[1] = -11.f*rho + 19.f*j2;
+    q[2] = 3.f*rho - 2.f*j2;
+    q[4] = (-2.f/3.f)*Jx;
+    q[6] = (-2.f/3.f)*Jy;
+    q[8] = (-2.f/3.f)*Jz;
+    q[9] = 2.f*xx - yy - zz;
+    q[10] = -q[9];
+    q[11] = yy - zz;
+    q[12] = -q[11];
+    q[13] = Jx*Jy*ir;
+    q[14] = Jy*Jz*ir;
+    q[15] = Jx*Jz*ir;
 }
 
 // f = sum_k r_k M[k][i] for the d3q19 moment basis of
@@ -223,6 +267,17 @@ __device__ __forceinline__ void gadd(double* g, int i, double x, const double* z
     if (GLOB) { g[i] += x; g[24] += zr[i]; }
 }
 
+// XY/XZ/YZ slice Globals vx, vy, vz, rho, area starting at index BASE
+template <bool GLOB, int BASE>
+__device__ __forceinline__ void slice_add(double* g, double ux, double uy, double uz, double rs,
+                                          const double* zr) {
+    gadd<GLOB>(g, BASE + 0, ux, zr);
+    gadd<GLOB>(g, BASE + 1, uy, zr);
+    gadd<GLOB>(g, BASE + 2, uz, zr);
+    gadd<GLOB>(g, BASE + 3, rs, zr);
+    gadd<GLOB>(g, BASE + 4, 1.0, zr);
+}
+
 // CollisionMRT, Dynamics.c.Rt:223-290.  Rates by moment group (0-based):
 // e {1}, eps {2}, q {4,6,8}, nu {9,11,13,14,15}, pi {10,12}, m {16,17,18};
 // the reference uses gamma1 = 1-omega for e/eps/nu/pi and
@@ -231,14 +286,9 @@ __device__ __forceinline__ void gadd(double* g, int i, double x, const double* z
 template <class R, bool GLOB>
 __device__ __forceinline__ void d3q19_mrt(R* f, unsigned nt, const double* par, const double* zr,
                                           double* g) {
-    const R omega = (R)par[0];
-    const R gamma1 = 1 - omega;
-    const R gamma2 = 1 - R(8.0) * ( R(2.0) - omega )/( R(8.0) - omega );
-    const R ge = par[6] > 0 ? 1 - (R)par[6] : gamma1;
-    const R geps = par[7] > 0 ? 1 - (R)par[7] : gamma1;
-    const R gq = par[8] > 0 ? 1 - (R)par[8] : gamma2;
-    const R gpi = par[9] > 0 ? 1 - (R)par[9] : gamma1;
-    const R gm = par[10] > 0 ? 1 - (R)par[10] : gamma2;
+    // relaxation factors of the moment groups, computed on the host
+    const R ge = (R)par[6], geps = (R)par[7], gq = (R)par[8], gpi = (R)par[9], gm = (R)par[10];
+    const R gamma1 = (R)par[11];
 
     // moments R = f M (Dynamics.R:1-21), by symmetry groups
     R A = f[1] + f[2] + f[3] + f[4] + f[5] + f[6];
@@ -274,7 +324,7 @@ __device__ __forceinline__ void d3q19_mrt(R* f, unsigned nt, const double* par, 
 
     const R rho = m[0];
     R q[19];
-    d3q19_req(q, rho, m[3], m[5], m[7]);
+    d3q19_req_fast(q, rho, m[3], m[5], m[7]);
     m[1] = ge * (m[1] - q[1]);
     m[2] = geps * (m[2] - q[2]);
     m[4] = gq * (m[4] - q[4]);
@@ -294,7 +344,7 @@ __device__ __forceinline__ void d3q19_mrt(R* f, unsigned nt, const double* par, 
     m[3] += rho * (R)par[3];
     m[5] += rho * (R)par[4];
     m[7] += rho * (R)par[5];
-    d3q19_req(q, rho, m[3], m[5], m[7]);
+    d3q19_req_fast(q, rho, m[3], m[5], m[7]);
     m[1] += q[1]; m[2] += q[2]; m[4] += q[4]; m[6] += q[6]; m[8] += q[8];
 #pragma unroll
     for (int k = 9; k < 16; k++) m[k] += q[k];
@@ -309,14 +359,111 @@ __device__ __forceinline__ void d3q19_mrt(R* f, unsigned nt, const double* par, 
         // rho/19 and J/10 -- kept as the reference computes them
         const R rs = m[0], jx = m[3], jy = m[5], jz = m[7];
         const R ux = jx / rs, uy = jy / rs, uz = jz / rs;
-        const unsigned add = nt & 0x3u;
-        if (add) {
-            const int base = add == 1 ? 1 : (add == 2 ? 6 : 11);
-            gadd<GLOB>(g, base + 0, ux, zr);
-            gadd<GLOB>(g, base + 1, uy, zr);
-            gadd<GLOB>(g, base + 2, uz, zr);
-            gadd<GLOB>(g, base + 3, rs, zr);
-            gadd<GLOB>(g, base + 4, 1.0, zr);
+        switch (nt & 0x3u) {  // slice Globals, register indices fixed at compile time
+        case 1: slice_add<GLOB, 1>(g, ux, uy, uz, rs, zr); break;
+        case 2: slice_add<GLOB, 6>(g, ux, uy, uz, rs, zr); break;
+        case 3: slice_add<GLOB, 11>(g, ux, uy, uz, rs, zr); break;
+        }
+        gadd<GLOB>(g, 16, ux, zr);
+        gadd<GLOB>(g, 17, uy, zr);
+        gadd<GLOB>(g, 18, uz, zr);
+        gadd<GLOB>(g, 19, jx, zr);
+        gadd<GLOB>(g, 20, jy, zr);
+        gadd<GLOB>(g, 21, jz, zr);
+        gadd<GLOB>(g, 22, rs, zr);
+        gadd<GLOB>(g, 23, 1.0, zr);
+        const double v = (double)sqrt(ux*ux + uy*uy + uz*uz);
+        g[25] = fmax(g[25], v);
+    }
+}
+
+// The d3q19 moment matrix M[k][i] (moment k of direction i), the listed
+// rows of MRTMAT in models/flow/d3q19/Dynamics.R:1-21, and colSums(M^2).
+__device__ constexpr int kM[19][19] = {
+ {  1,  1,  1,  1,  1,  1,  1,  1,  1,  1,  1,  1,  1,  1,  1,  1,  1,  1,  1},
+ {-30,-11,-11,-11,-11,-11,-11,  8,  8,  8,  8,  8,  8,  8,  8,  8,  8,  8,  8},
+ { 12, -4, -4, -4, -4, -4, -4,  1,  1,  1,  1,  1,  1,  1,  1,  1,  1,  1,  1},
+ {  0,  1, -1,  0,  0,  0,  0,  1, -1,  1, -1,  1, -1,  1, -1,  0,  0,  0,  0},
+ {  0, -4,  4,  0,  0,  0,  0,  1, -1,  1, -1,  1, -1,  1, -1,  0,  0,  0,  0},
+ {  0,  0,  0,  1, -1,  0,  0,  1,  1, -1, -1,  0,  0,  0,  0,  1, -1,  1, -1},
+ {  0,  0,  0, -4,  4,  0,  0,  1,  1, -1, -1,  0,  0,  0,  0,  1, -1,  1, -1},
+ {  0,  0,  0,  0,  0,  1, -1,  0,  0,  0,  0,  1,  1, -1, -1,  1,  1, -1, -1},
+ {  0,  0,  0,  0,  0, -4,  4,  0,  0,  0,  0,  1,  1, -1, -1,  1,  1, -1, -1},
+ {  0,  2,  2, -1, -1, -1, -1,  1,  1,  1,  1,  1,  1,  1,  1, -2, -2, -2, -2},
+ {  0, -4, -4,  2,  2,  2,  2,  1,  1,  1,  1,  1,  1,  1,  1, -2, -2, -2, -2},
+ {  0,  0,  0,  1,  1, -1, -1,  1,  1,  1,  1, -1, -1, -1, -1,  0,  0,  0,  0},
+ {  0,  0,  0, -2, -2,  2,  2,  1,  1,  1,  1, -1, -1, -1, -1,  0,  0,  0,  0},
+ {  0,  0,  0,  0,  0,  0,  0,  1, -1, -1,  1,  0,  0,  0,  0,  0,  0,  0,  0},
+ {  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  1, -1, -1,  1},
+ {  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  1, -1, -1,  1,  0,  0,  0,  0},
+ {  0,  0,  0,  0,  0,  0,  0,  1, -1,  1, -1, -1,  1, -1,  1,  0,  0,  0,  0},
+ {  0,  0,  0,  0,  0,  0,  0, -1, -1,  1,  1,  0,  0,  0,  0,  1, -1,  1, -1},
+ {  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  1,  1, -1, -1, -1, -1,  1,  1}};
+__device__ constexpr int kMw[19] = {19, 2394, 252, 10, 40, 10, 40, 10, 40, 36, 72, 12, 24, 4, 4, 4, 8, 8, 8};
+
+// CollisionMRT in the reference's dense form (Dynamics.c.Rt:253-264):
+// R = f M over the nonzero entries of M in ascending order, relaxation,
+// forcing, R /= Mw by division, f = R M^T.  Used for FP64 storage, where it
+// gives the same roundings as the CPU restatement of the reference.
+template <class R, bool GLOB>
+__device__ __forceinline__ void d3q19_mrt_dense(R* f, unsigned nt, const double* par, const double* zr,
+                                                double* g) {
+    // relaxation factors computed on the host in double with the same
+    // operations (gamma1 = 1 - omega, gamma2 = 1 - 8(2-omega)/(8-omega))
+    const R ge = (R)par[6], geps = (R)par[7], gq = (R)par[8], gpi = (R)par[9], gm = (R)par[10];
+    const R gamma1 = (R)par[11];
+    R m[19];
+#pragma unroll
+    for (int k = 0; k < 19; k++) {
+        R acc = 0;
+#pragma unroll
+        for (int i = 0; i < 19; i++)
+            if (kM[k][i] != 0) acc += f[i] * R(kM[k][i]);
+        m[k] = acc;
+    }
+    const R rho = m[0];
+    R q[19];
+    d3q19_req(q, rho, m[3], m[5], m[7]);
+    m[1] = ge * (m[1] - q[1]);
+    m[2] = geps * (m[2] - q[2]);
+    m[4] = gq * (m[4] - q[4]);
+    m[6] = gq * (m[6] - q[6]);
+    m[8] = gq * (m[8] - q[8]);
+    m[9] = gamma1 * (m[9] - q[9]);
+    m[10] = gpi * (m[10] - q[10]);
+    m[11] = gamma1 * (m[11] - q[11]);
+    m[12] = gpi * (m[12] - q[12]);
+    m[13] = gamma1 * (m[13] - q[13]);
+    m[14] = gamma1 * (m[14] - q[14]);
+    m[15] = gamma1 * (m[15] - q[15]);
+    m[16] = gm * (m[16] - 0);
+    m[17] = gm * (m[17] - 0);
+    m[18] = gm * (m[18] - 0);
+    m[3] += rho * (R)par[3];
+    m[5] += rho * (R)par[4];
+    m[7] += rho * (R)par[5];
+    d3q19_req(q, rho, m[3], m[5], m[7]);
+    m[1] += q[1]; m[2] += q[2]; m[4] += q[4]; m[6] += q[6]; m[8] += q[8];
+#pragma unroll
+    for (int k = 9; k < 16; k++) m[k] += q[k];
+    m[16] += R(0); m[17] += R(0); m[18] += R(0);
+#pragma unroll
+    for (int k = 0; k < 19; k++) m[k] /= R(kMw[k]);
+#pragma unroll
+    for (int i = 0; i < 19; i++) {
+        R acc = 0;
+#pragma unroll
+        for (int k = 0; k < 19; k++)
+            if (kM[k][i] != 0) acc += m[k] * R(kM[k][i]);
+        f[i] = acc;
+    }
+    if (GLOB) {
+        const R rs = m[0], jx = m[3], jy = m[5], jz = m[7];
+        const R ux = jx / rs, uy = jy / rs, uz = jz / rs;
+        switch (nt & 0x3u) {  // slice Globals, register indices fixed at compile time
+        case 1: slice_add<GLOB, 1>(g, ux, uy, uz, rs, zr); break;
+        case 2: slice_add<GLOB, 6>(g, ux, uy, uz, rs, zr); break;
+        case 3: slice_add<GLOB, 11>(g, ux, uy, uz, rs, zr); break;
         }
         gadd<GLOB>(g, 16, ux, zr);
         gadd<GLOB>(g, 17, uy, zr);
@@ -412,14 +559,33 @@ __device__ __forceinline__ void d3q19_node(R* f, unsigned nt, const double* par,
         break; }
     default: break;
     }
-    if ((nt & 0x20u) == 0x20u) d3q19_mrt<R, GLOB>(f, nt, par, zr, g);
+    if ((nt & 0x20u) == 0x20u) {
+        if constexpr (sizeof(R) == 8) d3q19_mrt_dense<R, GLOB>(f, nt, par, zr, g);
+        else d3q19_mrt<R, GLOB>(f, nt, par, zr, g);
+    }
 }
 
 // ---------------------------------------------------------- reductions ---
-__device__ __forceinline__ double wave_sum(double v) {
+// Sums are carried as unevaluated pairs (s, c), s + c, and combined with
+// Knuth's TwoSum, so the result is accurate to a few ulps of the exact sum
+// whatever the order: block and launch boundaries do not change it beyond
+// that, and cancelling sums keep their relative accuracy.
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+    s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+__device__ __forceinline__ void dd_add(double& s, double& c, double s2, double c2) {
+    double e;
+    two_sum(s, s2, s, e);
+    c = c + c2 + e;
+}
+__device__ __forceinline__ void wave_sum_dd(double& s, double& c) {
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    for (int o = 32; o > 0; o >>= 1) {
+        const double s2 = __shfl_xor(s, o, 64), c2 = __shfl_xor(c, o, 64);
+        dd_add(s, c, s2, c2);
+    }
 }
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
@@ -427,23 +593,30 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
-// per-block partials: wave shuffle, then LDS across the block's waves
+// per-block partials [block][G][2]: wave shuffles, then LDS across the
+// block's wavefronts in wave order
 template <int G>
 __device__ __forceinline__ void block_globals(const double* g, int sum_globals, double* gpart) {
-    __shared__ double lds[16][G];
+    __shared__ double lds[16][G][2];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nw = (blockDim.x + 63) >> 6;
 #pragma unroll
     for (int i = 0; i < G; i++) {
-        double v = i < sum_globals ? wave_sum(g[i]) : wave_max(g[i]);
-        if (lane == 0) lds[wave][i] = v;
+        double v = g[i], c = 0.0;
+        if (i < sum_globals) wave_sum_dd(v, c);
+        else v = wave_max(v);
+        if (lane == 0) { lds[wave][i][0] = v; lds[wave][i][1] = c; }
     }
     __syncthreads();
     if ((int)threadIdx.x < G) {
         const int i = threadIdx.x;
-        double v = lds[0][i];
-        for (int w = 1; w < nw; w++) v = i < sum_globals ? v + lds[w][i] : fmax(v, lds[w][i]);
-        gpart[(size_t)blockIdx.x * G + i] = v;
+        double v = lds[0][i][0], c = lds[0][i][1];
+        for (int w = 1; w < nw; w++) {
+            if (i < sum_globals) dd_add(v, c, lds[w][i][0], lds[w][i][1]);
+            else v = fmax(v, lds[w][i][0]);
+        }
+        gpart[((size_t)blockIdx.x * G + i) * 2] = v;
+        gpart[((size_t)blockIdx.x * G + i) * 2 + 1] = c;
     }
 }
 
@@ -476,7 +649,10 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a) {
     if (zone >= a.nzones) zone = 0;
     const double* zr = a.zone + (size_t)zone * a.zcols;
 
-    const T* fin = (const T*)a.fin;
+    // plane base pointers are uniform (scalar registers); the per-lane part
+    // of every address is a 32-bit byte offset (a plane is < 4 GiB)
+    const char* fin = (const char*)a.fin;
+    const size_t pbytes = (size_t)a.pitch * sizeof(T);
     R f[Q];
 #pragma unroll
     for (int i = 0; i < Q; i++) {
@@ -485,7 +661,8 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a) {
         const int sy = L::ey(i) > 0 ? ym : (L::ey(i) < 0 ? yp : y);
         const int sz = L::ez(i) > 0 ? zm : (L::ez(i) < 0 ? zp : z);
         const unsigned src = (unsigned)sx + (unsigned)nx * ((unsigned)sy + (unsigned)ny * (unsigned)sz);
-        f[i] = S::get(fin[(size_t)i * a.pitch + src], (R)L::w(i));
+        const T v = *(const T*)(fin + (size_t)i * pbytes + src * (unsigned)sizeof(T));
+        f[i] = S::get(v, (R)L::w(i));
     }
 
     double g[L::G > 0 ? L::G : 1];
@@ -499,9 +676,12 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a) {
         } else {
             d2q9_node<R, MODEL == MODEL_D2Q9_SRT_DRAG, GLOB>(f, nt, (R)a.par[0], (R)a.par[1], zr, g);
         }
-        T* fout = (T*)a.fout;
+        // streaming stores: nothing reads them back this step
+        char* fout = (char*)a.fout;
 #pragma unroll
-        for (int i = 0; i < Q; i++) fout[(size_t)i * a.pitch + n] = S::put(f[i], (R)L::w(i));
+        for (int i = 0; i < Q; i++)
+            __builtin_nontemporal_store(S::put(f[i], (R)L::w(i)),
+                                        (T*)(fout + (size_t)i * pbytes + n * (unsigned)sizeof(T)));
     }
     if (GLOB) {
         constexpr int SUMG = (MODEL == MODEL_D3Q19) ? 25 : L::G;
@@ -615,40 +795,49 @@ __global__ void k_set_density(void* buf, long long pitch, long long n, int q, do
     ((typename S::T*)buf)[q * pitch + i] = S::put((R)in[i], (R)w);
 }
 
-// deterministic second pass: one block per Global, fixed-order tree
+// deterministic second pass: one block per Global, fixed-order tree of
+// compensated pairs; the result is added to the device Globals (cleared
+// before a Globals step, accumulated over steps with ITER_GLOBS)
 __global__ void __launch_bounds__(256) k_reduce_globals(const double* gpart, int nblocks, int nglob,
                                                         int sum_globals, double* out) {
     const int gi = blockIdx.x;
     const bool is_sum = gi < sum_globals;
-    double v = 0.0;
+    double v = 0.0, c = 0.0;
     for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
-        const double x = gpart[(size_t)b * nglob + gi];
-        v = is_sum ? v + x : fmax(v, x);
+        const double x = gpart[((size_t)b * nglob + gi) * 2], xc = gpart[((size_t)b * nglob + gi) * 2 + 1];
+        if (is_sum) dd_add(v, c, x, xc);
+        else v = fmax(v, x);
     }
-    __shared__ double lds[4];
-    v = is_sum ? wave_sum(v) : wave_max(v);
+    __shared__ double lds[4][2];
+    if (is_sum) wave_sum_dd(v, c);
+    else v = wave_max(v);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) lds[wave] = v;
+    if (lane == 0) { lds[wave][0] = v; lds[wave][1] = c; }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double r = lds[0];
-        for (int w = 1; w < (int)(blockDim.x >> 6); w++) r = is_sum ? r + lds[w] : fmax(r, lds[w]);
-        // accumulate into the device Globals (cleared before a Globals step)
-        out[gi] = is_sum ? out[gi] + r : fmax(out[gi], r);
+        double r = lds[0][0], rc = lds[0][1];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+            if (is_sum) dd_add(r, rc, lds[w][0], lds[w][1]);
+            else r = fmax(r, lds[w][0]);
+        }
+        out[gi] = is_sum ? out[gi] + (r + rc) : fmax(out[gi], r);
     }
 }
 
-// plain device copy (16-byte vectors, grid-stride): the bandwidth ceiling
-// the node kernel is compared with, and a kernel of known traffic for
-// calibrating the HBM byte counters
+// plain device copy, one 16-byte vector per thread (the fastest of the copy
+// shapes measured on MI355X): the bandwidth ceiling the node kernel is
+// compared with, and a kernel of known traffic for calibrating the HBM byte
+// counters
 __global__ void __launch_bounds__(256) k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        dst[i] = src[i];
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dst[i] = src[i];
 }
 
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s) {
     const size_t n = bytes / sizeof(uint4);
-    hipLaunchKernelGGL(k_copy, dim3(256 * 64), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, n);
+    const size_t blocks = (n + 255) / 256;
+    if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, n);
     return hipGetLastError();
 }
 

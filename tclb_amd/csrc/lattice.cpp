@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 
@@ -38,6 +39,13 @@ Lattice::Lattice(const ModelDef* m, int nx_, int ny_, int nz_, int storage_)
     const int S = storage_bytes(storage);
     const size_t align = 256 / S;
     pitch = (N + align - 1) / align * align;
+    // optional extra spacing between density planes (bytes, a multiple of
+    // 256): keeps the Q streams of a step off the same HBM channels when the
+    // plane size is a large power of two
+    if (const char* e = std::getenv("TCLBX_PLANE_PAD_BYTES")) {
+        const long long pad = std::atoll(e);
+        if (pad > 0) pitch += ((size_t)pad + 255) / 256 * 256 / S;
+    }
 
     settings_.resize(m->settings.size());
     // defaults of the non-deriving settings (src/Solver.cpp.Rt:277-280)
@@ -88,7 +96,8 @@ void Lattice::set_block(int b) {
     nblocks_ = bpr_ * ny * nz;
     if (d_gpart_) (void)hipFree(d_gpart_);
     d_gpart_ = nullptr;
-    hip_check(hipMalloc((void**)&d_gpart_, (size_t)nblocks_ * kernel_nglobals(model->id) * sizeof(double)),
+    // compensated partial sums: two doubles per Global per block
+    hip_check(hipMalloc((void**)&d_gpart_, (size_t)nblocks_ * kernel_nglobals(model->id) * 2 * sizeof(double)),
               "hipMalloc(partials)");
     drop_graph();
 }
@@ -181,9 +190,22 @@ StepArgs Lattice::args(int from, int to, bool glob) const {
     a.nzones = nzones_;
     auto S = [&](const char* n) { int i = model->setting_id(n); return i >= 0 ? settings_[i] : 0.0; };
     if (model->id == MODEL_D3Q19) {
-        const char* names[] = {"omega", "InletVelocity", "InletDensity", "ForceX", "ForceY", "ForceZ",
-                               "MRT_Se", "MRT_Seps", "MRT_Sq", "MRT_Spi", "MRT_Sm"};
-        for (int i = 0; i < 11; i++) a.par[i] = S(names[i]);
+        const char* names[] = {"omega", "InletVelocity", "InletDensity", "ForceX", "ForceY", "ForceZ"};
+        for (int i = 0; i < 6; i++) a.par[i] = S(names[i]);
+        // relaxation factors of CollisionMRT (models/flow/d3q19/Dynamics.c.Rt:
+        // 246-252): gamma1 = 1 - omega on e, eps, nu, pi moments, gamma2 =
+        // 1 - 8(2-omega)/(8-omega) on q and m moments; a positive MRT_S*
+        // setting gives its group 1 - S instead
+        const double omega = a.par[0];
+        const double gamma1 = 1 - omega;
+        const double gamma2 = 1 - 8.0 * (2.0 - omega) / (8.0 - omega);
+        auto rate = [&](const char* n, double dflt) { double v = S(n); return v > 0 ? 1 - v : dflt; };
+        a.par[6] = rate("MRT_Se", gamma1);
+        a.par[7] = rate("MRT_Seps", gamma1);
+        a.par[8] = rate("MRT_Sq", gamma2);
+        a.par[9] = rate("MRT_Spi", gamma1);
+        a.par[10] = rate("MRT_Sm", gamma2);
+        a.par[11] = gamma1;
     } else {
         a.par[0] = S("omega");
         a.par[1] = S("Density");

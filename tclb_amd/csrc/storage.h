@@ -49,19 +49,12 @@ template <> struct Store<ST_F16S> {
 };
 
 template <> struct Store<ST_BF16S> {
-    using T = uint16_t;  // bfloat16 bits
+    using T = __bf16;
     using R = float;
     static constexpr bool shifted = true;
-    __device__ __forceinline__ static R get(T v, R s) {
-        return __uint_as_float((uint32_t)v << 16) + s;
-    }
-    // round to nearest even on the 16 dropped bits; NaN stays NaN
-    __device__ __forceinline__ static T put(R v, R s) {
-        uint32_t x = __float_as_uint(v - s);
-        if ((x & 0x7fffffffu) > 0x7f800000u) return (T)((x >> 16) | 0x40u);
-        x += 0x7fffu + ((x >> 16) & 1u);
-        return (T)(x >> 16);
-    }
+    __device__ __forceinline__ static R get(T v, R s) { return (float)v + s; }
+    // float -> bfloat16 rounds to nearest even (v_cvt_pk_bf16_f32)
+    __device__ __forceinline__ static T put(R v, R s) { return (__bf16)(v - s); }
 };
 
 }  // namespace tclbx
