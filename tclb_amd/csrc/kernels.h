@@ -18,14 +18,17 @@ struct StepArgs {
     void* fout;            // next snapshot
     const uint16_t* flags; // NodeType per node, x fastest
     const double* zone;    // zonal settings [zone][zcols]
-    double* gpart;         // per-block partial Globals [nblocks][nglob][2] (sum, compensation)
+    double* gpart;         // per-block partial Globals [nglob][nblocks][2] (sum, compensation)
     long long pitch;       // elements per density plane
     int nx, ny, nz;
     int bpr;               // blocks per (y,z) row
+    int rpb;               // rows walked by one block of a Globals step
     int zcols, nzones;
     double par[MAX_PAR];
 };
 
+// Blocks of one step launch (a Globals step walks a.rpb rows per block).
+int step_blocks(const StepArgs& a, bool globals);
 // One time step: pull streaming + NodeType dispatch + collision (+ Globals).
 hipError_t launch_step(int model, int storage, bool globals, const StepArgs& a, int block,
                        hipStream_t s);

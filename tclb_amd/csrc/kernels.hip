@@ -593,30 +593,38 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
-// per-block partials [block][G][2]: wave shuffles, then LDS across the
-// block's wavefronts in wave order
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Per-block partials, stored Global-major [G][nblocks][2] so the final pass
+// reads them contiguously.  A Global that is zero on every lane of a
+// wavefront is skipped (most slice Globals are, away from slice nodes); the
+// 64 lanes are summed in double, the wavefronts of the block with TwoSum.
 template <int G>
 __device__ __forceinline__ void block_globals(const double* g, int sum_globals, double* gpart) {
-    __shared__ double lds[16][G][2];
+    __shared__ double lds[16][G];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nw = (blockDim.x + 63) >> 6;
 #pragma unroll
     for (int i = 0; i < G; i++) {
-        double v = g[i], c = 0.0;
-        if (i < sum_globals) wave_sum_dd(v, c);
-        else v = wave_max(v);
-        if (lane == 0) { lds[wave][i][0] = v; lds[wave][i][1] = c; }
+        double v = g[i];
+        if (__ballot(v != 0.0) != 0) v = i < sum_globals ? wave_sum(v) : wave_max(v);
+        if (lane == 0) lds[wave][i] = v;
     }
     __syncthreads();
     if ((int)threadIdx.x < G) {
         const int i = threadIdx.x;
-        double v = lds[0][i][0], c = lds[0][i][1];
+        double v = lds[0][i], c = 0.0;
         for (int w = 1; w < nw; w++) {
-            if (i < sum_globals) dd_add(v, c, lds[w][i][0], lds[w][i][1]);
-            else v = fmax(v, lds[w][i][0]);
+            if (i < sum_globals) dd_add(v, c, lds[w][i], 0.0);
+            else v = fmax(v, lds[w][i]);
         }
-        gpart[((size_t)blockIdx.x * G + i) * 2] = v;
-        gpart[((size_t)blockIdx.x * G + i) * 2 + 1] = c;
+        const size_t k = ((size_t)i * gridDim.x + blockIdx.x) * 2;
+        gpart[k] = v;
+        gpart[k + 1] = c;
     }
 }
 
@@ -629,17 +637,28 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a) {
     using R = typename S::R;
     constexpr int Q = L::Q;
 
-    // block -> (x segment, row); uniform scalar arithmetic
-    const int row = blockIdx.x / a.bpr;
-    const int xb = blockIdx.x - row * a.bpr;
-    const int y = row % a.ny;
-    const int z = row / a.ny;
+    // block -> (x segment, first row); uniform scalar arithmetic.  A plain
+    // step block covers one x segment of one (y,z) row; a Globals block walks
+    // `rpb` consecutive rows so that one block reduction covers them all.
+    const int rpb = GLOB ? a.rpb : 1;
+    const int rowblk = blockIdx.x / a.bpr;
+    const int xb = blockIdx.x - rowblk * a.bpr;
     const int x = xb * blockDim.x + threadIdx.x;
-    const bool active = x < a.nx;
-    const int xc = active ? x : a.nx - 1;
-
+    const bool xin = x < a.nx;
+    const int xc = xin ? x : a.nx - 1;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int xm = xc == 0 ? nx - 1 : xc - 1, xp = xc == nx - 1 ? 0 : xc + 1;
+
+    double g[L::G > 0 ? L::G : 1];
+    if (GLOB) {
+#pragma unroll
+        for (int i = 0; i < L::G; i++) g[i] = 0.0;
+    }
+    const int nrows = ny * nz;
+    for (int row = rowblk * rpb; row < (rowblk + 1) * rpb && row < nrows; row++) {
+    const bool active = xin;
+    const int y = row % ny;
+    const int z = row / ny;
     const int ym = y == 0 ? ny - 1 : y - 1, yp = y == ny - 1 ? 0 : y + 1;
     const int zm = z == 0 ? nz - 1 : z - 1, zp = z == nz - 1 ? 0 : z + 1;
 
@@ -665,11 +684,6 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a) {
         f[i] = S::get(v, (R)L::w(i));
     }
 
-    double g[L::G > 0 ? L::G : 1];
-    if (GLOB) {
-#pragma unroll
-        for (int i = 0; i < L::G; i++) g[i] = 0.0;
-    }
     if (active) {
         if constexpr (MODEL == MODEL_D3Q19) {
             d3q19_node<R, GLOB>(f, nt, a.par, zr, g);
@@ -683,6 +697,7 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a) {
             __builtin_nontemporal_store(S::put(f[i], (R)L::w(i)),
                                         (T*)(fout + (size_t)i * pbytes + n * (unsigned)sizeof(T)));
     }
+    }  // rows
     if (GLOB) {
         constexpr int SUMG = (MODEL == MODEL_D3Q19) ? 25 : L::G;
         block_globals<L::G>(g, SUMG, a.gpart);
@@ -795,18 +810,19 @@ __global__ void k_set_density(void* buf, long long pitch, long long n, int q, do
     ((typename S::T*)buf)[q * pitch + i] = S::put((R)in[i], (R)w);
 }
 
-// deterministic second pass: one block per Global, fixed-order tree of
-// compensated pairs; the result is added to the device Globals (cleared
-// before a Globals step, accumulated over steps with ITER_GLOBS)
+// deterministic second pass: one block per Global, reading that Global's
+// per-block partials contiguously, TwoSum pairs throughout; the result is
+// added to the device Globals (cleared before a Globals step, accumulated
+// over steps with ITER_GLOBS)
 __global__ void __launch_bounds__(256) k_reduce_globals(const double* gpart, int nblocks, int nglob,
                                                         int sum_globals, double* out) {
     const int gi = blockIdx.x;
     const bool is_sum = gi < sum_globals;
+    const double* p = gpart + (size_t)gi * nblocks * 2;
     double v = 0.0, c = 0.0;
     for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
-        const double x = gpart[((size_t)b * nglob + gi) * 2], xc = gpart[((size_t)b * nglob + gi) * 2 + 1];
-        if (is_sum) dd_add(v, c, x, xc);
-        else v = fmax(v, x);
+        if (is_sum) dd_add(v, c, p[2 * b], p[2 * b + 1]);
+        else v = fmax(v, p[2 * b]);
     }
     __shared__ double lds[4][2];
     if (is_sum) wave_sum_dd(v, c);
@@ -847,7 +863,7 @@ int nblocks_of(const StepArgs& a, int block) { return a.bpr * a.ny * a.nz; }
 
 template <int MODEL, int ST>
 hipError_t step_ms(bool glob, const StepArgs& a, int block, hipStream_t s) {
-    dim3 grid(nblocks_of(a, block)), blk(block);
+    dim3 grid(glob ? step_blocks(a, true) : nblocks_of(a, block)), blk(block);
     if (glob) hipLaunchKernelGGL((k_step<MODEL, ST, true>), grid, blk, 0, s, a);
     else hipLaunchKernelGGL((k_step<MODEL, ST, false>), grid, blk, 0, s, a);
     return hipGetLastError();
@@ -910,6 +926,12 @@ double weight(int model, int q) {
     return model == MODEL_D3Q19 ? Lat<MODEL_D3Q19>::w(q) : Lat<MODEL_D2Q9_SRT>::w(q);
 }
 }  // namespace
+
+int step_blocks(const StepArgs& a, bool globals) {
+    const int rows = a.ny * a.nz;
+    const int rpb = globals ? a.rpb : 1;
+    return a.bpr * ((rows + rpb - 1) / rpb);
+}
 
 hipError_t launch_step(int model, int storage, bool globals, const StepArgs& a, int block, hipStream_t s) {
     switch (model) {

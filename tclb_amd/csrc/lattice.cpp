@@ -94,6 +94,10 @@ void Lattice::set_block(int b) {
     block_ = b;
     bpr_ = (nx + b - 1) / b;
     nblocks_ = bpr_ * ny * nz;
+    // rows per block in a Globals step: enough blocks to fill the GPU
+    // several times over, each reducing several rows at once
+    const long long rows = (long long)ny * nz;
+    rpb_ = (int)std::max(1LL, std::min(16LL, rows * bpr_ / 8192));
     if (d_gpart_) (void)hipFree(d_gpart_);
     d_gpart_ = nullptr;
     // compensated partial sums: two doubles per Global per block
@@ -186,6 +190,7 @@ StepArgs Lattice::args(int from, int to, bool glob) const {
     a.pitch = (long long)pitch;
     a.nx = nx; a.ny = ny; a.nz = nz;
     a.bpr = bpr_;
+    a.rpb = rpb_;
     a.zcols = zcols_;
     a.nzones = nzones_;
     auto S = [&](const char* n) { int i = model->setting_id(n); return i >= 0 ? settings_[i] : 0.0; };
@@ -217,8 +222,8 @@ void Lattice::launch(int from, int to, bool glob) {
     StepArgs a = args(from, to, glob);
     hip_check(launch_step(model->id, storage, glob, a, block_, stream_), "step kernel");
     if (glob) {
-        hip_check(launch_reduce_globals(d_gpart_, nblocks_, kernel_nglobals(model->id), model->sum_globals,
-                                        d_glob_, stream_), "globals reduction");
+        hip_check(launch_reduce_globals(d_gpart_, step_blocks(a, true), kernel_nglobals(model->id),
+                                        model->sum_globals, d_glob_, stream_), "globals reduction");
     }
 }
 

@@ -83,7 +83,7 @@ private:
     double* d_gpart_ = nullptr;
     double* d_glob_ = nullptr;
     std::vector<double> globals_;
-    int nblocks_ = 0, block_ = 256, bpr_ = 1;
+    int nblocks_ = 0, block_ = 256, bpr_ = 1, rpb_ = 1;
     hipStream_t stream_ = nullptr;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     double last_ms_ = 0, kernel_ms_ = 0;
