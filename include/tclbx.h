@@ -123,6 +123,46 @@ int tclbx_set_block(tclbx_lattice* h, int block);
 /* Steps per captured hipGraph replay in tclbx_iterate (0 = no graph). */
 int tclbx_set_graph_steps(tclbx_lattice* h, int steps);
 
+/* ---------------------------------------------------------------- batch --
+ * Many independent lattices ("cases") of one model and storage format in
+ * one ragged device arena, advanced together: one persistent kernel launch
+ * per step walks a tile list covering every case.  No counterpart in the
+ * reference (one Lattice per process there); each per-case call mirrors
+ * the Lattice call of the same name above.  A case is computed exactly as
+ * the same case alone (same node arithmetic, Globals reduced over the
+ * case's own tiles in a fixed order). */
+typedef struct tclbx_batch tclbx_batch;
+
+/* capacity_bytes: HBM the arena holds for snapshots and flags of all cases */
+int tclbx_batch_create(const char* model, const char* storage, long long capacity_bytes, tclbx_batch** out);
+int tclbx_batch_destroy(tclbx_batch* b);
+/* arena bytes one nx*ny*nz case needs */
+int tclbx_batch_case_bytes(const tclbx_batch* b, int nx, int ny, int nz, long long* bytes);
+/* *id = new case id, or -1 (and TCLBX_OK) when the arena has no room */
+int tclbx_batch_add_case(tclbx_batch* b, int nx, int ny, int nz, int* id);
+int tclbx_batch_remove_case(tclbx_batch* b, int id);
+/* id -1: every case */
+int tclbx_batch_set_setting(tclbx_batch* b, int id, int setting, double value);
+int tclbx_batch_set_zone_setting(tclbx_batch* b, int id, int zonal, int zone, double value);
+int tclbx_batch_flag_overwrite(tclbx_batch* b, int id, const uint16_t* flags);
+int tclbx_batch_init_case(tclbx_batch* b, int id);
+/* every case advances niter steps (iter_type as tclbx_iterate) */
+int tclbx_batch_iterate(tclbx_batch* b, int niter, int iter_type);
+int tclbx_batch_get_globals(const tclbx_batch* b, int id, double* out, int n);
+int tclbx_batch_get_iteration(const tclbx_batch* b, int id, long long* iter);
+int tclbx_batch_get_quantity(tclbx_batch* b, int id, int q, double* out, double scale);
+int tclbx_batch_get_density(tclbx_batch* b, int id, int q, double* out);
+int tclbx_batch_set_density(tclbx_batch* b, int id, int q, const double* in);
+/* time every tile shape (threads per block 64/128/256, rows per tile
+ * 1/2/4) and persistent grid size on the current cases for `steps` steps
+ * each, without changing their state, and keep the fastest */
+int tclbx_batch_autotune(tclbx_batch* b, int steps, int* block, int* rows_per_tile, int* grid, double* us_per_step);
+int tclbx_batch_set_tuning(tclbx_batch* b, int block, int rows_per_tile, int grid);
+int tclbx_batch_get_timing(const tclbx_batch* b, double* last_ms, double* total_ms);
+int tclbx_batch_info(const tclbx_batch* b, int* ncases, long long* nodes, long long* arena_used,
+                     long long* arena_capacity);
+int tclbx_batch_set_graph_steps(tclbx_batch* b, int steps);
+
 /* Measured device copy bandwidth: `reps` copies of `bytes` (rounded down
  * to 16) between two device buffers by a plain vector-copy kernel;
  * *gbps = 2*bytes*reps / time (read + write), *ms = time of one copy. */

@@ -33,6 +33,12 @@ SYMBOLS = [
     "tclbx_iterate", "tclbx_get_globals", "tclbx_get_iteration", "tclbx_get_quantity",
     "tclbx_get_density", "tclbx_set_density", "tclbx_save", "tclbx_load", "tclbx_get_timing",
     "tclbx_bytes_per_node", "tclbx_set_block", "tclbx_set_graph_steps", "tclbx_copy_bandwidth",
+    "tclbx_batch_create", "tclbx_batch_destroy", "tclbx_batch_case_bytes", "tclbx_batch_add_case",
+    "tclbx_batch_remove_case", "tclbx_batch_set_setting", "tclbx_batch_set_zone_setting",
+    "tclbx_batch_flag_overwrite", "tclbx_batch_init_case", "tclbx_batch_iterate", "tclbx_batch_get_globals",
+    "tclbx_batch_get_iteration", "tclbx_batch_get_quantity", "tclbx_batch_get_density",
+    "tclbx_batch_set_density", "tclbx_batch_autotune", "tclbx_batch_set_tuning", "tclbx_batch_get_timing",
+    "tclbx_batch_info", "tclbx_batch_set_graph_steps",
 ]
 
 
@@ -89,6 +95,27 @@ def lib():
     L.tclbx_set_block.argtypes = [vp, c_int]
     L.tclbx_set_graph_steps.argtypes = [vp, c_int]
     L.tclbx_copy_bandwidth.argtypes = [ctypes.c_longlong, c_int, dp, dp]
+    ll, llp = ctypes.c_longlong, ctypes.POINTER(ctypes.c_longlong)
+    L.tclbx_batch_create.argtypes = [c_char_p, c_char_p, ll, ctypes.POINTER(vp)]
+    L.tclbx_batch_destroy.argtypes = [vp]
+    L.tclbx_batch_case_bytes.argtypes = [vp, c_int, c_int, c_int, llp]
+    L.tclbx_batch_add_case.argtypes = [vp, c_int, c_int, c_int, ip]
+    L.tclbx_batch_remove_case.argtypes = [vp, c_int]
+    L.tclbx_batch_set_setting.argtypes = [vp, c_int, c_int, c_double]
+    L.tclbx_batch_set_zone_setting.argtypes = [vp, c_int, c_int, c_int, c_double]
+    L.tclbx_batch_flag_overwrite.argtypes = [vp, c_int, u16p]
+    L.tclbx_batch_init_case.argtypes = [vp, c_int]
+    L.tclbx_batch_iterate.argtypes = [vp, c_int, c_int]
+    L.tclbx_batch_get_globals.argtypes = [vp, c_int, dp, c_int]
+    L.tclbx_batch_get_iteration.argtypes = [vp, c_int, llp]
+    L.tclbx_batch_get_quantity.argtypes = [vp, c_int, c_int, dp, c_double]
+    L.tclbx_batch_get_density.argtypes = [vp, c_int, c_int, dp]
+    L.tclbx_batch_set_density.argtypes = [vp, c_int, c_int, dp]
+    L.tclbx_batch_autotune.argtypes = [vp, c_int, ip, ip, ip, dp]
+    L.tclbx_batch_set_tuning.argtypes = [vp, c_int, c_int, c_int]
+    L.tclbx_batch_get_timing.argtypes = [vp, dp, dp]
+    L.tclbx_batch_info.argtypes = [vp, ip, llp, llp, llp]
+    L.tclbx_batch_set_graph_steps.argtypes = [vp, c_int]
     _lib = L
     return L
 
@@ -320,3 +347,140 @@ class Lattice:
 
     def set_graph_steps(self, steps):
         _check(lib().tclbx_set_graph_steps(self._h, int(steps)))
+
+
+class Batch:
+    """Many independent lattices ("cases") of one model and storage format in
+    one ragged device arena, advanced together by one persistent kernel per
+    step.  Each per-case method mirrors the Lattice method of the same name;
+    a case computes exactly what the same case computes alone."""
+
+    def __init__(self, model, storage="f64", capacity_bytes=1 << 30):
+        h = ctypes.c_void_p()
+        _check(lib().tclbx_batch_create(model.encode(), storage.encode(), int(capacity_bytes), ctypes.byref(h)))
+        self._h = h
+        self.model, self.storage = model, storage
+        self.spec = Model(model)
+        self.Q, self.nglobals = self.spec.Q, self.spec.nglobals
+        self.global_names = self.spec.global_names
+        self.shapes = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().tclbx_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def case_bytes(self, nx, ny, nz=1):
+        v = ctypes.c_longlong()
+        _check(lib().tclbx_batch_case_bytes(self._h, nx, ny, nz, ctypes.byref(v)))
+        return v.value
+
+    def add_case(self, nx, ny, nz=1):
+        """New case id, or None when the arena has no room for it."""
+        i = ctypes.c_int()
+        _check(lib().tclbx_batch_add_case(self._h, nx, ny, nz, ctypes.byref(i)))
+        if i.value < 0:
+            return None
+        self.shapes[i.value] = (nz, ny, nx)
+        return i.value
+
+    def remove_case(self, cid):
+        _check(lib().tclbx_batch_remove_case(self._h, cid))
+        self.shapes.pop(cid, None)
+
+    def set_setting(self, cid, name, value):
+        i = self.spec.setting_id(name)
+        if i < 0:
+            raise TclbxError(f"unknown setting {name}")
+        _check(lib().tclbx_batch_set_setting(self._h, cid, i, float(value)))
+
+    def set_zone_setting(self, cid, name, value, zone=-1):
+        i = self.spec.zone_setting_id(name)
+        if i < 0:
+            raise TclbxError(f"unknown zonal setting {name}")
+        _check(lib().tclbx_batch_set_zone_setting(self._h, cid, i, int(zone), float(value)))
+
+    def set_flags(self, cid, flags):
+        f = np.ascontiguousarray(flags, dtype=np.uint16)
+        assert f.size == int(np.prod(self.shapes[cid]))
+        _check(lib().tclbx_batch_flag_overwrite(self._h, cid, f.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16))))
+
+    def init_case(self, cid):
+        _check(lib().tclbx_batch_init_case(self._h, cid))
+
+    def iterate(self, n, iter_type=ITER_NORM | ITER_LASTGLOB):
+        _check(lib().tclbx_batch_iterate(self._h, int(n), int(iter_type)))
+
+    def globals(self, cid):
+        out = np.zeros(self.nglobals)
+        _check(lib().tclbx_batch_get_globals(self._h, cid, _dp(out), self.nglobals))
+        return out
+
+    def iteration(self, cid):
+        v = ctypes.c_longlong()
+        _check(lib().tclbx_batch_get_iteration(self._h, cid, ctypes.byref(v)))
+        return v.value
+
+    def quantity(self, cid, name, scale=1.0):
+        q, isv = self.spec.quantity_id(name)
+        if q < 0:
+            raise TclbxError(f"unknown quantity {name}")
+        out = np.empty(self.shapes[cid] + ((3,) if isv else ()), dtype=np.float64)
+        _check(lib().tclbx_batch_get_quantity(self._h, cid, q, _dp(out), float(scale)))
+        return out
+
+    def get_f(self, cid):
+        out = np.empty((self.Q,) + self.shapes[cid], dtype=np.float64)
+        for q in range(self.Q):
+            _check(lib().tclbx_batch_get_density(self._h, cid, q, _dp(out[q])))
+        return out
+
+    def set_f(self, cid, f):
+        a = np.ascontiguousarray(f, dtype=np.float64).reshape((self.Q,) + self.shapes[cid])
+        for q in range(self.Q):
+            plane = np.ascontiguousarray(a[q])
+            _check(lib().tclbx_batch_set_density(self._h, cid, q, _dp(plane)))
+
+    def add(self, case):
+        """Admit a tclb_amd.cases.Case: flags, settings, zonal settings,
+        Init, optional initial state.  Returns the id or None (no room)."""
+        if case.model != self.model:
+            raise TclbxError(f"case model {case.model} != batch model {self.model}")
+        cid = self.add_case(case.nx, case.ny, case.nz)
+        if cid is None:
+            return None
+        self.set_flags(cid, case.flags)
+        for n, v in case.settings:
+            self.set_setting(cid, n, v)
+        for n, z, v in case.zonal:
+            self.set_zone_setting(cid, n, v, z)
+        self.init_case(cid)
+        if case.init_f is not None:
+            self.set_f(cid, case.init_f)
+        return cid
+
+    def autotune(self, steps=10):
+        b, r, g, us = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _check(lib().tclbx_batch_autotune(self._h, int(steps), ctypes.byref(b), ctypes.byref(r), ctypes.byref(g),
+                                          ctypes.byref(us)))
+        return {"block": b.value, "rows_per_tile": r.value, "grid": g.value, "us_per_step": us.value}
+
+    def set_tuning(self, block=256, rows_per_tile=1, grid=0):
+        _check(lib().tclbx_batch_set_tuning(self._h, int(block), int(rows_per_tile), int(grid)))
+
+    def timing(self):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        _check(lib().tclbx_batch_get_timing(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def info(self):
+        n, nodes, used, cap = ctypes.c_int(), ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
+        _check(lib().tclbx_batch_info(self._h, ctypes.byref(n), ctypes.byref(nodes), ctypes.byref(used),
+                                      ctypes.byref(cap)))
+        return {"cases": n.value, "nodes": nodes.value, "arena_used": used.value, "arena_capacity": cap.value}
+
+    def set_graph_steps(self, steps):
+        _check(lib().tclbx_batch_set_graph_steps(self._h, int(steps)))

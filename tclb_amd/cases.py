@@ -195,3 +195,24 @@ def d3q19_porous(n=256, fraction=0.10, rmin=4, rmax=12, force=1e-6, tau=0.6, se=
     flags[:, n - 1, :] = WALL
     settings = [("omega", 1.0 / tau), ("ForceX", force), ("MRT_Se", se), ("MRT_Seps", seps), ("MRT_Sq", sq)]
     return Case("d3q19_porous", "d3q19", n, n, n, flags, settings=settings, steps=steps)
+
+
+def d2q9_cylinder_sweep(ncases=256, seed=4, steps=8000, umax=0.02):
+    """Independent cylinder cases of uneven size: nx uniform in
+    {128, 144, ..., 1024}, ny uniform in {64, 80, ..., 256}, a Solid disc of
+    diameter ny/4 at (nx/4, ny/2), Reynolds number uniform in [20, 200] with
+    nu = umax * D / Re (umax the peak of the parabolic inlet); case k draws
+    from splitmix64 stream seed*1000 + k."""
+    out = []
+    for k in range(ncases):
+        st = Stream(seed * 1000 + k)
+        nx = 128 + 16 * int(st.uniform(None, 0, 57))
+        ny = 64 + 16 * int(st.uniform(None, 0, 13))
+        re = st.uniform(None, 20.0, 200.0)
+        d = ny / 4.0
+        nu = umax * d / re
+        c = d2q9_cylinder(nx=nx, ny=ny, diameter=d, cx=nx / 4.0, cy=ny / 2.0, umax=umax, nu=nu, steps=steps,
+                          seed=seed * 1000 + k)
+        c.name = f"cylinder_{k}_{nx}x{ny}_Re{re:.1f}"
+        out.append(c)
+    return out

@@ -4,6 +4,7 @@
 #include <exception>
 #include <string>
 
+#include "batch.h"
 #include "lattice.h"
 #include "model.h"
 
@@ -11,6 +12,10 @@ using namespace tclbx;
 
 struct tclbx_lattice {
     Lattice* lat;
+};
+
+struct tclbx_batch {
+    Batch* b;
 };
 
 namespace {
@@ -35,6 +40,8 @@ int guard(F&& f) {
 
 #define CHECK_H(h) \
     if (!(h) || !(h)->lat) return fail("null lattice handle")
+#define CHECK_B(h) \
+    if (!(h) || !(h)->b) return fail("null batch handle")
 }  // namespace
 
 extern "C" {
@@ -250,6 +257,133 @@ int tclbx_set_graph_steps(tclbx_lattice* h, int steps) {
     CHECK_H(h);
     if (steps < 0 || (steps % 2) != 0) return fail("graph steps must be a non-negative even number");
     h->lat->set_graph_steps(steps);
+    return TCLBX_OK;
+}
+
+int tclbx_batch_create(const char* model, const char* storage, long long capacity_bytes, tclbx_batch** out) {
+    return guard([&]() {
+        if (!out) return fail("null output pointer");
+        *out = nullptr;
+        const ModelDef* m = model ? model_by_name(model) : nullptr;
+        if (!m) return fail(std::string("unknown model: ") + (model ? model : "(null)"));
+        int st = storage_by_name(storage ? storage : "f64");
+        if (st < 0) return fail(std::string("unknown storage: ") + storage);
+        if (capacity_bytes <= 0) return fail("batch capacity must be positive");
+        Batch* b = new Batch(m, st, (size_t)capacity_bytes);
+        *out = new tclbx_batch{b};
+        return TCLBX_OK;
+    });
+}
+int tclbx_batch_destroy(tclbx_batch* b) {
+    if (!b) return TCLBX_OK;
+    delete b->b;
+    delete b;
+    return TCLBX_OK;
+}
+int tclbx_batch_case_bytes(const tclbx_batch* b, int nx, int ny, int nz, long long* bytes) {
+    CHECK_B(b);
+    if (bytes) *bytes = (long long)b->b->CaseBytes(nx, ny, nz);
+    return TCLBX_OK;
+}
+int tclbx_batch_add_case(tclbx_batch* b, int nx, int ny, int nz, int* id) {
+    CHECK_B(b);
+    if (!id) return fail("null output pointer");
+    return guard([&]() { *id = b->b->AddCase(nx, ny, nz); return TCLBX_OK; });
+}
+int tclbx_batch_remove_case(tclbx_batch* b, int id) {
+    CHECK_B(b);
+    return guard([&]() { b->b->RemoveCase(id); return TCLBX_OK; });
+}
+int tclbx_batch_set_setting(tclbx_batch* b, int id, int setting, double value) {
+    CHECK_B(b);
+    return guard([&]() { b->b->SetSetting(id, setting, value); return TCLBX_OK; });
+}
+int tclbx_batch_set_zone_setting(tclbx_batch* b, int id, int zonal, int zone, double value) {
+    CHECK_B(b);
+    return guard([&]() { b->b->SetZoneSetting(id, zonal, zone, value); return TCLBX_OK; });
+}
+int tclbx_batch_flag_overwrite(tclbx_batch* b, int id, const uint16_t* flags) {
+    CHECK_B(b);
+    if (!flags) return fail("null flags");
+    return guard([&]() { b->b->FlagOverwrite(id, flags); return TCLBX_OK; });
+}
+int tclbx_batch_init_case(tclbx_batch* b, int id) {
+    CHECK_B(b);
+    return guard([&]() { b->b->InitCase(id); return TCLBX_OK; });
+}
+int tclbx_batch_iterate(tclbx_batch* b, int niter, int iter_type) {
+    CHECK_B(b);
+    return guard([&]() { b->b->Iterate(niter, iter_type); return TCLBX_OK; });
+}
+int tclbx_batch_get_globals(const tclbx_batch* b, int id, double* out, int n) {
+    CHECK_B(b);
+    return guard([&]() {
+        const auto& g = b->b->Globals(id);
+        if (!out || n < (int)g.size()) return fail("globals buffer too small");
+        for (size_t i = 0; i < g.size(); i++) out[i] = g[i];
+        return TCLBX_OK;
+    });
+}
+int tclbx_batch_get_iteration(const tclbx_batch* b, int id, long long* iter) {
+    CHECK_B(b);
+    return guard([&]() { if (iter) *iter = b->b->CaseIteration(id); return TCLBX_OK; });
+}
+int tclbx_batch_get_quantity(tclbx_batch* b, int id, int q, double* out, double scale) {
+    CHECK_B(b);
+    if (!out) return fail("null output");
+    return guard([&]() { b->b->GetQuantity(id, q, out, scale); return TCLBX_OK; });
+}
+int tclbx_batch_get_density(tclbx_batch* b, int id, int q, double* out) {
+    CHECK_B(b);
+    if (!out) return fail("null output");
+    return guard([&]() { b->b->GetDensity(id, q, out); return TCLBX_OK; });
+}
+int tclbx_batch_set_density(tclbx_batch* b, int id, int q, const double* in) {
+    CHECK_B(b);
+    if (!in) return fail("null input");
+    return guard([&]() { b->b->SetDensity(id, q, in); return TCLBX_OK; });
+}
+int tclbx_batch_autotune(tclbx_batch* b, int steps, int* block, int* rows_per_tile, int* grid, double* us_per_step) {
+    CHECK_B(b);
+    return guard([&]() {
+        Tuning t = b->b->Autotune(steps);
+        if (block) *block = t.block;
+        if (rows_per_tile) *rows_per_tile = t.rpt;
+        if (grid) *grid = t.grid;
+        if (us_per_step) *us_per_step = t.us_per_step;
+        return TCLBX_OK;
+    });
+}
+int tclbx_batch_set_tuning(tclbx_batch* b, int block, int rows_per_tile, int grid) {
+    CHECK_B(b);
+    return guard([&]() {
+        Tuning t;
+        t.block = block;
+        t.rpt = rows_per_tile;
+        t.grid = grid;
+        b->b->SetTuning(t);
+        return TCLBX_OK;
+    });
+}
+int tclbx_batch_get_timing(const tclbx_batch* b, double* last_ms, double* total_ms) {
+    CHECK_B(b);
+    if (last_ms) *last_ms = b->b->last_iterate_ms();
+    if (total_ms) *total_ms = b->b->total_ms();
+    return TCLBX_OK;
+}
+int tclbx_batch_info(const tclbx_batch* b, int* ncases, long long* nodes, long long* arena_used,
+                     long long* arena_capacity) {
+    CHECK_B(b);
+    if (ncases) *ncases = b->b->ncases();
+    if (nodes) *nodes = b->b->nodes();
+    if (arena_used) *arena_used = (long long)b->b->arena_used();
+    if (arena_capacity) *arena_capacity = (long long)b->b->arena_capacity();
+    return TCLBX_OK;
+}
+int tclbx_batch_set_graph_steps(tclbx_batch* b, int steps) {
+    CHECK_B(b);
+    if (steps < 0 || (steps % 2) != 0) return fail("graph steps must be a non-negative even number");
+    b->b->set_graph_steps(steps);
     return TCLBX_OK;
 }
 

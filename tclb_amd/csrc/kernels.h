@@ -23,8 +23,18 @@ struct StepArgs {
     int nx, ny, nz;
     int bpr;               // blocks per (y,z) row
     int rpb;               // rows walked by one block of a Globals step
+    int rpt;               // rows walked by one block of a plain step
     int zcols, nzones;
     double par[MAX_PAR];
+};
+
+// One lattice of a batch: its step arguments (fin/fout = snapshots 0/1, rpb
+// of its Globals tiles) and which snapshot is current when the batch phase
+// is 0.
+struct BatchCase {
+    StepArgs args;
+    int parity;
+    int pad_[3];
 };
 
 // Blocks of one step launch (a Globals step walks a.rpb rows per block).
@@ -32,6 +42,14 @@ int step_blocks(const StepArgs& a, bool globals);
 // One time step: pull streaming + NodeType dispatch + collision (+ Globals).
 hipError_t launch_step(int model, int storage, bool globals, const StepArgs& a, int block,
                        hipStream_t s);
+// One step of a batch: a persistent grid of `grid` blocks walks `ntiles`
+// tiles {case, tile of the case}; Globals partials go to slot = tile index.
+hipError_t launch_batch_step(int model, int storage, bool globals, const BatchCase* cases, const int2* tiles,
+                             int ntiles, int phase, double* gpart, int grid, int block, hipStream_t s);
+// Per-case Globals: case c sums partial slots [first[c], first[c]+count[c])
+// into out[c*nglob ...].
+hipError_t launch_reduce_globals_cases(const double* gpart, int nslots, int nglob, int sum_globals, int ncases,
+                                       const int* first, const int* count, double* out, hipStream_t s);
 // Init(): equilibrium from the settings, written to a.fout.
 hipError_t launch_init(int model, int storage, const StepArgs& a, int block, hipStream_t s);
 // Deterministic reduction of the per-block partials into out[nglob]

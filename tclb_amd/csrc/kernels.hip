@@ -604,7 +604,8 @@ __device__ __forceinline__ double wave_sum(double v) {
 // wavefront is skipped (most slice Globals are, away from slice nodes); the
 // 64 lanes are summed in double, the wavefronts of the block with TwoSum.
 template <int G>
-__device__ __forceinline__ void block_globals(const double* g, int sum_globals, double* gpart) {
+__device__ __forceinline__ void block_globals(const double* g, int sum_globals, double* gpart, int slot,
+                                              int nslots) {
     __shared__ double lds[16][G];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nw = (blockDim.x + 63) >> 6;
@@ -622,41 +623,26 @@ __device__ __forceinline__ void block_globals(const double* g, int sum_globals, 
             if (i < sum_globals) dd_add(v, c, lds[w][i], 0.0);
             else v = fmax(v, lds[w][i]);
         }
-        const size_t k = ((size_t)i * gridDim.x + blockIdx.x) * 2;
+        const size_t k = ((size_t)i * nslots + slot) * 2;
         gpart[k] = v;
         gpart[k + 1] = c;
     }
+    __syncthreads();  // the LDS is reused by the next tile of a persistent block
 }
 
 // ---------------------------------------------------------- step kernel ---
+// One node of row `row` (y + ny*z) at x = xc: pull with periodic wrap, Run(),
+// store; `active` is false for the lanes past the end of the row (they load
+// a valid node but store nothing).
 template <int MODEL, int ST, bool GLOB>
-__global__ void __launch_bounds__(256) k_step(StepArgs a) {
+__device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, int xm, int xp, bool active,
+                                            double* g) {
     using L = Lat<MODEL>;
     using S = Store<ST>;
     using T = typename S::T;
     using R = typename S::R;
     constexpr int Q = L::Q;
-
-    // block -> (x segment, first row); uniform scalar arithmetic.  A plain
-    // step block covers one x segment of one (y,z) row; a Globals block walks
-    // `rpb` consecutive rows so that one block reduction covers them all.
-    const int rpb = GLOB ? a.rpb : 1;
-    const int rowblk = blockIdx.x / a.bpr;
-    const int xb = blockIdx.x - rowblk * a.bpr;
-    const int x = xb * blockDim.x + threadIdx.x;
-    const bool xin = x < a.nx;
-    const int xc = xin ? x : a.nx - 1;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const int xm = xc == 0 ? nx - 1 : xc - 1, xp = xc == nx - 1 ? 0 : xc + 1;
-
-    double g[L::G > 0 ? L::G : 1];
-    if (GLOB) {
-#pragma unroll
-        for (int i = 0; i < L::G; i++) g[i] = 0.0;
-    }
-    const int nrows = ny * nz;
-    for (int row = rowblk * rpb; row < (rowblk + 1) * rpb && row < nrows; row++) {
-    const bool active = xin;
     const int y = row % ny;
     const int z = row / ny;
     const int ym = y == 0 ? ny - 1 : y - 1, yp = y == ny - 1 ? 0 : y + 1;
@@ -697,10 +683,61 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a) {
             __builtin_nontemporal_store(S::put(f[i], (R)L::w(i)),
                                         (T*)(fout + (size_t)i * pbytes + n * (unsigned)sizeof(T)));
     }
-    }  // rows
+}
+
+// The rows of one tile: x segment `xb` of rows [rowblk*rpb, +rpb) of the
+// lattice `a`; with GLOB the tile's Globals go to partial slot `slot` of
+// `nslots`.
+template <int MODEL, int ST, bool GLOB>
+__device__ __forceinline__ void step_tile(const StepArgs& a, int tile, int rpb, int slot, int nslots) {
+    using L = Lat<MODEL>;
+    const int rowblk = tile / a.bpr;
+    const int xb = tile - rowblk * a.bpr;
+    const int x = xb * blockDim.x + threadIdx.x;
+    const bool xin = x < a.nx;
+    const int xc = xin ? x : a.nx - 1;
+    const int xm = xc == 0 ? a.nx - 1 : xc - 1, xp = xc == a.nx - 1 ? 0 : xc + 1;
+    double g[L::G > 0 ? L::G : 1];
+    if (GLOB) {
+#pragma unroll
+        for (int i = 0; i < L::G; i++) g[i] = 0.0;
+    }
+    const int nrows = a.ny * a.nz;
+    for (int row = rowblk * rpb; row < (rowblk + 1) * rpb && row < nrows; row++)
+        node_update<MODEL, ST, GLOB>(a, row, xc, xm, xp, xin, g);
     if (GLOB) {
         constexpr int SUMG = (MODEL == MODEL_D3Q19) ? 25 : L::G;
-        block_globals<L::G>(g, SUMG, a.gpart);
+        block_globals<L::G>(g, SUMG, a.gpart, slot, nslots);
+    }
+}
+
+// One step of one lattice: block = one tile.  A plain step tile is one x
+// segment of one (y,z) row; a Globals step tile walks `rpb` consecutive rows
+// so that one block reduction covers them all.
+template <int MODEL, int ST, bool GLOB>
+__global__ void __launch_bounds__(256) k_step(StepArgs a) {
+    step_tile<MODEL, ST, GLOB>(a, blockIdx.x, GLOB ? a.rpb : a.rpt, blockIdx.x, gridDim.x);
+}
+
+// One step of a batch of lattices (a ragged arena): a persistent grid walks
+// the tile list; tile t = {case, tile within the case}.  Each case has its
+// own descriptor (sizes, pitch, snapshots, flags, zone table, parameters);
+// `phase` selects which snapshot of every case is read this step.  Globals
+// partials of case c occupy the slots of its tiles, [tile0, tile0 + ntiles).
+template <int MODEL, int ST, bool GLOB>
+__global__ void __launch_bounds__(256) k_batch_step(const BatchCase* __restrict__ cases,
+                                                    const int2* __restrict__ tiles, int ntiles, int phase,
+                                                    double* gpart) {
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int2 tl = tiles[t];
+        StepArgs a = cases[tl.x].args;
+        if ((phase ^ cases[tl.x].parity) & 1) {
+            const void* tmp = a.fin;
+            a.fin = a.fout;
+            a.fout = (void*)tmp;
+        }
+        a.gpart = gpart;
+        step_tile<MODEL, ST, GLOB>(a, tl.y, GLOB ? a.rpb : a.rpt, t, ntiles);
     }
 }
 
@@ -810,25 +847,31 @@ __global__ void k_set_density(void* buf, long long pitch, long long n, int q, do
     ((typename S::T*)buf)[q * pitch + i] = S::put((R)in[i], (R)w);
 }
 
-// deterministic second pass: one block per Global, reading that Global's
-// per-block partials contiguously, TwoSum pairs throughout; the result is
-// added to the device Globals (cleared before a Globals step, accumulated
-// over steps with ITER_GLOBS)
-__global__ void __launch_bounds__(256) k_reduce_globals(const double* gpart, int nblocks, int nglob,
-                                                        int sum_globals, double* out) {
+// deterministic second pass: block (Global gi, lattice c) sums slots
+// [first_c, first_c + count_c) of that Global's partials contiguously, TwoSum
+// pairs throughout, and adds the result to out[c*nglob + gi] (cleared before
+// a Globals step, accumulated over steps with ITER_GLOBS).  The order depends
+// only on the lattice's own tiles, so a lattice gets the same Globals alone
+// and in a batch.
+__global__ void __launch_bounds__(256) k_reduce_globals(const double* gpart, int nslots, int nglob,
+                                                        int sum_globals, const int* first, const int* count,
+                                                        double* out) {
     const int gi = blockIdx.x;
+    const int c = blockIdx.y;
     const bool is_sum = gi < sum_globals;
-    const double* p = gpart + (size_t)gi * nblocks * 2;
-    double v = 0.0, c = 0.0;
-    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
-        if (is_sum) dd_add(v, c, p[2 * b], p[2 * b + 1]);
+    const int n0 = first ? first[c] : 0;
+    const int nb = count ? count[c] : nslots;
+    const double* p = gpart + ((size_t)gi * nslots + n0) * 2;
+    double v = 0.0, cc = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        if (is_sum) dd_add(v, cc, p[2 * b], p[2 * b + 1]);
         else v = fmax(v, p[2 * b]);
     }
     __shared__ double lds[4][2];
-    if (is_sum) wave_sum_dd(v, c);
+    if (is_sum) wave_sum_dd(v, cc);
     else v = wave_max(v);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) { lds[wave][0] = v; lds[wave][1] = c; }
+    if (lane == 0) { lds[wave][0] = v; lds[wave][1] = cc; }
     __syncthreads();
     if (threadIdx.x == 0) {
         double r = lds[0][0], rc = lds[0][1];
@@ -836,7 +879,8 @@ __global__ void __launch_bounds__(256) k_reduce_globals(const double* gpart, int
             if (is_sum) dd_add(r, rc, lds[w][0], lds[w][1]);
             else r = fmax(r, lds[w][0]);
         }
-        out[gi] = is_sum ? out[gi] + (r + rc) : fmax(out[gi], r);
+        double* o = out + (size_t)c * nglob + gi;
+        *o = is_sum ? *o + (r + rc) : fmax(*o, r);
     }
 }
 
@@ -863,7 +907,7 @@ int nblocks_of(const StepArgs& a, int block) { return a.bpr * a.ny * a.nz; }
 
 template <int MODEL, int ST>
 hipError_t step_ms(bool glob, const StepArgs& a, int block, hipStream_t s) {
-    dim3 grid(glob ? step_blocks(a, true) : nblocks_of(a, block)), blk(block);
+    dim3 grid(step_blocks(a, glob)), blk(block);
     if (glob) hipLaunchKernelGGL((k_step<MODEL, ST, true>), grid, blk, 0, s, a);
     else hipLaunchKernelGGL((k_step<MODEL, ST, false>), grid, blk, 0, s, a);
     return hipGetLastError();
@@ -929,7 +973,7 @@ double weight(int model, int q) {
 
 int step_blocks(const StepArgs& a, bool globals) {
     const int rows = a.ny * a.nz;
-    const int rpb = globals ? a.rpb : 1;
+    const int rpb = globals ? a.rpb : a.rpt;
     return a.bpr * ((rows + rpb - 1) / rpb);
 }
 
@@ -953,8 +997,55 @@ hipError_t launch_init(int model, int storage, const StepArgs& a, int block, hip
 
 hipError_t launch_reduce_globals(const double* gpart, int nblocks, int nglob, int sum_globals, double* out,
                                  hipStream_t s) {
-    hipLaunchKernelGGL(k_reduce_globals, dim3(nglob), dim3(256), 0, s, gpart, nblocks, nglob, sum_globals, out);
+    hipLaunchKernelGGL(k_reduce_globals, dim3(nglob, 1), dim3(256), 0, s, gpart, nblocks, nglob, sum_globals,
+                       (const int*)nullptr, (const int*)nullptr, out);
     return hipGetLastError();
+}
+
+hipError_t launch_reduce_globals_cases(const double* gpart, int nslots, int nglob, int sum_globals, int ncases,
+                                       const int* first, const int* count, double* out, hipStream_t s) {
+    if (ncases <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reduce_globals, dim3(nglob, ncases), dim3(256), 0, s, gpart, nslots, nglob, sum_globals,
+                       first, count, out);
+    return hipGetLastError();
+}
+
+namespace {
+template <int MODEL, int ST>
+hipError_t batch_ms(bool glob, const BatchCase* cases, const int2* tiles, int ntiles, int phase, double* gpart,
+                    int grid, int block, hipStream_t s) {
+    if (ntiles <= 0) return hipSuccess;
+    if (glob) hipLaunchKernelGGL((k_batch_step<MODEL, ST, true>), dim3(grid), dim3(block), 0, s, cases, tiles,
+                                 ntiles, phase, gpart);
+    else hipLaunchKernelGGL((k_batch_step<MODEL, ST, false>), dim3(grid), dim3(block), 0, s, cases, tiles,
+                            ntiles, phase, gpart);
+    return hipGetLastError();
+}
+template <int MODEL>
+hipError_t batch_m(int st, bool glob, const BatchCase* cases, const int2* tiles, int ntiles, int phase,
+                   double* gpart, int grid, int block, hipStream_t s) {
+    switch (st) {
+    case ST_F64: return batch_ms<MODEL, ST_F64>(glob, cases, tiles, ntiles, phase, gpart, grid, block, s);
+    case ST_F32: return batch_ms<MODEL, ST_F32>(glob, cases, tiles, ntiles, phase, gpart, grid, block, s);
+    case ST_F32S: return batch_ms<MODEL, ST_F32S>(glob, cases, tiles, ntiles, phase, gpart, grid, block, s);
+    case ST_F16S: return batch_ms<MODEL, ST_F16S>(glob, cases, tiles, ntiles, phase, gpart, grid, block, s);
+    case ST_BF16S: return batch_ms<MODEL, ST_BF16S>(glob, cases, tiles, ntiles, phase, gpart, grid, block, s);
+    }
+    return hipErrorInvalidValue;
+}
+}  // namespace
+
+hipError_t launch_batch_step(int model, int storage, bool globals, const BatchCase* cases, const int2* tiles,
+                             int ntiles, int phase, double* gpart, int grid, int block, hipStream_t s) {
+    switch (model) {
+    case MODEL_D2Q9_SRT: return batch_m<MODEL_D2Q9_SRT>(storage, globals, cases, tiles, ntiles, phase, gpart, grid,
+                                                        block, s);
+    case MODEL_D2Q9_SRT_DRAG:
+        return batch_m<MODEL_D2Q9_SRT_DRAG>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
+    case MODEL_D3Q19: return batch_m<MODEL_D3Q19>(storage, globals, cases, tiles, ntiles, phase, gpart, grid,
+                                                  block, s);
+    }
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_quantity(int model, int storage, int q, const StepArgs& a, int x0, int y0, int z0, int rx,

@@ -94,10 +94,7 @@ void Lattice::set_block(int b) {
     block_ = b;
     bpr_ = (nx + b - 1) / b;
     nblocks_ = bpr_ * ny * nz;
-    // rows per block in a Globals step: enough blocks to fill the GPU
-    // several times over, each reducing several rows at once
-    const long long rows = (long long)ny * nz;
-    rpb_ = (int)std::max(1LL, std::min(16LL, rows * bpr_ / 8192));
+    rpb_ = globals_rows_per_block((long long)ny * nz, bpr_);
     if (d_gpart_) (void)hipFree(d_gpart_);
     d_gpart_ = nullptr;
     // compensated partial sums: two doubles per Global per block
@@ -191,31 +188,43 @@ StepArgs Lattice::args(int from, int to, bool glob) const {
     a.nx = nx; a.ny = ny; a.nz = nz;
     a.bpr = bpr_;
     a.rpb = rpb_;
+    a.rpt = 1;
     a.zcols = zcols_;
     a.nzones = nzones_;
-    auto S = [&](const char* n) { int i = model->setting_id(n); return i >= 0 ? settings_[i] : 0.0; };
+    model_params(model, settings_, a.par);
+    return a;
+}
+
+void model_params(const ModelDef* model, const std::vector<double>& settings, double* par) {
+    auto S = [&](const char* n) { int i = model->setting_id(n); return i >= 0 ? settings[i] : 0.0; };
+    for (int i = 0; i < MAX_PAR; i++) par[i] = 0.0;
     if (model->id == MODEL_D3Q19) {
         const char* names[] = {"omega", "InletVelocity", "InletDensity", "ForceX", "ForceY", "ForceZ"};
-        for (int i = 0; i < 6; i++) a.par[i] = S(names[i]);
+        for (int i = 0; i < 6; i++) par[i] = S(names[i]);
         // relaxation factors of CollisionMRT (models/flow/d3q19/Dynamics.c.Rt:
         // 246-252): gamma1 = 1 - omega on e, eps, nu, pi moments, gamma2 =
         // 1 - 8(2-omega)/(8-omega) on q and m moments; a positive MRT_S*
         // setting gives its group 1 - S instead
-        const double omega = a.par[0];
+        const double omega = par[0];
         const double gamma1 = 1 - omega;
         const double gamma2 = 1 - 8.0 * (2.0 - omega) / (8.0 - omega);
         auto rate = [&](const char* n, double dflt) { double v = S(n); return v > 0 ? 1 - v : dflt; };
-        a.par[6] = rate("MRT_Se", gamma1);
-        a.par[7] = rate("MRT_Seps", gamma1);
-        a.par[8] = rate("MRT_Sq", gamma2);
-        a.par[9] = rate("MRT_Spi", gamma1);
-        a.par[10] = rate("MRT_Sm", gamma2);
-        a.par[11] = gamma1;
+        par[6] = rate("MRT_Se", gamma1);
+        par[7] = rate("MRT_Seps", gamma1);
+        par[8] = rate("MRT_Sq", gamma2);
+        par[9] = rate("MRT_Spi", gamma1);
+        par[10] = rate("MRT_Sm", gamma2);
+        par[11] = gamma1;
     } else {
-        a.par[0] = S("omega");
-        a.par[1] = S("Density");
+        par[0] = S("omega");
+        par[1] = S("Density");
     }
-    return a;
+}
+
+int globals_rows_per_block(long long rows, int bpr) {
+    // enough blocks to fill the GPU several times over, each reducing
+    // several rows at once
+    return (int)std::max(1LL, std::min(16LL, rows * bpr / 8192));
 }
 
 void Lattice::launch(int from, int to, bool glob) {

@@ -96,5 +96,9 @@ private:
 
 // hipError_t -> exception with a message naming the call
 void hip_check(hipError_t e, const char* what);
+// kernel parameter vector (StepArgs::par) of a model from its settings
+void model_params(const ModelDef* model, const std::vector<double>& settings, double* par);
+// rows walked by one block of a Globals step for a lattice of `rows` rows
+int globals_rows_per_block(long long rows, int bpr);
 
 }  // namespace tclbx

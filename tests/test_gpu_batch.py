@@ -1,0 +1,155 @@
+"""The batch (ragged arena, persistent tile kernel) against single lattices.
+
+A case in a batch runs the same node arithmetic over its own tiles, and its
+Globals are reduced over its own tiles only, so the populations and the
+Globals of a case are bit-identical whether it runs as a Lattice (with the
+same tile width), alone in a batch, or among other cases; the oracle checks
+the mixed batch in FP64.
+"""
+import numpy as np
+import pytest
+
+from conftest import oracle_for_case
+
+import tclb_amd
+from tclb_amd import cases
+from test_gpu_parity import glob_close, random_case
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def mixed_cases(model, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(5):
+        if model.startswith("d2q9"):
+            nx, ny, nz = int(rng.integers(9, 90)), int(rng.integers(5, 40)), 1
+        else:
+            nx, ny, nz = int(rng.integers(5, 30)), int(rng.integers(4, 12)), int(rng.integers(3, 9))
+        out.append(random_case(model, nx, ny, nz, seed=seed * 10 + k))
+    return out
+
+
+@pytest.mark.parametrize("model,storage", [("d2q9_SRT_drag", "f64"), ("d3q19", "f64"), ("d3q19", "f16s"),
+                                           ("d2q9_SRT", "bf16s")])
+def test_batch_matches_single_lattices(model, storage):
+    say(f"batch vs lattices {model} {storage}")
+    cs = mixed_cases(model, 3)
+    b = tclb_amd.Batch(model, storage, capacity_bytes=64 << 20)
+    b.set_tuning(block=64, rows_per_tile=2, grid=17)
+    ids = [b.add(c) for c in cs]
+    solo = []
+    for c in cs:
+        lat = c.build(storage)
+        lat.set_block(64)
+        solo.append(lat)
+    for n in (7, 1, 12):
+        b.iterate(n)
+        for cid, lat in zip(ids, solo):
+            lat.iterate(n)
+            assert np.array_equal(b.get_f(cid), lat.get_f(), equal_nan=True)
+            assert np.array_equal(b.globals(cid), lat.globals, equal_nan=True)
+            assert b.iteration(cid) == lat.iteration
+
+
+def test_batch_case_alone_and_in_mix_and_oracle(oracle_mod):
+    say("case alone vs in a mix vs oracle")
+    cs = mixed_cases("d3q19", 5)
+    mix = tclb_amd.Batch("d3q19", "f64", capacity_bytes=64 << 20)
+    ids = [mix.add(c) for c in cs]
+    alone = tclb_amd.Batch("d3q19", "f64", capacity_bytes=16 << 20)
+    a_id = alone.add(cs[2])
+    mix.iterate(15)
+    alone.iterate(15)
+    assert np.array_equal(mix.get_f(ids[2]), alone.get_f(a_id))
+    assert np.array_equal(mix.globals(ids[2]), alone.globals(a_id))
+    for c, cid in zip(cs, ids):
+        orc = oracle_for_case(oracle_mod, c, "f64")
+        g = orc.iterate(15)
+        assert np.array_equal(mix.get_f(cid), orc.get_f())
+        glob_close(mix.globals(cid), g[:mix.nglobals], 1e-10)
+
+
+def test_autotune_keeps_state():
+    say("autotune keeps the state")
+    cs = mixed_cases("d2q9_SRT", 7)
+    a = tclb_amd.Batch("d2q9_SRT", "f32s", capacity_bytes=32 << 20)
+    b = tclb_amd.Batch("d2q9_SRT", "f32s", capacity_bytes=32 << 20)
+    ia = [a.add(c) for c in cs]
+    ib = [b.add(c) for c in cs]
+    a.iterate(5)
+    b.iterate(5)
+    t = a.autotune(steps=2)
+    assert t["block"] in (64, 128, 256) and t["rows_per_tile"] in (1, 2, 4) and t["us_per_step"] > 0
+    a.iterate(8)
+    b.iterate(8)
+    for x, y in zip(ia, ib):
+        assert np.array_equal(a.get_f(x), b.get_f(y), equal_nan=True)
+
+
+def test_admission_by_capacity_and_reuse():
+    say("arena admission")
+    need = tclb_amd.Batch("d2q9_SRT", "f16s", capacity_bytes=1 << 20).case_bytes(64, 32)
+    b = tclb_amd.Batch("d2q9_SRT", "f16s", capacity_bytes=3 * need)
+    ids = [b.add_case(64, 32) for _ in range(3)]
+    assert None not in ids
+    assert b.add_case(64, 32) is None          # full
+    b.remove_case(ids[1])
+    mid = b.add_case(64, 32)                     # the freed slot is reused
+    assert mid is not None
+    assert b.info()["cases"] == 3
+    assert b.add_case(64, 64) is None
+    b.remove_case(ids[0])
+    b.remove_case(mid)                           # two adjacent free blocks coalesce
+    assert b.add_case(64, 64) is not None        # 64x64 needs exactly two 64x32 blocks
+
+
+def test_graph_replay_batch():
+    say("batch graph replay")
+    cs = mixed_cases("d3q19", 9)
+    a = tclb_amd.Batch("d3q19", "f64", capacity_bytes=64 << 20)
+    b = tclb_amd.Batch("d3q19", "f64", capacity_bytes=64 << 20)
+    ia = [a.add(c) for c in cs]
+    ib = [b.add(c) for c in cs]
+    a.set_graph_steps(4)
+    b.set_graph_steps(0)
+    for n in (3, 17, 1):
+        a.iterate(n)
+        b.iterate(n)
+    for x, y in zip(ia, ib):
+        assert np.array_equal(a.get_f(x), b.get_f(y), equal_nan=True)
+        assert np.array_equal(a.globals(x), b.globals(y), equal_nan=True)
+
+
+def test_late_admission_joins_mid_run():
+    """A case admitted while others are half-way through gives the same
+    result as when it starts alone."""
+    say("late admission")
+    cs = mixed_cases("d2q9_SRT_drag", 11)
+    b = tclb_amd.Batch("d2q9_SRT_drag", "f64", capacity_bytes=64 << 20)
+    first = [b.add(c) for c in cs[:3]]
+    b.iterate(5)
+    late = b.add(cs[3])
+    b.iterate(12)
+    solo = tclb_amd.Batch("d2q9_SRT_drag", "f64", capacity_bytes=16 << 20)
+    s = solo.add(cs[3])
+    solo.iterate(12)
+    assert b.iteration(late) == 12 and b.iteration(first[0]) == 17
+    assert np.array_equal(b.get_f(late), solo.get_f(s), equal_nan=True)
+    assert np.array_equal(b.globals(late), solo.globals(s), equal_nan=True)
+
+
+def test_sweep_generator_cases_run():
+    """A few cases of the 256-case cylinder sweep generator in one batch."""
+    say("cylinder sweep cases")
+    sweep = cases.d2q9_cylinder_sweep(ncases=6, steps=50)
+    b = tclb_amd.Batch("d2q9_SRT_drag", "f16s", capacity_bytes=256 << 20)
+    ids = [b.add(c) for c in sweep]
+    b.iterate(50)
+    for cid in ids:
+        assert np.all(np.isfinite(b.globals(cid)))
+        assert np.all(np.isfinite(b.quantity(cid, "U")))
