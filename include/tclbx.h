@@ -163,6 +163,45 @@ int tclbx_batch_info(const tclbx_batch* b, int* ncases, long long* nodes, long l
                      long long* arena_capacity);
 int tclbx_batch_set_graph_steps(tclbx_batch* b, int steps);
 
+/* --------------------------------------------------------------- server --
+ * Continuous-batching solve server: requests arrive from any thread; a
+ * worker thread admits them into one batch while the HBM budget has room,
+ * advances all resident requests together, and retires each on its step
+ * budget or its stop rule.  The stop rule is the reference's <Stop> handler
+ * (src/Handlers/cbStop.cpp:12-104): criteria on Globals (type 0
+ * <G>Change, 1 <G>PercentChange, 2 <G>Above, 3 <G>Below) checked every
+ * `stop_interval` steps, stopping after `stop_times` consecutive passing
+ * checks.  No counterpart in the reference. */
+typedef struct tclbx_server tclbx_server;
+
+#define TCLBX_RQ_REJECTED (-1)
+#define TCLBX_RQ_QUEUED 0
+#define TCLBX_RQ_RUNNING 1
+#define TCLBX_RQ_DONE_BUDGET 2
+#define TCLBX_RQ_DONE_STOP 3
+
+/* hbm_budget: bytes of the arena; max_chunk: most steps between admissions */
+int tclbx_server_create(const char* model, const char* storage, long long hbm_budget, int max_chunk,
+                        tclbx_server** out);
+int tclbx_server_destroy(tclbx_server* s);
+/* a request: lattice size, NodeTypes (x fastest), settings (ids and
+ * values, applied in order), zonal settings (ids, zones, values), step
+ * budget and stop rule (nstop = 0: budget only) */
+int tclbx_server_submit(tclbx_server* s, int nx, int ny, int nz, const uint16_t* flags, int nset,
+                        const int* set_ids, const double* set_values, int nzonal, const int* zonal_ids,
+                        const int* zonal_zones, const double* zonal_values, long long max_steps, int nstop,
+                        const int* stop_global, const int* stop_type, const double* stop_limit,
+                        int stop_interval, int stop_times, int* id);
+/* state (TCLBX_RQ_*), steps run, Globals at the end (n >= number of
+ * Globals) and the submit/admit/finish times in seconds since creation */
+int tclbx_server_poll(tclbx_server* s, int id, int* state, long long* steps, double* globals, int n,
+                      double* t_submit, double* t_admit, double* t_finish);
+/* 1 when nothing is queued or running, 0 on timeout */
+int tclbx_server_wait_all(tclbx_server* s, double timeout_s, int* idle);
+int tclbx_server_stats(tclbx_server* s, long long* completed, long long* rejected, long long* node_steps,
+                       double* busy_s, int* active, int* queued, double* now_s);
+int tclbx_server_set_tuning(tclbx_server* s, int block, int rows_per_tile, int grid);
+
 /* Measured device copy bandwidth: `reps` copies of `bytes` (rounded down
  * to 16) between two device buffers by a plain vector-copy kernel;
  * *gbps = 2*bytes*reps / time (read + write), *ms = time of one copy. */

@@ -39,6 +39,8 @@ SYMBOLS = [
     "tclbx_batch_get_iteration", "tclbx_batch_get_quantity", "tclbx_batch_get_density",
     "tclbx_batch_set_density", "tclbx_batch_autotune", "tclbx_batch_set_tuning", "tclbx_batch_get_timing",
     "tclbx_batch_info", "tclbx_batch_set_graph_steps",
+    "tclbx_server_create", "tclbx_server_destroy", "tclbx_server_submit", "tclbx_server_poll",
+    "tclbx_server_wait_all", "tclbx_server_stats", "tclbx_server_set_tuning",
 ]
 
 
@@ -116,6 +118,14 @@ def lib():
     L.tclbx_batch_get_timing.argtypes = [vp, dp, dp]
     L.tclbx_batch_info.argtypes = [vp, ip, llp, llp, llp]
     L.tclbx_batch_set_graph_steps.argtypes = [vp, c_int]
+    L.tclbx_server_create.argtypes = [c_char_p, c_char_p, ll, c_int, ctypes.POINTER(vp)]
+    L.tclbx_server_destroy.argtypes = [vp]
+    L.tclbx_server_submit.argtypes = [vp, c_int, c_int, c_int, u16p, c_int, ip, dp, c_int, ip, ip, dp, ll,
+                                      c_int, ip, ip, dp, c_int, c_int, ip]
+    L.tclbx_server_poll.argtypes = [vp, c_int, ip, llp, dp, c_int, dp, dp, dp]
+    L.tclbx_server_wait_all.argtypes = [vp, c_double, ip]
+    L.tclbx_server_stats.argtypes = [vp, llp, llp, llp, dp, ip, ip, dp]
+    L.tclbx_server_set_tuning.argtypes = [vp, c_int, c_int, c_int]
     _lib = L
     return L
 
@@ -484,3 +494,84 @@ class Batch:
 
     def set_graph_steps(self, steps):
         _check(lib().tclbx_batch_set_graph_steps(self._h, int(steps)))
+
+
+STOP_TYPES = {"Change": 0, "PercentChange": 1, "Above": 2, "Below": 3}
+REQUEST_STATES = {-1: "rejected", 0: "queued", 1: "running", 2: "done_budget", 3: "done_stop"}
+
+
+def _iarr(v):
+    return (ctypes.c_int * max(1, len(v)))(*[int(x) for x in v])
+
+
+def _darr(v):
+    return (ctypes.c_double * max(1, len(v)))(*[float(x) for x in v])
+
+
+class Server:
+    """Continuous-batching solve server (native worker thread over one
+    Batch): submit() from any thread; requests are admitted while the HBM
+    budget has room, run together, and retire on their step budget or stop
+    rule (the reference's <Stop>: [(Global, "Change"|"PercentChange"|
+    "Above"|"Below", limit)] checked every `interval` steps, `times`
+    consecutive passes)."""
+
+    def __init__(self, model, storage="f16s", hbm_budget=1 << 30, max_chunk=500):
+        h = ctypes.c_void_p()
+        _check(lib().tclbx_server_create(model.encode(), storage.encode(), int(hbm_budget), int(max_chunk),
+                                         ctypes.byref(h)))
+        self._h = h
+        self.model, self.storage = model, storage
+        self.spec = Model(model)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().tclbx_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, case, max_steps, stop=(), interval=500, times=1):
+        f = np.ascontiguousarray(case.flags, dtype=np.uint16).reshape(-1)
+        sids = [self.spec.setting_id(n) for n, _ in case.settings]
+        zids = [self.spec.zone_setting_id(n) for n, _, _ in case.zonal]
+        if min(sids + zids + [0]) < 0:
+            raise TclbxError("unknown setting in the case")
+        gids = [self.spec.global_id(g) for g, _, _ in stop]
+        if min(gids + [0]) < 0:
+            raise TclbxError("unknown Global in the stop rule")
+        rid = ctypes.c_int()
+        _check(lib().tclbx_server_submit(
+            self._h, case.nx, case.ny, case.nz, f.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)),
+            len(sids), _iarr(sids), _darr([v for _, v in case.settings]),
+            len(zids), _iarr(zids), _iarr([z for _, z, _ in case.zonal]), _darr([v for _, _, v in case.zonal]),
+            int(max_steps), len(gids), _iarr(gids), _iarr([STOP_TYPES[t] for _, t, _ in stop]),
+            _darr([lim for _, _, lim in stop]), int(interval), int(times), ctypes.byref(rid)))
+        return rid.value
+
+    def poll(self, rid):
+        st, steps = ctypes.c_int(), ctypes.c_longlong()
+        g = np.zeros(self.spec.nglobals)
+        ts, ta, tf = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _check(lib().tclbx_server_poll(self._h, rid, ctypes.byref(st), ctypes.byref(steps), _dp(g), len(g),
+                                       ctypes.byref(ts), ctypes.byref(ta), ctypes.byref(tf)))
+        return {"state": REQUEST_STATES[st.value], "steps": steps.value, "globals": g,
+                "t_submit": ts.value, "t_admit": ta.value, "t_finish": tf.value}
+
+    def wait_all(self, timeout=3600.0):
+        idle = ctypes.c_int()
+        _check(lib().tclbx_server_wait_all(self._h, float(timeout), ctypes.byref(idle)))
+        return bool(idle.value)
+
+    def stats(self):
+        c, r, ns = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
+        busy, now = ctypes.c_double(), ctypes.c_double()
+        a, q = ctypes.c_int(), ctypes.c_int()
+        _check(lib().tclbx_server_stats(self._h, ctypes.byref(c), ctypes.byref(r), ctypes.byref(ns),
+                                        ctypes.byref(busy), ctypes.byref(a), ctypes.byref(q), ctypes.byref(now)))
+        return {"completed": c.value, "rejected": r.value, "node_steps": ns.value, "busy_s": busy.value,
+                "active": a.value, "queued": q.value, "now": now.value}
+
+    def set_tuning(self, block=256, rows_per_tile=1, grid=0):
+        _check(lib().tclbx_server_set_tuning(self._h, int(block), int(rows_per_tile), int(grid)))

@@ -216,3 +216,39 @@ def d2q9_cylinder_sweep(ncases=256, seed=4, steps=8000, umax=0.02):
         c.name = f"cylinder_{k}_{nx}x{ny}_Re{re:.1f}"
         out.append(c)
     return out
+
+
+def d3q19_serving_requests(nreq, rate=50.0, seed=5, side=(48, 256), porosity=(0.0, 0.25), force=(1e-7, 1e-5),
+                           tau=(0.52, 0.9), budget=(1000, 20000)):
+    """A Poisson request stream of d3q19 porous boxes: arrival gaps
+    exponential with `rate` per second; each side length log-uniform in
+    `side`; Solid spheres (radius 4-12) up to a solid fraction uniform in
+    `porosity`; bounce-back walls at y = 0 and ny-1, periodic in x and z;
+    ForceX uniform in `force`; tau uniform in `tau` (omega = 1/tau, reference
+    MRT rates); a step budget uniform in `budget`.  Returns
+    [(arrival_s, Case, max_steps)], request k drawing from splitmix64
+    stream seed*100003 + k."""
+    m = Model("d3q19")
+    MRT = m.nodetype("MRT")[0]
+    WALL, SOLID = m.nodetype("Wall")[0], m.nodetype("Solid")[0]
+    out = []
+    t = 0.0
+    arrivals = Stream(seed)
+    for k in range(nreq):
+        t += -np.log(1.0 - arrivals.uniform()) / rate
+        st = Stream(seed * 100003 + k)
+        lo, hi = np.log(side[0]), np.log(side[1])
+        nx, ny, nz = [int(round(np.exp(st.uniform(None, lo, hi)))) for _ in range(3)]
+        frac = st.uniform(None, *porosity)
+        g = st.uniform(None, *force)
+        ta = st.uniform(None, *tau)
+        steps = int(st.uniform(None, budget[0], budget[1] + 1))
+        flags = np.full((nz, ny, nx), MRT, dtype=np.uint16)
+        if frac > 0:
+            flags[random_spheres(nx, ny, nz, frac, 4, 12, seed * 100003 + k)] = SOLID
+        flags[:, 0, :] = WALL
+        flags[:, ny - 1, :] = WALL
+        c = Case(f"request_{k}_{nx}x{ny}x{nz}", "d3q19", nx, ny, nz, flags,
+                 settings=[("omega", 1.0 / ta), ("ForceX", g)], steps=steps)
+        out.append((t, c, steps))
+    return out

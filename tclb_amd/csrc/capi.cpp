@@ -7,6 +7,7 @@
 #include "batch.h"
 #include "lattice.h"
 #include "model.h"
+#include "server.h"
 
 using namespace tclbx;
 
@@ -16,6 +17,10 @@ struct tclbx_lattice {
 
 struct tclbx_batch {
     Batch* b;
+};
+
+struct tclbx_server {
+    Server* s;
 };
 
 namespace {
@@ -42,6 +47,8 @@ int guard(F&& f) {
     if (!(h) || !(h)->lat) return fail("null lattice handle")
 #define CHECK_B(h) \
     if (!(h) || !(h)->b) return fail("null batch handle")
+#define CHECK_S(h) \
+    if (!(h) || !(h)->s) return fail("null server handle")
 }  // namespace
 
 extern "C" {
@@ -385,6 +392,116 @@ int tclbx_batch_set_graph_steps(tclbx_batch* b, int steps) {
     if (steps < 0 || (steps % 2) != 0) return fail("graph steps must be a non-negative even number");
     b->b->set_graph_steps(steps);
     return TCLBX_OK;
+}
+
+int tclbx_server_create(const char* model, const char* storage, long long hbm_budget, int max_chunk,
+                        tclbx_server** out) {
+    return guard([&]() {
+        if (!out) return fail("null output pointer");
+        *out = nullptr;
+        const ModelDef* m = model ? model_by_name(model) : nullptr;
+        if (!m) return fail(std::string("unknown model: ") + (model ? model : "(null)"));
+        int st = storage_by_name(storage ? storage : "f64");
+        if (st < 0) return fail(std::string("unknown storage: ") + storage);
+        if (hbm_budget <= 0) return fail("HBM budget must be positive");
+        Server* s = new Server(m, st, (size_t)hbm_budget, max_chunk);
+        *out = new tclbx_server{s};
+        return TCLBX_OK;
+    });
+}
+int tclbx_server_destroy(tclbx_server* s) {
+    if (!s) return TCLBX_OK;
+    delete s->s;
+    delete s;
+    return TCLBX_OK;
+}
+int tclbx_server_submit(tclbx_server* s, int nx, int ny, int nz, const uint16_t* flags, int nset,
+                        const int* set_ids, const double* set_values, int nzonal, const int* zonal_ids,
+                        const int* zonal_zones, const double* zonal_values, long long max_steps, int nstop,
+                        const int* stop_global, const int* stop_type, const double* stop_limit,
+                        int stop_interval, int stop_times, int* id) {
+    CHECK_S(s);
+    if (!flags || !id) return fail("null flags or id");
+    if (nx <= 0 || ny <= 0 || nz <= 0) return fail("lattice size must be positive");
+    if ((nset > 0 && (!set_ids || !set_values)) || (nzonal > 0 && (!zonal_ids || !zonal_zones || !zonal_values)) ||
+        (nstop > 0 && (!stop_global || !stop_type || !stop_limit)))
+        return fail("null array with a positive count");
+    return guard([&]() {
+        const ModelDef* m = s->s->model;
+        Request r;
+        r.nx = nx; r.ny = ny; r.nz = nz;
+        r.flags.assign(flags, flags + (size_t)nx * ny * nz);
+        for (int i = 0; i < nset; i++) {
+            if (set_ids[i] < 0 || set_ids[i] >= (int)m->settings.size()) return fail("setting id out of range");
+            r.settings.emplace_back(set_ids[i], set_values[i]);
+        }
+        for (int i = 0; i < nzonal; i++) {
+            if (zonal_ids[i] < 0 || zonal_ids[i] >= (int)m->zonal.size()) return fail("zonal id out of range");
+            if (zonal_zones[i] < -1 || zonal_zones[i] > m->zone_max) return fail("zone out of range");
+            r.zonal.push_back({zonal_ids[i], zonal_zones[i], zonal_values[i]});
+        }
+        r.max_steps = max_steps;
+        for (int i = 0; i < nstop; i++) r.stop.crit.push_back({stop_global[i], stop_type[i], stop_limit[i]});
+        r.stop.interval = nstop > 0 ? stop_interval : 0;
+        r.stop.times = stop_times;
+        if (nstop > 0 && stop_interval < 1) return fail("stop interval must be positive");
+        *id = s->s->Submit(std::move(r));
+        return TCLBX_OK;
+    });
+}
+int tclbx_server_poll(tclbx_server* s, int id, int* state, long long* steps, double* globals, int n,
+                      double* t_submit, double* t_admit, double* t_finish) {
+    CHECK_S(s);
+    return guard([&]() {
+        RequestResult r = s->s->Poll(id);
+        if (state) *state = r.state;
+        if (steps) *steps = r.steps;
+        if (globals) {
+            if (n < (int)s->s->model->globals.size()) return fail("globals buffer too small");
+            for (size_t i = 0; i < s->s->model->globals.size(); i++)
+                globals[i] = i < r.globals.size() ? r.globals[i] : 0.0;
+        }
+        if (t_submit) *t_submit = r.t_submit;
+        if (t_admit) *t_admit = r.t_admit;
+        if (t_finish) *t_finish = r.t_finish;
+        return TCLBX_OK;
+    });
+}
+int tclbx_server_wait_all(tclbx_server* s, double timeout_s, int* idle) {
+    CHECK_S(s);
+    return guard([&]() {
+        const bool ok = s->s->WaitAll(timeout_s);
+        if (idle) *idle = ok ? 1 : 0;
+        const std::string err = s->s->Error();
+        if (!err.empty()) return fail("server stopped after an error: " + err);
+        return TCLBX_OK;
+    });
+}
+int tclbx_server_stats(tclbx_server* s, long long* completed, long long* rejected, long long* node_steps,
+                       double* busy_s, int* active, int* queued, double* now_s) {
+    CHECK_S(s);
+    Server::Stats st = s->s->stats();
+    if (completed) *completed = st.completed;
+    if (rejected) *rejected = st.rejected;
+    if (node_steps) *node_steps = st.node_steps;
+    if (busy_s) *busy_s = st.busy_s;
+    if (active) *active = st.active;
+    if (queued) *queued = st.queued;
+    if (now_s) *now_s = s->s->now();
+    return TCLBX_OK;
+}
+int tclbx_server_set_tuning(tclbx_server* s, int block, int rows_per_tile, int grid) {
+    CHECK_S(s);
+    return guard([&]() {
+        if (block != 64 && block != 128 && block != 256) return fail("block must be 64, 128 or 256");
+        if (rows_per_tile < 1 || rows_per_tile > 64 || grid < 0) return fail("bad tuning");
+        Tuning t;
+        t.block = block;
+        t.rpt = rows_per_tile;
+        t.grid = grid;
+        s->s->SetTuning(t);
+        return TCLBX_OK;
+    });
 }
 
 int tclbx_copy_bandwidth(long long bytes, int reps, double* gbps, double* ms) {
