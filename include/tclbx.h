@@ -153,11 +153,13 @@ int tclbx_batch_get_iteration(const tclbx_batch* b, int id, long long* iter);
 int tclbx_batch_get_quantity(tclbx_batch* b, int id, int q, double* out, double scale);
 int tclbx_batch_get_density(tclbx_batch* b, int id, int q, double* out);
 int tclbx_batch_set_density(tclbx_batch* b, int id, int q, const double* in);
-/* time every tile shape (threads per block 64/128/256, rows per tile
- * 1/2/4) and persistent grid size on the current cases for `steps` steps
- * each, without changing their state, and keep the fastest */
-int tclbx_batch_autotune(tclbx_batch* b, int steps, int* block, int* rows_per_tile, int* grid, double* us_per_step);
-int tclbx_batch_set_tuning(tclbx_batch* b, int block, int rows_per_tile, int grid);
+/* time every tile shape (row segments or flat runs of the node index;
+ * threads per block 64/128/256; rows per tile 1/2/4) and persistent grid
+ * size on the current cases for `steps` steps each, without changing their
+ * state, and keep the fastest */
+int tclbx_batch_autotune(tclbx_batch* b, int steps, int* flat, int* block, int* rows_per_tile, int* grid,
+                         double* us_per_step);
+int tclbx_batch_set_tuning(tclbx_batch* b, int flat, int block, int rows_per_tile, int grid);
 int tclbx_batch_get_timing(const tclbx_batch* b, double* last_ms, double* total_ms);
 int tclbx_batch_info(const tclbx_batch* b, int* ncases, long long* nodes, long long* arena_used,
                      long long* arena_capacity);
@@ -200,7 +202,8 @@ int tclbx_server_poll(tclbx_server* s, int id, int* state, long long* steps, dou
 int tclbx_server_wait_all(tclbx_server* s, double timeout_s, int* idle);
 int tclbx_server_stats(tclbx_server* s, long long* completed, long long* rejected, long long* node_steps,
                        double* busy_s, int* active, int* queued, double* now_s);
-int tclbx_server_set_tuning(tclbx_server* s, int block, int rows_per_tile, int grid);
+/* tile shape of the server's batch (flat runs of the node index by default) */
+int tclbx_server_set_tuning(tclbx_server* s, int flat, int block, int rows_per_tile, int grid);
 
 /* Measured device copy bandwidth: `reps` copies of `bytes` (rounded down
  * to 16) between two device buffers by a plain vector-copy kernel;

@@ -113,8 +113,8 @@ def lib():
     L.tclbx_batch_get_quantity.argtypes = [vp, c_int, c_int, dp, c_double]
     L.tclbx_batch_get_density.argtypes = [vp, c_int, c_int, dp]
     L.tclbx_batch_set_density.argtypes = [vp, c_int, c_int, dp]
-    L.tclbx_batch_autotune.argtypes = [vp, c_int, ip, ip, ip, dp]
-    L.tclbx_batch_set_tuning.argtypes = [vp, c_int, c_int, c_int]
+    L.tclbx_batch_autotune.argtypes = [vp, c_int, ip, ip, ip, ip, dp]
+    L.tclbx_batch_set_tuning.argtypes = [vp, c_int, c_int, c_int, c_int]
     L.tclbx_batch_get_timing.argtypes = [vp, dp, dp]
     L.tclbx_batch_info.argtypes = [vp, ip, llp, llp, llp]
     L.tclbx_batch_set_graph_steps.argtypes = [vp, c_int]
@@ -125,7 +125,7 @@ def lib():
     L.tclbx_server_poll.argtypes = [vp, c_int, ip, llp, dp, c_int, dp, dp, dp]
     L.tclbx_server_wait_all.argtypes = [vp, c_double, ip]
     L.tclbx_server_stats.argtypes = [vp, llp, llp, llp, dp, ip, ip, dp]
-    L.tclbx_server_set_tuning.argtypes = [vp, c_int, c_int, c_int]
+    L.tclbx_server_set_tuning.argtypes = [vp, c_int, c_int, c_int, c_int]
     _lib = L
     return L
 
@@ -473,13 +473,14 @@ class Batch:
         return cid
 
     def autotune(self, steps=10):
-        b, r, g, us = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
-        _check(lib().tclbx_batch_autotune(self._h, int(steps), ctypes.byref(b), ctypes.byref(r), ctypes.byref(g),
-                                          ctypes.byref(us)))
-        return {"block": b.value, "rows_per_tile": r.value, "grid": g.value, "us_per_step": us.value}
+        fl, b, r, g, us = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _check(lib().tclbx_batch_autotune(self._h, int(steps), ctypes.byref(fl), ctypes.byref(b), ctypes.byref(r),
+                                          ctypes.byref(g), ctypes.byref(us)))
+        return {"flat": fl.value, "block": b.value, "rows_per_tile": r.value, "grid": g.value,
+                "us_per_step": us.value}
 
-    def set_tuning(self, block=256, rows_per_tile=1, grid=0):
-        _check(lib().tclbx_batch_set_tuning(self._h, int(block), int(rows_per_tile), int(grid)))
+    def set_tuning(self, block=256, rows_per_tile=1, grid=0, flat=False):
+        _check(lib().tclbx_batch_set_tuning(self._h, int(bool(flat)), int(block), int(rows_per_tile), int(grid)))
 
     def timing(self):
         a, b = ctypes.c_double(), ctypes.c_double()
@@ -573,5 +574,5 @@ class Server:
         return {"completed": c.value, "rejected": r.value, "node_steps": ns.value, "busy_s": busy.value,
                 "active": a.value, "queued": q.value, "now": now.value}
 
-    def set_tuning(self, block=256, rows_per_tile=1, grid=0):
-        _check(lib().tclbx_server_set_tuning(self._h, int(block), int(rows_per_tile), int(grid)))
+    def set_tuning(self, block=256, rows_per_tile=1, grid=0, flat=True):
+        _check(lib().tclbx_server_set_tuning(self._h, int(bool(flat)), int(block), int(rows_per_tile), int(grid)))

@@ -216,8 +216,16 @@ StepArgs Batch::case_args(const Case& c) const {
     a.zone = c.d_zone;
     a.pitch = (long long)c.pitch;
     a.nx = c.nx; a.ny = c.ny; a.nz = c.nz;
-    a.bpr = (c.nx + tune_.block - 1) / tune_.block;
-    a.rpb = globals_rows_per_block(c.ny * c.nz, a.bpr);
+    a.flat = tune_.flat;
+    a.inv_nx = 1.0 / c.nx;
+    if (tune_.flat) {
+        // a "row" of a flat tile is a run of `block` consecutive nodes
+        a.bpr = 1;
+        a.rpb = globals_rows_per_block(((long long)c.N + tune_.block - 1) / tune_.block, 1);
+    } else {
+        a.bpr = (c.nx + tune_.block - 1) / tune_.block;
+        a.rpb = globals_rows_per_block(c.ny * c.nz, a.bpr);
+    }
     a.rpt = tune_.rpt;
     a.zcols = (int)model->zonal.size();
     a.nzones = model->zone_max + 1;
@@ -274,10 +282,9 @@ void Batch::rebuild() {
         bc.args = case_args(c);
         bc.parity = c.parity;
         descs.push_back(bc);
-        const int rows = c.ny * c.nz;
-        const int nt = bc.args.bpr * ((rows + bc.args.rpt - 1) / bc.args.rpt);
+        const int nt = step_blocks(bc.args, false, tune_.block);
         for (int t = 0; t < nt; t++) tiles.push_back(make_int2(slot, t));
-        const int ng = bc.args.bpr * ((rows + bc.args.rpb - 1) / bc.args.rpb);
+        const int ng = step_blocks(bc.args, true, tune_.block);
         first.push_back((int)gtiles.size());
         count.push_back(ng);
         for (int t = 0; t < ng; t++) gtiles.push_back(make_int2(slot, t));
@@ -477,10 +484,12 @@ Tuning Batch::Autotune(int steps) {
     trials_.clear();
     Tuning best = saved;
     best.us_per_step = 1e300;
+    for (int flat : {0, 1})
     for (int block : {256, 128, 64})
         for (int rpt : {1, 2, 4})
-            for (int gmul : {0, 4, 8, 16}) {
+            for (int gmul : {0, 8}) {
                 Tuning t;
+                t.flat = flat;
                 t.block = block;
                 t.rpt = rpt;
                 t.grid = gmul * ncu_;

@@ -47,6 +47,7 @@ struct Tuning {
     int block = 256;  // x-segment width of a tile (threads per block)
     int rpt = 1;      // rows per plain tile (nodes per thread)
     int grid = 0;     // persistent blocks (0: one block per tile)
+    int flat = 0;     // 1: tiles are runs of the linear node index (no idle lanes at row ends)
     double us_per_step = 0;
 };
 

@@ -350,10 +350,12 @@ int tclbx_batch_set_density(tclbx_batch* b, int id, int q, const double* in) {
     if (!in) return fail("null input");
     return guard([&]() { b->b->SetDensity(id, q, in); return TCLBX_OK; });
 }
-int tclbx_batch_autotune(tclbx_batch* b, int steps, int* block, int* rows_per_tile, int* grid, double* us_per_step) {
+int tclbx_batch_autotune(tclbx_batch* b, int steps, int* flat, int* block, int* rows_per_tile, int* grid,
+                         double* us_per_step) {
     CHECK_B(b);
     return guard([&]() {
         Tuning t = b->b->Autotune(steps);
+        if (flat) *flat = t.flat;
         if (block) *block = t.block;
         if (rows_per_tile) *rows_per_tile = t.rpt;
         if (grid) *grid = t.grid;
@@ -361,10 +363,11 @@ int tclbx_batch_autotune(tclbx_batch* b, int steps, int* block, int* rows_per_ti
         return TCLBX_OK;
     });
 }
-int tclbx_batch_set_tuning(tclbx_batch* b, int block, int rows_per_tile, int grid) {
+int tclbx_batch_set_tuning(tclbx_batch* b, int flat, int block, int rows_per_tile, int grid) {
     CHECK_B(b);
     return guard([&]() {
         Tuning t;
+        t.flat = flat ? 1 : 0;
         t.block = block;
         t.rpt = rows_per_tile;
         t.grid = grid;
@@ -490,12 +493,13 @@ int tclbx_server_stats(tclbx_server* s, long long* completed, long long* rejecte
     if (now_s) *now_s = s->s->now();
     return TCLBX_OK;
 }
-int tclbx_server_set_tuning(tclbx_server* s, int block, int rows_per_tile, int grid) {
+int tclbx_server_set_tuning(tclbx_server* s, int flat, int block, int rows_per_tile, int grid) {
     CHECK_S(s);
     return guard([&]() {
         if (block != 64 && block != 128 && block != 256) return fail("block must be 64, 128 or 256");
         if (rows_per_tile < 1 || rows_per_tile > 64 || grid < 0) return fail("bad tuning");
         Tuning t;
+        t.flat = flat ? 1 : 0;
         t.block = block;
         t.rpt = rows_per_tile;
         t.grid = grid;

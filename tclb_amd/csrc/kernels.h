@@ -24,6 +24,8 @@ struct StepArgs {
     int bpr;               // blocks per (y,z) row
     int rpb;               // rows walked by one block of a Globals step
     int rpt;               // rows walked by one block of a plain step
+    int flat;              // 1: tiles are runs of the linear node index, not row segments
+    double inv_nx;         // 1/nx (flat tiles: node index -> row)
     int zcols, nzones;
     double par[MAX_PAR];
 };
@@ -37,8 +39,9 @@ struct BatchCase {
     int pad_[3];
 };
 
-// Blocks of one step launch (a Globals step walks a.rpb rows per block).
-int step_blocks(const StepArgs& a, bool globals);
+// Blocks (tiles) of one step launch of the lattice `a` with `block`
+// threads per block (a Globals step walks a.rpb rows per block).
+int step_blocks(const StepArgs& a, bool globals, int block);
 // One time step: pull streaming + NodeType dispatch + collision (+ Globals).
 hipError_t launch_step(int model, int storage, bool globals, const StepArgs& a, int block,
                        hipStream_t s);

@@ -688,9 +688,43 @@ __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, 
 // The rows of one tile: x segment `xb` of rows [rowblk*rpb, +rpb) of the
 // lattice `a`; with GLOB the tile's Globals go to partial slot `slot` of
 // `nslots`.
+// node index n -> (row, x) for flat tiles: a double-precision estimate of
+// n / nx corrected to the exact quotient
+__device__ __forceinline__ void split_node(unsigned n, int nx, double inv_nx, int& row, int& x) {
+    int r = (int)((double)n * inv_nx);
+    int xx = (int)n - r * nx;
+    if (xx < 0) { r--; xx += nx; }
+    if (xx >= nx) { r++; xx -= nx; }
+    row = r;
+    x = xx;
+}
+
 template <int MODEL, int ST, bool GLOB>
 __device__ __forceinline__ void step_tile(const StepArgs& a, int tile, int rpb, int slot, int nslots) {
     using L = Lat<MODEL>;
+    if (a.flat) {
+        // runs of blockDim consecutive nodes (rows of the lattice are
+        // concatenated, no lane is left idle at the end of a row)
+        double g[L::G > 0 ? L::G : 1];
+        if (GLOB) {
+#pragma unroll
+            for (int i = 0; i < L::G; i++) g[i] = 0.0;
+        }
+        const unsigned N = (unsigned)a.nx * (unsigned)a.ny * (unsigned)a.nz;
+        for (int r = 0; r < rpb; r++) {
+            const unsigned n = ((unsigned)tile * rpb + r) * blockDim.x + threadIdx.x;
+            const bool in = n < N;
+            int row, x;
+            split_node(in ? n : N - 1, a.nx, a.inv_nx, row, x);
+            const int xm = x == 0 ? a.nx - 1 : x - 1, xp = x == a.nx - 1 ? 0 : x + 1;
+            node_update<MODEL, ST, GLOB>(a, row, x, xm, xp, in, g);
+        }
+        if (GLOB) {
+            constexpr int SUMG = (MODEL == MODEL_D3Q19) ? 25 : L::G;
+            block_globals<L::G>(g, SUMG, a.gpart, slot, nslots);
+        }
+        return;
+    }
     const int rowblk = tile / a.bpr;
     const int xb = tile - rowblk * a.bpr;
     const int x = xb * blockDim.x + threadIdx.x;
@@ -907,7 +941,7 @@ int nblocks_of(const StepArgs& a, int block) { return a.bpr * a.ny * a.nz; }
 
 template <int MODEL, int ST>
 hipError_t step_ms(bool glob, const StepArgs& a, int block, hipStream_t s) {
-    dim3 grid(step_blocks(a, glob)), blk(block);
+    dim3 grid(step_blocks(a, glob, block)), blk(block);
     if (glob) hipLaunchKernelGGL((k_step<MODEL, ST, true>), grid, blk, 0, s, a);
     else hipLaunchKernelGGL((k_step<MODEL, ST, false>), grid, blk, 0, s, a);
     return hipGetLastError();
@@ -971,9 +1005,13 @@ double weight(int model, int q) {
 }
 }  // namespace
 
-int step_blocks(const StepArgs& a, bool globals) {
-    const int rows = a.ny * a.nz;
+int step_blocks(const StepArgs& a, bool globals, int block) {
     const int rpb = globals ? a.rpb : a.rpt;
+    if (a.flat) {
+        const long long chunks = ((long long)a.nx * a.ny * a.nz + block - 1) / block;
+        return (int)((chunks + rpb - 1) / rpb);
+    }
+    const int rows = a.ny * a.nz;
     return a.bpr * ((rows + rpb - 1) / rpb);
 }
 

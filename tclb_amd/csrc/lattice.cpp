@@ -189,6 +189,8 @@ StepArgs Lattice::args(int from, int to, bool glob) const {
     a.bpr = bpr_;
     a.rpb = rpb_;
     a.rpt = 1;
+    a.flat = 0;
+    a.inv_nx = 1.0 / nx;
     a.zcols = zcols_;
     a.nzones = nzones_;
     model_params(model, settings_, a.par);
@@ -231,7 +233,7 @@ void Lattice::launch(int from, int to, bool glob) {
     StepArgs a = args(from, to, glob);
     hip_check(launch_step(model->id, storage, glob, a, block_, stream_), "step kernel");
     if (glob) {
-        hip_check(launch_reduce_globals(d_gpart_, step_blocks(a, true), kernel_nglobals(model->id),
+        hip_check(launch_reduce_globals(d_gpart_, step_blocks(a, true, block_), kernel_nglobals(model->id),
                                         model->sum_globals, d_glob_, stream_), "globals reduction");
     }
 }

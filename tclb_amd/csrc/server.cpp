@@ -42,6 +42,11 @@ Server::Server(const ModelDef* m, int storage, size_t hbm_budget, int max_chunk)
     : model(m), max_chunk_(max_chunk), t0_(std::chrono::steady_clock::now()) {
     if (max_chunk < 1) throw std::invalid_argument("chunk must be at least one step");
     batch_.reset(new Batch(m, storage, hbm_budget));
+    // requests have arbitrary sizes: flat tiles leave no lane idle at the
+    // end of a row
+    Tuning t;
+    t.flat = 1;
+    batch_->SetTuning(t);
     worker_ = std::thread([this]() { loop(); });
 }
 

@@ -56,6 +56,25 @@ def test_batch_matches_single_lattices(model, storage):
             assert b.iteration(cid) == lat.iteration
 
 
+@pytest.mark.parametrize("flat", [False, True])
+def test_flat_tiles_match_row_tiles(flat):
+    """Flat tiles (runs of the node index) compute the same populations as
+    row tiles; their Globals agree to the reduction's rounding."""
+    say(f"flat tiles {flat}")
+    cs = mixed_cases("d3q19", 17)
+    a = tclb_amd.Batch("d3q19", "f64", capacity_bytes=64 << 20)
+    b = tclb_amd.Batch("d3q19", "f64", capacity_bytes=64 << 20)
+    a.set_tuning(block=128, rows_per_tile=3, grid=0, flat=flat)
+    b.set_tuning(block=64, rows_per_tile=1, grid=5, flat=False)
+    ia = [a.add(c) for c in cs]
+    ib = [b.add(c) for c in cs]
+    a.iterate(9)
+    b.iterate(9)
+    for x, y in zip(ia, ib):
+        assert np.array_equal(a.get_f(x), b.get_f(y), equal_nan=True)
+        glob_close(a.globals(x), b.globals(y), 1e-13)
+
+
 def test_batch_case_alone_and_in_mix_and_oracle(oracle_mod):
     say("case alone vs in a mix vs oracle")
     cs = mixed_cases("d3q19", 5)
@@ -85,6 +104,7 @@ def test_autotune_keeps_state():
     b.iterate(5)
     t = a.autotune(steps=2)
     assert t["block"] in (64, 128, 256) and t["rows_per_tile"] in (1, 2, 4) and t["us_per_step"] > 0
+    assert t["flat"] in (0, 1)
     a.iterate(8)
     b.iterate(8)
     for x, y in zip(ia, ib):
