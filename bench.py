@@ -52,7 +52,7 @@ def cpu_baseline(n, seconds, threads):
     t0 = time.perf_counter()
     L.iterate(1)
     t1 = time.perf_counter() - t0
-    steps = max(1, min(500, int(seconds / max(t1, 1e-6))))
+    steps = max(1, min(20000, int(seconds / max(t1, 1e-6))))
     t0 = time.perf_counter()
     L.iterate(steps)
     dt = time.perf_counter() - t0
@@ -66,6 +66,77 @@ def cpu_baseline(n, seconds, threads):
     }
 
 
+def batched_section(steps, seq_cases):
+    """The 256-case d2q9 cylinder sweep in one batch (FP16-shift storage,
+    FP32 collision, autotuned tiles) against the same cases solved one at a
+    time as FP64 lattices (a sample of `seq_cases`, scaled by nodes)."""
+    import tclb_amd
+    from tclb_amd import cases
+    sweep = cases.d2q9_cylinder_sweep(ncases=256, steps=steps)
+    nodes = sum(c.nodes for c in sweep)
+    probe = tclb_amd.Batch("d2q9_SRT_drag", "f16s", 1 << 20)
+    cap = sum(probe.case_bytes(c.nx, c.ny) for c in sweep) + (16 << 20)
+    probe.close()
+    b = tclb_amd.Batch("d2q9_SRT_drag", "f16s", capacity_bytes=cap)
+    for c in sweep:
+        b.add(c)
+    tune = b.autotune(steps=5)
+    t0 = time.perf_counter()
+    b.iterate(steps)
+    wall = time.perf_counter() - t0
+    b.close()
+    seq_nodes, seq_t = 0, 0.0
+    for c in sweep[:seq_cases]:
+        lat = c.build("f64")
+        t0 = time.perf_counter()
+        lat.iterate(steps)
+        seq_t += time.perf_counter() - t0
+        seq_nodes += c.nodes
+        lat.close()
+    seq_mlups = seq_nodes * steps / seq_t / 1e6
+    batch_mlups = nodes * steps / wall / 1e6
+    return {
+        "workload": f"256 d2q9 cylinder cases (nx 128-1024, ny 64-256, Re 20-200), {nodes} nodes, "
+                    f"{steps} steps timed, FP16-shift storage",
+        "MLUPS": round(batch_mlups, 1),
+        "solves_per_s_8000_steps": round(256 / (wall * 8000 / steps), 2),
+        "tuning": tune,
+        "sequential_f64_MLUPS": round(seq_mlups, 1),
+        "sequential_f64_sample_cases": seq_cases,
+        "speedup_solves_per_s": round(batch_mlups / seq_mlups, 2),
+    }
+
+
+def serving_section(rate, window, sample):
+    from tclb_amd import serve
+    r = serve.run(rate=rate, window=window, storage="f16s", budget_gb=240.0, sample=sample)
+    keep = ["rate_per_s", "window_s", "requests", "completed", "stopped_by_rule", "solves_per_s",
+            "latency_p50_s", "latency_p99_s", "queue_wait_p50_s", "MLUPS", "MLUPS_while_busy",
+            "sequential_f64_MLUPS", "sequential_f64_solves_per_s", "speedup_solves_per_s"]
+    out = {k: (round(r[k], 4) if isinstance(r.get(k), float) else r.get(k)) for k in keep}
+    out["workload"] = ("Poisson stream of d3q19 porous boxes (sides log-U[48,256], porosity U[0,25]%, "
+                       "g_x U[1e-7,1e-5], tau U[0.52,0.9], budget U[1k,20k] steps or VOLvx relative change "
+                       "< 1e-6 over 500 steps), FP16-shift storage, 240 GB HBM budget")
+    return out
+
+
+def cylinder_section(steps):
+    """The 1024x128 d2q9 cylinder in FP64 with the drag/lift Globals on the
+    last step of each 1000-step segment."""
+    import tclb_amd
+    from tclb_amd import cases
+    c = cases.d2q9_cylinder(steps=steps)
+    lat = c.build("f64")
+    lat.iterate(100)
+    t0 = time.perf_counter()
+    for _ in range(steps // 1000):
+        lat.iterate(1000)
+    wall = time.perf_counter() - t0
+    g = lat.global_dict()
+    return {"workload": "d2q9 cylinder 1024x128, FP64, Globals every 1000 steps", "steps": steps,
+            "MLUPS": round(c.nodes * steps / wall / 1e6, 1), "drag": g["Drag"], "lift": g["Lift"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +147,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-n", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the batched, serving and cylinder sections")
+    ap.add_argument("--batch-steps", type=int, default=1000)
+    ap.add_argument("--serve-rate", type=float, default=2.0)
+    ap.add_argument("--serve-window", type=float, default=15.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs one GPU only: the node kernel is measured on one MI355X "
@@ -152,6 +227,15 @@ def main():
     }
     if "f16s" in results and "f64" in results:
         out["f16s_over_f64"] = round(results["f16s"]["MLUPS"] / results["f64"]["MLUPS"], 3)
+    if not args.no_extra:
+        log("batched sweep")
+        out["batched"] = batched_section(args.batch_steps, 16)
+        log(f"batched: {out['batched']['MLUPS']} MLUPS, x{out['batched']['speedup_solves_per_s']} vs sequential FP64")
+        log("serving")
+        out["serving"] = serving_section(args.serve_rate, args.serve_window, 2)
+        log(f"serving: p50 {out['serving']['latency_p50_s']} s, p99 {out['serving']['latency_p99_s']} s")
+        log("cylinder 1024x128")
+        out["cylinder"] = cylinder_section(5000)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         log(f"CPU baseline: oracle on {args.cpu_n}^3, {threads} threads")
