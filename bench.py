@@ -137,6 +137,18 @@ def cylinder_section(steps):
             "MLUPS": round(c.nodes * steps / wall / 1e6, 1), "drag": g["Drag"], "lift": g["Lift"]}
 
 
+def cavity_section(n, steps):
+    """The 640^3 d3q19 lid-driven cavity in every storage format: MLUPS,
+    and rho/u/lid-torque errors against the FP64 run after `steps` steps."""
+    from tclb_amd import cavity
+    r = cavity.run(n=n, steps=steps, chunk=steps, log=log)
+    for v in r["formats"].values():
+        for k in list(v):
+            if isinstance(v[k], float):
+                v[k] = round(v[k], 6) if abs(v[k]) >= 1e-3 else float(f"{v[k]:.4g}")
+    return r
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,10 +159,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-n", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extra", action="store_true", help="skip the batched, serving and cylinder sections")
+    ap.add_argument("--no-extra", action="store_true", help="skip the batched, serving, cylinder and cavity sections")
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--serve-rate", type=float, default=2.0)
     ap.add_argument("--serve-window", type=float, default=15.0)
+    ap.add_argument("--cavity-n", type=int, default=640)
+    ap.add_argument("--cavity-steps", type=int, default=300)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs one GPU only: the node kernel is measured on one MI355X "
@@ -236,6 +250,8 @@ def main():
         log(f"serving: p50 {out['serving']['latency_p50_s']} s, p99 {out['serving']['latency_p99_s']} s")
         log("cylinder 1024x128")
         out["cylinder"] = cylinder_section(5000)
+        log(f"cavity {args.cavity_n}^3")
+        out["cavity"] = cavity_section(args.cavity_n, args.cavity_steps)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         log(f"CPU baseline: oracle on {args.cpu_n}^3, {threads} threads")

@@ -154,8 +154,8 @@ orc_lattice* orc_create(int model, int nx, int ny, int nz, int storage)
     d3q19_sparse_init_f();
     L->model = model; L->nx = nx; L->ny = ny; L->nz = nz; L->storage = storage;
     L->N = (size_t)nx * ny * nz;
-    if (model == ORC_D3Q19) {
-        L->Q = 19; L->nglob = 26;
+    if (model == ORC_D3Q19 || model == ORC_D3Q19_LID) {
+        L->Q = 19; L->nglob = model == ORC_D3Q19_LID ? 30 : 26;
         for (q = 0; q < 19; q++) {
             L->e[q][0] = M_d[3][q]; L->e[q][1] = M_d[5][q]; L->e[q][2] = M_d[7][q];
             L->w[q] = d3q19_w[q];
@@ -207,7 +207,8 @@ void orc_set_flags(orc_lattice* L, const uint16_t* flags)
     memcpy(L->flags, flags, L->N * sizeof(uint16_t));
 }
 
-static int zone_shift(const orc_lattice* L) { return L->model == ORC_D3Q19 ? 6 : 4; }
+static int is3d(const orc_lattice* L) { return L->model == ORC_D3Q19 || L->model == ORC_D3Q19_LID; }
+static int zone_shift(const orc_lattice* L) { return is3d(L) ? 6 : 4; }
 
 static const double* zone_row(const orc_lattice* L, uint16_t nt)
 {
@@ -222,7 +223,7 @@ void orc_init(orc_lattice* L)
     int q;
     for (n = 0; n < L->N; n++) {
         const double* zr = zone_row(L, L->flags[n]);
-        if (L->model == ORC_D3Q19) {
+        if (is3d(L)) {
             /* Init(): SetEquilibrum(1.0, InletVelocity, 0, 0), Dynamics.c.Rt:219-221 */
             if (L->storage == ORC_F64) {
                 double f[19];
@@ -272,6 +273,7 @@ static void step_rows(orc_lattice* L, long double* glob, long r0, long r1)
     int x, y, z, q;
     long r;
     int drag = L->model == ORC_D2Q9_SRT_DRAG;
+    int lid = L->model == ORC_D3Q19_LID;
     for (r = r0; r < r1; r++)
         for (z = (int)(r / L->ny), y = (int)(r % L->ny), x = 0; x < L->nx; x++) {
                 size_t n = (size_t)x + (size_t)L->nx * ((size_t)y + (size_t)L->ny * z);
@@ -280,13 +282,13 @@ static void step_rows(orc_lattice* L, long double* glob, long r0, long r1)
                 double fd[19];
                 pull(L, in, x, y, z, fd);
                 if (L->storage == ORC_F64) {
-                    if (L->model == ORC_D3Q19) d3q19_node_d(fd, nt, L->params, zr, glob);
+                    if (is3d(L)) d3q19_node_d(fd, nt, L->params, zr, glob, lid, L->flags, x, y, z, L->nx, L->ny, L->nz);
                     else d2q9_node_d(fd, nt, L->params[0], L->params[1], zr, drag, glob);
                     for (q = 0; q < L->Q; q++) st(L, out, (size_t)q * L->N + n, q, fd[q]);
                 } else {
                     float ff[19];
                     for (q = 0; q < L->Q; q++) ff[q] = (float)fd[q];
-                    if (L->model == ORC_D3Q19) d3q19_node_f(ff, nt, L->params, zr, glob);
+                    if (is3d(L)) d3q19_node_f(ff, nt, L->params, zr, glob, lid, L->flags, x, y, z, L->nx, L->ny, L->nz);
                     else d2q9_node_f(ff, nt, (float)L->params[0], (float)L->params[1], zr, drag, glob);
                     for (q = 0; q < L->Q; q++) st(L, out, (size_t)q * L->N + n, q, ff[q]);
                 }
@@ -303,6 +305,7 @@ static void step(orc_lattice* L, long double* glob)
 {
     long rows = (long)L->ny * L->nz;
     int nt = g_threads < rows ? g_threads : (int)rows;
+    int lid = L->model == ORC_D3Q19_LID;
     if (nt <= 1) {
         step_rows(L, glob, 0, rows);
     } else {
@@ -318,7 +321,7 @@ static void step(orc_lattice* L, long double* glob)
         if (glob)
             for (t = 0; t < nt; t++)
                 for (g = 0; g < 32; g++) {
-                    if (L->model == ORC_D3Q19 && g == 25) { if (part[t][g] > glob[g]) glob[g] = part[t][g]; }
+                    if (is3d(L) && g == G_MAXV_OF(lid)) { if (part[t][g] > glob[g]) glob[g] = part[t][g]; }
                     else glob[g] += part[t][g];
                 }
     }
@@ -368,7 +371,7 @@ void orc_get_quantity(const orc_lattice* L, int which, double* out)
                 uint16_t nt = L->flags[n];
                 double f[19], rho = 0, jx = 0, jy = 0, jz = 0;
                 pull(L, L->buf[L->cur], x, y, z, f);
-                if (L->model == ORC_D3Q19) {
+                if (is3d(L)) {
                     /* getRho / getU / getP, Dynamics.c.Rt:23-40 */
                     for (q = 0; q < 19; q++) {
                         rho += f[q];
@@ -379,6 +382,9 @@ void orc_get_quantity(const orc_lattice* L, int which, double* out)
                     else {
                         double ux = jx / rho, uy = jy / rho, uz = jz / rho;
                         if ((nt & 0x1C) == 0x0C) ux = uy = uz = 0;
+                        if (L->model == ORC_D3Q19_LID && (nt & 0x1C) == 0x1C) {
+                            ux = L->params[11]; uy = L->params[12]; uz = L->params[13];
+                        }
                         out[3*n] = ux; out[3*n+1] = uy; out[3*n+2] = uz;
                     }
                 } else {
@@ -401,5 +407,5 @@ void orc_d3q19_feq(double feq[19], double rho, double jx, double jy, double jz) 
 void orc_d3q19_collide(double f[19], const double* params)
 {
     static const double zone[32] = {0};
-    d3q19_mrt_d(f, 0x20, params, zone, NULL);
+    d3q19_mrt_d(f, 0x20, params, zone, NULL, 0);
 }

@@ -12,6 +12,10 @@
  *                  in the form of models/flow/d3q27/Dynamics.c.Rt:68-82
  *   d3q19          models/flow/d3q19/Dynamics.c.Rt:23-290, Dynamics.R:1-95,
  *                  src/lib/d3q19.R:24-96
+ *   d3q19_lid      d3q19 plus a MovingWall node type (bounce-back with Ladd's
+ *                  wall-velocity term) and its force/torque Globals; the
+ *                  reference has no moving wall, so this part is pinned by
+ *                  closed forms only (Couette flow, momentum balance)
  * with pull streaming on a periodic torus (src/LatticeAccess.inc.cpp.Rt:
  * 319-347, src/Lattice.cu.Rt:445-453), the Run() node order of
  * src/cuda.cu.Rt:322-327 and Globals on the last step of a segment
@@ -32,7 +36,7 @@
 extern "C" {
 #endif
 
-enum { ORC_D2Q9_SRT = 0, ORC_D2Q9_SRT_DRAG = 1, ORC_D3Q19 = 2 };
+enum { ORC_D2Q9_SRT = 0, ORC_D2Q9_SRT_DRAG = 1, ORC_D3Q19 = 2, ORC_D3Q19_LID = 3 };
 enum { ORC_F64 = 0, ORC_F32 = 1, ORC_F32S = 2, ORC_F16S = 3, ORC_BF16S = 4 };
 
 /* d2q9 parameter vector:   [omega, Density]
@@ -40,8 +44,10 @@ enum { ORC_F64 = 0, ORC_F32 = 1, ORC_F32S = 2, ORC_F16S = 3, ORC_BF16S = 4 };
  *                           GravitationY, (drag model only) DragInObj, LiftInObj]
  * d3q19 parameter vector:  [omega, InletVelocity, InletDensity, ForceX, ForceY,
  *                           ForceZ, S_e, S_eps, S_q, S_pi, S_m]   (S_* <= 0: the
- *                           reference's fixed rates)
- * d3q19 zone columns:      24 <Global>InObj weights in Globals order.        */
+ *                           reference's fixed rates); d3q19_lid appends
+ *                           [MovingWallVelocityX, Y, Z]
+ * d3q19 zone columns:      24 <Global>InObj weights in Globals order (28 for
+ *                           d3q19_lid).                                       */
 
 typedef struct orc_lattice orc_lattice;
 

@@ -14,9 +14,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 
-D2Q9_SRT, D2Q9_SRT_DRAG, D3Q19 = 0, 1, 2
+D2Q9_SRT, D2Q9_SRT_DRAG, D3Q19, D3Q19_LID = 0, 1, 2, 3
 F64, F32, F32S, F16S, BF16S = 0, 1, 2, 3, 4
-MODELS = {"d2q9_SRT": D2Q9_SRT, "d2q9_SRT_drag": D2Q9_SRT_DRAG, "d3q19": D3Q19}
+MODELS = {"d2q9_SRT": D2Q9_SRT, "d2q9_SRT_drag": D2Q9_SRT_DRAG, "d3q19": D3Q19, "d3q19_lid": D3Q19_LID}
 STORAGE = {"f64": F64, "f32": F32, "f32s": F32S, "f16s": F16S, "bf16s": BF16S}
 
 _lib = None
@@ -152,6 +152,7 @@ def d3q19_collide(f, params):
 
 D3Q19_PARAMS = ["omega", "InletVelocity", "InletDensity", "ForceX", "ForceY", "ForceZ",
                 "MRT_Se", "MRT_Seps", "MRT_Sq", "MRT_Spi", "MRT_Sm"]
+D3Q19_LID_PARAMS = D3Q19_PARAMS + ["MovingWallVelocityX", "MovingWallVelocityY", "MovingWallVelocityZ"]
 
 
 def resolved_settings(model, settings):
@@ -181,6 +182,8 @@ def lattice_for_case(case, storage="f64"):
     vals = resolved_settings(case.model, case.settings)
     if case.model == "d3q19":
         L.set_params([vals[n] for n in D3Q19_PARAMS])
+    elif case.model == "d3q19_lid":
+        L.set_params([vals[n] for n in D3Q19_LID_PARAMS])
     else:
         L.set_params([vals["omega"], vals["Density"]])
     zshift = m.group("SETTINGZONE")[1]

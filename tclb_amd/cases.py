@@ -197,6 +197,28 @@ def d3q19_porous(n=256, fraction=0.10, rmin=4, rmax=12, force=1e-6, tau=0.6, se=
     return Case("d3q19_porous", "d3q19", n, n, n, flags, settings=settings, steps=steps)
 
 
+def d3q19_cavity(n=640, u_lid=0.05, tau=0.52, steps=20000, ny=None, nz=None):
+    """d3q19_lid lid-driven cavity: the top face y = ny-1 is a MovingWall
+    (mask ALL) moving at (u_lid, 0, 0); the other five faces are Wall (mask
+    ALL) bounce-back; MRT inside with omega = 1/tau and the reference's
+    rates; rho = 1 and u = 0 initially.  The lid covers the whole top face,
+    its edges included."""
+    ny = n if ny is None else ny
+    nz = n if nz is None else nz
+    m = Model("d3q19_lid")
+    MRT = m.nodetype("MRT")[0]
+    WALL, LID = m.nodetype("Wall")[0], m.nodetype("MovingWall")[0]
+    flags = np.full((nz, ny, n), MRT, dtype=np.uint16)
+    flags[:, :, 0] = WALL
+    flags[:, :, n - 1] = WALL
+    flags[:, 0, :] = WALL
+    flags[0, :, :] = WALL
+    flags[nz - 1, :, :] = WALL
+    flags[:, ny - 1, :] = LID
+    settings = [("omega", 1.0 / tau), ("MovingWallVelocityX", u_lid)]
+    return Case(f"d3q19_cavity_{n}x{ny}x{nz}", "d3q19_lid", n, ny, nz, flags, settings=settings, steps=steps)
+
+
 def d2q9_cylinder_sweep(ncases=256, seed=4, steps=8000, umax=0.02):
     """Independent cylinder cases of uneven size: nx uniform in
     {128, 144, ..., 1024}, ny uniform in {64, 80, ..., 256}, a Solid disc of

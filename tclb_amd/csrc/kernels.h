@@ -9,8 +9,9 @@ namespace tclbx {
 // Model parameters passed by value in every launch (no per-step constant
 // upload).  d2q9: [omega, Density].  d3q19: [omega, InletVelocity,
 // InletDensity, ForceX, ForceY, ForceZ, then the relaxation factors of the
-// e, eps, q, pi and m moment groups, then gamma1 = 1 - omega (nu group)].
-constexpr int MAX_PAR = 12;
+// e, eps, q, pi and m moment groups, then gamma1 = 1 - omega (nu group),
+// then (d3q19_lid) MovingWallVelocityX/Y/Z].
+constexpr int MAX_PAR = 16;
 constexpr int MAX_GLOBALS = 32;
 
 struct StepArgs {

@@ -29,18 +29,20 @@ namespace tclbx {
 
 // ------------------------------------------------------------ lattices ---
 template <int MODEL> struct Lat;
+// Q densities, G Globals of which the first SUMG are sums (the rest MAX),
+// zone shift of the NodeType flag
 template <> struct Lat<MODEL_D2Q9_SRT> {
-    static constexpr int Q = 9, G = 1, ZSHIFT = 4;
+    static constexpr int Q = 9, G = 1, SUMG = 1, ZSHIFT = 4;
     __host__ __device__ static constexpr int ex(int i) { return i == 1 || i == 5 || i == 8 ? 1 : (i == 3 || i == 6 || i == 7 ? -1 : 0); }
     __host__ __device__ static constexpr int ey(int i) { return i == 2 || i == 5 || i == 6 ? 1 : (i == 4 || i == 7 || i == 8 ? -1 : 0); }
     __host__ __device__ static constexpr int ez(int) { return 0; }
     __host__ __device__ static constexpr double w(int i) { return i == 0 ? 4. / 9 : (i < 5 ? 1. / 9 : 1. / 36); }
 };
 template <> struct Lat<MODEL_D2Q9_SRT_DRAG> : Lat<MODEL_D2Q9_SRT> {
-    static constexpr int G = 3;
+    static constexpr int G = 3, SUMG = 3;
 };
 template <> struct Lat<MODEL_D3Q19> {
-    static constexpr int Q = 19, G = 26, ZSHIFT = 6;
+    static constexpr int Q = 19, G = 26, SUMG = 25, ZSHIFT = 6;
     __host__ __device__ static constexpr int ex(int i) {
         return (i == 1 || i == 7 || i == 9 || i == 11 || i == 13) ? 1
              : (i == 2 || i == 8 || i == 10 || i == 12 || i == 14) ? -1 : 0;
@@ -55,12 +57,19 @@ template <> struct Lat<MODEL_D3Q19> {
     }
     __host__ __device__ static constexpr double w(int i) { return i == 0 ? 1. / 3 : (i < 7 ? 1. / 18 : 1. / 36); }
 };
+// d3q19 + MovingWall: Globals MovingWallFx/Fy/Fz/Torque (24..27) before
+// Objective (28) and MaxV (29)
+template <> struct Lat<MODEL_D3Q19_LID> : Lat<MODEL_D3Q19> {
+    static constexpr int G = 30, SUMG = 29;
+};
+template <int MODEL> constexpr bool is_d3q19 = MODEL == MODEL_D3Q19 || MODEL == MODEL_D3Q19_LID;
 
 int kernel_nglobals(int model) {
     switch (model) {
     case MODEL_D2Q9_SRT: return Lat<MODEL_D2Q9_SRT>::G;
     case MODEL_D2Q9_SRT_DRAG: return Lat<MODEL_D2Q9_SRT_DRAG>::G;
     case MODEL_D3Q19: return Lat<MODEL_D3Q19>::G;
+    case MODEL_D3Q19_LID: return Lat<MODEL_D3Q19_LID>::G;
     }
     return 0;
 }
@@ -262,20 +271,44 @@ __device__ __forceinline__ void d3q19_feq(R* f, R rho, R Jx, R Jy, R Jz) {
     d3q19_from_moments(f, q);
 }
 
-template <bool GLOB>
+// Global i += x; Objective (index OBJ) += the zone's <Global>InObj weight
+// (src/cuda.cu.Rt:114-119)
+template <bool GLOB, int OBJ>
 __device__ __forceinline__ void gadd(double* g, int i, double x, const double* zr) {
-    if (GLOB) { g[i] += x; g[24] += zr[i]; }
+    if (GLOB) { g[i] += x; g[OBJ] += zr[i]; }
 }
 
 // XY/XZ/YZ slice Globals vx, vy, vz, rho, area starting at index BASE
-template <bool GLOB, int BASE>
+template <bool GLOB, int OBJ, int BASE>
 __device__ __forceinline__ void slice_add(double* g, double ux, double uy, double uz, double rs,
                                           const double* zr) {
-    gadd<GLOB>(g, BASE + 0, ux, zr);
-    gadd<GLOB>(g, BASE + 1, uy, zr);
-    gadd<GLOB>(g, BASE + 2, uz, zr);
-    gadd<GLOB>(g, BASE + 3, rs, zr);
-    gadd<GLOB>(g, BASE + 4, 1.0, zr);
+    gadd<GLOB, OBJ>(g, BASE + 0, ux, zr);
+    gadd<GLOB, OBJ>(g, BASE + 1, uy, zr);
+    gadd<GLOB, OBJ>(g, BASE + 2, uz, zr);
+    gadd<GLOB, OBJ>(g, BASE + 3, rs, zr);
+    gadd<GLOB, OBJ>(g, BASE + 4, 1.0, zr);
+}
+
+// the volume and slice Globals of one collided node (Dynamics.c.Rt:266-289)
+template <bool GLOB, bool LID, class R>
+__device__ __forceinline__ void d3q19_globals(double* g, unsigned nt, R rs, R jx, R jy, R jz, const double* zr) {
+    constexpr int OBJ = LID ? 28 : 24, MAXV = LID ? 29 : 25;
+    const R ux = jx / rs, uy = jy / rs, uz = jz / rs;
+    switch (nt & 0x3u) {  // slice Globals, register indices fixed at compile time
+    case 1: slice_add<GLOB, OBJ, 1>(g, ux, uy, uz, rs, zr); break;
+    case 2: slice_add<GLOB, OBJ, 6>(g, ux, uy, uz, rs, zr); break;
+    case 3: slice_add<GLOB, OBJ, 11>(g, ux, uy, uz, rs, zr); break;
+    }
+    gadd<GLOB, OBJ>(g, 16, ux, zr);
+    gadd<GLOB, OBJ>(g, 17, uy, zr);
+    gadd<GLOB, OBJ>(g, 18, uz, zr);
+    gadd<GLOB, OBJ>(g, 19, jx, zr);
+    gadd<GLOB, OBJ>(g, 20, jy, zr);
+    gadd<GLOB, OBJ>(g, 21, jz, zr);
+    gadd<GLOB, OBJ>(g, 22, rs, zr);
+    gadd<GLOB, OBJ>(g, 23, 1.0, zr);
+    const double v = (double)sqrt(ux*ux + uy*uy + uz*uz);
+    g[MAXV] = fmax(g[MAXV], v);
 }
 
 // CollisionMRT, Dynamics.c.Rt:223-290.  Rates by moment group (0-based):
@@ -283,7 +316,7 @@ __device__ __forceinline__ void slice_add(double* g, double ux, double uy, doubl
 // the reference uses gamma1 = 1-omega for e/eps/nu/pi and
 // gamma2 = 1-8(2-omega)/(8-omega) for q/m; a positive MRT_S* setting
 // replaces the rate of its group by 1 - S.
-template <class R, bool GLOB>
+template <class R, bool GLOB, bool LID>
 __device__ __forceinline__ void d3q19_mrt(R* f, unsigned nt, const double* par, const double* zr,
                                           double* g) {
     // relaxation factors of the moment groups, computed on the host
@@ -354,27 +387,9 @@ __device__ __forceinline__ void d3q19_mrt(R* f, unsigned nt, const double* par, 
     for (int k = 0; k < 19; k++) m[k] = m[k] * R(iv[k]);
     d3q19_from_moments(f, m);
 
-    if (GLOB) {
-        // Dynamics.c.Rt:266-289 evaluates these after "R /= Mw", i.e. on
-        // rho/19 and J/10 -- kept as the reference computes them
-        const R rs = m[0], jx = m[3], jy = m[5], jz = m[7];
-        const R ux = jx / rs, uy = jy / rs, uz = jz / rs;
-        switch (nt & 0x3u) {  // slice Globals, register indices fixed at compile time
-        case 1: slice_add<GLOB, 1>(g, ux, uy, uz, rs, zr); break;
-        case 2: slice_add<GLOB, 6>(g, ux, uy, uz, rs, zr); break;
-        case 3: slice_add<GLOB, 11>(g, ux, uy, uz, rs, zr); break;
-        }
-        gadd<GLOB>(g, 16, ux, zr);
-        gadd<GLOB>(g, 17, uy, zr);
-        gadd<GLOB>(g, 18, uz, zr);
-        gadd<GLOB>(g, 19, jx, zr);
-        gadd<GLOB>(g, 20, jy, zr);
-        gadd<GLOB>(g, 21, jz, zr);
-        gadd<GLOB>(g, 22, rs, zr);
-        gadd<GLOB>(g, 23, 1.0, zr);
-        const double v = (double)sqrt(ux*ux + uy*uy + uz*uz);
-        g[25] = fmax(g[25], v);
-    }
+    // Dynamics.c.Rt:266-289 evaluates the Globals after "R /= Mw", i.e. on
+    // rho/19 and J/10 -- kept as the reference computes them
+    if (GLOB) d3q19_globals<GLOB, LID>(g, nt, m[0], m[3], m[5], m[7], zr);
 }
 
 // The d3q19 moment matrix M[k][i] (moment k of direction i), the listed
@@ -405,7 +420,7 @@ __device__ constexpr int kMw[19] = {19, 2394, 252, 10, 40, 10, 40, 10, 40, 36, 7
 // R = f M over the nonzero entries of M in ascending order, relaxation,
 // forcing, R /= Mw by division, f = R M^T.  Used for FP64 storage, where it
 // gives the same roundings as the CPU restatement of the reference.
-template <class R, bool GLOB>
+template <class R, bool GLOB, bool LID>
 __device__ __forceinline__ void d3q19_mrt_dense(R* f, unsigned nt, const double* par, const double* zr,
                                                 double* g) {
     // relaxation factors computed on the host in double with the same
@@ -457,25 +472,7 @@ __device__ __forceinline__ void d3q19_mrt_dense(R* f, unsigned nt, const double*
             if (kM[k][i] != 0) acc += m[k] * R(kM[k][i]);
         f[i] = acc;
     }
-    if (GLOB) {
-        const R rs = m[0], jx = m[3], jy = m[5], jz = m[7];
-        const R ux = jx / rs, uy = jy / rs, uz = jz / rs;
-        switch (nt & 0x3u) {  // slice Globals, register indices fixed at compile time
-        case 1: slice_add<GLOB, 1>(g, ux, uy, uz, rs, zr); break;
-        case 2: slice_add<GLOB, 6>(g, ux, uy, uz, rs, zr); break;
-        case 3: slice_add<GLOB, 11>(g, ux, uy, uz, rs, zr); break;
-        }
-        gadd<GLOB>(g, 16, ux, zr);
-        gadd<GLOB>(g, 17, uy, zr);
-        gadd<GLOB>(g, 18, uz, zr);
-        gadd<GLOB>(g, 19, jx, zr);
-        gadd<GLOB>(g, 20, jy, zr);
-        gadd<GLOB>(g, 21, jz, zr);
-        gadd<GLOB>(g, 22, rs, zr);
-        gadd<GLOB>(g, 23, 1.0, zr);
-        const double v = (double)sqrt(ux*ux + uy*uy + uz*uz);
-        g[25] = fmax(g[25], v);
-    }
+    if (GLOB) d3q19_globals<GLOB, LID>(g, nt, m[0], m[3], m[5], m[7], zr);
 }
 
 // Run(), Dynamics.c.Rt:181-210, with the Zou-He conditions of :98-179.
@@ -501,9 +498,77 @@ __device__ __forceinline__ R sum_x0(const R* f) {
     return f[0] + f[3] + f[4] + f[5] + f[6] + f[15] + f[16] + f[17] + f[18];
 }
 
+// Bounce-back swap of every opposite pair (Dynamics.c.Rt:54-75)
+template <class R>
+__device__ __forceinline__ void d3q19_bounce(R* f) {
+    R t;
+    t = f[1]; f[1] = f[2]; f[2] = t;
+    t = f[3]; f[3] = f[4]; f[4] = t;
+    t = f[5]; f[5] = f[6]; f[6] = t;
+    t = f[7]; f[7] = f[10]; f[10] = t;
+    t = f[8]; f[8] = f[9]; f[9] = t;
+    t = f[11]; f[11] = f[14]; f[14] = t;
+    t = f[12]; f[12] = f[13]; f[13] = t;
+    t = f[15]; f[15] = f[18]; f[18] = t;
+    t = f[16]; f[16] = f[17]; f[17] = t;
+}
+
+// MovingWall (d3q19_lid only; the reference has no moving wall): full-way
+// bounce-back with Ladd's correction for a wall moving at U (wall density
+// 1): the population leaving in direction j gains 6 w_j (e_j . U), i.e.
+// (e_j . U)/3 on the axes and (e_j . U)/6 on the diagonals.  With GLOB the
+// force of the fluid on the node, momentum exchange over the links whose
+// source node is fluid (not Solid, Wall or MovingWall):
+// F = sum_i e_i (f_in,i + f_out,opp(i)), goes to MovingWallFx/Fy/Fz and its z
+// torque about the axis through ((nx-1)/2, (ny-1)/2) to MovingWallTorque.
+__host__ __device__ constexpr int d3q19_opp(int i) {
+    return i == 0 ? 0 : (i < 7 ? ((i & 1) ? i + 1 : i - 1) : (i < 11 ? 17 - i : (i < 15 ? 25 - i : 33 - i)));
+}
+
 template <class R, bool GLOB>
+__device__ __forceinline__ void d3q19_moving_wall(R* f, const double* par, const double* zr, double* g,
+                                                  const uint16_t* flags, int x, int y, int z, int nx, int ny,
+                                                  int nz) {
+    using L = Lat<MODEL_D3Q19>;
+    R fin[19];
+    if (GLOB) {
+#pragma unroll
+        for (int i = 0; i < 19; i++) fin[i] = f[i];
+    }
+    d3q19_bounce(f);
+    const R Ux = (R)par[12], Uy = (R)par[13], Uz = (R)par[14];
+#pragma unroll
+    for (int j = 1; j < 19; j++) {
+        const R cu = (R)L::ex(j) * Ux + (R)L::ey(j) * Uy + (R)L::ez(j) * Uz;
+        f[j] = f[j] + (j < 7 ? cu / R(3.) : cu / R(6.));
+    }
+    if (GLOB) {
+        double Fx = 0, Fy = 0, Fz = 0;
+#pragma unroll
+        for (int i = 1; i < 19; i++) {
+            int sx = x - L::ex(i), sy = y - L::ey(i), sz = z - L::ez(i);
+            sx = sx < 0 ? sx + nx : (sx >= nx ? sx - nx : sx);
+            sy = sy < 0 ? sy + ny : (sy >= ny ? sy - ny : sy);
+            sz = sz < 0 ? sz + nz : (sz >= nz ? sz - nz : sz);
+            const unsigned b = flags[(unsigned)sx + (unsigned)nx * ((unsigned)sy + (unsigned)ny * (unsigned)sz)] & 0x1Cu;
+            if (b != 0x08u && b != 0x0Cu && b != 0x1Cu) {
+                const double mom = (double)fin[i] + (double)f[d3q19_opp(i)];
+                Fx += mom * L::ex(i); Fy += mom * L::ey(i); Fz += mom * L::ez(i);
+            }
+        }
+        const double tq = (x - 0.5 * (nx - 1)) * Fy - (y - 0.5 * (ny - 1)) * Fx;
+        g[24] += Fx; g[28] += zr[24];
+        g[25] += Fy; g[28] += zr[25];
+        g[26] += Fz; g[28] += zr[26];
+        g[27] += tq; g[28] += zr[27];
+    }
+}
+
+// x, y, z and the flags locate the node for the MovingWall of d3q19_lid
+template <class R, bool GLOB, bool LID>
 __device__ __forceinline__ void d3q19_node(R* f, unsigned nt, const double* par, const double* zr,
-                                           double* g) {
+                                           double* g, const uint16_t* flags, int x, int y, int z, int nx, int ny,
+                                           int nz) {
     const R InletVelocity = (R)par[1], InletDensity = (R)par[2];
     switch (nt & 0x1Cu) {
     case 0x14: {  // WPressureL
@@ -545,23 +610,17 @@ __device__ __forceinline__ void d3q19_node(R* f, unsigned nt, const double* par,
         f[12] = f[13] - rho * ux / R(6.0) - Nxz;
         break; }
     case 0x0C:    // Wall
-    case 0x08: {  // Solid: swap every opposite pair (Dynamics.c.Rt:54-75)
-        R t;
-        t = f[1]; f[1] = f[2]; f[2] = t;
-        t = f[3]; f[3] = f[4]; f[4] = t;
-        t = f[5]; f[5] = f[6]; f[6] = t;
-        t = f[7]; f[7] = f[10]; f[10] = t;
-        t = f[8]; f[8] = f[9]; f[9] = t;
-        t = f[11]; f[11] = f[14]; f[14] = t;
-        t = f[12]; f[12] = f[13]; f[13] = t;
-        t = f[15]; f[15] = f[18]; f[18] = t;
-        t = f[16]; f[16] = f[17]; f[17] = t;
-        break; }
+    case 0x08:    // Solid
+        d3q19_bounce(f);
+        break;
+    case 0x1C:    // MovingWall
+        if constexpr (LID) d3q19_moving_wall<R, GLOB>(f, par, zr, g, flags, x, y, z, nx, ny, nz);
+        break;
     default: break;
     }
     if ((nt & 0x20u) == 0x20u) {
-        if constexpr (sizeof(R) == 8) d3q19_mrt_dense<R, GLOB>(f, nt, par, zr, g);
-        else d3q19_mrt<R, GLOB>(f, nt, par, zr, g);
+        if constexpr (sizeof(R) == 8) d3q19_mrt_dense<R, GLOB, LID>(f, nt, par, zr, g);
+        else d3q19_mrt<R, GLOB, LID>(f, nt, par, zr, g);
     }
 }
 
@@ -671,8 +730,8 @@ __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, 
     }
 
     if (active) {
-        if constexpr (MODEL == MODEL_D3Q19) {
-            d3q19_node<R, GLOB>(f, nt, a.par, zr, g);
+        if constexpr (is_d3q19<MODEL>) {
+            d3q19_node<R, GLOB, MODEL == MODEL_D3Q19_LID>(f, nt, a.par, zr, g, a.flags, xc, y, z, nx, ny, nz);
         } else {
             d2q9_node<R, MODEL == MODEL_D2Q9_SRT_DRAG, GLOB>(f, nt, (R)a.par[0], (R)a.par[1], zr, g);
         }
@@ -720,8 +779,7 @@ __device__ __forceinline__ void step_tile(const StepArgs& a, int tile, int rpb, 
             node_update<MODEL, ST, GLOB>(a, row, x, xm, xp, in, g);
         }
         if (GLOB) {
-            constexpr int SUMG = (MODEL == MODEL_D3Q19) ? 25 : L::G;
-            block_globals<L::G>(g, SUMG, a.gpart, slot, nslots);
+            block_globals<L::G>(g, L::SUMG, a.gpart, slot, nslots);
         }
         return;
     }
@@ -740,8 +798,7 @@ __device__ __forceinline__ void step_tile(const StepArgs& a, int tile, int rpb, 
     for (int row = rowblk * rpb; row < (rowblk + 1) * rpb && row < nrows; row++)
         node_update<MODEL, ST, GLOB>(a, row, xc, xm, xp, xin, g);
     if (GLOB) {
-        constexpr int SUMG = (MODEL == MODEL_D3Q19) ? 25 : L::G;
-        block_globals<L::G>(g, SUMG, a.gpart, slot, nslots);
+        block_globals<L::G>(g, L::SUMG, a.gpart, slot, nslots);
     }
 }
 
@@ -797,7 +854,7 @@ __global__ void __launch_bounds__(256) k_init(StepArgs a) {
     if (zone >= a.nzones) zone = 0;
     const double* zr = a.zone + (size_t)zone * a.zcols;
     R f[Q];
-    if constexpr (MODEL == MODEL_D3Q19) {
+    if constexpr (is_d3q19<MODEL>) {
         d3q19_feq<R>(f, R(1.0), (R)a.par[1], R(0.), R(0.));
     } else {
         d2q9_feq<R>(f, (R)a.par[1], (R)zr[1], (R)zr[2]);
@@ -808,7 +865,8 @@ __global__ void __launch_bounds__(256) k_init(StepArgs a) {
 }
 
 // Quantities on pulled populations (src/LatticeContainer.inc.cpp.Rt:437-473):
-// d2q9: U (J/rho + G/2) = 0, Rho = 1; d3q19: P = 0, U (J/rho, 0 on Wall) = 1.
+// d2q9: U (J/rho + G/2) = 0, Rho = 1; d3q19: P = 0, U (J/rho, 0 on Wall) = 1;
+// d3q19_lid reports the wall velocity on MovingWall nodes.
 template <int MODEL, int ST>
 __global__ void __launch_bounds__(256) k_quantity(StepArgs a, int qid, int x0, int y0, int z0, int rx,
                                                   int ry, int rz, double scale, double* out) {
@@ -836,7 +894,7 @@ __global__ void __launch_bounds__(256) k_quantity(StepArgs a, int qid, int x0, i
         const unsigned src = (unsigned)sx + (unsigned)nx * ((unsigned)sy + (unsigned)ny * (unsigned)sz);
         f[i] = (double)S::get(fin[(size_t)i * a.pitch + src], (typename S::R)L::w(i));
     }
-    if constexpr (MODEL == MODEL_D3Q19) {
+    if constexpr (is_d3q19<MODEL>) {
         double rho = 0, jx = 0, jy = 0, jz = 0;
 #pragma unroll
         for (int i = 0; i < Q; i++) {
@@ -847,6 +905,7 @@ __global__ void __launch_bounds__(256) k_quantity(StepArgs a, int qid, int x0, i
         } else {
             double ux = jx / rho, uy = jy / rho, uz = jz / rho;
             if ((nt & 0x1Cu) == 0x0Cu) ux = uy = uz = 0;
+            if (MODEL == MODEL_D3Q19_LID && (nt & 0x1Cu) == 0x1Cu) { ux = a.par[12]; uy = a.par[13]; uz = a.par[14]; }
             out[3 * tid] = ux * scale; out[3 * tid + 1] = uy * scale; out[3 * tid + 2] = uz * scale;
         }
     } else {
@@ -1001,7 +1060,7 @@ hipError_t quantity_m(int st, int q, const StepArgs& a, int x0, int y0, int z0, 
 }
 
 double weight(int model, int q) {
-    return model == MODEL_D3Q19 ? Lat<MODEL_D3Q19>::w(q) : Lat<MODEL_D2Q9_SRT>::w(q);
+    return (model == MODEL_D3Q19 || model == MODEL_D3Q19_LID) ? Lat<MODEL_D3Q19>::w(q) : Lat<MODEL_D2Q9_SRT>::w(q);
 }
 }  // namespace
 
@@ -1020,6 +1079,7 @@ hipError_t launch_step(int model, int storage, bool globals, const StepArgs& a, 
     case MODEL_D2Q9_SRT: return step_m<MODEL_D2Q9_SRT>(storage, globals, a, block, s);
     case MODEL_D2Q9_SRT_DRAG: return step_m<MODEL_D2Q9_SRT_DRAG>(storage, globals, a, block, s);
     case MODEL_D3Q19: return step_m<MODEL_D3Q19>(storage, globals, a, block, s);
+    case MODEL_D3Q19_LID: return step_m<MODEL_D3Q19_LID>(storage, globals, a, block, s);
     }
     return hipErrorInvalidValue;
 }
@@ -1029,6 +1089,7 @@ hipError_t launch_init(int model, int storage, const StepArgs& a, int block, hip
     case MODEL_D2Q9_SRT: return init_m<MODEL_D2Q9_SRT>(storage, a, block, s);
     case MODEL_D2Q9_SRT_DRAG: return init_m<MODEL_D2Q9_SRT_DRAG>(storage, a, block, s);
     case MODEL_D3Q19: return init_m<MODEL_D3Q19>(storage, a, block, s);
+    case MODEL_D3Q19_LID: return init_m<MODEL_D3Q19_LID>(storage, a, block, s);
     }
     return hipErrorInvalidValue;
 }
@@ -1082,6 +1143,8 @@ hipError_t launch_batch_step(int model, int storage, bool globals, const BatchCa
         return batch_m<MODEL_D2Q9_SRT_DRAG>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
     case MODEL_D3Q19: return batch_m<MODEL_D3Q19>(storage, globals, cases, tiles, ntiles, phase, gpart, grid,
                                                   block, s);
+    case MODEL_D3Q19_LID:
+        return batch_m<MODEL_D3Q19_LID>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
     }
     return hipErrorInvalidValue;
 }
@@ -1093,6 +1156,8 @@ hipError_t launch_quantity(int model, int storage, int q, const StepArgs& a, int
     case MODEL_D2Q9_SRT_DRAG:
         return quantity_m<MODEL_D2Q9_SRT_DRAG>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
     case MODEL_D3Q19: return quantity_m<MODEL_D3Q19>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
+    case MODEL_D3Q19_LID:
+        return quantity_m<MODEL_D3Q19_LID>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
     }
     return hipErrorInvalidValue;
 }

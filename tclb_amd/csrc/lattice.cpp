@@ -200,7 +200,7 @@ StepArgs Lattice::args(int from, int to, bool glob) const {
 void model_params(const ModelDef* model, const std::vector<double>& settings, double* par) {
     auto S = [&](const char* n) { int i = model->setting_id(n); return i >= 0 ? settings[i] : 0.0; };
     for (int i = 0; i < MAX_PAR; i++) par[i] = 0.0;
-    if (model->id == MODEL_D3Q19) {
+    if (model->id == MODEL_D3Q19 || model->id == MODEL_D3Q19_LID) {
         const char* names[] = {"omega", "InletVelocity", "InletDensity", "ForceX", "ForceY", "ForceZ"};
         for (int i = 0; i < 6; i++) par[i] = S(names[i]);
         // relaxation factors of CollisionMRT (models/flow/d3q19/Dynamics.c.Rt:
@@ -217,6 +217,9 @@ void model_params(const ModelDef* model, const std::vector<double>& settings, do
         par[9] = rate("MRT_Spi", gamma1);
         par[10] = rate("MRT_Sm", gamma2);
         par[11] = gamma1;
+        par[12] = S("MovingWallVelocityX");
+        par[13] = S("MovingWallVelocityY");
+        par[14] = S("MovingWallVelocityZ");
     } else {
         par[0] = S("omega");
         par[1] = S("Density");

@@ -123,10 +123,17 @@ ModelDef make_d2q9(bool drag) {
     return m;
 }
 
-ModelDef make_d3q19() {
+// d3q19, and with `lid` the model d3q19_lid: d3q19 plus a MovingWall node
+// type (bounce-back with Ladd's wall-velocity correction), the wall velocity
+// settings MovingWallVelocityX/Y/Z and the Globals MovingWallFx/Fy/Fz (force
+// of the fluid on the MovingWall nodes, momentum exchange) and
+// MovingWallTorque (z torque of that force about the axis through the
+// lattice centre).  The reference has no moving wall; d3q19 itself keeps the
+// reference's settings, Globals and NodeType bits unchanged.
+ModelDef make_d3q19(bool lid) {
     ModelDef m;
-    m.id = MODEL_D3Q19;
-    m.name = "d3q19";
+    m.id = lid ? MODEL_D3Q19_LID : MODEL_D3Q19;
+    m.name = lid ? "d3q19_lid" : "d3q19";
     m.Q = 19; m.dim = 3;
     // columns 4/6/8 of MRTMAT, models/flow/d3q19/Dynamics.R:1-35
     m.ex = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
@@ -143,24 +150,33 @@ ModelDef make_d3q19() {
                   {"ForceY", 0.0, -1, 0},        {"ForceZ", 0.0, -1, 0},
                   {"MRT_Se", 0.0, -1, 0},        {"MRT_Seps", 0.0, -1, 0},
                   {"MRT_Sq", 0.0, -1, 0},        {"MRT_Spi", 0.0, -1, 0},
-                  {"MRT_Sm", 0.0, -1, 0},        {"Threshold", 0.5, -1, 0}};
+                  {"MRT_Sm", 0.0, -1, 0}};
+    if (lid)
+        for (const char* c : {"X", "Y", "Z"}) m.settings.push_back({std::string("MovingWallVelocity") + c, 0.0, -1, 0});
+    m.settings.push_back({"Threshold", 0.5, -1, 0});
     std::vector<GlobalDef> g = {{"Flux", false}};
     for (const char* p : {"XY", "XZ", "YZ"})
         for (const char* s : {"vx", "vy", "vz", "rho", "area"}) g.push_back({std::string(p) + s, false});
     for (const char* s : {"vx", "vy", "vz", "px", "py", "pz", "rho", "volume"})
         g.push_back({std::string("VOL") + s, false});
+    if (lid)
+        for (const char* s : {"Fx", "Fy", "Fz", "Torque"}) g.push_back({std::string("MovingWall") + s, false});
     g.push_back({"MaxV", true});
     finish_globals(m, g);
     m.quantities = {{"P", false}, {"U", true}};  // Dynamics.R:37-38
-    pack_nodetypes(m, {{"XYslice", "ADDITIONALS"}, {"XZslice", "ADDITIONALS"}, {"YZslice", "ADDITIONALS"},
-                       {"EPressure", "BOUNDARY"}, {"Solid", "BOUNDARY"}, {"Wall", "BOUNDARY"},
-                       {"WPressure", "BOUNDARY"}, {"WPressureL", "BOUNDARY"}, {"WVelocity", "BOUNDARY"},
-                       {"MRT", "COLLISION"}});
+    std::vector<std::pair<std::string, std::string>> types = {
+        {"XYslice", "ADDITIONALS"}, {"XZslice", "ADDITIONALS"}, {"YZslice", "ADDITIONALS"},
+        {"EPressure", "BOUNDARY"},  {"Solid", "BOUNDARY"},      {"Wall", "BOUNDARY"},
+        {"WPressure", "BOUNDARY"},  {"WPressureL", "BOUNDARY"}, {"WVelocity", "BOUNDARY"}};
+    // the seventh BOUNDARY type still fits the group's 3 bits (value 0x1C)
+    if (lid) types.push_back({"MovingWall", "BOUNDARY"});
+    types.push_back({"MRT", "COLLISION"});
+    pack_nodetypes(m, types);
     return m;
 }
 
 const std::vector<ModelDef>& registry() {
-    static std::vector<ModelDef> r = {make_d2q9(false), make_d2q9(true), make_d3q19()};
+    static std::vector<ModelDef> r = {make_d2q9(false), make_d2q9(true), make_d3q19(false), make_d3q19(true)};
     return r;
 }
 

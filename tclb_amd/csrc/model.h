@@ -18,7 +18,13 @@
 
 namespace tclbx {
 
-enum ModelId { MODEL_D2Q9_SRT = 0, MODEL_D2Q9_SRT_DRAG = 1, MODEL_D3Q19 = 2, MODEL_COUNT = 3 };
+enum ModelId {
+    MODEL_D2Q9_SRT = 0,
+    MODEL_D2Q9_SRT_DRAG = 1,
+    MODEL_D3Q19 = 2,
+    MODEL_D3Q19_LID = 3,
+    MODEL_COUNT = 4
+};
 
 // storage format of the distributions in HBM
 enum Storage { ST_F64 = 0, ST_F32 = 1, ST_F32S = 2, ST_F16S = 3, ST_BF16S = 4, ST_COUNT = 5 };

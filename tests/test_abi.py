@@ -28,7 +28,7 @@ def test_every_declared_symbol_is_exported():
 
 
 def test_models():
-    assert tclb_amd.models() == ["d2q9_SRT", "d2q9_SRT_drag", "d3q19"]
+    assert tclb_amd.models() == ["d2q9_SRT", "d2q9_SRT_drag", "d3q19", "d3q19_lid"]
 
 
 def test_d2q9_srt_contract():
@@ -64,6 +64,24 @@ def test_d3q19_contract():
     assert len(m.zonal_settings) == 24 and m.zonal_settings[0] == "FluxInObj"
     assert m.quantities == ["P", "U"]
     assert m.quantity_id("U") == (1, True)
+
+
+def test_lid_model_extends_d3q19():
+    """d3q19_lid keeps every d3q19 bit, setting and Global at the same
+    place, adds MovingWall as the seventh BOUNDARY type (0x1C) and the wall
+    force/torque Globals before Objective."""
+    a, b = tclb_amd.Model("d3q19"), tclb_amd.Model("d3q19_lid")
+    for n in ("XYslice", "EPressure", "Solid", "Wall", "WPressure", "WPressureL", "WVelocity", "MRT"):
+        assert a.nodetype(n) == b.nodetype(n)
+    assert b.nodetype("MovingWall") == (0x1C, 0x1C)
+    assert a.group("SETTINGZONE") == b.group("SETTINGZONE")
+    assert b.settings[:13] == a.settings[:13]
+    assert b.settings[13:] == ["MovingWallVelocityX", "MovingWallVelocityY", "MovingWallVelocityZ", "Threshold"]
+    ga, gb = a.global_names, b.global_names
+    assert gb[:24] == ga[:24]
+    assert gb[24:] == ["MovingWallFx", "MovingWallFy", "MovingWallFz", "MovingWallTorque", "Objective", "MaxV"]
+    assert b.zonal_settings[24:] == ["MovingWallFxInObj", "MovingWallFyInObj", "MovingWallFzInObj",
+                                     "MovingWallTorqueInObj"]
 
 
 def test_drag_model_globals():
