@@ -35,7 +35,7 @@ typedef struct tclbx_lattice tclbx_lattice;
 const char* tclbx_last_error(void);
 int tclbx_version(void);
 
-/* Models compiled in: "d2q9_SRT", "d2q9_SRT_drag", "d3q19".
+/* Models compiled in: "d2q9_SRT", "d2q9_SRT_drag", "d3q19", "d3q19_lid".
  * Replaces the per-model binaries CLB/<model>/main (src/makefile.main.Rt:331-349). */
 int tclbx_model_count(void);
 const char* tclbx_model_name(int i);
@@ -209,6 +209,43 @@ int tclbx_server_set_tuning(tclbx_server* s, int flat, int block, int rows_per_t
  * to 16) between two device buffers by a plain vector-copy kernel;
  * *gbps = 2*bytes*reps / time (read + write), *ms = time of one copy. */
 int tclbx_copy_bandwidth(long long bytes, int reps, double* gbps, double* ms);
+
+/* ---------------------------------------------------------- case files ---
+ * A TCLB case file (<CLBConfig>) run on the engine: the work of the
+ * reference executable CLB/<model>/main <case.xml> (src/main.cpp:171-420)
+ * and of its handlers (src/Handlers/: Units, Geometry, Model, Param, Init,
+ * Solve, Log, Stop, Failcheck, VTK, SaveBinary, LoadBinary).  Outputs keep
+ * the reference's names, <output><case>_<Name>_P00_<iteration>.<ext>, and
+ * formats (Log CSV of src/Solver.cpp.Rt:120-206, VTI of src/vtkOutput.cpp). */
+typedef struct tclbx_case tclbx_case;
+
+/* Reads the case (file path, or XML text with `name` standing for the file
+ * name), solves the <Units> gauges (src/main.cpp:31-62) and sizes the
+ * lattice from <Geometry nx ny nz> (src/main.cpp:370-374).  Host only. */
+int tclbx_case_load(const char* model, const char* storage, const char* path, tclbx_case** out);
+int tclbx_case_load_string(const char* model, const char* storage, const char* xml, const char* name,
+                           tclbx_case** out);
+int tclbx_case_destroy(tclbx_case* c);
+int tclbx_case_size(const tclbx_case* c, int* nx, int* ny, int* nz);
+/* UnitEnv::alt (src/unit.h:167-195): text with units -> lattice units. */
+int tclbx_case_units(const tclbx_case* c, const char* text, double* value);
+/* Geometry::load of every <Geometry> element (src/Geometry.cpp.Rt:1140-1184)
+ * into flags[n], n = nx*ny*nz, x fastest.  Host only. */
+int tclbx_case_rasterize(tclbx_case* c, uint16_t* flags, long long n);
+/* Setting zones (Geometry::SettingZones): count, and the name of zone k. */
+int tclbx_case_zone_count(const tclbx_case* c);
+const char* tclbx_case_zone_name(const tclbx_case* c, int k);
+/* Directory prepended to every output path. */
+int tclbx_case_set_output_root(tclbx_case* c, const char* dir);
+/* Creates the lattice on the GPU and runs every element of <CLBConfig>. */
+int tclbx_case_run(tclbx_case* c);
+/* The case's lattice after tclbx_case_run, for reading Globals, quantities
+ * and flags; it belongs to the case (tclbx_lattice_destroy refuses it). */
+int tclbx_case_lattice(tclbx_case* c, tclbx_lattice** out);
+int tclbx_case_iteration(const tclbx_case* c);
+/* Notices, warnings and errors the reference would print, in order. */
+int tclbx_case_message_count(const tclbx_case* c);
+const char* tclbx_case_message(const tclbx_case* c, int i);
 
 #ifdef __cplusplus
 }

@@ -41,6 +41,10 @@ SYMBOLS = [
     "tclbx_batch_info", "tclbx_batch_set_graph_steps",
     "tclbx_server_create", "tclbx_server_destroy", "tclbx_server_submit", "tclbx_server_poll",
     "tclbx_server_wait_all", "tclbx_server_stats", "tclbx_server_set_tuning",
+    "tclbx_case_load", "tclbx_case_load_string", "tclbx_case_destroy", "tclbx_case_size", "tclbx_case_units",
+    "tclbx_case_rasterize", "tclbx_case_zone_count", "tclbx_case_zone_name", "tclbx_case_set_output_root",
+    "tclbx_case_run", "tclbx_case_lattice", "tclbx_case_iteration", "tclbx_case_message_count",
+    "tclbx_case_message",
 ]
 
 
@@ -126,6 +130,22 @@ def lib():
     L.tclbx_server_wait_all.argtypes = [vp, c_double, ip]
     L.tclbx_server_stats.argtypes = [vp, llp, llp, llp, dp, ip, ip, dp]
     L.tclbx_server_set_tuning.argtypes = [vp, c_int, c_int, c_int, c_int]
+    L.tclbx_case_load.argtypes = [c_char_p, c_char_p, c_char_p, ctypes.POINTER(vp)]
+    L.tclbx_case_load_string.argtypes = [c_char_p, c_char_p, c_char_p, c_char_p, ctypes.POINTER(vp)]
+    L.tclbx_case_destroy.argtypes = [vp]
+    L.tclbx_case_size.argtypes = [vp, ip, ip, ip]
+    L.tclbx_case_units.argtypes = [vp, c_char_p, dp]
+    L.tclbx_case_rasterize.argtypes = [vp, u16p, ll]
+    L.tclbx_case_zone_count.argtypes = [vp]
+    L.tclbx_case_zone_name.argtypes = [vp, c_int]
+    L.tclbx_case_zone_name.restype = c_char_p
+    L.tclbx_case_set_output_root.argtypes = [vp, c_char_p]
+    L.tclbx_case_run.argtypes = [vp]
+    L.tclbx_case_lattice.argtypes = [vp, ctypes.POINTER(vp)]
+    L.tclbx_case_iteration.argtypes = [vp]
+    L.tclbx_case_message_count.argtypes = [vp]
+    L.tclbx_case_message.argtypes = [vp, c_int]
+    L.tclbx_case_message.restype = c_char_p
     _lib = L
     return L
 
@@ -212,10 +232,15 @@ class Lattice:
     """One lattice on the GPU (one subdomain, periodic unless NodeTypes say
     otherwise), mirroring the reference Lattice (src/Lattice.h.Rt:39-191)."""
 
-    def __init__(self, model, nx, ny, nz=1, storage="f64"):
+    def __init__(self, model, nx, ny, nz=1, storage="f64", _handle=None):
         L = lib()
-        h = ctypes.c_void_p()
-        _check(L.tclbx_lattice_create(model.encode(), nx, ny, nz, storage.encode(), ctypes.byref(h)))
+        if _handle is None:
+            h = ctypes.c_void_p()
+            _check(L.tclbx_lattice_create(model.encode(), nx, ny, nz, storage.encode(), ctypes.byref(h)))
+            self._owned = True
+        else:
+            h = _handle  # a case's lattice: the case frees it
+            self._owned = False
         self._h = h
         self.model, self.storage = model, storage
         self.nx, self.ny, self.nz = nx, ny, nz
@@ -227,7 +252,8 @@ class Lattice:
 
     def close(self):
         if getattr(self, "_h", None):
-            lib().tclbx_lattice_destroy(self._h)
+            if self._owned:
+                lib().tclbx_lattice_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -576,3 +602,86 @@ class Server:
 
     def set_tuning(self, block=256, rows_per_tile=1, grid=0, flat=True):
         _check(lib().tclbx_server_set_tuning(self._h, int(bool(flat)), int(block), int(rows_per_tile), int(grid)))
+
+
+class Case:
+    """A TCLB case file run on the engine, as CLB/<model>/main <case.xml>
+    runs it (src/main.cpp:171-420): Units, the Geometry size, then every
+    element of <CLBConfig> as a handler.  `path` is a case file, or pass
+    `xml=` text (with `name` standing for the file name, which names the
+    outputs)."""
+
+    def __init__(self, model, path=None, storage="f64", xml=None, name="case.xml", output_root=None):
+        L = lib()
+        h = ctypes.c_void_p()
+        if xml is not None:
+            _check(L.tclbx_case_load_string(model.encode(), storage.encode(), xml.encode(), name.encode(),
+                                            ctypes.byref(h)))
+        else:
+            _check(L.tclbx_case_load(model.encode(), storage.encode(), os.fspath(path).encode(), ctypes.byref(h)))
+        self._h = h
+        self.model, self.storage = model, storage
+        if output_root is not None:
+            _check(L.tclbx_case_set_output_root(h, os.fspath(output_root).encode()))
+        nx, ny, nz = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(L.tclbx_case_size(h, ctypes.byref(nx), ctypes.byref(ny), ctypes.byref(nz)))
+        self.nx, self.ny, self.nz = nx.value, ny.value, nz.value
+        self._lat = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            if self._lat is not None:
+                self._lat.close()
+            lib().tclbx_case_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def shape(self):
+        return (self.nz, self.ny, self.nx)
+
+    def units(self, text):
+        """Text with units -> lattice units (UnitEnv::alt)."""
+        v = ctypes.c_double()
+        _check(lib().tclbx_case_units(self._h, text.encode(), ctypes.byref(v)))
+        return v.value
+
+    def rasterize(self):
+        """The NodeType map of the case's Geometry, uint16 [nz, ny, nx] (host only)."""
+        out = np.zeros(self.shape, dtype=np.uint16)
+        _check(lib().tclbx_case_rasterize(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)), out.size))
+        return out
+
+    @property
+    def zones(self):
+        """Setting zones, name -> number (DefaultZone = 0)."""
+        L = lib()
+        out = {}
+        for k in range(L.tclbx_case_zone_count(self._h)):
+            n = L.tclbx_case_zone_name(self._h, k)
+            if n is not None:
+                out[n.decode()] = k
+        return out
+
+    def run(self):
+        _check(lib().tclbx_case_run(self._h))
+        return self
+
+    @property
+    def lattice(self):
+        if self._lat is None:
+            h = ctypes.c_void_p()
+            _check(lib().tclbx_case_lattice(self._h, ctypes.byref(h)))
+            self._lat = Lattice(self.model, self.nx, self.ny, self.nz, self.storage, _handle=h)
+        return self._lat
+
+    @property
+    def iteration(self):
+        return lib().tclbx_case_iteration(self._h)
+
+    @property
+    def messages(self):
+        L = lib()
+        return [L.tclbx_case_message(self._h, i).decode() for i in range(L.tclbx_case_message_count(self._h))]

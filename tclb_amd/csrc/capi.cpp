@@ -8,11 +8,18 @@
 #include "lattice.h"
 #include "model.h"
 #include "server.h"
+#include "solver.h"
 
 using namespace tclbx;
 
 struct tclbx_lattice {
     Lattice* lat;
+    bool borrowed = false;  // the lattice of a case: the case frees it
+};
+
+struct tclbx_case {
+    Solver* s;
+    tclbx_lattice view;
 };
 
 struct tclbx_batch {
@@ -78,6 +85,7 @@ int tclbx_lattice_create(const char* model, int nx, int ny, int nz, const char* 
 
 int tclbx_lattice_destroy(tclbx_lattice* h) {
     if (!h) return TCLBX_OK;
+    if (h->borrowed) return fail("this lattice belongs to a case: destroy the case");
     delete h->lat;
     delete h;
     return TCLBX_OK;
@@ -538,6 +546,135 @@ int tclbx_copy_bandwidth(long long bytes, int reps, double* gbps, double* ms) {
         if (ms) *ms = t / reps;
         return TCLBX_OK;
     });
+}
+
+// --------------------------------------------------------- case files ---
+#define CHECK_C(c) \
+    if (!(c) || !(c)->s) return fail("null case handle")
+
+static int case_new(const char* model, const char* storage, tclbx_case** out, Solver*& s) {
+    if (!out) return fail("null output pointer");
+    *out = nullptr;
+    s = new Solver(model ? model : "", storage ? storage : "f64");
+    return TCLBX_OK;
+}
+
+int tclbx_case_load(const char* model, const char* storage, const char* path, tclbx_case** out) {
+    return guard([&]() {
+        Solver* s = nullptr;
+        if (case_new(model, storage, out, s)) return TCLBX_ERR;
+        try {
+            s->load_file(path ? path : "");
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = new tclbx_case{s, {nullptr, true}};
+        return TCLBX_OK;
+    });
+}
+
+int tclbx_case_load_string(const char* model, const char* storage, const char* xml, const char* name,
+                           tclbx_case** out) {
+    return guard([&]() {
+        Solver* s = nullptr;
+        if (case_new(model, storage, out, s)) return TCLBX_ERR;
+        try {
+            s->load_string(xml ? xml : "", name ? name : "case.xml");
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = new tclbx_case{s, {nullptr, true}};
+        return TCLBX_OK;
+    });
+}
+
+int tclbx_case_destroy(tclbx_case* c) {
+    if (!c) return TCLBX_OK;
+    delete c->s;
+    delete c;
+    return TCLBX_OK;
+}
+
+int tclbx_case_size(const tclbx_case* c, int* nx, int* ny, int* nz) {
+    CHECK_C(c);
+    if (nx) *nx = c->s->nx;
+    if (ny) *ny = c->s->ny;
+    if (nz) *nz = c->s->nz;
+    return TCLBX_OK;
+}
+
+int tclbx_case_units(const tclbx_case* c, const char* text, double* value) {
+    CHECK_C(c);
+    return guard([&]() {
+        if (!value) return fail("null output pointer");
+        *value = c->s->units.alt(std::string(text ? text : ""));
+        return TCLBX_OK;
+    });
+}
+
+int tclbx_case_rasterize(tclbx_case* c, uint16_t* flags, long long n) {
+    CHECK_C(c);
+    return guard([&]() {
+        const long long N = (long long)c->s->nx * c->s->ny * c->s->nz;
+        if (!flags || n != N) return fail("flags buffer must hold nx*ny*nz = " + std::to_string(N) + " values");
+        c->s->rasterize();
+        const auto& g = c->s->geometry->flags();
+        std::copy(g.begin(), g.end(), flags);
+        return TCLBX_OK;
+    });
+}
+
+int tclbx_case_zone_count(const tclbx_case* c) {
+    CHECK_C(c);
+    return (int)c->s->geometry->zones().size();
+}
+
+const char* tclbx_case_zone_name(const tclbx_case* c, int k) {
+    if (!c || !c->s) return nullptr;
+    for (auto& z : c->s->geometry->zones())
+        if (z.second == k) return z.first.c_str();
+    return nullptr;
+}
+
+int tclbx_case_set_output_root(tclbx_case* c, const char* dir) {
+    CHECK_C(c);
+    c->s->set_output_root(dir ? dir : "");
+    return TCLBX_OK;
+}
+
+int tclbx_case_run(tclbx_case* c) {
+    CHECK_C(c);
+    return guard([&]() {
+        c->s->run();
+        c->view.lat = c->s->lattice.get();
+        return TCLBX_OK;
+    });
+}
+
+int tclbx_case_lattice(tclbx_case* c, tclbx_lattice** out) {
+    CHECK_C(c);
+    if (!out) return fail("null output pointer");
+    if (!c->s->lattice) return fail("the case has not run");
+    c->view.lat = c->s->lattice.get();
+    *out = &c->view;
+    return TCLBX_OK;
+}
+
+int tclbx_case_iteration(const tclbx_case* c) {
+    CHECK_C(c);
+    return c->s->iter;
+}
+
+int tclbx_case_message_count(const tclbx_case* c) {
+    CHECK_C(c);
+    return (int)c->s->messages.size();
+}
+
+const char* tclbx_case_message(const tclbx_case* c, int i) {
+    if (!c || !c->s || i < 0 || i >= (int)c->s->messages.size()) return nullptr;
+    return c->s->messages[i].c_str();
 }
 
 }  // extern "C"

@@ -116,7 +116,8 @@ ModelDef make_d2q9(bool drag) {
     std::vector<GlobalDef> g;
     if (drag) g = {{"Drag", false}, {"Lift", false}};
     finish_globals(m, g);
-    m.quantities = {{"U", true}, {"Rho", false}};  // Dynamics.R:18-19
+    m.quantities = {{"U", true, "m/s"}, {"Rho", false, "kg/m3"}};  // Dynamics.R:18-19
+    if (drag) m.globals[0].unit = m.globals[1].unit = "N";
     pack_nodetypes(m, {{"EPressure", "BOUNDARY"}, {"WPressure", "BOUNDARY"}, {"WVelocity", "BOUNDARY"},
                        {"EVelocity", "BOUNDARY"}, {"Solid", "BOUNDARY"}, {"Wall", "BOUNDARY"},
                        {"BGK", "COLLISION"}});
@@ -144,26 +145,38 @@ ModelDef make_d3q19(bool lid) {
     for (int i = 1; i <= 6; i++) m.w[i] = 1. / 18;
     // Dynamics.R:40-48, then the free MRT rates of this engine (0 = the
     // reference's fixed rate of that moment group), then Threshold
-    m.settings = {{"omega", 0.0, -1, 0},        {"nu", 0.16666666, 0, 1},
-                  {"InletVelocity", 0.0, -1, 0}, {"InletPressure", 0.0, 4, 2},
-                  {"InletDensity", 1.0, -1, 0},  {"ForceX", 0.0, -1, 0},
-                  {"ForceY", 0.0, -1, 0},        {"ForceZ", 0.0, -1, 0},
-                  {"MRT_Se", 0.0, -1, 0},        {"MRT_Seps", 0.0, -1, 0},
-                  {"MRT_Sq", 0.0, -1, 0},        {"MRT_Spi", 0.0, -1, 0},
-                  {"MRT_Sm", 0.0, -1, 0}};
+    m.settings = {{"omega", 0.0, -1, 0},
+                  {"nu", 0.16666666, 0, 1, "1m2/s"},
+                  {"InletVelocity", 0.0, -1, 0, "1m/s"},
+                  {"InletPressure", 0.0, 4, 2, "1Pa"},
+                  {"InletDensity", 1.0, -1, 0, "1kg/m3"},
+                  {"ForceX", 0.0, -1, 0},
+                  {"ForceY", 0.0, -1, 0},
+                  {"ForceZ", 0.0, -1, 0}};
+    for (const char* r : {"MRT_Se", "MRT_Seps", "MRT_Sq", "MRT_Spi", "MRT_Sm"})
+        m.settings.push_back({r, 0.0, -1, 0, "1", !lid});
     if (lid)
-        for (const char* c : {"X", "Y", "Z"}) m.settings.push_back({std::string("MovingWallVelocity") + c, 0.0, -1, 0});
+        for (const char* c : {"X", "Y", "Z"})
+            m.settings.push_back({std::string("MovingWallVelocity") + c, 0.0, -1, 0, "1m/s"});
     m.settings.push_back({"Threshold", 0.5, -1, 0});
-    std::vector<GlobalDef> g = {{"Flux", false}};
-    for (const char* p : {"XY", "XZ", "YZ"})
-        for (const char* s : {"vx", "vy", "vz", "rho", "area"}) g.push_back({std::string(p) + s, false});
-    for (const char* s : {"vx", "vy", "vz", "px", "py", "pz", "rho", "volume"})
-        g.push_back({std::string("VOL") + s, false});
-    if (lid)
-        for (const char* s : {"Fx", "Fy", "Fz", "Torque"}) g.push_back({std::string("MovingWall") + s, false});
-    g.push_back({"MaxV", true});
+    // Dynamics.R:50-81 (units as declared there)
+    std::vector<GlobalDef> g = {{"Flux", false, "m3/s"}};
+    for (const char* p : {"XY", "XZ", "YZ"}) {
+        for (const char* s : {"vx", "vy", "vz"}) g.push_back({std::string(p) + s, false, "m3/s"});
+        g.push_back({std::string(p) + "rho", false, "kg/m"});
+        g.push_back({std::string(p) + "area", false, "m2"});
+    }
+    for (const char* s : {"vx", "vy", "vz"}) g.push_back({std::string("VOL") + s, false, "m4/s"});
+    for (const char* s : {"px", "py", "pz"}) g.push_back({std::string("VOL") + s, false, "mkg/s"});
+    g.push_back({"VOLrho", false, "kg"});
+    g.push_back({"VOLvolume", false, "m3"});
+    if (lid) {
+        for (const char* s : {"Fx", "Fy", "Fz"}) g.push_back({std::string("MovingWall") + s, false, "N"});
+        g.push_back({"MovingWallTorque", false, "Nm"});
+    }
+    g.push_back({"MaxV", true, "m3"});
     finish_globals(m, g);
-    m.quantities = {{"P", false}, {"U", true}};  // Dynamics.R:37-38
+    m.quantities = {{"P", false, "Pa"}, {"U", true, "m/s"}};  // Dynamics.R:37-38
     std::vector<std::pair<std::string, std::string>> types = {
         {"XYslice", "ADDITIONALS"}, {"XZslice", "ADDITIONALS"}, {"YZslice", "ADDITIONALS"},
         {"EPressure", "BOUNDARY"},  {"Solid", "BOUNDARY"},      {"Wall", "BOUNDARY"},

@@ -34,6 +34,8 @@ struct SettingDef {
     double def;            // default value
     int derived;           // index of the setting this one sets, or -1
     int derive_kind;       // 0 none, 1 omega = 1/(3x+0.5), 2 InletDensity = 1+3x
+    std::string unit = "1";  // unit of the value (Log's _si column divides by it)
+    bool extension = false;  // added by this engine to a reference model (not in its Log)
 };
 
 struct NodeTypeDef {
@@ -50,11 +52,13 @@ struct GroupDef {
 struct GlobalDef {
     std::string name;
     bool is_max;
+    std::string unit = "1";
 };
 
 struct QuantityDef {
     std::string name;
     bool vector;
+    std::string unit = "1";
 };
 
 struct ModelDef {
