@@ -225,6 +225,11 @@ NODETYPES = {
               ("EPressure", "BOUNDARY"), ("Solid", "BOUNDARY"), ("Wall", "BOUNDARY"), ("WPressure", "BOUNDARY"),
               ("WPressureL", "BOUNDARY"), ("WVelocity", "BOUNDARY"), ("MRT", "COLLISION")],
 }
+# models/flow/d2q9/Dynamics.R:62-79
+NODETYPES["d2q9"] = [("EPressure", "BOUNDARY"), ("WPressure", "BOUNDARY"), ("NVelocity", "BOUNDARY"),
+                     ("SVelocity", "BOUNDARY"), ("WVelocity", "BOUNDARY"), ("EVelocity", "BOUNDARY"),
+                     ("NSymmetry", "BOUNDARY"), ("SSymmetry", "BOUNDARY"), ("Inlet", "OBJECTIVE"),
+                     ("Outlet", "OBJECTIVE"), ("Solid", "BOUNDARY"), ("Wall", "BOUNDARY"), ("MRT", "COLLISION")]
 NODETYPES["d2q9_SRT_drag"] = NODETYPES["d2q9_SRT"]
 NODETYPES["d3q19_lid"] = NODETYPES["d3q19"][:9] + [("MovingWall", "BOUNDARY"), ("MRT", "COLLISION")]
 

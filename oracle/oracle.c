@@ -161,7 +161,7 @@ orc_lattice* orc_create(int model, int nx, int ny, int nz, int storage)
             L->w[q] = d3q19_w[q];
         }
     } else {
-        L->Q = 9; L->nglob = (model == ORC_D2Q9_SRT_DRAG) ? 3 : 1;
+        L->Q = 9; L->nglob = model == ORC_D2Q9_SRT_DRAG ? 3 : (model == ORC_D2Q9 ? 4 : 1);
         for (q = 0; q < 9; q++) {
             for (a = 0; a < 3; a++) L->e[q][a] = d2q9_e[q][a];
             L->w[q] = d2q9_w[q];
@@ -208,7 +208,7 @@ void orc_set_flags(orc_lattice* L, const uint16_t* flags)
 }
 
 static int is3d(const orc_lattice* L) { return L->model == ORC_D3Q19 || L->model == ORC_D3Q19_LID; }
-static int zone_shift(const orc_lattice* L) { return is3d(L) ? 6 : 4; }
+static int zone_shift(const orc_lattice* L) { return is3d(L) ? 6 : (L->model == ORC_D2Q9 ? 7 : 4); }
 
 static const double* zone_row(const orc_lattice* L, uint16_t nt)
 {
@@ -233,6 +233,21 @@ void orc_init(orc_lattice* L)
                 float f[19];
                 d3q19_feq_f(f, 1.0f, (float)L->params[1], 0.f, 0.f);
                 for (q = 0; q < 19; q++) st(L, L->buf[0], (size_t)q * L->N + n, q, f[q]);
+            }
+        } else if (L->model == ORC_D2Q9) {
+            /* Init(), models/flow/d2q9/Dynamics.c.Rt:89-102: rho = 1 + 3 Pressure,
+             * u = (VelocityX, VelocityY); Wall nodes at rest with rho = 1 */
+            int wall = (L->flags[n] & 0xF) == 10;
+            if (L->storage == ORC_F64) {
+                double f[9], rho = 1 + zr[2] * 3, ux = zr[0], uy = zr[1];
+                if (wall) { rho = 1; ux = 0; uy = 0; }
+                d2q9m_feq_d(f, rho, ux * rho, uy * rho);
+                for (q = 0; q < 9; q++) st(L, L->buf[0], (size_t)q * L->N + n, q, f[q]);
+            } else {
+                float f[9], rho = 1 + (float)zr[2] * 3, ux = (float)zr[0], uy = (float)zr[1];
+                if (wall) { rho = 1; ux = 0; uy = 0; }
+                d2q9m_feq_f(f, rho, ux * rho, uy * rho);
+                for (q = 0; q < 9; q++) st(L, L->buf[0], (size_t)q * L->N + n, q, f[q]);
             }
         } else {
             /* Init(): SetEquilibrium(Density, Velocity_x, Velocity_y), Dynamics.c:26-38 */
@@ -283,12 +298,14 @@ static void step_rows(orc_lattice* L, long double* glob, long r0, long r1)
                 pull(L, in, x, y, z, fd);
                 if (L->storage == ORC_F64) {
                     if (is3d(L)) d3q19_node_d(fd, nt, L->params, zr, glob, lid, L->flags, x, y, z, L->nx, L->ny, L->nz);
+                    else if (L->model == ORC_D2Q9) d2q9m_node_d(fd, nt, L->params, zr, glob);
                     else d2q9_node_d(fd, nt, L->params[0], L->params[1], zr, drag, glob);
                     for (q = 0; q < L->Q; q++) st(L, out, (size_t)q * L->N + n, q, fd[q]);
                 } else {
                     float ff[19];
                     for (q = 0; q < L->Q; q++) ff[q] = (float)fd[q];
                     if (is3d(L)) d3q19_node_f(ff, nt, L->params, zr, glob, lid, L->flags, x, y, z, L->nx, L->ny, L->nz);
+                    else if (L->model == ORC_D2Q9) d2q9m_node_f(ff, nt, L->params, zr, glob);
                     else d2q9_node_f(ff, nt, (float)L->params[0], (float)L->params[1], zr, drag, glob);
                     for (q = 0; q < L->Q; q++) st(L, out, (size_t)q * L->N + n, q, ff[q]);
                 }
@@ -387,6 +404,16 @@ void orc_get_quantity(const orc_lattice* L, int which, double* out)
                         }
                         out[3*n] = ux; out[3*n+1] = uy; out[3*n+2] = uz;
                     }
+                } else if (L->model == ORC_D2Q9) {
+                    /* getRho / getU, models/flow/d2q9/Dynamics.c.Rt:46-63 */
+                    double d = 0, jx = 0, jy = 0;
+                    for (q = 0; q < 9; q++) { d += f[q]; jx += f[q] * L->e[q][0]; jy += f[q] * L->e[q][1]; }
+                    if (which == 0) out[n] = d;
+                    else if (which == 1) {
+                        out[3*n] = jx / d + L->params[3] * 0.5;
+                        out[3*n+1] = jy / d + L->params[4] * 0.5;
+                        out[3*n+2] = 0;
+                    } else out[n] = 0;
                 } else {
                     /* getRho / getU, Dynamics.c:102-116 */
                     const double* zr = zone_row(L, nt);

@@ -12,6 +12,8 @@
  *                  in the form of models/flow/d3q27/Dynamics.c.Rt:68-82
  *   d3q19          models/flow/d3q19/Dynamics.c.Rt:23-290, Dynamics.R:1-95,
  *                  src/lib/d3q19.R:24-96
+ *   d2q9           models/flow/d2q9/Dynamics.c.Rt, Dynamics.R with the basis of
+ *                  src/lib/feq.R and the Zou-He of src/lib/boundary.R:155-207
  *   d3q19_lid      d3q19 plus a MovingWall node type (bounce-back with Ladd's
  *                  wall-velocity term) and its force/torque Globals; the
  *                  reference has no moving wall, so this part is pinned by
@@ -36,7 +38,7 @@
 extern "C" {
 #endif
 
-enum { ORC_D2Q9_SRT = 0, ORC_D2Q9_SRT_DRAG = 1, ORC_D3Q19 = 2, ORC_D3Q19_LID = 3 };
+enum { ORC_D2Q9_SRT = 0, ORC_D2Q9_SRT_DRAG = 1, ORC_D3Q19 = 2, ORC_D3Q19_LID = 3, ORC_D2Q9 = 4 };
 enum { ORC_F64 = 0, ORC_F32 = 1, ORC_F32S = 2, ORC_F16S = 3, ORC_BF16S = 4 };
 
 /* d2q9 parameter vector:   [omega, Density]
@@ -47,7 +49,10 @@ enum { ORC_F64 = 0, ORC_F32 = 1, ORC_F32S = 2, ORC_F16S = 3, ORC_BF16S = 4 };
  *                           reference's fixed rates); d3q19_lid appends
  *                           [MovingWallVelocityX, Y, Z]
  * d3q19 zone columns:      24 <Global>InObj weights in Globals order (28 for
- *                           d3q19_lid).                                       */
+ *                           d3q19_lid).
+ * d2q9 (MRT) parameters:   [S2, S3, S4, GravitationX, GravitationY]
+ * d2q9 (MRT) zone columns: [VelocityX, VelocityY, Pressure, PressureLossInObj,
+ *                           OutletFluxInObj, InletFluxInObj]                  */
 
 typedef struct orc_lattice orc_lattice;
 

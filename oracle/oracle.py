@@ -14,9 +14,10 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 
-D2Q9_SRT, D2Q9_SRT_DRAG, D3Q19, D3Q19_LID = 0, 1, 2, 3
+D2Q9_SRT, D2Q9_SRT_DRAG, D3Q19, D3Q19_LID, D2Q9 = 0, 1, 2, 3, 4
 F64, F32, F32S, F16S, BF16S = 0, 1, 2, 3, 4
-MODELS = {"d2q9_SRT": D2Q9_SRT, "d2q9_SRT_drag": D2Q9_SRT_DRAG, "d3q19": D3Q19, "d3q19_lid": D3Q19_LID}
+MODELS = {"d2q9_SRT": D2Q9_SRT, "d2q9_SRT_drag": D2Q9_SRT_DRAG, "d3q19": D3Q19, "d3q19_lid": D3Q19_LID,
+          "d2q9": D2Q9}
 STORAGE = {"f64": F64, "f32": F32, "f32s": F32S, "f16s": F16S, "bf16s": BF16S}
 
 _lib = None
@@ -153,6 +154,15 @@ def d3q19_collide(f, params):
 D3Q19_PARAMS = ["omega", "InletVelocity", "InletDensity", "ForceX", "ForceY", "ForceZ",
                 "MRT_Se", "MRT_Seps", "MRT_Sq", "MRT_Spi", "MRT_Sm"]
 D3Q19_LID_PARAMS = D3Q19_PARAMS + ["MovingWallVelocityX", "MovingWallVelocityY", "MovingWallVelocityZ"]
+D2Q9_PARAMS = ["S2", "S3", "S4", "GravitationX", "GravitationY"]
+
+# derived settings of the models' Dynamics.R (AddSetting(name, <target>=expr))
+DERIVED = {
+    "nu": ("omega", lambda v: 1.0 / (3 * v + 0.5)),
+    "InletPressure": ("InletDensity", lambda v: 1.0 + v * 3),
+    "Viscosity": ("RelaxationRate", lambda v: 1.0 / (3 * v + 0.5)),
+    "RelaxationRate": ("S2", lambda v: 1 - v),
+}
 
 
 def resolved_settings(model, settings):
@@ -163,14 +173,14 @@ def resolved_settings(model, settings):
     m = Model(model)
     vals = {n: 0.0 for n in m.settings}
     for n in m.settings:
-        if n not in ("nu", "InletPressure"):
+        if n not in DERIVED:
             vals[n] = m.setting_default(n)
     for n, v in settings:
-        vals[n] = v
-        if n == "nu":
-            vals["omega"] = 1.0 / (3 * v + 0.5)
-        if n == "InletPressure":
-            vals["InletDensity"] = 1.0 + v * 3
+        while True:
+            vals[n] = v
+            if n not in DERIVED or DERIVED[n][0] not in vals:
+                break
+            n, v = DERIVED[n][0], DERIVED[n][1](v)
     return vals
 
 
@@ -184,6 +194,8 @@ def lattice_for_case(case, storage="f64"):
         L.set_params([vals[n] for n in D3Q19_PARAMS])
     elif case.model == "d3q19_lid":
         L.set_params([vals[n] for n in D3Q19_LID_PARAMS])
+    elif case.model == "d2q9":
+        L.set_params([vals[n] for n in D2Q9_PARAMS])
     else:
         L.set_params([vals["omega"], vals["Density"]])
     zshift = m.group("SETTINGZONE")[1]

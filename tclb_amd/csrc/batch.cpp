@@ -172,8 +172,7 @@ void Batch::SetSetting(int id, int setting, double v) {
             c->settings[s] = val;
             const SettingDef& d = model->settings[s];
             if (d.derived < 0) break;
-            if (d.derive_kind == 1) val = 1.0 / (3 * val + 0.5);
-            else if (d.derive_kind == 2) val = 1.0 + val * 3;
+            val = derived_value(d, val);
             s = d.derived;
         }
     }

@@ -10,7 +10,8 @@ namespace tclbx {
 // upload).  d2q9: [omega, Density].  d3q19: [omega, InletVelocity,
 // InletDensity, ForceX, ForceY, ForceZ, then the relaxation factors of the
 // e, eps, q, pi and m moment groups, then gamma1 = 1 - omega (nu group),
-// then (d3q19_lid) MovingWallVelocityX/Y/Z].
+// then (d3q19_lid) MovingWallVelocityX/Y/Z].  d2q9 (MRT): [S2, S3, S4,
+// GravitationX, GravitationY].
 constexpr int MAX_PAR = 16;
 constexpr int MAX_GLOBALS = 32;
 

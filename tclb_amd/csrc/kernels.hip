@@ -41,6 +41,11 @@ template <> struct Lat<MODEL_D2Q9_SRT> {
 template <> struct Lat<MODEL_D2Q9_SRT_DRAG> : Lat<MODEL_D2Q9_SRT> {
     static constexpr int G = 3, SUMG = 3;
 };
+// d2q9 MRT: same directions; NodeType groups BOUNDARY (4 bits), COLLISION,
+// OBJECTIVE (2 bits), zone from bit 7
+template <> struct Lat<MODEL_D2Q9> : Lat<MODEL_D2Q9_SRT> {
+    static constexpr int G = 4, SUMG = 4, ZSHIFT = 7;
+};
 template <> struct Lat<MODEL_D3Q19> {
     static constexpr int Q = 19, G = 26, SUMG = 25, ZSHIFT = 6;
     __host__ __device__ static constexpr int ex(int i) {
@@ -70,6 +75,7 @@ int kernel_nglobals(int model) {
     case MODEL_D2Q9_SRT_DRAG: return Lat<MODEL_D2Q9_SRT_DRAG>::G;
     case MODEL_D3Q19: return Lat<MODEL_D3Q19>::G;
     case MODEL_D3Q19_LID: return Lat<MODEL_D3Q19_LID>::G;
+    case MODEL_D2Q9: return Lat<MODEL_D2Q9>::G;
     }
     return 0;
 }
@@ -176,6 +182,168 @@ __device__ __forceinline__ void d2q9_node(R* f, unsigned nt, R omega, R Density,
 #pragma unroll
         for (int i = 0; i < 9; i++) f[i] = f[i] + omega*(f_eq[i]-f[i]);
     }
+}
+
+// ------------------------------------------------------------ d2q9 MRT ---
+// models/flow/d2q9/Dynamics.c.Rt.  The moment basis the reference generates
+// (src/lib/feq.R MRT_eq with the Cholesky weighting of Dynamics.c.Rt:11-20)
+// is the integer Hermite basis: moment k of direction i is kB9[i][k], for
+// e = (ex, ey): 1, ex, ey, 3ex^2-1, 3ey^2-1, 3ex ey, ey(3ex^2-1),
+// ex(3ey^2-1), (3ex^2-1)(3ey^2-1), of orders 0,1,1,2,2,2,3,3,4.  Its
+// equilibrium moments are rho, Jx, Jy, 3Jx^2/rho, 3Jy^2/rho, 3JxJy/rho, 0,
+// 0, 0, and the inverse is kB9i[k][i] = w_i kB9[i][k] / |moment k|^2
+// (exact fractions below).
+__device__ constexpr int kB9[9][9] = {
+    {1, 0, 0, -1, -1, 0, 0, 0, 1},   {1, 1, 0, 2, -1, 0, 0, -1, -2},  {1, 0, 1, -1, 2, 0, -1, 0, -2},
+    {1, -1, 0, 2, -1, 0, 0, 1, -2},  {1, 0, -1, -1, 2, 0, 1, 0, -2},  {1, 1, 1, 2, 2, 3, 2, 2, 4},
+    {1, -1, 1, 2, 2, -3, 2, -2, 4},  {1, -1, -1, 2, 2, 3, -2, -2, 4}, {1, 1, -1, 2, 2, -3, -2, 2, 4}};
+__device__ constexpr double kB9i[9][9] = {
+    {4. / 9, 1. / 9, 1. / 9, 1. / 9, 1. / 9, 1. / 36, 1. / 36, 1. / 36, 1. / 36},
+    {0, 1. / 3, 0, -1. / 3, 0, 1. / 12, -1. / 12, -1. / 12, 1. / 12},
+    {0, 0, 1. / 3, 0, -1. / 3, 1. / 12, 1. / 12, -1. / 12, -1. / 12},
+    {-2. / 9, 1. / 9, -1. / 18, 1. / 9, -1. / 18, 1. / 36, 1. / 36, 1. / 36, 1. / 36},
+    {-2. / 9, -1. / 18, 1. / 9, -1. / 18, 1. / 9, 1. / 36, 1. / 36, 1. / 36, 1. / 36},
+    {0, 0, 0, 0, 0, 1. / 12, -1. / 12, 1. / 12, -1. / 12},
+    {0, 0, -1. / 6, 0, 1. / 6, 1. / 12, 1. / 12, -1. / 12, -1. / 12},
+    {0, -1. / 6, 0, 1. / 6, 0, 1. / 12, -1. / 12, -1. / 12, 1. / 12},
+    {1. / 9, -1. / 18, -1. / 18, -1. / 18, -1. / 18, 1. / 36, 1. / 36, 1. / 36, 1. / 36}};
+
+// f = R B^-1 over the nonzero coefficients in ascending k
+template <class R>
+__device__ __forceinline__ void d2q9m_from_moments(R* f, const R* m) {
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        R acc = 0;
+#pragma unroll
+        for (int k = 0; k < 9; k++)
+            if (kB9i[k][i] != 0) acc += m[k] * R(kB9i[k][i]);
+        f[i] = acc;
+    }
+}
+
+// SetEquilibrum(rho, Jx, Jy), Dynamics.c.Rt:73-79
+template <class R>
+__device__ __forceinline__ void d2q9m_feq(R* f, R rho, R Jx, R Jy) {
+    R m[9] = {rho, Jx, Jy, R(3.) * Jx * Jx / rho, R(3.) * Jy * Jy / rho, R(3.) * Jx * Jy / rho, R(0), R(0), R(0)};
+    d2q9m_from_moments(f, m);
+}
+
+// ZouHeRewrite (src/lib/boundary.R:155-207) for the side whose outward
+// normal is s along axis A (0 = x, 1 = y): the populations entering the
+// domain (e.n < 0) become f_opp + 6 w (e.R), R solved so that the node gets
+// velocity (VelocityX, VelocityY) or density 1 + 3 Pressure with no
+// tangential momentum.  With S0 = sum of the tangential f + 2 sum of the
+// outgoing f and T their tangential momentum, the equations give
+//   velocity: R_n = V_n S0 / (1 + s V_n),  R_t = 3 (V_t (S0 - s R_n) - T)
+//   pressure: R_n = s (S0 - (3 P + 1)),    R_t = -3 T.
+__host__ __device__ constexpr int d2q9_opp(int i) { return i == 0 ? 0 : (i < 5 ? (i + 1) % 4 + 1 : (i - 3) % 4 + 5); }
+
+template <class R, int A, int SGN, bool VEL>
+__device__ __forceinline__ void d2q9m_zouhe(R* f, R Vx, R Vy, R P) {
+    using L = Lat<MODEL_D2Q9_SRT>;
+    R S0 = 0, T = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        const int en = SGN * (A == 0 ? L::ex(i) : L::ey(i));
+        const int et = A == 0 ? L::ey(i) : L::ex(i);
+        if (en == 0) {
+            S0 += f[i];
+            T += f[i] * R(et);
+        } else if (en > 0) {
+            S0 += R(2.) * f[i];
+        }
+    }
+    R Rn, Rt;
+    if (VEL) {
+        const R Vn = A == 0 ? Vx : Vy, Vt = A == 0 ? Vy : Vx;
+        Rn = Vn * S0 / (R(1.) + R(SGN) * Vn);
+        Rt = R(3.) * (Vt * (S0 - R(SGN) * Rn) - T);
+    } else {
+        Rn = R(SGN) * (S0 - (P * R(3.) + R(1.)));
+        Rt = R(-3.) * T;
+    }
+    const R Rx = A == 0 ? Rn : Rt, Ry = A == 0 ? Rt : Rn;
+#pragma unroll
+    for (int i = 1; i < 9; i++) {
+        const int en = SGN * (A == 0 ? L::ex(i) : L::ey(i));
+        if (en < 0) {
+            const R cu = R(L::ex(i)) * Rx + R(L::ey(i)) * Ry;
+            f[i] = f[d2q9_opp(i)] + (i < 5 ? cu * R(2.) / R(3.) : cu / R(6.));
+        }
+    }
+}
+
+// Symmetry(2, -1 / +1) (src/lib/boundary.R:33-50): the populations entering
+// through a north (south) symmetry plane copy their mirror images
+template <class R, bool NORTH>
+__device__ __forceinline__ void d2q9m_symmetry(R* f) {
+    if (NORTH) { f[4] = f[2]; f[8] = f[5]; f[7] = f[6]; }
+    else { f[2] = f[4]; f[5] = f[8]; f[6] = f[7]; }
+}
+
+// Run(), Dynamics.c.Rt:104-138 and CollisionMRT, :196-268.  Zone columns:
+// [VelocityX, VelocityY, Pressure, PressureLossInObj, OutletFluxInObj,
+// InletFluxInObj]; Globals [PressureLoss, OutletFlux, InletFlux, Objective].
+template <class R, bool GLOB>
+__device__ __forceinline__ void d2q9m_node(R* f, unsigned nt, const double* par, const double* zr, double* g) {
+    const R Vx = (R)zr[0], Vy = (R)zr[1], P = (R)zr[2];
+    switch (nt & 0xFu) {
+    case 9:   // Solid
+    case 10: {  // Wall: FullBounceBack
+        R t;
+        t = f[1]; f[1] = f[3]; f[3] = t;
+        t = f[2]; f[2] = f[4]; f[4] = t;
+        t = f[5]; f[5] = f[7]; f[7] = t;
+        t = f[6]; f[6] = f[8]; f[8] = t;
+        break; }
+    case 6: d2q9m_zouhe<R, 0, 1, true>(f, Vx, Vy, P); break;    // EVelocity
+    case 2: d2q9m_zouhe<R, 0, -1, false>(f, Vx, Vy, P); break;  // WPressure
+    case 5: d2q9m_zouhe<R, 0, -1, true>(f, Vx, Vy, P); break;   // WVelocity
+    case 1: d2q9m_zouhe<R, 0, 1, false>(f, Vx, Vy, P); break;   // EPressure
+    case 3: d2q9m_zouhe<R, 1, 1, true>(f, Vx, Vy, P); break;    // NVelocity
+    case 4: d2q9m_zouhe<R, 1, -1, true>(f, Vx, Vy, P); break;   // SVelocity
+    case 7: d2q9m_symmetry<R, true>(f); break;                  // NSymmetry
+    case 8: d2q9m_symmetry<R, false>(f); break;                 // SSymmetry
+    default: break;
+    }
+    if ((nt & 0x10u) == 0) return;  // MRT
+    R m[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        R acc = 0;
+#pragma unroll
+        for (int i = 0; i < 9; i++)
+            if (kB9[i][k] != 0) acc += f[i] * R(kB9[i][k]);
+        m[k] = acc;
+    }
+    const R rho = m[0];
+    R Jx = m[1], Jy = m[2];
+    if (GLOB) {
+        const unsigned obj = nt & 0x60u;
+        if (obj == 0x40u || obj == 0x20u) {  // Outlet / Inlet
+            const R Usq = (Jx * Jx + Jy * Jy) / (rho * rho);
+            const R flux = Jx / rho / rho;
+            const R loss = (Jx / rho) / rho * ((rho - R(1.)) / R(3.) + Usq / rho / R(2.));
+            const int k = obj == 0x40u ? 1 : 2;
+            g[k] += (double)flux; g[3] += zr[3 + k];
+            g[0] += (double)(obj == 0x40u ? -loss : loss); g[3] += zr[3];
+        }
+    }
+    const R S2 = (R)par[0], S3 = (R)par[1], S4 = (R)par[2];
+    m[3] = S2 * (m[3] - R(3.) * Jx * Jx / rho);
+    m[4] = S2 * (m[4] - R(3.) * Jy * Jy / rho);
+    m[5] = S2 * (m[5] - R(3.) * Jx * Jy / rho);
+    m[6] = S3 * m[6];
+    m[7] = S3 * m[7];
+    m[8] = S4 * m[8];
+    Jx = Jx + (R)par[3] * rho;
+    Jy = Jy + (R)par[4] * rho;
+    m[1] = Jx;
+    m[2] = Jy;
+    m[3] = m[3] + R(3.) * Jx * Jx / rho;
+    m[4] = m[4] + R(3.) * Jy * Jy / rho;
+    m[5] = m[5] + R(3.) * Jx * Jy / rho;
+    d2q9m_from_moments(f, m);
 }
 
 // --------------------------------------------------------------- d3q19 ---
@@ -732,6 +900,8 @@ __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, 
     if (active) {
         if constexpr (is_d3q19<MODEL>) {
             d3q19_node<R, GLOB, MODEL == MODEL_D3Q19_LID>(f, nt, a.par, zr, g, a.flags, xc, y, z, nx, ny, nz);
+        } else if constexpr (MODEL == MODEL_D2Q9) {
+            d2q9m_node<R, GLOB>(f, nt, a.par, zr, g);
         } else {
             d2q9_node<R, MODEL == MODEL_D2Q9_SRT_DRAG, GLOB>(f, nt, (R)a.par[0], (R)a.par[1], zr, g);
         }
@@ -856,6 +1026,12 @@ __global__ void __launch_bounds__(256) k_init(StepArgs a) {
     R f[Q];
     if constexpr (is_d3q19<MODEL>) {
         d3q19_feq<R>(f, R(1.0), (R)a.par[1], R(0.), R(0.));
+    } else if constexpr (MODEL == MODEL_D2Q9) {
+        // Init(), models/flow/d2q9/Dynamics.c.Rt:89-102
+        R rho = R(1) + (R)zr[2] * R(3);
+        R ux = (R)zr[0], uy = (R)zr[1];
+        if ((nt & 0xFu) == 10u) { rho = R(1); ux = R(0); uy = R(0); }
+        d2q9m_feq<R>(f, rho, ux * rho, uy * rho);
     } else {
         d2q9_feq<R>(f, (R)a.par[1], (R)zr[1], (R)zr[2]);
     }
@@ -907,6 +1083,18 @@ __global__ void __launch_bounds__(256) k_quantity(StepArgs a, int qid, int x0, i
             if ((nt & 0x1Cu) == 0x0Cu) ux = uy = uz = 0;
             if (MODEL == MODEL_D3Q19_LID && (nt & 0x1Cu) == 0x1Cu) { ux = a.par[12]; uy = a.par[13]; uz = a.par[14]; }
             out[3 * tid] = ux * scale; out[3 * tid + 1] = uy * scale; out[3 * tid + 2] = uz * scale;
+        }
+    } else if constexpr (MODEL == MODEL_D2Q9) {
+        // getRho / getU, Dynamics.c.Rt:46-63: Rho = 0, U = J/rho + G/2 = 1
+        double d = 0, jx = 0, jy = 0;
+#pragma unroll
+        for (int i = 0; i < Q; i++) { d += f[i]; jx += f[i] * L::ex(i); jy += f[i] * L::ey(i); }
+        if (qid == 0) {
+            out[tid] = d * scale;
+        } else {
+            out[3 * tid] = (jx / d + a.par[3] * 0.5) * scale;
+            out[3 * tid + 1] = (jy / d + a.par[4] * 0.5) * scale;
+            out[3 * tid + 2] = 0.0;
         }
     } else {
         int zone = (int)(nt >> L::ZSHIFT);
@@ -1080,6 +1268,7 @@ hipError_t launch_step(int model, int storage, bool globals, const StepArgs& a, 
     case MODEL_D2Q9_SRT_DRAG: return step_m<MODEL_D2Q9_SRT_DRAG>(storage, globals, a, block, s);
     case MODEL_D3Q19: return step_m<MODEL_D3Q19>(storage, globals, a, block, s);
     case MODEL_D3Q19_LID: return step_m<MODEL_D3Q19_LID>(storage, globals, a, block, s);
+    case MODEL_D2Q9: return step_m<MODEL_D2Q9>(storage, globals, a, block, s);
     }
     return hipErrorInvalidValue;
 }
@@ -1090,6 +1279,7 @@ hipError_t launch_init(int model, int storage, const StepArgs& a, int block, hip
     case MODEL_D2Q9_SRT_DRAG: return init_m<MODEL_D2Q9_SRT_DRAG>(storage, a, block, s);
     case MODEL_D3Q19: return init_m<MODEL_D3Q19>(storage, a, block, s);
     case MODEL_D3Q19_LID: return init_m<MODEL_D3Q19_LID>(storage, a, block, s);
+    case MODEL_D2Q9: return init_m<MODEL_D2Q9>(storage, a, block, s);
     }
     return hipErrorInvalidValue;
 }
@@ -1145,6 +1335,7 @@ hipError_t launch_batch_step(int model, int storage, bool globals, const BatchCa
                                                   block, s);
     case MODEL_D3Q19_LID:
         return batch_m<MODEL_D3Q19_LID>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
+    case MODEL_D2Q9: return batch_m<MODEL_D2Q9>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
     }
     return hipErrorInvalidValue;
 }
@@ -1158,6 +1349,7 @@ hipError_t launch_quantity(int model, int storage, int q, const StepArgs& a, int
     case MODEL_D3Q19: return quantity_m<MODEL_D3Q19>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
     case MODEL_D3Q19_LID:
         return quantity_m<MODEL_D3Q19_LID>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
+    case MODEL_D2Q9: return quantity_m<MODEL_D2Q9>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
     }
     return hipErrorInvalidValue;
 }

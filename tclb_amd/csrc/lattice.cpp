@@ -112,12 +112,7 @@ void Lattice::SetSetting(int id, double v) {
     settings_[id] = v;
     params_version_++;
     const SettingDef& s = model->settings[id];
-    if (s.derived >= 0) {
-        double d = v;
-        if (s.derive_kind == 1) d = 1.0 / (3 * v + 0.5);       // omega = 1/(3 nu + 0.5)
-        else if (s.derive_kind == 2) d = 1.0 + v * 3;          // InletDensity = 1 + 3 p
-        SetSetting(s.derived, d);
-    }
+    if (s.derived >= 0) SetSetting(s.derived, derived_value(s, v));
 }
 
 void Lattice::SetZoneSetting(int id, int zone, double v) {
@@ -220,6 +215,14 @@ void model_params(const ModelDef* model, const std::vector<double>& settings, do
         par[12] = S("MovingWallVelocityX");
         par[13] = S("MovingWallVelocityY");
         par[14] = S("MovingWallVelocityZ");
+    } else if (model->id == MODEL_D2Q9) {
+        // [S2, S3, S4, GravitationX, GravitationY]: the relaxation factors of
+        // the 2nd, 3rd and 4th order moments and the body force
+        par[0] = S("S2");
+        par[1] = S("S3");
+        par[2] = S("S4");
+        par[3] = S("GravitationX");
+        par[4] = S("GravitationY");
     } else {
         par[0] = S("omega");
         par[1] = S("Density");

@@ -188,12 +188,54 @@ ModelDef make_d3q19(bool lid) {
     return m;
 }
 
+// d2q9: MRT in the weighted-orthogonal (Hermite) moment basis the reference
+// generates from src/lib/feq.R (models/flow/d2q9/Dynamics.c.Rt:1-40), stored
+// shifted by w_i, Zou-He conditions on all four sides, symmetry planes,
+// pressure-loss and flux Globals on Inlet/Outlet nodes
+// (models/flow/d2q9/Dynamics.R).
+ModelDef make_d2q9_mrt() {
+    ModelDef m;
+    m.id = MODEL_D2Q9;
+    m.name = "d2q9";
+    m.Q = 9; m.dim = 2;
+    m.ex = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+    m.ey = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+    m.ez = std::vector<int>(9, 0);
+    m.w = {4. / 9, 1. / 9, 1. / 9, 1. / 9, 1. / 9, 1. / 36, 1. / 36, 1. / 36, 1. / 36};
+    // Dynamics.R:38-60: RelaxationRate sets S2 = 1 - RelaxationRate,
+    // Viscosity sets RelaxationRate = 1/(3 Viscosity + 0.5)
+    m.settings = {{"RelaxationRate", 0.0, 4, 3}, {"Viscosity", 0.16666666, 0, 1},
+                  {"GravitationX", 0.0, -1, 0},  {"GravitationY", 0.0, -1, 0},
+                  {"S2", 0.0, -1, 0},            {"S3", 0.0, -1, 0},
+                  {"S4", 0.0, -1, 0},            {"Threshold", 0.5, -1, 0}};
+    m.zonal = {{"VelocityX", 0, -1, 0}, {"VelocityY", 0, -1, 0}, {"Pressure", 0, -1, 0}};
+    finish_globals(m, {{"PressureLoss", false, "1mPa"}, {"OutletFlux", false, "1m2/s"},
+                       {"InletFlux", false, "1m2/s"}});
+    m.quantities = {{"Rho", false, "kg/m3"}, {"U", true, "m/s"}};
+    pack_nodetypes(m, {{"EPressure", "BOUNDARY"}, {"WPressure", "BOUNDARY"}, {"NVelocity", "BOUNDARY"},
+                       {"SVelocity", "BOUNDARY"}, {"WVelocity", "BOUNDARY"}, {"EVelocity", "BOUNDARY"},
+                       {"NSymmetry", "BOUNDARY"}, {"SSymmetry", "BOUNDARY"}, {"Inlet", "OBJECTIVE"},
+                       {"Outlet", "OBJECTIVE"}, {"Solid", "BOUNDARY"}, {"Wall", "BOUNDARY"},
+                       {"MRT", "COLLISION"}});
+    return m;
+}
+
 const std::vector<ModelDef>& registry() {
-    static std::vector<ModelDef> r = {make_d2q9(false), make_d2q9(true), make_d3q19(false), make_d3q19(true)};
+    static std::vector<ModelDef> r = {make_d2q9(false), make_d2q9(true), make_d3q19(false), make_d3q19(true),
+                                      make_d2q9_mrt()};
     return r;
 }
 
 }  // namespace
+
+double derived_value(const SettingDef& s, double v) {
+    switch (s.derive_kind) {
+    case 1: return 1.0 / (3 * v + 0.5);  // omega / RelaxationRate = 1/(3 nu + 0.5)
+    case 2: return 1.0 + v * 3;          // InletDensity = 1 + 3 InletPressure
+    case 3: return 1 - v;                // S2 = 1 - RelaxationRate
+    default: return v;
+    }
+}
 
 const ModelDef* model_by_name(const std::string& name) {
     for (auto& m : registry()) if (m.name == name) return &m;

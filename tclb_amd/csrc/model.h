@@ -23,7 +23,8 @@ enum ModelId {
     MODEL_D2Q9_SRT_DRAG = 1,
     MODEL_D3Q19 = 2,
     MODEL_D3Q19_LID = 3,
-    MODEL_COUNT = 4
+    MODEL_D2Q9 = 4,
+    MODEL_COUNT = 5
 };
 
 // storage format of the distributions in HBM
@@ -33,7 +34,7 @@ struct SettingDef {
     std::string name;
     double def;            // default value
     int derived;           // index of the setting this one sets, or -1
-    int derive_kind;       // 0 none, 1 omega = 1/(3x+0.5), 2 InletDensity = 1+3x
+    int derive_kind;       // 0 none, 1 1/(3x+0.5), 2 1+3x, 3 1-x
     std::string unit = "1";  // unit of the value (Log's _si column divides by it)
     bool extension = false;  // added by this engine to a reference model (not in its Log)
 };
@@ -83,6 +84,10 @@ struct ModelDef {
     const NodeTypeDef* nodetype(const std::string& n) const;
     const GroupDef* group(const std::string& n) const;
 };
+
+// value a setting gives the setting it derives (Model::Setting::derivedValue,
+// the equations of AddSetting in Dynamics.R)
+double derived_value(const SettingDef& s, double v);
 
 const ModelDef* model_by_name(const std::string& name);
 const ModelDef* model_by_id(int id);

@@ -28,7 +28,26 @@ def test_every_declared_symbol_is_exported():
 
 
 def test_models():
-    assert tclb_amd.models() == ["d2q9_SRT", "d2q9_SRT_drag", "d3q19", "d3q19_lid"]
+    assert tclb_amd.models() == ["d2q9_SRT", "d2q9_SRT_drag", "d3q19", "d3q19_lid", "d2q9"]
+
+
+def test_d2q9_mrt_contract():
+    """models/flow/d2q9/Dynamics.R: BOUNDARY has 10 types (4 bits), then
+    COLLISION (MRT), then OBJECTIVE (Inlet, Outlet); zone from bit 7."""
+    m = tclb_amd.Model("d2q9")
+    order = ["EPressure", "WPressure", "NVelocity", "SVelocity", "WVelocity", "EVelocity", "NSymmetry",
+             "SSymmetry", "Solid", "Wall"]
+    for k, n in enumerate(order):
+        assert m.nodetype(n) == (k + 1, 0xF)
+    assert m.nodetype("MRT") == (0x10, 0x10)
+    assert m.nodetype("Inlet") == (0x20, 0x60) and m.nodetype("Outlet") == (0x40, 0x60)
+    assert m.group("SETTINGZONE") == (0xFF80, 7)
+    assert m.settings == ["RelaxationRate", "Viscosity", "GravitationX", "GravitationY", "S2", "S3", "S4",
+                          "Threshold"]
+    assert m.zonal_settings == ["VelocityX", "VelocityY", "Pressure", "PressureLossInObj", "OutletFluxInObj",
+                                "InletFluxInObj"]
+    assert m.global_names == ["PressureLoss", "OutletFlux", "InletFlux", "Objective"]
+    assert m.quantities == ["Rho", "U"]
 
 
 def test_d2q9_srt_contract():

@@ -1,0 +1,238 @@
+"""Case files run end to end on the GPU (tclbx_case_run): the handler order
+and scheduling of the reference (Model, Param, Solve with Log / Stop /
+Failcheck / VTK / SaveBinary callbacks, LoadBinary, Units, deprecated
+Params), the Log CSV and VTI outputs, and parity of the lattice a case
+produces with the oracle run from the oracle's own rasterization of the
+same file."""
+import base64
+import csv
+import os
+import struct
+import xml.etree.ElementTree as ET
+
+import numpy as np
+import pytest
+
+from conftest import oracle_for_case
+
+import tclb_amd
+from tclb_amd import cases
+from oracle import geometry as G
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def run_case(model, xml, tmp_path, name="case.xml", storage="f64"):
+    c = tclb_amd.Case(model, xml=xml, name=name, storage=storage, output_root=str(tmp_path) + "/")
+    c.run()
+    return c
+
+
+CHANNEL = """<CLBConfig output="out/">
+  <Geometry nx="64" ny="24">
+    <BGK><Box/></BGK>
+    <WVelocity name="Inlet"><Inlet/></WVelocity>
+    <EPressure name="Outlet"><Outlet/></EPressure>
+    <Wall mask="ALL"><Channel/></Wall>
+    <Solid name="body"><OffgridSphere x="16" y="12" R="4.3"/></Solid>
+  </Geometry>
+  <Model>
+    <Param name="nu" value="0.05"/>
+    <Param name="Velocity_x" value="0.02" zone="Inlet"/>
+    <Param name="DragInObj" value="0.5" zone="body"/>
+  </Model>
+  <Log Iterations="50"/>
+  <Solve Iterations="200"/>
+</CLBConfig>"""
+
+
+def read_log(path):
+    with open(path) as f:
+        rows = list(csv.reader(f, skipinitialspace=True))
+    return rows[0], [[float(v) for v in r] for r in rows[1:]]
+
+
+def test_channel_case_matches_oracle_and_logs(oracle_mod, tmp_path):
+    """The d2q9_SRT_drag channel case: the final rho/u within 1e-12 of the
+    oracle run from the oracle's rasterization, the Log rows (every 50
+    steps) carrying the oracle's Drag/Lift/Objective of those steps."""
+    say("case: d2q9 channel with Log")
+    c = run_case("d2q9_SRT_drag", CHANNEL, tmp_path, "chan.xml")
+    assert c.iteration == 200
+    flags, zones, _ = G.read_case(CHANNEL, "d2q9_SRT_drag", is_text=True)
+    assert np.array_equal(c.lattice.get_flags(), flags)
+    oc = cases.Case("chan", "d2q9_SRT_drag", 64, 24, 1, flags, settings=[("nu", 0.05)],
+                    zonal=[("Velocity_x", zones["Inlet"], 0.02), ("DragInObj", zones["body"], 0.5)])
+    orc = oracle_for_case(oracle_mod, oc, "f64")
+    head, rows = read_log(tmp_path / "out" / "chan_Log_P00_00000000.csv")
+    assert [r[0] for r in rows] == [50, 100, 150, 200]
+    for r in rows:
+        g = orc.iterate(50)
+        for name, want in zip(["Drag", "Lift", "Objective"], g):
+            got = r[head.index(name)]
+            assert abs(got - want) <= 1e-10 * max(abs(want), 1e-300) or abs(want) < 1e-10, (name, got, want)
+        assert r[head.index("nu")] == 0.05 and r[head.index("Velocity_x-Inlet")] == 0.02
+    u, uo = c.lattice.quantity("U"), orc.quantity("U")
+    assert np.max(np.abs(u - uo)) <= 1e-12 * np.max(np.abs(uo))
+
+
+def test_log_header_is_the_reference_layout(tmp_path):
+    """d2q9_SRT: Iteration, Time_si, Walltime, Optimization, then value and
+    _si of every setting, of every zonal setting per zone (zones sorted by
+    name), of every Global, then the dx/dt/dm scales (src/Solver.cpp.Rt:
+    126-172)."""
+    xml = CHANNEL.replace('<Param name="DragInObj" value="0.5" zone="body"/>', "")
+    c = run_case("d2q9_SRT", xml, tmp_path, "hdr.xml")
+    head, rows = read_log(tmp_path / "out" / "hdr_Log_P00_00000000.csv")
+    want = ["Iteration", "Time_si", "Walltime", "Optimization"]
+    for s in ["omega", "nu", "Density", "Threshold"]:
+        want += [s, s + "_si"]
+    for s in ["Velocity", "Velocity_x", "Velocity_y", "GravitationX", "GravitationY"]:
+        for z in ["DefaultZone", "Inlet", "Outlet", "body"]:
+            want += [f"{s}-{z}", f"{s}-{z}_si"]
+    want += ["Objective", "Objective_si", "dx_si", "dt_si", "dm_si"]
+    assert head == want
+    assert len(rows) == 4 and all(len(r) == len(want) for r in rows)
+    assert rows[0][1] == 50.0 and rows[0][-3:] == [1.0, 1.0, 1.0]
+    text = open(tmp_path / "out" / "hdr_Log_P00_00000000.csv").read().splitlines()[1]
+    assert text.startswith("50, 5.0000000000000e+01, ")   # %d, %.13le, ...
+
+
+def test_stop_ends_the_solve(tmp_path):
+    """<Stop VOLvolumeChange=..>: the fluid volume is constant, so the
+    second check (and with Times=3 the fourth) passes and stops the Solve."""
+    say("case: Stop")
+    base = """<CLBConfig>
+      <Geometry nx="16" ny="12" nz="10"><MRT><Box/></MRT><Wall mask="ALL"><Channel/></Wall></Geometry>
+      <Model><Param name="nu" value="0.1"/><Param name="ForceX" value="1e-5"/></Model>
+      {stop}
+      <Solve Iterations="10000"/>
+    </CLBConfig>"""
+    c = run_case("d3q19", base.format(stop='<Stop VOLvolumeChange="0.5" Iterations="100"/>'), tmp_path, "s1.xml")
+    assert c.iteration == 200
+    c = run_case("d3q19", base.format(stop='<Stop VOLvolumeChange="0.5" Times="3" Iterations="100"/>'), tmp_path,
+                 "s2.xml")
+    assert c.iteration == 400
+    c = run_case("d3q19", base.format(stop='<Stop VOLvxAbove="1e9" Iterations="100"/>'), tmp_path, "s3.xml")
+    assert c.iteration == 10000
+
+
+def test_save_load_and_failcheck(tmp_path):
+    """SaveBinary writes <output>_<file>_0.pri; a NaN planted in it and read
+    back with LoadBinary is found by Failcheck at its first check, which
+    stops the Solve there."""
+    say("case: SaveBinary, LoadBinary, Failcheck")
+    geom = '<Geometry nx="32" ny="16"><BGK><Box/></BGK><Wall mask="ALL"><Channel/></Wall></Geometry>'
+    a = run_case("d2q9_SRT", f"""<CLBConfig output="out/">{geom}
+        <Model><Param name="nu" value="0.1"/><Param name="GravitationX" value="1e-5"/></Model>
+        <SaveBinary file="state" Iterations="20"/><Solve Iterations="20"/></CLBConfig>""", tmp_path, "a.xml")
+    pri = tmp_path / "out" / "a_state_0.pri"
+    f = np.fromfile(pri, dtype=np.float64).reshape(9, 16, 32)
+    assert np.allclose(f, a.lattice.get_f(), rtol=0, atol=0)
+    b = run_case("d2q9_SRT", f"""<CLBConfig output="out/">{geom}
+        <Model><Param name="nu" value="0.1"/><Param name="GravitationX" value="1e-5"/></Model>
+        <LoadBinary file="{tmp_path}/out/a_state"/><Solve Iterations="30"/></CLBConfig>""", tmp_path, "b.xml")
+    a2 = run_case("d2q9_SRT", f"""<CLBConfig output="out/">{geom}
+        <Model><Param name="nu" value="0.1"/><Param name="GravitationX" value="1e-5"/></Model>
+        <Solve Iterations="50"/></CLBConfig>""", tmp_path, "a2.xml")
+    assert np.array_equal(b.lattice.get_f(), a2.lattice.get_f())   # 20 + 30 steps = 50 steps
+    f[3, 8, 10] = np.nan
+    f.tofile(tmp_path / "out" / "bad_0.pri")
+    c = run_case("d2q9_SRT", f"""<CLBConfig output="out/">{geom}
+        <Model><Param name="nu" value="0.1"/></Model>
+        <LoadBinary file="{tmp_path}/out/bad"/>
+        <Failcheck Iterations="10" nx="32" ny="16"/><Solve Iterations="100"/></CLBConfig>""", tmp_path, "c.xml")
+    assert c.iteration == 10
+    assert any("NaN" in m for m in c.messages)
+
+
+def _vti_arrays(path):
+    root = ET.parse(path).getroot()
+    out = {}
+    for da in root.iter("DataArray"):
+        text = da.text.strip()
+        n = struct.unpack("<i", base64.b64decode(text[:8]))[0]
+        raw = base64.b64decode(text[8:])
+        assert len(raw) == n
+        dt = {"Float64": np.float64, "UInt8": np.uint8, "UInt16": np.uint16}[da.get("type")]
+        out[da.get("Name")] = (np.frombuffer(raw, dtype=dt), int(da.get("NumberOfComponents")))
+    return root, out
+
+
+def test_vtk_output(tmp_path):
+    """The VTI file of a <VTK> callback: extent, the NodeType groups as
+    UInt8 fields, P and U as Float64 fields equal to the lattice's
+    quantities, and the .pvti that names it."""
+    say("case: VTK")
+    xml = """<CLBConfig output="out/">
+      <Geometry nx="16" ny="12" nz="10"><MRT><Box/></MRT><Wall mask="ALL"><Channel/></Wall>
+        <XYslice><Box dz="5" nz="1"/></XYslice></Geometry>
+      <Model><Param name="nu" value="0.1"/><Param name="ForceX" value="1e-5"/></Model>
+      <VTK Iterations="40"/><Solve Iterations="40"/></CLBConfig>"""
+    c = run_case("d3q19", xml, tmp_path, "v.xml")
+    root, arr = _vti_arrays(tmp_path / "out" / "v_VTK_P00_00000040.vti")
+    assert root.find("ImageData").get("WholeExtent") == "0 16 0 12 0 10"
+    assert set(arr) == {"ADDITIONALS", "BOUNDARY", "COLLISION", "SETTINGZONE", "P", "U"}
+    fl = c.lattice.get_flags().reshape(-1)
+    assert np.array_equal(arr["BOUNDARY"][0], ((fl & 0x1C) >> 2).astype(np.uint8))
+    assert np.array_equal(arr["ADDITIONALS"][0], (fl & 0x3).astype(np.uint8))
+    assert np.array_equal(arr["P"][0], c.lattice.quantity("P").reshape(-1))
+    assert arr["U"][1] == 3 and np.array_equal(arr["U"][0], c.lattice.quantity("U").reshape(-1))
+    p = ET.parse(tmp_path / "out" / "v_VTK_P00_00000040.pvti").getroot()
+    assert p.find("PImageData/Piece").get("Source") == "v_VTK_P00_00000040.vti"
+
+
+def test_units_and_deprecated_params(tmp_path):
+    """Units gauges convert Solve Iterations ("0.05s" = 50 steps at 1000
+    steps per second) and Param values; a deprecated <Params a="v"/>
+    becomes <Param name="a" value="v"/> and makes the case permissive."""
+    say("case: Units, Params")
+    xml = """<CLBConfig output="out/">
+      <Units><Param name="size" value="1mm" gauge="1"/><Param name="time" value="1s" gauge="1000"/></Units>
+      <Geometry nx="20mm" ny="8mm"><BGK><Box/></BGK><Wall mask="ALL"><Channel/></Wall></Geometry>
+      <Model><Params nu="1e-4m2/s" GravitationX="1e-6" NotASetting="3"/></Model>
+      <Solve Iterations="0.05s"/></CLBConfig>"""
+    c = run_case("d2q9_SRT", xml, tmp_path, "u.xml")
+    assert (c.nx, c.ny) == (20, 8)
+    assert c.iteration == 50
+    assert c.lattice.get_setting("nu") == pytest.approx(1e-4 * 1e6 / 1000, rel=1e-12)
+    assert any("Unknown setting NotASetting" in m for m in c.messages)
+
+
+def test_unknown_setting_fails_when_not_permissive(tmp_path):
+    xml = """<CLBConfig><Geometry nx="8" ny="4"><BGK><Box/></BGK></Geometry>
+      <Model><Param name="NoSuch" value="1"/></Model></CLBConfig>"""
+    with pytest.raises(tclb_amd.TclbxError):
+        run_case("d2q9_SRT", xml, tmp_path, "bad.xml")
+
+
+def test_d2q9_mrt_channel_case(oracle_mod, tmp_path):
+    """The d2q9 (MRT) channel case file: the reference's Inlet/Outlet
+    objective nodes and pressure-loss/flux Globals, Log rows against the
+    oracle, final u within 1e-12."""
+    say("case: d2q9 MRT channel")
+    path = os.path.join(os.path.dirname(__file__), "golden", "cases", "channel_d2q9.xml")
+    c = tclb_amd.Case("d2q9", path, output_root=str(tmp_path) + "/")
+    c.run()
+    assert c.iteration == 300
+    flags, zones, _ = G.read_case(path, "d2q9")
+    assert np.array_equal(c.lattice.get_flags(), flags)
+    oc = cases.Case("chan", "d2q9", 160, 40, 1, flags, settings=[("Viscosity", 0.02), ("S3", 0.2)],
+                    zonal=[("VelocityX", zones["Inlet"], 0.02), ("PressureLossInObj", -1, 1.0)])
+    orc = oracle_for_case(oracle_mod, oc, "f64")
+    head, rows = read_log(tmp_path / "output" / "channel_d2q9_Log_P00_00000000.csv")
+    assert head[4:8] == ["RelaxationRate", "RelaxationRate_si", "Viscosity", "Viscosity_si"]
+    assert [r[0] for r in rows] == [100, 200, 300]
+    for r in rows:
+        g = orc.iterate(100)
+        for name, want in zip(["PressureLoss", "OutletFlux", "InletFlux", "Objective"], g):
+            got = r[head.index(name)]
+            assert abs(got - want) <= 1e-10 * abs(want) or abs(want) < 1e-10, (name, got, want)
+        assert r[head.index("S2")] == pytest.approx(1 - 1 / (3 * 0.02 + 0.5), rel=1e-15)
+    u, uo = c.lattice.quantity("U"), orc.quantity("U")
+    assert np.max(np.abs(u - uo)) <= 1e-12 * np.max(np.abs(uo))
+    assert rows[-1][head.index("InletFlux")] > 0

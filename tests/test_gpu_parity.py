@@ -53,7 +53,22 @@ def random_case(model, nx, ny, nz, seed, steps=10):
     rng = np.random.default_rng(seed)
     m = tclb_amd.Model(model)
     zs = m.group("SETTINGZONE")[1]
-    if model.startswith("d2q9"):
+    if model == "d2q9":
+        MRT = m.flag("MRT")
+        kinds = [MRT] * 12 + [m.flag("Wall"), m.flag("Solid"), m.flag("Wall", "MRT"), m.flag("Solid", "MRT")]
+        for b in ("EPressure", "WPressure", "NVelocity", "SVelocity", "WVelocity", "EVelocity", "NSymmetry",
+                  "SSymmetry"):
+            kinds += [m.flag(b, "MRT")]
+        kinds += [m.flag("Inlet", "MRT"), m.flag("Outlet", "MRT"), m.flag("WVelocity", "MRT", "Inlet"),
+                  m.flag("EPressure", "MRT", "Outlet"), m.flag("NSymmetry"), 0]
+        w = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+        settings = [("Viscosity", 0.04), ("S3", 0.15), ("S4", -0.1), ("GravitationX", 1e-4), ("GravitationY", -2e-5)]
+        zonal = []
+        for z in range(4):
+            zonal += [("VelocityX", z, 0.02 * rng.standard_normal()), ("VelocityY", z, 0.01 * rng.standard_normal()),
+                      ("Pressure", z, 0.002 * rng.standard_normal())]
+        zonal += [("PressureLossInObj", 1, 0.5), ("OutletFluxInObj", 2, -0.25), ("InletFluxInObj", -1, 0.125)]
+    elif model.startswith("d2q9"):
         BGK = m.flag("BGK")
         kinds = [BGK] * 12 + [m.flag("Wall"), m.flag("Solid"), m.flag("Wall", "BGK"), m.flag("Solid", "BGK"),
                              m.flag("WVelocity", "BGK"), m.flag("EVelocity", "BGK"), m.flag("WPressure", "BGK"),
@@ -90,13 +105,15 @@ def random_case(model, nx, ny, nz, seed, steps=10):
     return cases.Case("random", model, nx, ny, nz, flags, settings, zonal, init_f=f, steps=steps)
 
 
-MODELS = [("d2q9_SRT", 24, 10, 1), ("d2q9_SRT_drag", 37, 9, 1), ("d3q19", 13, 7, 6), ("d3q19_lid", 11, 9, 7)]
+MODELS = [("d2q9_SRT", 24, 10, 1), ("d2q9_SRT_drag", 37, 9, 1), ("d3q19", 13, 7, 6), ("d3q19_lid", 11, 9, 7),
+          ("d2q9", 29, 11, 1)]
 
 
 @pytest.mark.parametrize("model,nx,ny,nz", MODELS)
 def test_random_case_f64(oracle_mod, model, nx, ny, nz):
     say(f"random case {model} {nx}x{ny}x{nz} f64")
-    case = random_case(model, nx, ny, nz, seed={"d2q9_SRT": 11, "d2q9_SRT_drag": 12, "d3q19": 13, "d3q19_lid": 14}[model])
+    case = random_case(model, nx, ny, nz,
+                       seed={"d2q9_SRT": 11, "d2q9_SRT_drag": 12, "d3q19": 13, "d3q19_lid": 14, "d2q9": 15}[model])
     lat = case.build("f64")
     orc = oracle_for_case(oracle_mod, case, "f64")
     for n in (1, 6, 13):

@@ -556,3 +556,186 @@ def test_lid_conserves_mass(O):
     m0 = L.get_f().sum()
     L.iterate(500)
     assert abs(L.get_f().sum() - m0) < 1e-11 * m0
+
+
+# ------------------------------------------------------------ d2q9 MRT ---
+def _d2q9m_basis_from_recipe():
+    """The moment matrix of models/flow/d2q9/Dynamics.c.Rt:11-20 restated
+    numerically: monomial moments ex^a ey^b ordered by a+b (src/lib/feq.R
+    MRT_polyMatrix), weights from the rest equilibrium, Cholesky factor of
+    M^-1 diag(1/w) M^-T taken on the reversed order, rows rescaled."""
+    U = np.stack([D2_EX, D2_EY], 1)
+    p = np.where(U < 0, 2, U)
+    p = p[np.argsort(p.sum(1), kind="stable")]
+    W = np.stack([np.prod(U.astype(float) ** pp, axis=1) for pp in p], 1)
+    req0 = np.array([np.prod([1 / 3 if e == 2 else (0.0 if e == 1 else 1.0) for e in pp]) for pp in p])
+    w = req0 @ np.linalg.inv(W)
+    A = np.linalg.inv(W) @ np.diag(1 / w) @ np.linalg.inv(W.T)
+    r = np.arange(9)[::-1]
+    H = np.linalg.cholesky(A[np.ix_(r, r)]).T[np.ix_(r, r)]
+    H = H * np.array([1, 3 ** .5 / 3, 3 ** .5 / 3, 2 ** .5, 2 ** .5, 1, 6 ** .5 / 3, 6 ** .5 / 3, 2])[:, None]
+    B = W @ H.T
+    order = [max(p.sum(1)[np.abs(np.linalg.inv(W) @ B[:, k]) > 1e-10]) for k in range(9)]
+    return B, np.array(order), w
+
+
+def _d2q9m_collide(O, f, params):
+    L = O.OracleLattice("d2q9", 1, 1, 1)
+    L.set_params(params)
+    L.set_zone_table(np.zeros((1, 6)))
+    L.set_flags(np.array([0x10], np.uint16))
+    L.set_f(np.asarray(f).reshape(9, 1, 1, 1))
+    g = L.iterate(1)
+    return L.get_f().reshape(9), g
+
+
+def _feq_std(rho, jx, jy):
+    w = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+    eu = D2_EX * jx + D2_EY * jy
+    return w * (rho + 3 * eu + 4.5 * eu ** 2 / rho - 1.5 * (jx * jx + jy * jy) / rho)
+
+
+def test_d2q9_mrt_relaxes_each_moment_order_at_its_rate(O):
+    """In the basis the recipe produces, the collision keeps rho, adds rho G
+    to J, relaxes the non-equilibrium part of the 2nd, 3rd and 4th order
+    moments by S2, S3, S4 and rebuilds around the equilibrium at the new J;
+    the equilibrium moments are those of the standard second-order feq."""
+    B, order, w = _d2q9m_basis_from_recipe()
+    assert np.allclose(B, np.round(B), atol=1e-12)            # an integer basis
+    assert list(order) == [0, 1, 1, 2, 2, 2, 3, 3, 4]
+    assert np.allclose(w, [4 / 9] + [1 / 9] * 4 + [1 / 36] * 4, atol=1e-15)
+    rng = np.random.default_rng(21)
+    S = {2: 0.37, 3: -0.21, 4: 0.55}
+    G = (2e-4, -1e-4)
+    for _ in range(5):
+        f = _random_f(rng, 9, 0.05)
+        g, _ = _d2q9m_collide(O, f, [S[2], S[3], S[4], *G])
+        m0, m1 = B.T @ f, B.T @ g
+        rho, J = m0[0], m0[1:3]
+        Jn = J + rho * np.array(G)
+        eq0, eq1 = B.T @ _feq_std(rho, *J), B.T @ _feq_std(rho, *Jn)
+        assert abs(m1[0] - rho) < 1e-15 and np.allclose(m1[1:3], Jn, atol=1e-15)
+        for k in range(3, 9):
+            assert abs(m1[k] - (S[order[k]] * (m0[k] - eq0[k]) + eq1[k])) < 1e-14, k
+
+
+def test_d2q9_mrt_equal_rates_is_bgk(O):
+    """S2 = S3 = S4 = 1 - omega: f' = feq(rho, J + rho G) + (1 - omega)(f - feq(rho, J))."""
+    rng = np.random.default_rng(22)
+    om, G = 1.37, (1e-4, 3e-4)
+    for _ in range(5):
+        f = _random_f(rng, 9, 0.05)
+        g, _ = _d2q9m_collide(O, f, [1 - om] * 3 + list(G))
+        rho, jx, jy = f.sum(), (f * D2_EX).sum(), (f * D2_EY).sum()
+        want = _feq_std(rho, jx + rho * G[0], jy + rho * G[1]) + (1 - om) * (f - _feq_std(rho, jx, jy))
+        assert np.max(np.abs(g - want)) < 1e-15
+
+
+@pytest.mark.parametrize("side", ["W", "E", "N", "S"])
+@pytest.mark.parametrize("kind", ["Velocity", "Pressure"])
+def test_d2q9_mrt_zou_he_solves_the_reference_equations(O, side, kind):
+    """ZouHeRewrite (src/lib/boundary.R:155-207): the populations entering
+    through the side become feq(R) + f_opp - feq_opp(R); R is the solution of
+    the velocity equations (sum fs e = V sum fs) or the pressure equations
+    (sum fs = 1 + 3P, no tangential momentum) -- solved here with numpy."""
+    flags = {"EPressure": 1, "WPressure": 2, "NVelocity": 3, "SVelocity": 4, "WVelocity": 5, "EVelocity": 6}
+    name = side + kind
+    if name not in flags:
+        pytest.skip("the model has no " + name)
+    n = {"W": (-1, 0), "E": (1, 0), "N": (0, 1), "S": (0, -1)}[side]
+    rng = np.random.default_rng(23)
+    f = _random_f(rng, 9, 0.05)
+    V, P = (0.031, -0.012), 0.004
+    g = _node_after(O, "d2q9", flags[name], f, [0, 0, 0, 0, 0], [V[0], V[1], P, 0, 0, 0])
+    inc = (D2_EX * n[0] + D2_EY * n[1]) < 0
+    opp = np.array([0, 3, 4, 1, 2, 7, 8, 5, 6])
+
+    def fs_of(R):
+        fs = f.copy()
+        fs[inc] = (_feq_std(1.0, *R) + f[opp] - _feq_std(1.0, *R)[opp])[inc]
+        return fs
+
+    def eqs(R):
+        fs = fs_of(R)
+        mom = np.array([(fs * D2_EX).sum(), (fs * D2_EY).sum()])
+        if kind == "Velocity":
+            return mom - np.array(V) * fs.sum()
+        a = 0 if n[0] else 1
+        e = mom.copy()
+        e[a] = fs.sum() - (1 + 3 * P)
+        return e
+
+    b = eqs(np.zeros(2))
+    A = np.stack([eqs(np.eye(2)[0]) - b, eqs(np.eye(2)[1]) - b], 1)
+    R = np.linalg.solve(A, -b)
+    assert np.max(np.abs(g - fs_of(R))) < 1e-15
+    assert np.array_equal(g[~inc], f[~inc])
+
+
+def test_d2q9_mrt_symmetry_and_bounce_back(O):
+    rng = np.random.default_rng(24)
+    f = _random_f(rng, 9, 0.05)
+    g = _node_after(O, "d2q9", 7, f, [0] * 5, [0] * 6)          # NSymmetry
+    assert np.array_equal(g[[4, 8, 7]], f[[2, 5, 6]]) and np.array_equal(g[[0, 1, 2, 3, 5, 6]], f[[0, 1, 2, 3, 5, 6]])
+    g = _node_after(O, "d2q9", 8, f, [0] * 5, [0] * 6)          # SSymmetry
+    assert np.array_equal(g[[2, 5, 6]], f[[4, 8, 7]])
+    g = _node_after(O, "d2q9", 10, f, [0] * 5, [0] * 6)         # Wall
+    assert np.array_equal(g, f[[0, 3, 4, 1, 2, 7, 8, 5, 6]])
+
+
+def test_d2q9_mrt_inlet_outlet_globals(O):
+    """On Outlet (Inlet) nodes CollisionMRT adds Jx/rho^2 to OutletFlux
+    (InletFlux) and -(+) Jx/rho^2 ((rho-1)/3 + |J|^2/rho^3/2) to
+    PressureLoss, before relaxation (Dynamics.c.Rt:221-236)."""
+    rng = np.random.default_rng(25)
+    f = _random_f(rng, 9, 0.05)
+    rho, jx, jy = f.sum(), (f * D2_EX).sum(), (f * D2_EY).sum()
+    loss = jx / rho / rho * ((rho - 1) / 3 + (jx * jx + jy * jy) / rho ** 2 / rho / 2)
+    for flag, k, sign in ((0x10 | 0x40, 1, -1), (0x10 | 0x20, 2, 1)):
+        L = O.OracleLattice("d2q9", 1, 1, 1)
+        L.set_params([0.3, 0, 0, 0, 0])
+        L.set_zone_table(np.array([[0, 0, 0, 0.5, 0.25, 0.125]]))
+        L.set_flags(np.array([flag], np.uint16))
+        L.set_f(f.reshape(9, 1, 1, 1))
+        g = L.iterate(1)
+        assert abs(g[k] - jx / rho / rho) < 1e-16 and abs(g[0] - sign * loss) < 1e-16
+        assert g[3] == 0.5 + [0, 0.25, 0.125][k]     # Objective: the zone's InObj weights
+
+
+def test_d2q9_mrt_init(O):
+    """Init: the standard equilibrium at rho = 1 + 3 Pressure and velocity
+    (VelocityX, VelocityY) of the node's zone; Wall nodes at rest, rho = 1."""
+    L = O.OracleLattice("d2q9", 2, 1, 1)
+    L.set_params([0] * 5)
+    L.set_zone_table(np.array([[0.02, -0.01, 0.003, 0, 0, 0]]))
+    L.set_flags(np.array([[0x10, 10]], np.uint16))
+    L.init()
+    f = L.get_f()[:, 0, 0]
+    rho = 1 + 3 * 0.003
+    assert np.max(np.abs(f[:, 0] - _feq_std(rho, 0.02 * rho, -0.01 * rho))) < 1e-16
+    assert np.max(np.abs(f[:, 1] - _feq_std(1.0, 0, 0))) < 1e-16
+
+
+@pytest.mark.parametrize("tau", [0.6, 0.8])
+def test_d2q9_mrt_poiseuille_exact_with_magic_rates(O, tau):
+    """A force-driven channel between bounce-back walls: when the 3rd-order
+    rate s3 satisfies (1/s2 - 1/2)(1/s3 - 1/2) = 3/16 (s = 1 - S), the MRT
+    steady state is the exact parabola with walls half-way between nodes,
+    u = g/(2 nu) (y - 1/2)(ny - 3/2 - y) (a rate given to a wrong moment
+    order or a misplaced force breaks this)."""
+    ny, g = 12, 1e-6
+    om = 1 / tau
+    nu = (tau - 0.5) / 3
+    s3 = 1 / ((3 / 16) / (1 / om - 0.5) + 0.5)
+    L = O.OracleLattice("d2q9", 1, ny, 1)
+    L.set_params([1 - om, 1 - s3, 0.0, g, 0])
+    L.set_zone_table(np.zeros((1, 6)))
+    fl = np.full((1, ny, 1), 0x10, np.uint16)
+    fl[0, 0, 0] = fl[0, ny - 1, 0] = 10
+    L.set_flags(fl)
+    L.init()
+    L.iterate(40000)
+    u = L.quantity("U")[0, 1:-1, 0, 0]
+    y = np.arange(1, ny - 1)
+    want = g / (2 * nu) * (y - 0.5) * (ny - 1.5 - y)
+    assert np.max(np.abs(u - want)) / want.max() < 1e-9
