@@ -439,7 +439,7 @@ struct CbSaveBinary : Callback {
     std::string fn;
     int init() override {
         Callback::init();
-        if (const XmlAttr* a = node->attr("file")) fn = solver->outpath + "_" + a->value;
+        if (const XmlAttr* a = node->attr("file")) fn = solver->out_prefix() + "_" + a->value;
         else if (const XmlAttr* b = node->attr("filename")) fn = b->value;
         else fn = solver->out_iter_file("Save", "");
         return 0;
