@@ -122,6 +122,12 @@ int tclbx_bytes_per_node(const tclbx_lattice* h, double* bytes);
 int tclbx_set_block(tclbx_lattice* h, int block);
 /* Steps per captured hipGraph replay in tclbx_iterate (0 = no graph). */
 int tclbx_set_graph_steps(tclbx_lattice* h, int steps);
+/* Rows walked by one block of a plain step: 1, 2 or 4; 0 (the default) times
+ * the three on the first long tclbx_iterate of a lattice of 2^20 nodes or
+ * more and keeps the fastest (the state is not changed).  A launch-shape
+ * choice of this engine; the reference's shape is fixed (32x4 threads). */
+int tclbx_set_rows_per_tile(tclbx_lattice* h, int rows);
+int tclbx_get_rows_per_tile(const tclbx_lattice* h, int* rows);
 
 /* ---------------------------------------------------------------- batch --
  * Many independent lattices ("cases") of one model and storage format in

@@ -33,6 +33,7 @@ SYMBOLS = [
     "tclbx_iterate", "tclbx_get_globals", "tclbx_get_iteration", "tclbx_get_quantity",
     "tclbx_get_density", "tclbx_set_density", "tclbx_save", "tclbx_load", "tclbx_get_timing",
     "tclbx_bytes_per_node", "tclbx_set_block", "tclbx_set_graph_steps", "tclbx_copy_bandwidth",
+    "tclbx_set_rows_per_tile", "tclbx_get_rows_per_tile",
     "tclbx_batch_create", "tclbx_batch_destroy", "tclbx_batch_case_bytes", "tclbx_batch_add_case",
     "tclbx_batch_remove_case", "tclbx_batch_set_setting", "tclbx_batch_set_zone_setting",
     "tclbx_batch_flag_overwrite", "tclbx_batch_init_case", "tclbx_batch_iterate", "tclbx_batch_get_globals",
@@ -100,6 +101,8 @@ def lib():
     L.tclbx_bytes_per_node.argtypes = [vp, dp]
     L.tclbx_set_block.argtypes = [vp, c_int]
     L.tclbx_set_graph_steps.argtypes = [vp, c_int]
+    L.tclbx_set_rows_per_tile.argtypes = [vp, c_int]
+    L.tclbx_get_rows_per_tile.argtypes = [vp, ip]
     L.tclbx_copy_bandwidth.argtypes = [ctypes.c_longlong, c_int, dp, dp]
     ll, llp = ctypes.c_longlong, ctypes.POINTER(ctypes.c_longlong)
     L.tclbx_batch_create.argtypes = [c_char_p, c_char_p, ll, ctypes.POINTER(vp)]
@@ -383,6 +386,18 @@ class Lattice:
 
     def set_graph_steps(self, steps):
         _check(lib().tclbx_set_graph_steps(self._h, int(steps)))
+
+    @property
+    def rows_per_tile(self):
+        """Rows walked by one block of a plain step (tuned on the first long
+        iterate unless set)."""
+        v = ctypes.c_int()
+        _check(lib().tclbx_get_rows_per_tile(self._h, ctypes.byref(v)))
+        return v.value
+
+    @rows_per_tile.setter
+    def rows_per_tile(self, rows):
+        _check(lib().tclbx_set_rows_per_tile(self._h, int(rows)))
 
 
 class Batch:

@@ -275,6 +275,21 @@ int tclbx_set_graph_steps(tclbx_lattice* h, int steps) {
     return TCLBX_OK;
 }
 
+int tclbx_set_rows_per_tile(tclbx_lattice* h, int rows) {
+    CHECK_H(h);
+    return guard([&]() {
+        h->lat->set_rows_per_tile(rows);
+        return TCLBX_OK;
+    });
+}
+
+int tclbx_get_rows_per_tile(const tclbx_lattice* h, int* rows) {
+    CHECK_H(h);
+    if (!rows) return fail("null output pointer");
+    *rows = h->lat->rows_per_tile();
+    return TCLBX_OK;
+}
+
 int tclbx_batch_create(const char* model, const char* storage, long long capacity_bytes, tclbx_batch** out) {
     return guard([&]() {
         if (!out) return fail("null output pointer");

@@ -183,7 +183,7 @@ StepArgs Lattice::args(int from, int to, bool glob) const {
     a.nx = nx; a.ny = ny; a.nz = nz;
     a.bpr = bpr_;
     a.rpb = rpb_;
-    a.rpt = 1;
+    a.rpt = rpt_;
     a.flat = 0;
     a.inv_nx = 1.0 / nx;
     a.zcols = zcols_;
@@ -286,9 +286,45 @@ void Lattice::Init() {
     clearGlobals();
 }
 
+void Lattice::set_rows_per_tile(int r) {
+    if (r != 0 && r != 1 && r != 2 && r != 4) throw std::invalid_argument("rows per tile must be 0, 1, 2 or 4");
+    tune_rpt_ = r == 0;
+    rpt_ = r == 0 ? 1 : r;
+    drop_graph();
+}
+
+// Times plain steps with 1, 2 and 4 rows per tile and keeps the fastest.
+// Every trial step reads the current snapshot and writes the other one,
+// which the next real step overwrites: the state does not change.  Which
+// is fastest depends on the model, the storage format and the occupancy
+// the kernel reaches (one row for FP64 d3q19, more for the 16-bit formats
+// of heavier node types).
+void Lattice::tune_rows() {
+    tune_rpt_ = false;
+    float best = 0;
+    int best_r = rpt_;
+    for (int r : {1, 2, 4}) {
+        if (r > 1 && (long long)ny * nz < 4LL * r) continue;
+        rpt_ = r;
+        launch(snap_, 1 - snap_, false);  // warm-up
+        hip_check(hipEventRecord(ev0_, stream_), "event record");
+        for (int k = 0; k < 3; k++) launch(snap_, 1 - snap_, false);
+        hip_check(hipEventRecord(ev1_, stream_), "event record");
+        hip_check(hipStreamSynchronize(stream_), "tune rows");
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, ev0_, ev1_), "event elapsed");
+        if (best == 0 || ms < best) { best = ms; best_r = r; }
+    }
+    rpt_ = best_r;
+    drop_graph();
+}
+
 void Lattice::Iterate(int niter, int iter_type) {
     if (niter < 0) throw std::invalid_argument("negative iteration count");
     upload_zone();
+    // lattices large enough for the tiling to matter are tuned once, on
+    // the first Iterate that runs enough steps to repay the trial launches
+    if (tune_rpt_ && niter >= 16 && N >= (1u << 20)) tune_rows();
     int last_glob = iter_type & ITER_LASTGLOB;
     if (iter_type & ITER_INTEG) last_glob = 0;
     const bool every_glob = (iter_type & ITER_GLOBS) != 0;

@@ -59,6 +59,10 @@ public:
     void set_block(int b);
     int block() const { return block_; }
     void set_graph_steps(int k) { graph_steps_ = k; drop_graph(); }
+    // rows of the lattice walked by one block of a plain step: 1, 2 or 4,
+    // or 0 to let the first long Iterate time the three and keep the fastest
+    void set_rows_per_tile(int r);
+    int rows_per_tile() const { return rpt_; }
 
     const ModelDef* model;
     const int nx, ny, nz, storage;
@@ -70,6 +74,7 @@ private:
     void upload_zone();
     void launch(int from, int to, bool glob);
     void drop_graph();
+    void tune_rows();
     void run_graph_steps(int nsteps);
 
     std::vector<double> settings_;
@@ -91,6 +96,8 @@ private:
     hipGraphExec_t graph_exec_ = nullptr;
     hipGraph_t graph_ = nullptr;
     int graph_steps_ = 32;
+    int rpt_ = 1;
+    bool tune_rpt_ = true;
     unsigned params_version_ = 0, graph_version_ = ~0u;
 };
 

@@ -132,7 +132,7 @@ def test_save_load_and_failcheck(tmp_path):
         <SaveBinary file="state" Iterations="20"/><Solve Iterations="20"/></CLBConfig>""", tmp_path, "a.xml")
     pri = tmp_path / "out" / "a_state_0.pri"
     f = np.fromfile(pri, dtype=np.float64).reshape(9, 16, 32)
-    assert np.allclose(f, a.lattice.get_f(), rtol=0, atol=0)
+    assert np.array_equal(f, a.lattice.get_f().reshape(9, 16, 32))
     b = run_case("d2q9_SRT", f"""<CLBConfig output="out/">{geom}
         <Model><Param name="nu" value="0.1"/><Param name="GravitationX" value="1e-5"/></Model>
         <LoadBinary file="{tmp_path}/out/a_state"/><Solve Iterations="30"/></CLBConfig>""", tmp_path, "b.xml")

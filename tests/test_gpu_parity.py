@@ -347,3 +347,25 @@ def test_cavity_reduced_storage_tracks_f64(storage):
     assert l2 < bound, l2
     t0, t1 = ref.global_dict()["MovingWallTorque"], lat.global_dict()["MovingWallTorque"]
     assert abs(t1 - t0) <= (0.25 if storage == "bf16s" else 0.05) * abs(t0), (t0, t1)
+
+
+@pytest.mark.parametrize("storage", ["f64", "f16s"])
+def test_rows_per_tile_and_tuning_change_nothing(storage):
+    """1, 2 or 4 rows per tile, or the first-Iterate tuning that times them
+    on the current snapshot, give bit-identical states and Globals."""
+    say(f"rows per tile, {storage}")
+    case = cases.d3q19_cavity(n=104, u_lid=0.05, tau=0.52)    # 1.12M nodes: large enough to be tuned
+    runs = []
+    for r in (0, 1, 2, 4):
+        lat = case.build(storage)
+        lat.rows_per_tile = r
+        lat.iterate(40)
+        lat.iterate(7)
+        if r == 0:
+            assert lat.rows_per_tile in (1, 2, 4)
+        else:
+            assert lat.rows_per_tile == r
+        runs.append((lat.get_f(), lat.globals))
+        lat.close()
+    for f, g in runs[1:]:
+        assert np.array_equal(f, runs[0][0]) and np.array_equal(g, runs[0][1])
