@@ -67,17 +67,19 @@ def cpu_baseline(n, seconds, threads):
 
 
 def batched_section(steps, seq_cases):
-    """The 256-case d2q9 cylinder sweep in one batch (FP16-shift storage,
-    FP32 collision, autotuned tiles) against the same cases solved one at a
-    time as FP64 lattices (a sample of `seq_cases`, scaled by nodes)."""
+    """The 256-case d2q9 cylinder sweep (d2q9 MRT with Drag/Lift) in one
+    batch (FP16-shift storage, FP32 collision, autotuned tiles) against the
+    same cases solved one at a time as FP64 lattices (a sample of
+    `seq_cases`, scaled by nodes)."""
     import tclb_amd
     from tclb_amd import cases
-    sweep = cases.d2q9_cylinder_sweep(ncases=256, steps=steps)
+    model = "d2q9_drag"
+    sweep = cases.d2q9_cylinder_sweep(ncases=256, steps=steps, model=model)
     nodes = sum(c.nodes for c in sweep)
-    probe = tclb_amd.Batch("d2q9_SRT_drag", "f16s", 1 << 20)
+    probe = tclb_amd.Batch(model, "f16s", 1 << 20)
     cap = sum(probe.case_bytes(c.nx, c.ny) for c in sweep) + (16 << 20)
     probe.close()
-    b = tclb_amd.Batch("d2q9_SRT_drag", "f16s", capacity_bytes=cap)
+    b = tclb_amd.Batch(model, "f16s", capacity_bytes=cap)
     for c in sweep:
         b.add(c)
     tune = b.autotune(steps=5)
@@ -96,7 +98,7 @@ def batched_section(steps, seq_cases):
     seq_mlups = seq_nodes * steps / seq_t / 1e6
     batch_mlups = nodes * steps / wall / 1e6
     return {
-        "workload": f"256 d2q9 cylinder cases (nx 128-1024, ny 64-256, Re 20-200), {nodes} nodes, "
+        "workload": f"256 d2q9 (MRT) cylinder cases (nx 128-1024, ny 64-256, Re 20-200), {nodes} nodes, "
                     f"{steps} steps timed, FP16-shift storage",
         "MLUPS": round(batch_mlups, 1),
         "solves_per_s_8000_steps": round(256 / (wall * 8000 / steps), 2),

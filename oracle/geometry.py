@@ -231,6 +231,7 @@ NODETYPES["d2q9"] = [("EPressure", "BOUNDARY"), ("WPressure", "BOUNDARY"), ("NVe
                      ("NSymmetry", "BOUNDARY"), ("SSymmetry", "BOUNDARY"), ("Inlet", "OBJECTIVE"),
                      ("Outlet", "OBJECTIVE"), ("Solid", "BOUNDARY"), ("Wall", "BOUNDARY"), ("MRT", "COLLISION")]
 NODETYPES["d2q9_SRT_drag"] = NODETYPES["d2q9_SRT"]
+NODETYPES["d2q9_drag"] = NODETYPES["d2q9"]
 NODETYPES["d3q19_lid"] = NODETYPES["d3q19"][:9] + [("MovingWall", "BOUNDARY"), ("MRT", "COLLISION")]
 
 

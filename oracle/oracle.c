@@ -161,7 +161,8 @@ orc_lattice* orc_create(int model, int nx, int ny, int nz, int storage)
             L->w[q] = d3q19_w[q];
         }
     } else {
-        L->Q = 9; L->nglob = model == ORC_D2Q9_SRT_DRAG ? 3 : (model == ORC_D2Q9 ? 4 : 1);
+        L->Q = 9;
+        L->nglob = model == ORC_D2Q9_SRT_DRAG ? 3 : (model == ORC_D2Q9 ? 4 : (model == ORC_D2Q9_DRAG ? 6 : 1));
         for (q = 0; q < 9; q++) {
             for (a = 0; a < 3; a++) L->e[q][a] = d2q9_e[q][a];
             L->w[q] = d2q9_w[q];
@@ -208,7 +209,8 @@ void orc_set_flags(orc_lattice* L, const uint16_t* flags)
 }
 
 static int is3d(const orc_lattice* L) { return L->model == ORC_D3Q19 || L->model == ORC_D3Q19_LID; }
-static int zone_shift(const orc_lattice* L) { return is3d(L) ? 6 : (L->model == ORC_D2Q9 ? 7 : 4); }
+static int is_d2q9m(const orc_lattice* L) { return L->model == ORC_D2Q9 || L->model == ORC_D2Q9_DRAG; }
+static int zone_shift(const orc_lattice* L) { return is3d(L) ? 6 : (is_d2q9m(L) ? 7 : 4); }
 
 static const double* zone_row(const orc_lattice* L, uint16_t nt)
 {
@@ -234,7 +236,7 @@ void orc_init(orc_lattice* L)
                 d3q19_feq_f(f, 1.0f, (float)L->params[1], 0.f, 0.f);
                 for (q = 0; q < 19; q++) st(L, L->buf[0], (size_t)q * L->N + n, q, f[q]);
             }
-        } else if (L->model == ORC_D2Q9) {
+        } else if (is_d2q9m(L)) {
             /* Init(), models/flow/d2q9/Dynamics.c.Rt:89-102: rho = 1 + 3 Pressure,
              * u = (VelocityX, VelocityY); Wall nodes at rest with rho = 1 */
             int wall = (L->flags[n] & 0xF) == 10;
@@ -298,14 +300,14 @@ static void step_rows(orc_lattice* L, long double* glob, long r0, long r1)
                 pull(L, in, x, y, z, fd);
                 if (L->storage == ORC_F64) {
                     if (is3d(L)) d3q19_node_d(fd, nt, L->params, zr, glob, lid, L->flags, x, y, z, L->nx, L->ny, L->nz);
-                    else if (L->model == ORC_D2Q9) d2q9m_node_d(fd, nt, L->params, zr, glob);
+                    else if (is_d2q9m(L)) d2q9m_node_d(fd, nt, L->params, zr, glob, L->model == ORC_D2Q9_DRAG);
                     else d2q9_node_d(fd, nt, L->params[0], L->params[1], zr, drag, glob);
                     for (q = 0; q < L->Q; q++) st(L, out, (size_t)q * L->N + n, q, fd[q]);
                 } else {
                     float ff[19];
                     for (q = 0; q < L->Q; q++) ff[q] = (float)fd[q];
                     if (is3d(L)) d3q19_node_f(ff, nt, L->params, zr, glob, lid, L->flags, x, y, z, L->nx, L->ny, L->nz);
-                    else if (L->model == ORC_D2Q9) d2q9m_node_f(ff, nt, L->params, zr, glob);
+                    else if (is_d2q9m(L)) d2q9m_node_f(ff, nt, L->params, zr, glob, L->model == ORC_D2Q9_DRAG);
                     else d2q9_node_f(ff, nt, (float)L->params[0], (float)L->params[1], zr, drag, glob);
                     for (q = 0; q < L->Q; q++) st(L, out, (size_t)q * L->N + n, q, ff[q]);
                 }
@@ -404,7 +406,7 @@ void orc_get_quantity(const orc_lattice* L, int which, double* out)
                         }
                         out[3*n] = ux; out[3*n+1] = uy; out[3*n+2] = uz;
                     }
-                } else if (L->model == ORC_D2Q9) {
+                } else if (is_d2q9m(L)) {
                     /* getRho / getU, models/flow/d2q9/Dynamics.c.Rt:46-63 */
                     double d = 0, jx = 0, jy = 0;
                     for (q = 0; q < 9; q++) { d += f[q]; jx += f[q] * L->e[q][0]; jy += f[q] * L->e[q][1]; }

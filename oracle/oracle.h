@@ -14,6 +14,7 @@
  *                  src/lib/d3q19.R:24-96
  *   d2q9           models/flow/d2q9/Dynamics.c.Rt, Dynamics.R with the basis of
  *                  src/lib/feq.R and the Zou-He of src/lib/boundary.R:155-207
+ *   d2q9_drag      d2q9 plus Drag/Lift on Solid nodes (as d2q9_SRT_drag)
  *   d3q19_lid      d3q19 plus a MovingWall node type (bounce-back with Ladd's
  *                  wall-velocity term) and its force/torque Globals; the
  *                  reference has no moving wall, so this part is pinned by
@@ -38,7 +39,8 @@
 extern "C" {
 #endif
 
-enum { ORC_D2Q9_SRT = 0, ORC_D2Q9_SRT_DRAG = 1, ORC_D3Q19 = 2, ORC_D3Q19_LID = 3, ORC_D2Q9 = 4 };
+enum { ORC_D2Q9_SRT = 0, ORC_D2Q9_SRT_DRAG = 1, ORC_D3Q19 = 2, ORC_D3Q19_LID = 3, ORC_D2Q9 = 4,
+       ORC_D2Q9_DRAG = 5 };
 enum { ORC_F64 = 0, ORC_F32 = 1, ORC_F32S = 2, ORC_F16S = 3, ORC_BF16S = 4 };
 
 /* d2q9 parameter vector:   [omega, Density]

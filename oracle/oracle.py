@@ -14,10 +14,10 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 
-D2Q9_SRT, D2Q9_SRT_DRAG, D3Q19, D3Q19_LID, D2Q9 = 0, 1, 2, 3, 4
+D2Q9_SRT, D2Q9_SRT_DRAG, D3Q19, D3Q19_LID, D2Q9, D2Q9_DRAG = 0, 1, 2, 3, 4, 5
 F64, F32, F32S, F16S, BF16S = 0, 1, 2, 3, 4
 MODELS = {"d2q9_SRT": D2Q9_SRT, "d2q9_SRT_drag": D2Q9_SRT_DRAG, "d3q19": D3Q19, "d3q19_lid": D3Q19_LID,
-          "d2q9": D2Q9}
+          "d2q9": D2Q9, "d2q9_drag": D2Q9_DRAG}
 STORAGE = {"f64": F64, "f32": F32, "f32s": F32S, "f16s": F16S, "bf16s": BF16S}
 
 _lib = None
@@ -194,7 +194,7 @@ def lattice_for_case(case, storage="f64"):
         L.set_params([vals[n] for n in D3Q19_PARAMS])
     elif case.model == "d3q19_lid":
         L.set_params([vals[n] for n in D3Q19_LID_PARAMS])
-    elif case.model == "d2q9":
+    elif case.model in ("d2q9", "d2q9_drag"):
         L.set_params([vals[n] for n in D2Q9_PARAMS])
     else:
         L.set_params([vals["omega"], vals["Density"]])

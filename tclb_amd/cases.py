@@ -112,17 +112,21 @@ def parabola(y, lo, hi, umax):
 
 def d2q9_cylinder(nx=1024, ny=128, diameter=32, cx=256, cy=64, umax=0.01, nu=0.02, steps=10000,
                   seed=1, perturb=1e-4, model="d2q9_SRT_drag"):
-    """2D channel with a disc: BGK everywhere in the fluid, no-slip
-    bounce-back walls (mask ALL, no collision) on rows 0 and ny-1, Zou-He
-    velocity inlet at x = 0 with a parabolic profile (one zone per row
-    carrying Velocity_x), Zou-He pressure outlet (Density = 1) at x = nx-1,
-    and a Solid disc (mask ALL) of the given diameter, rasterized as
-    OffgridSphere does: (x+.5-cx)^2 + (y+.5-cy)^2 < r^2
-    (src/Geometry.cpp.Rt:793-834).  Initial state: equilibrium at rho = 1,
-    u = the inlet profile, u_y perturbed uniformly in +-perturb (splitmix64,
-    `seed`); walls and disc at rest."""
+    """2D channel with a disc: BGK (d2q9_SRT models) or MRT (d2q9 models)
+    everywhere in the fluid, no-slip bounce-back walls (mask ALL, no
+    collision) on rows 0 and ny-1, Zou-He velocity inlet at x = 0 with a
+    parabolic profile (one zone per row carrying Velocity_x, or VelocityX
+    for d2q9), Zou-He pressure outlet (rho = 1) at x = nx-1, and a Solid
+    disc (mask ALL) of the given diameter, rasterized as OffgridSphere does:
+    (x+.5-cx)^2 + (y+.5-cy)^2 < r^2 (src/Geometry.cpp.Rt:793-834).  Initial
+    state: equilibrium at rho = 1, u = the inlet profile, u_y perturbed
+    uniformly in +-perturb (splitmix64, `seed`); walls and disc at rest.
+    With the d2q9 models the viscosity sets S2 through RelaxationRate and
+    the 3rd/4th order moments relax fully (S3 = S4 = 0, the model's
+    defaults)."""
     m = Model(model)
-    BGK = m.nodetype("BGK")[0]
+    mrt = not model.startswith("d2q9_SRT")
+    BGK = m.nodetype("MRT" if mrt else "BGK")[0]
     WALL, SOLID = m.nodetype("Wall")[0], m.nodetype("Solid")[0]
     WVEL, EPRES = m.nodetype("WVelocity")[0], m.nodetype("EPressure")[0]
     zshift = m.group("SETTINGZONE")[1]
@@ -131,7 +135,7 @@ def d2q9_cylinder(nx=1024, ny=128, diameter=32, cx=256, cy=64, umax=0.01, nu=0.0
     zonal = []
     for j in range(1, ny - 1):
         flags[0, j, 0] = BGK | WVEL | (j << zshift)
-        zonal.append(("Velocity_x", j, parabola(j, 0.5, ny - 1.5, umax)))
+        zonal.append(("VelocityX" if mrt else "Velocity_x", j, parabola(j, 0.5, ny - 1.5, umax)))
         flags[0, j, nx - 1] = BGK | EPRES
     X, Y = np.meshgrid(np.arange(nx), y)
     r = diameter / 2.0
@@ -145,8 +149,9 @@ def d2q9_cylinder(nx=1024, ny=128, diameter=32, cx=256, cy=64, umax=0.01, nu=0.0
     ux[solid] = 0.0
     uy[solid] = 0.0
     init_f = d2q9_feq(1.0, ux, uy)[:, None]
-    return Case("d2q9_cylinder", model, nx, ny, 1, flags,
-                settings=[("nu", nu), ("Density", 1.0)], zonal=zonal, init_f=init_f, steps=steps)
+    settings = [("Viscosity", nu)] if mrt else [("nu", nu), ("Density", 1.0)]
+    return Case("d2q9_cylinder", model, nx, ny, 1, flags, settings=settings, zonal=zonal, init_f=init_f,
+                steps=steps)
 
 
 def random_spheres(nx, ny, nz, fraction, rmin, rmax, seed, periodic=(True, False, True)):
@@ -219,12 +224,15 @@ def d3q19_cavity(n=640, u_lid=0.05, tau=0.52, steps=20000, ny=None, nz=None):
     return Case(f"d3q19_cavity_{n}x{ny}x{nz}", "d3q19_lid", n, ny, nz, flags, settings=settings, steps=steps)
 
 
-def d2q9_cylinder_sweep(ncases=256, seed=4, steps=8000, umax=0.02):
+def d2q9_cylinder_sweep(ncases=256, seed=4, steps=8000, umax=0.02, model="d2q9_drag"):
     """Independent cylinder cases of uneven size: nx uniform in
     {128, 144, ..., 1024}, ny uniform in {64, 80, ..., 256}, a Solid disc of
     diameter ny/4 at (nx/4, ny/2), Reynolds number uniform in [20, 200] with
     nu = umax * D / Re (umax the peak of the parabolic inlet); case k draws
-    from splitmix64 stream seed*1000 + k."""
+    from splitmix64 stream seed*1000 + k.  The default model is the MRT
+    d2q9 (with Drag/Lift): BGK (d2q9_SRT_drag) with the Zou-He velocity
+    inlet is unstable below tau of about 0.55, which most of this Re range
+    reaches."""
     out = []
     for k in range(ncases):
         st = Stream(seed * 1000 + k)
@@ -234,7 +242,7 @@ def d2q9_cylinder_sweep(ncases=256, seed=4, steps=8000, umax=0.02):
         d = ny / 4.0
         nu = umax * d / re
         c = d2q9_cylinder(nx=nx, ny=ny, diameter=d, cx=nx / 4.0, cy=ny / 2.0, umax=umax, nu=nu, steps=steps,
-                          seed=seed * 1000 + k)
+                          seed=seed * 1000 + k, model=model)
         c.name = f"cylinder_{k}_{nx}x{ny}_Re{re:.1f}"
         out.append(c)
     return out

@@ -46,6 +46,11 @@ template <> struct Lat<MODEL_D2Q9_SRT_DRAG> : Lat<MODEL_D2Q9_SRT> {
 template <> struct Lat<MODEL_D2Q9> : Lat<MODEL_D2Q9_SRT> {
     static constexpr int G = 4, SUMG = 4, ZSHIFT = 7;
 };
+// d2q9 + Drag, Lift (Globals 3, 4; Objective 5)
+template <> struct Lat<MODEL_D2Q9_DRAG> : Lat<MODEL_D2Q9> {
+    static constexpr int G = 6, SUMG = 6;
+};
+template <int MODEL> constexpr bool is_d2q9m = MODEL == MODEL_D2Q9 || MODEL == MODEL_D2Q9_DRAG;
 template <> struct Lat<MODEL_D3Q19> {
     static constexpr int Q = 19, G = 26, SUMG = 25, ZSHIFT = 6;
     __host__ __device__ static constexpr int ex(int i) {
@@ -76,6 +81,7 @@ int kernel_nglobals(int model) {
     case MODEL_D3Q19: return Lat<MODEL_D3Q19>::G;
     case MODEL_D3Q19_LID: return Lat<MODEL_D3Q19_LID>::G;
     case MODEL_D2Q9: return Lat<MODEL_D2Q9>::G;
+    case MODEL_D2Q9_DRAG: return Lat<MODEL_D2Q9_DRAG>::G;
     }
     return 0;
 }
@@ -283,13 +289,23 @@ __device__ __forceinline__ void d2q9m_symmetry(R* f) {
 
 // Run(), Dynamics.c.Rt:104-138 and CollisionMRT, :196-268.  Zone columns:
 // [VelocityX, VelocityY, Pressure, PressureLossInObj, OutletFluxInObj,
-// InletFluxInObj]; Globals [PressureLoss, OutletFlux, InletFlux, Objective].
-template <class R, bool GLOB>
+// InletFluxInObj, (d2q9_drag) DragInObj, LiftInObj]; Globals
+// [PressureLoss, OutletFlux, InletFlux, (DRAG) Drag, Lift, Objective].
+// DRAG adds the momentum exchange of Solid nodes, 2 sum_i f_i e_i of the
+// incoming populations (as d2q9_SRT_drag).
+template <class R, bool GLOB, bool DRAG>
 __device__ __forceinline__ void d2q9m_node(R* f, unsigned nt, const double* par, const double* zr, double* g) {
+    constexpr int OBJ = DRAG ? 5 : 3;
     const R Vx = (R)zr[0], Vy = (R)zr[1], P = (R)zr[2];
     switch (nt & 0xFu) {
     case 9:   // Solid
     case 10: {  // Wall: FullBounceBack
+        if (DRAG && GLOB && (nt & 0xFu) == 9u) {
+            const R fx = R(2.0) * (f[1] - f[3] + f[5] - f[6] - f[7] + f[8]);
+            const R fy = R(2.0) * (f[2] - f[4] + f[5] + f[6] - f[7] - f[8]);
+            g[3] += (double)fx; g[OBJ] += zr[6];
+            g[4] += (double)fy; g[OBJ] += zr[7];
+        }
         R t;
         t = f[1]; f[1] = f[3]; f[3] = t;
         t = f[2]; f[2] = f[4]; f[4] = t;
@@ -325,8 +341,8 @@ __device__ __forceinline__ void d2q9m_node(R* f, unsigned nt, const double* par,
             const R flux = Jx / rho / rho;
             const R loss = (Jx / rho) / rho * ((rho - R(1.)) / R(3.) + Usq / rho / R(2.));
             const int k = obj == 0x40u ? 1 : 2;
-            g[k] += (double)flux; g[3] += zr[3 + k];
-            g[0] += (double)(obj == 0x40u ? -loss : loss); g[3] += zr[3];
+            g[k] += (double)flux; g[OBJ] += zr[3 + k];
+            g[0] += (double)(obj == 0x40u ? -loss : loss); g[OBJ] += zr[3];
         }
     }
     const R S2 = (R)par[0], S3 = (R)par[1], S4 = (R)par[2];
@@ -900,8 +916,8 @@ __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, 
     if (active) {
         if constexpr (is_d3q19<MODEL>) {
             d3q19_node<R, GLOB, MODEL == MODEL_D3Q19_LID>(f, nt, a.par, zr, g, a.flags, xc, y, z, nx, ny, nz);
-        } else if constexpr (MODEL == MODEL_D2Q9) {
-            d2q9m_node<R, GLOB>(f, nt, a.par, zr, g);
+        } else if constexpr (is_d2q9m<MODEL>) {
+            d2q9m_node<R, GLOB, MODEL == MODEL_D2Q9_DRAG>(f, nt, a.par, zr, g);
         } else {
             d2q9_node<R, MODEL == MODEL_D2Q9_SRT_DRAG, GLOB>(f, nt, (R)a.par[0], (R)a.par[1], zr, g);
         }
@@ -1026,7 +1042,7 @@ __global__ void __launch_bounds__(256) k_init(StepArgs a) {
     R f[Q];
     if constexpr (is_d3q19<MODEL>) {
         d3q19_feq<R>(f, R(1.0), (R)a.par[1], R(0.), R(0.));
-    } else if constexpr (MODEL == MODEL_D2Q9) {
+    } else if constexpr (is_d2q9m<MODEL>) {
         // Init(), models/flow/d2q9/Dynamics.c.Rt:89-102
         R rho = R(1) + (R)zr[2] * R(3);
         R ux = (R)zr[0], uy = (R)zr[1];
@@ -1084,7 +1100,7 @@ __global__ void __launch_bounds__(256) k_quantity(StepArgs a, int qid, int x0, i
             if (MODEL == MODEL_D3Q19_LID && (nt & 0x1Cu) == 0x1Cu) { ux = a.par[12]; uy = a.par[13]; uz = a.par[14]; }
             out[3 * tid] = ux * scale; out[3 * tid + 1] = uy * scale; out[3 * tid + 2] = uz * scale;
         }
-    } else if constexpr (MODEL == MODEL_D2Q9) {
+    } else if constexpr (is_d2q9m<MODEL>) {
         // getRho / getU, Dynamics.c.Rt:46-63: Rho = 0, U = J/rho + G/2 = 1
         double d = 0, jx = 0, jy = 0;
 #pragma unroll
@@ -1269,6 +1285,7 @@ hipError_t launch_step(int model, int storage, bool globals, const StepArgs& a, 
     case MODEL_D3Q19: return step_m<MODEL_D3Q19>(storage, globals, a, block, s);
     case MODEL_D3Q19_LID: return step_m<MODEL_D3Q19_LID>(storage, globals, a, block, s);
     case MODEL_D2Q9: return step_m<MODEL_D2Q9>(storage, globals, a, block, s);
+    case MODEL_D2Q9_DRAG: return step_m<MODEL_D2Q9_DRAG>(storage, globals, a, block, s);
     }
     return hipErrorInvalidValue;
 }
@@ -1280,6 +1297,7 @@ hipError_t launch_init(int model, int storage, const StepArgs& a, int block, hip
     case MODEL_D3Q19: return init_m<MODEL_D3Q19>(storage, a, block, s);
     case MODEL_D3Q19_LID: return init_m<MODEL_D3Q19_LID>(storage, a, block, s);
     case MODEL_D2Q9: return init_m<MODEL_D2Q9>(storage, a, block, s);
+    case MODEL_D2Q9_DRAG: return init_m<MODEL_D2Q9_DRAG>(storage, a, block, s);
     }
     return hipErrorInvalidValue;
 }
@@ -1336,6 +1354,8 @@ hipError_t launch_batch_step(int model, int storage, bool globals, const BatchCa
     case MODEL_D3Q19_LID:
         return batch_m<MODEL_D3Q19_LID>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
     case MODEL_D2Q9: return batch_m<MODEL_D2Q9>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
+    case MODEL_D2Q9_DRAG:
+        return batch_m<MODEL_D2Q9_DRAG>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
     }
     return hipErrorInvalidValue;
 }
@@ -1350,6 +1370,8 @@ hipError_t launch_quantity(int model, int storage, int q, const StepArgs& a, int
     case MODEL_D3Q19_LID:
         return quantity_m<MODEL_D3Q19_LID>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
     case MODEL_D2Q9: return quantity_m<MODEL_D2Q9>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
+    case MODEL_D2Q9_DRAG:
+        return quantity_m<MODEL_D2Q9_DRAG>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
     }
     return hipErrorInvalidValue;
 }

@@ -215,7 +215,7 @@ void model_params(const ModelDef* model, const std::vector<double>& settings, do
         par[12] = S("MovingWallVelocityX");
         par[13] = S("MovingWallVelocityY");
         par[14] = S("MovingWallVelocityZ");
-    } else if (model->id == MODEL_D2Q9) {
+    } else if (model->id == MODEL_D2Q9 || model->id == MODEL_D2Q9_DRAG) {
         // [S2, S3, S4, GravitationX, GravitationY]: the relaxation factors of
         // the 2nd, 3rd and 4th order moments and the body force
         par[0] = S("S2");

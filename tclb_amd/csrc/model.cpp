@@ -192,11 +192,13 @@ ModelDef make_d3q19(bool lid) {
 // generates from src/lib/feq.R (models/flow/d2q9/Dynamics.c.Rt:1-40), stored
 // shifted by w_i, Zou-He conditions on all four sides, symmetry planes,
 // pressure-loss and flux Globals on Inlet/Outlet nodes
-// (models/flow/d2q9/Dynamics.R).
-ModelDef make_d2q9_mrt() {
+// (models/flow/d2q9/Dynamics.R).  With `drag` the model d2q9_drag: the
+// same plus Drag/Lift Globals by momentum exchange on Solid nodes, as
+// d2q9_SRT_drag adds them to d2q9_SRT.
+ModelDef make_d2q9_mrt(bool drag) {
     ModelDef m;
-    m.id = MODEL_D2Q9;
-    m.name = "d2q9";
+    m.id = drag ? MODEL_D2Q9_DRAG : MODEL_D2Q9;
+    m.name = drag ? "d2q9_drag" : "d2q9";
     m.Q = 9; m.dim = 2;
     m.ex = {0, 1, 0, -1, 0, 1, -1, -1, 1};
     m.ey = {0, 0, 1, 0, -1, 1, 1, -1, -1};
@@ -209,8 +211,13 @@ ModelDef make_d2q9_mrt() {
                   {"S2", 0.0, -1, 0},            {"S3", 0.0, -1, 0},
                   {"S4", 0.0, -1, 0},            {"Threshold", 0.5, -1, 0}};
     m.zonal = {{"VelocityX", 0, -1, 0}, {"VelocityY", 0, -1, 0}, {"Pressure", 0, -1, 0}};
-    finish_globals(m, {{"PressureLoss", false, "1mPa"}, {"OutletFlux", false, "1m2/s"},
-                       {"InletFlux", false, "1m2/s"}});
+    std::vector<GlobalDef> g = {{"PressureLoss", false, "1mPa"}, {"OutletFlux", false, "1m2/s"},
+                                {"InletFlux", false, "1m2/s"}};
+    if (drag) {
+        g.push_back({"Drag", false, "N"});
+        g.push_back({"Lift", false, "N"});
+    }
+    finish_globals(m, g);
     m.quantities = {{"Rho", false, "kg/m3"}, {"U", true, "m/s"}};
     pack_nodetypes(m, {{"EPressure", "BOUNDARY"}, {"WPressure", "BOUNDARY"}, {"NVelocity", "BOUNDARY"},
                        {"SVelocity", "BOUNDARY"}, {"WVelocity", "BOUNDARY"}, {"EVelocity", "BOUNDARY"},
@@ -222,7 +229,7 @@ ModelDef make_d2q9_mrt() {
 
 const std::vector<ModelDef>& registry() {
     static std::vector<ModelDef> r = {make_d2q9(false), make_d2q9(true), make_d3q19(false), make_d3q19(true),
-                                      make_d2q9_mrt()};
+                                      make_d2q9_mrt(false), make_d2q9_mrt(true)};
     return r;
 }
 

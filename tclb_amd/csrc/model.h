@@ -24,7 +24,8 @@ enum ModelId {
     MODEL_D3Q19 = 2,
     MODEL_D3Q19_LID = 3,
     MODEL_D2Q9 = 4,
-    MODEL_COUNT = 5
+    MODEL_D2Q9_DRAG = 5,
+    MODEL_COUNT = 6
 };
 
 // storage format of the distributions in HBM

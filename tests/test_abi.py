@@ -28,7 +28,7 @@ def test_every_declared_symbol_is_exported():
 
 
 def test_models():
-    assert tclb_amd.models() == ["d2q9_SRT", "d2q9_SRT_drag", "d3q19", "d3q19_lid", "d2q9"]
+    assert tclb_amd.models() == ["d2q9_SRT", "d2q9_SRT_drag", "d3q19", "d3q19_lid", "d2q9", "d2q9_drag"]
 
 
 def test_d2q9_mrt_contract():
@@ -101,6 +101,13 @@ def test_lid_model_extends_d3q19():
     assert gb[24:] == ["MovingWallFx", "MovingWallFy", "MovingWallFz", "MovingWallTorque", "Objective", "MaxV"]
     assert b.zonal_settings[24:] == ["MovingWallFxInObj", "MovingWallFyInObj", "MovingWallFzInObj",
                                      "MovingWallTorqueInObj"]
+
+
+def test_d2q9_drag_extends_d2q9():
+    a, b = tclb_amd.Model("d2q9"), tclb_amd.Model("d2q9_drag")
+    assert b.global_names == ["PressureLoss", "OutletFlux", "InletFlux", "Drag", "Lift", "Objective"]
+    assert b.zonal_settings == a.zonal_settings + ["DragInObj", "LiftInObj"]
+    assert b.settings == a.settings and b.group("SETTINGZONE") == a.group("SETTINGZONE")
 
 
 def test_drag_model_globals():

@@ -53,7 +53,7 @@ def random_case(model, nx, ny, nz, seed, steps=10):
     rng = np.random.default_rng(seed)
     m = tclb_amd.Model(model)
     zs = m.group("SETTINGZONE")[1]
-    if model == "d2q9":
+    if model in ("d2q9", "d2q9_drag"):
         MRT = m.flag("MRT")
         kinds = [MRT] * 12 + [m.flag("Wall"), m.flag("Solid"), m.flag("Wall", "MRT"), m.flag("Solid", "MRT")]
         for b in ("EPressure", "WPressure", "NVelocity", "SVelocity", "WVelocity", "EVelocity", "NSymmetry",
@@ -68,6 +68,8 @@ def random_case(model, nx, ny, nz, seed, steps=10):
             zonal += [("VelocityX", z, 0.02 * rng.standard_normal()), ("VelocityY", z, 0.01 * rng.standard_normal()),
                       ("Pressure", z, 0.002 * rng.standard_normal())]
         zonal += [("PressureLossInObj", 1, 0.5), ("OutletFluxInObj", 2, -0.25), ("InletFluxInObj", -1, 0.125)]
+        if model == "d2q9_drag":
+            zonal += [("DragInObj", 3, 0.75), ("LiftInObj", -1, -0.5)]
     elif model.startswith("d2q9"):
         BGK = m.flag("BGK")
         kinds = [BGK] * 12 + [m.flag("Wall"), m.flag("Solid"), m.flag("Wall", "BGK"), m.flag("Solid", "BGK"),
@@ -106,14 +108,15 @@ def random_case(model, nx, ny, nz, seed, steps=10):
 
 
 MODELS = [("d2q9_SRT", 24, 10, 1), ("d2q9_SRT_drag", 37, 9, 1), ("d3q19", 13, 7, 6), ("d3q19_lid", 11, 9, 7),
-          ("d2q9", 29, 11, 1)]
+          ("d2q9", 29, 11, 1), ("d2q9_drag", 23, 13, 1)]
 
 
 @pytest.mark.parametrize("model,nx,ny,nz", MODELS)
 def test_random_case_f64(oracle_mod, model, nx, ny, nz):
     say(f"random case {model} {nx}x{ny}x{nz} f64")
     case = random_case(model, nx, ny, nz,
-                       seed={"d2q9_SRT": 11, "d2q9_SRT_drag": 12, "d3q19": 13, "d3q19_lid": 14, "d2q9": 15}[model])
+                       seed={"d2q9_SRT": 11, "d2q9_SRT_drag": 12, "d3q19": 13, "d3q19_lid": 14, "d2q9": 15,
+                             "d2q9_drag": 16}[model])
     lat = case.build("f64")
     orc = oracle_for_case(oracle_mod, case, "f64")
     for n in (1, 6, 13):

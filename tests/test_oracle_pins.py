@@ -739,3 +739,36 @@ def test_d2q9_mrt_poiseuille_exact_with_magic_rates(O, tau):
     y = np.arange(1, ny - 1)
     want = g / (2 * nu) * (y - 0.5) * (ny - 1.5 - y)
     assert np.max(np.abs(u - want)) / want.max() < 1e-9
+
+
+def test_d2q9_drag_is_d2q9_plus_momentum_exchange(O):
+    """d2q9_drag evolves exactly as d2q9; on Solid nodes it adds
+    2 sum_i f_i e_i of the incoming populations to Drag/Lift (and the zone's
+    DragInObj/LiftInObj to Objective)."""
+    rng = np.random.default_rng(26)
+    nx, ny = 9, 7
+    kinds = np.array([0x10] * 6 + [9, 10, 0x10 | 0x20], np.uint16)
+    fl = kinds[rng.integers(0, len(kinds), nx * ny)].reshape(1, ny, nx)
+    w = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+    f0 = w[:, None, None, None] * (1 + 0.03 * rng.standard_normal((9, 1, ny, nx)))
+    runs = []
+    for model, ncol in (("d2q9", 6), ("d2q9_drag", 8)):
+        L = O.OracleLattice(model, nx, ny, 1)
+        L.set_params([0.3, 0.1, -0.2, 1e-4, 0])
+        L.set_zone_table(np.array([[0, 0, 0, 0.5, 0, 0, 0.25, 0.125][:ncol]]))
+        L.set_flags(fl)
+        L.init()
+        L.set_f(f0)
+        runs.append((L.iterate(1), L.get_f()))
+    (g0, f_a), (g1, f_b) = runs
+    assert np.array_equal(f_a, f_b)
+    assert np.array_equal(g0[:3], g1[:3])
+    # the pulled populations of the Solid nodes
+    ex = np.array([0, 1, 0, -1, 0, 1, -1, -1, 1]); ey = np.array([0, 0, 1, 0, -1, 1, 1, -1, -1])
+    fx = fy = 0.0
+    for y, x in zip(*np.nonzero(fl[0] == 9)):
+        fin = np.array([f0[q, 0, (y - ey[q]) % ny, (x - ex[q]) % nx] for q in range(9)])
+        fx += 2 * (fin * ex).sum(); fy += 2 * (fin * ey).sum()
+    assert abs(g1[3] - fx) < 1e-14 and abs(g1[4] - fy) < 1e-14
+    nsolid = int(np.count_nonzero(fl == 9))
+    assert abs(g1[5] - (g0[3] + nsolid * (0.25 + 0.125))) < 1e-12
