@@ -11,7 +11,13 @@ numpy over each primitive's box of nodes), what the reference computes:
     masks, modes, zones, nested regions, the primitives Box, Sphere,
     HalfSphere, OffgridSphere, OffgridPipe, XPipe, YPipe, ZPipe, XAnnulus,
     Pipe, PipeY, PipeZ, Cylinder, Wedge, Text and named zones, with the
-    defaults of src/def.cpp.Rt:10-33.
+    defaults of src/def.cpp.Rt:10-33;
+  * loadSTL / transformSTL / calcCut (src/Geometry.cpp.Rt:381-686): binary
+    STL meshes, side in/out by ray parity along ray_axis, side surface by
+    the 26 links of each node against each triangle (GaussSolve of
+    src/unit.cpp:13-49, batched over numpy arrays);
+  * loadSweep (src/Geometry.cpp.Rt:688-755) with the clamped B-spline of
+    src/spline.h:9-43.
 """
 import math
 import xml.etree.ElementTree as ET
@@ -306,6 +312,118 @@ class Node:
         return None
 
 
+# ---------------------------------------------------------------- STL ---
+_STL_REC = np.dtype([("n", "<f4", (3,)), ("p", "<f4", (3, 3)), ("v", "<i2")])
+_D26 = np.array([(x, y, z) for z in (0, 1, -1) for y in (0, 1, -1) for x in (0, 1, -1)][1:], np.float64)
+
+
+def stl_read(path):
+    """float32 vertices [ntri, 3, 3] of a binary STL file"""
+    data = open(path, "rb").read()
+    if data[:5] == b"solid":
+        raise ValueError("'STL' element %s is not in binary format!" % path)
+    ntri = int(np.frombuffer(data, "<i4", 1, 80)[0])
+    return np.frombuffer(data, _STL_REC, ntri, 84)["p"].copy()
+
+
+def stl_transform(P, attrib, u):
+    """transformSTL: X/Y/Z rotations, scale, shifts in double rounded to
+    float after each step, then a 1e-5 grid and a shift by -1/2 + a small
+    fixed offset per axis"""
+    f64, f32 = np.float64, np.float32
+    for d, (i1, i2) in enumerate([(1, 2), (2, 0), (0, 1)]):
+        k = "XYZ"[d] + "rot"
+        if k in attrib:
+            v = u.alt(attrib[k])
+            c, s = math.cos(v), math.sin(v)
+            y, z = P[:, :, i1].astype(f64), P[:, :, i2].astype(f64)
+            P[:, :, i1] = (c * y - s * z).astype(f32)
+            P[:, :, i2] = (s * y + c * z).astype(f32)
+    if "scale" in attrib:
+        P[:] = (u.alt(attrib["scale"]) * P.astype(f64)).astype(f32)
+    for j, k in enumerate("xyz"):
+        if k in attrib:
+            P[:, :, j] = (P[:, :, j].astype(f64) + u.alt(attrib[k])).astype(f32)
+    t = P.astype(f64) * 1e5
+    r = np.where(t > 0, np.trunc(t + 0.5), np.trunc(t - 0.5))   # myround
+    P[:] = (r * 1e-5).astype(f32)
+    sm = np.array([0.1403e-4, 0.1687e-4, 0.1987e-4]) - 0.5
+    P[:] = (P.astype(f64) + sm[None, None, :]).astype(f32)
+    return P
+
+
+def gauss4(A, b):
+    """GaussSolve on a batch of column-major 4x4 systems A[:, c*4+r]"""
+    A, b = A.copy(), b.copy()
+    z = lambda v: np.abs(v) < 1e-10
+    B = A.shape[0]
+    for k in range(3):
+        need = z(A[:, k * 4 + k])
+        for i in range(k + 1, 4):
+            sel = need & ~z(A[:, k * 4 + i])
+            if sel.any():
+                for j in range(4):
+                    t = A[sel, j * 4 + k].copy()
+                    A[sel, j * 4 + k] = A[sel, j * 4 + i]
+                    A[sel, j * 4 + i] = t
+                t = b[sel, k].copy()
+                b[sel, k] = b[sel, i]
+                b[sel, i] = t
+            need = need & ~sel
+        for i in range(k + 1, 4):
+            m = A[:, k * 4 + i] / A[:, k * 4 + k]
+            for j in range(4):
+                A[:, j * 4 + i] -= m * A[:, j * 4 + k]
+            b[:, i] -= m * b[:, k]
+    x = np.zeros((B, 4))
+    for i in range(3, -1, -1):
+        s = np.zeros(B)
+        for j in range(3, i, -1):
+            s += A[:, j * 4 + i] * x[:, j]
+        x[:, i] = (b[:, i] - s) / A[:, i * 4 + i]
+    return x
+
+
+def stl_cut(P, xyz, d):
+    """calcCut: does the link from node xyz along d cross triangle P
+    (batched: P [B,3,3] float32, xyz and d [B,3])?  NaNs (a link in the
+    triangle's plane) pass every test, as in the reference"""
+    B = xyz.shape[0]
+    A = np.empty((B, 16))
+    for v in range(3):
+        A[:, v * 4:v * 4 + 3] = xyz - P[:, v, :].astype(np.float64)
+        A[:, v * 4 + 3] = 1.0
+    A[:, 12:15] = d
+    A[:, 15] = 0.0
+    b = np.zeros((B, 4))
+    b[:, 3] = 1.0
+    r = gauss4(A, b)
+    return ~((r[:, 0] < 0) | (r[:, 1] < 0) | (r[:, 2] < 0) | (r[:, 3] < 0) | (r[:, 3] > 1))
+
+
+def bspline(x, p, k):
+    """bspline / bspline_mod of src/spline.h:16-43 (clamped knots)"""
+    p = list(p)
+    n = len(p)
+
+    def knot(i):
+        if i < k + 1:
+            return 0.0
+        if i < n:
+            return 1.0 * (i - k) / (n - k)
+        return 1.0
+
+    i = int(math.floor(x * (n - k))) + k
+    if k > n - 1:
+        k = n - 1
+    i = max(min(i, n - 1), k)
+    for j in range(k, 0, -1):
+        for l in range(j):
+            a = (x - knot(i - l)) / (knot(i - l + j) - knot(i - l))
+            p[i - l] = a * p[i - l] + (1 - a) * p[i - l - 1]
+    return p[i]
+
+
 class Rasterizer:
     def __init__(self, model, nx, ny, nz, units):
         self.pk, self.u = Packing(model), units
@@ -314,6 +432,7 @@ class Rasterizer:
         self.zones = {"DefaultZone": 0}
         self.fg = self.mask = 0
         self.mode = "overwrite"
+        self.notes = []
 
     # --- flag state
     def set_flag(self, name):
@@ -496,6 +615,10 @@ class Rasterizer:
                         delta = np.zeros(1)
                     sel = np.broadcast_to(delta < 1e-10, (max(nz, 0), max(ny, 0), max(nx, 0)))
                     self.paint(dx, dy, dz, sel)
+                elif t == "STL":
+                    self.stl(r, n)
+                elif t == "Sweep":
+                    self.sweep(r, n)
                 elif t == "Text":
                     crop = self.region(n.parent)
                     vals = [int(v) for v in open(n.attrib["file"]).read().split()]
@@ -510,6 +633,128 @@ class Rasterizer:
                     if z is None:
                         raise KeyError("Unknown geometry element: " + t)
                     self.draw(z)
+
+    def stl(self, reg, n):
+        a = n.attrib
+        side = a.get("side", "in")
+        if side not in ("in", "out", "surface"):
+            raise KeyError("'side' in 'STL' element have to be 'in', 'out' or 'surface'")
+        io = ("in", "out", "surface").index(side)
+        if "ray_axis" in a and io == 2:
+            raise KeyError("the 'ray_axis' setting in 'STL' makes sens only with 'in' or 'out' side")
+        ra = a.get("ray_axis", "y")
+        if ra not in ("x", "y", "z"):
+            raise KeyError("'ray_axis' in 'STL' element have to be 'x', 'y' or 'z'")
+        P = stl_transform(stl_read(a["file"]), a, self.u)
+        lo = np.ceil(P.min(axis=1)).astype(np.int64) - 1
+        hi = np.floor(P.max(axis=1)).astype(np.int64) + 1
+        N = self.shape
+        if io == 2:
+            # every node of the lattice in each triangle's box, 26 links each
+            sel = np.zeros(N[::-1], bool)
+            idx, pts = [], []
+            for i in range(P.shape[0]):
+                ax = [np.arange(max(lo[i, j], 0), min(hi[i, j], N[j] - 1) + 1) for j in range(3)]
+                if min(len(v) for v in ax) == 0:
+                    continue
+                g = np.stack(np.meshgrid(*ax, indexing="ij"), -1).reshape(-1, 3)
+                pts.append(g)
+                idx.append(np.full(len(g), i))
+            if pts:
+                pts, idx = np.concatenate(pts), np.concatenate(idx)
+                for c0 in range(0, len(pts), 20000):
+                    q, t = pts[c0:c0 + 20000], idx[c0:c0 + 20000]
+                    B = len(q)
+                    xyz = np.repeat(q.astype(np.float64), 26, axis=0)
+                    d = np.tile(_D26, (B, 1))
+                    with np.errstate(all="ignore"):
+                        hit = stl_cut(P[np.repeat(t, 26)], xyz, d).reshape(B, 26).any(axis=1)
+                    h = q[hit]
+                    sel[h[:, 2], h[:, 1], h[:, 0]] = True
+            self.paint(0, 0, 0, sel)
+            return
+        a1, a2, a3 = ("x", "y", "z").index(ra), 0, 0
+        a2, a3 = (a1 + 1) % 3, (a1 + 2) % 3
+        o, s = reg[:3], reg[3:]
+        l1 = o[a1]
+        diff = np.zeros((max(s[a1], 0) + 1, max(s[a2], 0), max(s[a3], 0)), np.int64)
+        f32, f64 = np.float32, np.float64
+        dv = (-0.5694552, 0.8220224)
+        for i in range(P.shape[0]):
+            p = P[i]
+            v1 = (f64(p[1, a2] - p[0, a2]), f64(p[1, a3] - p[0, a3]))   # float differences
+            v2 = (f64(p[2, a2] - p[0, a2]), f64(p[2, a3] - p[0, a3]))
+            c0 = v1[0] * v2[1] - v1[1] * v2[0]
+            X2, X3 = np.meshgrid(np.arange(lo[i, a2], hi[i, a2] + 1), np.arange(lo[i, a3], hi[i, a3] + 1),
+                                 indexing="ij")
+            w0 = (X2.astype(f32) - p[0, a2]).astype(f64)
+            w1 = (X3.astype(f32) - p[0, a3]).astype(f64)
+            with np.errstate(all="ignore"):
+                c1 = (v1[0] * w1 - v1[1] * w0) / c0
+                c2 = (w0 * v2[1] - w1 * v2[0]) / c0
+                c3 = 1. - c1 - c2
+                dc1 = (v1[0] * dv[1] - v1[1] * dv[0]) / c0
+                dc2 = (dv[0] * v2[1] - dv[1] * v2[0]) / c0
+                dc3 = -dc1 - dc2
+                topo = sum(np.where(c == 0, dc > 0, c > 0).astype(int) for c, dc in ((c1, dc1), (c2, dc2), (c3, dc3)))
+                hit = topo == 3
+                h = f64(p[0, a1]) * c3[hit] + f64(p[1, a1]) * c2[hit] + f64(p[2, a1]) * c1[hit]
+            x2, x3 = X2[hit] - o[a2], X3[hit] - o[a3]
+            ok = (x2 >= 0) & (x2 < s[a2]) & (x3 >= 0) & (x3 < s[a3]) & (h >= l1)
+            top = np.minimum(np.floor(h[ok]), l1 + s[a1] - 1).astype(np.int64) - l1 + 1
+            np.add.at(diff, (np.zeros_like(top), x2[ok], x3[ok]), 1)
+            np.add.at(diff, (top, x2[ok], x3[ok]), -1)
+        lev = io + np.cumsum(diff, axis=0)[:-1]
+        lev = (lev + 128) % 256 - 128            # the count is a C char
+        draw = np.fmod(lev, 2) == 1
+        perm = [(a1, a2, a3).index(c) for c in (2, 1, 0)]
+        self.paint(o[0], o[1], o[2], np.transpose(draw, perm))
+
+    def sweep(self, reg, n):
+        a = n.attrib
+        order = _as_int(a["order"]) if "order" in a else 1
+        dl = 1e-4
+        if "step" in a:
+            dl = _atof(a["step"])
+        if "steps" in a:
+            dl = 1 / self.u.alt(a["steps"])
+        def_r = self.u.alt(a["r"]) if "r" in a else 1.0
+        pts = []
+        for c in n.children:
+            if c.tag == "Point":
+                ca = c.attrib
+                pts.append([self.u.alt(ca.get("x", "")), self.u.alt(ca.get("y", "")), self.u.alt(ca.get("z", "")),
+                            self.u.alt(ca["r"], def_r) if "r" in ca else def_r])
+            else:
+                self.notes.append("Unknown element %s in Sweep" % c.tag)
+        if not pts:
+            self.notes.append("No points in Sweep")
+            return
+        if order > len(pts) - 1:
+            self.notes.append("For order %d b-spline you need at least %d points" % (order, order + 1))
+            order = len(pts) - 1
+        cols = list(zip(*pts))
+        N = self.shape
+        sel = np.zeros(N[::-1], bool)
+        last = [-999.0] * 4
+        l = 0.0
+        while l < 1:
+            c = [bspline(l, col, order) for col in cols]
+            if all(abs(c[k] - last[k]) < 0.25 for k in range(4)):
+                l += dl
+                continue
+            last = c
+            x0, y0, z0, r = c
+            b0 = [int(x0 - r - 1), int(y0 - r - 1), int(z0 - r - 1)]
+            w = int(2 * r + 2)
+            lo = [max(reg[j], b0[j], 0) for j in range(3)]
+            hi = [min(reg[j] + reg[3 + j], b0[j] + w, N[j]) for j in range(3)]
+            if all(h > q for q, h in zip(lo, hi)):
+                X, Y, Z = self.grid(lo[0], lo[1], lo[2], hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2])
+                ball = (.5 + X - x0) * (.5 + X - x0) + (.5 + Y - y0) * (.5 + Y - y0) + (.5 + Z - z0) * (.5 + Z - z0) < r * r
+                sel[lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]] |= ball
+            l += dl
+        self.paint(0, 0, 0, sel)
 
     def load(self, gnode):
         self.root = gnode

@@ -5,6 +5,7 @@
 #include "geometry.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <stdexcept>
@@ -378,13 +379,305 @@ void Geometry::draw(const XmlNode& node) {
                         if (v != 0 && crop.in(x, y, z)) dot(x, y, z);
                     }
             std::fclose(f);
-        } else if (e == "STL" || e == "Sweep") {
-            fail("Geometry element " + e + " is not supported by this engine");
+        } else if (e == "STL") {
+            load_stl(reg, n);
+        } else if (e == "Sweep") {
+            load_sweep(reg, n);
         } else {
             const XmlNode* zn = fg_xml_->find_child_by_attribute("Zone", "name", e);
             if (!zn) fail("Unknown geometry element: " + e);
             draw(*zn);
         }
+    }
+}
+
+// ------------------------------------------------------------------ STL ---
+namespace {
+
+// a binary STL record (50 bytes): normal, three vertices, attribute
+#pragma pack(push, 1)
+struct StlTri {
+    float norm[3], p1[3], p2[3], p3[3];
+    short v;
+};
+#pragma pack(pop)
+static_assert(sizeof(StlTri) == 50, "STL record");
+
+// GaussSolve of src/unit.cpp:13-49 on a column-major n x n matrix
+// (A[col*n + row]), row swap only on a zero pivot
+void gauss_solve(double* A, double* b, double* x, int n) {
+    auto z = [](double v) { return std::fabs(v) < 1e-10; };
+    auto I = [n](int r, int c) { return c * n + r; };
+    for (int k = 0; k < n - 1; k++) {
+        if (z(A[I(k, k)]))
+            for (int i = k + 1; i < n; i++)
+                if (!z(A[I(i, k)])) {
+                    for (int j = 0; j < n; j++) std::swap(A[I(k, j)], A[I(i, j)]);
+                    std::swap(b[k], b[i]);
+                    break;
+                }
+        for (int i = k + 1; i < n; i++) {
+            const double m = A[I(i, k)] / A[I(k, k)];
+            for (int j = 0; j < n; j++) A[I(i, j)] -= m * A[I(k, j)];
+            b[i] -= m * b[k];
+        }
+    }
+    for (int i = n - 1; i >= 0; i--) {
+        double sum = 0.0;
+        int j;
+        for (j = n - 1; j >= i + 1; j--) sum += A[I(i, j)] * x[j];
+        x[i] = (b[i] - sum) / A[I(i, j)];
+    }
+}
+
+// calcCut (src/Geometry.cpp.Rt:479-504): does the link from (x,y,z) along
+// d cross the triangle?  Solves x + r3 d = r0 p1 + r1 p2 + r2 p3 with
+// r0 + r1 + r2 = 1; a cut when every r >= 0 and r3 <= 1.  A singular
+// system (a link in the triangle's plane) gives NaNs, which pass every
+// test as they do there, so the link counts as cut.
+bool cut(const StlTri& t, double x, double y, double z, int dx, int dy, int dz) {
+    double m[16], b[4] = {0, 0, 0, 1}, r[4] = {0, 0, 0, 0};
+    m[0] = x - t.p1[0]; m[1] = y - t.p1[1]; m[2] = z - t.p1[2]; m[3] = 1;
+    m[4] = x - t.p2[0]; m[5] = y - t.p2[1]; m[6] = z - t.p2[2]; m[7] = 1;
+    m[8] = x - t.p3[0]; m[9] = y - t.p3[1]; m[10] = z - t.p3[2]; m[11] = 1;
+    m[12] = dx; m[13] = dy; m[14] = dz; m[15] = 0;
+    gauss_solve(m, b, r, 4);
+    return !(r[0] < 0 || r[1] < 0 || r[2] < 0 || r[3] < 0 || r[3] > 1);
+}
+
+}  // namespace
+
+// Geometry::loadSTL with transformSTL (src/Geometry.cpp.Rt:381-472,507-686):
+// the mesh is rotated (Xrot, Yrot, Zrot), scaled, moved (x, y, z), rounded
+// to 1e-5 and shifted by -1/2 plus a small fixed offset, all on float
+// vertices.  side="in"/"out": a node is drawn when a ray along ray_axis from
+// the region's start crosses the mesh an odd (in) or even (out) number of
+// times before it; side="surface": when any of its 26 links crosses a
+// triangle.
+void Geometry::load_stl(const GeomRegion& reg, const XmlNode& n) {
+    int inside_out = 0, axis = 1;
+    if (!n.has("file")) fail("No 'file' attribute in 'STL' element in xml conf");
+    if (n.has("side")) {
+        const std::string side = n.get("side");
+        if (side == "in") inside_out = 0;
+        else if (side == "out") inside_out = 1;
+        else if (side == "surface") inside_out = 2;
+        else fail("'side' in 'STL' element have to be 'in', 'out' or 'surface'");
+    }
+    if (n.has("ray_axis")) {
+        if (inside_out > 1) fail("the 'ray_axis' setting in 'STL' makes sens only with 'in' or 'out' side");
+        const std::string a = n.get("ray_axis");
+        if (a == "x") axis = 0;
+        else if (a == "y") axis = 1;
+        else if (a == "z") axis = 2;
+        else fail("'ray_axis' in 'STL' element have to be 'x', 'y' or 'z'");
+    }
+    FILE* f = std::fopen(n.get("file").c_str(), "rb");
+    if (!f) fail("'STL' element: " + n.get("file") + " doesn't exists or cannot be opened");
+    char header[80];
+    int ntri = 0;
+    const bool ok = std::fread(header, 1, 80, f) == 80 && std::fread(&ntri, sizeof(int), 1, f) == 1;
+    if (!ok || std::strncmp(header, "solid", 5) == 0) {
+        std::fclose(f);
+        fail("'STL' element " + n.get("file") + " is not in binary format!");
+    }
+    if (ntri < 0) { std::fclose(f); fail("bad STL triangle count"); }
+    std::vector<StlTri> tri((size_t)ntri);
+    const size_t got = std::fread(tri.data(), sizeof(StlTri), (size_t)ntri, f);
+    std::fclose(f);
+    if (got != (size_t)ntri) fail("STL file " + n.get("file") + " ended early");
+
+    // transformSTL
+    const char* axes[3] = {"X", "Y", "Z"};
+    const int id1s[3] = {1, 2, 0}, id2s[3] = {2, 0, 1};
+    for (int d = 0; d < 3; d++) {
+        const XmlAttr* a = n.attr(std::string(axes[d]) + "rot");
+        if (!a) continue;
+        const double v = units_.alt(a->value);
+        const int i1 = id1s[d], i2 = id2s[d];
+        for (auto& t : tri)
+            for (float* p : {t.p1, t.p2, t.p3}) {
+                const double yy = p[i1], zz = p[i2];
+                p[i1] = (float)(std::cos(v) * yy - std::sin(v) * zz);
+                p[i2] = (float)(std::sin(v) * yy + std::cos(v) * zz);
+            }
+    }
+    if (n.has("scale")) {
+        const double v = units_.alt(n.get("scale"));
+        for (auto& t : tri)
+            for (float* p : {t.p1, t.p2, t.p3})
+                for (int j = 0; j < 3; j++) p[j] = (float)(v * p[j]);
+    }
+    for (int j = 0; j < 3; j++) {
+        const char* nm[3] = {"x", "y", "z"};
+        if (!n.has(nm[j])) continue;
+        const double v = units_.alt(n.get(nm[j]));
+        for (auto& t : tri)
+            for (float* p : {t.p1, t.p2, t.p3}) p[j] = (float)(p[j] + v);
+    }
+    const double sm_diff[3] = {0.1403e-4, 0.1687e-4, 0.1987e-4};
+    for (auto& t : tri)
+        for (int j = 0; j < 3; j++)
+            for (float* p : {t.p1, t.p2, t.p3}) {
+                p[j] = (float)(myround(p[j] * 1e5) * 1e-5);
+                p[j] = (float)(p[j] + (sm_diff[j] - 0.5));
+            }
+
+    std::vector<signed char> lev((size_t)std::max(0LL, reg.size()), (signed char)inside_out);
+    auto roff = [&reg](int x, int y, int z) {
+        return (size_t)(x - reg.dx) + (size_t)reg.nx * ((size_t)(y - reg.dy) + (size_t)reg.ny * (z - reg.dz));
+    };
+    static const int d27[27][3] = {
+        {0, 0, 0},   {1, 0, 0},   {-1, 0, 0}, {0, 1, 0},   {1, 1, 0},   {-1, 1, 0},  {0, -1, 0},
+        {1, -1, 0},  {-1, -1, 0}, {0, 0, 1},  {1, 0, 1},   {-1, 0, 1},  {0, 1, 1},   {1, 1, 1},
+        {-1, 1, 1},  {0, -1, 1},  {1, -1, 1}, {-1, -1, 1}, {0, 0, -1},  {1, 0, -1},  {-1, 0, -1},
+        {0, 1, -1},  {1, 1, -1},  {-1, 1, -1}, {0, -1, -1}, {1, -1, -1}, {-1, -1, -1}};
+    for (const StlTri& t : tri) {
+        int mn[3], mx[3];
+        for (int j = 0; j < 3; j++) {
+            mn[j] = (int)std::ceil(t.p1[j]);
+            if (t.p2[j] < mn[j]) mn[j] = (int)std::ceil(t.p2[j]);
+            if (t.p3[j] < mn[j]) mn[j] = (int)std::ceil(t.p3[j]);
+            mx[j] = (int)std::floor(t.p1[j]);
+            if (t.p2[j] > mx[j]) mx[j] = (int)std::floor(t.p2[j]);
+            if (t.p3[j] > mx[j]) mx[j] = (int)std::floor(t.p3[j]);
+            mn[j] -= 1;
+            mx[j] += 1;
+        }
+        if (inside_out == 2) {
+            for (int x = mn[0]; x <= mx[0]; x++)
+                for (int z = mn[2]; z <= mx[2]; z++)
+                    for (int y = mn[1]; y <= mx[1]; y++) {
+                        if (!region_.in(x, y, z)) continue;
+                        for (int d = 1; d < 27; d++)
+                            if (cut(t, x, y, z, d27[d][0], d27[d][1], d27[d][2])) dot(x, y, z);
+                    }
+            continue;
+        }
+        const int a1 = axis, a2 = (axis + 1) % 3, a3 = (axis + 2) % 3;
+        const int l1 = a1 == 0 ? reg.dx : (a1 == 1 ? reg.dy : reg.dz);
+        // the differences are taken in float, as the vertices are floats
+        const double v1[2] = {t.p2[a2] - t.p1[a2], t.p2[a3] - t.p1[a3]};
+        const double v2[2] = {t.p3[a2] - t.p1[a2], t.p3[a3] - t.p1[a3]};
+        const double c0 = v1[0] * v2[1] - v1[1] * v2[0];
+        const double dv[2] = {-0.5694552, 0.8220224};  // tie-break direction for rays through edges
+        for (int x2 = mn[a2]; x2 <= mx[a2]; x2++)
+            for (int x3 = mn[a3]; x3 <= mx[a3]; x3++) {
+                const double v[2] = {x2 - t.p1[a2], x3 - t.p1[a3]};  // float
+                double c1 = v1[0] * v[1] - v1[1] * v[0];
+                double c2 = v[0] * v2[1] - v[1] * v2[0];
+                c1 /= c0;
+                c2 /= c0;
+                const double c3 = 1. - c1 - c2;
+                const double dc1 = (v1[0] * dv[1] - v1[1] * dv[0]) / c0;
+                const double dc2 = (dv[0] * v2[1] - dv[1] * v2[0]) / c0;
+                const double dc3 = -dc1 - dc2;
+                const double cs[3] = {c1, c2, c3}, dcs[3] = {dc1, dc2, dc3};
+                int topo = 0;
+                for (int k = 0; k < 3; k++) {
+                    if (cs[k] == 0) { if (dcs[k] > 0) topo++; }
+                    else if (cs[k] > 0) topo++;
+                }
+                if (topo != 3) continue;
+                const double h = t.p1[a1] * c3 + t.p2[a1] * c2 + t.p3[a1] * c1;
+                for (int x1 = l1; x1 <= h; x1++) {
+                    int x, y, z;
+                    if (axis == 0) { x = x1; y = x2; z = x3; }
+                    else if (axis == 1) { x = x3; y = x1; z = x2; }
+                    else { x = x2; y = x3; z = x1; }
+                    if (reg.in(x, y, z)) lev[roff(x, y, z)]++;
+                }
+            }
+    }
+    if (inside_out != 2)
+        for (int z = reg.dz; z < reg.dz + reg.nz; z++)
+            for (int y = reg.dy; y < reg.dy + reg.ny; y++)
+                for (int x = reg.dx; x < reg.dx + reg.nx; x++)
+                    if (lev[roff(x, y, z)] % 2 == 1) dot(x, y, z);
+}
+
+// ---------------------------------------------------------------- Sweep ---
+namespace {
+
+// knots of a clamped uniform B-spline: k+1 zeros, (i-k)/(n-k), ones
+// (knot_bs of src/spline.h:9-14)
+double knot(int i, int n, int k) {
+    if (i < k + 1) return 0;
+    if (i < n) return 1.0 * (i - k) / (n - k);
+    return 1;
+}
+
+// de Boor's recursion for the order-k clamped B-spline through the control
+// values p at x in [0,1) (bspline_mod of src/spline.h:16-35)
+double bspline(double x, std::vector<double> p, int k) {
+    const int n = (int)p.size();
+    int i = (int)std::floor(x * (n - k)) + k;
+    if (k > n - 1) k = n - 1;
+    if (i > n - 1) i = n - 1;
+    if (i < k) i = k;
+    for (int j = k; j > 0; j--)
+        for (int l = 0; l < j; l++) {
+            const double a = (x - knot(i - l, n, k)) / (knot(i - l + j, n, k) - knot(i - l, n, k));
+            p[i - l] = a * p[i - l] + (1 - a) * p[i - l - 1];
+        }
+    return p[i];
+}
+
+}  // namespace
+
+// Geometry::loadSweep (src/Geometry.cpp.Rt:688-755): balls of radius r
+// centred on a B-spline through the <Point x y z r> children, sampled every
+// `step` of the curve parameter (default 1e-4, or 1/steps); a sample is
+// drawn only when centre or radius moved by 1/4 node or more in some
+// coordinate since the last drawn one.
+void Geometry::load_sweep(const GeomRegion& reg, const XmlNode& node) {
+    int order = 1;
+    if (node.has("order")) order = xml_as_int(node.get("order"));
+    double dl = 1e-4;
+    if (node.has("step")) dl = xml_as_double(node.get("step"));
+    if (node.has("steps")) dl = 1 / units_.alt(node.get("steps"));
+    double def_r = 1;
+    if (node.has("r")) def_r = units_.alt(node.get("r"));
+    std::vector<double> px, py, pz, pr;
+    for (const auto& c : node.children) {
+        if (c->name == "Point") {
+            px.push_back(units_.alt(c->get("x")));
+            py.push_back(units_.alt(c->get("y")));
+            pz.push_back(units_.alt(c->get("z")));
+            pr.push_back(c->has("r") ? units_.alt(c->get("r"), def_r) : def_r);
+        } else {
+            note("Unknown element " + c->name + " in Sweep");
+        }
+    }
+    const int n = (int)px.size();
+    if (n == 0) {
+        note("No points in Sweep");
+        return;
+    }
+    if (order > n - 1) {
+        note("For order " + std::to_string(order) + " b-spline you need at least " + std::to_string(order + 1) +
+             " points");
+        order = n - 1;
+    }
+    if (!(dl > 0)) fail("Sweep: the step has to be positive");
+    double x0_ = -999.0, y0_ = -999.0, z0_ = -999.0, r0_ = -999.0;
+    for (double l = 0; l < 1; l += dl) {
+        const double x0 = bspline(l, px, order), y0 = bspline(l, py, order);
+        const double z0 = bspline(l, pz, order), r = bspline(l, pr, order);
+        if (std::fabs(x0 - x0_) < 0.25 && std::fabs(y0 - y0_) < 0.25 && std::fabs(z0 - z0_) < 0.25 &&
+            std::fabs(r - r0_) < 0.25)
+            continue;
+        x0_ = x0; y0_ = y0; z0_ = z0; r0_ = r;
+        GeomRegion ball;
+        ball.dx = (int)(x0 - r - 1); ball.dy = (int)(y0 - r - 1); ball.dz = (int)(z0 - r - 1);
+        ball.nx = ball.ny = ball.nz = (int)(2 * r + 2);
+        const GeomRegion t = reg.intersect(ball);
+        for (int x = t.dx; x < t.dx + t.nx; x++)
+            for (int y = t.dy; y < t.dy + t.ny; y++)
+                for (int z = t.dz; z < t.dz + t.nz; z++)
+                    if ((.5 + x - x0) * (.5 + x - x0) + (.5 + y - y0) * (.5 + y - y0) + (.5 + z - z0) * (.5 + z - z0) <
+                        r * r)
+                        dot(x, y, z);
     }
 }
 

@@ -53,6 +53,9 @@ public:
     // setting zones by name -> number, DefaultZone = 0, in order of first use
     const std::map<std::string, int>& zones() const { return zones_; }
     const GeomRegion& region() const { return region_; }
+    // notices the reference prints while drawing (e.g. about a Sweep's
+    // points), in order; cleared by nothing
+    const std::vector<std::string>& notes() const { return notes_; }
 
 private:
     enum Mode { OVERWRITE, FILL, CHANGE };
@@ -65,8 +68,11 @@ private:
     void set_zone(const std::string& name);
     GeomRegion get_region(const XmlNode* n) const;
     void dot(int x, int y, int z);
+    void note(const std::string& m) { notes_.push_back(m); }
     void draw(const XmlNode& n);
     void load_zone(const std::string& name);
+    void load_stl(const GeomRegion& reg, const XmlNode& n);
+    void load_sweep(const GeomRegion& reg, const XmlNode& n);
 
     const ModelDef& model_;
     const Units& units_;
@@ -76,6 +82,7 @@ private:
     XmlNode* fg_xml_ = nullptr;
     uint32_t fg_ = 0, fg_mask_ = 0;
     Mode mode_ = OVERWRITE;
+    std::vector<std::string> notes_;
 };
 
 }  // namespace tclbx

@@ -155,7 +155,7 @@ struct GenericContainer : GenericAction {
 // <Geometry> (src/Handlers/acGeometry.cpp)
 struct AcGeometry : Action {
     int init() override {
-        solver->geometry->load(*node);
+        solver->load_geometry(*node);
         const GeomRegion& r = solver->geometry->region();
         if (solver->lattice) {
             Region o;
@@ -598,7 +598,13 @@ void Solver::prepare() {
 void Solver::rasterize() {
     if (!config) fail("no case loaded");
     for (auto& c : config->children)
-        if (c->name == "Geometry") geometry->load(*c);
+        if (c->name == "Geometry") load_geometry(*c);
+}
+
+void Solver::load_geometry(XmlNode& g) {
+    const size_t k = geometry->notes().size();
+    geometry->load(g);
+    for (size_t i = k; i < geometry->notes().size(); i++) note(geometry->notes()[i]);
 }
 
 void Solver::run() {

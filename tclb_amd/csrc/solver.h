@@ -90,6 +90,8 @@ public:
     int write_vtk(const std::string& name, const std::string& what, const GeomRegion& reg);
     double walltime() const;
     void note(const std::string& m) { messages.push_back(m); }
+    // draws a <Geometry> element; its notices join `messages`
+    void load_geometry(XmlNode& g);
     // log columns: settings of the model that the Log lists
     std::vector<int> log_settings() const;
 
