@@ -239,6 +239,11 @@ NODETYPES["d2q9"] = [("EPressure", "BOUNDARY"), ("WPressure", "BOUNDARY"), ("NVe
 NODETYPES["d2q9_SRT_drag"] = NODETYPES["d2q9_SRT"]
 NODETYPES["d2q9_drag"] = NODETYPES["d2q9"]
 NODETYPES["d3q19_lid"] = NODETYPES["d3q19"][:9] + [("MovingWall", "BOUNDARY"), ("MRT", "COLLISION")]
+# models/flow/d3q27/Dynamics.R:35-45
+NODETYPES["d3q27"] = [("Smagorinsky", "LES"), ("Stab", "ENTROPIC"), ("NSymmetry", "BOUNDARY"),
+                      ("SSymmetry", "BOUNDARY"), ("ISymmetry", "BOUNDARY"), ("EPressure", "BOUNDARY"),
+                      ("Solid", "BOUNDARY"), ("Wall", "BOUNDARY"), ("WPressure", "BOUNDARY"),
+                      ("WVelocity", "BOUNDARY"), ("MRT", "COLLISION")]
 
 
 class Packing:

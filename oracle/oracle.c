@@ -12,6 +12,7 @@
  */
 #include "oracle.h"
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -106,8 +107,8 @@ struct orc_lattice {
     double params[16];
     double* zone;               /* [nzones][ncols] */
     int nzones, ncols;
-    int e[19][3];
-    double w[19];
+    int e[27][3];
+    double w[27];
 };
 
 static size_t esize(int storage)
@@ -154,7 +155,15 @@ orc_lattice* orc_create(int model, int nx, int ny, int nz, int storage)
     d3q19_sparse_init_f();
     L->model = model; L->nx = nx; L->ny = ny; L->nz = nz; L->storage = storage;
     L->N = (size_t)nx * ny * nz;
-    if (model == ORC_D3Q19 || model == ORC_D3Q19_LID) {
+    if (model == ORC_D3Q27) {
+        q27_init_d();
+        q27_init_f();
+        L->Q = 27; L->nglob = 7;
+        for (q = 0; q < 27; q++) {
+            for (a = 0; a < 3; a++) L->e[q][a] = q27_e_d[q][a];
+            L->w[q] = q27_w_d[q];
+        }
+    } else if (model == ORC_D3Q19 || model == ORC_D3Q19_LID) {
         L->Q = 19; L->nglob = model == ORC_D3Q19_LID ? 30 : 26;
         for (q = 0; q < 19; q++) {
             L->e[q][0] = M_d[3][q]; L->e[q][1] = M_d[5][q]; L->e[q][2] = M_d[7][q];
@@ -210,7 +219,8 @@ void orc_set_flags(orc_lattice* L, const uint16_t* flags)
 
 static int is3d(const orc_lattice* L) { return L->model == ORC_D3Q19 || L->model == ORC_D3Q19_LID; }
 static int is_d2q9m(const orc_lattice* L) { return L->model == ORC_D2Q9 || L->model == ORC_D2Q9_DRAG; }
-static int zone_shift(const orc_lattice* L) { return is3d(L) ? 6 : (is_d2q9m(L) ? 7 : 4); }
+static int is_q27(const orc_lattice* L) { return L->model == ORC_D3Q27; }
+static int zone_shift(const orc_lattice* L) { return is3d(L) ? 6 : (is_d2q9m(L) || is_q27(L) ? 7 : 4); }
 
 static const double* zone_row(const orc_lattice* L, uint16_t nt)
 {
@@ -225,7 +235,19 @@ void orc_init(orc_lattice* L)
     int q;
     for (n = 0; n < L->N; n++) {
         const double* zr = zone_row(L, L->flags[n]);
-        if (is3d(L)) {
+        if (is_q27(L)) {
+            /* Init(), models/flow/d3q27/Dynamics.c.Rt:184-190, no synthetic
+             * turbulence modes: SetEquilibrum(1 + 3 Pressure, Velocity, 0, 0) */
+            if (L->storage == ORC_F64) {
+                double f[27];
+                q27_feq_d(f, 1.0 + zr[1] * 3.0, zr[0], 0., 0.);
+                for (q = 0; q < 27; q++) st(L, L->buf[0], (size_t)q * L->N + n, q, f[q]);
+            } else {
+                float f[27];
+                q27_feq_f(f, 1.0f + (float)zr[1] * 3.0f, (float)zr[0], 0.f, 0.f);
+                for (q = 0; q < 27; q++) st(L, L->buf[0], (size_t)q * L->N + n, q, f[q]);
+            }
+        } else if (is3d(L)) {
             /* Init(): SetEquilibrum(1.0, InletVelocity, 0, 0), Dynamics.c.Rt:219-221 */
             if (L->storage == ORC_F64) {
                 double f[19];
@@ -296,17 +318,19 @@ static void step_rows(orc_lattice* L, long double* glob, long r0, long r1)
                 size_t n = (size_t)x + (size_t)L->nx * ((size_t)y + (size_t)L->ny * z);
                 uint16_t nt = L->flags[n];
                 const double* zr = zone_row(L, nt);
-                double fd[19];
+                double fd[27];
                 pull(L, in, x, y, z, fd);
                 if (L->storage == ORC_F64) {
-                    if (is3d(L)) d3q19_node_d(fd, nt, L->params, zr, glob, lid, L->flags, x, y, z, L->nx, L->ny, L->nz);
+                    if (is_q27(L)) d3q27_node_d(fd, nt, L->params, zr, glob);
+                    else if (is3d(L)) d3q19_node_d(fd, nt, L->params, zr, glob, lid, L->flags, x, y, z, L->nx, L->ny, L->nz);
                     else if (is_d2q9m(L)) d2q9m_node_d(fd, nt, L->params, zr, glob, L->model == ORC_D2Q9_DRAG);
                     else d2q9_node_d(fd, nt, L->params[0], L->params[1], zr, drag, glob);
                     for (q = 0; q < L->Q; q++) st(L, out, (size_t)q * L->N + n, q, fd[q]);
                 } else {
-                    float ff[19];
+                    float ff[27];
                     for (q = 0; q < L->Q; q++) ff[q] = (float)fd[q];
-                    if (is3d(L)) d3q19_node_f(ff, nt, L->params, zr, glob, lid, L->flags, x, y, z, L->nx, L->ny, L->nz);
+                    if (is_q27(L)) d3q27_node_f(ff, nt, L->params, zr, glob);
+                    else if (is3d(L)) d3q19_node_f(ff, nt, L->params, zr, glob, lid, L->flags, x, y, z, L->nx, L->ny, L->nz);
                     else if (is_d2q9m(L)) d2q9m_node_f(ff, nt, L->params, zr, glob, L->model == ORC_D2Q9_DRAG);
                     else d2q9_node_f(ff, nt, (float)L->params[0], (float)L->params[1], zr, drag, glob);
                     for (q = 0; q < L->Q; q++) st(L, out, (size_t)q * L->N + n, q, ff[q]);
@@ -388,9 +412,27 @@ void orc_get_quantity(const orc_lattice* L, int which, double* out)
             for (x = 0; x < L->nx; x++) {
                 size_t n = (size_t)x + (size_t)L->nx * ((size_t)y + (size_t)L->ny * z);
                 uint16_t nt = L->flags[n];
-                double f[19], rho = 0, jx = 0, jy = 0, jz = 0;
+                double f[27], rho = 0, jx = 0, jy = 0, jz = 0;
                 pull(L, L->buf[L->cur], x, y, z, f);
-                if (is3d(L)) {
+                if (is_q27(L)) {
+                    /* getP, getU, getFd, models/flow/d3q27/Dynamics.c.Rt:33-73 */
+                    for (q = 0; q < 27; q++) {
+                        rho += f[q];
+                        jx += f[q] * L->e[q][0]; jy += f[q] * L->e[q][1]; jz += f[q] * L->e[q][2];
+                    }
+                    if (which == 0) out[n] = rho;
+                    else if (which == 2) out[n] = (rho - 1.0) / 3.0;
+                    else if (which == 1) {
+                        out[3*n] = (jx + L->params[2] * 0.5) / rho;
+                        out[3*n+1] = (jy + L->params[3] * 0.5) / rho;
+                        out[3*n+2] = (jz + L->params[4] * 0.5) / rho;
+                    } else {
+                        int wall = (nt & 0xF) == 6;
+                        out[3*n] = wall ? -2.0 * jx : 0.0;
+                        out[3*n+1] = wall ? -2.0 * jy : 0.0;
+                        out[3*n+2] = wall ? -2.0 * jz : 0.0;
+                    }
+                } else if (is3d(L)) {
                     /* getRho / getU / getP, Dynamics.c.Rt:23-40 */
                     for (q = 0; q < 19; q++) {
                         rho += f[q];
@@ -433,6 +475,29 @@ void orc_get_quantity(const orc_lattice* L, int which, double* out)
 /* ---------------------------------------------- single-node exports --- */
 void orc_d2q9_feq(double feq[9], double d, double ux, double uy) { d2q9_feq_d(feq, d, ux, uy); }
 void orc_d3q19_feq(double feq[19], double rho, double jx, double jy, double jz) { d3q19_feq_d(feq, rho, jx, jy, jz); }
+void orc_d3q27_feq(double feq[27], double rho, double jx, double jy, double jz)
+{
+    q27_init_d();
+    q27_feq_d(feq, rho, jx, jy, jz);
+}
+void orc_d3q27_node(double f[27], uint16_t nt, const double* params, const double* zone, double* globals)
+{
+    long double g[8] = {0};
+    int i;
+    q27_init_d();
+    d3q27_node_d(f, nt, params, zone, globals ? g : NULL);
+    if (globals) for (i = 0; i < 7; i++) globals[i] = (double)g[i];
+}
+void orc_d3q27_basis(double M[27 * 27], int order[27], int p[27 * 3])
+{
+    int i, j;
+    q27_init_d();
+    for (i = 0; i < 27; i++) {
+        order[i] = q27_ord_d[i];
+        for (j = 0; j < 3; j++) p[3 * i + j] = q27_p_d[i][j];
+        for (j = 0; j < 27; j++) M[27 * i + j] = q27_M_d[i][j];
+    }
+}
 void orc_d3q19_collide(double f[19], const double* params)
 {
     static const double zone[32] = {0};

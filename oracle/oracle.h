@@ -15,6 +15,9 @@
  *   d2q9           models/flow/d2q9/Dynamics.c.Rt, Dynamics.R with the basis of
  *                  src/lib/feq.R and the Zou-He of src/lib/boundary.R:155-207
  *   d2q9_drag      d2q9 plus Drag/Lift on Solid nodes (as d2q9_SRT_drag)
+ *   d3q27          models/flow/d3q27/Dynamics.c.Rt, Dynamics.R with the raw
+ *                  moment basis of src/lib/feq.R (ortogonal=FALSE), Zou-He of
+ *                  src/lib/boundary.R:66-109, Symmetry of :33-50
  *   d3q19_lid      d3q19 plus a MovingWall node type (bounce-back with Ladd's
  *                  wall-velocity term) and its force/torque Globals; the
  *                  reference has no moving wall, so this part is pinned by
@@ -40,7 +43,7 @@ extern "C" {
 #endif
 
 enum { ORC_D2Q9_SRT = 0, ORC_D2Q9_SRT_DRAG = 1, ORC_D3Q19 = 2, ORC_D3Q19_LID = 3, ORC_D2Q9 = 4,
-       ORC_D2Q9_DRAG = 5 };
+       ORC_D2Q9_DRAG = 5, ORC_D3Q27 = 6 };
 enum { ORC_F64 = 0, ORC_F32 = 1, ORC_F32S = 2, ORC_F16S = 3, ORC_BF16S = 4 };
 
 /* d2q9 parameter vector:   [omega, Density]
@@ -54,7 +57,10 @@ enum { ORC_F64 = 0, ORC_F32 = 1, ORC_F32S = 2, ORC_F16S = 3, ORC_BF16S = 4 };
  *                           d3q19_lid).
  * d2q9 (MRT) parameters:   [S2, S3, S4, GravitationX, GravitationY]
  * d2q9 (MRT) zone columns: [VelocityX, VelocityY, Pressure, PressureLossInObj,
- *                           OutletFluxInObj, InletFluxInObj]                  */
+ *                           OutletFluxInObj, InletFluxInObj]
+ * d3q27 parameters:        [omega, Smag, ForceX, ForceY, ForceZ]
+ * d3q27 zone columns:      [Velocity, Pressure, Turbulence, then the six
+ *                           <Global>InObj weights in Globals order]        */
 
 typedef struct orc_lattice orc_lattice;
 
@@ -71,7 +77,8 @@ void orc_iterate(orc_lattice* L, int niter, int glob_last, double* globals_out);
 /* f as real values (storage -> real + shift), layout [q][z][y][x]. */
 void orc_get_f(const orc_lattice* L, double* out);
 void orc_set_f(orc_lattice* L, const double* in);
-/* quantity: 0 = Rho, 1 = U (3 doubles per node, interleaved), 2 = P.
+/* quantity: 0 = Rho, 1 = U (3 doubles per node, interleaved), 2 = P,
+ * 3 = Fd (d3q27, 3 doubles per node).
  * Evaluated like the reference get<Q> kernels: on pulled (streamed)
  * populations of the current snapshot, without boundary or collision. */
 void orc_get_quantity(const orc_lattice* L, int which, double* out);
@@ -89,6 +96,11 @@ float    orc_bf16_to_f32(uint16_t h);
 void orc_d2q9_feq(double feq[9], double d, double ux, double uy);
 void orc_d3q19_feq(double feq[19], double rho, double jx, double jy, double jz);
 void orc_d3q19_collide(double f[19], const double* params);
+void orc_d3q27_feq(double feq[27], double rho, double jx, double jy, double jz);
+/* one node update of d3q27 (boundary + collision); globals[7] if not NULL */
+void orc_d3q27_node(double f[27], uint16_t nt, const double* params, const double* zone, double* globals);
+/* the moment basis: M[27*i + j] = moment j of direction i, its order and powers */
+void orc_d3q27_basis(double M[27 * 27], int order[27], int p[27 * 3]);
 
 #ifdef __cplusplus
 }

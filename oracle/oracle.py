@@ -14,10 +14,10 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 
-D2Q9_SRT, D2Q9_SRT_DRAG, D3Q19, D3Q19_LID, D2Q9, D2Q9_DRAG = 0, 1, 2, 3, 4, 5
+D2Q9_SRT, D2Q9_SRT_DRAG, D3Q19, D3Q19_LID, D2Q9, D2Q9_DRAG, D3Q27 = 0, 1, 2, 3, 4, 5, 6
 F64, F32, F32S, F16S, BF16S = 0, 1, 2, 3, 4
 MODELS = {"d2q9_SRT": D2Q9_SRT, "d2q9_SRT_drag": D2Q9_SRT_DRAG, "d3q19": D3Q19, "d3q19_lid": D3Q19_LID,
-          "d2q9": D2Q9, "d2q9_drag": D2Q9_DRAG}
+          "d2q9": D2Q9, "d2q9_drag": D2Q9_DRAG, "d3q27": D3Q27}
 STORAGE = {"f64": F64, "f32": F32, "f32s": F32S, "f16s": F16S, "bf16s": BF16S}
 
 _lib = None
@@ -58,6 +58,9 @@ def lib():
         L.orc_d2q9_feq.argtypes = [dp, ctypes.c_double, ctypes.c_double, ctypes.c_double]
         L.orc_d3q19_feq.argtypes = [dp] + [ctypes.c_double] * 4
         L.orc_d3q19_collide.argtypes = [dp, dp]
+        L.orc_d3q27_feq.argtypes = [dp] + [ctypes.c_double] * 4
+        L.orc_d3q27_node.argtypes = [dp, ctypes.c_uint16, dp, dp, dp]
+        L.orc_d3q27_basis.argtypes = [dp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.orc_set_threads.argtypes = [ip]
         _lib = L
     return _lib
@@ -119,11 +122,12 @@ class OracleLattice:
         lib().orc_set_f(self.h, _dp(a))
 
     def quantity(self, which):
-        w = {"Rho": 0, "U": 1, "P": 2}[which] if isinstance(which, str) else which
+        w = {"Rho": 0, "U": 1, "P": 2, "Fd": 3}[which] if isinstance(which, str) else which
         n = self.nx * self.ny * self.nz
-        out = np.empty(n * (3 if w == 1 else 1), dtype=np.float64)
+        vec = w in (1, 3)
+        out = np.empty(n * (3 if vec else 1), dtype=np.float64)
         lib().orc_get_quantity(self.h, w, _dp(out))
-        return out.reshape(self.shape + ((3,) if w == 1 else ()))
+        return out.reshape(self.shape + ((3,) if vec else ()))
 
 
 def set_threads(n):
@@ -151,10 +155,40 @@ def d3q19_collide(f, params):
     return a
 
 
+def d3q27_feq(rho, jx, jy, jz):
+    out = np.empty(27)
+    lib().orc_d3q27_feq(_dp(out), rho, jx, jy, jz)
+    return out
+
+
+def d3q27_node(f, nt, params, zone=None):
+    """one d3q27 node update; returns (f, Globals[7])"""
+    a = np.array(f, dtype=np.float64)
+    p = np.zeros(16)
+    p[: len(params)] = params
+    z = np.zeros(32)
+    if zone is not None:
+        z[: len(zone)] = zone
+    g = np.zeros(7)
+    lib().orc_d3q27_node(_dp(a), nt, _dp(p), _dp(z), _dp(g))
+    return a, g
+
+
+def d3q27_basis():
+    """(M [direction, moment], order [moment], powers [moment, axis])"""
+    M = np.empty(27 * 27)
+    o = np.empty(27, dtype=np.int32)
+    pw = np.empty(81, dtype=np.int32)
+    lib().orc_d3q27_basis(_dp(M), o.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                          pw.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    return M.reshape(27, 27), o, pw.reshape(27, 3)
+
+
 D3Q19_PARAMS = ["omega", "InletVelocity", "InletDensity", "ForceX", "ForceY", "ForceZ",
                 "MRT_Se", "MRT_Seps", "MRT_Sq", "MRT_Spi", "MRT_Sm"]
 D3Q19_LID_PARAMS = D3Q19_PARAMS + ["MovingWallVelocityX", "MovingWallVelocityY", "MovingWallVelocityZ"]
 D2Q9_PARAMS = ["S2", "S3", "S4", "GravitationX", "GravitationY"]
+D3Q27_PARAMS = ["omega", "Smag", "ForceX", "ForceY", "ForceZ"]
 
 # derived settings of the models' Dynamics.R (AddSetting(name, <target>=expr))
 DERIVED = {
@@ -196,6 +230,8 @@ def lattice_for_case(case, storage="f64"):
         L.set_params([vals[n] for n in D3Q19_LID_PARAMS])
     elif case.model in ("d2q9", "d2q9_drag"):
         L.set_params([vals[n] for n in D2Q9_PARAMS])
+    elif case.model == "d3q27":
+        L.set_params([vals[n] for n in D3Q27_PARAMS])
     else:
         L.set_params([vals["omega"], vals["Density"]])
     zshift = m.group("SETTINGZONE")[1]

@@ -73,6 +73,21 @@ template <> struct Lat<MODEL_D3Q19_LID> : Lat<MODEL_D3Q19> {
     static constexpr int G = 30, SUMG = 29;
 };
 template <int MODEL> constexpr bool is_d3q19 = MODEL == MODEL_D3Q19 || MODEL == MODEL_D3Q19_LID;
+// d3q27: density i at e = (i%3 - 1, i/3%3 - 1, i/9 - 1) (expand.grid order,
+// models/flow/d3q27/Dynamics.R:1-10), opposite 26 - i; NodeType groups
+// BOUNDARY (4 bits), COLLISION, ENTROPIC, LES, zone from bit 7; Globals
+// XFlux, YFlux, ZFlux, XDragForce, YDragForce, ZDragForce, Objective
+template <> struct Lat<MODEL_D3Q27> {
+    static constexpr int Q = 27, G = 7, SUMG = 7, ZSHIFT = 7;
+    __host__ __device__ static constexpr int ex(int i) { return i % 3 - 1; }
+    __host__ __device__ static constexpr int ey(int i) { return i / 3 % 3 - 1; }
+    __host__ __device__ static constexpr int ez(int i) { return i / 9 - 1; }
+    __host__ __device__ static constexpr double w(int i) {
+        return ((i % 3 != 1) + (i / 3 % 3 != 1) + (i / 9 != 1)) == 0 ? 8. / 27
+             : ((i % 3 != 1) + (i / 3 % 3 != 1) + (i / 9 != 1)) == 1 ? 2. / 27
+             : ((i % 3 != 1) + (i / 3 % 3 != 1) + (i / 9 != 1)) == 2 ? 1. / 54 : 1. / 216;
+    }
+};
 
 int kernel_nglobals(int model) {
     switch (model) {
@@ -82,6 +97,7 @@ int kernel_nglobals(int model) {
     case MODEL_D3Q19_LID: return Lat<MODEL_D3Q19_LID>::G;
     case MODEL_D2Q9: return Lat<MODEL_D2Q9>::G;
     case MODEL_D2Q9_DRAG: return Lat<MODEL_D2Q9_DRAG>::G;
+    case MODEL_D3Q27: return Lat<MODEL_D3Q27>::G;
     }
     return 0;
 }
@@ -808,6 +824,232 @@ __device__ __forceinline__ void d3q19_node(R* f, unsigned nt, const double* par,
     }
 }
 
+// --------------------------------------------------------------- d3q27 ---
+// The moment basis of MRT_eq(U, ortogonal=FALSE) (src/lib/feq.R:15-22,
+// 37-84) is every monomial ex^p ey^q ez^r with p, q, r in {0, 1, 2}: the
+// tensor product of the 1-D basis {1, e, e^2} on e in {-1, 0, 1}.  Moment
+// k = p + 3q + 9r is computed by three passes of the 1-D transform
+//   (f-, f0, f+) -> (f- + f0 + f+, f+ - f-, f+ + f-)
+// along x, y and z in registers; its inverse is
+//   (m0, m1, m2) -> ((m2 - m1)/2, m0 - m2, (m2 + m1)/2).
+// The order of moment k is p + q + r.
+__host__ __device__ constexpr int d3q27_order(int k) { return k % 3 + k / 3 % 3 + k / 9; }
+
+template <class R>
+__device__ __forceinline__ void d3q27_fwd(R* m) {
+#pragma unroll
+    for (int s = 1; s <= 9; s *= 3)
+#pragma unroll
+        for (int b = 0; b < 27; b++) {
+            if ((b / s) % 3 != 0) continue;
+            const R a = m[b], c = m[b + s], d = m[b + 2 * s];
+            m[b] = a + c + d;
+            m[b + s] = d - a;
+            m[b + 2 * s] = d + a;
+        }
+}
+
+template <class R>
+__device__ __forceinline__ void d3q27_inv(R* m) {
+#pragma unroll
+    for (int s = 1; s <= 9; s *= 3)
+#pragma unroll
+        for (int b = 0; b < 27; b++) {
+            if ((b / s) % 3 != 0) continue;
+            const R m0 = m[b], m1 = m[b + s], m2 = m[b + 2 * s];
+            m[b] = (m2 - m1) * R(0.5);
+            m[b + s] = m0 - m2;
+            m[b + 2 * s] = (m2 + m1) * R(0.5);
+        }
+}
+
+// Equilibrium moment k: rho times the product over the axes of J_a/rho
+// (power 1) or J_a^2/rho^2 + 1/3 (power 2), truncated to second order in J
+// (MRT_eq, feq.R:40-57).  With n2 axes at power 2:
+//   no power-1 axis:  rho/3^n2 + sum over power-2 axes of J_b^2/rho / 3^(n2-1)
+//   one (a):          J_a / 3^n2
+//   two (a, a'):      J_a J_a' / rho / 3^n2
+//   three:            0
+template <class R>
+__device__ __forceinline__ R d3q27_req(int k, R rho, R Jx, R Jy, R Jz, R ir) {
+    const int p[3] = {k % 3, k / 3 % 3, k / 9};
+    const R J[3] = {Jx, Jy, Jz};
+    int n1 = 0, n2 = 0, a1 = -1, a2 = -1;
+    for (int d = 0; d < 3; d++) {
+        if (p[d] == 1) { if (a1 < 0) a1 = d; else a2 = d; n1++; }
+        if (p[d] == 2) n2++;
+    }
+    const R c = n2 == 0 ? R(1) : n2 == 1 ? R(1) / R(3) : n2 == 2 ? R(1) / R(9) : R(1) / R(27);
+    if (n1 == 0) {
+        R s = rho * c;
+        for (int d = 0; d < 3; d++)
+            if (p[d] == 2) s += J[d] * J[d] * ir * (c * R(3));
+        return s;
+    }
+    if (n1 == 1) return J[a1] * c;
+    if (n1 == 2) return J[a1] * J[a2] * ir * c;
+    return R(0);
+}
+
+// Zou-He on an x face (ZouHe(EQ, 1, sign, type), src/lib/boundary.R:66-109):
+// the entering populations (sign*ex > 0) become f_opp + 6 w_i (e_i . J),
+// the non-equilibrium bounce-back of the second-order equilibrium, with J
+// solved so that the node has density rho and no tangential momentum.
+// With A = sum of the ex = 0 populations + 2 * the outgoing ones and
+// K_y, K_z the tangential momentum of the ex = 0 populations:
+//   velocity V: rho = A / (1 - sign V), Jx = rho V;  pressure P: rho =
+//   3P + 1, Jx = sign (rho - A);  and Jy = -3 K_y, Jz = -3 K_z.
+template <class R, int SGN, bool VEL>
+__device__ __forceinline__ void d3q27_zouhe(R* f, R V, R P) {
+    using L = Lat<MODEL_D3Q27>;
+    R A = 0, Ky = 0, Kz = 0;
+#pragma unroll
+    for (int i = 0; i < 27; i++) {
+        const int en = SGN * L::ex(i);
+        if (en == 0) {
+            A += f[i];
+            Ky += f[i] * R(L::ey(i));
+            Kz += f[i] * R(L::ez(i));
+        } else if (en < 0) {
+            A += R(2) * f[i];
+        }
+    }
+    R rho, Jx;
+    if (VEL) {
+        rho = A / (R(1) - R(SGN) * V);
+        Jx = rho * V;
+    } else {
+        rho = P * R(3) + R(1);
+        Jx = R(SGN) * (rho - A);
+    }
+    const R Jy = R(-3) * Ky, Jz = R(-3) * Kz;
+#pragma unroll
+    for (int i = 0; i < 27; i++)
+        if (SGN * L::ex(i) > 0)
+            f[i] = f[26 - i] + R(6 * L::w(i)) * (R(L::ex(i)) * Jx + R(L::ey(i)) * Jy + R(L::ez(i)) * Jz);
+}
+
+// Symmetry(dir, sign) (src/lib/boundary.R:33-50): the populations with
+// sign*e_dir > 0 copy their mirror images in the plane normal to dir
+template <class R, int DIR, int SGN>
+__device__ __forceinline__ void d3q27_symmetry(R* f) {
+    constexpr int st = DIR == 1 ? 3 : 9;
+#pragma unroll
+    for (int i = 0; i < 27; i++) {
+        const int c = i / st % 3 - 1;
+        if (SGN * c > 0) f[i] = f[i - 2 * c * st];
+    }
+}
+
+// Run() and CollisionMRT of models/flow/d3q27/Dynamics.c.Rt:147-245.  par =
+// [omega, Smag, ForceX, ForceY, ForceZ]; zone columns [Velocity, Pressure,
+// Turbulence, XFluxInObj, YFluxInObj, ZFluxInObj, XDragForceInObj,
+// YDragForceInObj, ZDragForceInObj].
+template <class R, bool GLOB>
+__device__ __forceinline__ void d3q27_node(R* f, unsigned nt, const double* par, const double* zr, double* g) {
+    switch (nt & 0xFu) {
+    case 7: d3q27_zouhe<R, 1, false>(f, (R)zr[0], (R)zr[1]); break;   // WPressure
+    case 8: d3q27_zouhe<R, 1, true>(f, (R)zr[0], (R)zr[1]); break;    // WVelocity
+    case 4: d3q27_zouhe<R, -1, false>(f, (R)zr[0], (R)zr[1]); break;  // EPressure
+    case 1: d3q27_symmetry<R, 1, -1>(f); break;                       // NSymmetry
+    case 2: d3q27_symmetry<R, 1, 1>(f); break;                        // SSymmetry
+    case 3: d3q27_symmetry<R, 2, 1>(f); break;                        // ISymmetry
+    case 6: {                                                         // Wall
+#pragma unroll
+        for (int i = 0; i < 13; i++) { const R t = f[i]; f[i] = f[26 - i]; f[26 - i] = t; }
+        if (GLOB) {
+            // Ladd's momentum exchange, 2 sum_i f_i e_i after the swap
+            R fx = 0, fy = 0, fz = 0;
+#pragma unroll
+            for (int i = 0; i < 27; i++) {
+                fx += f[i] * R(Lat<MODEL_D3Q27>::ex(i));
+                fy += f[i] * R(Lat<MODEL_D3Q27>::ey(i));
+                fz += f[i] * R(Lat<MODEL_D3Q27>::ez(i));
+            }
+            g[3] += (double)(R(2) * fx); g[6] += zr[6];
+            g[4] += (double)(R(2) * fy); g[6] += zr[7];
+            g[5] += (double)(R(2) * fz); g[6] += zr[8];
+        }
+        break; }
+    default: break;
+    }
+    if ((nt & 0x10u) == 0) return;  // MRT
+    R m[27];
+#pragma unroll
+    for (int i = 0; i < 27; i++) m[i] = f[i];
+    d3q27_fwd(m);
+    const R rho = m[0], ir = R(1) / rho;
+    R Jx = m[1], Jy = m[3], Jz = m[9];
+#pragma unroll
+    for (int k = 0; k < 27; k++)
+        if (d3q27_order(k) >= 2) m[k] -= d3q27_req<R>(k, rho, Jx, Jy, Jz, ir);
+    R gamma = R(1) - (R)par[0];
+    if (nt & 0x40u) {  // Smagorinsky: Q = sum of the squared non-equilibrium stress
+        const R Q = m[2] * m[2] + m[6] * m[6] + m[18] * m[18] +
+                    R(2) * (m[4] * m[4] + m[10] * m[10] + m[12] * m[12]);
+        const R q = R(18) * sqrt(Q) * (R)par[1];
+        const R tau0 = R(1) / (R(1) - gamma);
+        R tau = sqrt(tau0 * tau0 + q);
+        tau = (tau + tau0) / R(2);
+        gamma = R(1) - R(1) / tau;
+    }
+    R gamma2 = gamma;
+    if (nt & 0x20u) {
+        // entropic stabiliser: a = ds.P.dh, b = dh.P.dh with ds, dh the order-2
+        // and order>2 non-equilibrium moments and P = M^-1 diag(1/w) M^-T,
+        // the tensor cube of the 1-D Gram matrix [[3/2,0,-3/2],[0,3,0],[-3/2,0,9/2]]
+        R y[27];
+#pragma unroll
+        for (int k = 0; k < 27; k++) y[k] = d3q27_order(k) > 2 ? m[k] : R(0);
+#pragma unroll
+        for (int s = 1; s <= 9; s *= 3)
+#pragma unroll
+            for (int b = 0; b < 27; b++) {
+                if ((b / s) % 3 != 0) continue;
+                const R u0 = y[b], u1 = y[b + s], u2 = y[b + 2 * s];
+                y[b] = R(1.5) * (u0 - u2);
+                y[b + s] = R(3) * u1;
+                y[b + 2 * s] = R(4.5) * u2 - R(1.5) * u0;
+            }
+        R a = 0, bb = 0;
+#pragma unroll
+        for (int k = 0; k < 27; k++) {
+            if (d3q27_order(k) == 2) a += m[k] * y[k];
+            if (d3q27_order(k) > 2) bb += m[k] * y[k];
+        }
+        gamma2 = -gamma2 * a / bb;
+    }
+#pragma unroll
+    for (int k = 0; k < 27; k++) {
+        if (d3q27_order(k) == 2) m[k] *= gamma;
+        if (d3q27_order(k) > 2) m[k] *= gamma2;
+    }
+    const R Fx = (R)par[2], Fy = (R)par[3], Fz = (R)par[4];
+    Jx += Fx; Jy += Fy; Jz += Fz;
+    if (GLOB) {
+        g[0] += (double)((Jx + Fx / R(2)) / rho); g[6] += zr[3];
+        g[1] += (double)((Jy + Fy / R(2)) / rho); g[6] += zr[4];
+        g[2] += (double)((Jz + Fz / R(2)) / rho); g[6] += zr[5];
+    }
+    if ((nt & 0xFu) == 5u) { Jx = 0; Jy = 0; Jz = 0; }  // Solid
+#pragma unroll
+    for (int k = 0; k < 27; k++)
+        if (d3q27_order(k) >= 2) m[k] += d3q27_req<R>(k, rho, Jx, Jy, Jz, ir);
+    m[0] = rho; m[1] = Jx; m[3] = Jy; m[9] = Jz;
+    d3q27_inv(m);
+#pragma unroll
+    for (int i = 0; i < 27; i++) f[i] = m[i];
+}
+
+// SetEquilibrum(rho, Jx, Jy, Jz) = Req M^-1
+template <class R>
+__device__ __forceinline__ void d3q27_feq(R* f, R rho, R Jx, R Jy, R Jz) {
+    const R ir = R(1) / rho;
+#pragma unroll
+    for (int k = 0; k < 27; k++) f[k] = d3q27_req<R>(k, rho, Jx, Jy, Jz, ir);
+    d3q27_inv(f);
+}
+
 // ---------------------------------------------------------- reductions ---
 // Sums are carried as unevaluated pairs (s, c), s + c, and combined with
 // Knuth's TwoSum, so the result is accurate to a few ulps of the exact sum
@@ -918,6 +1160,8 @@ __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, 
             d3q19_node<R, GLOB, MODEL == MODEL_D3Q19_LID>(f, nt, a.par, zr, g, a.flags, xc, y, z, nx, ny, nz);
         } else if constexpr (is_d2q9m<MODEL>) {
             d2q9m_node<R, GLOB, MODEL == MODEL_D2Q9_DRAG>(f, nt, a.par, zr, g);
+        } else if constexpr (MODEL == MODEL_D3Q27) {
+            d3q27_node<R, GLOB>(f, nt, a.par, zr, g);
         } else {
             d2q9_node<R, MODEL == MODEL_D2Q9_SRT_DRAG, GLOB>(f, nt, (R)a.par[0], (R)a.par[1], zr, g);
         }
@@ -1048,6 +1292,11 @@ __global__ void __launch_bounds__(256) k_init(StepArgs a) {
         R ux = (R)zr[0], uy = (R)zr[1];
         if ((nt & 0xFu) == 10u) { rho = R(1); ux = R(0); uy = R(0); }
         d2q9m_feq<R>(f, rho, ux * rho, uy * rho);
+    } else if constexpr (MODEL == MODEL_D3Q27) {
+        // Init(), models/flow/d3q27/Dynamics.c.Rt:184-190: SetEquilibrum(1 +
+        // 3 Pressure, Velocity + Turbulence*ST.x, ...) with no synthetic
+        // turbulence modes (ST = 0)
+        d3q27_feq<R>(f, R(1) + (R)zr[1] * R(3), (R)zr[0], R(0), R(0));
     } else {
         d2q9_feq<R>(f, (R)a.par[1], (R)zr[1], (R)zr[2]);
     }
@@ -1099,6 +1348,26 @@ __global__ void __launch_bounds__(256) k_quantity(StepArgs a, int qid, int x0, i
             if ((nt & 0x1Cu) == 0x0Cu) ux = uy = uz = 0;
             if (MODEL == MODEL_D3Q19_LID && (nt & 0x1Cu) == 0x1Cu) { ux = a.par[12]; uy = a.par[13]; uz = a.par[14]; }
             out[3 * tid] = ux * scale; out[3 * tid + 1] = uy * scale; out[3 * tid + 2] = uz * scale;
+        }
+    } else if constexpr (MODEL == MODEL_D3Q27) {
+        // getP, getU, getFd (models/flow/d3q27/Dynamics.c.Rt:33-73): P = 0,
+        // U = (J + F/2)/rho = 1, Fd = -2 J on Wall nodes (0 elsewhere) = 2
+        double d = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+        for (int i = 0; i < Q; i++) {
+            d += f[i]; jx += f[i] * L::ex(i); jy += f[i] * L::ey(i); jz += f[i] * L::ez(i);
+        }
+        if (qid == 0) {
+            out[tid] = (d - 1.0) / 3.0 * scale;
+        } else if (qid == 1) {
+            out[3 * tid] = (jx + a.par[2] * 0.5) / d * scale;
+            out[3 * tid + 1] = (jy + a.par[3] * 0.5) / d * scale;
+            out[3 * tid + 2] = (jz + a.par[4] * 0.5) / d * scale;
+        } else {
+            const bool wall = (nt & 0xFu) == 6u;
+            out[3 * tid] = wall ? -2.0 * jx * scale : 0.0;
+            out[3 * tid + 1] = wall ? -2.0 * jy * scale : 0.0;
+            out[3 * tid + 2] = wall ? -2.0 * jz * scale : 0.0;
         }
     } else if constexpr (is_d2q9m<MODEL>) {
         // getRho / getU, Dynamics.c.Rt:46-63: Rho = 0, U = J/rho + G/2 = 1
@@ -1264,6 +1533,7 @@ hipError_t quantity_m(int st, int q, const StepArgs& a, int x0, int y0, int z0, 
 }
 
 double weight(int model, int q) {
+    if (model == MODEL_D3Q27) return Lat<MODEL_D3Q27>::w(q);
     return (model == MODEL_D3Q19 || model == MODEL_D3Q19_LID) ? Lat<MODEL_D3Q19>::w(q) : Lat<MODEL_D2Q9_SRT>::w(q);
 }
 }  // namespace
@@ -1286,6 +1556,7 @@ hipError_t launch_step(int model, int storage, bool globals, const StepArgs& a, 
     case MODEL_D3Q19_LID: return step_m<MODEL_D3Q19_LID>(storage, globals, a, block, s);
     case MODEL_D2Q9: return step_m<MODEL_D2Q9>(storage, globals, a, block, s);
     case MODEL_D2Q9_DRAG: return step_m<MODEL_D2Q9_DRAG>(storage, globals, a, block, s);
+    case MODEL_D3Q27: return step_m<MODEL_D3Q27>(storage, globals, a, block, s);
     }
     return hipErrorInvalidValue;
 }
@@ -1298,6 +1569,7 @@ hipError_t launch_init(int model, int storage, const StepArgs& a, int block, hip
     case MODEL_D3Q19_LID: return init_m<MODEL_D3Q19_LID>(storage, a, block, s);
     case MODEL_D2Q9: return init_m<MODEL_D2Q9>(storage, a, block, s);
     case MODEL_D2Q9_DRAG: return init_m<MODEL_D2Q9_DRAG>(storage, a, block, s);
+    case MODEL_D3Q27: return init_m<MODEL_D3Q27>(storage, a, block, s);
     }
     return hipErrorInvalidValue;
 }
@@ -1356,6 +1628,8 @@ hipError_t launch_batch_step(int model, int storage, bool globals, const BatchCa
     case MODEL_D2Q9: return batch_m<MODEL_D2Q9>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
     case MODEL_D2Q9_DRAG:
         return batch_m<MODEL_D2Q9_DRAG>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
+    case MODEL_D3Q27:
+        return batch_m<MODEL_D3Q27>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
     }
     return hipErrorInvalidValue;
 }
@@ -1372,6 +1646,7 @@ hipError_t launch_quantity(int model, int storage, int q, const StepArgs& a, int
     case MODEL_D2Q9: return quantity_m<MODEL_D2Q9>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
     case MODEL_D2Q9_DRAG:
         return quantity_m<MODEL_D2Q9_DRAG>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
+    case MODEL_D3Q27: return quantity_m<MODEL_D3Q27>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
     }
     return hipErrorInvalidValue;
 }

@@ -227,9 +227,43 @@ ModelDef make_d2q9_mrt(bool drag) {
     return m;
 }
 
+// d3q27 (models/flow/d3q27/Dynamics.R): all 27 velocities of {-1,0,1}^3,
+// density i at ((i%3)-1, (i/3)%3-1, i/9-1) as expand.grid orders them; MRT
+// in raw moments relaxing every order >= 2 (BGK unless Stab or Smagorinsky
+// nodes change the rates), Zou-He on W/E, symmetry planes, drag by momentum
+// exchange on Wall nodes, flux Globals.
+ModelDef make_d3q27() {
+    ModelDef m;
+    m.id = MODEL_D3Q27;
+    m.name = "d3q27";
+    m.Q = 27; m.dim = 3;
+    for (int i = 0; i < 27; i++) {
+        m.ex.push_back(i % 3 - 1);
+        m.ey.push_back(i / 3 % 3 - 1);
+        m.ez.push_back(i / 9 - 1);
+        const int nz = (m.ex[i] != 0) + (m.ey[i] != 0) + (m.ez[i] != 0);
+        m.w.push_back(nz == 0 ? 8. / 27 : nz == 1 ? 2. / 27 : nz == 2 ? 1. / 54 : 1. / 216);
+    }
+    // Dynamics.R:16-26 (no units declared), Threshold from src/conf.R:769
+    m.settings = {{"omega", 0.0, -1, 0}, {"nu", 0.16666666, 0, 1}, {"Smag", 0.0, -1, 0},
+                  {"ForceX", 0.0, -1, 0}, {"ForceY", 0.0, -1, 0}, {"ForceZ", 0.0, -1, 0},
+                  {"Threshold", 0.5, -1, 0}};
+    m.zonal = {{"Velocity", 0, -1, 0}, {"Pressure", 0, -1, 0}, {"Turbulence", 0, -1, 0}};
+    std::vector<GlobalDef> g;
+    for (const char* a : {"X", "Y", "Z"}) g.push_back({std::string(a) + "Flux", false, "m3/s"});
+    for (const char* a : {"X", "Y", "Z"}) g.push_back({std::string(a) + "DragForce", false, "N"});
+    finish_globals(m, g);
+    m.quantities = {{"P", false, "Pa"}, {"U", true, "m/s"}, {"Fd", true, "N"}};
+    pack_nodetypes(m, {{"Smagorinsky", "LES"}, {"Stab", "ENTROPIC"}, {"NSymmetry", "BOUNDARY"},
+                       {"SSymmetry", "BOUNDARY"}, {"ISymmetry", "BOUNDARY"}, {"EPressure", "BOUNDARY"},
+                       {"Solid", "BOUNDARY"}, {"Wall", "BOUNDARY"}, {"WPressure", "BOUNDARY"},
+                       {"WVelocity", "BOUNDARY"}, {"MRT", "COLLISION"}});
+    return m;
+}
+
 const std::vector<ModelDef>& registry() {
     static std::vector<ModelDef> r = {make_d2q9(false), make_d2q9(true), make_d3q19(false), make_d3q19(true),
-                                      make_d2q9_mrt(false), make_d2q9_mrt(true)};
+                                      make_d2q9_mrt(false), make_d2q9_mrt(true), make_d3q27()};
     return r;
 }
 

@@ -25,7 +25,8 @@ enum ModelId {
     MODEL_D3Q19_LID = 3,
     MODEL_D2Q9 = 4,
     MODEL_D2Q9_DRAG = 5,
-    MODEL_COUNT = 6
+    MODEL_D3Q27 = 6,
+    MODEL_COUNT = 7
 };
 
 // storage format of the distributions in HBM

@@ -28,7 +28,7 @@ def test_every_declared_symbol_is_exported():
 
 
 def test_models():
-    assert tclb_amd.models() == ["d2q9_SRT", "d2q9_SRT_drag", "d3q19", "d3q19_lid", "d2q9", "d2q9_drag"]
+    assert tclb_amd.models() == ["d2q9_SRT", "d2q9_SRT_drag", "d3q19", "d3q19_lid", "d2q9", "d2q9_drag", "d3q27"]
 
 
 def test_d2q9_mrt_contract():
@@ -124,4 +124,24 @@ def test_lattice_creation_fails_loudly_without_gpu():
 
 def test_unknown_model_is_an_error():
     with pytest.raises(tclb_amd.TclbxError):
-        tclb_amd.Lattice("d3q27", 8, 8, 8)
+        tclb_amd.Lattice("d3q27_cumulant", 8, 8, 8)
+
+
+def test_d3q27_contract():
+    """models/flow/d3q27/Dynamics.R: groups BOUNDARY (8 types, 4 bits),
+    COLLISION, ENTROPIC, LES in alphabetical order, zone from bit 7."""
+    m = tclb_amd.Model("d3q27")
+    assert m.Q == 27
+    order = ["NSymmetry", "SSymmetry", "ISymmetry", "EPressure", "Solid", "Wall", "WPressure", "WVelocity"]
+    for k, n in enumerate(order):
+        assert m.nodetype(n) == (k + 1, 0xF)
+    assert m.nodetype("MRT") == (0x10, 0x10)
+    assert m.nodetype("Stab") == (0x20, 0x20)
+    assert m.nodetype("Smagorinsky") == (0x40, 0x40)
+    assert m.group("SETTINGZONE") == (0xFF80, 7)
+    assert m.settings == ["omega", "nu", "Smag", "ForceX", "ForceY", "ForceZ", "Threshold"]
+    assert m.zonal_settings == ["Velocity", "Pressure", "Turbulence", "XFluxInObj", "YFluxInObj", "ZFluxInObj",
+                                "XDragForceInObj", "YDragForceInObj", "ZDragForceInObj"]
+    assert m.global_names == ["XFlux", "YFlux", "ZFlux", "XDragForce", "YDragForce", "ZDragForce", "Objective"]
+    assert m.quantities == ["P", "U", "Fd"]
+    assert m.quantity_id("Fd") == (2, True)

@@ -772,3 +772,152 @@ def test_d2q9_drag_is_d2q9_plus_momentum_exchange(O):
     assert abs(g1[3] - fx) < 1e-14 and abs(g1[4] - fy) < 1e-14
     nsolid = int(np.count_nonzero(fl == 9))
     assert abs(g1[5] - (g0[3] + nsolid * (0.25 + 0.125))) < 1e-12
+
+
+# --------------------------------------------------------------- d3q27 ---
+E27 = np.array([(i % 3 - 1, i // 3 % 3 - 1, i // 9 - 1) for i in range(27)], float)
+W27 = np.array([[8 / 27, 2 / 27, 1 / 54, 1 / 216][int((e != 0).sum())] for e in E27])
+
+
+def _feq27(rho, J):
+    """the textbook second-order D3Q27 equilibrium"""
+    u = np.asarray(J, float) / rho
+    cu = E27 @ u
+    return W27 * rho * (1 + 3 * cu + 4.5 * cu ** 2 - 1.5 * (u @ u))
+
+
+def _f27(rng, amp=0.02):
+    rho, J = 1 + 0.01 * rng.standard_normal(), 0.03 * rng.standard_normal(3)
+    return _feq27(rho, J) * (1 + amp * rng.standard_normal(27))
+
+
+def test_d3q27_basis_and_equilibrium(O):
+    """The moments are the 27 monomials ex^p ey^q ez^r, p,q,r in {0,1,2},
+    by nondecreasing order (rho, then Jx, Jy, Jz); Req M^-1 is the textbook
+    second-order equilibrium and the rest weights 8/27, 2/27, 1/54, 1/216."""
+    M, order, pw = O.d3q27_basis()
+    assert sorted(map(tuple, pw)) == sorted((p, q, r) for p in range(3) for q in range(3) for r in range(3))
+    assert list(order) == sorted(order) and list(order) == list(pw.sum(1))
+    assert [tuple(v) for v in pw[:4]] == [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1)]
+    mono = np.array([[np.prod([1.0 if p == 0 else e[d] ** (1 if p == 1 else 2) for d, p in enumerate(c)])
+                      for c in pw] for e in E27])
+    assert np.array_equal(M, mono)
+    assert np.allclose(O.d3q27_feq(1, 0, 0, 0), W27, rtol=0, atol=4e-16)
+    rng = np.random.default_rng(270)
+    for _ in range(5):
+        rho, J = 1 + 0.05 * rng.standard_normal(), 0.05 * rng.standard_normal(3)
+        assert np.allclose(O.d3q27_feq(rho, *J), _feq27(rho, J), rtol=0, atol=4e-16)
+
+
+def test_d3q27_mrt_is_bgk_with_momentum_shift(O):
+    """Every moment of order >= 2 relaxes by 1 - omega: f' = feq(rho, J + F)
+    + (1 - omega)(f - feq(rho, J)); the flux Globals add (J + F + F/2)/rho
+    and their zone weights; a Solid node ends with zero momentum."""
+    rng = np.random.default_rng(271)
+    omega, F = 1.3, np.array([1e-4, -2e-5, 3e-5])
+    zone = [0, 0, 0, 0.5, 0.25, 0.125, 0, 0, 0]
+    for nt, solid in ((0x10, False), (0x15, True)):
+        f = _f27(rng)
+        rho, J = f.sum(), E27.T @ f
+        out, g = O.d3q27_node(f, nt, [omega, 0, *F], zone)
+        want = _feq27(rho, np.zeros(3) if solid else J + F) + (1 - omega) * (f - _feq27(rho, J))
+        assert np.allclose(out, want, rtol=0, atol=1e-15)
+        assert np.allclose(g[:3], (J + 1.5 * F) / rho, rtol=1e-14, atol=0)
+        assert abs(g[6] - 0.875) < 1e-15 and not g[3:6].any()
+
+
+def test_d3q27_smagorinsky_rate(O):
+    """Smagorinsky nodes relax at 1/tau with tau = (tau0 + sqrt(tau0^2 +
+    18 Smag |Pi|))/2, |Pi| the Frobenius norm of the non-equilibrium
+    stress sum_i e_a e_b (f_i - feq_i)."""
+    rng = np.random.default_rng(272)
+    omega, smag = 1.6, 0.17
+    f = _f27(rng, 0.05)
+    rho, J = f.sum(), E27.T @ f
+    fneq = f - _feq27(rho, J)
+    Pi = np.einsum("ia,ib,i->ab", E27, E27, fneq)
+    tau0 = 1 / omega
+    tau = (tau0 + np.sqrt(tau0 ** 2 + 18 * smag * np.sqrt((Pi ** 2).sum()))) / 2
+    out, _ = O.d3q27_node(f, 0x50, [omega, smag, 0, 0, 0])
+    assert np.allclose(out, _feq27(rho, J) + (1 - 1 / tau) * fneq, rtol=0, atol=1e-15)
+    out0, _ = O.d3q27_node(f, 0x50, [omega, 0, 0, 0, 0])
+    assert np.allclose(out0, _feq27(rho, J) + (1 - omega) * fneq, rtol=0, atol=1e-15)
+
+
+def test_d3q27_entropic_rate(O):
+    """Stab nodes relax the order > 2 part dh of the non-equilibrium at
+    gamma2 = -gamma (ds.P.dh)/(dh.P.dh), ds its order-2 part, P =
+    M^-1 diag(1/w) M^-T: numpy's inverse of the monomial matrix and the
+    textbook equilibrium give the same update."""
+    rng = np.random.default_rng(273)
+    omega = 1.2
+    M, order, _ = O.d3q27_basis()
+    MI = np.linalg.inv(M)
+    f = _f27(rng, 0.05)
+    rho, J = f.sum(), E27.T @ f
+    neq = (f - _feq27(rho, J)) @ M
+    ds = np.where(order == 2, neq, 0)
+    dh = np.where(order > 2, neq, 0)
+    P = MI @ np.diag(1 / W27) @ MI.T
+    g = 1 - omega
+    g2 = -g * (ds @ P @ dh) / (dh @ P @ dh)
+    want = _feq27(rho, J) + (g * ds + g2 * dh) @ MI
+    out, _ = O.d3q27_node(f, 0x30, [omega, 0, 0, 0, 0])
+    assert np.allclose(out, want, rtol=0, atol=1e-15)
+    assert abs(g2 - g) > 0.05          # the stabiliser changed the rate
+
+
+@pytest.mark.parametrize("nt,sgn,vel", [(8, 1, True), (7, 1, False), (4, -1, False)])
+def test_d3q27_zou_he(O, nt, sgn, vel):
+    """W/E Zou-He: the entering populations (sign*ex > 0) become
+    f_opp + 6 w_i (e_i . J) for one vector J, and the node then has the
+    prescribed density (pressure 3P+1) or velocity (rho V along x) and no
+    tangential momentum; a velocity inlet's density is A/(1 - V)."""
+    rng = np.random.default_rng(274 + nt)
+    V, Pr = 0.04, 0.003
+    f = _f27(rng)
+    out, _ = O.d3q27_node(f, nt, [1.0, 0, 0, 0, 0], [V, Pr, 0])
+    sel = sgn * E27[:, 0] > 0
+    assert np.array_equal(out[~sel], f[~sel])
+    d = out[sel] - f[::-1][sel]                     # opposite of i is 26 - i
+    A = 6 * W27[sel, None] * E27[sel]
+    Jfit, *_ = np.linalg.lstsq(A, d, rcond=None)
+    assert np.allclose(A @ Jfit, d, rtol=0, atol=1e-15)
+    rho, Jn = out.sum(), E27.T @ out
+    if vel:
+        known = f[E27[:, 0] == 0].sum() + 2 * f[sgn * E27[:, 0] < 0].sum()
+        assert abs(rho - known / (1 - sgn * V)) < 1e-14
+        assert abs(Jn[0] - rho * V) < 1e-15
+    else:
+        assert abs(rho - (3 * Pr + 1)) < 1e-14
+    assert np.allclose(Jn[1:], 0, atol=1e-15)
+    assert abs(Jfit[0] - Jn[0]) < 1e-14           # x: the node's own momentum
+
+
+@pytest.mark.parametrize("nt,axis,sgn", [(1, 1, -1), (2, 1, 1), (3, 2, 1)])
+def test_d3q27_symmetry(O, nt, axis, sgn):
+    """N/S/I symmetry: the populations with sign*e_axis > 0 become their
+    mirror images in the plane normal to the axis, the rest stay."""
+    rng = np.random.default_rng(280 + nt)
+    f = _f27(rng)
+    out, _ = O.d3q27_node(f, nt, [1.0, 0, 0, 0, 0])
+    for i in range(27):
+        e = E27[i].copy()
+        if sgn * e[axis] > 0:
+            e[axis] = -e[axis]
+            j = int(np.nonzero((E27 == e).all(1))[0][0])
+            assert out[i] == f[j]
+        else:
+            assert out[i] == f[i]
+
+
+def test_d3q27_wall_bounce_back_and_drag(O):
+    """Wall nodes reverse every population; the drag Globals add
+    2 sum_i f'_i e_i = -2 J of the incoming populations, and the zone's
+    XDragForceInObj.. weights go to Objective."""
+    rng = np.random.default_rng(285)
+    f = _f27(rng)
+    out, g = O.d3q27_node(f, 6, [1.0, 0, 0, 0, 0], [0, 0, 0, 0, 0, 0, 0.5, 0.25, 0.125])
+    assert np.array_equal(out, f[::-1])
+    assert np.allclose(g[3:6], -2 * (E27.T @ f), rtol=0, atol=1e-15)
+    assert not g[:3].any() and abs(g[6] - 0.875) < 1e-15
