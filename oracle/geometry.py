@@ -244,6 +244,12 @@ NODETYPES["d3q27"] = [("Smagorinsky", "LES"), ("Stab", "ENTROPIC"), ("NSymmetry"
                       ("SSymmetry", "BOUNDARY"), ("ISymmetry", "BOUNDARY"), ("EPressure", "BOUNDARY"),
                       ("Solid", "BOUNDARY"), ("Wall", "BOUNDARY"), ("WPressure", "BOUNDARY"),
                       ("WVelocity", "BOUNDARY"), ("MRT", "COLLISION")]
+# models/flow/d3q27_cumulant/Dynamics.R:41-49,109-115 (no build options)
+NODETYPES["d3q27_cumulant"] = [
+    ("Buffer", "BOUNDARY"), ("WVelocityTurbulent", "BOUNDARY"), ("NVelocity", "BOUNDARY"), ("SVelocity", "BOUNDARY"),
+    ("NPressure", "BOUNDARY"), ("SPressure", "BOUNDARY"), ("NSymmetry", "ADDITIONALS"), ("SSymmetry", "ADDITIONALS"),
+    ("Body", "BODY"), ("EPressure", "BOUNDARY"), ("EVelocity", "BOUNDARY"), ("Solid", "BOUNDARY"),
+    ("Wall", "BOUNDARY"), ("WPressure", "BOUNDARY"), ("WVelocity", "BOUNDARY"), ("MRT", "COLLISION")]
 
 
 class Packing:

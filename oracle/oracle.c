@@ -155,13 +155,15 @@ orc_lattice* orc_create(int model, int nx, int ny, int nz, int storage)
     d3q19_sparse_init_f();
     L->model = model; L->nx = nx; L->ny = ny; L->nz = nz; L->storage = storage;
     L->N = (size_t)nx * ny * nz;
-    if (model == ORC_D3Q27) {
-        q27_init_d();
-        q27_init_f();
-        L->Q = 27; L->nglob = 7;
+    if (model == ORC_D3Q27 || model == ORC_D3Q27_CUMULANT) {
+        int cum = model == ORC_D3Q27_CUMULANT;
+        cum_init_d();
+        cum_init_f();
+        L->Q = 27; L->nglob = cum ? 10 : 7;
         for (q = 0; q < 27; q++) {
-            for (a = 0; a < 3; a++) L->e[q][a] = q27_e_d[q][a];
-            L->w[q] = q27_w_d[q];
+            int j = cum ? cum_map_d[q] : q;
+            for (a = 0; a < 3; a++) L->e[q][a] = q27_e_d[j][a];
+            L->w[q] = q27_w_d[j];
         }
     } else if (model == ORC_D3Q19 || model == ORC_D3Q19_LID) {
         L->Q = 19; L->nglob = model == ORC_D3Q19_LID ? 30 : 26;
@@ -220,7 +222,11 @@ void orc_set_flags(orc_lattice* L, const uint16_t* flags)
 static int is3d(const orc_lattice* L) { return L->model == ORC_D3Q19 || L->model == ORC_D3Q19_LID; }
 static int is_d2q9m(const orc_lattice* L) { return L->model == ORC_D2Q9 || L->model == ORC_D2Q9_DRAG; }
 static int is_q27(const orc_lattice* L) { return L->model == ORC_D3Q27; }
-static int zone_shift(const orc_lattice* L) { return is3d(L) ? 6 : (is_d2q9m(L) || is_q27(L) ? 7 : 4); }
+static int is_cum(const orc_lattice* L) { return L->model == ORC_D3Q27_CUMULANT; }
+static int zone_shift(const orc_lattice* L)
+{
+    return is3d(L) ? 6 : (is_cum(L) ? 8 : (is_d2q9m(L) || is_q27(L) ? 7 : 4));
+}
 
 static const double* zone_row(const orc_lattice* L, uint16_t nt)
 {
@@ -235,7 +241,21 @@ void orc_init(orc_lattice* L)
     int q;
     for (n = 0; n < L->N; n++) {
         const double* zr = zone_row(L, L->flags[n]);
-        if (is_q27(L)) {
+        if (is_cum(L)) {
+            /* Init(), models/flow/d3q27_cumulant/Dynamics.c.Rt:266-275 */
+            int wall = ((L->flags[n] >> 3) & 0xF) == 10;
+            if (L->storage == ORC_F64) {
+                double f[27];
+                if (wall) cum_feq_d(f, 1.0, 0., 0., 0.);
+                else cum_feq_d(f, 1.0 + zr[1] * 3.0, zr[0], 0., 0.);
+                for (q = 0; q < 27; q++) st(L, L->buf[0], (size_t)q * L->N + n, q, f[q]);
+            } else {
+                float f[27];
+                if (wall) cum_feq_f(f, 1.0f, 0.f, 0.f, 0.f);
+                else cum_feq_f(f, 1.0f + (float)zr[1] * 3.0f, (float)zr[0], 0.f, 0.f);
+                for (q = 0; q < 27; q++) st(L, L->buf[0], (size_t)q * L->N + n, q, f[q]);
+            }
+        } else if (is_q27(L)) {
             /* Init(), models/flow/d3q27/Dynamics.c.Rt:184-190, no synthetic
              * turbulence modes: SetEquilibrum(1 + 3 Pressure, Velocity, 0, 0) */
             if (L->storage == ORC_F64) {
@@ -321,7 +341,8 @@ static void step_rows(orc_lattice* L, long double* glob, long r0, long r1)
                 double fd[27];
                 pull(L, in, x, y, z, fd);
                 if (L->storage == ORC_F64) {
-                    if (is_q27(L)) d3q27_node_d(fd, nt, L->params, zr, glob);
+                    if (is_cum(L)) cum_node_d(fd, nt, L->params, zr, glob);
+                    else if (is_q27(L)) d3q27_node_d(fd, nt, L->params, zr, glob);
                     else if (is3d(L)) d3q19_node_d(fd, nt, L->params, zr, glob, lid, L->flags, x, y, z, L->nx, L->ny, L->nz);
                     else if (is_d2q9m(L)) d2q9m_node_d(fd, nt, L->params, zr, glob, L->model == ORC_D2Q9_DRAG);
                     else d2q9_node_d(fd, nt, L->params[0], L->params[1], zr, drag, glob);
@@ -329,7 +350,8 @@ static void step_rows(orc_lattice* L, long double* glob, long r0, long r1)
                 } else {
                     float ff[27];
                     for (q = 0; q < L->Q; q++) ff[q] = (float)fd[q];
-                    if (is_q27(L)) d3q27_node_f(ff, nt, L->params, zr, glob);
+                    if (is_cum(L)) cum_node_f(ff, nt, L->params, zr, glob);
+                    else if (is_q27(L)) d3q27_node_f(ff, nt, L->params, zr, glob);
                     else if (is3d(L)) d3q19_node_f(ff, nt, L->params, zr, glob, lid, L->flags, x, y, z, L->nx, L->ny, L->nz);
                     else if (is_d2q9m(L)) d2q9m_node_f(ff, nt, L->params, zr, glob, L->model == ORC_D2Q9_DRAG);
                     else d2q9_node_f(ff, nt, (float)L->params[0], (float)L->params[1], zr, drag, glob);
@@ -414,7 +436,27 @@ void orc_get_quantity(const orc_lattice* L, int which, double* out)
                 uint16_t nt = L->flags[n];
                 double f[27], rho = 0, jx = 0, jy = 0, jz = 0;
                 pull(L, L->buf[L->cur], x, y, z, f);
-                if (is_q27(L)) {
+                if (is_cum(L)) {
+                    /* getP on the pulled f; getU after RunBoundaries (0 on
+                     * Wall, else (J + F/2)/rho); getSolid (1 on Wall),
+                     * models/flow/d3q27_cumulant/Dynamics.c.Rt:23-62 */
+                    int wall = ((nt >> 3) & 0xF) == 10;
+                    for (q = 0; q < 27; q++) rho += f[q];
+                    if (which == 0) out[n] = rho;
+                    else if (which == 2) out[n] = (rho - 1.0) / 3.0;
+                    else if (which == 4) out[n] = wall ? 1.0 : 0.0;
+                    else if (which == 1) {
+                        cum_boundaries_d(f, nt, zone_row(L, nt), NULL);
+                        rho = 0;
+                        for (q = 0; q < 27; q++) {
+                            rho += f[q];
+                            jx += f[q] * L->e[q][0]; jy += f[q] * L->e[q][1]; jz += f[q] * L->e[q][2];
+                        }
+                        out[3*n] = wall ? 0.0 : (jx + L->params[3] / 2.) / rho;
+                        out[3*n+1] = wall ? 0.0 : (jy + L->params[4] / 2.) / rho;
+                        out[3*n+2] = wall ? 0.0 : (jz + L->params[5] / 2.) / rho;
+                    } else out[3*n] = out[3*n+1] = out[3*n+2] = 0;
+                } else if (is_q27(L)) {
                     /* getP, getU, getFd, models/flow/d3q27/Dynamics.c.Rt:33-73 */
                     for (q = 0; q < 27; q++) {
                         rho += f[q];
@@ -487,6 +529,14 @@ void orc_d3q27_node(double f[27], uint16_t nt, const double* params, const doubl
     q27_init_d();
     d3q27_node_d(f, nt, params, zone, globals ? g : NULL);
     if (globals) for (i = 0; i < 7; i++) globals[i] = (double)g[i];
+}
+void orc_cum_node(double f[27], uint16_t nt, const double* params, const double* zone, double* globals)
+{
+    long double g[10] = {0};
+    int i;
+    cum_init_d();
+    cum_node_d(f, nt, params, zone, globals ? g : NULL);
+    if (globals) for (i = 0; i < 10; i++) globals[i] = (double)g[i];
 }
 void orc_d3q27_basis(double M[27 * 27], int order[27], int p[27 * 3])
 {

@@ -18,6 +18,8 @@
  *   d3q27          models/flow/d3q27/Dynamics.c.Rt, Dynamics.R with the raw
  *                  moment basis of src/lib/feq.R (ortogonal=FALSE), Zou-He of
  *                  src/lib/boundary.R:66-109, Symmetry of :33-50
+ *   d3q27_cumulant models/flow/d3q27_cumulant/Dynamics.c.Rt, Dynamics.R without
+ *                  the AVG/IB/SMAG build options
  *   d3q19_lid      d3q19 plus a MovingWall node type (bounce-back with Ladd's
  *                  wall-velocity term) and its force/torque Globals; the
  *                  reference has no moving wall, so this part is pinned by
@@ -43,7 +45,7 @@ extern "C" {
 #endif
 
 enum { ORC_D2Q9_SRT = 0, ORC_D2Q9_SRT_DRAG = 1, ORC_D3Q19 = 2, ORC_D3Q19_LID = 3, ORC_D2Q9 = 4,
-       ORC_D2Q9_DRAG = 5, ORC_D3Q27 = 6 };
+       ORC_D2Q9_DRAG = 5, ORC_D3Q27 = 6, ORC_D3Q27_CUMULANT = 7 };
 enum { ORC_F64 = 0, ORC_F32 = 1, ORC_F32S = 2, ORC_F16S = 3, ORC_BF16S = 4 };
 
 /* d2q9 parameter vector:   [omega, Density]
@@ -60,7 +62,11 @@ enum { ORC_F64 = 0, ORC_F32 = 1, ORC_F32S = 2, ORC_F16S = 3, ORC_BF16S = 4 };
  *                           OutletFluxInObj, InletFluxInObj]
  * d3q27 parameters:        [omega, Smag, ForceX, ForceY, ForceZ]
  * d3q27 zone columns:      [Velocity, Pressure, Turbulence, then the six
- *                           <Global>InObj weights in Globals order]        */
+ *                           <Global>InObj weights in Globals order]
+ * d3q27_cumulant params:   [nu, nubuffer, GalileanCorrection, ForceX, ForceY,
+ *                           ForceZ, Omega]
+ * d3q27_cumulant zones:    [Velocity, Pressure, Turbulence, then the nine
+ *                           <Global>InObj weights in Globals order]         */
 
 typedef struct orc_lattice orc_lattice;
 
@@ -78,7 +84,7 @@ void orc_iterate(orc_lattice* L, int niter, int glob_last, double* globals_out);
 void orc_get_f(const orc_lattice* L, double* out);
 void orc_set_f(orc_lattice* L, const double* in);
 /* quantity: 0 = Rho, 1 = U (3 doubles per node, interleaved), 2 = P,
- * 3 = Fd (d3q27, 3 doubles per node).
+ * 3 = Fd (d3q27, 3 doubles per node), 4 = Solid (d3q27_cumulant).
  * Evaluated like the reference get<Q> kernels: on pulled (streamed)
  * populations of the current snapshot, without boundary or collision. */
 void orc_get_quantity(const orc_lattice* L, int which, double* out);
@@ -101,6 +107,8 @@ void orc_d3q27_feq(double feq[27], double rho, double jx, double jy, double jz);
 void orc_d3q27_node(double f[27], uint16_t nt, const double* params, const double* zone, double* globals);
 /* the moment basis: M[27*i + j] = moment j of direction i, its order and powers */
 void orc_d3q27_basis(double M[27 * 27], int order[27], int p[27 * 3]);
+/* one node update of d3q27_cumulant; globals[10] if not NULL */
+void orc_cum_node(double f[27], uint16_t nt, const double* params, const double* zone, double* globals);
 
 #ifdef __cplusplus
 }

@@ -12,6 +12,8 @@ namespace tclbx {
 // e, eps, q, pi and m moment groups, then gamma1 = 1 - omega (nu group),
 // then (d3q19_lid) MovingWallVelocityX/Y/Z].  d2q9 (MRT): [S2, S3, S4,
 // GravitationX, GravitationY].  d3q27: [omega, Smag, ForceX, ForceY, ForceZ].
+// d3q27_cumulant: [nu, nubuffer, GalileanCorrection, ForceX, ForceY, ForceZ,
+// Omega].
 constexpr int MAX_PAR = 16;
 constexpr int MAX_GLOBALS = 32;
 

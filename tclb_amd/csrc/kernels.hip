@@ -21,6 +21,8 @@
 // reference's per-warp atomicAdd, src/cross.hpp:202-218).
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 #include "kernels.h"
 #include "model.h"
 #include "storage.h"
@@ -82,10 +84,39 @@ template <> struct Lat<MODEL_D3Q27> {
     __host__ __device__ static constexpr int ex(int i) { return i % 3 - 1; }
     __host__ __device__ static constexpr int ey(int i) { return i / 3 % 3 - 1; }
     __host__ __device__ static constexpr int ez(int i) { return i / 9 - 1; }
+    // direction with the component on axis ax (3: every axis) negated
+    __host__ __device__ static constexpr int mirror(int i, int ax) {
+        return ax == 3 ? 26 - i : i + (2 - 2 * (i / (ax == 0 ? 1 : ax == 1 ? 3 : 9) % 3)) * (ax == 0 ? 1 : ax == 1 ? 3 : 9);
+    }
     __host__ __device__ static constexpr double w(int i) {
         return ((i % 3 != 1) + (i / 3 % 3 != 1) + (i / 9 != 1)) == 0 ? 8. / 27
              : ((i % 3 != 1) + (i / 3 % 3 != 1) + (i / 9 != 1)) == 1 ? 2. / 27
              : ((i % 3 != 1) + (i / 3 % 3 != 1) + (i / 9 != 1)) == 2 ? 1. / 54 : 1. / 216;
+    }
+};
+
+// d3q27_cumulant: density i = px + 3py + 9pz at e = {0,1,-1}[p] per axis
+// (models/flow/d3q27_cumulant/Dynamics.R:1-13); NodeType groups ADDITIONALS
+// (2 bits), BODY, BOUNDARY (4 bits from bit 3), COLLISION (bit 7), zone
+// from bit 8; Globals Density, Flux, Drag, Lift, Lateral, Mass, X/Y/Z
+// Momentum, Objective
+template <> struct Lat<MODEL_D3Q27_CUMULANT> {
+    static constexpr int Q = 27, G = 10, SUMG = 10, ZSHIFT = 8;
+    __host__ __device__ static constexpr int e1(int p) { return p == 0 ? 0 : (p == 1 ? 1 : -1); }
+    __host__ __device__ static constexpr int ex(int i) { return e1(i % 3); }
+    __host__ __device__ static constexpr int ey(int i) { return e1(i / 3 % 3); }
+    __host__ __device__ static constexpr int ez(int i) { return e1(i / 9); }
+    // digit p -> (3 - p) % 3 negates a component
+    __host__ __device__ static constexpr int neg(int i, int st) {
+        return i + ((3 - i / st % 3) % 3 - i / st % 3) * st;
+    }
+    __host__ __device__ static constexpr int mirror(int i, int ax) {
+        return ax == 3 ? neg(neg(neg(i, 1), 3), 9) : neg(i, ax == 0 ? 1 : ax == 1 ? 3 : 9);
+    }
+    __host__ __device__ static constexpr double w(int i) {
+        return ((i % 3 != 0) + (i / 3 % 3 != 0) + (i / 9 != 0)) == 0 ? 8. / 27
+             : ((i % 3 != 0) + (i / 3 % 3 != 0) + (i / 9 != 0)) == 1 ? 2. / 27
+             : ((i % 3 != 0) + (i / 3 % 3 != 0) + (i / 9 != 0)) == 2 ? 1. / 54 : 1. / 216;
     }
 };
 
@@ -98,6 +129,7 @@ int kernel_nglobals(int model) {
     case MODEL_D2Q9: return Lat<MODEL_D2Q9>::G;
     case MODEL_D2Q9_DRAG: return Lat<MODEL_D2Q9_DRAG>::G;
     case MODEL_D3Q27: return Lat<MODEL_D3Q27>::G;
+    case MODEL_D3Q27_CUMULANT: return Lat<MODEL_D3Q27_CUMULANT>::G;
     }
     return 0;
 }
@@ -891,54 +923,79 @@ __device__ __forceinline__ R d3q27_req(int k, R rho, R Jx, R Jy, R Jz, R ir) {
     return R(0);
 }
 
-// Zou-He on an x face (ZouHe(EQ, 1, sign, type), src/lib/boundary.R:66-109):
-// the entering populations (sign*ex > 0) become f_opp + 6 w_i (e_i . J),
-// the non-equilibrium bounce-back of the second-order equilibrium, with J
-// solved so that the node has density rho and no tangential momentum.
-// With A = sum of the ex = 0 populations + 2 * the outgoing ones and
-// K_y, K_z the tangential momentum of the ex = 0 populations:
-//   velocity V: rho = A / (1 - sign V), Jx = rho V;  pressure P: rho =
-//   3P + 1, Jx = sign (rho - A);  and Jy = -3 K_y, Jz = -3 K_z.
-template <class R, int SGN, bool VEL>
-__device__ __forceinline__ void d3q27_zouhe(R* f, R V, R P) {
-    using L = Lat<MODEL_D3Q27>;
-    R A = 0, Ky = 0, Kz = 0;
+template <class L> __host__ __device__ constexpr int q27_e(int i, int ax) {
+    return ax == 0 ? L::ex(i) : (ax == 1 ? L::ey(i) : L::ez(i));
+}
+
+// Zou-He on a face normal to axis AX (ZouHe(EQ, AX+1, sign, type),
+// src/lib/boundary.R:66-109, with the equilibrium of MRT_eq(ortogonal=
+// FALSE)): the entering populations (SGN*e_AX > 0) become f_opp + 6 w_i
+// (e_i . J), the non-equilibrium bounce-back of the second-order
+// equilibrium, with J solved so that the node has density rho and no
+// tangential momentum.  With A = sum of the e_AX = 0 populations + 2 * the
+// outgoing ones and K_t the tangential momentum of the e_AX = 0 populations:
+//   velocity V: rho = A / (1 - SGN V), J_AX = rho V;  pressure P: rho =
+//   3P + 1, J_AX = SGN (rho - A);  and J_t = -3 K_t.
+template <class R, class L, int AX, int SGN, bool VEL>
+__device__ __forceinline__ void q27_zouhe(R* f, R V, R P) {
+    R A = 0, K[3] = {0, 0, 0};
 #pragma unroll
     for (int i = 0; i < 27; i++) {
-        const int en = SGN * L::ex(i);
+        const int en = SGN * q27_e<L>(i, AX);
         if (en == 0) {
             A += f[i];
-            Ky += f[i] * R(L::ey(i));
-            Kz += f[i] * R(L::ez(i));
+            K[0] += f[i] * R(L::ex(i));
+            K[1] += f[i] * R(L::ey(i));
+            K[2] += f[i] * R(L::ez(i));
         } else if (en < 0) {
             A += R(2) * f[i];
         }
     }
-    R rho, Jx;
+    R rho, Jn;
     if (VEL) {
         rho = A / (R(1) - R(SGN) * V);
-        Jx = rho * V;
+        Jn = rho * V;
     } else {
         rho = P * R(3) + R(1);
-        Jx = R(SGN) * (rho - A);
+        Jn = R(SGN) * (rho - A);
     }
-    const R Jy = R(-3) * Ky, Jz = R(-3) * Kz;
+    R J[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) J[a] = a == AX ? Jn : R(-3) * K[a];
 #pragma unroll
     for (int i = 0; i < 27; i++)
-        if (SGN * L::ex(i) > 0)
-            f[i] = f[26 - i] + R(6 * L::w(i)) * (R(L::ex(i)) * Jx + R(L::ey(i)) * Jy + R(L::ez(i)) * Jz);
+        if (SGN * q27_e<L>(i, AX) > 0)
+            f[i] = f[L::mirror(i, 3)] +
+                   R(6 * L::w(i)) * (R(L::ex(i)) * J[0] + R(L::ey(i)) * J[1] + R(L::ez(i)) * J[2]);
 }
 
 // Symmetry(dir, sign) (src/lib/boundary.R:33-50): the populations with
-// sign*e_dir > 0 copy their mirror images in the plane normal to dir
-template <class R, int DIR, int SGN>
-__device__ __forceinline__ void d3q27_symmetry(R* f) {
-    constexpr int st = DIR == 1 ? 3 : 9;
-#pragma unroll
-    for (int i = 0; i < 27; i++) {
-        const int c = i / st % 3 - 1;
-        if (SGN * c > 0) f[i] = f[i - 2 * c * st];
-    }
+// sign*e_dir > 0 copy their mirror images in the plane normal to dir (the
+// indices are template constants, so f stays in registers)
+template <class R, class L, int AX, int SGN, int... I>
+__device__ __forceinline__ void q27_symmetry_seq(R* f, std::integer_sequence<int, I...>) {
+    ((SGN * q27_e<L>(I, AX) > 0 ? (void)(f[I] = f[L::mirror(I, AX)]) : (void)0), ...);
+}
+template <class R, class L, int AX, int SGN>
+__device__ __forceinline__ void q27_symmetry(R* f) {
+    q27_symmetry_seq<R, L, AX, SGN>(f, std::make_integer_sequence<int, 27>{});
+}
+
+// Both symmetry planes normal to AX as selects: `minus` copies the e_AX > 0
+// populations onto their e_AX < 0 mirrors (Symmetry(AX+1, -1)), `plus` the
+// reverse (Symmetry(AX+1, 1)).  Written without branches so that the
+// populations stay in registers behind a second NodeType switch.
+template <class R, class L, int AX, int... I>
+__device__ __forceinline__ void q27_symmetry_sel_seq(R* f, bool minus, bool plus, std::integer_sequence<int, I...>) {
+    R g[27];
+    ((g[I] = f[I]), ...);
+    ((q27_e<L>(I, AX) < 0 ? (void)(f[I] = minus ? g[L::mirror(I, AX)] : g[I])
+                          : (q27_e<L>(I, AX) > 0 ? (void)(f[I] = plus ? g[L::mirror(I, AX)] : g[I]) : (void)0)),
+     ...);
+}
+template <class R, class L, int AX>
+__device__ __forceinline__ void q27_symmetry_sel(R* f, bool minus, bool plus) {
+    q27_symmetry_sel_seq<R, L, AX>(f, minus, plus, std::make_integer_sequence<int, 27>{});
 }
 
 // Run() and CollisionMRT of models/flow/d3q27/Dynamics.c.Rt:147-245.  par =
@@ -947,13 +1004,14 @@ __device__ __forceinline__ void d3q27_symmetry(R* f) {
 // YDragForceInObj, ZDragForceInObj].
 template <class R, bool GLOB>
 __device__ __forceinline__ void d3q27_node(R* f, unsigned nt, const double* par, const double* zr, double* g) {
+    using L = Lat<MODEL_D3Q27>;
     switch (nt & 0xFu) {
-    case 7: d3q27_zouhe<R, 1, false>(f, (R)zr[0], (R)zr[1]); break;   // WPressure
-    case 8: d3q27_zouhe<R, 1, true>(f, (R)zr[0], (R)zr[1]); break;    // WVelocity
-    case 4: d3q27_zouhe<R, -1, false>(f, (R)zr[0], (R)zr[1]); break;  // EPressure
-    case 1: d3q27_symmetry<R, 1, -1>(f); break;                       // NSymmetry
-    case 2: d3q27_symmetry<R, 1, 1>(f); break;                        // SSymmetry
-    case 3: d3q27_symmetry<R, 2, 1>(f); break;                        // ISymmetry
+    case 7: q27_zouhe<R, L, 0, 1, false>(f, (R)zr[0], (R)zr[1]); break;   // WPressure
+    case 8: q27_zouhe<R, L, 0, 1, true>(f, (R)zr[0], (R)zr[1]); break;    // WVelocity
+    case 4: q27_zouhe<R, L, 0, -1, false>(f, (R)zr[0], (R)zr[1]); break;  // EPressure
+    case 1: q27_symmetry<R, L, 1, -1>(f); break;                          // NSymmetry
+    case 2: q27_symmetry<R, L, 1, 1>(f); break;                           // SSymmetry
+    case 3: q27_symmetry<R, L, 2, 1>(f); break;                           // ISymmetry
     case 6: {                                                         // Wall
 #pragma unroll
         for (int i = 0; i < 13; i++) { const R t = f[i]; f[i] = f[26 - i]; f[26 - i] = t; }
@@ -1048,6 +1106,205 @@ __device__ __forceinline__ void d3q27_feq(R* f, R rho, R Jx, R Jy, R Jz) {
 #pragma unroll
     for (int k = 0; k < 27; k++) f[k] = d3q27_req<R>(k, rho, Jx, Jy, Jz, ir);
     d3q27_inv(f);
+}
+
+// ------------------------------------------------------ d3q27_cumulant ---
+// Raw moments in place for the digit order (0, +1, -1) of this lattice:
+// (f0, f+, f-) -> (f0 + f+ + f-, f+ - f-, f+ + f-) along x, y, z; moment
+// i = px + 3py + 9pz has powers (px, py, pz).  Inverse per line:
+// (m0, m1, m2) -> (m0 - m2, (m2 + m1)/2, (m2 - m1)/2).
+template <class R>
+__device__ __forceinline__ void cum_fwd(R* m) {
+#pragma unroll
+    for (int s = 1; s <= 9; s *= 3)
+#pragma unroll
+        for (int b = 0; b < 27; b++) {
+            if ((b / s) % 3 != 0) continue;
+            const R a = m[b], c = m[b + s], d = m[b + 2 * s];
+            m[b] = a + c + d;
+            m[b + s] = c - d;
+            m[b + 2 * s] = c + d;
+        }
+}
+
+template <class R>
+__device__ __forceinline__ void cum_inv(R* m) {
+#pragma unroll
+    for (int s = 1; s <= 9; s *= 3)
+#pragma unroll
+        for (int b = 0; b < 27; b++) {
+            if ((b / s) % 3 != 0) continue;
+            const R m0 = m[b], m1 = m[b + s], m2 = m[b + 2 * s];
+            m[b] = m0 - m2;
+            m[b + s] = (m2 + m1) * R(0.5);
+            m[b + 2 * s] = (m2 - m1) * R(0.5);
+        }
+}
+
+// Moments <-> cumulants of the normalised distribution f/m0: with j the
+// first axis on which n has a nonzero power and n' = n - e_j,
+//   m_n = sum over k <= n' of C(n', k) c_{k+e_j} m_{n'-k},
+// the Leibniz rule on the moment generating function M = m0 exp(K); the
+// term k = n' is c_n m0.  The cumulants of models/flow/d3q27_cumulant/
+// Dynamics.c.Rt:291-318 are this recursion written out.
+__host__ __device__ constexpr int cum_first_axis(int i) { return i % 3 ? 0 : (i / 3 % 3 ? 1 : 2); }
+
+// term k of the sum for moment i (all indices compile-time constants)
+template <int I, int K>
+struct CumTerm {
+    static constexpr int j = cum_first_axis(I);
+    static constexpr int st = j == 0 ? 1 : (j == 1 ? 3 : 9);
+    static constexpr int nx = I % 3 - (j == 0), ny = I / 3 % 3 - (j == 1), nz = I / 9 - (j == 2);
+    static constexpr int kx = K % 3, ky = K / 3 % 3, kz = K / 9;
+    static constexpr bool on = kx <= nx && ky <= ny && kz <= nz && !(kx == nx && ky == ny && kz == nz);
+    static constexpr int bin = (nx == 2 && kx == 1 ? 2 : 1) * (ny == 2 && ky == 1 ? 2 : 1) * (nz == 2 && kz == 1 ? 2 : 1);
+    static constexpr int ck = on ? K + st : 0;
+    static constexpr int mk = on ? (nx - kx) + 3 * (ny - ky) + 9 * (nz - kz) : 0;
+};
+
+template <class R, int I, int... K>
+__device__ __forceinline__ R cum_sum(const R* c, const R* m, std::integer_sequence<int, K...>) {
+    R s = 0;
+    ((CumTerm<I, K>::on ? (void)(s += R(CumTerm<I, K>::bin) * c[CumTerm<I, K>::ck] * m[CumTerm<I, K>::mk])
+                        : (void)0), ...);
+    return s;
+}
+
+template <class R, bool TO_CUM, int I>
+__device__ __forceinline__ void cum_one(R* m, R* c, R m0) {
+    const R s = cum_sum<R, I>(c, m, std::make_integer_sequence<int, 27>{});
+    if (TO_CUM) c[I] = (m[I] - s) / m0;
+    else m[I] = c[I] * m0 + s;
+}
+
+template <class R, bool TO_CUM, int... I>
+__device__ __forceinline__ void cum_convert_seq(R* m, R* c, std::integer_sequence<int, I...>) {
+    const R m0 = m[0];
+    (cum_one<R, TO_CUM, I + 1>(m, c, m0), ...);  // i = 1..26 in order
+}
+
+template <class R, bool TO_CUM>
+__device__ __forceinline__ void cum_convert(R* m, R* c) {
+    cum_convert_seq<R, TO_CUM>(m, c, std::make_integer_sequence<int, 26>{});
+}
+
+// Run() = RunBoundaries() then CollisionMRT (Dynamics.c.Rt:226-290,
+// 326-390), without the AVG, IB and SMAG build options.  par = [nu,
+// nubuffer, GalileanCorrection, ForceX, ForceY, ForceZ, Omega]; zone =
+// [Velocity, Pressure, Turbulence, DensityInObj .. ZMomentumInObj].
+// BOUNDARY values: Buffer 1, WVelocityTurbulent 2, NVelocity 3, SVelocity 4,
+// NPressure 5, SPressure 6, EPressure 7, EVelocity 8, Solid 9, Wall 10,
+// WPressure 11, WVelocity 12.
+template <class R, bool GLOB>
+__device__ __forceinline__ void cum_boundaries(R* f, unsigned nt, const double* zr, double* g) {
+    using L = Lat<MODEL_D3Q27_CUMULANT>;
+    q27_symmetry_sel<R, L, 1>(f, (nt & 0x3u) == 1u, (nt & 0x3u) == 2u);  // NSymmetry, SSymmetry
+    const R V = (R)zr[0], P = (R)zr[1];
+    switch ((nt >> 3) & 0xFu) {
+    case 2:   // WVelocityTurbulent: no synthetic turbulence modes, so V = Velocity
+    case 12: q27_zouhe<R, L, 0, 1, true>(f, V, P); break;    // WVelocity
+    case 7: q27_zouhe<R, L, 0, -1, false>(f, V, P); break;   // EPressure
+    case 11: q27_zouhe<R, L, 0, 1, false>(f, V, P); break;   // WPressure
+    case 6: q27_zouhe<R, L, 1, 1, false>(f, V, P); break;    // SPressure
+    case 5: q27_zouhe<R, L, 1, -1, false>(f, V, P); break;   // NPressure
+    case 3: q27_zouhe<R, L, 1, -1, true>(f, V, P); break;    // NVelocity
+    case 4: q27_zouhe<R, L, 1, 1, true>(f, V, P); break;     // SVelocity
+    case 8: q27_zouhe<R, L, 0, -1, true>(f, V, P); break;    // EVelocity
+    case 10: {  // Wall: FullBounceBack; on Body nodes the momentum change
+        R p1[3] = {0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < 27; i++) {
+            p1[0] += f[i] * R(L::ex(i)); p1[1] += f[i] * R(L::ey(i)); p1[2] += f[i] * R(L::ez(i));
+        }
+#pragma unroll
+        for (int i = 0; i < 27; i++) {
+            const int o = L::mirror(i, 3);
+            if (o > i) { const R t = f[i]; f[i] = f[o]; f[o] = t; }
+        }
+        if (GLOB && (nt & 0x4u)) {
+            R p2[3] = {0, 0, 0};
+#pragma unroll
+            for (int i = 0; i < 27; i++) {
+                p2[0] += f[i] * R(L::ex(i)); p2[1] += f[i] * R(L::ey(i)); p2[2] += f[i] * R(L::ez(i));
+            }
+            g[2] += (double)(-(p2[0] - p1[0])); g[9] += zr[5];   // Drag
+            g[4] += (double)(-(p2[1] - p1[1])); g[9] += zr[7];   // Lateral
+            g[3] += (double)(-(p2[2] - p1[2])); g[9] += zr[6];   // Lift
+        }
+        break; }
+    default: break;
+    }
+}
+
+template <class R, bool GLOB>
+__device__ __forceinline__ void cum_node(R* f, unsigned nt, const double* par, const double* zr, double* g) {
+    using L = Lat<MODEL_D3Q27_CUMULANT>;
+    cum_boundaries<R, GLOB>(f, nt, zr, g);
+    if ((nt & 0x80u) == 0) return;  // MRT
+    const R Fx = (R)par[3], Fy = (R)par[4], Fz = (R)par[5];
+    R m[27], c[27];
+#pragma unroll
+    for (int i = 0; i < 27; i++) m[i] = f[i];
+    cum_fwd(m);
+    const R rho = m[0];
+    if (GLOB) {
+        // getU_: 0 on Wall nodes, else (J + F/2)/rho
+        const bool wall = ((nt >> 3) & 0xFu) == 10u;
+        const R ux = wall ? R(0) : (m[1] + Fx / R(2)) / rho;
+        const R uy = wall ? R(0) : (m[3] + Fy / R(2)) / rho;
+        const R uz = wall ? R(0) : (m[9] + Fz / R(2)) / rho;
+        g[0] += (double)rho; g[9] += zr[3];
+        g[1] += (double)ux; g[9] += zr[4];
+        g[5] += (double)rho; g[9] += zr[8];
+        g[6] += (double)(rho * ux); g[9] += zr[9];
+        g[7] += (double)(rho * uy); g[9] += zr[10];
+        g[8] += (double)(rho * uz); g[9] += zr[11];
+    }
+    const R w0 = ((nt >> 3) & 0xFu) != 0 ? R(1) / (R(3) * (R)par[1] + R(0.5))
+                                          : R(1) / (R(3) * (R)par[0] + R(0.5));
+    const R w1 = R(1);
+    c[0] = R(1);
+    cum_convert<R, true>(m, c);
+    const R ux = c[1] + Fx / (R(2) * rho), uy = c[3] + Fy / (R(2) * rho), uz = c[9] + Fz / (R(2) * rho);
+    // second order: c200 = c[2], c020 = c[6], c002 = c[18], c110 = c[4],
+    // c101 = c[10], c011 = c[12]
+    const R dxu = -w0 / R(2) * (R(2) * c[2] - c[6] - c[18]) - w1 / R(2) * (c[2] + c[6] + c[18] - R(1));
+    const R dyv = dxu + R(3) * w0 / R(2) * (c[2] - c[6]);
+    const R dzw = dxu + R(3) * w0 / R(2) * (c[2] - c[18]);
+    const R gcor1 = R(3) * (R(1) - w0 / R(2)) * (ux * ux * dxu - uy * uy * dyv);
+    const R gcor2 = R(3) * (R(1) - w0 / R(2)) * (ux * ux * dxu - uz * uz * dzw);
+    const R gcor3 = R(3) * (R(1) - w1 / R(2)) * (ux * ux * dxu + uy * uy * dyv + uz * uz * dzw);
+    const R GC = (R)par[2];
+    const R a = (R(1) - w0) * (c[2] - c[6]) - gcor1 * GC;
+    const R b = (R(1) - w0) * (c[2] - c[18]) - gcor2 * GC;
+    const R cc = w1 + (R(1) - w1) * (c[2] + c[6] + c[18]) - gcor3 * GC;
+    c[1] += Fx; c[3] += Fy; c[9] += Fz;
+    c[2] = (a + b + cc) / R(3);
+    c[6] = (cc - R(2) * a + b) / R(3);
+    c[18] = (cc - R(2) * b + a) / R(3);
+    c[4] *= R(1) - w0;
+    c[12] *= R(1) - w0;
+    c[10] *= R(1) - w0;
+    const R om3 = R(1) - (R)par[6];
+#pragma unroll
+    for (int i = 0; i < 27; i++) {
+        const int o = i % 3 + i / 3 % 3 + i / 9;
+        if (o == 3) c[i] *= om3;
+        if (o > 3) c[i] = R(0);
+    }
+    cum_convert<R, false>(m, c);
+    cum_inv(m);
+#pragma unroll
+    for (int i = 0; i < 27; i++) f[i] = m[i];
+}
+
+// SetEquilibrum = Req M^-1 for this lattice's digit order
+template <class R>
+__device__ __forceinline__ void cum_feq(R* f, R rho, R Jx, R Jy, R Jz) {
+    const R ir = R(1) / rho;
+#pragma unroll
+    for (int k = 0; k < 27; k++) f[k] = d3q27_req<R>(k, rho, Jx, Jy, Jz, ir);
+    cum_inv(f);
 }
 
 // ---------------------------------------------------------- reductions ---
@@ -1162,6 +1419,8 @@ __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, 
             d2q9m_node<R, GLOB, MODEL == MODEL_D2Q9_DRAG>(f, nt, a.par, zr, g);
         } else if constexpr (MODEL == MODEL_D3Q27) {
             d3q27_node<R, GLOB>(f, nt, a.par, zr, g);
+        } else if constexpr (MODEL == MODEL_D3Q27_CUMULANT) {
+            cum_node<R, GLOB>(f, nt, a.par, zr, g);
         } else {
             d2q9_node<R, MODEL == MODEL_D2Q9_SRT_DRAG, GLOB>(f, nt, (R)a.par[0], (R)a.par[1], zr, g);
         }
@@ -1297,6 +1556,11 @@ __global__ void __launch_bounds__(256) k_init(StepArgs a) {
         // 3 Pressure, Velocity + Turbulence*ST.x, ...) with no synthetic
         // turbulence modes (ST = 0)
         d3q27_feq<R>(f, R(1) + (R)zr[1] * R(3), (R)zr[0], R(0), R(0));
+    } else if constexpr (MODEL == MODEL_D3Q27_CUMULANT) {
+        // Init(), models/flow/d3q27_cumulant/Dynamics.c.Rt:266-275: Wall
+        // nodes at rest, the rest at (1 + 3 Pressure, Velocity, 0, 0)
+        if (((nt >> 3) & 0xFu) == 10u) cum_feq<R>(f, R(1), R(0), R(0), R(0));
+        else cum_feq<R>(f, R(1) + (R)zr[1] * R(3), (R)zr[0], R(0), R(0));
     } else {
         d2q9_feq<R>(f, (R)a.par[1], (R)zr[1], (R)zr[2]);
     }
@@ -1368,6 +1632,31 @@ __global__ void __launch_bounds__(256) k_quantity(StepArgs a, int qid, int x0, i
             out[3 * tid] = wall ? -2.0 * jx * scale : 0.0;
             out[3 * tid + 1] = wall ? -2.0 * jy * scale : 0.0;
             out[3 * tid + 2] = wall ? -2.0 * jz * scale : 0.0;
+        }
+    } else if constexpr (MODEL == MODEL_D3Q27_CUMULANT) {
+        // getP = 0 on the pulled populations; getU = 1 after RunBoundaries()
+        // (0 on Wall nodes, else (J + F/2)/rho); getSolid = 2 (1 on Wall)
+        // (models/flow/d3q27_cumulant/Dynamics.c.Rt:23-62)
+        const bool wall = ((nt >> 3) & 0xFu) == 10u;
+        if (qid == 0) {
+            double d = 0;
+#pragma unroll
+            for (int i = 0; i < Q; i++) d += f[i];
+            out[tid] = (d - 1.0) / 3.0 * scale;
+        } else if (qid == 1) {
+            int zone = (int)(nt >> L::ZSHIFT);
+            if (zone >= a.nzones) zone = 0;
+            cum_boundaries<double, false>(f, nt, a.zone + (size_t)zone * a.zcols, nullptr);
+            double d = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+            for (int i = 0; i < Q; i++) {
+                d += f[i]; jx += f[i] * L::ex(i); jy += f[i] * L::ey(i); jz += f[i] * L::ez(i);
+            }
+            out[3 * tid] = wall ? 0.0 : (jx + a.par[3] / 2.) / d * scale;
+            out[3 * tid + 1] = wall ? 0.0 : (jy + a.par[4] / 2.) / d * scale;
+            out[3 * tid + 2] = wall ? 0.0 : (jz + a.par[5] / 2.) / d * scale;
+        } else {
+            out[tid] = (wall ? 1.0 : 0.0) * scale;
         }
     } else if constexpr (is_d2q9m<MODEL>) {
         // getRho / getU, Dynamics.c.Rt:46-63: Rho = 0, U = J/rho + G/2 = 1
@@ -1534,6 +1823,7 @@ hipError_t quantity_m(int st, int q, const StepArgs& a, int x0, int y0, int z0, 
 
 double weight(int model, int q) {
     if (model == MODEL_D3Q27) return Lat<MODEL_D3Q27>::w(q);
+    if (model == MODEL_D3Q27_CUMULANT) return Lat<MODEL_D3Q27_CUMULANT>::w(q);
     return (model == MODEL_D3Q19 || model == MODEL_D3Q19_LID) ? Lat<MODEL_D3Q19>::w(q) : Lat<MODEL_D2Q9_SRT>::w(q);
 }
 }  // namespace
@@ -1557,6 +1847,7 @@ hipError_t launch_step(int model, int storage, bool globals, const StepArgs& a, 
     case MODEL_D2Q9: return step_m<MODEL_D2Q9>(storage, globals, a, block, s);
     case MODEL_D2Q9_DRAG: return step_m<MODEL_D2Q9_DRAG>(storage, globals, a, block, s);
     case MODEL_D3Q27: return step_m<MODEL_D3Q27>(storage, globals, a, block, s);
+    case MODEL_D3Q27_CUMULANT: return step_m<MODEL_D3Q27_CUMULANT>(storage, globals, a, block, s);
     }
     return hipErrorInvalidValue;
 }
@@ -1570,6 +1861,7 @@ hipError_t launch_init(int model, int storage, const StepArgs& a, int block, hip
     case MODEL_D2Q9: return init_m<MODEL_D2Q9>(storage, a, block, s);
     case MODEL_D2Q9_DRAG: return init_m<MODEL_D2Q9_DRAG>(storage, a, block, s);
     case MODEL_D3Q27: return init_m<MODEL_D3Q27>(storage, a, block, s);
+    case MODEL_D3Q27_CUMULANT: return init_m<MODEL_D3Q27_CUMULANT>(storage, a, block, s);
     }
     return hipErrorInvalidValue;
 }
@@ -1630,6 +1922,8 @@ hipError_t launch_batch_step(int model, int storage, bool globals, const BatchCa
         return batch_m<MODEL_D2Q9_DRAG>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
     case MODEL_D3Q27:
         return batch_m<MODEL_D3Q27>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
+    case MODEL_D3Q27_CUMULANT:
+        return batch_m<MODEL_D3Q27_CUMULANT>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
     }
     return hipErrorInvalidValue;
 }
@@ -1647,6 +1941,8 @@ hipError_t launch_quantity(int model, int storage, int q, const StepArgs& a, int
     case MODEL_D2Q9_DRAG:
         return quantity_m<MODEL_D2Q9_DRAG>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
     case MODEL_D3Q27: return quantity_m<MODEL_D3Q27>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
+    case MODEL_D3Q27_CUMULANT:
+        return quantity_m<MODEL_D3Q27_CUMULANT>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
     }
     return hipErrorInvalidValue;
 }

@@ -261,9 +261,48 @@ ModelDef make_d3q27() {
     return m;
 }
 
+// d3q27_cumulant (models/flow/d3q27_cumulant/Dynamics.R, no build options):
+// density i = px + 3 py + 9 pz at e = ({0,1,-1}[px], {0,1,-1}[py],
+// {0,1,-1}[pz]) (named f<px><py><pz> there); the cumulant collision with a
+// Galilean correction, Zou-He on the W/E and S/N faces, symmetry planes,
+// momentum-exchange drag on Wall nodes marked Body, buffer-layer viscosity.
+ModelDef make_d3q27_cumulant() {
+    ModelDef m;
+    m.id = MODEL_D3Q27_CUMULANT;
+    m.name = "d3q27_cumulant";
+    m.Q = 27; m.dim = 3;
+    const int e1[3] = {0, 1, -1};
+    for (int i = 0; i < 27; i++) {
+        m.ex.push_back(e1[i % 3]);
+        m.ey.push_back(e1[i / 3 % 3]);
+        m.ez.push_back(e1[i / 9]);
+        const int nz = (i % 3 != 0) + (i / 3 % 3 != 0) + (i / 9 != 0);
+        m.w.push_back(nz == 0 ? 8. / 27 : nz == 1 ? 2. / 27 : nz == 2 ? 1. / 54 : 1. / 216);
+    }
+    // Dynamics.R:18-28 (nu has no derived setting here), Threshold from src/conf.R:769
+    m.settings = {{"nu", 0.16666666, -1, 0},         {"nubuffer", 0.01, -1, 0}, {"GalileanCorrection", 1.0, -1, 0},
+                  {"ForceX", 0.0, -1, 0},            {"ForceY", 0.0, -1, 0},    {"ForceZ", 0.0, -1, 0},
+                  {"Omega", 1.0, -1, 0},             {"Threshold", 0.5, -1, 0}};
+    m.zonal = {{"Velocity", 0, -1, 0}, {"Pressure", 0, -1, 0}, {"Turbulence", 0, -1, 0}};
+    std::vector<GlobalDef> g = {{"Density", false, "kg/m3"}, {"Flux", false, "m3/s"},     {"Drag", false, "N"},
+                                {"Lift", false, "N"},        {"Lateral", false, "N"},     {"Mass", false, "kg"},
+                                {"XMomentum", false, "kgm/s"}, {"YMomentum", false, "kgm/s"},
+                                {"ZMomentum", false, "kgm/s"}};
+    finish_globals(m, g);
+    m.quantities = {{"P", false, "Pa"}, {"U", true, "m/s"}, {"Solid", false, "1"}};
+    pack_nodetypes(m, {{"Buffer", "BOUNDARY"},    {"WVelocityTurbulent", "BOUNDARY"}, {"NVelocity", "BOUNDARY"},
+                       {"SVelocity", "BOUNDARY"}, {"NPressure", "BOUNDARY"},          {"SPressure", "BOUNDARY"},
+                       {"NSymmetry", "ADDITIONALS"}, {"SSymmetry", "ADDITIONALS"},    {"Body", "BODY"},
+                       {"EPressure", "BOUNDARY"}, {"EVelocity", "BOUNDARY"},          {"Solid", "BOUNDARY"},
+                       {"Wall", "BOUNDARY"},      {"WPressure", "BOUNDARY"},          {"WVelocity", "BOUNDARY"},
+                       {"MRT", "COLLISION"}});
+    return m;
+}
+
 const std::vector<ModelDef>& registry() {
     static std::vector<ModelDef> r = {make_d2q9(false), make_d2q9(true), make_d3q19(false), make_d3q19(true),
-                                      make_d2q9_mrt(false), make_d2q9_mrt(true), make_d3q27()};
+                                      make_d2q9_mrt(false), make_d2q9_mrt(true), make_d3q27(),
+                                      make_d3q27_cumulant()};
     return r;
 }
 

@@ -26,7 +26,8 @@ enum ModelId {
     MODEL_D2Q9 = 4,
     MODEL_D2Q9_DRAG = 5,
     MODEL_D3Q27 = 6,
-    MODEL_COUNT = 7
+    MODEL_D3Q27_CUMULANT = 7,
+    MODEL_COUNT = 8
 };
 
 // storage format of the distributions in HBM

@@ -28,7 +28,8 @@ def test_every_declared_symbol_is_exported():
 
 
 def test_models():
-    assert tclb_amd.models() == ["d2q9_SRT", "d2q9_SRT_drag", "d3q19", "d3q19_lid", "d2q9", "d2q9_drag", "d3q27"]
+    assert tclb_amd.models() == ["d2q9_SRT", "d2q9_SRT_drag", "d3q19", "d3q19_lid", "d2q9", "d2q9_drag", "d3q27",
+                                 "d3q27_cumulant"]
 
 
 def test_d2q9_mrt_contract():
@@ -124,7 +125,7 @@ def test_lattice_creation_fails_loudly_without_gpu():
 
 def test_unknown_model_is_an_error():
     with pytest.raises(tclb_amd.TclbxError):
-        tclb_amd.Lattice("d3q27_cumulant", 8, 8, 8)
+        tclb_amd.Lattice("d3q27_cumulant_IB", 8, 8, 8)
 
 
 def test_d3q27_contract():
@@ -145,3 +146,22 @@ def test_d3q27_contract():
     assert m.global_names == ["XFlux", "YFlux", "ZFlux", "XDragForce", "YDragForce", "ZDragForce", "Objective"]
     assert m.quantities == ["P", "U", "Fd"]
     assert m.quantity_id("Fd") == (2, True)
+
+
+def test_d3q27_cumulant_contract():
+    """models/flow/d3q27_cumulant/Dynamics.R (no build options): groups
+    ADDITIONALS, BODY, BOUNDARY (12 types, 4 bits), COLLISION, zone from bit 8."""
+    m = tclb_amd.Model("d3q27_cumulant")
+    assert m.nodetype("NSymmetry") == (1, 3) and m.nodetype("SSymmetry") == (2, 3)
+    assert m.nodetype("Body") == (4, 4)
+    order = ["Buffer", "WVelocityTurbulent", "NVelocity", "SVelocity", "NPressure", "SPressure", "EPressure",
+             "EVelocity", "Solid", "Wall", "WPressure", "WVelocity"]
+    for k, n in enumerate(order):
+        assert m.nodetype(n) == ((k + 1) << 3, 0x78)
+    assert m.nodetype("MRT") == (0x80, 0x80)
+    assert m.group("SETTINGZONE") == (0xFF00, 8)
+    assert m.settings == ["nu", "nubuffer", "GalileanCorrection", "ForceX", "ForceY", "ForceZ", "Omega", "Threshold"]
+    assert m.zonal_settings[:3] == ["Velocity", "Pressure", "Turbulence"]
+    assert m.global_names == ["Density", "Flux", "Drag", "Lift", "Lateral", "Mass", "XMomentum", "YMomentum",
+                              "ZMomentum", "Objective"]
+    assert m.quantities == ["P", "U", "Solid"]

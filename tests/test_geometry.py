@@ -493,7 +493,7 @@ def _examples():
 
 
 @pytest.mark.skipif(not _examples(), reason="the reference's example case files are not present")
-@pytest.mark.parametrize("model", ["d2q9_SRT", "d2q9", "d3q19", "d3q27"])
+@pytest.mark.parametrize("model", ["d2q9_SRT", "d2q9", "d3q19", "d3q27", "d3q27_cumulant"])
 def test_reference_examples(monkeypatch, model):
     """Each example case file of the reference (example/flow/2d, 3d) either
     rasterizes bit-identically in the engine and the oracle, or fails in

@@ -46,13 +46,15 @@ def say(msg):
     print(msg, flush=True)
 
 
-def lattice_weights(Q):
-    """w_i of the d2q9, d3q19 and d3q27 velocity sets, in the models' order"""
+def lattice_weights(Q, cumulant=False):
+    """w_i of the d2q9, d3q19 and d3q27 velocity sets, in the models' order
+    (d3q27_cumulant numbers the 27 directions by digits 0, +1, -1)"""
     if Q == 9:
         return np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
     if Q == 19:
         return np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
-    nz = [(i % 3 != 1) + (i // 3 % 3 != 1) + (i // 9 != 1) for i in range(27)]
+    z = 0 if cumulant else 1
+    nz = [(i % 3 != z) + (i // 3 % 3 != z) + (i // 9 != z) for i in range(27)]
     return np.array([[8 / 27, 2 / 27, 1 / 54, 1 / 216][k] for k in nz])
 
 
@@ -94,6 +96,22 @@ def random_case(model, nx, ny, nz, seed, steps=10):
             zonal += [("Velocity", z, 0.02 * rng.standard_normal()), ("Pressure", z, 0.002 * rng.standard_normal())]
         zonal += [("XFluxInObj", 1, 0.5), ("ZFluxInObj", -1, 0.25), ("XDragForceInObj", 2, -0.75),
                   ("YDragForceInObj", -1, 0.125)]
+    elif model == "d3q27_cumulant":
+        MRT = m.flag("MRT")
+        kinds = [MRT] * 12 + [m.flag("Wall"), m.flag("Wall", "Body"), m.flag("Wall", "MRT"), m.flag("Solid", "MRT")]
+        for b in ("Buffer", "WVelocityTurbulent", "NVelocity", "SVelocity", "NPressure", "SPressure", "EPressure",
+                  "EVelocity", "WPressure", "WVelocity"):
+            kinds += [m.flag(b, "MRT")]
+        kinds += [m.flag("NSymmetry", "MRT"), m.flag("SSymmetry", "MRT"), m.flag("NSymmetry", "WVelocity", "MRT"),
+                  m.flag("WVelocity"), 0]
+        w = lattice_weights(27, cumulant=True)
+        settings = [("nu", 0.04), ("nubuffer", 0.1), ("GalileanCorrection", 1.0), ("ForceX", 1e-5),
+                    ("ForceY", -2e-6), ("ForceZ", 3e-6), ("Omega", 0.7)]
+        zonal = []
+        for z in range(4):
+            zonal += [("Velocity", z, 0.02 * rng.standard_normal()), ("Pressure", z, 0.002 * rng.standard_normal())]
+        zonal += [("DensityInObj", 1, 0.5), ("DragInObj", -1, 0.25), ("LiftInObj", 2, -0.75),
+                  ("ZMomentumInObj", -1, 0.125)]
     elif model.startswith("d2q9"):
         BGK = m.flag("BGK")
         kinds = [BGK] * 12 + [m.flag("Wall"), m.flag("Solid"), m.flag("Wall", "BGK"), m.flag("Solid", "BGK"),
@@ -132,7 +150,7 @@ def random_case(model, nx, ny, nz, seed, steps=10):
 
 
 MODELS = [("d2q9_SRT", 24, 10, 1), ("d2q9_SRT_drag", 37, 9, 1), ("d3q19", 13, 7, 6), ("d3q19_lid", 11, 9, 7),
-          ("d2q9", 29, 11, 1), ("d2q9_drag", 23, 13, 1), ("d3q27", 12, 7, 5)]
+          ("d2q9", 29, 11, 1), ("d2q9_drag", 23, 13, 1), ("d3q27", 12, 7, 5), ("d3q27_cumulant", 11, 8, 5)]
 
 
 @pytest.mark.parametrize("model,nx,ny,nz", MODELS)
@@ -140,7 +158,7 @@ def test_random_case_f64(oracle_mod, model, nx, ny, nz):
     say(f"random case {model} {nx}x{ny}x{nz} f64")
     case = random_case(model, nx, ny, nz,
                        seed={"d2q9_SRT": 11, "d2q9_SRT_drag": 12, "d3q19": 13, "d3q19_lid": 14, "d2q9": 15,
-                             "d2q9_drag": 16, "d3q27": 18}[model])
+                             "d2q9_drag": 16, "d3q27": 18, "d3q27_cumulant": 19}[model])
     lat = case.build("f64")
     orc = oracle_for_case(oracle_mod, case, "f64")
     for n in (1, 6, 13):
@@ -204,7 +222,7 @@ def test_reduced_storage(oracle_mod, model, nx, ny, nz, storage):
     lat = case.build(storage)
     orc = oracle_for_case(oracle_mod, case, storage)
     shifted = storage.endswith("s")
-    w = lattice_weights(lat.Q)
+    w = lattice_weights(lat.Q, cumulant=model == "d3q27_cumulant")
     for step in range(3):
         a, b = lat.get_f(), orc.get_f()
         stored = b - (w[:, None, None, None] if shifted else 0)

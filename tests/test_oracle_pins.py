@@ -921,3 +921,147 @@ def test_d3q27_wall_bounce_back_and_drag(O):
     assert np.array_equal(out, f[::-1])
     assert np.allclose(g[3:6], -2 * (E27.T @ f), rtol=0, atol=1e-15)
     assert not g[:3].any() and abs(g[6] - 0.875) < 1e-15
+
+
+# ------------------------------------------------------ d3q27_cumulant ---
+_E1 = np.array([0, 1, -1])
+EC = np.array([(_E1[i % 3], _E1[i // 3 % 3], _E1[i // 9]) for i in range(27)], float)
+WC = np.array([[8 / 27, 2 / 27, 1 / 54, 1 / 216][int((e != 0).sum())] for e in EC])
+
+
+def _feqc(rho, J):
+    u = np.asarray(J, float) / rho
+    cu = EC @ u
+    return WC * rho * (1 + 3 * cu + 4.5 * cu ** 2 - 1.5 * (u @ u))
+
+
+def _cumulants(f, orders):
+    """cumulants c_(p,q,r) of f/sum(f): derivatives of log(sum_i f_i exp(t.e_i)) at t = 0
+    (sympy, exact rational arithmetic on the given values)"""
+    sp = pytest.importorskip("sympy")
+    t = sp.symbols("tx ty tz")
+    fr = [sp.Rational(float(v)) for v in f]
+    M = sum(fr[i] * sp.exp(sum(int(EC[i][a]) * t[a] for a in range(3))) for i in range(27))
+    K = sp.log(M)
+    out = {}
+    for p in orders:
+        d = K
+        for a in range(3):
+            if p[a]:
+                d = sp.diff(d, t[a], p[a])
+        out[p] = float(d.subs({t[0]: 0, t[1]: 0, t[2]: 0}))
+    return out
+
+
+def test_cumulant_collision_relaxes_the_cumulants(O):
+    """CollisionMRT of d3q27_cumulant, read through cumulants computed from
+    the moment generating function: mass kept, mean velocity + F, the
+    deviatoric second cumulants relaxed by 1 - w0, the trace set to 1 (w1 =
+    1), the off-diagonal ones by 1 - w0, the third order by 1 - Omega, and
+    the fourth order and above set to zero (GalileanCorrection = 0)."""
+    rng = np.random.default_rng(290)
+    f = _feqc(1.02, [0.03, -0.02, 0.01]) * (1 + 0.05 * rng.standard_normal(27))
+    nu, Omega, F = 0.03, 0.4, np.array([1e-4, -2e-4, 3e-4])
+    w0 = 1 / (3 * nu + 0.5)
+    out, _ = O.cum_node(f, 0x80, [nu, 0.01, 0.0, *F, Omega])
+    orders = [(1, 0, 0), (0, 1, 0), (0, 0, 1), (2, 0, 0), (0, 2, 0), (0, 0, 2), (1, 1, 0), (1, 0, 1), (0, 1, 1),
+              (2, 1, 0), (1, 1, 1), (0, 1, 2), (2, 2, 0), (2, 1, 1), (1, 2, 2)]
+    c0, c1 = _cumulants(f, orders), _cumulants(out, orders)
+    assert abs(out.sum() - f.sum()) < 1e-15
+    for k, p in enumerate([(1, 0, 0), (0, 1, 0), (0, 0, 1)]):
+        assert abs(c1[p] - (c0[p] + F[k])) < 1e-14
+    tr0 = c0[(2, 0, 0)] + c0[(0, 2, 0)] + c0[(0, 0, 2)]
+    assert abs(c1[(2, 0, 0)] + c1[(0, 2, 0)] + c1[(0, 0, 2)] - 1) < 1e-14
+    assert abs((c1[(2, 0, 0)] - c1[(0, 2, 0)]) - (1 - w0) * (c0[(2, 0, 0)] - c0[(0, 2, 0)])) < 1e-14
+    assert abs((c1[(2, 0, 0)] - c1[(0, 0, 2)]) - (1 - w0) * (c0[(2, 0, 0)] - c0[(0, 0, 2)])) < 1e-14
+    for p in [(1, 1, 0), (1, 0, 1), (0, 1, 1)]:
+        assert abs(c1[p] - (1 - w0) * c0[p]) < 1e-14
+    for p in [(2, 1, 0), (1, 1, 1), (0, 1, 2)]:
+        assert abs(c1[p] - (1 - Omega) * c0[p]) < 1e-14, p
+    for p in [(2, 2, 0), (2, 1, 1), (1, 2, 2)]:
+        assert abs(c1[p]) < 1e-14, p
+    assert abs(tr0 - 1) > 1e-3          # the test state was off equilibrium
+
+
+def test_cumulant_buffer_nodes_use_nubuffer(O):
+    """Any BOUNDARY type (here Buffer, which has no boundary condition) makes
+    the collision relax at 1/(3 nubuffer + 0.5)."""
+    rng = np.random.default_rng(291)
+    f = _feqc(1.0, [0.02, 0.0, -0.01]) * (1 + 0.05 * rng.standard_normal(27))
+    a, _ = O.cum_node(f, 0x80 | (1 << 3), [0.05, 0.2, 0.0, 0, 0, 0, 1.0])
+    b, _ = O.cum_node(f, 0x80, [0.2, 0.05, 0.0, 0, 0, 0, 1.0])
+    assert np.allclose(a, b, rtol=0, atol=1e-16)
+
+
+def test_cumulant_galilean_correction_vanishes_at_rest(O):
+    """The Galilean correction terms are proportional to u^2: at zero
+    velocity and force they change nothing; with flow they change only the
+    diagonal second-order cumulants (not the mass, the momentum or the
+    off-diagonal ones)."""
+    rng = np.random.default_rng(292)
+    f = _feqc(1.0, [0.0, 0.0, 0.0]) * (1 + 0.05 * rng.standard_normal(27))
+    J = EC.T @ f
+    f0 = f - WC * 3 * (EC @ J)                     # remove the first moment
+    a, _ = O.cum_node(f0, 0x80, [0.05, 0.01, 1.0, 0, 0, 0, 1.0])
+    b, _ = O.cum_node(f0, 0x80, [0.05, 0.01, 0.0, 0, 0, 0, 1.0])
+    assert np.abs(EC.T @ f0).max() < 1e-16
+    assert np.allclose(a, b, rtol=0, atol=1e-16)
+    f1 = _feqc(1.0, [0.08, -0.05, 0.03]) * (1 + 0.05 * rng.standard_normal(27))
+    a, _ = O.cum_node(f1, 0x80, [0.05, 0.01, 1.0, 0, 0, 0, 1.0])
+    b, _ = O.cum_node(f1, 0x80, [0.05, 0.01, 0.0, 0, 0, 0, 1.0])
+    assert abs(a.sum() - b.sum()) < 1e-15 and np.allclose(EC.T @ a, EC.T @ b, rtol=0, atol=1e-15)
+    ca, cb = (_cumulants(v, [(2, 0, 0), (1, 1, 0), (0, 1, 1)]) for v in (a, b))
+    assert abs(ca[(1, 1, 0)] - cb[(1, 1, 0)]) < 1e-14 and abs(ca[(0, 1, 1)] - cb[(0, 1, 1)]) < 1e-14
+    assert abs(ca[(2, 0, 0)] - cb[(2, 0, 0)]) > 1e-9
+
+
+@pytest.mark.parametrize("btype,axis,sgn,vel", [(12, 0, 1, True), (2, 0, 1, True), (8, 0, -1, True),
+                                               (11, 0, 1, False), (7, 0, -1, False), (3, 1, -1, True),
+                                               (4, 1, 1, True), (5, 1, -1, False), (6, 1, 1, False)])
+def test_cumulant_zou_he(O, btype, axis, sgn, vel):
+    """Zou-He on the W/E and S/N faces: entering populations f_opp +
+    6 w_i (e_i . J) for one J; density 3P+1 or velocity V along the normal,
+    no tangential momentum."""
+    rng = np.random.default_rng(300 + btype)
+    V, Pr = 0.03, -0.002
+    f = _feqc(1.0, 0.02 * rng.standard_normal(3)) * (1 + 0.02 * rng.standard_normal(27))
+    out, _ = O.cum_node(f, btype << 3, [0.1, 0.1, 1.0, 0, 0, 0, 1.0], [V, Pr, 0])
+    sel = sgn * EC[:, axis] > 0
+    opp = [int(np.nonzero((EC == -EC[i]).all(1))[0][0]) for i in range(27)]
+    assert np.array_equal(out[~sel], f[~sel])
+    d = out[sel] - f[opp][sel]
+    A = 6 * WC[sel, None] * EC[sel]
+    Jfit, *_ = np.linalg.lstsq(A, d, rcond=None)
+    assert np.allclose(A @ Jfit, d, rtol=0, atol=1e-15)
+    rho, Jn = out.sum(), EC.T @ out
+    if vel:
+        assert abs(Jn[axis] - rho * V) < 1e-15
+    else:
+        assert abs(rho - (3 * Pr + 1)) < 1e-14
+    assert np.allclose(np.delete(Jn, axis), 0, atol=1e-15)
+
+
+def test_cumulant_symmetry_and_body_drag(O):
+    """N/S symmetry (ADDITIONALS) mirror the populations entering through the
+    plane; a Wall node reverses every population and, marked Body, adds
+    -(p_after - p_before) = 2 J to Drag (x), Lateral (y), Lift (z)."""
+    rng = np.random.default_rng(310)
+    f = _feqc(1.0, 0.02 * rng.standard_normal(3)) * (1 + 0.02 * rng.standard_normal(27))
+    for nt, sgn in ((1, -1), (2, 1)):
+        out, _ = O.cum_node(f, nt, [0.1, 0.1, 1.0, 0, 0, 0, 1.0])
+        for i in range(27):
+            e = EC[i].copy()
+            if sgn * e[1] > 0:
+                e[1] = -e[1]
+                assert out[i] == f[int(np.nonzero((EC == e).all(1))[0][0])]
+            else:
+                assert out[i] == f[i]
+    zone = [0, 0, 0, 0, 0, 0.5, 0.25, 0.125, 0, 0, 0, 0]
+    out, g = O.cum_node(f, (10 << 3) | 0x4, [0.1, 0.1, 1.0, 0, 0, 0, 1.0], zone)
+    J = EC.T @ f
+    opp = [int(np.nonzero((EC == -EC[i]).all(1))[0][0]) for i in range(27)]
+    assert np.array_equal(out, f[opp])
+    assert np.allclose(g[[2, 4, 3]], 2 * J, rtol=0, atol=1e-15)
+    assert abs(g[9] - 0.875) < 1e-15
+    _, g = O.cum_node(f, 10 << 3, [0.1, 0.1, 1.0, 0, 0, 0, 1.0], zone)
+    assert not g.any()
