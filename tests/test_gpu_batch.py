@@ -167,9 +167,11 @@ def test_sweep_generator_cases_run():
     """A few cases of the 256-case cylinder sweep generator in one batch."""
     say("cylinder sweep cases")
     sweep = cases.d2q9_cylinder_sweep(ncases=6, steps=50)
-    b = tclb_amd.Batch("d2q9_SRT_drag", "f16s", capacity_bytes=256 << 20)
+    b = tclb_amd.Batch(sweep[0].model, "f16s", capacity_bytes=256 << 20)
     ids = [b.add(c) for c in sweep]
     b.iterate(50)
+    gd = tclb_amd.Model(sweep[0].model).global_id("Drag")
     for cid in ids:
         assert np.all(np.isfinite(b.globals(cid)))
+        assert b.globals(cid)[gd] > 0          # the flow pushes every disc downstream
         assert np.all(np.isfinite(b.quantity(cid, "U")))
