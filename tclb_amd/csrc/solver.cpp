@@ -5,10 +5,13 @@
 
 #include <sys/stat.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
+#include <fstream>
 #include <set>
 #include <sstream>
 #include <stdexcept>
@@ -455,6 +458,72 @@ struct CbSaveBinary : Callback {
     }
 };
 
+// <SaveCheckpoint keep=N|all> (src/Handlers/cbSaveCheckpoint.cpp): the state
+// to <output>_checkpoint_<iter>_0.pri and a restart case
+// <output>_restart_<iter>.xml -- the case file with a <LoadBinary> of that
+// checkpoint before its first <Solve> (or the existing one re-pointed).
+// Only the last `keep` checkpoints stay on disk (default 1; all: every one).
+struct CbSaveCheckpoint : Callback {
+    int keep = 1;
+    std::deque<std::string> pri_, xml_;
+    int init() override {
+        Callback::init();
+        if (const XmlAttr* a = node->attr("keep")) {
+            if (a->value == "all") {
+                keep = 0;
+            } else {
+                keep = xml_as_int(a->value);
+                if (keep < 0) {
+                    solver->note("ERROR: Keeping a negative no. of chckpnts not allowed, returning default behaviour.");
+                    keep = 1;
+                }
+            }
+        }
+        return 0;
+    }
+    int doit() override {
+        const std::string base = solver->out_collective_file("checkpoint", "");
+        const std::string restart = solver->out_collective_file("restart", ".xml");
+        const std::string pri = base + "_0.pri";
+        if (solver->lattice->Save(pri) != 0) return -1;
+        write_restart(base, restart);
+        if (keep != 0) {
+            pri_.push_back(pri);
+            xml_.push_back(restart);
+            if ((int)pri_.size() > keep) {
+                if (std::remove(pri_.front().c_str()) != 0)
+                    solver->note("ERROR: Checkpoint file was not deleted: " + pri_.front());
+                if (std::remove(xml_.front().c_str()) != 0)
+                    solver->note("ERROR: Restart file was not deleted: " + xml_.front());
+                pri_.pop_front();
+                xml_.pop_front();
+            }
+        }
+        return 0;
+    }
+    void write_restart(const std::string& base, const std::string& path) {
+        std::unique_ptr<XmlNode> doc = solver->doc->clone();
+        XmlNode* cfg = doc->child("CLBConfig");
+        if (!cfg) return;
+        if (XmlNode* lb = cfg->child("LoadBinary")) {
+            lb->attrs.erase(std::remove_if(lb->attrs.begin(), lb->attrs.end(),
+                                           [](const XmlAttr& a) { return a.name == "file"; }),
+                            lb->attrs.end());
+            lb->attrs.push_back({"file", base});
+        } else {
+            auto n = std::make_unique<XmlNode>();
+            n->name = "LoadBinary";
+            n->attrs.push_back({"file", base});
+            n->parent = cfg;
+            size_t at = 0;
+            while (at < cfg->children.size() && cfg->children[at]->name != "Solve") at++;
+            cfg->children.insert(cfg->children.begin() + (long)at, std::move(n));
+        }
+        std::ofstream f(path, std::ios::binary);
+        f << xml_write(*doc);
+    }
+};
+
 // <LoadBinary file|filename> (src/Handlers/acLoadBinary.cpp)
 struct AcLoadBinary : Action {
     int init() override {
@@ -639,6 +708,7 @@ HandlerRef Solver::make_handler(XmlNode* n) {
     else if (e == "Failcheck") h = new CbFailcheck;
     else if (e == "VTK") h = new CbVTK;
     else if (e == "SaveBinary") h = new CbSaveBinary;
+    else if (e == "SaveCheckpoint") h = new CbSaveCheckpoint;
     else if (e == "LoadBinary") h = new AcLoadBinary;
     else if (e == "Run") h = new NullHandler;
     else {
@@ -670,6 +740,14 @@ void Solver::set_output(const std::string& prefix) { outpath = prefix + stripbar
 std::string Solver::out_iter_file(const std::string& name, const std::string& suffix) const {
     char num[32];
     std::snprintf(num, sizeof num, "_P%02d_%08d", 0, iter);
+    std::string p = root_ + outpath + "_" + name + num + suffix;
+    mkpath(p);
+    return p;
+}
+
+std::string Solver::out_collective_file(const std::string& name, const std::string& suffix) const {
+    char num[32];
+    std::snprintf(num, sizeof num, "_%08d", iter);
     std::string p = root_ + outpath + "_" + name + num + suffix;
     mkpath(p);
     return p;

@@ -83,6 +83,8 @@ public:
     void set_output(const std::string& prefix);
     std::string out_iter_file(const std::string& name, const std::string& suffix) const;
     std::string out_global_file(const std::string& name, const std::string& suffix) const;
+    // <output><case>_<name>_<iter><suffix> (outIterCollectiveFile, src/Solver.h.Rt:126-129)
+    std::string out_collective_file(const std::string& name, const std::string& suffix) const;
     // the output root followed by the output path (<output><case>)
     std::string out_prefix() const { return root_ + outpath; }
     void init_log(const std::string& path);

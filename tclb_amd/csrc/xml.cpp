@@ -236,6 +236,40 @@ std::unique_ptr<XmlNode> xml_load_file(const std::string& path) {
     return xml_parse(ss.str());
 }
 
+namespace {
+void write_node(const XmlNode& n, int depth, std::string& out) {
+    out.append((size_t)depth, '\t');
+    out += "<" + n.name;
+    for (const XmlAttr& a : n.attrs) {
+        out += " " + a.name + "=\"";
+        for (char c : a.value) {
+            switch (c) {
+            case '&': out += "&amp;"; break;
+            case '<': out += "&lt;"; break;
+            case '>': out += "&gt;"; break;
+            case '"': out += "&quot;"; break;
+            default: out += c;
+            }
+        }
+        out += "\"";
+    }
+    if (n.children.empty()) {
+        out += " />\n";
+        return;
+    }
+    out += ">\n";
+    for (const auto& c : n.children) write_node(*c, depth + 1, out);
+    out.append((size_t)depth, '\t');
+    out += "</" + n.name + ">\n";
+}
+}  // namespace
+
+std::string xml_write(const XmlNode& doc) {
+    std::string out = "<?xml version=\"1.0\"?>\n";
+    for (const auto& c : doc.children) write_node(*c, 0, out);
+    return out;
+}
+
 int xml_as_int(const std::string& str) {
     const char* s = str.c_str();
     while (*s == ' ' || *s == '\t' || *s == '\r' || *s == '\n') s++;

@@ -47,6 +47,10 @@ struct XmlNode {
 // with the line number on malformed input.
 std::unique_ptr<XmlNode> xml_parse(const std::string& text);
 std::unique_ptr<XmlNode> xml_load_file(const std::string& path);
+// The document as pugixml's save_file writes it by default: the declaration
+// <?xml version="1.0"?>, one element per line indented by tabs, empty
+// elements as <name a="v" />, and &, <, >, " escaped in attribute values.
+std::string xml_write(const XmlNode& doc);
 
 // The value of a present attribute as pugixml's xml_attribute reads it:
 // as_int skips leading space, takes an optional sign and decimal or 0x-hex

@@ -236,3 +236,37 @@ def test_d2q9_mrt_channel_case(oracle_mod, tmp_path):
     u, uo = c.lattice.quantity("U"), orc.quantity("U")
     assert np.max(np.abs(u - uo)) <= 1e-12 * np.max(np.abs(uo))
     assert rows[-1][head.index("InletFlux")] > 0
+
+
+def test_save_checkpoint_keeps_the_last_and_restarts(tmp_path):
+    """SaveCheckpoint writes <output>_checkpoint_<iter>_0.pri and
+    <output>_restart_<iter>.xml, keeping the last `keep` pairs; the restart
+    case is the case file with a LoadBinary of the checkpoint before the
+    first Solve, and running it continues the flow exactly (the kernels are
+    deterministic): checkpoint at 40 + 60 steps = 100 steps."""
+    say("case: SaveCheckpoint and restart")
+    geom = '<Geometry nx="32" ny="16"><BGK><Box/></BGK><Wall mask="ALL"><Channel/></Wall></Geometry>'
+    model = '<Model><Param name="nu" value="0.1"/><Param name="GravitationX" value="1e-5"/></Model>'
+    a = run_case("d2q9_SRT", f"""<CLBConfig output="out/">{geom}{model}
+        <Solve Iterations="60"><SaveCheckpoint Iterations="20" keep="2"/></Solve></CLBConfig>""", tmp_path, "ck.xml")
+    out = tmp_path / "out"
+    names = sorted(p.name for p in out.iterdir())
+    assert names == ["ck_checkpoint_00000040_0.pri", "ck_checkpoint_00000060_0.pri",
+                     "ck_restart_00000040.xml", "ck_restart_00000060.xml"], names
+    pri = np.fromfile(out / "ck_checkpoint_00000060_0.pri", dtype=np.float64).reshape(9, 16, 32)
+    assert np.array_equal(pri, a.lattice.get_f().reshape(9, 16, 32))
+    text = (out / "ck_restart_00000040.xml").read_text()
+    assert text.startswith('<?xml version="1.0"?>')
+    root = ET.fromstring(text)
+    kids = [c.tag for c in root]
+    assert kids.index("LoadBinary") == kids.index("Solve") - 1
+    assert root.find("LoadBinary").get("file") == str(out / "ck_checkpoint_00000040")
+    r = tclb_amd.Case("d2q9_SRT", str(out / "ck_restart_00000040.xml"), output_root=str(tmp_path / "r") + "/")
+    r.run()
+    b = run_case("d2q9_SRT", f"""<CLBConfig output="out2/">{geom}{model}
+        <Solve Iterations="100"/></CLBConfig>""", tmp_path, "ref.xml")
+    assert np.array_equal(r.lattice.get_f(), b.lattice.get_f())
+    # keep="all" keeps every checkpoint
+    run_case("d2q9_SRT", f"""<CLBConfig output="all/">{geom}{model}
+        <Solve Iterations="60"><SaveCheckpoint Iterations="20" keep="all"/></Solve></CLBConfig>""", tmp_path, "k.xml")
+    assert len(list((tmp_path / "all").glob("k_checkpoint_*_0.pri"))) == 3
