@@ -270,3 +270,40 @@ def test_save_checkpoint_keeps_the_last_and_restarts(tmp_path):
     run_case("d2q9_SRT", f"""<CLBConfig output="all/">{geom}{model}
         <Solve Iterations="60"><SaveCheckpoint Iterations="20" keep="all"/></Solve></CLBConfig>""", tmp_path, "k.xml")
     assert len(list((tmp_path / "all").glob("k_checkpoint_*_0.pri"))) == 3
+
+
+@pytest.mark.parametrize("model,fname,settings,zonal", [
+    ("d3q27", "channel_d3q27.xml", [("nu", 0.04), ("Smag", 0.16)],
+     [("Velocity", "Inlet", 0.03), ("XDragForceInObj", -1, 1.0)]),
+    ("d3q27_cumulant", "channel_cumulant.xml", [("nu", 0.03), ("nubuffer", 0.1), ("Omega", 0.8)],
+     [("Velocity", "Inlet", 0.03), ("DragInObj", -1, 1.0)]),
+])
+def test_d3q27_channel_cases(oracle_mod, tmp_path, model, fname, settings, zonal):
+    """The 3D channel case files of the 27-velocity models: the NodeType map
+    equals the oracle's rasterization, each Log row's Globals are within
+    1e-10 of the oracle run on that map, and the final u within 1e-12."""
+    say(f"case: {model} channel")
+    path = os.path.join(os.path.dirname(__file__), "golden", "cases", fname)
+    c = tclb_amd.Case(model, path, output_root=str(tmp_path) + "/")
+    c.run()
+    assert c.iteration == 300
+    flags, zones, _ = G.read_case(path, model)
+    assert np.array_equal(c.lattice.get_flags(), flags)
+    nz, ny, nx = flags.shape
+    zon = [(n, zones[z] if isinstance(z, str) else z, v) for n, z, v in zonal]
+    oc = cases.Case("chan", model, nx, ny, nz, flags, settings=settings, zonal=zon)
+    orc = oracle_for_case(oracle_mod, oc, "f64")
+    head, rows = read_log(tmp_path / "output" / (fname[:-4] + "_Log_P00_00000000.csv"))
+    assert [r[0] for r in rows] == [100, 200, 300]
+    names = tclb_amd.Model(model).global_names
+    for r in rows:
+        g = orc.iterate(100)
+        for name, want in zip(names, g):
+            got = r[head.index(name)]
+            assert abs(got - want) <= 1e-10 * abs(want) or abs(want) < 1e-10, (name, got, want)
+    u, uo = c.lattice.quantity("U"), orc.quantity("U")
+    assert np.max(np.abs(u - uo)) <= 1e-12 * np.max(np.abs(uo))
+    # d3q27's XDragForce is 2 sum f e after the swap, minus the momentum the
+    # flow delivers to the walls; the cumulant model's Drag is that momentum
+    drag = rows[-1][head.index("XDragForce" if model == "d3q27" else "Drag")]
+    assert (-drag if model == "d3q27" else drag) > 0
