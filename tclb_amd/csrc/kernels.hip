@@ -1171,16 +1171,18 @@ __device__ __forceinline__ R cum_sum(const R* c, const R* m, std::integer_sequen
 }
 
 template <class R, bool TO_CUM, int I>
-__device__ __forceinline__ void cum_one(R* m, R* c, R m0) {
+__device__ __forceinline__ void cum_one(R* m, R* c, R m0, R im0) {
     const R s = cum_sum<R, I>(c, m, std::make_integer_sequence<int, 27>{});
-    if (TO_CUM) c[I] = (m[I] - s) / m0;
+    if (TO_CUM) c[I] = (m[I] - s) * im0;
     else m[I] = c[I] * m0 + s;
 }
 
+// one reciprocal of m0 instead of 26 divisions (the float path is
+// arithmetic-bound here)
 template <class R, bool TO_CUM, int... I>
 __device__ __forceinline__ void cum_convert_seq(R* m, R* c, std::integer_sequence<int, I...>) {
-    const R m0 = m[0];
-    (cum_one<R, TO_CUM, I + 1>(m, c, m0), ...);  // i = 1..26 in order
+    const R m0 = m[0], im0 = R(1) / m0;
+    (cum_one<R, TO_CUM, I + 1>(m, c, m0, im0), ...);  // i = 1..26 in order
 }
 
 template <class R, bool TO_CUM>
@@ -1265,7 +1267,8 @@ __device__ __forceinline__ void cum_node(R* f, unsigned nt, const double* par, c
     const R w1 = R(1);
     c[0] = R(1);
     cum_convert<R, true>(m, c);
-    const R ux = c[1] + Fx / (R(2) * rho), uy = c[3] + Fy / (R(2) * rho), uz = c[9] + Fz / (R(2) * rho);
+    const R h = R(0.5) / rho;
+    const R ux = c[1] + Fx * h, uy = c[3] + Fy * h, uz = c[9] + Fz * h;
     // second order: c200 = c[2], c020 = c[6], c002 = c[18], c110 = c[4],
     // c101 = c[10], c011 = c[12]
     const R dxu = -w0 / R(2) * (R(2) * c[2] - c[6] - c[18]) - w1 / R(2) * (c[2] + c[6] + c[18] - R(1));
@@ -1279,9 +1282,10 @@ __device__ __forceinline__ void cum_node(R* f, unsigned nt, const double* par, c
     const R b = (R(1) - w0) * (c[2] - c[18]) - gcor2 * GC;
     const R cc = w1 + (R(1) - w1) * (c[2] + c[6] + c[18]) - gcor3 * GC;
     c[1] += Fx; c[3] += Fy; c[9] += Fz;
-    c[2] = (a + b + cc) / R(3);
-    c[6] = (cc - R(2) * a + b) / R(3);
-    c[18] = (cc - R(2) * b + a) / R(3);
+    const R third = R(1) / R(3);
+    c[2] = (a + b + cc) * third;
+    c[6] = (cc - R(2) * a + b) * third;
+    c[18] = (cc - R(2) * b + a) * third;
     c[4] *= R(1) - w0;
     c[12] *= R(1) - w0;
     c[10] *= R(1) - w0;
