@@ -434,12 +434,14 @@ __device__ __forceinline__ void d3q19_req(R* q, R rho, R Jx, R Jy, R Jz) {
 // The same moments for the float path: one reciprocal of rho and constant
 // products instead of the nine IEEE divisions (the float formats are held to
 // a tolerance of a few float ulps, not to identical roundings).
-__device__ __forceinline__ void d3q19_req_fast(float* q, float rho, float Jx, float Jy, float Jz) {
+// `lin` is the density of the rho-linear terms (rho itself, or rho - 1 for
+// deviations from the rest state), `rho` the one that divides J^2.
+__device__ __forceinline__ void d3q19_req_fast(float* q, float rho, float Jx, float Jy, float Jz, float lin) {
     const float ir = __builtin_amdgcn_rcpf(rho);
     const float xx = Jx*Jx*ir, yy = Jy*Jy*ir, zz = Jz*Jz*ir;
     const float j2 = xx + yy + zz;
-    q[1] = -11.f*rho + 19.f*j2;
-    q[2] = 3.f*rho - 2.f*j2;
+    q[1] = -11.f*lin + 19.f*j2;
+    q[2] = 3.f*lin - 2.f*j2;
     q[4] = (-2.f/3.f)*Jx;
     q[6] = (-2.f/3.f)*Jy;
     q[8] = (-2.f/3.f)*Jz;
@@ -548,7 +550,12 @@ __device__ __forceinline__ void d3q19_globals(double* g, unsigned nt, R rs, R jx
 // the reference uses gamma1 = 1-omega for e/eps/nu/pi and
 // gamma2 = 1-8(2-omega)/(8-omega) for q/m; a positive MRT_S* setting
 // replaces the rate of its group by 1 - S.
-template <class R, bool GLOB, bool LID>
+// DEV: f holds the deviations f - w of a shifted storage format (the rest
+// populations w have moments rho = 1, m1 = -11, m2 = 3 and no others), and
+// the collision is evaluated on them: the linear terms of Req take rho - 1,
+// J^2/rho and rho F take the full rho, and the result is again f - w.  The
+// large constant parts cancel exactly instead of being rounded in float.
+template <class R, bool GLOB, bool LID, bool DEV = false>
 __device__ __forceinline__ void d3q19_mrt(R* f, unsigned nt, const double* par, const double* zr,
                                           double* g) {
     // relaxation factors of the moment groups, computed on the host
@@ -587,9 +594,10 @@ __device__ __forceinline__ void d3q19_mrt(R* f, unsigned nt, const double* par, 
     m[17] = -f[7] - f[8] + f[9] + f[10] + f[15] - f[16] + f[17] - f[18];
     m[18] = f[11] + f[12] - f[13] - f[14] - f[15] - f[16] + f[17] + f[18];
 
-    const R rho = m[0];
+    // DEV: Req - Req(1, 0), i.e. rho - 1 in the rho-linear moments
+    const R rho = DEV ? R(1) + m[0] : m[0];
     R q[19];
-    d3q19_req_fast(q, rho, m[3], m[5], m[7]);
+    d3q19_req_fast(q, rho, m[3], m[5], m[7], m[0]);
     m[1] = ge * (m[1] - q[1]);
     m[2] = geps * (m[2] - q[2]);
     m[4] = gq * (m[4] - q[4]);
@@ -609,7 +617,7 @@ __device__ __forceinline__ void d3q19_mrt(R* f, unsigned nt, const double* par, 
     m[3] += rho * (R)par[3];
     m[5] += rho * (R)par[4];
     m[7] += rho * (R)par[5];
-    d3q19_req_fast(q, rho, m[3], m[5], m[7]);
+    d3q19_req_fast(q, rho, m[3], m[5], m[7], m[0]);
     m[1] += q[1]; m[2] += q[2]; m[4] += q[4]; m[6] += q[6]; m[8] += q[8];
 #pragma unroll
     for (int k = 9; k < 16; k++) m[k] += q[k];
@@ -621,7 +629,7 @@ __device__ __forceinline__ void d3q19_mrt(R* f, unsigned nt, const double* par, 
 
     // Dynamics.c.Rt:266-289 evaluates the Globals after "R /= Mw", i.e. on
     // rho/19 and J/10 -- kept as the reference computes them
-    if (GLOB) d3q19_globals<GLOB, LID>(g, nt, m[0], m[3], m[5], m[7], zr);
+    if (GLOB) d3q19_globals<GLOB, LID>(g, nt, DEV ? rho * R(iv[0]) : m[0], m[3], m[5], m[7], zr);
 }
 
 // The d3q19 moment matrix M[k][i] (moment k of direction i), the listed
@@ -796,8 +804,9 @@ __device__ __forceinline__ void d3q19_moving_wall(R* f, const double* par, const
     }
 }
 
-// x, y, z and the flags locate the node for the MovingWall of d3q19_lid
-template <class R, bool GLOB, bool LID>
+// x, y, z and the flags locate the node for the MovingWall of d3q19_lid;
+// DEV: the boundary condition only (the collision follows on deviations)
+template <class R, bool GLOB, bool LID, bool DEV = false>
 __device__ __forceinline__ void d3q19_node(R* f, unsigned nt, const double* par, const double* zr,
                                            double* g, const uint16_t* flags, int x, int y, int z, int nx, int ny,
                                            int nz) {
@@ -850,10 +859,29 @@ __device__ __forceinline__ void d3q19_node(R* f, unsigned nt, const double* par,
         break;
     default: break;
     }
-    if ((nt & 0x20u) == 0x20u) {
-        if constexpr (sizeof(R) == 8) d3q19_mrt_dense<R, GLOB, LID>(f, nt, par, zr, g);
-        else d3q19_mrt<R, GLOB, LID>(f, nt, par, zr, g);
+    if constexpr (!DEV) {
+        if ((nt & 0x20u) == 0x20u) {
+            if constexpr (sizeof(R) == 8) d3q19_mrt_dense<R, GLOB, LID>(f, nt, par, zr, g);
+            else d3q19_mrt<R, GLOB, LID>(f, nt, par, zr, g);
+        }
     }
+}
+
+// Node of a shifted storage format, on the deviations d = f - w: boundary
+// nodes add w back for their boundary condition, the collision runs on d.
+template <class R, bool GLOB, bool LID>
+__device__ __forceinline__ void d3q19_node_dev(R* d, unsigned nt, const double* par, const double* zr,
+                                               double* g, const uint16_t* flags, int x, int y, int z, int nx,
+                                               int ny, int nz) {
+    using L = Lat<MODEL_D3Q19>;
+    if ((nt & 0x1Cu) != 0u) {
+#pragma unroll
+        for (int i = 0; i < 19; i++) d[i] += R(L::w(i));
+        d3q19_node<R, GLOB, LID, true>(d, nt, par, zr, g, flags, x, y, z, nx, ny, nz);
+#pragma unroll
+        for (int i = 0; i < 19; i++) d[i] -= R(L::w(i));
+    }
+    if ((nt & 0x20u) == 0x20u) d3q19_mrt<R, GLOB, LID, true>(d, nt, par, zr, g);
 }
 
 // --------------------------------------------------------------- d3q27 ---
@@ -1164,9 +1192,12 @@ struct CumTerm {
 
 template <class R, int I, int... K>
 __device__ __forceinline__ R cum_sum(const R* c, const R* m, std::integer_sequence<int, K...>) {
+    // fused multiply-adds: the float path of this model is arithmetic-bound
     R s = 0;
-    ((CumTerm<I, K>::on ? (void)(s += R(CumTerm<I, K>::bin) * c[CumTerm<I, K>::ck] * m[CumTerm<I, K>::mk])
-                        : (void)0), ...);
+    ((CumTerm<I, K>::on
+          ? (void)(s = fma(CumTerm<I, K>::bin == 1 ? c[CumTerm<I, K>::ck] : R(CumTerm<I, K>::bin) * c[CumTerm<I, K>::ck],
+                           m[CumTerm<I, K>::mk], s))
+          : (void)0), ...);
     return s;
 }
 
@@ -1174,7 +1205,7 @@ template <class R, bool TO_CUM, int I>
 __device__ __forceinline__ void cum_one(R* m, R* c, R m0, R im0) {
     const R s = cum_sum<R, I>(c, m, std::make_integer_sequence<int, 27>{});
     if (TO_CUM) c[I] = (m[I] - s) * im0;
-    else m[I] = c[I] * m0 + s;
+    else m[I] = fma(c[I], m0, s);
 }
 
 // one reciprocal of m0 instead of 26 divisions (the float path is
@@ -1404,6 +1435,8 @@ __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, 
     // of every address is a 32-bit byte offset (a plane is < 4 GiB)
     const char* fin = (const char*)a.fin;
     const size_t pbytes = (size_t)a.pitch * sizeof(T);
+    // d3q19 in a shifted format collides on the stored deviations f - w
+    constexpr bool DEV = is_d3q19<MODEL> && S::shifted;
     R f[Q];
 #pragma unroll
     for (int i = 0; i < Q; i++) {
@@ -1413,11 +1446,13 @@ __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, 
         const int sz = L::ez(i) > 0 ? zm : (L::ez(i) < 0 ? zp : z);
         const unsigned src = (unsigned)sx + (unsigned)nx * ((unsigned)sy + (unsigned)ny * (unsigned)sz);
         const T v = *(const T*)(fin + (size_t)i * pbytes + src * (unsigned)sizeof(T));
-        f[i] = S::get(v, (R)L::w(i));
+        f[i] = DEV ? S::raw(v) : S::get(v, (R)L::w(i));
     }
 
     if (active) {
-        if constexpr (is_d3q19<MODEL>) {
+        if constexpr (DEV) {
+            d3q19_node_dev<R, GLOB, MODEL == MODEL_D3Q19_LID>(f, nt, a.par, zr, g, a.flags, xc, y, z, nx, ny, nz);
+        } else if constexpr (is_d3q19<MODEL>) {
             d3q19_node<R, GLOB, MODEL == MODEL_D3Q19_LID>(f, nt, a.par, zr, g, a.flags, xc, y, z, nx, ny, nz);
         } else if constexpr (is_d2q9m<MODEL>) {
             d2q9m_node<R, GLOB, MODEL == MODEL_D2Q9_DRAG>(f, nt, a.par, zr, g);
@@ -1432,7 +1467,7 @@ __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, 
         char* fout = (char*)a.fout;
 #pragma unroll
         for (int i = 0; i < Q; i++)
-            __builtin_nontemporal_store(S::put(f[i], (R)L::w(i)),
+            __builtin_nontemporal_store(DEV ? S::unraw(f[i]) : S::put(f[i], (R)L::w(i)),
                                         (T*)(fout + (size_t)i * pbytes + n * (unsigned)sizeof(T)));
     }
 }

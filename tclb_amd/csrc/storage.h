@@ -21,6 +21,8 @@ template <> struct Store<ST_F64> {
     static constexpr bool shifted = false;
     __device__ __forceinline__ static R get(T v, R) { return v; }
     __device__ __forceinline__ static T put(R v, R) { return v; }
+    __device__ __forceinline__ static R raw(T v) { return v; }
+    __device__ __forceinline__ static T unraw(R v) { return v; }
 };
 
 template <> struct Store<ST_F32> {
@@ -29,6 +31,8 @@ template <> struct Store<ST_F32> {
     static constexpr bool shifted = false;
     __device__ __forceinline__ static R get(T v, R) { return v; }
     __device__ __forceinline__ static T put(R v, R) { return v; }
+    __device__ __forceinline__ static R raw(T v) { return v; }
+    __device__ __forceinline__ static T unraw(R v) { return v; }
 };
 
 template <> struct Store<ST_F32S> {
@@ -37,6 +41,9 @@ template <> struct Store<ST_F32S> {
     static constexpr bool shifted = true;
     __device__ __forceinline__ static R get(T v, R s) { return v + s; }
     __device__ __forceinline__ static T put(R v, R s) { return v - s; }
+    // the stored value itself (f - w)
+    __device__ __forceinline__ static R raw(T v) { return v; }
+    __device__ __forceinline__ static T unraw(R v) { return v; }
 };
 
 template <> struct Store<ST_F16S> {
@@ -46,6 +53,8 @@ template <> struct Store<ST_F16S> {
     __device__ __forceinline__ static R get(T v, R s) { return (float)v + s; }
     // float -> half conversion rounds to nearest even (v_cvt_f16_f32)
     __device__ __forceinline__ static T put(R v, R s) { return (_Float16)(v - s); }
+    __device__ __forceinline__ static R raw(T v) { return (float)v; }
+    __device__ __forceinline__ static T unraw(R v) { return (_Float16)v; }
 };
 
 template <> struct Store<ST_BF16S> {
@@ -55,6 +64,8 @@ template <> struct Store<ST_BF16S> {
     __device__ __forceinline__ static R get(T v, R s) { return (float)v + s; }
     // float -> bfloat16 rounds to nearest even (v_cvt_pk_bf16_f32)
     __device__ __forceinline__ static T put(R v, R s) { return (__bf16)(v - s); }
+    __device__ __forceinline__ static R raw(T v) { return (float)v; }
+    __device__ __forceinline__ static T unraw(R v) { return (__bf16)v; }
 };
 
 }  // namespace tclbx
