@@ -308,3 +308,11 @@ def test_d3q27_channel_cases(oracle_mod, tmp_path, model, fname, settings, zonal
     # model the momentum change)
     drag = rows[-1][head.index("XDragForce" if model == "d3q27" else "Drag")]
     assert abs(drag) > 1e-6
+    if model == "d3q27":   # a 3D VTI: every quantity of the model, x fastest
+        vti = tmp_path / "output" / "channel_d3q27_VTK_P00_00000300.vti"
+        root, arr = _vti_arrays(vti)
+        assert root.find("ImageData").get("WholeExtent") == f"0 {nx} 0 {ny} 0 {nz}"
+        for q, ncomp in (("P", 1), ("U", 3), ("Fd", 3)):
+            v, k = arr[q]
+            assert k == ncomp and v.size == nx * ny * nz * ncomp
+        assert np.allclose(arr["U"][0].reshape(nz, ny, nx, 3), u, rtol=0, atol=0)
