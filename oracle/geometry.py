@@ -244,6 +244,11 @@ NODETYPES["d3q27"] = [("Smagorinsky", "LES"), ("Stab", "ENTROPIC"), ("NSymmetry"
                       ("SSymmetry", "BOUNDARY"), ("ISymmetry", "BOUNDARY"), ("EPressure", "BOUNDARY"),
                       ("Solid", "BOUNDARY"), ("Wall", "BOUNDARY"), ("WPressure", "BOUNDARY"),
                       ("WVelocity", "BOUNDARY"), ("MRT", "COLLISION")]
+# the autosym builds of d2q9 (src/conf.R:433-441)
+NODETYPES["d2q9_autosym"] = NODETYPES["d2q9"] + [("SymmetryX_plus", "SYMX"), ("SymmetryX_minus", "SYMX"),
+                                                 ("SymmetryY_plus", "SYMY"), ("SymmetryY_minus", "SYMY")]
+NODETYPES["d2q9_autosym2"] = NODETYPES["d2q9"] + [("SymmetryEdgeX_plus", "SYMX"), ("SymmetryEdgeX_minus", "SYMX"),
+                                                  ("SymmetryEdgeY_plus", "SYMY"), ("SymmetryEdgeY_minus", "SYMY")]
 # models/flow/d3q27_cumulant/Dynamics.R:41-49,109-115 (no build options)
 NODETYPES["d3q27_cumulant"] = [
     ("Buffer", "BOUNDARY"), ("WVelocityTurbulent", "BOUNDARY"), ("NVelocity", "BOUNDARY"), ("SVelocity", "BOUNDARY"),

@@ -173,7 +173,8 @@ orc_lattice* orc_create(int model, int nx, int ny, int nz, int storage)
         }
     } else {
         L->Q = 9;
-        L->nglob = model == ORC_D2Q9_SRT_DRAG ? 3 : (model == ORC_D2Q9 ? 4 : (model == ORC_D2Q9_DRAG ? 6 : 1));
+        L->nglob = model == ORC_D2Q9_SRT_DRAG ? 3 : (model == ORC_D2Q9_DRAG ? 6 : 1);
+        if (model == ORC_D2Q9 || model == ORC_D2Q9_AUTOSYM || model == ORC_D2Q9_AUTOSYM2) L->nglob = 4;
         for (q = 0; q < 9; q++) {
             for (a = 0; a < 3; a++) L->e[q][a] = d2q9_e[q][a];
             L->w[q] = d2q9_w[q];
@@ -220,12 +221,17 @@ void orc_set_flags(orc_lattice* L, const uint16_t* flags)
 }
 
 static int is3d(const orc_lattice* L) { return L->model == ORC_D3Q19 || L->model == ORC_D3Q19_LID; }
-static int is_d2q9m(const orc_lattice* L) { return L->model == ORC_D2Q9 || L->model == ORC_D2Q9_DRAG; }
+static int is_d2q9m(const orc_lattice* L)
+{
+    return L->model == ORC_D2Q9 || L->model == ORC_D2Q9_DRAG || L->model == ORC_D2Q9_AUTOSYM ||
+           L->model == ORC_D2Q9_AUTOSYM2;
+}
+static int is_autosym(const orc_lattice* L) { return L->model == ORC_D2Q9_AUTOSYM || L->model == ORC_D2Q9_AUTOSYM2; }
 static int is_q27(const orc_lattice* L) { return L->model == ORC_D3Q27; }
 static int is_cum(const orc_lattice* L) { return L->model == ORC_D3Q27_CUMULANT; }
 static int zone_shift(const orc_lattice* L)
 {
-    return is3d(L) ? 6 : (is_cum(L) ? 8 : (is_d2q9m(L) || is_q27(L) ? 7 : 4));
+    return is3d(L) ? 6 : (is_cum(L) ? 8 : (is_autosym(L) ? 11 : (is_d2q9m(L) || is_q27(L) ? 7 : 4)));
 }
 
 static const double* zone_row(const orc_lattice* L, uint16_t nt)
@@ -324,6 +330,54 @@ static void pull(const orc_lattice* L, const void* b, int x, int y, int z, doubl
     }
 }
 
+/* SymmetryAccess of the autosym builds (src/conf.R:373-445,
+ * src/LatticeAccess.inc.cpp.Rt:233-260,376-470) on the pulled f of node
+ * (x, y): SYMX = bits 7-8, SYMY = bits 9-10, 1 = _plus, 2 = _minus. */
+static int d2q9_mirror(int q, int axis)
+{
+    int j;
+    for (j = 0; j < 9; j++)
+        if (d2q9_e[j][axis] == -d2q9_e[q][axis] && d2q9_e[j][1 - axis] == d2q9_e[q][1 - axis]) return j;
+    return q;
+}
+
+static void autosym_access(const orc_lattice* L, const void* b, int x, int y, uint16_t nt, double* f)
+{
+    int sym[2], axis, q;
+    sym[0] = (nt >> 7) & 3;
+    sym[1] = (nt >> 9) & 3;
+    if (L->model == ORC_D2Q9_AUTOSYM) {
+        /* resolve.symmetries after the pull: on a _plus node the densities
+         * with e_axis < 0 take their mirrors, on a _minus node those with
+         * e_axis > 0; X, then Y */
+        for (axis = 0; axis < 2; axis++) {
+            double g[9];
+            for (q = 0; q < 9; q++) g[q] = f[q];
+            for (q = 0; q < 9; q++) {
+                if (sym[axis] == 1 && d2q9_e[q][axis] < 0) f[q] = g[d2q9_mirror(q, axis)];
+                if (sym[axis] == 2 && d2q9_e[q][axis] > 0) f[q] = g[d2q9_mirror(q, axis)];
+            }
+        }
+        return;
+    }
+    if (!sym[0] && !sym[1]) return;
+    /* SymmetryEdge: load_ -> loadX_ -> loadY_ -> plain load, each step
+     * replacing (field, offset) when the offset crosses its plane */
+    for (q = 0; q < 9; q++) {
+        int fld = q, d[2], i;
+        size_t n;
+        d[0] = -d2q9_e[q][0];
+        d[1] = -d2q9_e[q][1];
+        for (axis = 0; axis < 2; axis++) {
+            if (d[axis] > 0 && sym[axis] == 1) { fld = d2q9_mirror(fld, axis); d[axis] = 1 - d[axis]; }
+            else if (d[axis] < 0 && sym[axis] == 2) { fld = d2q9_mirror(fld, axis); d[axis] = -1 - d[axis]; }
+        }
+        n = (size_t)wrap(x + d[0], L->nx) + (size_t)L->nx * (size_t)wrap(y + d[1], L->ny);
+        i = fld;
+        f[q] = L->storage == ORC_F64 ? ((const double*)b)[(size_t)i * L->N + n] : ld(L, b, (size_t)i * L->N + n, i);
+    }
+}
+
 /* rows r0 <= y + ny*z < r1 of one step */
 static void step_rows(orc_lattice* L, long double* glob, long r0, long r1)
 {
@@ -340,6 +394,7 @@ static void step_rows(orc_lattice* L, long double* glob, long r0, long r1)
                 const double* zr = zone_row(L, nt);
                 double fd[27];
                 pull(L, in, x, y, z, fd);
+                if (is_autosym(L)) autosym_access(L, in, x, y, nt, fd);
                 if (L->storage == ORC_F64) {
                     if (is_cum(L)) cum_node_d(fd, nt, L->params, zr, glob);
                     else if (is_q27(L)) d3q27_node_d(fd, nt, L->params, zr, glob);
@@ -436,6 +491,7 @@ void orc_get_quantity(const orc_lattice* L, int which, double* out)
                 uint16_t nt = L->flags[n];
                 double f[27], rho = 0, jx = 0, jy = 0, jz = 0;
                 pull(L, L->buf[L->cur], x, y, z, f);
+                if (is_autosym(L)) autosym_access(L, L->buf[L->cur], x, y, nt, f);
                 if (is_cum(L)) {
                     /* getP on the pulled f; getU after RunBoundaries (0 on
                      * Wall, else (J + F/2)/rho); getSolid (1 on Wall),

@@ -18,6 +18,9 @@
  *   d3q27          models/flow/d3q27/Dynamics.c.Rt, Dynamics.R with the raw
  *                  moment basis of src/lib/feq.R (ortogonal=FALSE), Zou-He of
  *                  src/lib/boundary.R:66-109, Symmetry of :33-50
+ *   d2q9_autosym, d2q9_autosym2
+ *                  d2q9 with the symmetric access of its autosym builds
+ *                  (src/conf.R:373-445, src/LatticeAccess.inc.cpp.Rt:233-470)
  *   d3q27_cumulant models/flow/d3q27_cumulant/Dynamics.c.Rt, Dynamics.R without
  *                  the AVG/IB/SMAG build options
  *   d3q19_lid      d3q19 plus a MovingWall node type (bounce-back with Ladd's
@@ -45,7 +48,7 @@ extern "C" {
 #endif
 
 enum { ORC_D2Q9_SRT = 0, ORC_D2Q9_SRT_DRAG = 1, ORC_D3Q19 = 2, ORC_D3Q19_LID = 3, ORC_D2Q9 = 4,
-       ORC_D2Q9_DRAG = 5, ORC_D3Q27 = 6, ORC_D3Q27_CUMULANT = 7 };
+       ORC_D2Q9_DRAG = 5, ORC_D3Q27 = 6, ORC_D3Q27_CUMULANT = 7, ORC_D2Q9_AUTOSYM = 8, ORC_D2Q9_AUTOSYM2 = 9 };
 enum { ORC_F64 = 0, ORC_F32 = 1, ORC_F32S = 2, ORC_F16S = 3, ORC_BF16S = 4 };
 
 /* d2q9 parameter vector:   [omega, Density]

@@ -15,9 +15,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 
 D2Q9_SRT, D2Q9_SRT_DRAG, D3Q19, D3Q19_LID, D2Q9, D2Q9_DRAG, D3Q27, D3Q27_CUMULANT = 0, 1, 2, 3, 4, 5, 6, 7
+D2Q9_AUTOSYM, D2Q9_AUTOSYM2 = 8, 9
 F64, F32, F32S, F16S, BF16S = 0, 1, 2, 3, 4
 MODELS = {"d2q9_SRT": D2Q9_SRT, "d2q9_SRT_drag": D2Q9_SRT_DRAG, "d3q19": D3Q19, "d3q19_lid": D3Q19_LID,
-          "d2q9": D2Q9, "d2q9_drag": D2Q9_DRAG, "d3q27": D3Q27, "d3q27_cumulant": D3Q27_CUMULANT}
+          "d2q9": D2Q9, "d2q9_drag": D2Q9_DRAG, "d3q27": D3Q27, "d3q27_cumulant": D3Q27_CUMULANT,
+          "d2q9_autosym": D2Q9_AUTOSYM, "d2q9_autosym2": D2Q9_AUTOSYM2}
 STORAGE = {"f64": F64, "f32": F32, "f32s": F32S, "f16s": F16S, "bf16s": BF16S}
 
 _lib = None
@@ -243,7 +245,7 @@ def lattice_for_case(case, storage="f64"):
         L.set_params([vals[n] for n in D3Q19_PARAMS])
     elif case.model == "d3q19_lid":
         L.set_params([vals[n] for n in D3Q19_LID_PARAMS])
-    elif case.model in ("d2q9", "d2q9_drag"):
+    elif case.model in ("d2q9", "d2q9_drag", "d2q9_autosym", "d2q9_autosym2"):
         L.set_params([vals[n] for n in D2Q9_PARAMS])
     elif case.model == "d3q27":
         L.set_params([vals[n] for n in D3Q27_PARAMS])

@@ -52,7 +52,16 @@ template <> struct Lat<MODEL_D2Q9> : Lat<MODEL_D2Q9_SRT> {
 template <> struct Lat<MODEL_D2Q9_DRAG> : Lat<MODEL_D2Q9> {
     static constexpr int G = 6, SUMG = 6;
 };
-template <int MODEL> constexpr bool is_d2q9m = MODEL == MODEL_D2Q9 || MODEL == MODEL_D2Q9_DRAG;
+// d2q9_autosym / d2q9_autosym2: d2q9 plus the groups SYMX (bits 7-8) and
+// SYMY (bits 9-10) of the reference's autosym builds; zone from bit 11
+template <> struct Lat<MODEL_D2Q9_AUTOSYM> : Lat<MODEL_D2Q9> {
+    static constexpr int ZSHIFT = 11;
+};
+template <> struct Lat<MODEL_D2Q9_AUTOSYM2> : Lat<MODEL_D2Q9> {
+    static constexpr int ZSHIFT = 11;
+};
+template <int MODEL> constexpr bool is_d2q9m =
+    MODEL == MODEL_D2Q9 || MODEL == MODEL_D2Q9_DRAG || MODEL == MODEL_D2Q9_AUTOSYM || MODEL == MODEL_D2Q9_AUTOSYM2;
 template <> struct Lat<MODEL_D3Q19> {
     static constexpr int Q = 19, G = 26, SUMG = 25, ZSHIFT = 6;
     __host__ __device__ static constexpr int ex(int i) {
@@ -130,6 +139,8 @@ int kernel_nglobals(int model) {
     case MODEL_D2Q9_DRAG: return Lat<MODEL_D2Q9_DRAG>::G;
     case MODEL_D3Q27: return Lat<MODEL_D3Q27>::G;
     case MODEL_D3Q27_CUMULANT: return Lat<MODEL_D3Q27_CUMULANT>::G;
+    case MODEL_D2Q9_AUTOSYM: return Lat<MODEL_D2Q9_AUTOSYM>::G;
+    case MODEL_D2Q9_AUTOSYM2: return Lat<MODEL_D2Q9_AUTOSYM2>::G;
     }
     return 0;
 }
@@ -290,6 +301,13 @@ __device__ __forceinline__ void d2q9m_feq(R* f, R rho, R Jx, R Jy) {
 // outgoing f and T their tangential momentum, the equations give
 //   velocity: R_n = V_n S0 / (1 + s V_n),  R_t = 3 (V_t (S0 - s R_n) - T)
 //   pressure: R_n = s (S0 - (3 P + 1)),    R_t = -3 T.
+// d2q9 density with the x (y) component of its velocity negated
+__host__ __device__ constexpr int d2q9_symx(int i) {
+    return i == 1 ? 3 : i == 3 ? 1 : i == 5 ? 6 : i == 6 ? 5 : i == 7 ? 8 : i == 8 ? 7 : i;
+}
+__host__ __device__ constexpr int d2q9_symy(int i) {
+    return i == 2 ? 4 : i == 4 ? 2 : i == 5 ? 8 : i == 8 ? 5 : i == 6 ? 7 : i == 7 ? 6 : i;
+}
 __host__ __device__ constexpr int d2q9_opp(int i) { return i == 0 ? 0 : (i < 5 ? (i + 1) % 4 + 1 : (i - 3) % 4 + 5); }
 
 template <class R, int A, int SGN, bool VEL>
@@ -1407,6 +1425,50 @@ __device__ __forceinline__ void block_globals(const double* g, int sum_globals, 
     __syncthreads();  // the LDS is reused by the next tile of a persistent block
 }
 
+// ------------------------------------------------------------ autosym ---
+// SymmetryAccess of the reference's autosym builds of d2q9
+// (src/LatticeAccess.inc.cpp.Rt:233-260, 376-470), applied to the pulled
+// populations f of a node with NodeType nt.  SYMX is bits 7-8, SYMY bits
+// 9-10 (1 = _plus: the plane on the node's + side, 2 = _minus).
+//   autosym = 1 (Symmetry, plane through the node): the populations that
+//     crossed the plane take their mirror images' values, X first, then Y.
+//   autosym = 2 (SymmetryEdge, plane half a node out): a population pulled
+//     from beyond the plane is read from its mirror image's field at the
+//     mirrored offset -- X first, then Y on the result (loadX_ -> loadY_).
+// load(field, ox, oy) reads a field at an offset from the node.
+template <int MODEL, class R, class LOAD>
+__device__ __forceinline__ void d2q9_autosym(R* f, unsigned nt, LOAD load) {
+    const unsigned sx = (nt >> 7) & 3u, sy = (nt >> 9) & 3u;
+    if constexpr (MODEL == MODEL_D2Q9_AUTOSYM) {
+        const bool xp = sx == 1u, xm = sx == 2u, yp = sy == 1u, ym = sy == 2u;
+        const R a1 = f[1], a3 = f[3], a5 = f[5], a6 = f[6], a7 = f[7], a8 = f[8];
+        f[3] = xp ? a1 : a3; f[6] = xp ? a5 : a6; f[7] = xp ? a8 : a7;
+        f[1] = xm ? a3 : a1; f[5] = xm ? a6 : a5; f[8] = xm ? a7 : a8;
+        const R b2 = f[2], b4 = f[4], b5 = f[5], b6 = f[6], b7 = f[7], b8 = f[8];
+        f[4] = yp ? b2 : b4; f[7] = yp ? b6 : b7; f[8] = yp ? b5 : b8;
+        f[2] = ym ? b4 : b2; f[5] = ym ? b8 : b5; f[6] = ym ? b7 : b6;
+    } else {
+        if ((sx | sy) == 0u) return;
+        using L = Lat<MODEL_D2Q9_SRT>;
+#pragma unroll
+        for (int i = 1; i < 9; i++) {
+            int fi = i, ox = -L::ex(i), oy = -L::ey(i);
+            bool ch = false;
+            if ((ox > 0 && sx == 1u) || (ox < 0 && sx == 2u)) {
+                fi = d2q9_symx(fi);
+                ox = (ox > 0 ? 1 : -1) - ox;
+                ch = true;
+            }
+            if ((oy > 0 && sy == 1u) || (oy < 0 && sy == 2u)) {
+                fi = d2q9_symy(fi);
+                oy = (oy > 0 ? 1 : -1) - oy;
+                ch = true;
+            }
+            if (ch) f[i] = load(fi, ox, oy);
+        }
+    }
+}
+
 // ---------------------------------------------------------- step kernel ---
 // One node of row `row` (y + ny*z) at x = xc: pull with periodic wrap, Run(),
 // store; `active` is false for the lanes past the end of the row (they load
@@ -1447,6 +1509,14 @@ __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, 
         const unsigned src = (unsigned)sx + (unsigned)nx * ((unsigned)sy + (unsigned)ny * (unsigned)sz);
         const T v = *(const T*)(fin + (size_t)i * pbytes + src * (unsigned)sizeof(T));
         f[i] = DEV ? S::raw(v) : S::get(v, (R)L::w(i));
+    }
+    if constexpr (MODEL == MODEL_D2Q9_AUTOSYM || MODEL == MODEL_D2Q9_AUTOSYM2) {
+        d2q9_autosym<MODEL>(f, nt, [&](int fi, int ox, int oy) {
+            const int sxn = ox > 0 ? xp : (ox < 0 ? xm : xc);
+            const int syn = oy > 0 ? yp : (oy < 0 ? ym : y);
+            const unsigned src = (unsigned)sxn + (unsigned)nx * (unsigned)syn;
+            return S::get(*(const T*)(fin + (size_t)fi * pbytes + src * (unsigned)sizeof(T)), (R)L::w(fi));
+        });
     }
 
     if (active) {
@@ -1637,6 +1707,12 @@ __global__ void __launch_bounds__(256) k_quantity(StepArgs a, int qid, int x0, i
         const int sz = (z - L::ez(i) + nz) % nz;
         const unsigned src = (unsigned)sx + (unsigned)nx * ((unsigned)sy + (unsigned)ny * (unsigned)sz);
         f[i] = (double)S::get(fin[(size_t)i * a.pitch + src], (typename S::R)L::w(i));
+    }
+    if constexpr (MODEL == MODEL_D2Q9_AUTOSYM || MODEL == MODEL_D2Q9_AUTOSYM2) {
+        d2q9_autosym<MODEL>(f, nt, [&](int fi, int ox, int oy) {
+            const unsigned src = (unsigned)((x + ox + nx) % nx) + (unsigned)nx * (unsigned)((y + oy + ny) % ny);
+            return (double)S::get(fin[(size_t)fi * a.pitch + src], (typename S::R)L::w(fi));
+        });
     }
     if constexpr (is_d3q19<MODEL>) {
         double rho = 0, jx = 0, jy = 0, jz = 0;
@@ -1887,6 +1963,8 @@ hipError_t launch_step(int model, int storage, bool globals, const StepArgs& a, 
     case MODEL_D2Q9_DRAG: return step_m<MODEL_D2Q9_DRAG>(storage, globals, a, block, s);
     case MODEL_D3Q27: return step_m<MODEL_D3Q27>(storage, globals, a, block, s);
     case MODEL_D3Q27_CUMULANT: return step_m<MODEL_D3Q27_CUMULANT>(storage, globals, a, block, s);
+    case MODEL_D2Q9_AUTOSYM: return step_m<MODEL_D2Q9_AUTOSYM>(storage, globals, a, block, s);
+    case MODEL_D2Q9_AUTOSYM2: return step_m<MODEL_D2Q9_AUTOSYM2>(storage, globals, a, block, s);
     }
     return hipErrorInvalidValue;
 }
@@ -1901,6 +1979,8 @@ hipError_t launch_init(int model, int storage, const StepArgs& a, int block, hip
     case MODEL_D2Q9_DRAG: return init_m<MODEL_D2Q9_DRAG>(storage, a, block, s);
     case MODEL_D3Q27: return init_m<MODEL_D3Q27>(storage, a, block, s);
     case MODEL_D3Q27_CUMULANT: return init_m<MODEL_D3Q27_CUMULANT>(storage, a, block, s);
+    case MODEL_D2Q9_AUTOSYM: return init_m<MODEL_D2Q9_AUTOSYM>(storage, a, block, s);
+    case MODEL_D2Q9_AUTOSYM2: return init_m<MODEL_D2Q9_AUTOSYM2>(storage, a, block, s);
     }
     return hipErrorInvalidValue;
 }
@@ -1963,6 +2043,10 @@ hipError_t launch_batch_step(int model, int storage, bool globals, const BatchCa
         return batch_m<MODEL_D3Q27>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
     case MODEL_D3Q27_CUMULANT:
         return batch_m<MODEL_D3Q27_CUMULANT>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
+    case MODEL_D2Q9_AUTOSYM:
+        return batch_m<MODEL_D2Q9_AUTOSYM>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
+    case MODEL_D2Q9_AUTOSYM2:
+        return batch_m<MODEL_D2Q9_AUTOSYM2>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
     }
     return hipErrorInvalidValue;
 }
@@ -1982,6 +2066,10 @@ hipError_t launch_quantity(int model, int storage, int q, const StepArgs& a, int
     case MODEL_D3Q27: return quantity_m<MODEL_D3Q27>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
     case MODEL_D3Q27_CUMULANT:
         return quantity_m<MODEL_D3Q27_CUMULANT>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
+    case MODEL_D2Q9_AUTOSYM:
+        return quantity_m<MODEL_D2Q9_AUTOSYM>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
+    case MODEL_D2Q9_AUTOSYM2:
+        return quantity_m<MODEL_D2Q9_AUTOSYM2>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
     }
     return hipErrorInvalidValue;
 }

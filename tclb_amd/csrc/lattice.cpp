@@ -223,7 +223,8 @@ void model_params(const ModelDef* model, const std::vector<double>& settings, do
         // [omega, Smag, ForceX, ForceY, ForceZ]
         const char* names[] = {"omega", "Smag", "ForceX", "ForceY", "ForceZ"};
         for (int i = 0; i < 5; i++) par[i] = S(names[i]);
-    } else if (model->id == MODEL_D2Q9 || model->id == MODEL_D2Q9_DRAG) {
+    } else if (model->id == MODEL_D2Q9 || model->id == MODEL_D2Q9_DRAG || model->id == MODEL_D2Q9_AUTOSYM ||
+               model->id == MODEL_D2Q9_AUTOSYM2) {
         // [S2, S3, S4, GravitationX, GravitationY]: the relaxation factors of
         // the 2nd, 3rd and 4th order moments and the body force
         par[0] = S("S2");

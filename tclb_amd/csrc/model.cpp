@@ -195,10 +195,15 @@ ModelDef make_d3q19(bool lid) {
 // (models/flow/d2q9/Dynamics.R).  With `drag` the model d2q9_drag: the
 // same plus Drag/Lift Globals by momentum exchange on Solid nodes, as
 // d2q9_SRT_drag adds them to d2q9_SRT.
-ModelDef make_d2q9_mrt(bool drag) {
+// autosym = 1, 2: the reference's d2q9_autosym / d2q9_autosym2 builds
+// (models/flow/d2q9/conf.mk OPT="bc*autosym", src/conf.R:373-445): NodeType
+// groups SYMX and SYMY with <name>X_plus/_minus, <name>Y_plus/_minus, name
+// Symmetry (plane through the node) or SymmetryEdge (plane half a node out),
+// acting in the streaming step.
+ModelDef make_d2q9_mrt(bool drag, int autosym = 0) {
     ModelDef m;
-    m.id = drag ? MODEL_D2Q9_DRAG : MODEL_D2Q9;
-    m.name = drag ? "d2q9_drag" : "d2q9";
+    m.id = drag ? MODEL_D2Q9_DRAG : (autosym == 1 ? MODEL_D2Q9_AUTOSYM : autosym == 2 ? MODEL_D2Q9_AUTOSYM2 : MODEL_D2Q9);
+    m.name = drag ? "d2q9_drag" : (autosym == 1 ? "d2q9_autosym" : autosym == 2 ? "d2q9_autosym2" : "d2q9");
     m.Q = 9; m.dim = 2;
     m.ex = {0, 1, 0, -1, 0, 1, -1, -1, 1};
     m.ey = {0, 0, 1, 0, -1, 1, 1, -1, -1};
@@ -219,11 +224,18 @@ ModelDef make_d2q9_mrt(bool drag) {
     }
     finish_globals(m, g);
     m.quantities = {{"Rho", false, "kg/m3"}, {"U", true, "m/s"}};
-    pack_nodetypes(m, {{"EPressure", "BOUNDARY"}, {"WPressure", "BOUNDARY"}, {"NVelocity", "BOUNDARY"},
-                       {"SVelocity", "BOUNDARY"}, {"WVelocity", "BOUNDARY"}, {"EVelocity", "BOUNDARY"},
-                       {"NSymmetry", "BOUNDARY"}, {"SSymmetry", "BOUNDARY"}, {"Inlet", "OBJECTIVE"},
-                       {"Outlet", "OBJECTIVE"}, {"Solid", "BOUNDARY"}, {"Wall", "BOUNDARY"},
-                       {"MRT", "COLLISION"}});
+    std::vector<std::pair<std::string, std::string>> types = {
+        {"EPressure", "BOUNDARY"}, {"WPressure", "BOUNDARY"}, {"NVelocity", "BOUNDARY"},
+        {"SVelocity", "BOUNDARY"}, {"WVelocity", "BOUNDARY"}, {"EVelocity", "BOUNDARY"},
+        {"NSymmetry", "BOUNDARY"}, {"SSymmetry", "BOUNDARY"}, {"Inlet", "OBJECTIVE"},
+        {"Outlet", "OBJECTIVE"}, {"Solid", "BOUNDARY"}, {"Wall", "BOUNDARY"},
+        {"MRT", "COLLISION"}};
+    if (autosym) {
+        const std::string s = autosym == 1 ? "Symmetry" : "SymmetryEdge";
+        for (const char* t : {"X_plus", "X_minus"}) types.push_back({s + t, "SYMX"});
+        for (const char* t : {"Y_plus", "Y_minus"}) types.push_back({s + t, "SYMY"});
+    }
+    pack_nodetypes(m, types);
     return m;
 }
 
@@ -302,7 +314,7 @@ ModelDef make_d3q27_cumulant() {
 const std::vector<ModelDef>& registry() {
     static std::vector<ModelDef> r = {make_d2q9(false), make_d2q9(true), make_d3q19(false), make_d3q19(true),
                                       make_d2q9_mrt(false), make_d2q9_mrt(true), make_d3q27(),
-                                      make_d3q27_cumulant()};
+                                      make_d3q27_cumulant(), make_d2q9_mrt(false, 1), make_d2q9_mrt(false, 2)};
     return r;
 }
 

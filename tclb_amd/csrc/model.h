@@ -27,7 +27,9 @@ enum ModelId {
     MODEL_D2Q9_DRAG = 5,
     MODEL_D3Q27 = 6,
     MODEL_D3Q27_CUMULANT = 7,
-    MODEL_COUNT = 8
+    MODEL_D2Q9_AUTOSYM = 8,
+    MODEL_D2Q9_AUTOSYM2 = 9,
+    MODEL_COUNT = 10
 };
 
 // storage format of the distributions in HBM

@@ -29,7 +29,7 @@ def test_every_declared_symbol_is_exported():
 
 def test_models():
     assert tclb_amd.models() == ["d2q9_SRT", "d2q9_SRT_drag", "d3q19", "d3q19_lid", "d2q9", "d2q9_drag", "d3q27",
-                                 "d3q27_cumulant"]
+                                 "d3q27_cumulant", "d2q9_autosym", "d2q9_autosym2"]
 
 
 def test_d2q9_mrt_contract():

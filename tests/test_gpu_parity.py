@@ -65,7 +65,7 @@ def random_case(model, nx, ny, nz, seed, steps=10):
     rng = np.random.default_rng(seed)
     m = tclb_amd.Model(model)
     zs = m.group("SETTINGZONE")[1]
-    if model in ("d2q9", "d2q9_drag"):
+    if model in ("d2q9", "d2q9_drag", "d2q9_autosym", "d2q9_autosym2"):
         MRT = m.flag("MRT")
         kinds = [MRT] * 12 + [m.flag("Wall"), m.flag("Solid"), m.flag("Wall", "MRT"), m.flag("Solid", "MRT")]
         for b in ("EPressure", "WPressure", "NVelocity", "SVelocity", "WVelocity", "EVelocity", "NSymmetry",
@@ -142,6 +142,9 @@ def random_case(model, nx, ny, nz, seed, steps=10):
     flags = np.array(kinds, dtype=np.uint16)[rng.integers(0, len(kinds), n)]
     if model.startswith("d3q19"):
         flags |= (rng.integers(0, 4, n) * ((flags & 0x20) > 0)).astype(np.uint16)  # slices on MRT nodes
+    if model.startswith("d2q9_autosym"):   # SYMX / SYMY planes on a third of the nodes each
+        flags |= ((rng.integers(0, 3, n) * (rng.random(n) < 0.33)) << 7).astype(np.uint16)
+        flags |= ((rng.integers(0, 3, n) * (rng.random(n) < 0.33)) << 9).astype(np.uint16)
     flags |= (rng.integers(0, 4, n) << zs).astype(np.uint16)
     flags = flags.reshape(nz, ny, nx)
     Q = len(w)
@@ -150,7 +153,8 @@ def random_case(model, nx, ny, nz, seed, steps=10):
 
 
 MODELS = [("d2q9_SRT", 24, 10, 1), ("d2q9_SRT_drag", 37, 9, 1), ("d3q19", 13, 7, 6), ("d3q19_lid", 11, 9, 7),
-          ("d2q9", 29, 11, 1), ("d2q9_drag", 23, 13, 1), ("d3q27", 12, 7, 5), ("d3q27_cumulant", 11, 8, 5)]
+          ("d2q9", 29, 11, 1), ("d2q9_drag", 23, 13, 1), ("d3q27", 12, 7, 5), ("d3q27_cumulant", 11, 8, 5),
+          ("d2q9_autosym", 17, 11, 1), ("d2q9_autosym2", 19, 12, 1)]
 
 
 @pytest.mark.parametrize("model,nx,ny,nz", MODELS)
@@ -158,7 +162,8 @@ def test_random_case_f64(oracle_mod, model, nx, ny, nz):
     say(f"random case {model} {nx}x{ny}x{nz} f64")
     case = random_case(model, nx, ny, nz,
                        seed={"d2q9_SRT": 11, "d2q9_SRT_drag": 12, "d3q19": 13, "d3q19_lid": 14, "d2q9": 15,
-                             "d2q9_drag": 16, "d3q27": 18, "d3q27_cumulant": 19}[model])
+                             "d2q9_drag": 16, "d3q27": 18, "d3q27_cumulant": 19, "d2q9_autosym": 20,
+                             "d2q9_autosym2": 21}[model])
     lat = case.build("f64")
     orc = oracle_for_case(oracle_mod, case, "f64")
     for n in (1, 6, 13):

@@ -1065,3 +1065,103 @@ def test_cumulant_symmetry_and_body_drag(O):
     assert abs(g[9] - 0.875) < 1e-15
     _, g = O.cum_node(f, 10 << 3, [0.1, 0.1, 1.0, 0, 0, 0, 1.0], zone)
     assert not g.any()
+
+
+# ------------------------------------------------------------- autosym ---
+D2E = np.array([(0, 0), (1, 0), (0, 1), (-1, 0), (0, -1), (1, 1), (-1, 1), (-1, -1), (1, -1)])
+D2W = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+
+
+def _mirror_map(axis):
+    """q -> the density with the velocity component on `axis` (0 x, 1 y) negated"""
+    out = []
+    for e in D2E:
+        m = e.copy()
+        m[axis] = -m[axis]
+        out.append(int(np.nonzero((D2E == m).all(1))[0][0]))
+    return np.array(out)
+
+
+def _extend(f, axis, kind):
+    """the periodic mirror extension of f [9, 1, ny, nx] across its two ends
+    along `axis` (0 x, 1 y): planes through the end nodes (kind 1, period
+    2(n-1)) or half a node beyond them (kind 2, period 2n)"""
+    ax = 3 - axis
+    mp = _mirror_map(axis)
+    n = f.shape[ax]
+    idx = list(range(n)) + (list(range(n - 2, 0, -1)) if kind == 1 else list(range(n - 1, -1, -1)))
+    g = np.take(f, idx, axis=ax).copy()
+    back = np.take(g, range(n, len(idx)), axis=ax)[mp]
+    sl = [slice(None)] * 4
+    sl[ax] = slice(n, len(idx))
+    g[tuple(sl)] = back
+    return g
+
+
+def _symmetrize_ends(f, axis):
+    ax = 3 - axis
+    mp = _mirror_map(axis)
+    for end in (0, f.shape[ax] - 1):
+        sl = [slice(None)] * 4
+        sl[ax] = end
+        v = f[tuple(sl)]
+        f[tuple(sl)] = 0.5 * (v + v[mp])
+    return f
+
+
+@pytest.mark.parametrize("kind", [1, 2])
+@pytest.mark.parametrize("axes", [(1,), (0,), (0, 1)])
+def test_autosym_equals_the_mirror_extended_lattice(O, kind, axes):
+    """d2q9_autosym (planes through the end nodes) and d2q9_autosym2 (planes
+    half a node beyond them): a lattice with Symmetry*_minus on its first
+    and *_plus on its last row/column evolves exactly like the part of the
+    periodic mirror-extended lattice it covers, along y, x or both
+    (corners: X then Y)."""
+    model = "d2q9_autosym" if kind == 1 else "d2q9_autosym2"
+    nx, ny = 7, 6
+    rng = np.random.default_rng(320 + 10 * kind + len(axes) + axes[0])
+    f = D2W[:, None, None, None] * (1 + 0.05 * rng.standard_normal((9, 1, ny, nx)))
+    if kind == 1:
+        for a in axes:
+            f = _symmetrize_ends(f, a)
+    flags = np.full((1, ny, nx), 0x10, np.uint16)            # MRT
+    for a in axes:
+        sh = 7 if a == 0 else 9                                 # SYMX / SYMY bits
+        if a == 0:
+            flags[0, :, 0] |= 2 << sh; flags[0, :, -1] |= 1 << sh
+        else:
+            flags[0, 0, :] |= 2 << sh; flags[0, -1, :] |= 1 << sh
+    full = f
+    for a in axes:
+        full = _extend(full, a, kind)
+    params = [1 - 1 / (3 * 0.05 + 0.5), 0.3, -0.2, 0, 0]
+    runs = []
+    for g, fl in ((f, flags), (full, np.full(full.shape[1:], 0x10, np.uint16))):
+        L = O.OracleLattice(model, g.shape[3], g.shape[2], 1)
+        L.set_params(params)
+        L.set_zone_table(np.zeros((1, 8)))
+        L.set_flags(fl)
+        L.init()
+        L.set_f(g)
+        L.iterate(25, globals_last=False)
+        runs.append(L.get_f())
+    half, ext = runs
+    got = ext[:, :, :ny, :nx]
+    assert np.max(np.abs(half - got)) < 1e-14
+    assert np.max(np.abs(half - f)) > 1e-4          # the state did evolve
+
+
+def test_autosym_models_contract(O):
+    """Both autosym builds add the groups SYMX and SYMY after d2q9's, with
+    <name>X_plus = 1, X_minus = 2 in each (src/conf.R:433-441), and move
+    the zone to bit 11."""
+    import tclb_amd
+    for model, name in (("d2q9_autosym", "Symmetry"), ("d2q9_autosym2", "SymmetryEdge")):
+        m = tclb_amd.Model(model)
+        assert m.nodetype(name + "X_plus") == (0x80, 0x180)
+        assert m.nodetype(name + "X_minus") == (0x100, 0x180)
+        assert m.nodetype(name + "Y_plus") == (0x200, 0x600)
+        assert m.nodetype(name + "Y_minus") == (0x400, 0x600)
+        assert m.group("SETTINGZONE") == (0xF800, 11)
+        d = tclb_amd.Model("d2q9")
+        assert m.settings == d.settings and m.global_names == d.global_names
