@@ -36,8 +36,9 @@ const char* tclbx_last_error(void);
 int tclbx_version(void);
 
 /* Models compiled in: "d2q9_SRT", "d2q9_SRT_drag", "d3q19", "d3q19_lid", "d2q9",
- * "d2q9_drag", "d3q27", "d3q27_cumulant" (without its AVG/IB/SMAG build
- * options), "d2q9_autosym", "d2q9_autosym2" (d2q9's autosym builds).
+ * "d2q9_drag", "d3q27", "d3q27_cumulant" (and its SMAG build
+ * "d3q27_cumulant_SMAG"; the AVG and IB builds are not compiled),
+ * "d2q9_autosym", "d2q9_autosym2" (d2q9's autosym builds).
  * Replaces the per-model binaries CLB/<model>/main (src/makefile.main.Rt:331-349). */
 int tclbx_model_count(void);
 const char* tclbx_model_name(int i);

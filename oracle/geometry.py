@@ -255,6 +255,7 @@ NODETYPES["d3q27_cumulant"] = [
     ("NPressure", "BOUNDARY"), ("SPressure", "BOUNDARY"), ("NSymmetry", "ADDITIONALS"), ("SSymmetry", "ADDITIONALS"),
     ("Body", "BODY"), ("EPressure", "BOUNDARY"), ("EVelocity", "BOUNDARY"), ("Solid", "BOUNDARY"),
     ("Wall", "BOUNDARY"), ("WPressure", "BOUNDARY"), ("WVelocity", "BOUNDARY"), ("MRT", "COLLISION")]
+NODETYPES["d3q27_cumulant_SMAG"] = NODETYPES["d3q27_cumulant"]
 
 
 class Packing:

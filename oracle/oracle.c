@@ -155,8 +155,8 @@ orc_lattice* orc_create(int model, int nx, int ny, int nz, int storage)
     d3q19_sparse_init_f();
     L->model = model; L->nx = nx; L->ny = ny; L->nz = nz; L->storage = storage;
     L->N = (size_t)nx * ny * nz;
-    if (model == ORC_D3Q27 || model == ORC_D3Q27_CUMULANT) {
-        int cum = model == ORC_D3Q27_CUMULANT;
+    if (model == ORC_D3Q27 || model == ORC_D3Q27_CUMULANT || model == ORC_D3Q27_CUMULANT_SMAG) {
+        int cum = model != ORC_D3Q27;
         cum_init_d();
         cum_init_f();
         L->Q = 27; L->nglob = cum ? 10 : 7;
@@ -228,7 +228,7 @@ static int is_d2q9m(const orc_lattice* L)
 }
 static int is_autosym(const orc_lattice* L) { return L->model == ORC_D2Q9_AUTOSYM || L->model == ORC_D2Q9_AUTOSYM2; }
 static int is_q27(const orc_lattice* L) { return L->model == ORC_D3Q27; }
-static int is_cum(const orc_lattice* L) { return L->model == ORC_D3Q27_CUMULANT; }
+static int is_cum(const orc_lattice* L) { return L->model == ORC_D3Q27_CUMULANT || L->model == ORC_D3Q27_CUMULANT_SMAG; }
 static int zone_shift(const orc_lattice* L)
 {
     return is3d(L) ? 6 : (is_cum(L) ? 8 : (is_autosym(L) ? 11 : (is_d2q9m(L) || is_q27(L) ? 7 : 4)));
@@ -396,7 +396,7 @@ static void step_rows(orc_lattice* L, long double* glob, long r0, long r1)
                 pull(L, in, x, y, z, fd);
                 if (is_autosym(L)) autosym_access(L, in, x, y, nt, fd);
                 if (L->storage == ORC_F64) {
-                    if (is_cum(L)) cum_node_d(fd, nt, L->params, zr, glob);
+                    if (is_cum(L)) cum_node_d(fd, nt, L->params, zr, glob, L->model == ORC_D3Q27_CUMULANT_SMAG);
                     else if (is_q27(L)) d3q27_node_d(fd, nt, L->params, zr, glob);
                     else if (is3d(L)) d3q19_node_d(fd, nt, L->params, zr, glob, lid, L->flags, x, y, z, L->nx, L->ny, L->nz);
                     else if (is_d2q9m(L)) d2q9m_node_d(fd, nt, L->params, zr, glob, L->model == ORC_D2Q9_DRAG);
@@ -405,7 +405,7 @@ static void step_rows(orc_lattice* L, long double* glob, long r0, long r1)
                 } else {
                     float ff[27];
                     for (q = 0; q < L->Q; q++) ff[q] = (float)fd[q];
-                    if (is_cum(L)) cum_node_f(ff, nt, L->params, zr, glob);
+                    if (is_cum(L)) cum_node_f(ff, nt, L->params, zr, glob, L->model == ORC_D3Q27_CUMULANT_SMAG);
                     else if (is_q27(L)) d3q27_node_f(ff, nt, L->params, zr, glob);
                     else if (is3d(L)) d3q19_node_f(ff, nt, L->params, zr, glob, lid, L->flags, x, y, z, L->nx, L->ny, L->nz);
                     else if (is_d2q9m(L)) d2q9m_node_f(ff, nt, L->params, zr, glob, L->model == ORC_D2Q9_DRAG);
@@ -586,12 +586,12 @@ void orc_d3q27_node(double f[27], uint16_t nt, const double* params, const doubl
     d3q27_node_d(f, nt, params, zone, globals ? g : NULL);
     if (globals) for (i = 0; i < 7; i++) globals[i] = (double)g[i];
 }
-void orc_cum_node(double f[27], uint16_t nt, const double* params, const double* zone, double* globals)
+void orc_cum_node(double f[27], uint16_t nt, const double* params, const double* zone, double* globals, int smag)
 {
     long double g[10] = {0};
     int i;
     cum_init_d();
-    cum_node_d(f, nt, params, zone, globals ? g : NULL);
+    cum_node_d(f, nt, params, zone, globals ? g : NULL, smag);
     if (globals) for (i = 0; i < 10; i++) globals[i] = (double)g[i];
 }
 void orc_d3q27_basis(double M[27 * 27], int order[27], int p[27 * 3])

@@ -48,7 +48,8 @@ extern "C" {
 #endif
 
 enum { ORC_D2Q9_SRT = 0, ORC_D2Q9_SRT_DRAG = 1, ORC_D3Q19 = 2, ORC_D3Q19_LID = 3, ORC_D2Q9 = 4,
-       ORC_D2Q9_DRAG = 5, ORC_D3Q27 = 6, ORC_D3Q27_CUMULANT = 7, ORC_D2Q9_AUTOSYM = 8, ORC_D2Q9_AUTOSYM2 = 9 };
+       ORC_D2Q9_DRAG = 5, ORC_D3Q27 = 6, ORC_D3Q27_CUMULANT = 7, ORC_D2Q9_AUTOSYM = 8, ORC_D2Q9_AUTOSYM2 = 9,
+       ORC_D3Q27_CUMULANT_SMAG = 10 };
 enum { ORC_F64 = 0, ORC_F32 = 1, ORC_F32S = 2, ORC_F16S = 3, ORC_BF16S = 4 };
 
 /* d2q9 parameter vector:   [omega, Density]
@@ -67,7 +68,7 @@ enum { ORC_F64 = 0, ORC_F32 = 1, ORC_F32S = 2, ORC_F16S = 3, ORC_BF16S = 4 };
  * d3q27 zone columns:      [Velocity, Pressure, Turbulence, then the six
  *                           <Global>InObj weights in Globals order]
  * d3q27_cumulant params:   [nu, nubuffer, GalileanCorrection, ForceX, ForceY,
- *                           ForceZ, Omega]
+ *                           ForceZ, Omega, (d3q27_cumulant_SMAG) Smag]
  * d3q27_cumulant zones:    [Velocity, Pressure, Turbulence, then the nine
  *                           <Global>InObj weights in Globals order]         */
 
@@ -110,8 +111,8 @@ void orc_d3q27_feq(double feq[27], double rho, double jx, double jy, double jz);
 void orc_d3q27_node(double f[27], uint16_t nt, const double* params, const double* zone, double* globals);
 /* the moment basis: M[27*i + j] = moment j of direction i, its order and powers */
 void orc_d3q27_basis(double M[27 * 27], int order[27], int p[27 * 3]);
-/* one node update of d3q27_cumulant; globals[10] if not NULL */
-void orc_cum_node(double f[27], uint16_t nt, const double* params, const double* zone, double* globals);
+/* one node update of d3q27_cumulant (smag: its SMAG build); globals[10] if not NULL */
+void orc_cum_node(double f[27], uint16_t nt, const double* params, const double* zone, double* globals, int smag);
 
 #ifdef __cplusplus
 }

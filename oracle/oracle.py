@@ -15,11 +15,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 
 D2Q9_SRT, D2Q9_SRT_DRAG, D3Q19, D3Q19_LID, D2Q9, D2Q9_DRAG, D3Q27, D3Q27_CUMULANT = 0, 1, 2, 3, 4, 5, 6, 7
-D2Q9_AUTOSYM, D2Q9_AUTOSYM2 = 8, 9
+D2Q9_AUTOSYM, D2Q9_AUTOSYM2, D3Q27_CUMULANT_SMAG = 8, 9, 10
 F64, F32, F32S, F16S, BF16S = 0, 1, 2, 3, 4
 MODELS = {"d2q9_SRT": D2Q9_SRT, "d2q9_SRT_drag": D2Q9_SRT_DRAG, "d3q19": D3Q19, "d3q19_lid": D3Q19_LID,
           "d2q9": D2Q9, "d2q9_drag": D2Q9_DRAG, "d3q27": D3Q27, "d3q27_cumulant": D3Q27_CUMULANT,
-          "d2q9_autosym": D2Q9_AUTOSYM, "d2q9_autosym2": D2Q9_AUTOSYM2}
+          "d2q9_autosym": D2Q9_AUTOSYM, "d2q9_autosym2": D2Q9_AUTOSYM2, "d3q27_cumulant_SMAG": D3Q27_CUMULANT_SMAG}
 STORAGE = {"f64": F64, "f32": F32, "f32s": F32S, "f16s": F16S, "bf16s": BF16S}
 
 _lib = None
@@ -62,7 +62,7 @@ def lib():
         L.orc_d3q19_collide.argtypes = [dp, dp]
         L.orc_d3q27_feq.argtypes = [dp] + [ctypes.c_double] * 4
         L.orc_d3q27_node.argtypes = [dp, ctypes.c_uint16, dp, dp, dp]
-        L.orc_cum_node.argtypes = [dp, ctypes.c_uint16, dp, dp, dp]
+        L.orc_cum_node.argtypes = [dp, ctypes.c_uint16, dp, dp, dp, ip]
         L.orc_d3q27_basis.argtypes = [dp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.orc_set_threads.argtypes = [ip]
         _lib = L
@@ -177,8 +177,8 @@ def d3q27_node(f, nt, params, zone=None):
     return a, g
 
 
-def cum_node(f, nt, params, zone=None):
-    """one d3q27_cumulant node update; returns (f, Globals[10])"""
+def cum_node(f, nt, params, zone=None, smag=False):
+    """one d3q27_cumulant node update (smag: the SMAG build); returns (f, Globals[10])"""
     a = np.array(f, dtype=np.float64)
     p = np.zeros(16)
     p[: len(params)] = params
@@ -186,7 +186,7 @@ def cum_node(f, nt, params, zone=None):
     if zone is not None:
         z[: len(zone)] = zone
     g = np.zeros(10)
-    lib().orc_cum_node(_dp(a), nt, _dp(p), _dp(z), _dp(g))
+    lib().orc_cum_node(_dp(a), nt, _dp(p), _dp(z), _dp(g), 1 if smag else 0)
     return a, g
 
 
@@ -206,6 +206,7 @@ D3Q19_LID_PARAMS = D3Q19_PARAMS + ["MovingWallVelocityX", "MovingWallVelocityY",
 D2Q9_PARAMS = ["S2", "S3", "S4", "GravitationX", "GravitationY"]
 D3Q27_PARAMS = ["omega", "Smag", "ForceX", "ForceY", "ForceZ"]
 CUMULANT_PARAMS = ["nu", "nubuffer", "GalileanCorrection", "ForceX", "ForceY", "ForceZ", "Omega"]
+CUMULANT_SMAG_PARAMS = CUMULANT_PARAMS + ["Smag"]
 
 # derived settings of the models' Dynamics.R (AddSetting(name, <target>=expr))
 DERIVED = {
@@ -251,6 +252,8 @@ def lattice_for_case(case, storage="f64"):
         L.set_params([vals[n] for n in D3Q27_PARAMS])
     elif case.model == "d3q27_cumulant":
         L.set_params([vals[n] for n in CUMULANT_PARAMS])
+    elif case.model == "d3q27_cumulant_SMAG":
+        L.set_params([vals[n] for n in CUMULANT_SMAG_PARAMS])
     else:
         L.set_params([vals["omega"], vals["Density"]])
     zshift = m.group("SETTINGZONE")[1]

@@ -109,6 +109,7 @@ template <> struct Lat<MODEL_D3Q27> {
 // (2 bits), BODY, BOUNDARY (4 bits from bit 3), COLLISION (bit 7), zone
 // from bit 8; Globals Density, Flux, Drag, Lift, Lateral, Mass, X/Y/Z
 // Momentum, Objective
+template <int MODEL> constexpr bool is_cum = MODEL == MODEL_D3Q27_CUMULANT || MODEL == MODEL_D3Q27_CUMULANT_SMAG;
 template <> struct Lat<MODEL_D3Q27_CUMULANT> {
     static constexpr int Q = 27, G = 10, SUMG = 10, ZSHIFT = 8;
     __host__ __device__ static constexpr int e1(int p) { return p == 0 ? 0 : (p == 1 ? 1 : -1); }
@@ -128,6 +129,7 @@ template <> struct Lat<MODEL_D3Q27_CUMULANT> {
              : ((i % 3 != 0) + (i / 3 % 3 != 0) + (i / 9 != 0)) == 2 ? 1. / 54 : 1. / 216;
     }
 };
+template <> struct Lat<MODEL_D3Q27_CUMULANT_SMAG> : Lat<MODEL_D3Q27_CUMULANT> {};
 
 int kernel_nglobals(int model) {
     switch (model) {
@@ -139,6 +141,7 @@ int kernel_nglobals(int model) {
     case MODEL_D2Q9_DRAG: return Lat<MODEL_D2Q9_DRAG>::G;
     case MODEL_D3Q27: return Lat<MODEL_D3Q27>::G;
     case MODEL_D3Q27_CUMULANT: return Lat<MODEL_D3Q27_CUMULANT>::G;
+    case MODEL_D3Q27_CUMULANT_SMAG: return Lat<MODEL_D3Q27_CUMULANT_SMAG>::G;
     case MODEL_D2Q9_AUTOSYM: return Lat<MODEL_D2Q9_AUTOSYM>::G;
     case MODEL_D2Q9_AUTOSYM2: return Lat<MODEL_D2Q9_AUTOSYM2>::G;
     }
@@ -1287,7 +1290,8 @@ __device__ __forceinline__ void cum_boundaries(R* f, unsigned nt, const double* 
     }
 }
 
-template <class R, bool GLOB>
+// SMAG: the d3q27_cumulant_SMAG build (par[7] = Smag)
+template <class R, bool GLOB, bool SMAG = false>
 __device__ __forceinline__ void cum_node(R* f, unsigned nt, const double* par, const double* zr, double* g) {
     using L = Lat<MODEL_D3Q27_CUMULANT>;
     cum_boundaries<R, GLOB>(f, nt, zr, g);
@@ -1311,11 +1315,24 @@ __device__ __forceinline__ void cum_node(R* f, unsigned nt, const double* par, c
         g[7] += (double)(rho * uy); g[9] += zr[10];
         g[8] += (double)(rho * uz); g[9] += zr[11];
     }
-    const R w0 = ((nt >> 3) & 0xFu) != 0 ? R(1) / (R(3) * (R)par[1] + R(0.5))
-                                          : R(1) / (R(3) * (R)par[0] + R(0.5));
+    R w0 = ((nt >> 3) & 0xFu) != 0 ? R(1) / (R(3) * (R)par[1] + R(0.5))
+                                    : R(1) / (R(3) * (R)par[0] + R(0.5));
     const R w1 = R(1);
     c[0] = R(1);
     cum_convert<R, true>(m, c);
+    if constexpr (SMAG) {
+        // Smagorinsky (Dynamics.c.Rt:362-374): tau_t from the deviation of the
+        // second cumulants from 1/3; buffer-layer nodes keep nubuffer
+        const R third = R(1) / R(3);
+        const R q = sqrt((c[2] - third) * (c[2] - third) + (c[6] - third) * (c[6] - third) +
+                         (c[18] - third) * (c[18] - third) + R(2) * (c[4] * c[4]) + R(2) * (c[10] * c[10]) +
+                         R(2) * (c[12] * c[12]));
+        const R tau0 = R(3) * (R)par[0] + R(0.5);
+        const R Smag = (R)par[7];
+        const R taut = R(0.5) * (sqrt(tau0 * tau0 + R(18) * Smag * Smag * q) - tau0);
+        w0 = R(1) / (tau0 + taut);
+        if (((nt >> 3) & 0xFu) != 0) w0 = R(1) / (R(3) * (R)par[1] + R(0.5));
+    }
     const R h = R(0.5) / rho;
     const R ux = c[1] + Fx * h, uy = c[3] + Fy * h, uz = c[9] + Fz * h;
     // second order: c200 = c[2], c020 = c[6], c002 = c[18], c110 = c[4],
@@ -1528,8 +1545,8 @@ __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, 
             d2q9m_node<R, GLOB, MODEL == MODEL_D2Q9_DRAG>(f, nt, a.par, zr, g);
         } else if constexpr (MODEL == MODEL_D3Q27) {
             d3q27_node<R, GLOB>(f, nt, a.par, zr, g);
-        } else if constexpr (MODEL == MODEL_D3Q27_CUMULANT) {
-            cum_node<R, GLOB>(f, nt, a.par, zr, g);
+        } else if constexpr (is_cum<MODEL>) {
+            cum_node<R, GLOB, MODEL == MODEL_D3Q27_CUMULANT_SMAG>(f, nt, a.par, zr, g);
         } else {
             d2q9_node<R, MODEL == MODEL_D2Q9_SRT_DRAG, GLOB>(f, nt, (R)a.par[0], (R)a.par[1], zr, g);
         }
@@ -1665,7 +1682,7 @@ __global__ void __launch_bounds__(256) k_init(StepArgs a) {
         // 3 Pressure, Velocity + Turbulence*ST.x, ...) with no synthetic
         // turbulence modes (ST = 0)
         d3q27_feq<R>(f, R(1) + (R)zr[1] * R(3), (R)zr[0], R(0), R(0));
-    } else if constexpr (MODEL == MODEL_D3Q27_CUMULANT) {
+    } else if constexpr (is_cum<MODEL>) {
         // Init(), models/flow/d3q27_cumulant/Dynamics.c.Rt:266-275: Wall
         // nodes at rest, the rest at (1 + 3 Pressure, Velocity, 0, 0)
         if (((nt >> 3) & 0xFu) == 10u) cum_feq<R>(f, R(1), R(0), R(0), R(0));
@@ -1748,7 +1765,7 @@ __global__ void __launch_bounds__(256) k_quantity(StepArgs a, int qid, int x0, i
             out[3 * tid + 1] = wall ? -2.0 * jy * scale : 0.0;
             out[3 * tid + 2] = wall ? -2.0 * jz * scale : 0.0;
         }
-    } else if constexpr (MODEL == MODEL_D3Q27_CUMULANT) {
+    } else if constexpr (is_cum<MODEL>) {
         // getP = 0 on the pulled populations; getU = 1 after RunBoundaries()
         // (0 on Wall nodes, else (J + F/2)/rho); getSolid = 2 (1 on Wall)
         // (models/flow/d3q27_cumulant/Dynamics.c.Rt:23-62)
@@ -1938,7 +1955,7 @@ hipError_t quantity_m(int st, int q, const StepArgs& a, int x0, int y0, int z0, 
 
 double weight(int model, int q) {
     if (model == MODEL_D3Q27) return Lat<MODEL_D3Q27>::w(q);
-    if (model == MODEL_D3Q27_CUMULANT) return Lat<MODEL_D3Q27_CUMULANT>::w(q);
+    if (model == MODEL_D3Q27_CUMULANT || model == MODEL_D3Q27_CUMULANT_SMAG) return Lat<MODEL_D3Q27_CUMULANT>::w(q);
     return (model == MODEL_D3Q19 || model == MODEL_D3Q19_LID) ? Lat<MODEL_D3Q19>::w(q) : Lat<MODEL_D2Q9_SRT>::w(q);
 }
 }  // namespace
@@ -1963,6 +1980,7 @@ hipError_t launch_step(int model, int storage, bool globals, const StepArgs& a, 
     case MODEL_D2Q9_DRAG: return step_m<MODEL_D2Q9_DRAG>(storage, globals, a, block, s);
     case MODEL_D3Q27: return step_m<MODEL_D3Q27>(storage, globals, a, block, s);
     case MODEL_D3Q27_CUMULANT: return step_m<MODEL_D3Q27_CUMULANT>(storage, globals, a, block, s);
+    case MODEL_D3Q27_CUMULANT_SMAG: return step_m<MODEL_D3Q27_CUMULANT_SMAG>(storage, globals, a, block, s);
     case MODEL_D2Q9_AUTOSYM: return step_m<MODEL_D2Q9_AUTOSYM>(storage, globals, a, block, s);
     case MODEL_D2Q9_AUTOSYM2: return step_m<MODEL_D2Q9_AUTOSYM2>(storage, globals, a, block, s);
     }
@@ -1979,6 +1997,7 @@ hipError_t launch_init(int model, int storage, const StepArgs& a, int block, hip
     case MODEL_D2Q9_DRAG: return init_m<MODEL_D2Q9_DRAG>(storage, a, block, s);
     case MODEL_D3Q27: return init_m<MODEL_D3Q27>(storage, a, block, s);
     case MODEL_D3Q27_CUMULANT: return init_m<MODEL_D3Q27_CUMULANT>(storage, a, block, s);
+    case MODEL_D3Q27_CUMULANT_SMAG: return init_m<MODEL_D3Q27_CUMULANT_SMAG>(storage, a, block, s);
     case MODEL_D2Q9_AUTOSYM: return init_m<MODEL_D2Q9_AUTOSYM>(storage, a, block, s);
     case MODEL_D2Q9_AUTOSYM2: return init_m<MODEL_D2Q9_AUTOSYM2>(storage, a, block, s);
     }
@@ -2043,6 +2062,8 @@ hipError_t launch_batch_step(int model, int storage, bool globals, const BatchCa
         return batch_m<MODEL_D3Q27>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
     case MODEL_D3Q27_CUMULANT:
         return batch_m<MODEL_D3Q27_CUMULANT>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
+    case MODEL_D3Q27_CUMULANT_SMAG:
+        return batch_m<MODEL_D3Q27_CUMULANT_SMAG>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
     case MODEL_D2Q9_AUTOSYM:
         return batch_m<MODEL_D2Q9_AUTOSYM>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
     case MODEL_D2Q9_AUTOSYM2:
@@ -2066,6 +2087,8 @@ hipError_t launch_quantity(int model, int storage, int q, const StepArgs& a, int
     case MODEL_D3Q27: return quantity_m<MODEL_D3Q27>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
     case MODEL_D3Q27_CUMULANT:
         return quantity_m<MODEL_D3Q27_CUMULANT>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
+    case MODEL_D3Q27_CUMULANT_SMAG:
+        return quantity_m<MODEL_D3Q27_CUMULANT_SMAG>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
     case MODEL_D2Q9_AUTOSYM:
         return quantity_m<MODEL_D2Q9_AUTOSYM>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
     case MODEL_D2Q9_AUTOSYM2:

@@ -215,10 +215,10 @@ void model_params(const ModelDef* model, const std::vector<double>& settings, do
         par[12] = S("MovingWallVelocityX");
         par[13] = S("MovingWallVelocityY");
         par[14] = S("MovingWallVelocityZ");
-    } else if (model->id == MODEL_D3Q27_CUMULANT) {
-        // [nu, nubuffer, GalileanCorrection, ForceX, ForceY, ForceZ, Omega]
-        const char* names[] = {"nu", "nubuffer", "GalileanCorrection", "ForceX", "ForceY", "ForceZ", "Omega"};
-        for (int i = 0; i < 7; i++) par[i] = S(names[i]);
+    } else if (model->id == MODEL_D3Q27_CUMULANT || model->id == MODEL_D3Q27_CUMULANT_SMAG) {
+        // [nu, nubuffer, GalileanCorrection, ForceX, ForceY, ForceZ, Omega, Smag]
+        const char* names[] = {"nu", "nubuffer", "GalileanCorrection", "ForceX", "ForceY", "ForceZ", "Omega", "Smag"};
+        for (int i = 0; i < 8; i++) par[i] = S(names[i]);
     } else if (model->id == MODEL_D3Q27) {
         // [omega, Smag, ForceX, ForceY, ForceZ]
         const char* names[] = {"omega", "Smag", "ForceX", "ForceY", "ForceZ"};

@@ -278,10 +278,12 @@ ModelDef make_d3q27() {
 // {0,1,-1}[pz]) (named f<px><py><pz> there); the cumulant collision with a
 // Galilean correction, Zou-He on the W/E and S/N faces, symmetry planes,
 // momentum-exchange drag on Wall nodes marked Body, buffer-layer viscosity.
-ModelDef make_d3q27_cumulant() {
+// smag: the SMAG build (d3q27_cumulant_SMAG): a Smag setting and the
+// Smagorinsky viscosity of Dynamics.c.Rt:362-374
+ModelDef make_d3q27_cumulant(bool smag = false) {
     ModelDef m;
-    m.id = MODEL_D3Q27_CUMULANT;
-    m.name = "d3q27_cumulant";
+    m.id = smag ? MODEL_D3Q27_CUMULANT_SMAG : MODEL_D3Q27_CUMULANT;
+    m.name = smag ? "d3q27_cumulant_SMAG" : "d3q27_cumulant";
     m.Q = 27; m.dim = 3;
     const int e1[3] = {0, 1, -1};
     for (int i = 0; i < 27; i++) {
@@ -294,7 +296,9 @@ ModelDef make_d3q27_cumulant() {
     // Dynamics.R:18-28 (nu has no derived setting here), Threshold from src/conf.R:769
     m.settings = {{"nu", 0.16666666, -1, 0},         {"nubuffer", 0.01, -1, 0}, {"GalileanCorrection", 1.0, -1, 0},
                   {"ForceX", 0.0, -1, 0},            {"ForceY", 0.0, -1, 0},    {"ForceZ", 0.0, -1, 0},
-                  {"Omega", 1.0, -1, 0},             {"Threshold", 0.5, -1, 0}};
+                  {"Omega", 1.0, -1, 0}};
+    if (smag) m.settings.push_back({"Smag", 0.0, -1, 0});
+    m.settings.push_back({"Threshold", 0.5, -1, 0});
     m.zonal = {{"Velocity", 0, -1, 0}, {"Pressure", 0, -1, 0}, {"Turbulence", 0, -1, 0}};
     std::vector<GlobalDef> g = {{"Density", false, "kg/m3"}, {"Flux", false, "m3/s"},     {"Drag", false, "N"},
                                 {"Lift", false, "N"},        {"Lateral", false, "N"},     {"Mass", false, "kg"},
@@ -314,7 +318,8 @@ ModelDef make_d3q27_cumulant() {
 const std::vector<ModelDef>& registry() {
     static std::vector<ModelDef> r = {make_d2q9(false), make_d2q9(true), make_d3q19(false), make_d3q19(true),
                                       make_d2q9_mrt(false), make_d2q9_mrt(true), make_d3q27(),
-                                      make_d3q27_cumulant(), make_d2q9_mrt(false, 1), make_d2q9_mrt(false, 2)};
+                                      make_d3q27_cumulant(), make_d2q9_mrt(false, 1), make_d2q9_mrt(false, 2),
+                                      make_d3q27_cumulant(true)};
     return r;
 }
 

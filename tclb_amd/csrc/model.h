@@ -29,7 +29,8 @@ enum ModelId {
     MODEL_D3Q27_CUMULANT = 7,
     MODEL_D2Q9_AUTOSYM = 8,
     MODEL_D2Q9_AUTOSYM2 = 9,
-    MODEL_COUNT = 10
+    MODEL_D3Q27_CUMULANT_SMAG = 10,
+    MODEL_COUNT = 11
 };
 
 // storage format of the distributions in HBM

@@ -29,7 +29,7 @@ def test_every_declared_symbol_is_exported():
 
 def test_models():
     assert tclb_amd.models() == ["d2q9_SRT", "d2q9_SRT_drag", "d3q19", "d3q19_lid", "d2q9", "d2q9_drag", "d3q27",
-                                 "d3q27_cumulant", "d2q9_autosym", "d2q9_autosym2"]
+                                 "d3q27_cumulant", "d2q9_autosym", "d2q9_autosym2", "d3q27_cumulant_SMAG"]
 
 
 def test_d2q9_mrt_contract():
@@ -165,3 +165,6 @@ def test_d3q27_cumulant_contract():
     assert m.global_names == ["Density", "Flux", "Drag", "Lift", "Lateral", "Mass", "XMomentum", "YMomentum",
                               "ZMomentum", "Objective"]
     assert m.quantities == ["P", "U", "Solid"]
+    s = tclb_amd.Model("d3q27_cumulant_SMAG")
+    assert s.settings == m.settings[:-1] + ["Smag", "Threshold"]
+    assert s.global_names == m.global_names and s.group("SETTINGZONE") == m.group("SETTINGZONE")

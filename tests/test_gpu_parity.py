@@ -96,7 +96,7 @@ def random_case(model, nx, ny, nz, seed, steps=10):
             zonal += [("Velocity", z, 0.02 * rng.standard_normal()), ("Pressure", z, 0.002 * rng.standard_normal())]
         zonal += [("XFluxInObj", 1, 0.5), ("ZFluxInObj", -1, 0.25), ("XDragForceInObj", 2, -0.75),
                   ("YDragForceInObj", -1, 0.125)]
-    elif model == "d3q27_cumulant":
+    elif model.startswith("d3q27_cumulant"):
         MRT = m.flag("MRT")
         kinds = [MRT] * 12 + [m.flag("Wall"), m.flag("Wall", "Body"), m.flag("Wall", "MRT"), m.flag("Solid", "MRT")]
         for b in ("Buffer", "WVelocityTurbulent", "NVelocity", "SVelocity", "NPressure", "SPressure", "EPressure",
@@ -107,6 +107,8 @@ def random_case(model, nx, ny, nz, seed, steps=10):
         w = lattice_weights(27, cumulant=True)
         settings = [("nu", 0.04), ("nubuffer", 0.1), ("GalileanCorrection", 1.0), ("ForceX", 1e-5),
                     ("ForceY", -2e-6), ("ForceZ", 3e-6), ("Omega", 0.7)]
+        if model.endswith("SMAG"):
+            settings += [("Smag", 0.2)]
         zonal = []
         for z in range(4):
             zonal += [("Velocity", z, 0.02 * rng.standard_normal()), ("Pressure", z, 0.002 * rng.standard_normal())]
@@ -154,7 +156,7 @@ def random_case(model, nx, ny, nz, seed, steps=10):
 
 MODELS = [("d2q9_SRT", 24, 10, 1), ("d2q9_SRT_drag", 37, 9, 1), ("d3q19", 13, 7, 6), ("d3q19_lid", 11, 9, 7),
           ("d2q9", 29, 11, 1), ("d2q9_drag", 23, 13, 1), ("d3q27", 12, 7, 5), ("d3q27_cumulant", 11, 8, 5),
-          ("d2q9_autosym", 17, 11, 1), ("d2q9_autosym2", 19, 12, 1)]
+          ("d2q9_autosym", 17, 11, 1), ("d2q9_autosym2", 19, 12, 1), ("d3q27_cumulant_SMAG", 9, 8, 6)]
 
 
 @pytest.mark.parametrize("model,nx,ny,nz", MODELS)
@@ -163,7 +165,7 @@ def test_random_case_f64(oracle_mod, model, nx, ny, nz):
     case = random_case(model, nx, ny, nz,
                        seed={"d2q9_SRT": 11, "d2q9_SRT_drag": 12, "d3q19": 13, "d3q19_lid": 14, "d2q9": 15,
                              "d2q9_drag": 16, "d3q27": 18, "d3q27_cumulant": 19, "d2q9_autosym": 20,
-                             "d2q9_autosym2": 21}[model])
+                             "d2q9_autosym2": 21, "d3q27_cumulant_SMAG": 22}[model])
     lat = case.build("f64")
     orc = oracle_for_case(oracle_mod, case, "f64")
     for n in (1, 6, 13):
@@ -227,7 +229,7 @@ def test_reduced_storage(oracle_mod, model, nx, ny, nz, storage):
     lat = case.build(storage)
     orc = oracle_for_case(oracle_mod, case, storage)
     shifted = storage.endswith("s")
-    w = lattice_weights(lat.Q, cumulant=model == "d3q27_cumulant")
+    w = lattice_weights(lat.Q, cumulant=model.startswith("d3q27_cumulant"))
     for step in range(3):
         a, b = lat.get_f(), orc.get_f()
         stored = b - (w[:, None, None, None] if shifted else 0)

@@ -1165,3 +1165,29 @@ def test_autosym_models_contract(O):
         assert m.group("SETTINGZONE") == (0xF800, 11)
         d = tclb_amd.Model("d2q9")
         assert m.settings == d.settings and m.global_names == d.global_names
+
+
+def test_cumulant_smag_build(O):
+    """d3q27_cumulant_SMAG: with Smag = 0 it is the plain model; otherwise
+    the deviatoric second cumulants relax by 1 - w0 with w0 = 1/(tau0 +
+    tau_t), tau_t = (sqrt(tau0^2 + 18 Smag^2 q) - tau0)/2 and q the norm of
+    the second cumulants' deviation from 1/3 (off-diagonal twice), taken
+    from cumulants computed independently of the oracle."""
+    rng = np.random.default_rng(330)
+    f = _feqc(1.01, [0.02, -0.01, 0.015]) * (1 + 0.08 * rng.standard_normal(27))
+    nu, smag = 0.02, 0.3
+    base, _ = O.cum_node(f, 0x80, [nu, 0.01, 0.0, 0, 0, 0, 1.0])
+    zero, _ = O.cum_node(f, 0x80, [nu, 0.01, 0.0, 0, 0, 0, 1.0, 0.0], smag=True)
+    assert np.array_equal(base, zero)
+    out, _ = O.cum_node(f, 0x80, [nu, 0.01, 0.0, 0, 0, 0, 1.0, smag], smag=True)
+    ords = [(2, 0, 0), (0, 2, 0), (0, 0, 2), (1, 1, 0), (1, 0, 1), (0, 1, 1)]
+    c0, c1 = _cumulants(f, ords), _cumulants(out, ords)
+    q = np.sqrt(sum((c0[p] - 1 / 3) ** 2 for p in ords[:3]) + 2 * sum(c0[p] ** 2 for p in ords[3:]))
+    tau0 = 3 * nu + 0.5
+    w0 = 1 / (tau0 + 0.5 * (np.sqrt(tau0 ** 2 + 18 * smag ** 2 * q) - tau0))
+    assert abs(w0 - 1 / tau0) > 1e-3
+    assert abs((c1[(2, 0, 0)] - c1[(0, 2, 0)]) - (1 - w0) * (c0[(2, 0, 0)] - c0[(0, 2, 0)])) < 1e-14
+    assert abs(c1[(1, 1, 0)] - (1 - w0) * c0[(1, 1, 0)]) < 1e-14
+    buf, _ = O.cum_node(f, 0x80 | (1 << 3), [nu, 0.01, 0.0, 0, 0, 0, 1.0, smag], smag=True)
+    plain_buf, _ = O.cum_node(f, 0x80 | (1 << 3), [nu, 0.01, 0.0, 0, 0, 0, 1.0])
+    assert np.array_equal(buf, plain_buf)       # buffer nodes keep nubuffer
