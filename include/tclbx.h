@@ -36,9 +36,9 @@ const char* tclbx_last_error(void);
 int tclbx_version(void);
 
 /* Models compiled in: "d2q9_SRT", "d2q9_SRT_drag", "d3q19", "d3q19_lid", "d2q9",
- * "d2q9_drag", "d3q27", "d3q27_cumulant" (and its SMAG build
- * "d3q27_cumulant_SMAG"; the AVG and IB builds are not compiled),
- * "d2q9_autosym", "d2q9_autosym2" (d2q9's autosym builds).
+ * "d2q9_drag", "d3q27", "d3q27_cumulant" (and its builds "d3q27_cumulant_SMAG",
+ * "d3q27_cumulant_IB", "d3q27_cumulant_IB_SMAG"; the AVG build is not
+ * compiled), "d2q9_autosym", "d2q9_autosym2" (d2q9's autosym builds).
  * Replaces the per-model binaries CLB/<model>/main (src/makefile.main.Rt:331-349). */
 int tclbx_model_count(void);
 const char* tclbx_model_name(int i);
@@ -87,6 +87,23 @@ int tclbx_set_zone_setting(tclbx_lattice* h, int id, int zone, double value);
 int tclbx_flag_overwrite(tclbx_lattice* h, const uint16_t* flags, int dx, int dy, int dz, int nx, int ny,
                          int nz);
 int tclbx_get_flags(tclbx_lattice* h, int dx, int dy, int dz, int nx, int ny, int nz, uint16_t* out);
+
+/* Lattice::CutsOverwrite (src/Lattice.cu.Rt:1056-1072): cut distances of
+ * the 26 links of each node of the box, cuts[d*(nx*ny*nz) + node], d =
+ * direction - 1 in the d3q27 order (0, 1, -1 per axis, x fastest), as
+ * r * TCLBX_CUT_MAX for a crossing at the fraction r of the link or
+ * TCLBX_NO_CUT.  The first call gives the lattice its cuts, zeroed
+ * (LatticeContainer::ActivateCuts, src/LatticeContainer.inc.cpp.Rt:174-185);
+ * without a call every link reads NO_CUT.  The IB builds of d3q27_cumulant
+ * interpolate on them. */
+#define TCLBX_NO_CUT 65535
+#define TCLBX_CUT_MAX 65000
+int tclbx_cuts_overwrite(tclbx_lattice* h, const uint16_t* cuts, int dx, int dy, int dz, int nx, int ny,
+                         int nz);
+/* The cuts of a box, laid out as above (NO_CUT everywhere when the lattice
+ * has none).  The reference reads them only into a VTI file of the
+ * geometry (src/Geometry.cpp.Rt:1200-1245); this is that read-back. */
+int tclbx_get_cuts(tclbx_lattice* h, int dx, int dy, int dz, int nx, int ny, int nz, uint16_t* out);
 
 /* Lattice::Init (src/Lattice.cu.Rt:799-821): Init action, Iter = 0,
  * Globals cleared. */
@@ -241,6 +258,10 @@ int tclbx_case_units(const tclbx_case* c, const char* text, double* value);
 /* Geometry::load of every <Geometry> element (src/Geometry.cpp.Rt:1140-1184)
  * into flags[n], n = nx*ny*nz, x fastest.  Host only. */
 int tclbx_case_rasterize(tclbx_case* c, uint16_t* flags, long long n);
+/* Geometry::Q after tclbx_case_rasterize: the cut distances a side="surface"
+ * STL draws, n = 26*nx*ny*nz in the layout of tclbx_cuts_overwrite;
+ * *present = 0 (and nothing written) when the geometry has none. */
+int tclbx_case_cuts(tclbx_case* c, uint16_t* cuts, long long n, int* present);
 /* Setting zones (Geometry::SettingZones): count, and the name of zone k. */
 int tclbx_case_zone_count(const tclbx_case* c);
 const char* tclbx_case_zone_name(const tclbx_case* c, int k);

@@ -256,6 +256,10 @@ NODETYPES["d3q27_cumulant"] = [
     ("Body", "BODY"), ("EPressure", "BOUNDARY"), ("EVelocity", "BOUNDARY"), ("Solid", "BOUNDARY"),
     ("Wall", "BOUNDARY"), ("WPressure", "BOUNDARY"), ("WVelocity", "BOUNDARY"), ("MRT", "COLLISION")]
 NODETYPES["d3q27_cumulant_SMAG"] = NODETYPES["d3q27_cumulant"]
+# the IB builds add IB in group HO_BOUNDARY (Dynamics.R:61-63)
+NODETYPES["d3q27_cumulant_IB"] = NODETYPES["d3q27_cumulant"][:9] + [("IB", "HO_BOUNDARY")] + \
+    NODETYPES["d3q27_cumulant"][9:]
+NODETYPES["d3q27_cumulant_IB_SMAG"] = NODETYPES["d3q27_cumulant_IB"]
 
 
 class Packing:
@@ -401,10 +405,16 @@ def gauss4(A, b):
     return x
 
 
+NO_CUT, CUT_MAX = 65535, 65000    # src/types.h:25-28
+
+
 def stl_cut(P, xyz, d):
-    """calcCut: does the link from node xyz along d cross triangle P
-    (batched: P [B,3,3] float32, xyz and d [B,3])?  NaNs (a link in the
-    triangle's plane) pass every test, as in the reference"""
+    """calcCut (src/Geometry.cpp.Rt:479-504): the cut of the link from node
+    xyz along d by triangle P (batched: P [B,3,3] float32, xyz and d [B,3]),
+    uint16: r3 * CUT_MAX truncated where x + r3 d meets the triangle with
+    0 <= r3 <= 1, NO_CUT where it does not.  NaNs (a link in the triangle's
+    plane) pass every test, as in the reference, whose conversion of the NaN
+    product to cut_t gives 0 on x86-64"""
     B = xyz.shape[0]
     A = np.empty((B, 16))
     for v in range(3):
@@ -415,7 +425,10 @@ def stl_cut(P, xyz, d):
     b = np.zeros((B, 4))
     b[:, 3] = 1.0
     r = gauss4(A, b)
-    return ~((r[:, 0] < 0) | (r[:, 1] < 0) | (r[:, 2] < 0) | (r[:, 3] < 0) | (r[:, 3] > 1))
+    miss = (r[:, 0] < 0) | (r[:, 1] < 0) | (r[:, 2] < 0) | (r[:, 3] < 0) | (r[:, 3] > 1)
+    q = r[:, 3] * CUT_MAX
+    q = np.where(np.isnan(q), 0.0, q)
+    return np.where(miss, NO_CUT, np.trunc(np.where(miss, 0.0, q))).astype(np.uint16)
 
 
 def bspline(x, p, k):
@@ -446,6 +459,7 @@ class Rasterizer:
         self.pk, self.u = Packing(model), units
         self.shape = (nx, ny, nz)
         self.geom = np.zeros((nz, ny, nx), dtype=np.uint16)
+        self.cuts = None        # Geometry::Q: [26, nz, ny, nx] once a surface STL is drawn
         self.zones = {"DefaultZone": 0}
         self.fg = self.mask = 0
         self.mode = "overwrite"
@@ -685,8 +699,12 @@ class Rasterizer:
                     xyz = np.repeat(q.astype(np.float64), 26, axis=0)
                     d = np.tile(_D26, (B, 1))
                     with np.errstate(all="ignore"):
-                        hit = stl_cut(P[np.repeat(t, 26)], xyz, d).reshape(B, 26).any(axis=1)
-                    h = q[hit]
+                        cq = stl_cut(P[np.repeat(t, 26)], xyz, d).reshape(B, 26)
+                    if self.cuts is None:
+                        self.cuts = np.full((26,) + N[::-1], NO_CUT, np.uint16)
+                    for k in range(26):   # the nearest crossing of each link
+                        np.minimum.at(self.cuts[k], (q[:, 2], q[:, 1], q[:, 0]), cq[:, k])
+                    h = q[(cq != NO_CUT).any(axis=1)]
                     sel[h[:, 2], h[:, 1], h[:, 0]] = True
             self.paint(0, 0, 0, sel)
             return
@@ -808,9 +826,10 @@ def _mk(spec, parent):
     return n
 
 
-def read_case(path_or_text, model, is_text=False):
+def read_case(path_or_text, model, is_text=False, cuts=False):
     """Units and the NodeType map of every <Geometry> of a case:
-    returns (flags [nz,ny,nx] uint16, zones dict, Units)."""
+    returns (flags [nz,ny,nx] uint16, zones dict, Units), and with cuts=True
+    also the cut distances [26,nz,ny,nx] of surface STLs (None if none)."""
     root = ET.fromstring(path_or_text) if is_text else ET.parse(path_or_text).getroot()
     config = Node.of(root)
     u = Units()
@@ -825,4 +844,6 @@ def read_case(path_or_text, model, is_text=False):
     R = Rasterizer(model, nx, ny, nz, u)
     for g in geoms:
         R.load(g)
+    if cuts:
+        return R.geom, R.zones, u, R.cuts
     return R.geom, R.zones, u

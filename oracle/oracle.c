@@ -104,6 +104,7 @@ struct orc_lattice {
     void* buf[2];               /* A/B snapshots, storage format */
     int cur;                    /* snapshot holding the current state */
     uint16_t* flags;
+    uint16_t* cuts;             /* [26][N] or NULL: no cuts */
     double params[16];
     double* zone;               /* [nzones][ncols] */
     int nzones, ncols;
@@ -155,7 +156,8 @@ orc_lattice* orc_create(int model, int nx, int ny, int nz, int storage)
     d3q19_sparse_init_f();
     L->model = model; L->nx = nx; L->ny = ny; L->nz = nz; L->storage = storage;
     L->N = (size_t)nx * ny * nz;
-    if (model == ORC_D3Q27 || model == ORC_D3Q27_CUMULANT || model == ORC_D3Q27_CUMULANT_SMAG) {
+    if (model == ORC_D3Q27 || model == ORC_D3Q27_CUMULANT || model == ORC_D3Q27_CUMULANT_SMAG ||
+        model == ORC_D3Q27_CUMULANT_IB || model == ORC_D3Q27_CUMULANT_IB_SMAG) {
         int cum = model != ORC_D3Q27;
         cum_init_d();
         cum_init_f();
@@ -191,7 +193,7 @@ orc_lattice* orc_create(int model, int nx, int ny, int nz, int storage)
 void orc_destroy(orc_lattice* L)
 {
     if (!L) return;
-    free(L->buf[0]); free(L->buf[1]); free(L->flags); free(L->zone); free(L);
+    free(L->buf[0]); free(L->buf[1]); free(L->flags); free(L->cuts); free(L->zone); free(L);
 }
 
 int orc_q(const orc_lattice* L) { return L->Q; }
@@ -220,6 +222,12 @@ void orc_set_flags(orc_lattice* L, const uint16_t* flags)
     memcpy(L->flags, flags, L->N * sizeof(uint16_t));
 }
 
+void orc_set_cuts(orc_lattice* L, const uint16_t* cuts)
+{
+    if (!L->cuts) L->cuts = (uint16_t*)malloc(26 * L->N * sizeof(uint16_t));
+    memcpy(L->cuts, cuts, 26 * L->N * sizeof(uint16_t));
+}
+
 static int is3d(const orc_lattice* L) { return L->model == ORC_D3Q19 || L->model == ORC_D3Q19_LID; }
 static int is_d2q9m(const orc_lattice* L)
 {
@@ -228,10 +236,28 @@ static int is_d2q9m(const orc_lattice* L)
 }
 static int is_autosym(const orc_lattice* L) { return L->model == ORC_D2Q9_AUTOSYM || L->model == ORC_D2Q9_AUTOSYM2; }
 static int is_q27(const orc_lattice* L) { return L->model == ORC_D3Q27; }
-static int is_cum(const orc_lattice* L) { return L->model == ORC_D3Q27_CUMULANT || L->model == ORC_D3Q27_CUMULANT_SMAG; }
+static int is_cum(const orc_lattice* L)
+{
+    return L->model == ORC_D3Q27_CUMULANT || L->model == ORC_D3Q27_CUMULANT_SMAG || L->model == ORC_D3Q27_CUMULANT_IB ||
+           L->model == ORC_D3Q27_CUMULANT_IB_SMAG;
+}
+static int is_ib(const orc_lattice* L) { return L->model == ORC_D3Q27_CUMULANT_IB || L->model == ORC_D3Q27_CUMULANT_IB_SMAG; }
+static int is_smag(const orc_lattice* L)
+{
+    return L->model == ORC_D3Q27_CUMULANT_SMAG || L->model == ORC_D3Q27_CUMULANT_IB_SMAG;
+}
+/* the IB builds add group HO_BOUNDARY (IB = bit 8) before the zone */
 static int zone_shift(const orc_lattice* L)
 {
-    return is3d(L) ? 6 : (is_cum(L) ? 8 : (is_autosym(L) ? 11 : (is_d2q9m(L) || is_q27(L) ? 7 : 4)));
+    return is3d(L) ? 6 : (is_cum(L) ? (is_ib(L) ? 9 : 8) : (is_autosym(L) ? 11 : (is_d2q9m(L) || is_q27(L) ? 7 : 4)));
+}
+/* the cuts of node n when it is an IB node of an IB build that has cuts */
+static const uint16_t* ib_cuts(const orc_lattice* L, size_t n, uint16_t nt, uint16_t* tmp)
+{
+    int d;
+    if (!is_ib(L) || !(nt & 0x100) || !L->cuts) return NULL;
+    for (d = 0; d < 26; d++) tmp[d] = L->cuts[(size_t)d * L->N + n];
+    return tmp;
 }
 
 static const double* zone_row(const orc_lattice* L, uint16_t nt)
@@ -393,10 +419,16 @@ static void step_rows(orc_lattice* L, long double* glob, long r0, long r1)
                 uint16_t nt = L->flags[n];
                 const double* zr = zone_row(L, nt);
                 double fd[27];
+                uint16_t cbuf[26];
+                const uint16_t* cut = ib_cuts(L, n, nt, cbuf);
+                double old[27];
                 pull(L, in, x, y, z, fd);
                 if (is_autosym(L)) autosym_access(L, in, x, y, nt, fd);
+                if (cut)   /* the node's own populations in the snapshot being read */
+                    for (q = 0; q < 27; q++) old[q] = ld(L, in, (size_t)q * L->N + n, q);
                 if (L->storage == ORC_F64) {
-                    if (is_cum(L)) cum_node_d(fd, nt, L->params, zr, glob, L->model == ORC_D3Q27_CUMULANT_SMAG);
+                    if (cut) cum_ib_d(fd, nt, cut, old, zr, glob);
+                    if (is_cum(L)) cum_node_d(fd, nt, L->params, zr, glob, is_smag(L));
                     else if (is_q27(L)) d3q27_node_d(fd, nt, L->params, zr, glob);
                     else if (is3d(L)) d3q19_node_d(fd, nt, L->params, zr, glob, lid, L->flags, x, y, z, L->nx, L->ny, L->nz);
                     else if (is_d2q9m(L)) d2q9m_node_d(fd, nt, L->params, zr, glob, L->model == ORC_D2Q9_DRAG);
@@ -405,7 +437,12 @@ static void step_rows(orc_lattice* L, long double* glob, long r0, long r1)
                 } else {
                     float ff[27];
                     for (q = 0; q < L->Q; q++) ff[q] = (float)fd[q];
-                    if (is_cum(L)) cum_node_f(ff, nt, L->params, zr, glob, L->model == ORC_D3Q27_CUMULANT_SMAG);
+                    if (cut) {
+                        float oldf[27];
+                        for (q = 0; q < 27; q++) oldf[q] = (float)old[q];
+                        cum_ib_f(ff, nt, cut, oldf, zr, glob);
+                    }
+                    if (is_cum(L)) cum_node_f(ff, nt, L->params, zr, glob, is_smag(L));
                     else if (is_q27(L)) d3q27_node_f(ff, nt, L->params, zr, glob);
                     else if (is3d(L)) d3q19_node_f(ff, nt, L->params, zr, glob, lid, L->flags, x, y, z, L->nx, L->ny, L->nz);
                     else if (is_d2q9m(L)) d2q9m_node_f(ff, nt, L->params, zr, glob, L->model == ORC_D2Q9_DRAG);
@@ -502,6 +539,13 @@ void orc_get_quantity(const orc_lattice* L, int which, double* out)
                     else if (which == 2) out[n] = (rho - 1.0) / 3.0;
                     else if (which == 4) out[n] = wall ? 1.0 : 0.0;
                     else if (which == 1) {
+                        uint16_t cbuf[26];
+                        const uint16_t* cut = ib_cuts(L, n, nt, cbuf);
+                        if (cut) {
+                            double old[27];
+                            for (q = 0; q < 27; q++) old[q] = ld(L, L->buf[L->cur], (size_t)q * L->N + n, q);
+                            cum_ib_d(f, nt, cut, old, zone_row(L, nt), NULL);
+                        }
                         cum_boundaries_d(f, nt, zone_row(L, nt), NULL);
                         rho = 0;
                         for (q = 0; q < 27; q++) {
@@ -592,6 +636,14 @@ void orc_cum_node(double f[27], uint16_t nt, const double* params, const double*
     int i;
     cum_init_d();
     cum_node_d(f, nt, params, zone, globals ? g : NULL, smag);
+    if (globals) for (i = 0; i < 10; i++) globals[i] = (double)g[i];
+}
+void orc_cum_ib(double f[27], uint16_t nt, const uint16_t cut[26], const double old[27], const double* zone,
+                double* globals)
+{
+    long double g[10] = {0};
+    int i;
+    cum_ib_d(f, nt, cut, old, zone, globals ? g : NULL);
     if (globals) for (i = 0; i < 10; i++) globals[i] = (double)g[i];
 }
 void orc_d3q27_basis(double M[27 * 27], int order[27], int p[27 * 3])

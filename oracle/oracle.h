@@ -49,7 +49,7 @@ extern "C" {
 
 enum { ORC_D2Q9_SRT = 0, ORC_D2Q9_SRT_DRAG = 1, ORC_D3Q19 = 2, ORC_D3Q19_LID = 3, ORC_D2Q9 = 4,
        ORC_D2Q9_DRAG = 5, ORC_D3Q27 = 6, ORC_D3Q27_CUMULANT = 7, ORC_D2Q9_AUTOSYM = 8, ORC_D2Q9_AUTOSYM2 = 9,
-       ORC_D3Q27_CUMULANT_SMAG = 10 };
+       ORC_D3Q27_CUMULANT_SMAG = 10, ORC_D3Q27_CUMULANT_IB = 11, ORC_D3Q27_CUMULANT_IB_SMAG = 12 };
 enum { ORC_F64 = 0, ORC_F32 = 1, ORC_F32S = 2, ORC_F16S = 3, ORC_BF16S = 4 };
 
 /* d2q9 parameter vector:   [omega, Density]
@@ -81,6 +81,16 @@ int  orc_nglobals(const orc_lattice* L);
 void orc_set_params(orc_lattice* L, const double* p, int n);
 void orc_set_zone_table(orc_lattice* L, const double* table, int nzones, int ncols);
 void orc_set_flags(orc_lattice* L, const uint16_t* flags);
+/* Cut distances of the 26 links of every node, cuts[d*N + n], d = direction
+ * - 1 in the d3q27_cumulant order (Lattice::CutsOverwrite of the whole
+ * lattice, src/Lattice.cu.Rt:1056-1072); without a call every link reads
+ * NO_CUT (src/LatticeAccess.inc.cpp.Rt:279-283). */
+void orc_set_cuts(orc_lattice* L, const uint16_t* cuts);
+/* InterpolatedBounceBack alone on the pulled f of one node (FP64): cut[26]
+ * of its links, old[27] its populations in the snapshot being read, and
+ * Drag/Lateral/Lift added to globals[2], [4], [3] on Body nodes. */
+void orc_cum_ib(double f[27], uint16_t nt, const uint16_t cut[26], const double old[27], const double* zone,
+                double* globals);
 void orc_init(orc_lattice* L);
 /* niter steps; if glob_last, Globals of the last step go to globals_out. */
 void orc_iterate(orc_lattice* L, int niter, int glob_last, double* globals_out);

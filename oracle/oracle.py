@@ -15,11 +15,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 
 D2Q9_SRT, D2Q9_SRT_DRAG, D3Q19, D3Q19_LID, D2Q9, D2Q9_DRAG, D3Q27, D3Q27_CUMULANT = 0, 1, 2, 3, 4, 5, 6, 7
-D2Q9_AUTOSYM, D2Q9_AUTOSYM2, D3Q27_CUMULANT_SMAG = 8, 9, 10
+D2Q9_AUTOSYM, D2Q9_AUTOSYM2, D3Q27_CUMULANT_SMAG, D3Q27_CUMULANT_IB, D3Q27_CUMULANT_IB_SMAG = 8, 9, 10, 11, 12
 F64, F32, F32S, F16S, BF16S = 0, 1, 2, 3, 4
 MODELS = {"d2q9_SRT": D2Q9_SRT, "d2q9_SRT_drag": D2Q9_SRT_DRAG, "d3q19": D3Q19, "d3q19_lid": D3Q19_LID,
           "d2q9": D2Q9, "d2q9_drag": D2Q9_DRAG, "d3q27": D3Q27, "d3q27_cumulant": D3Q27_CUMULANT,
-          "d2q9_autosym": D2Q9_AUTOSYM, "d2q9_autosym2": D2Q9_AUTOSYM2, "d3q27_cumulant_SMAG": D3Q27_CUMULANT_SMAG}
+          "d2q9_autosym": D2Q9_AUTOSYM, "d2q9_autosym2": D2Q9_AUTOSYM2, "d3q27_cumulant_SMAG": D3Q27_CUMULANT_SMAG,
+          "d3q27_cumulant_IB": D3Q27_CUMULANT_IB, "d3q27_cumulant_IB_SMAG": D3Q27_CUMULANT_IB_SMAG}
 STORAGE = {"f64": F64, "f32": F32, "f32s": F32S, "f16s": F16S, "bf16s": BF16S}
 
 _lib = None
@@ -44,6 +45,8 @@ def lib():
         L.orc_set_params.argtypes = [vp, dp, ip]
         L.orc_set_zone_table.argtypes = [vp, dp, ip, ip]
         L.orc_set_flags.argtypes = [vp, ctypes.POINTER(ctypes.c_uint16)]
+        L.orc_set_cuts.argtypes = [vp, ctypes.POINTER(ctypes.c_uint16)]
+        L.orc_cum_ib.argtypes = [dp, ctypes.c_uint16, ctypes.POINTER(ctypes.c_uint16), dp, dp, dp]
         L.orc_init.argtypes = [vp]
         L.orc_iterate.argtypes = [vp, ip, ip, dp]
         L.orc_get_f.argtypes = [vp, dp]
@@ -105,6 +108,11 @@ class OracleLattice:
         f = np.ascontiguousarray(flags, dtype=np.uint16).reshape(-1)
         assert f.size == self.nx * self.ny * self.nz
         lib().orc_set_flags(self.h, f.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)))
+
+    def set_cuts(self, cuts):
+        c = np.ascontiguousarray(cuts, dtype=np.uint16).reshape(-1)
+        assert c.size == 26 * self.nx * self.ny * self.nz
+        lib().orc_set_cuts(self.h, c.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)))
 
     def init(self):
         lib().orc_init(self.h)
@@ -190,6 +198,21 @@ def cum_node(f, nt, params, zone=None, smag=False):
     return a, g
 
 
+def cum_ib(f, nt, cut, old, zone=None):
+    """InterpolatedBounceBack of the IB builds on one node: f pulled, cut[26]
+    its links' cut distances, old[27] its populations in the read snapshot;
+    returns (f, Globals[10])"""
+    a = np.array(f, dtype=np.float64)
+    c = np.ascontiguousarray(cut, dtype=np.uint16)
+    o = np.ascontiguousarray(old, dtype=np.float64)
+    z = np.zeros(32)
+    if zone is not None:
+        z[: len(zone)] = zone
+    g = np.zeros(10)
+    lib().orc_cum_ib(_dp(a), nt, c.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)), _dp(o), _dp(z), _dp(g))
+    return a, g
+
+
 def d3q27_basis():
     """(M [direction, moment], order [moment], powers [moment, axis])"""
     M = np.empty(27 * 27)
@@ -250,9 +273,9 @@ def lattice_for_case(case, storage="f64"):
         L.set_params([vals[n] for n in D2Q9_PARAMS])
     elif case.model == "d3q27":
         L.set_params([vals[n] for n in D3Q27_PARAMS])
-    elif case.model == "d3q27_cumulant":
+    elif case.model in ("d3q27_cumulant", "d3q27_cumulant_IB"):
         L.set_params([vals[n] for n in CUMULANT_PARAMS])
-    elif case.model == "d3q27_cumulant_SMAG":
+    elif case.model in ("d3q27_cumulant_SMAG", "d3q27_cumulant_IB_SMAG"):
         L.set_params([vals[n] for n in CUMULANT_SMAG_PARAMS])
     else:
         L.set_params([vals["omega"], vals["Density"]])
@@ -268,6 +291,8 @@ def lattice_for_case(case, storage="f64"):
             table[z, c] = v
     L.set_zone_table(table)
     L.set_flags(case.flags)
+    if getattr(case, "cuts", None) is not None:
+        L.set_cuts(case.cuts)
     L.init()
     if case.init_f is not None:
         L.set_f(case.init_f)

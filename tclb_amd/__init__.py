@@ -29,7 +29,8 @@ SYMBOLS = [
     "tclbx_setting_id", "tclbx_zone_setting_id", "tclbx_global_id", "tclbx_global_name",
     "tclbx_setting_name", "tclbx_zone_setting_name", "tclbx_quantity_name", "tclbx_quantity_id",
     "tclbx_nodetype", "tclbx_nodetype_group", "tclbx_set_setting", "tclbx_get_setting",
-    "tclbx_set_zone_setting", "tclbx_flag_overwrite", "tclbx_get_flags", "tclbx_init",
+    "tclbx_set_zone_setting", "tclbx_flag_overwrite", "tclbx_get_flags", "tclbx_cuts_overwrite",
+    "tclbx_get_cuts", "tclbx_init",
     "tclbx_iterate", "tclbx_get_globals", "tclbx_get_iteration", "tclbx_get_quantity",
     "tclbx_get_density", "tclbx_set_density", "tclbx_save", "tclbx_load", "tclbx_get_timing",
     "tclbx_bytes_per_node", "tclbx_set_block", "tclbx_set_graph_steps", "tclbx_copy_bandwidth",
@@ -43,7 +44,7 @@ SYMBOLS = [
     "tclbx_server_create", "tclbx_server_destroy", "tclbx_server_submit", "tclbx_server_poll",
     "tclbx_server_wait_all", "tclbx_server_stats", "tclbx_server_set_tuning",
     "tclbx_case_load", "tclbx_case_load_string", "tclbx_case_destroy", "tclbx_case_size", "tclbx_case_units",
-    "tclbx_case_rasterize", "tclbx_case_zone_count", "tclbx_case_zone_name", "tclbx_case_set_output_root",
+    "tclbx_case_rasterize", "tclbx_case_cuts", "tclbx_case_zone_count", "tclbx_case_zone_name", "tclbx_case_set_output_root",
     "tclbx_case_run", "tclbx_case_lattice", "tclbx_case_iteration", "tclbx_case_message_count",
     "tclbx_case_message",
 ]
@@ -88,6 +89,8 @@ def lib():
     L.tclbx_set_zone_setting.argtypes = [vp, c_int, c_int, c_double]
     L.tclbx_flag_overwrite.argtypes = [vp, u16p] + [c_int] * 6
     L.tclbx_get_flags.argtypes = [vp] + [c_int] * 6 + [u16p]
+    L.tclbx_cuts_overwrite.argtypes = [vp, u16p] + [c_int] * 6
+    L.tclbx_get_cuts.argtypes = [vp] + [c_int] * 6 + [u16p]
     L.tclbx_init.argtypes = [vp]
     L.tclbx_iterate.argtypes = [vp, c_int, c_int]
     L.tclbx_get_globals.argtypes = [vp, dp, c_int]
@@ -139,6 +142,7 @@ def lib():
     L.tclbx_case_size.argtypes = [vp, ip, ip, ip]
     L.tclbx_case_units.argtypes = [vp, c_char_p, dp]
     L.tclbx_case_rasterize.argtypes = [vp, u16p, ll]
+    L.tclbx_case_cuts.argtypes = [vp, u16p, ll, ip]
     L.tclbx_case_zone_count.argtypes = [vp]
     L.tclbx_case_zone_name.argtypes = [vp, c_int]
     L.tclbx_case_zone_name.restype = c_char_p
@@ -309,6 +313,26 @@ class Lattice:
         assert f.size == nx * ny * nz
         _check(lib().tclbx_flag_overwrite(self._h, f.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)),
                                           dx, dy, dz, nx, ny, nz))
+
+    def set_cuts(self, cuts, region=None):
+        """Lattice::CutsOverwrite: uint16 [26, nz, ny, nx] cut distances of
+        the 26 links (direction - 1 in the d3q27 order), NO_CUT = 65535."""
+        c = np.ascontiguousarray(cuts, dtype=np.uint16)
+        if region is None:
+            region = (0, 0, 0, self.nx, self.ny, self.nz)
+        dx, dy, dz, nx, ny, nz = region
+        assert c.size == 26 * nx * ny * nz
+        _check(lib().tclbx_cuts_overwrite(self._h, c.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)),
+                                          dx, dy, dz, nx, ny, nz))
+
+    def get_cuts(self, region=None):
+        if region is None:
+            region = (0, 0, 0, self.nx, self.ny, self.nz)
+        dx, dy, dz, nx, ny, nz = region
+        out = np.zeros((26, nz, ny, nx), dtype=np.uint16)
+        _check(lib().tclbx_get_cuts(self._h, dx, dy, dz, nx, ny, nz,
+                                    out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16))))
+        return out
 
     def get_flags(self, region=None):
         if region is None:
@@ -668,6 +692,15 @@ class Case:
         out = np.zeros(self.shape, dtype=np.uint16)
         _check(lib().tclbx_case_rasterize(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)), out.size))
         return out
+
+    def cuts(self):
+        """Cut distances a side="surface" STL drew, uint16 [26, nz, ny, nx],
+        or None when the geometry has none (call rasterize() first)."""
+        out = np.zeros((26,) + tuple(self.shape), dtype=np.uint16)
+        present = ctypes.c_int(0)
+        _check(lib().tclbx_case_cuts(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)), out.size,
+                                     ctypes.byref(present)))
+        return out if present.value else None
 
     @property
     def zones(self):

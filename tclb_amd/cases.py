@@ -63,6 +63,7 @@ class Case:
     zonal: list = field(default_factory=list)      # [(name, zone, value)]
     init_f: np.ndarray = None              # optional [Q, nz, ny, nx] float64
     steps: int = 1
+    cuts: np.ndarray = None                # optional uint16 [26, nz, ny, nx] cut distances
 
     @property
     def nodes(self):
@@ -76,6 +77,8 @@ class Case:
 
     def apply(self, lat):
         lat.set_flags(self.flags)
+        if self.cuts is not None:
+            lat.set_cuts(self.cuts)
         for n, v in self.settings:
             lat.set_setting(n, v)
         for n, z, v in self.zonal:

@@ -193,6 +193,17 @@ int tclbx_get_flags(tclbx_lattice* h, int dx, int dy, int dz, int nx, int ny, in
     return guard([&]() { h->lat->GetFlags(Region{dx, dy, dz, nx, ny, nz}, out); return TCLBX_OK; });
 }
 
+int tclbx_cuts_overwrite(tclbx_lattice* h, const uint16_t* cuts, int dx, int dy, int dz, int nx, int ny, int nz) {
+    CHECK_H(h);
+    if (!cuts) return fail("null cuts");
+    return guard([&]() { h->lat->CutsOverwrite(cuts, Region{dx, dy, dz, nx, ny, nz}); return TCLBX_OK; });
+}
+int tclbx_get_cuts(tclbx_lattice* h, int dx, int dy, int dz, int nx, int ny, int nz, uint16_t* out) {
+    CHECK_H(h);
+    if (!out) return fail("null output");
+    return guard([&]() { h->lat->GetCuts(Region{dx, dy, dz, nx, ny, nz}, out); return TCLBX_OK; });
+}
+
 int tclbx_init(tclbx_lattice* h) {
     CHECK_H(h);
     return guard([&]() { h->lat->Init(); return TCLBX_OK; });
@@ -637,6 +648,20 @@ int tclbx_case_rasterize(tclbx_case* c, uint16_t* flags, long long n) {
         c->s->rasterize();
         const auto& g = c->s->geometry->flags();
         std::copy(g.begin(), g.end(), flags);
+        return TCLBX_OK;
+    });
+}
+
+int tclbx_case_cuts(tclbx_case* c, uint16_t* cuts, long long n, int* present) {
+    CHECK_C(c);
+    return guard([&]() {
+        const long long N = 26LL * c->s->nx * c->s->ny * c->s->nz;
+        if (!present) return fail("null present flag");
+        const auto& q = c->s->geometry->cuts();
+        *present = q.empty() ? 0 : 1;
+        if (q.empty()) return TCLBX_OK;
+        if (!cuts || n != N) return fail("cuts buffer must hold 26*nx*ny*nz = " + std::to_string(N) + " values");
+        std::copy(q.begin(), q.end(), cuts);
         return TCLBX_OK;
     });
 }

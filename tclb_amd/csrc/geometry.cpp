@@ -430,19 +430,25 @@ void gauss_solve(double* A, double* b, double* x, int n) {
     }
 }
 
-// calcCut (src/Geometry.cpp.Rt:479-504): does the link from (x,y,z) along
-// d cross the triangle?  Solves x + r3 d = r0 p1 + r1 p2 + r2 p3 with
-// r0 + r1 + r2 = 1; a cut when every r >= 0 and r3 <= 1.  A singular
-// system (a link in the triangle's plane) gives NaNs, which pass every
-// test as they do there, so the link counts as cut.
-bool cut(const StlTri& t, double x, double y, double z, int dx, int dy, int dz) {
+// calcCut (src/Geometry.cpp.Rt:479-504): where does the link from (x,y,z)
+// along d cross the triangle?  Solves x + r3 d = r0 p1 + r1 p2 + r2 p3 with
+// r0 + r1 + r2 = 1; a cut when every r >= 0 and r3 <= 1, at
+// r3 * CUT_MAX (truncated to cut_t), else NO_CUT.  A singular system (a
+// link in the triangle's plane) gives NaNs, which pass every test as they do
+// there; the reference's conversion of NaN * CUT_MAX to cut_t (a 32-bit
+// truncating conversion on x86-64, then the low 16 bits) gives 0, so such a
+// link is cut at 0.
+uint16_t calc_cut(const StlTri& t, double x, double y, double z, int dx, int dy, int dz) {
     double m[16], b[4] = {0, 0, 0, 1}, r[4] = {0, 0, 0, 0};
     m[0] = x - t.p1[0]; m[1] = y - t.p1[1]; m[2] = z - t.p1[2]; m[3] = 1;
     m[4] = x - t.p2[0]; m[5] = y - t.p2[1]; m[6] = z - t.p2[2]; m[7] = 1;
     m[8] = x - t.p3[0]; m[9] = y - t.p3[1]; m[10] = z - t.p3[2]; m[11] = 1;
     m[12] = dx; m[13] = dy; m[14] = dz; m[15] = 0;
     gauss_solve(m, b, r, 4);
-    return !(r[0] < 0 || r[1] < 0 || r[2] < 0 || r[3] < 0 || r[3] > 1);
+    if (r[0] < 0 || r[1] < 0 || r[2] < 0 || r[3] < 0 || r[3] > 1) return NO_CUT;
+    const double q = r[3] * CUT_MAX;
+    if (std::isnan(q)) return 0;
+    return (uint16_t)(int)q;
 }
 
 }  // namespace
@@ -453,7 +459,7 @@ bool cut(const StlTri& t, double x, double y, double z, int dx, int dy, int dz) 
 // vertices.  side="in"/"out": a node is drawn when a ray along ray_axis from
 // the region's start crosses the mesh an odd (in) or even (out) number of
 // times before it; side="surface": when any of its 26 links crosses a
-// triangle.
+// triangle, and each link keeps the nearest crossing as its cut distance.
 void Geometry::load_stl(const GeomRegion& reg, const XmlNode& n) {
     int inside_out = 0, axis = 1;
     if (!n.has("file")) fail("No 'file' attribute in 'STL' element in xml conf");
@@ -549,8 +555,16 @@ void Geometry::load_stl(const GeomRegion& reg, const XmlNode& n) {
                 for (int z = mn[2]; z <= mx[2]; z++)
                     for (int y = mn[1]; y <= mx[1]; y++) {
                         if (!region_.in(x, y, z)) continue;
-                        for (int d = 1; d < 27; d++)
-                            if (cut(t, x, y, z, d27[d][0], d27[d][1], d27[d][2])) dot(x, y, z);
+                        // ActivateCuts: the first surface allocates the cuts, all NO_CUT
+                        if (cuts_.empty()) cuts_.assign(26 * (size_t)region_.size(), NO_CUT);
+                        const size_t k = (size_t)(x - region_.dx) +
+                                         (size_t)region_.nx * ((size_t)(y - region_.dy) + (size_t)region_.ny * (z - region_.dz));
+                        for (int d = 1; d < 27; d++) {
+                            const uint16_t q = calc_cut(t, x, y, z, d27[d][0], d27[d][1], d27[d][2]);
+                            uint16_t& c = cuts_[(size_t)(d - 1) * (size_t)region_.size() + k];
+                            if (q < c) c = q;  // the nearest crossing along the link
+                            if (q != NO_CUT) dot(x, y, z);
+                        }
                     }
             continue;
         }

@@ -27,6 +27,11 @@
 namespace tclbx {
 
 // lbRegion (src/Region.h:4-42): origin and size
+// cut distances (src/types.h:25-28): cut_t = uint16_t, a crossing at the
+// fraction r of a link is r * CUT_MAX, NO_CUT marks an uncut link
+constexpr uint16_t NO_CUT = 65535;
+constexpr int CUT_MAX = 65000;
+
 struct GeomRegion {
     int dx = 0, dy = 0, dz = 0, nx = 1, ny = 1, nz = 1;
     bool in(int x, int y, int z) const {
@@ -50,6 +55,10 @@ public:
     // element
     void load(XmlNode& geom);
     const std::vector<uint16_t>& flags() const { return geom_; }
+    // cut distances (Geometry::Q, src/Geometry.h:29): empty until a surface
+    // STL is drawn, then cuts[d][node] for the 26 links d = direction - 1 in
+    // the d3q27 order, NO_CUT where no triangle crosses the link
+    const std::vector<uint16_t>& cuts() const { return cuts_; }
     // setting zones by name -> number, DefaultZone = 0, in order of first use
     const std::map<std::string, int>& zones() const { return zones_; }
     const GeomRegion& region() const { return region_; }
@@ -78,6 +87,7 @@ private:
     const Units& units_;
     GeomRegion region_, total_;
     std::vector<uint16_t> geom_;
+    std::vector<uint16_t> cuts_;
     std::map<std::string, int> zones_;
     XmlNode* fg_xml_ = nullptr;
     uint32_t fg_ = 0, fg_mask_ = 0;

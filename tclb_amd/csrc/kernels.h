@@ -12,8 +12,8 @@ namespace tclbx {
 // e, eps, q, pi and m moment groups, then gamma1 = 1 - omega (nu group),
 // then (d3q19_lid) MovingWallVelocityX/Y/Z].  d2q9 (MRT): [S2, S3, S4,
 // GravitationX, GravitationY].  d3q27: [omega, Smag, ForceX, ForceY, ForceZ].
-// d3q27_cumulant: [nu, nubuffer, GalileanCorrection, ForceX, ForceY, ForceZ,
-// Omega].
+// d3q27_cumulant and its builds: [nu, nubuffer, GalileanCorrection, ForceX,
+// ForceY, ForceZ, Omega, Smag].
 constexpr int MAX_PAR = 16;
 constexpr int MAX_GLOBALS = 32;
 
@@ -21,6 +21,7 @@ struct StepArgs {
     const void* fin;       // current snapshot: Q planes of `pitch` elements
     void* fout;            // next snapshot
     const uint16_t* flags; // NodeType per node, x fastest
+    const uint16_t* cuts;  // cut distances [26][node] (IB builds), or null: no cuts
     const double* zone;    // zonal settings [zone][zcols]
     double* gpart;         // per-block partial Globals [nglob][nblocks][2] (sum, compensation)
     long long pitch;       // elements per density plane

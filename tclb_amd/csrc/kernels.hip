@@ -109,7 +109,8 @@ template <> struct Lat<MODEL_D3Q27> {
 // (2 bits), BODY, BOUNDARY (4 bits from bit 3), COLLISION (bit 7), zone
 // from bit 8; Globals Density, Flux, Drag, Lift, Lateral, Mass, X/Y/Z
 // Momentum, Objective
-template <int MODEL> constexpr bool is_cum = MODEL == MODEL_D3Q27_CUMULANT || MODEL == MODEL_D3Q27_CUMULANT_SMAG;
+template <int MODEL> constexpr bool is_cum = MODEL == MODEL_D3Q27_CUMULANT || MODEL == MODEL_D3Q27_CUMULANT_SMAG ||
+                                            MODEL == MODEL_D3Q27_CUMULANT_IB || MODEL == MODEL_D3Q27_CUMULANT_IB_SMAG;
 template <> struct Lat<MODEL_D3Q27_CUMULANT> {
     static constexpr int Q = 27, G = 10, SUMG = 10, ZSHIFT = 8;
     __host__ __device__ static constexpr int e1(int p) { return p == 0 ? 0 : (p == 1 ? 1 : -1); }
@@ -130,6 +131,15 @@ template <> struct Lat<MODEL_D3Q27_CUMULANT> {
     }
 };
 template <> struct Lat<MODEL_D3Q27_CUMULANT_SMAG> : Lat<MODEL_D3Q27_CUMULANT> {};
+// the IB builds: NodeType IB is group HO_BOUNDARY at bit 8, zone from bit 9
+template <> struct Lat<MODEL_D3Q27_CUMULANT_IB> : Lat<MODEL_D3Q27_CUMULANT> {
+    static constexpr int ZSHIFT = 9;
+};
+template <> struct Lat<MODEL_D3Q27_CUMULANT_IB_SMAG> : Lat<MODEL_D3Q27_CUMULANT_IB> {};
+template <int MODEL> constexpr bool is_cum_ib =
+    MODEL == MODEL_D3Q27_CUMULANT_IB || MODEL == MODEL_D3Q27_CUMULANT_IB_SMAG;
+template <int MODEL> constexpr bool is_cum_smag =
+    MODEL == MODEL_D3Q27_CUMULANT_SMAG || MODEL == MODEL_D3Q27_CUMULANT_IB_SMAG;
 
 int kernel_nglobals(int model) {
     switch (model) {
@@ -142,6 +152,8 @@ int kernel_nglobals(int model) {
     case MODEL_D3Q27: return Lat<MODEL_D3Q27>::G;
     case MODEL_D3Q27_CUMULANT: return Lat<MODEL_D3Q27_CUMULANT>::G;
     case MODEL_D3Q27_CUMULANT_SMAG: return Lat<MODEL_D3Q27_CUMULANT_SMAG>::G;
+    case MODEL_D3Q27_CUMULANT_IB: return Lat<MODEL_D3Q27_CUMULANT_IB>::G;
+    case MODEL_D3Q27_CUMULANT_IB_SMAG: return Lat<MODEL_D3Q27_CUMULANT_IB_SMAG>::G;
     case MODEL_D2Q9_AUTOSYM: return Lat<MODEL_D2Q9_AUTOSYM>::G;
     case MODEL_D2Q9_AUTOSYM2: return Lat<MODEL_D2Q9_AUTOSYM2>::G;
     }
@@ -1290,10 +1302,46 @@ __device__ __forceinline__ void cum_boundaries(R* f, unsigned nt, const double* 
     }
 }
 
+// InterpolatedBounceBack of the IB builds (Dynamics.c.Rt:180-222), run
+// first in RunBoundaries on nodes of NodeType IB.  For each direction
+// i = 1..26 in order whose link to x + e_i is cut at the fraction
+// q/CUT_MAX (cut[(i-1)*N] != NO_CUT), with p = 2q/CUT_MAX and fa = f_i
+// of the node itself in the snapshot being read (its post-collision value):
+//   p > 1:  f_opp(i) = fa/p + old_opp(i) (p-1)/p
+//   p <= 1: f_opp(i) = fa p + f_i (1-p)
+// (Bouzidi's linear interpolation); a later direction reads what an earlier
+// one wrote.  F = sum of (f_opp(i) - fa) e_i is added to Drag, Lateral and
+// Lift on Body nodes.  old(i) loads f_i of the node from the read snapshot.
+template <class R, bool GLOB, class OLD>
+__device__ __forceinline__ void cum_ib(R* f, unsigned nt, const uint16_t* cut, size_t N, OLD old,
+                                       const double* zr, double* g) {
+    using L = Lat<MODEL_D3Q27_CUMULANT>;
+    R F[3] = {0, 0, 0};
+#pragma unroll
+    for (int i = 1; i < 27; i++) {
+        const unsigned q = cut[(size_t)(i - 1) * N];
+        if (q != 65535u) {  // NO_CUT
+            const R p = (R)((double)q * (2.0 / 65000.0));  // CUT_MAX = 65000
+            const R fa = old(i);
+            const int b = L::mirror(i, 3);
+            if (p > R(1)) f[b] = fa / p + old(b) * (p - R(1)) / p;
+            else f[b] = fa * p + f[i] * (R(1) - p);
+            const R d = f[b] - fa;
+            if (L::ex(i)) F[0] += L::ex(i) > 0 ? d : -d;
+            if (L::ey(i)) F[1] += L::ey(i) > 0 ? d : -d;
+            if (L::ez(i)) F[2] += L::ez(i) > 0 ? d : -d;
+        }
+    }
+    if (GLOB && (nt & 0x4u)) {  // Body
+        g[2] += (double)F[0]; g[9] += zr[5];   // Drag
+        g[4] += (double)F[1]; g[9] += zr[7];   // Lateral
+        g[3] += (double)F[2]; g[9] += zr[6];   // Lift
+    }
+}
+
 // SMAG: the d3q27_cumulant_SMAG build (par[7] = Smag)
 template <class R, bool GLOB, bool SMAG = false>
 __device__ __forceinline__ void cum_node(R* f, unsigned nt, const double* par, const double* zr, double* g) {
-    using L = Lat<MODEL_D3Q27_CUMULANT>;
     cum_boundaries<R, GLOB>(f, nt, zr, g);
     if ((nt & 0x80u) == 0) return;  // MRT
     const R Fx = (R)par[3], Fy = (R)par[4], Fz = (R)par[5];
@@ -1546,7 +1594,13 @@ __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, 
         } else if constexpr (MODEL == MODEL_D3Q27) {
             d3q27_node<R, GLOB>(f, nt, a.par, zr, g);
         } else if constexpr (is_cum<MODEL>) {
-            cum_node<R, GLOB, MODEL == MODEL_D3Q27_CUMULANT_SMAG>(f, nt, a.par, zr, g);
+            if constexpr (is_cum_ib<MODEL>) {
+                if ((nt & 0x100u) && a.cuts != nullptr)  // NodeType IB, cuts present
+                    cum_ib<R, GLOB>(f, nt, a.cuts + n, (size_t)nx * ny * nz, [&](int i) {
+                        return S::get(*(const T*)(fin + (size_t)i * pbytes + n * (unsigned)sizeof(T)), (R)L::w(i));
+                    }, zr, g);
+            }
+            cum_node<R, GLOB, is_cum_smag<MODEL>>(f, nt, a.par, zr, g);
         } else {
             d2q9_node<R, MODEL == MODEL_D2Q9_SRT_DRAG, GLOB>(f, nt, (R)a.par[0], (R)a.par[1], zr, g);
         }
@@ -1778,6 +1832,12 @@ __global__ void __launch_bounds__(256) k_quantity(StepArgs a, int qid, int x0, i
         } else if (qid == 1) {
             int zone = (int)(nt >> L::ZSHIFT);
             if (zone >= a.nzones) zone = 0;
+            if constexpr (is_cum_ib<MODEL>) {
+                if ((nt & 0x100u) && a.cuts != nullptr)
+                    cum_ib<double, false>(f, nt, a.cuts + n, (size_t)nx * ny * nz, [&](int i) {
+                        return (double)S::get(fin[(size_t)i * a.pitch + n], (typename S::R)L::w(i));
+                    }, nullptr, nullptr);
+            }
             cum_boundaries<double, false>(f, nt, a.zone + (size_t)zone * a.zcols, nullptr);
             double d = 0, jx = 0, jy = 0, jz = 0;
 #pragma unroll
@@ -1955,7 +2015,9 @@ hipError_t quantity_m(int st, int q, const StepArgs& a, int x0, int y0, int z0, 
 
 double weight(int model, int q) {
     if (model == MODEL_D3Q27) return Lat<MODEL_D3Q27>::w(q);
-    if (model == MODEL_D3Q27_CUMULANT || model == MODEL_D3Q27_CUMULANT_SMAG) return Lat<MODEL_D3Q27_CUMULANT>::w(q);
+    if (model == MODEL_D3Q27_CUMULANT || model == MODEL_D3Q27_CUMULANT_SMAG || model == MODEL_D3Q27_CUMULANT_IB ||
+        model == MODEL_D3Q27_CUMULANT_IB_SMAG)
+        return Lat<MODEL_D3Q27_CUMULANT>::w(q);
     return (model == MODEL_D3Q19 || model == MODEL_D3Q19_LID) ? Lat<MODEL_D3Q19>::w(q) : Lat<MODEL_D2Q9_SRT>::w(q);
 }
 }  // namespace
@@ -1981,6 +2043,8 @@ hipError_t launch_step(int model, int storage, bool globals, const StepArgs& a, 
     case MODEL_D3Q27: return step_m<MODEL_D3Q27>(storage, globals, a, block, s);
     case MODEL_D3Q27_CUMULANT: return step_m<MODEL_D3Q27_CUMULANT>(storage, globals, a, block, s);
     case MODEL_D3Q27_CUMULANT_SMAG: return step_m<MODEL_D3Q27_CUMULANT_SMAG>(storage, globals, a, block, s);
+    case MODEL_D3Q27_CUMULANT_IB: return step_m<MODEL_D3Q27_CUMULANT_IB>(storage, globals, a, block, s);
+    case MODEL_D3Q27_CUMULANT_IB_SMAG: return step_m<MODEL_D3Q27_CUMULANT_IB_SMAG>(storage, globals, a, block, s);
     case MODEL_D2Q9_AUTOSYM: return step_m<MODEL_D2Q9_AUTOSYM>(storage, globals, a, block, s);
     case MODEL_D2Q9_AUTOSYM2: return step_m<MODEL_D2Q9_AUTOSYM2>(storage, globals, a, block, s);
     }
@@ -1998,6 +2062,8 @@ hipError_t launch_init(int model, int storage, const StepArgs& a, int block, hip
     case MODEL_D3Q27: return init_m<MODEL_D3Q27>(storage, a, block, s);
     case MODEL_D3Q27_CUMULANT: return init_m<MODEL_D3Q27_CUMULANT>(storage, a, block, s);
     case MODEL_D3Q27_CUMULANT_SMAG: return init_m<MODEL_D3Q27_CUMULANT_SMAG>(storage, a, block, s);
+    case MODEL_D3Q27_CUMULANT_IB: return init_m<MODEL_D3Q27_CUMULANT_IB>(storage, a, block, s);
+    case MODEL_D3Q27_CUMULANT_IB_SMAG: return init_m<MODEL_D3Q27_CUMULANT_IB_SMAG>(storage, a, block, s);
     case MODEL_D2Q9_AUTOSYM: return init_m<MODEL_D2Q9_AUTOSYM>(storage, a, block, s);
     case MODEL_D2Q9_AUTOSYM2: return init_m<MODEL_D2Q9_AUTOSYM2>(storage, a, block, s);
     }
@@ -2064,6 +2130,11 @@ hipError_t launch_batch_step(int model, int storage, bool globals, const BatchCa
         return batch_m<MODEL_D3Q27_CUMULANT>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
     case MODEL_D3Q27_CUMULANT_SMAG:
         return batch_m<MODEL_D3Q27_CUMULANT_SMAG>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
+    case MODEL_D3Q27_CUMULANT_IB:
+        return batch_m<MODEL_D3Q27_CUMULANT_IB>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
+    case MODEL_D3Q27_CUMULANT_IB_SMAG:
+        return batch_m<MODEL_D3Q27_CUMULANT_IB_SMAG>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block,
+                                                    s);
     case MODEL_D2Q9_AUTOSYM:
         return batch_m<MODEL_D2Q9_AUTOSYM>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
     case MODEL_D2Q9_AUTOSYM2:
@@ -2089,6 +2160,10 @@ hipError_t launch_quantity(int model, int storage, int q, const StepArgs& a, int
         return quantity_m<MODEL_D3Q27_CUMULANT>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
     case MODEL_D3Q27_CUMULANT_SMAG:
         return quantity_m<MODEL_D3Q27_CUMULANT_SMAG>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
+    case MODEL_D3Q27_CUMULANT_IB:
+        return quantity_m<MODEL_D3Q27_CUMULANT_IB>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
+    case MODEL_D3Q27_CUMULANT_IB_SMAG:
+        return quantity_m<MODEL_D3Q27_CUMULANT_IB_SMAG>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
     case MODEL_D2Q9_AUTOSYM:
         return quantity_m<MODEL_D2Q9_AUTOSYM>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
     case MODEL_D2Q9_AUTOSYM2:

@@ -81,6 +81,7 @@ Lattice::~Lattice() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     for (void* p : snaps_) if (p) (void)hipFree(p);
     if (d_flags_) (void)hipFree(d_flags_);
+    if (d_cuts_) (void)hipFree(d_cuts_);
     if (d_zone_) (void)hipFree(d_zone_);
     if (d_gpart_) (void)hipFree(d_gpart_);
     if (d_glob_) (void)hipFree(d_glob_);
@@ -171,12 +172,58 @@ void Lattice::GetFlags(const Region& over, uint16_t* out) {
         }
 }
 
+void Lattice::CutsOverwrite(const uint16_t* cuts, const Region& over) {
+    if (!region_in(over, nx, ny, nz)) throw std::out_of_range("cut region outside the lattice");
+    if (d_cuts_ == nullptr) {
+        hip_check(hipMalloc((void**)&d_cuts_, 26 * N * sizeof(uint16_t)), "hipMalloc(cuts)");
+        hip_check(hipMemset(d_cuts_, 0, 26 * N * sizeof(uint16_t)), "hipMemset(cuts)");
+        params_version_++;  // a captured step loop holds the old (null) pointer
+    }
+    const size_t osize = (size_t)over.nx * over.ny * over.nz;
+    if (osize == N) {  // the whole lattice: one copy
+        hip_check(hipMemcpy(d_cuts_, cuts, 26 * N * sizeof(uint16_t), hipMemcpyHostToDevice), "CutsOverwrite");
+        return;
+    }
+    for (int d = 0; d < 26; d++)
+        for (int z = 0; z < over.nz; z++)
+            for (int y = 0; y < over.ny; y++) {
+                const size_t dst = (size_t)d * N + (size_t)over.dx +
+                                   (size_t)nx * ((size_t)(over.dy + y) + (size_t)ny * (over.dz + z));
+                const size_t src = (size_t)d * osize + (size_t)over.nx * ((size_t)y + (size_t)over.ny * z);
+                hip_check(hipMemcpy(d_cuts_ + dst, cuts + src, over.nx * sizeof(uint16_t), hipMemcpyHostToDevice),
+                          "CutsOverwrite");
+            }
+}
+
+void Lattice::GetCuts(const Region& over, uint16_t* out) {
+    if (!region_in(over, nx, ny, nz)) throw std::out_of_range("cut region outside the lattice");
+    const size_t osize = (size_t)over.nx * over.ny * over.nz;
+    if (d_cuts_ == nullptr) {  // no cuts: every link reads NO_CUT (src/LatticeAccess.inc.cpp.Rt:279-283)
+        std::fill(out, out + 26 * osize, (uint16_t)65535);
+        return;
+    }
+    if (osize == N) {
+        hip_check(hipMemcpy(out, d_cuts_, 26 * N * sizeof(uint16_t), hipMemcpyDeviceToHost), "GetCuts");
+        return;
+    }
+    for (int d = 0; d < 26; d++)
+        for (int z = 0; z < over.nz; z++)
+            for (int y = 0; y < over.ny; y++) {
+                const size_t src = (size_t)d * N + (size_t)over.dx +
+                                   (size_t)nx * ((size_t)(over.dy + y) + (size_t)ny * (over.dz + z));
+                const size_t dst = (size_t)d * osize + (size_t)over.nx * ((size_t)y + (size_t)over.ny * z);
+                hip_check(hipMemcpy(out + dst, d_cuts_ + src, over.nx * sizeof(uint16_t), hipMemcpyDeviceToHost),
+                          "GetCuts");
+            }
+}
+
 StepArgs Lattice::args(int from, int to, bool glob) const {
     StepArgs a;
     std::memset(&a, 0, sizeof(a));
     a.fin = snaps_[from];
     a.fout = snaps_[to];
     a.flags = d_flags_;
+    a.cuts = d_cuts_;
     a.zone = d_zone_;
     a.gpart = glob ? d_gpart_ : nullptr;
     a.pitch = (long long)pitch;
@@ -215,7 +262,8 @@ void model_params(const ModelDef* model, const std::vector<double>& settings, do
         par[12] = S("MovingWallVelocityX");
         par[13] = S("MovingWallVelocityY");
         par[14] = S("MovingWallVelocityZ");
-    } else if (model->id == MODEL_D3Q27_CUMULANT || model->id == MODEL_D3Q27_CUMULANT_SMAG) {
+    } else if (model->id == MODEL_D3Q27_CUMULANT || model->id == MODEL_D3Q27_CUMULANT_SMAG ||
+               model->id == MODEL_D3Q27_CUMULANT_IB || model->id == MODEL_D3Q27_CUMULANT_IB_SMAG) {
         // [nu, nubuffer, GalileanCorrection, ForceX, ForceY, ForceZ, Omega, Smag]
         const char* names[] = {"nu", "nubuffer", "GalileanCorrection", "ForceX", "ForceY", "ForceZ", "Omega", "Smag"};
         for (int i = 0; i < 8; i++) par[i] = S(names[i]);

@@ -38,6 +38,13 @@ public:
 
     void FlagOverwrite(const uint16_t* flags, const Region& over);
     void GetFlags(const Region& over, uint16_t* out);
+    // cut distances of the 26 links of each node, cuts[d][over node] with
+    // d = direction - 1 in the d3q27 order (0, 1, -1 per axis, x fastest):
+    // the first call allocates them zeroed, then copies the region
+    // (src/Lattice.cu.Rt:1056-1072, src/LatticeContainer.inc.cpp.Rt:174-185)
+    void CutsOverwrite(const uint16_t* cuts, const Region& over);
+    void GetCuts(const Region& over, uint16_t* out);
+    bool has_cuts() const { return d_cuts_ != nullptr; }
 
     void Init();                              // src/Lattice.cu.Rt:799-821
     void Iterate(int niter, int iter_type);   // src/Lattice.cu.Rt:900-989
@@ -84,6 +91,7 @@ private:
     void* snaps_[2] = {nullptr, nullptr};
     int snap_ = 0, iter_ = 0;
     uint16_t* d_flags_ = nullptr;
+    uint16_t* d_cuts_ = nullptr;
     double* d_zone_ = nullptr;
     double* d_gpart_ = nullptr;
     double* d_glob_ = nullptr;

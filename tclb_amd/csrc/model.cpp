@@ -279,11 +279,15 @@ ModelDef make_d3q27() {
 // Galilean correction, Zou-He on the W/E and S/N faces, symmetry planes,
 // momentum-exchange drag on Wall nodes marked Body, buffer-layer viscosity.
 // smag: the SMAG build (d3q27_cumulant_SMAG): a Smag setting and the
-// Smagorinsky viscosity of Dynamics.c.Rt:362-374
-ModelDef make_d3q27_cumulant(bool smag = false) {
+// Smagorinsky viscosity of Dynamics.c.Rt:362-374.  ib: the IB build
+// (d3q27_cumulant_IB): NodeType IB, interpolated bounce-back on the cut
+// distances of the lattice (Dynamics.c.Rt:180-222).  Option builds are
+// named <model>_<options in the order of conf.mk's OPT> (src/models.R:54-66).
+ModelDef make_d3q27_cumulant(bool smag = false, bool ib = false) {
     ModelDef m;
-    m.id = smag ? MODEL_D3Q27_CUMULANT_SMAG : MODEL_D3Q27_CUMULANT;
-    m.name = smag ? "d3q27_cumulant_SMAG" : "d3q27_cumulant";
+    m.id = ib ? (smag ? MODEL_D3Q27_CUMULANT_IB_SMAG : MODEL_D3Q27_CUMULANT_IB)
+              : (smag ? MODEL_D3Q27_CUMULANT_SMAG : MODEL_D3Q27_CUMULANT);
+    m.name = std::string("d3q27_cumulant") + (ib ? "_IB" : "") + (smag ? "_SMAG" : "");
     m.Q = 27; m.dim = 3;
     const int e1[3] = {0, 1, -1};
     for (int i = 0; i < 27; i++) {
@@ -306,12 +310,16 @@ ModelDef make_d3q27_cumulant(bool smag = false) {
                                 {"ZMomentum", false, "kgm/s"}};
     finish_globals(m, g);
     m.quantities = {{"P", false, "Pa"}, {"U", true, "m/s"}, {"Solid", false, "1"}};
-    pack_nodetypes(m, {{"Buffer", "BOUNDARY"},    {"WVelocityTurbulent", "BOUNDARY"}, {"NVelocity", "BOUNDARY"},
-                       {"SVelocity", "BOUNDARY"}, {"NPressure", "BOUNDARY"},          {"SPressure", "BOUNDARY"},
-                       {"NSymmetry", "ADDITIONALS"}, {"SSymmetry", "ADDITIONALS"},    {"Body", "BODY"},
-                       {"EPressure", "BOUNDARY"}, {"EVelocity", "BOUNDARY"},          {"Solid", "BOUNDARY"},
-                       {"Wall", "BOUNDARY"},      {"WPressure", "BOUNDARY"},          {"WVelocity", "BOUNDARY"},
-                       {"MRT", "COLLISION"}});
+    std::vector<std::pair<std::string, std::string>> types = {
+        {"Buffer", "BOUNDARY"},       {"WVelocityTurbulent", "BOUNDARY"}, {"NVelocity", "BOUNDARY"},
+        {"SVelocity", "BOUNDARY"},    {"NPressure", "BOUNDARY"},          {"SPressure", "BOUNDARY"},
+        {"NSymmetry", "ADDITIONALS"}, {"SSymmetry", "ADDITIONALS"},       {"Body", "BODY"}};
+    // the IB build adds NodeType IB in its own group HO_BOUNDARY (Dynamics.R:61-63)
+    if (ib) types.push_back({"IB", "HO_BOUNDARY"});
+    for (const char* t : {"EPressure", "EVelocity", "Solid", "Wall", "WPressure", "WVelocity"})
+        types.push_back({t, "BOUNDARY"});
+    types.push_back({"MRT", "COLLISION"});
+    pack_nodetypes(m, types);
     return m;
 }
 
@@ -319,7 +327,8 @@ const std::vector<ModelDef>& registry() {
     static std::vector<ModelDef> r = {make_d2q9(false), make_d2q9(true), make_d3q19(false), make_d3q19(true),
                                       make_d2q9_mrt(false), make_d2q9_mrt(true), make_d3q27(),
                                       make_d3q27_cumulant(), make_d2q9_mrt(false, 1), make_d2q9_mrt(false, 2),
-                                      make_d3q27_cumulant(true)};
+                                      make_d3q27_cumulant(true), make_d3q27_cumulant(false, true),
+                                      make_d3q27_cumulant(true, true)};
     return r;
 }
 

@@ -30,7 +30,9 @@ enum ModelId {
     MODEL_D2Q9_AUTOSYM = 8,
     MODEL_D2Q9_AUTOSYM2 = 9,
     MODEL_D3Q27_CUMULANT_SMAG = 10,
-    MODEL_COUNT = 11
+    MODEL_D3Q27_CUMULANT_IB = 11,
+    MODEL_D3Q27_CUMULANT_IB_SMAG = 12,
+    MODEL_COUNT = 13
 };
 
 // storage format of the distributions in HBM

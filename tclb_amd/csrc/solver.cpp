@@ -164,6 +164,8 @@ struct AcGeometry : Action {
             Region o;
             o.dx = r.dx; o.dy = r.dy; o.dz = r.dz; o.nx = r.nx; o.ny = r.ny; o.nz = r.nz;
             solver->lattice->FlagOverwrite(solver->geometry->flags().data(), o);
+            // Lattice::CutsOverwrite does nothing without cuts (src/Lattice.cu.Rt:1058)
+            if (!solver->geometry->cuts().empty()) solver->lattice->CutsOverwrite(solver->geometry->cuts().data(), o);
         }
         return 0;
     }

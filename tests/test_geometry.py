@@ -27,9 +27,13 @@ def case(model, body, units="", size='nx="10" ny="6"'):
 def flags(model, body, units="", size='nx="10" ny="6"'):
     c, xml = case(model, body, units, size)
     a = c.rasterize()
-    b, z, _ = G.read_case(xml, model, is_text=True)
+    b, z, _, q = G.read_case(xml, model, is_text=True, cuts=True)
     assert np.array_equal(a, b)
     assert c.zones == z
+    ca = c.cuts()     # surface STLs' cut distances, bit for bit
+    assert (ca is None) == (q is None)
+    if q is not None:
+        assert np.array_equal(ca, q)
     return a
 
 
@@ -329,6 +333,43 @@ def test_stl_surface_closed_form(tmp_path):
                         break
     assert want.sum() > 300
     assert np.array_equal(got, want)
+
+
+def test_stl_surface_cuts_closed_form(tmp_path):
+    """The cut distance of each link is where it first meets the mesh: for a
+    box, the entry point of the segment p -> p + d when p is outside, the
+    exit point when p is inside, as r * CUT_MAX (the faces taken where the
+    transform puts the float vertices); links that miss read NO_CUT.  The
+    IB NodeType of d3q27_cumulant_IB marks the same nodes as any flag."""
+    lo, hi = (3.31, 2.73, 2.17), (11.62, 9.48, 8.91)
+    f = write_stl(tmp_path / "box.stl", box_mesh(lo, hi))
+    body = f'<IB><STL file="{f}" side="surface"/></IB>'
+    a = flags("d3q27_cumulant_IB", body, size=SIZE3)      # engine == oracle, flags and cuts
+    c, _ = case("d3q27_cumulant_IB", body, size=SIZE3)
+    c.rasterize()
+    q = c.cuts()
+    P = G.stl_transform(G.stl_read(f), {}, G.Units()).reshape(-1, 3).astype(np.float64)
+    flo, fhi = P.min(0), P.max(0)
+    z, y, x = np.meshgrid(np.arange(12), np.arange(14), np.arange(16), indexing="ij")
+    p = np.stack([x, y, z], -1).astype(np.float64)
+    inside = ((p > flo) & (p < fhi)).all(-1)
+    want = np.full((26,) + x.shape, 65535.0)
+    for k, d in enumerate(G._D26):
+        with np.errstate(divide="ignore", invalid="ignore"):
+            t0 = np.where(d != 0, (flo - p) / np.where(d != 0, d, 1), -np.inf)
+            t1 = np.where(d != 0, (fhi - p) / np.where(d != 0, d, 1), np.inf)
+        ok_axis = (d != 0) | ((p > flo) & (p < fhi))
+        tin = np.minimum(t0, t1).max(-1)
+        tout = np.maximum(t0, t1).min(-1)
+        meets = ok_axis.all(-1) & (tin <= tout)
+        r = np.where(inside, tout, tin)
+        hit = meets & (r >= 0) & (r <= 1)
+        want[k][hit] = r[hit] * 65000
+    miss = want == 65535
+    assert np.array_equal(q == 65535, miss)
+    assert np.abs(q[~miss] - want[~miss]).max() <= 1.0
+    assert (~miss).sum() > 1000
+    assert np.array_equal((a & 0x100) != 0, (~miss).any(0))
 
 
 def icosphere(c, r, levels=2):

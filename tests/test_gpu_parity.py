@@ -142,6 +142,13 @@ def random_case(model, nx, ny, nz, seed, steps=10):
             zonal += [("MovingWallFxInObj", 3, 0.5), ("MovingWallTorqueInObj", -1, 0.25)]
     n = nx * ny * nz
     flags = np.array(kinds, dtype=np.uint16)[rng.integers(0, len(kinds), n)]
+    cuts = None
+    if "_IB" in model:   # IB on 40 % of the nodes; a third of the links cut, the edge values often
+        flags |= (m.flag("IB") * (rng.random(n) < 0.4)).astype(np.uint16)
+        cuts = np.where(rng.random((26, n)) < 0.35, rng.integers(0, 65001, (26, n)), 65535)
+        edge = rng.random((26, n)) < 0.05
+        cuts[edge] = np.array([0, 1, 32499, 32500, 32501, 65000])[rng.integers(0, 6, int(edge.sum()))]
+        cuts = cuts.astype(np.uint16).reshape(26, nz, ny, nx)
     if model.startswith("d3q19"):
         flags |= (rng.integers(0, 4, n) * ((flags & 0x20) > 0)).astype(np.uint16)  # slices on MRT nodes
     if model.startswith("d2q9_autosym"):   # SYMX / SYMY planes on a third of the nodes each
@@ -151,12 +158,13 @@ def random_case(model, nx, ny, nz, seed, steps=10):
     flags = flags.reshape(nz, ny, nx)
     Q = len(w)
     f = w[:, None, None, None] * (1 + 0.02 * rng.standard_normal((Q, nz, ny, nx)))
-    return cases.Case("random", model, nx, ny, nz, flags, settings, zonal, init_f=f, steps=steps)
+    return cases.Case("random", model, nx, ny, nz, flags, settings, zonal, init_f=f, steps=steps, cuts=cuts)
 
 
 MODELS = [("d2q9_SRT", 24, 10, 1), ("d2q9_SRT_drag", 37, 9, 1), ("d3q19", 13, 7, 6), ("d3q19_lid", 11, 9, 7),
           ("d2q9", 29, 11, 1), ("d2q9_drag", 23, 13, 1), ("d3q27", 12, 7, 5), ("d3q27_cumulant", 11, 8, 5),
-          ("d2q9_autosym", 17, 11, 1), ("d2q9_autosym2", 19, 12, 1), ("d3q27_cumulant_SMAG", 9, 8, 6)]
+          ("d2q9_autosym", 17, 11, 1), ("d2q9_autosym2", 19, 12, 1), ("d3q27_cumulant_SMAG", 9, 8, 6),
+          ("d3q27_cumulant_IB", 10, 9, 5), ("d3q27_cumulant_IB_SMAG", 9, 7, 6)]
 
 
 @pytest.mark.parametrize("model,nx,ny,nz", MODELS)
@@ -165,7 +173,8 @@ def test_random_case_f64(oracle_mod, model, nx, ny, nz):
     case = random_case(model, nx, ny, nz,
                        seed={"d2q9_SRT": 11, "d2q9_SRT_drag": 12, "d3q19": 13, "d3q19_lid": 14, "d2q9": 15,
                              "d2q9_drag": 16, "d3q27": 18, "d3q27_cumulant": 19, "d2q9_autosym": 20,
-                             "d2q9_autosym2": 21, "d3q27_cumulant_SMAG": 22}[model])
+                             "d2q9_autosym2": 21, "d3q27_cumulant_SMAG": 22, "d3q27_cumulant_IB": 23,
+                             "d3q27_cumulant_IB_SMAG": 24}[model])
     lat = case.build("f64")
     orc = oracle_for_case(oracle_mod, case, "f64")
     for n in (1, 6, 13):
@@ -209,6 +218,25 @@ def test_flags_roundtrip(model, nx, ny, nz):
     want[:, 1:3, 2:5] = patch
     assert np.array_equal(lat.get_flags(), want)
     assert np.array_equal(lat.get_flags(region=(2, 1, 0, 3, 2, nz)), patch)
+
+
+def test_cuts_roundtrip():
+    """Lattice::CutsOverwrite: no cuts read as NO_CUT; the first overwrite
+    of a box gives the lattice zeroed cuts around it; a whole-lattice
+    overwrite replaces them all."""
+    nx, ny, nz = 9, 7, 5
+    lat = tclb_amd.Lattice("d3q27_cumulant_IB", nx, ny, nz)
+    assert (lat.get_cuts() == 65535).all()
+    rng = np.random.default_rng(31)
+    box = rng.integers(0, 65536, (26, nz, 3, 4)).astype(np.uint16)
+    lat.set_cuts(box, region=(2, 1, 0, 4, 3, nz))
+    want = np.zeros((26, nz, ny, nx), np.uint16)
+    want[:, :, 1:4, 2:6] = box
+    assert np.array_equal(lat.get_cuts(), want)
+    assert np.array_equal(lat.get_cuts(region=(2, 1, 0, 4, 3, nz)), box)
+    full = rng.integers(0, 65536, (26, nz, ny, nx)).astype(np.uint16)
+    lat.set_cuts(full)
+    assert np.array_equal(lat.get_cuts(), full)
 
 
 # unit roundoff and smallest subnormal of the stored format

@@ -1191,3 +1191,134 @@ def test_cumulant_smag_build(O):
     buf, _ = O.cum_node(f, 0x80 | (1 << 3), [nu, 0.01, 0.0, 0, 0, 0, 1.0, smag], smag=True)
     plain_buf, _ = O.cum_node(f, 0x80 | (1 << 3), [nu, 0.01, 0.0, 0, 0, 0, 1.0])
     assert np.array_equal(buf, plain_buf)       # buffer nodes keep nubuffer
+
+
+# ------------------------------------------------ interpolated bounce-back ---
+NO_CUT, CUT_MAX = 65535, 65000
+
+
+def _opp(i):
+    return int(np.nonzero((EC == -EC[i]).all(1))[0][0])
+
+
+def test_ib_halfway_cut_is_plain_bounce_back(O):
+    """A wall halfway along every link (cut = CUT_MAX/2, p = 1) gives the
+    halfway bounce-back: each incoming population is the node's own
+    post-collision population of the opposite direction, whatever streamed
+    in, and the rest population stays."""
+    rng = np.random.default_rng(340)
+    f = _feqc(1.0, [0.02, -0.01, 0.01]) * (1 + 0.05 * rng.standard_normal(27))
+    old = _feqc(1.0, [0.01, 0.02, -0.01]) * (1 + 0.05 * rng.standard_normal(27))
+    out, g = O.cum_ib(f, 0x100 | 0x4, np.full(26, CUT_MAX // 2), old)
+    assert out[0] == f[0]
+    for i in range(1, 27):
+        assert out[_opp(i)] == old[i]
+    assert not g.any()          # (f_b - fa) vanishes link by link
+
+
+@pytest.mark.parametrize("cut", [0, 1, 9750, 32499, 32500, 32501, 48100, 64999, 65000])
+def test_ib_single_link_is_bouzidi(O, cut):
+    """One cut link: the population coming back along it follows Bouzidi,
+    Firdaouss & Lallemand (Phys. Fluids 13, 3452 (2001)), linear
+    interpolation with the wall at the fraction q of the link:
+      q < 1/2:  f_opp(x) = 2q f*_i(x) + (1-2q) f*_i(x - e_i)
+      q >= 1/2: f_opp(x) = f*_i(x)/(2q) + (2q-1)/(2q) f*_opp(x)
+    (f* post-collision; f*_i(x - e_i) is what streamed in), every other
+    population is untouched, and on a Body node (f_opp - f*_i) e_i goes to
+    Drag, Lateral and Lift."""
+    rng = np.random.default_rng(341 + cut)
+    q = cut / CUT_MAX
+    zone = [0, 0, 0, 0, 0, 0.5, 0.25, 0.125, 0, 0, 0, 0]
+    for i in range(1, 27):
+        f = _feqc(1.0, [0.02, -0.01, 0.01]) * (1 + 0.05 * rng.standard_normal(27))
+        old = _feqc(1.0, [0.01, 0.02, -0.01]) * (1 + 0.05 * rng.standard_normal(27))
+        c = np.full(26, NO_CUT)
+        c[i - 1] = cut
+        b = _opp(i)
+        want = 2 * q * old[i] + (1 - 2 * q) * f[i] if q < 0.5 else old[i] / (2 * q) + (2 * q - 1) / (2 * q) * old[b]
+        out, g = O.cum_ib(f, 0x100 | 0x4, c, old, zone)
+        assert abs(out[b] - want) <= 4e-16, (i, out[b], want)
+        assert np.array_equal(np.delete(out, b), np.delete(f, b))
+        F = (out[b] - old[i]) * EC[i]
+        assert np.allclose(g[[2, 4, 3]], F, rtol=0, atol=1e-17)
+        assert abs(g[9] - 0.875) < 1e-15
+        _, g = O.cum_ib(f, 0x100, c, old, zone)     # not a Body node: no force
+        assert not g.any()
+
+
+def test_ib_links_are_taken_in_order(O):
+    """With both directions of a line cut, the lower-numbered direction is
+    taken first (Dynamics.c.Rt:190-214 loops over the densities in order),
+    so the later one interpolates on what the first wrote when its wall is
+    nearer than half a link."""
+    rng = np.random.default_rng(350)
+    f = _feqc(1.0, [0.02, -0.01, 0.01]) * (1 + 0.05 * rng.standard_normal(27))
+    old = _feqc(1.0, [0.01, 0.02, -0.01]) * (1 + 0.05 * rng.standard_normal(27))
+    cuts = rng.integers(0, CUT_MAX + 1, 26)
+    cuts[rng.random(26) < 0.3] = NO_CUT
+    out, _ = O.cum_ib(f, 0x100, cuts, old)
+    want = f.copy()
+    for i in range(1, 27):            # the reference's loop, written out
+        if cuts[i - 1] == NO_CUT:
+            continue
+        p = cuts[i - 1] * (2.0 / CUT_MAX)
+        b = _opp(i)
+        want[b] = old[i] / p + old[b] * (p - 1) / p if p > 1 else old[i] * p + want[i] * (1 - p)
+    assert np.allclose(out, want, rtol=0, atol=1e-16)
+
+
+def test_ib_poiseuille_walls_off_the_grid(O):
+    """Force-driven flow between two planes at y = 2.3 and 19.6 (between
+    nodes), their cut distances taken from the geometry of the planes: the
+    steady profile is the parabola g/(2 nu) (y - ya)(yb - y) with the walls
+    where they are, to 0.2 % of its peak; the same run with each cut
+    measured from the wrong end of its link (q -> 1 - q) misses by several
+    per cent."""
+    import tclb_amd
+    m = tclb_amd.Model("d3q27_cumulant_IB")
+    ny, ya, yb, g, nu = 24, 2.3, 19.6, 1e-6, 1 / 6
+    cuts = np.full((26, 1, ny, 1), NO_CUT, np.uint16)
+    for y in range(ny):
+        for d in range(1, 27):
+            ey = EC[d][1]
+            r = [(w - y) / ey for w in (ya, yb, ya + ny, yb - ny)] if ey else []
+            r = [v for v in r if 0 <= v <= 1]
+            if r:
+                cuts[d - 1, 0, y, 0] = int(min(r) * CUT_MAX)
+    flags = np.full((1, ny, 1), m.flag("MRT"), np.uint16)
+    flags[0, :, 0] |= np.where((cuts[:, 0, :, 0] != NO_CUT).any(0), m.flag("IB"), 0).astype(np.uint16)
+    assert (flags & m.flag("IB")).astype(bool).sum() == 4     # the nodes either side of each wall
+
+    def peak_error(c):
+        L = O.OracleLattice("d3q27_cumulant_IB", 1, ny, 1)
+        L.set_params([nu, 0.01, 1.0, g, 0, 0, 1.0])
+        L.set_zone_table(np.zeros((1, 32)))
+        L.set_flags(flags)
+        L.set_cuts(c)
+        L.init()
+        L.iterate(10000, False)
+        ux = L.quantity("U").reshape(-1, 3)[:, 0]
+        y = np.arange(ny)
+        want = g / (2 * nu) * (y - ya) * (yb - y)
+        inside = (y > ya) & (y < yb)
+        return np.abs(ux - want)[inside].max() / want.max()
+
+    assert peak_error(cuts) < 2e-3
+    wrong = np.where(cuts != NO_CUT, CUT_MAX - cuts, cuts).astype(np.uint16)
+    assert peak_error(wrong) > 3e-2
+
+
+def test_ib_contract():
+    """d3q27_cumulant_IB and _IB_SMAG (src/models.R:54-66 names option
+    builds model_<options>): NodeType IB in its own group HO_BOUNDARY, after
+    COLLISION in the alphabetical packing, so the zone moves up one bit; the
+    rest of the model is the plain (or SMAG) one."""
+    import tclb_amd
+    for name, base in (("d3q27_cumulant_IB", "d3q27_cumulant"), ("d3q27_cumulant_IB_SMAG", "d3q27_cumulant_SMAG")):
+        m, b = tclb_amd.Model(name), tclb_amd.Model(base)
+        assert m.nodetype("IB") == (0x100, 0x100)
+        assert m.group("HO_BOUNDARY") == (0x100, 8)
+        assert m.group("SETTINGZONE")[1] == 9 and b.group("SETTINGZONE")[1] == 8
+        for t in ("Wall", "MRT", "Body", "NSymmetry", "WVelocity", "Buffer"):
+            assert m.nodetype(t) == b.nodetype(t)
+        assert m.settings == b.settings and m.global_names == b.global_names
