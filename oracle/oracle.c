@@ -105,6 +105,8 @@ struct orc_lattice {
     int cur;                    /* snapshot holding the current state */
     uint16_t* flags;
     uint16_t* cuts;             /* [26][N] or NULL: no cuts */
+    double* avg;                /* AVG builds: [13][N] averaged densities, else NULL */
+    int iter, reset_iter;       /* iterations since Init; of the last average reset */
     double params[16];
     double* zone;               /* [nzones][ncols] */
     int nzones, ncols;
@@ -157,7 +159,7 @@ orc_lattice* orc_create(int model, int nx, int ny, int nz, int storage)
     L->model = model; L->nx = nx; L->ny = ny; L->nz = nz; L->storage = storage;
     L->N = (size_t)nx * ny * nz;
     if (model == ORC_D3Q27 || model == ORC_D3Q27_CUMULANT || model == ORC_D3Q27_CUMULANT_SMAG ||
-        model == ORC_D3Q27_CUMULANT_IB || model == ORC_D3Q27_CUMULANT_IB_SMAG) {
+        model >= ORC_D3Q27_CUMULANT_IB) {
         int cum = model != ORC_D3Q27;
         cum_init_d();
         cum_init_f();
@@ -182,6 +184,7 @@ orc_lattice* orc_create(int model, int nx, int ny, int nz, int storage)
             L->w[q] = d2q9_w[q];
         }
     }
+    if (model >= ORC_D3Q27_CUMULANT_AVG) L->avg = (double*)calloc(13 * L->N, sizeof(double));
     L->buf[0] = calloc(L->N * L->Q, esize(storage));
     L->buf[1] = calloc(L->N * L->Q, esize(storage));
     L->flags = (uint16_t*)calloc(L->N, sizeof(uint16_t));
@@ -193,7 +196,7 @@ orc_lattice* orc_create(int model, int nx, int ny, int nz, int storage)
 void orc_destroy(orc_lattice* L)
 {
     if (!L) return;
-    free(L->buf[0]); free(L->buf[1]); free(L->flags); free(L->cuts); free(L->zone); free(L);
+    free(L->buf[0]); free(L->buf[1]); free(L->flags); free(L->cuts); free(L->avg); free(L->zone); free(L);
 }
 
 int orc_q(const orc_lattice* L) { return L->Q; }
@@ -238,13 +241,17 @@ static int is_autosym(const orc_lattice* L) { return L->model == ORC_D2Q9_AUTOSY
 static int is_q27(const orc_lattice* L) { return L->model == ORC_D3Q27; }
 static int is_cum(const orc_lattice* L)
 {
-    return L->model == ORC_D3Q27_CUMULANT || L->model == ORC_D3Q27_CUMULANT_SMAG || L->model == ORC_D3Q27_CUMULANT_IB ||
-           L->model == ORC_D3Q27_CUMULANT_IB_SMAG;
+    return L->model == ORC_D3Q27_CUMULANT || L->model == ORC_D3Q27_CUMULANT_SMAG || L->model >= ORC_D3Q27_CUMULANT_IB;
 }
-static int is_ib(const orc_lattice* L) { return L->model == ORC_D3Q27_CUMULANT_IB || L->model == ORC_D3Q27_CUMULANT_IB_SMAG; }
+static int is_ib(const orc_lattice* L)
+{
+    return L->model == ORC_D3Q27_CUMULANT_IB || L->model == ORC_D3Q27_CUMULANT_IB_SMAG ||
+           L->model == ORC_D3Q27_CUMULANT_AVG_IB || L->model == ORC_D3Q27_CUMULANT_AVG_IB_SMAG;
+}
 static int is_smag(const orc_lattice* L)
 {
-    return L->model == ORC_D3Q27_CUMULANT_SMAG || L->model == ORC_D3Q27_CUMULANT_IB_SMAG;
+    return L->model == ORC_D3Q27_CUMULANT_SMAG || L->model == ORC_D3Q27_CUMULANT_IB_SMAG ||
+           L->model == ORC_D3Q27_CUMULANT_AVG_SMAG || L->model == ORC_D3Q27_CUMULANT_AVG_IB_SMAG;
 }
 /* the IB builds add group HO_BOUNDARY (IB = bit 8) before the zone */
 static int zone_shift(const orc_lattice* L)
@@ -271,6 +278,10 @@ void orc_init(orc_lattice* L)
 {
     size_t n;
     int q;
+    /* Lattice::Init (src/Lattice.cu.Rt:799-821): iter = 0, reset_iter kept;
+     * the averaged densities start at their default 0 (src/conf.R:70) */
+    L->iter = 0;
+    if (L->avg) memset(L->avg, 0, 13 * L->N * sizeof(double));
     for (n = 0; n < L->N; n++) {
         const double* zr = zone_row(L, L->flags[n]);
         if (is_cum(L)) {
@@ -428,7 +439,7 @@ static void step_rows(orc_lattice* L, long double* glob, long r0, long r1)
                     for (q = 0; q < 27; q++) old[q] = ld(L, in, (size_t)q * L->N + n, q);
                 if (L->storage == ORC_F64) {
                     if (cut) cum_ib_d(fd, nt, cut, old, zr, glob);
-                    if (is_cum(L)) cum_node_d(fd, nt, L->params, zr, glob, is_smag(L));
+                    if (is_cum(L)) cum_node_d(fd, nt, L->params, zr, glob, is_smag(L), L->avg ? L->avg + n : NULL, L->N);
                     else if (is_q27(L)) d3q27_node_d(fd, nt, L->params, zr, glob);
                     else if (is3d(L)) d3q19_node_d(fd, nt, L->params, zr, glob, lid, L->flags, x, y, z, L->nx, L->ny, L->nz);
                     else if (is_d2q9m(L)) d2q9m_node_d(fd, nt, L->params, zr, glob, L->model == ORC_D2Q9_DRAG);
@@ -442,7 +453,7 @@ static void step_rows(orc_lattice* L, long double* glob, long r0, long r1)
                         for (q = 0; q < 27; q++) oldf[q] = (float)old[q];
                         cum_ib_f(ff, nt, cut, oldf, zr, glob);
                     }
-                    if (is_cum(L)) cum_node_f(ff, nt, L->params, zr, glob, is_smag(L));
+                    if (is_cum(L)) cum_node_f(ff, nt, L->params, zr, glob, is_smag(L), L->avg ? L->avg + n : NULL, L->N);
                     else if (is_q27(L)) d3q27_node_f(ff, nt, L->params, zr, glob);
                     else if (is3d(L)) d3q19_node_f(ff, nt, L->params, zr, glob, lid, L->flags, x, y, z, L->nx, L->ny, L->nz);
                     else if (is_d2q9m(L)) d2q9m_node_f(ff, nt, L->params, zr, glob, L->model == ORC_D2Q9_DRAG);
@@ -493,6 +504,7 @@ void orc_iterate(orc_lattice* L, int niter, int glob_last, double* globals_out)
         int last = glob_last && i == niter - 1;
         if (last) for (g = 0; g < 32; g++) glob[g] = 0.0L;
         step(L, last ? glob : NULL);
+        L->iter++;
     }
     if (glob_last && niter > 0 && globals_out)
         for (g = 0; g < L->nglob; g++) globals_out[g] = (double)glob[g];
@@ -518,6 +530,60 @@ void orc_set_f(orc_lattice* L, const double* in)
         }
 }
 
+/* Lattice::resetAverage (src/Lattice.cu.Rt:1360-1365) */
+void orc_reset_average(orc_lattice* L)
+{
+    L->reset_iter = L->iter;
+    if (L->avg) memset(L->avg, 0, 13 * L->N * sizeof(double));
+}
+
+int orc_get_averages(const orc_lattice* L, double* out)
+{
+    if (!L->avg) return -1;
+    memcpy(out, L->avg, 13 * L->N * sizeof(double));
+    return 0;
+}
+
+/* the AVG quantities (models/flow/d3q27_cumulant/Dynamics.c.Rt:62-99) of
+ * node (x,y,z): which = 5 KinE, 6 ReStr, 7 Dissipation, 8 avgU, 9 varU,
+ * 10 averageP, over average_iter = iter - reset_iter iterations */
+static void avg_quantity(const orc_lattice* L, int which, int x, int y, int z, double* out)
+{
+    const double* A = L->avg;
+    const int it = L->iter - L->reset_iter;
+    const double N = (double)it;
+#define AV(k, xx, yy, zz) A[(size_t)(k) * L->N + (size_t)(((xx) + L->nx) % L->nx) + \
+                             (size_t)L->nx * ((size_t)(((yy) + L->ny) % L->ny) + (size_t)L->ny * (((zz) + L->nz) % L->nz))]
+    double mu[3], var[3];
+    int k;
+    for (k = 0; k < 3; k++) mu[k] = AV(10 + k, x, y, z) / N;
+    for (k = 0; k < 3; k++) var[k] = AV(1 + k, x, y, z) / N - mu[k] * mu[k];
+    switch (which) {
+    case 5: out[0] = 0.5 * (var[0] + var[1] + var[2]); break;
+    case 6:
+        out[0] = AV(6, x, y, z) / N - mu[1] * mu[2];
+        out[1] = AV(5, x, y, z) / N - mu[0] * mu[2];
+        out[2] = AV(4, x, y, z) / N - mu[0] * mu[1];
+        break;
+    case 7: {
+        /* 4*average_iter*average_iter is int arithmetic in the reference */
+        const double q4 = (double)(int32_t)(4u * (uint32_t)it * (uint32_t)it);
+        const double nu = L->params[0];
+        double gx = AV(10, x + 1, y, z) - AV(10, x - 1, y, z);
+        double gy = AV(11, x, y + 1, z) - AV(11, x, y - 1, z);
+        double gz = AV(12, x, y, z + 1) - AV(12, x, y, z - 1);
+        double d = nu * (AV(7, x, y, z) / N - gx * gx / q4);
+        d = d + nu * (AV(8, x, y, z) / N - gy * gy / q4);
+        d = d + nu * (AV(9, x, y, z) / N - gz * gz / q4);
+        out[0] = d;
+        break; }
+    case 8: for (k = 0; k < 3; k++) out[k] = mu[k]; break;
+    case 9: for (k = 0; k < 3; k++) out[k] = var[k]; break;
+    default: out[0] = AV(0, x, y, z) / N; break;
+    }
+#undef AV
+}
+
 void orc_get_quantity(const orc_lattice* L, int which, double* out)
 {
     int x, y, z, q;
@@ -529,6 +595,11 @@ void orc_get_quantity(const orc_lattice* L, int which, double* out)
                 double f[27], rho = 0, jx = 0, jy = 0, jz = 0;
                 pull(L, L->buf[L->cur], x, y, z, f);
                 if (is_autosym(L)) autosym_access(L, L->buf[L->cur], x, y, nt, f);
+                if (which >= 5) {
+                    const int vec = which == 6 || which == 8 || which == 9;
+                    avg_quantity(L, which, x, y, z, vec ? out + 3 * n : out + n);
+                    continue;
+                }
                 if (is_cum(L)) {
                     /* getP on the pulled f; getU after RunBoundaries (0 on
                      * Wall, else (J + F/2)/rho); getSolid (1 on Wall),
@@ -635,7 +706,7 @@ void orc_cum_node(double f[27], uint16_t nt, const double* params, const double*
     long double g[10] = {0};
     int i;
     cum_init_d();
-    cum_node_d(f, nt, params, zone, globals ? g : NULL, smag);
+    cum_node_d(f, nt, params, zone, globals ? g : NULL, smag, NULL, 0);
     if (globals) for (i = 0; i < 10; i++) globals[i] = (double)g[i];
 }
 void orc_cum_ib(double f[27], uint16_t nt, const uint16_t cut[26], const double old[27], const double* zone,

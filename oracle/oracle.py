@@ -16,11 +16,17 @@ LIB = os.path.join(HERE, "liboracle.so")
 
 D2Q9_SRT, D2Q9_SRT_DRAG, D3Q19, D3Q19_LID, D2Q9, D2Q9_DRAG, D3Q27, D3Q27_CUMULANT = 0, 1, 2, 3, 4, 5, 6, 7
 D2Q9_AUTOSYM, D2Q9_AUTOSYM2, D3Q27_CUMULANT_SMAG, D3Q27_CUMULANT_IB, D3Q27_CUMULANT_IB_SMAG = 8, 9, 10, 11, 12
+D3Q27_CUMULANT_AVG, D3Q27_CUMULANT_AVG_IB, D3Q27_CUMULANT_AVG_SMAG, D3Q27_CUMULANT_AVG_IB_SMAG = 13, 14, 15, 16
 F64, F32, F32S, F16S, BF16S = 0, 1, 2, 3, 4
 MODELS = {"d2q9_SRT": D2Q9_SRT, "d2q9_SRT_drag": D2Q9_SRT_DRAG, "d3q19": D3Q19, "d3q19_lid": D3Q19_LID,
           "d2q9": D2Q9, "d2q9_drag": D2Q9_DRAG, "d3q27": D3Q27, "d3q27_cumulant": D3Q27_CUMULANT,
           "d2q9_autosym": D2Q9_AUTOSYM, "d2q9_autosym2": D2Q9_AUTOSYM2, "d3q27_cumulant_SMAG": D3Q27_CUMULANT_SMAG,
-          "d3q27_cumulant_IB": D3Q27_CUMULANT_IB, "d3q27_cumulant_IB_SMAG": D3Q27_CUMULANT_IB_SMAG}
+          "d3q27_cumulant_IB": D3Q27_CUMULANT_IB, "d3q27_cumulant_IB_SMAG": D3Q27_CUMULANT_IB_SMAG,
+          "d3q27_cumulant_AVG": D3Q27_CUMULANT_AVG, "d3q27_cumulant_AVG_IB": D3Q27_CUMULANT_AVG_IB,
+          "d3q27_cumulant_AVG_SMAG": D3Q27_CUMULANT_AVG_SMAG,
+          "d3q27_cumulant_AVG_IB_SMAG": D3Q27_CUMULANT_AVG_IB_SMAG}
+AVERAGES = ["avgP", "varUX", "varUY", "varUZ", "varUXUY", "varUXUZ", "varUYUZ", "avgdxu2", "avgdyv2", "avgdzw2",
+            "avgUX", "avgUY", "avgUZ"]
 STORAGE = {"f64": F64, "f32": F32, "f32s": F32S, "f16s": F16S, "bf16s": BF16S}
 
 _lib = None
@@ -46,6 +52,8 @@ def lib():
         L.orc_set_zone_table.argtypes = [vp, dp, ip, ip]
         L.orc_set_flags.argtypes = [vp, ctypes.POINTER(ctypes.c_uint16)]
         L.orc_set_cuts.argtypes = [vp, ctypes.POINTER(ctypes.c_uint16)]
+        L.orc_reset_average.argtypes = [vp]
+        L.orc_get_averages.argtypes = [vp, dp]
         L.orc_cum_ib.argtypes = [dp, ctypes.c_uint16, ctypes.POINTER(ctypes.c_uint16), dp, dp, dp]
         L.orc_init.argtypes = [vp]
         L.orc_iterate.argtypes = [vp, ip, ip, dp]
@@ -132,10 +140,20 @@ class OracleLattice:
         assert a.size == self.Q * self.nx * self.ny * self.nz
         lib().orc_set_f(self.h, _dp(a))
 
+    def reset_average(self):
+        lib().orc_reset_average(self.h)
+
+    def averages(self):
+        """dict name -> [nz, ny, nx] of the AVG builds' averaged densities"""
+        a = np.empty((13,) + self.shape)
+        assert lib().orc_get_averages(self.h, _dp(a)) == 0
+        return dict(zip(AVERAGES, a))
+
     def quantity(self, which):
-        w = {"Rho": 0, "U": 1, "P": 2, "Fd": 3, "Solid": 4}[which] if isinstance(which, str) else which
+        w = {"Rho": 0, "U": 1, "P": 2, "Fd": 3, "Solid": 4, "KinE": 5, "ReStr": 6, "Dissipation": 7, "avgU": 8,
+             "varU": 9, "averageP": 10}[which] if isinstance(which, str) else which
         n = self.nx * self.ny * self.nz
-        vec = w in (1, 3)
+        vec = w in (1, 3, 6, 8, 9)
         out = np.empty(n * (3 if vec else 1), dtype=np.float64)
         lib().orc_get_quantity(self.h, w, _dp(out))
         return out.reshape(self.shape + ((3,) if vec else ()))
@@ -273,10 +291,8 @@ def lattice_for_case(case, storage="f64"):
         L.set_params([vals[n] for n in D2Q9_PARAMS])
     elif case.model == "d3q27":
         L.set_params([vals[n] for n in D3Q27_PARAMS])
-    elif case.model in ("d3q27_cumulant", "d3q27_cumulant_IB"):
-        L.set_params([vals[n] for n in CUMULANT_PARAMS])
-    elif case.model in ("d3q27_cumulant_SMAG", "d3q27_cumulant_IB_SMAG"):
-        L.set_params([vals[n] for n in CUMULANT_SMAG_PARAMS])
+    elif case.model.startswith("d3q27_cumulant"):
+        L.set_params([vals[n] for n in (CUMULANT_SMAG_PARAMS if case.model.endswith("SMAG") else CUMULANT_PARAMS)])
     else:
         L.set_params([vals["omega"], vals["Density"]])
     zshift = m.group("SETTINGZONE")[1]

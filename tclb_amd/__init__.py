@@ -32,7 +32,8 @@ SYMBOLS = [
     "tclbx_set_zone_setting", "tclbx_flag_overwrite", "tclbx_get_flags", "tclbx_cuts_overwrite",
     "tclbx_get_cuts", "tclbx_init",
     "tclbx_iterate", "tclbx_get_globals", "tclbx_get_iteration", "tclbx_get_quantity",
-    "tclbx_get_density", "tclbx_set_density", "tclbx_save", "tclbx_load", "tclbx_get_timing",
+    "tclbx_get_density", "tclbx_set_density", "tclbx_average_name", "tclbx_reset_average", "tclbx_save",
+    "tclbx_load", "tclbx_get_timing",
     "tclbx_bytes_per_node", "tclbx_set_block", "tclbx_set_graph_steps", "tclbx_copy_bandwidth",
     "tclbx_set_rows_per_tile", "tclbx_get_rows_per_tile",
     "tclbx_batch_create", "tclbx_batch_destroy", "tclbx_batch_case_bytes", "tclbx_batch_add_case",
@@ -77,7 +78,8 @@ def lib():
     L.tclbx_model_info.argtypes = [c_char_p, ip, ip, ip, ip, ip]
     for n in ["tclbx_setting_id", "tclbx_zone_setting_id", "tclbx_global_id"]:
         getattr(L, n).argtypes = [c_char_p, c_char_p]
-    for n in ["tclbx_global_name", "tclbx_setting_name", "tclbx_zone_setting_name", "tclbx_quantity_name"]:
+    for n in ["tclbx_global_name", "tclbx_setting_name", "tclbx_zone_setting_name", "tclbx_quantity_name",
+              "tclbx_average_name"]:
         getattr(L, n).argtypes = [c_char_p, c_int]
         getattr(L, n).restype = c_char_p
     L.tclbx_setting_default.argtypes = [c_char_p, c_int, dp]
@@ -97,6 +99,7 @@ def lib():
     L.tclbx_get_iteration.argtypes = [vp, ip]
     L.tclbx_get_quantity.argtypes = [vp, c_int] + [c_int] * 6 + [dp, c_double]
     L.tclbx_get_density.argtypes = [vp, c_int, dp]
+    L.tclbx_reset_average.argtypes = [vp]
     L.tclbx_set_density.argtypes = [vp, c_int, dp]
     L.tclbx_save.argtypes = [vp, c_char_p]
     L.tclbx_load.argtypes = [vp, c_char_p]
@@ -195,6 +198,9 @@ class Model:
         self.zonal_settings = [L.tclbx_zone_setting_name(b, i).decode() for i in range(self.nzonal)]
         self.global_names = [L.tclbx_global_name(b, i).decode() for i in range(self.nglobals)]
         self.quantities = [L.tclbx_quantity_name(b, i).decode() for i in range(self.nquantities)]
+        self.averages = []     # averaged densities (density Q + k)
+        while L.tclbx_average_name(b, len(self.averages)):
+            self.averages.append(L.tclbx_average_name(b, len(self.averages)).decode())
 
     def setting_id(self, name):
         return lib().tclbx_setting_id(self.name.encode(), name.encode())
@@ -388,6 +394,19 @@ class Lattice:
         for q in range(self.Q):
             plane = np.ascontiguousarray(a[q])
             _check(lib().tclbx_set_density(self._h, q, _dp(plane)))
+
+    def get_averages(self):
+        """The averaged densities (AVG builds), dict name -> [nz, ny, nx]."""
+        out = {}
+        for k, n in enumerate(self.spec.averages):
+            a = np.empty(self.shape, dtype=np.float64)
+            _check(lib().tclbx_get_density(self._h, self.Q + k, _dp(a)))
+            out[n] = a
+        return out
+
+    def reset_average(self):
+        """Lattice::resetAverage: the averages to zero, counted from now."""
+        _check(lib().tclbx_reset_average(self._h))
 
     def save(self, path):
         _check(lib().tclbx_save(self._h, str(path).encode()))

@@ -241,6 +241,15 @@ int tclbx_set_density(tclbx_lattice* h, int q, const double* in) {
     if (!in) return fail("null input");
     return guard([&]() { h->lat->SetDensity(q, in); return TCLBX_OK; });
 }
+const char* tclbx_average_name(const char* model, int k) {
+    const ModelDef* m = M(model);
+    if (!m || k < 0 || k >= (int)m->averages.size()) return nullptr;
+    return m->averages[k].c_str();
+}
+int tclbx_reset_average(tclbx_lattice* h) {
+    CHECK_H(h);
+    return guard([&]() { h->lat->resetAverage(); return TCLBX_OK; });
+}
 int tclbx_save(tclbx_lattice* h, const char* path) {
     CHECK_H(h);
     if (!path) return fail("null path");

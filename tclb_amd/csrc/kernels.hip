@@ -110,7 +110,10 @@ template <> struct Lat<MODEL_D3Q27> {
 // from bit 8; Globals Density, Flux, Drag, Lift, Lateral, Mass, X/Y/Z
 // Momentum, Objective
 template <int MODEL> constexpr bool is_cum = MODEL == MODEL_D3Q27_CUMULANT || MODEL == MODEL_D3Q27_CUMULANT_SMAG ||
-                                            MODEL == MODEL_D3Q27_CUMULANT_IB || MODEL == MODEL_D3Q27_CUMULANT_IB_SMAG;
+                                            MODEL == MODEL_D3Q27_CUMULANT_IB || MODEL == MODEL_D3Q27_CUMULANT_IB_SMAG ||
+                                            MODEL == MODEL_D3Q27_CUMULANT_AVG || MODEL == MODEL_D3Q27_CUMULANT_AVG_IB ||
+                                            MODEL == MODEL_D3Q27_CUMULANT_AVG_SMAG ||
+                                            MODEL == MODEL_D3Q27_CUMULANT_AVG_IB_SMAG;
 template <> struct Lat<MODEL_D3Q27_CUMULANT> {
     static constexpr int Q = 27, G = 10, SUMG = 10, ZSHIFT = 8;
     __host__ __device__ static constexpr int e1(int p) { return p == 0 ? 0 : (p == 1 ? 1 : -1); }
@@ -136,10 +139,23 @@ template <> struct Lat<MODEL_D3Q27_CUMULANT_IB> : Lat<MODEL_D3Q27_CUMULANT> {
     static constexpr int ZSHIFT = 9;
 };
 template <> struct Lat<MODEL_D3Q27_CUMULANT_IB_SMAG> : Lat<MODEL_D3Q27_CUMULANT_IB> {};
-template <int MODEL> constexpr bool is_cum_ib =
-    MODEL == MODEL_D3Q27_CUMULANT_IB || MODEL == MODEL_D3Q27_CUMULANT_IB_SMAG;
-template <int MODEL> constexpr bool is_cum_smag =
-    MODEL == MODEL_D3Q27_CUMULANT_SMAG || MODEL == MODEL_D3Q27_CUMULANT_IB_SMAG;
+// the AVG builds keep the NodeType bits of the build without AVG
+template <> struct Lat<MODEL_D3Q27_CUMULANT_AVG> : Lat<MODEL_D3Q27_CUMULANT> {};
+template <> struct Lat<MODEL_D3Q27_CUMULANT_AVG_SMAG> : Lat<MODEL_D3Q27_CUMULANT> {};
+template <> struct Lat<MODEL_D3Q27_CUMULANT_AVG_IB> : Lat<MODEL_D3Q27_CUMULANT_IB> {};
+template <> struct Lat<MODEL_D3Q27_CUMULANT_AVG_IB_SMAG> : Lat<MODEL_D3Q27_CUMULANT_IB> {};
+template <int MODEL> constexpr bool is_cum_ib = MODEL == MODEL_D3Q27_CUMULANT_IB ||
+                                               MODEL == MODEL_D3Q27_CUMULANT_IB_SMAG ||
+                                               MODEL == MODEL_D3Q27_CUMULANT_AVG_IB ||
+                                               MODEL == MODEL_D3Q27_CUMULANT_AVG_IB_SMAG;
+template <int MODEL> constexpr bool is_cum_smag = MODEL == MODEL_D3Q27_CUMULANT_SMAG ||
+                                                 MODEL == MODEL_D3Q27_CUMULANT_IB_SMAG ||
+                                                 MODEL == MODEL_D3Q27_CUMULANT_AVG_SMAG ||
+                                                 MODEL == MODEL_D3Q27_CUMULANT_AVG_IB_SMAG;
+template <int MODEL> constexpr bool is_cum_avg = MODEL == MODEL_D3Q27_CUMULANT_AVG ||
+                                                MODEL == MODEL_D3Q27_CUMULANT_AVG_IB ||
+                                                MODEL == MODEL_D3Q27_CUMULANT_AVG_SMAG ||
+                                                MODEL == MODEL_D3Q27_CUMULANT_AVG_IB_SMAG;
 
 int kernel_nglobals(int model) {
     switch (model) {
@@ -154,6 +170,10 @@ int kernel_nglobals(int model) {
     case MODEL_D3Q27_CUMULANT_SMAG: return Lat<MODEL_D3Q27_CUMULANT_SMAG>::G;
     case MODEL_D3Q27_CUMULANT_IB: return Lat<MODEL_D3Q27_CUMULANT_IB>::G;
     case MODEL_D3Q27_CUMULANT_IB_SMAG: return Lat<MODEL_D3Q27_CUMULANT_IB_SMAG>::G;
+    case MODEL_D3Q27_CUMULANT_AVG: return Lat<MODEL_D3Q27_CUMULANT_AVG>::G;
+    case MODEL_D3Q27_CUMULANT_AVG_IB: return Lat<MODEL_D3Q27_CUMULANT_AVG_IB>::G;
+    case MODEL_D3Q27_CUMULANT_AVG_SMAG: return Lat<MODEL_D3Q27_CUMULANT_AVG_SMAG>::G;
+    case MODEL_D3Q27_CUMULANT_AVG_IB_SMAG: return Lat<MODEL_D3Q27_CUMULANT_AVG_IB_SMAG>::G;
     case MODEL_D2Q9_AUTOSYM: return Lat<MODEL_D2Q9_AUTOSYM>::G;
     case MODEL_D2Q9_AUTOSYM2: return Lat<MODEL_D2Q9_AUTOSYM2>::G;
     }
@@ -1339,9 +1359,16 @@ __device__ __forceinline__ void cum_ib(R* f, unsigned nt, const uint16_t* cut, s
     }
 }
 
-// SMAG: the d3q27_cumulant_SMAG build (par[7] = Smag)
-template <class R, bool GLOB, bool SMAG = false>
-__device__ __forceinline__ void cum_node(R* f, unsigned nt, const double* par, const double* zr, double* g) {
+// SMAG: the d3q27_cumulant_SMAG build (par[7] = Smag).  AVG: the AVG
+// builds; av (null: no averages kept) points at the node's first averaged
+// density, the k-th at av[k*NN], in the order avgP, varUX, varUY, varUZ,
+// varUXUY, varUXUZ, varUYUZ, avgdxu2, avgdyv2, avgdzw2, avgUX, avgUY,
+// avgUZ.  On collision nodes they gain dxu^2, dyv^2, dzw^2 of the collision
+// (Dynamics.c.Rt:410-414) and P, u, u_i u_j of the post-collision
+// populations (getP, getU_: 0 on Wall nodes; Dynamics.c.Rt:507-518).
+template <class R, bool GLOB, bool SMAG = false, bool AVG = false>
+__device__ __forceinline__ void cum_node(R* f, unsigned nt, const double* par, const double* zr, double* g,
+                                         double* av = nullptr, size_t NN = 0) {
     cum_boundaries<R, GLOB>(f, nt, zr, g);
     if ((nt & 0x80u) == 0) return;  // MRT
     const R Fx = (R)par[3], Fy = (R)par[4], Fz = (R)par[5];
@@ -1388,6 +1415,13 @@ __device__ __forceinline__ void cum_node(R* f, unsigned nt, const double* par, c
     const R dxu = -w0 / R(2) * (R(2) * c[2] - c[6] - c[18]) - w1 / R(2) * (c[2] + c[6] + c[18] - R(1));
     const R dyv = dxu + R(3) * w0 / R(2) * (c[2] - c[6]);
     const R dzw = dxu + R(3) * w0 / R(2) * (c[2] - c[18]);
+    if constexpr (AVG) {
+        if (av != nullptr) {
+            av[7 * NN] += (double)(dxu * dxu);
+            av[8 * NN] += (double)(dyv * dyv);
+            av[9 * NN] += (double)(dzw * dzw);
+        }
+    }
     const R gcor1 = R(3) * (R(1) - w0 / R(2)) * (ux * ux * dxu - uy * uy * dyv);
     const R gcor2 = R(3) * (R(1) - w0 / R(2)) * (ux * ux * dxu - uz * uz * dzw);
     const R gcor3 = R(3) * (R(1) - w1 / R(2)) * (ux * ux * dxu + uy * uy * dyv + uz * uz * dzw);
@@ -1414,6 +1448,33 @@ __device__ __forceinline__ void cum_node(R* f, unsigned nt, const double* par, c
     cum_inv(m);
 #pragma unroll
     for (int i = 0; i < 27; i++) f[i] = m[i];
+    if constexpr (AVG) {
+        if (av != nullptr) {
+            using L = Lat<MODEL_D3Q27_CUMULANT>;
+            R d = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+            for (int i = 0; i < 27; i++) {
+                d += f[i];
+                if (L::ex(i)) jx += L::ex(i) > 0 ? f[i] : -f[i];
+                if (L::ey(i)) jy += L::ey(i) > 0 ? f[i] : -f[i];
+                if (L::ez(i)) jz += L::ez(i) > 0 ? f[i] : -f[i];
+            }
+            const bool wall = ((nt >> 3) & 0xFu) == 10u;
+            const R vx = wall ? R(0) : (jx + Fx / R(2)) / d;
+            const R vy = wall ? R(0) : (jy + Fy / R(2)) / d;
+            const R vz = wall ? R(0) : (jz + Fz / R(2)) / d;
+            av[0] += (double)((d - R(1)) / R(3));
+            av[10 * NN] += (double)vx;
+            av[11 * NN] += (double)vy;
+            av[12 * NN] += (double)vz;
+            av[1 * NN] += (double)(vx * vx);
+            av[2 * NN] += (double)(vy * vy);
+            av[3 * NN] += (double)(vz * vz);
+            av[4 * NN] += (double)(vx * vy);
+            av[5 * NN] += (double)(vx * vz);
+            av[6 * NN] += (double)(vy * vz);
+        }
+    }
 }
 
 // SetEquilibrum = Req M^-1 for this lattice's digit order
@@ -1600,7 +1661,11 @@ __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, 
                         return S::get(*(const T*)(fin + (size_t)i * pbytes + n * (unsigned)sizeof(T)), (R)L::w(i));
                     }, zr, g);
             }
-            cum_node<R, GLOB, is_cum_smag<MODEL>>(f, nt, a.par, zr, g);
+            if constexpr (is_cum_avg<MODEL>)
+                cum_node<R, GLOB, is_cum_smag<MODEL>, true>(f, nt, a.par, zr, g, a.avg ? a.avg + n : nullptr,
+                                                            (size_t)nx * ny * nz);
+            else
+                cum_node<R, GLOB, is_cum_smag<MODEL>>(f, nt, a.par, zr, g);
         } else {
             d2q9_node<R, MODEL == MODEL_D2Q9_SRT_DRAG, GLOB>(f, nt, (R)a.par[0], (R)a.par[1], zr, g);
         }
@@ -1847,8 +1912,50 @@ __global__ void __launch_bounds__(256) k_quantity(StepArgs a, int qid, int x0, i
             out[3 * tid] = wall ? 0.0 : (jx + a.par[3] / 2.) / d * scale;
             out[3 * tid + 1] = wall ? 0.0 : (jy + a.par[4] / 2.) / d * scale;
             out[3 * tid + 2] = wall ? 0.0 : (jz + a.par[5] / 2.) / d * scale;
-        } else {
+        } else if (qid == 2 || !is_cum_avg<MODEL>) {
             out[tid] = (wall ? 1.0 : 0.0) * scale;
+        } else if (a.avg == nullptr) {  // a lattice that keeps no averages
+            const bool vec = qid == 4 || qid == 6 || qid == 7;
+            for (int k = 0; k < (vec ? 3 : 1); k++) out[(vec ? 3 : 1) * tid + k] = __builtin_nan("");
+        } else {
+            // the AVG quantities (Dynamics.c.Rt:62-99) over average_iter
+            // iterations: KinE = 3, ReStr = 4, Dissipation = 5, avgU = 6,
+            // varU = 7, averageP = 8
+            const size_t NN = (size_t)nx * ny * nz;
+            auto A = [&](int k, int xx, int yy, int zz) {
+                xx = (xx + nx) % nx; yy = (yy + ny) % ny; zz = (zz + nz) % nz;
+                return a.avg[(size_t)k * NN + (size_t)xx + (size_t)nx * ((size_t)yy + (size_t)ny * zz)];
+            };
+            const double it = a.avg_n;
+            const double mx = A(10, x, y, z) / it, my = A(11, x, y, z) / it, mz = A(12, x, y, z) / it;
+            const double vx = A(1, x, y, z) / it - mx * mx, vy = A(2, x, y, z) / it - my * my,
+                         vz = A(3, x, y, z) / it - mz * mz;
+            if (qid == 3) {
+                out[tid] = 0.5 * (vx + vy + vz) * scale;
+            } else if (qid == 4) {
+                out[3 * tid] = (A(6, x, y, z) / it - my * mz) * scale;
+                out[3 * tid + 1] = (A(5, x, y, z) / it - mx * mz) * scale;
+                out[3 * tid + 2] = (A(4, x, y, z) / it - mx * my) * scale;
+            } else if (qid == 5) {
+                // 4*average_iter*average_iter is int arithmetic there: it
+                // wraps past 2^31 (average_iter > 23170), as it does here
+                const int ni = (int)it;
+                const double q4 = (double)(int)(4u * (unsigned)ni * (unsigned)ni);
+                const double nu = a.par[0];
+                const double gx = A(10, x + 1, y, z) - A(10, x - 1, y, z);
+                const double gy = A(11, x, y + 1, z) - A(11, x, y - 1, z);
+                const double gz = A(12, x, y, z + 1) - A(12, x, y, z - 1);
+                double diss = nu * ((A(7, x, y, z) / it) - gx * gx / q4);
+                diss = diss + nu * ((A(8, x, y, z) / it) - gy * gy / q4);
+                diss = diss + nu * ((A(9, x, y, z) / it) - gz * gz / q4);
+                out[tid] = diss * scale;
+            } else if (qid == 6) {
+                out[3 * tid] = mx * scale; out[3 * tid + 1] = my * scale; out[3 * tid + 2] = mz * scale;
+            } else if (qid == 7) {
+                out[3 * tid] = vx * scale; out[3 * tid + 1] = vy * scale; out[3 * tid + 2] = vz * scale;
+            } else {
+                out[tid] = A(0, x, y, z) / it * scale;
+            }
         }
     } else if constexpr (is_d2q9m<MODEL>) {
         // getRho / getU, Dynamics.c.Rt:46-63: Rho = 0, U = J/rho + G/2 = 1
@@ -2015,8 +2122,7 @@ hipError_t quantity_m(int st, int q, const StepArgs& a, int x0, int y0, int z0, 
 
 double weight(int model, int q) {
     if (model == MODEL_D3Q27) return Lat<MODEL_D3Q27>::w(q);
-    if (model == MODEL_D3Q27_CUMULANT || model == MODEL_D3Q27_CUMULANT_SMAG || model == MODEL_D3Q27_CUMULANT_IB ||
-        model == MODEL_D3Q27_CUMULANT_IB_SMAG)
+    if (model >= MODEL_D3Q27_CUMULANT_IB || model == MODEL_D3Q27_CUMULANT || model == MODEL_D3Q27_CUMULANT_SMAG)
         return Lat<MODEL_D3Q27_CUMULANT>::w(q);
     return (model == MODEL_D3Q19 || model == MODEL_D3Q19_LID) ? Lat<MODEL_D3Q19>::w(q) : Lat<MODEL_D2Q9_SRT>::w(q);
 }
@@ -2045,6 +2151,11 @@ hipError_t launch_step(int model, int storage, bool globals, const StepArgs& a, 
     case MODEL_D3Q27_CUMULANT_SMAG: return step_m<MODEL_D3Q27_CUMULANT_SMAG>(storage, globals, a, block, s);
     case MODEL_D3Q27_CUMULANT_IB: return step_m<MODEL_D3Q27_CUMULANT_IB>(storage, globals, a, block, s);
     case MODEL_D3Q27_CUMULANT_IB_SMAG: return step_m<MODEL_D3Q27_CUMULANT_IB_SMAG>(storage, globals, a, block, s);
+    case MODEL_D3Q27_CUMULANT_AVG: return step_m<MODEL_D3Q27_CUMULANT_AVG>(storage, globals, a, block, s);
+    case MODEL_D3Q27_CUMULANT_AVG_IB: return step_m<MODEL_D3Q27_CUMULANT_AVG_IB>(storage, globals, a, block, s);
+    case MODEL_D3Q27_CUMULANT_AVG_SMAG: return step_m<MODEL_D3Q27_CUMULANT_AVG_SMAG>(storage, globals, a, block, s);
+    case MODEL_D3Q27_CUMULANT_AVG_IB_SMAG:
+        return step_m<MODEL_D3Q27_CUMULANT_AVG_IB_SMAG>(storage, globals, a, block, s);
     case MODEL_D2Q9_AUTOSYM: return step_m<MODEL_D2Q9_AUTOSYM>(storage, globals, a, block, s);
     case MODEL_D2Q9_AUTOSYM2: return step_m<MODEL_D2Q9_AUTOSYM2>(storage, globals, a, block, s);
     }
@@ -2064,6 +2175,11 @@ hipError_t launch_init(int model, int storage, const StepArgs& a, int block, hip
     case MODEL_D3Q27_CUMULANT_SMAG: return init_m<MODEL_D3Q27_CUMULANT_SMAG>(storage, a, block, s);
     case MODEL_D3Q27_CUMULANT_IB: return init_m<MODEL_D3Q27_CUMULANT_IB>(storage, a, block, s);
     case MODEL_D3Q27_CUMULANT_IB_SMAG: return init_m<MODEL_D3Q27_CUMULANT_IB_SMAG>(storage, a, block, s);
+    // the AVG builds initialise like the builds without AVG
+    case MODEL_D3Q27_CUMULANT_AVG: return init_m<MODEL_D3Q27_CUMULANT>(storage, a, block, s);
+    case MODEL_D3Q27_CUMULANT_AVG_SMAG: return init_m<MODEL_D3Q27_CUMULANT_SMAG>(storage, a, block, s);
+    case MODEL_D3Q27_CUMULANT_AVG_IB: return init_m<MODEL_D3Q27_CUMULANT_IB>(storage, a, block, s);
+    case MODEL_D3Q27_CUMULANT_AVG_IB_SMAG: return init_m<MODEL_D3Q27_CUMULANT_IB_SMAG>(storage, a, block, s);
     case MODEL_D2Q9_AUTOSYM: return init_m<MODEL_D2Q9_AUTOSYM>(storage, a, block, s);
     case MODEL_D2Q9_AUTOSYM2: return init_m<MODEL_D2Q9_AUTOSYM2>(storage, a, block, s);
     }
@@ -2135,6 +2251,17 @@ hipError_t launch_batch_step(int model, int storage, bool globals, const BatchCa
     case MODEL_D3Q27_CUMULANT_IB_SMAG:
         return batch_m<MODEL_D3Q27_CUMULANT_IB_SMAG>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block,
                                                     s);
+    case MODEL_D3Q27_CUMULANT_AVG:
+        return batch_m<MODEL_D3Q27_CUMULANT_AVG>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
+    case MODEL_D3Q27_CUMULANT_AVG_IB:
+        return batch_m<MODEL_D3Q27_CUMULANT_AVG_IB>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block,
+                                                   s);
+    case MODEL_D3Q27_CUMULANT_AVG_SMAG:
+        return batch_m<MODEL_D3Q27_CUMULANT_AVG_SMAG>(storage, globals, cases, tiles, ntiles, phase, gpart, grid,
+                                                     block, s);
+    case MODEL_D3Q27_CUMULANT_AVG_IB_SMAG:
+        return batch_m<MODEL_D3Q27_CUMULANT_AVG_IB_SMAG>(storage, globals, cases, tiles, ntiles, phase, gpart, grid,
+                                                        block, s);
     case MODEL_D2Q9_AUTOSYM:
         return batch_m<MODEL_D2Q9_AUTOSYM>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
     case MODEL_D2Q9_AUTOSYM2:
@@ -2164,6 +2291,14 @@ hipError_t launch_quantity(int model, int storage, int q, const StepArgs& a, int
         return quantity_m<MODEL_D3Q27_CUMULANT_IB>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
     case MODEL_D3Q27_CUMULANT_IB_SMAG:
         return quantity_m<MODEL_D3Q27_CUMULANT_IB_SMAG>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
+    case MODEL_D3Q27_CUMULANT_AVG:
+        return quantity_m<MODEL_D3Q27_CUMULANT_AVG>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
+    case MODEL_D3Q27_CUMULANT_AVG_IB:
+        return quantity_m<MODEL_D3Q27_CUMULANT_AVG_IB>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
+    case MODEL_D3Q27_CUMULANT_AVG_SMAG:
+        return quantity_m<MODEL_D3Q27_CUMULANT_AVG_SMAG>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
+    case MODEL_D3Q27_CUMULANT_AVG_IB_SMAG:
+        return quantity_m<MODEL_D3Q27_CUMULANT_AVG_IB_SMAG>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
     case MODEL_D2Q9_AUTOSYM:
         return quantity_m<MODEL_D2Q9_AUTOSYM>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
     case MODEL_D2Q9_AUTOSYM2:

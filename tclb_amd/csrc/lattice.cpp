@@ -67,6 +67,11 @@ Lattice::Lattice(const ModelDef* m, int nx_, int ny_, int nz_, int storage_)
     }
     hip_check(hipMalloc((void**)&d_flags_, N * sizeof(uint16_t)), "hipMalloc(flags)");
     hip_check(hipMemsetAsync(d_flags_, 0, N * sizeof(uint16_t), stream_), "hipMemset(flags)");
+    if (!m->averages.empty()) {
+        const size_t ab = m->averages.size() * N * sizeof(double);
+        hip_check(hipMalloc((void**)&d_avg_, ab), "hipMalloc(averages)");
+        hip_check(hipMemsetAsync(d_avg_, 0, ab, stream_), "hipMemset(averages)");
+    }
     hip_check(hipMalloc((void**)&d_zone_, std::max<size_t>(1, zone_.size()) * sizeof(double)), "hipMalloc(zone)");
     set_block(pick_block(nx));
     const int G = kernel_nglobals(m->id);
@@ -82,6 +87,7 @@ Lattice::~Lattice() {
     for (void* p : snaps_) if (p) (void)hipFree(p);
     if (d_flags_) (void)hipFree(d_flags_);
     if (d_cuts_) (void)hipFree(d_cuts_);
+    if (d_avg_) (void)hipFree(d_avg_);
     if (d_zone_) (void)hipFree(d_zone_);
     if (d_gpart_) (void)hipFree(d_gpart_);
     if (d_glob_) (void)hipFree(d_glob_);
@@ -105,7 +111,8 @@ void Lattice::set_block(int b) {
 }
 
 size_t Lattice::bytes_per_node_step() const {
-    return (size_t)2 * model->Q * storage_bytes(storage) + sizeof(uint16_t);
+    // the averaged densities are read and written in place on collision nodes
+    return (size_t)2 * model->Q * storage_bytes(storage) + sizeof(uint16_t) + 2 * model->averages.size() * sizeof(double);
 }
 
 void Lattice::SetSetting(int id, double v) {
@@ -224,6 +231,8 @@ StepArgs Lattice::args(int from, int to, bool glob) const {
     a.fout = snaps_[to];
     a.flags = d_flags_;
     a.cuts = d_cuts_;
+    a.avg = d_avg_;
+    a.avg_n = (double)(iter_ - reset_iter_);
     a.zone = d_zone_;
     a.gpart = glob ? d_gpart_ : nullptr;
     a.pitch = (long long)pitch;
@@ -262,8 +271,7 @@ void model_params(const ModelDef* model, const std::vector<double>& settings, do
         par[12] = S("MovingWallVelocityX");
         par[13] = S("MovingWallVelocityY");
         par[14] = S("MovingWallVelocityZ");
-    } else if (model->id == MODEL_D3Q27_CUMULANT || model->id == MODEL_D3Q27_CUMULANT_SMAG ||
-               model->id == MODEL_D3Q27_CUMULANT_IB || model->id == MODEL_D3Q27_CUMULANT_IB_SMAG) {
+    } else if (model->name.compare(0, 14, "d3q27_cumulant") == 0) {
         // [nu, nubuffer, GalileanCorrection, ForceX, ForceY, ForceZ, Omega, Smag]
         const char* names[] = {"nu", "nubuffer", "GalileanCorrection", "ForceX", "ForceY", "ForceZ", "Omega", "Smag"};
         for (int i = 0; i < 8; i++) par[i] = S(names[i]);
@@ -339,8 +347,20 @@ void Lattice::Init() {
     snap_ = 0;
     StepArgs a = args(1, 0, false);
     hip_check(launch_init(model->id, storage, a, block_, stream_), "init kernel");
+    // the averaged densities start from their default, 0 (src/conf.R:70);
+    // reset_iter is left as it is, as Init leaves it there
+    if (d_avg_)
+        hip_check(hipMemsetAsync(d_avg_, 0, model->averages.size() * N * sizeof(double), stream_), "Init(averages)");
     hip_check(hipStreamSynchronize(stream_), "Init");
     clearGlobals();
+}
+
+void Lattice::resetAverage() {
+    reset_iter_ = iter_;
+    if (d_avg_) {
+        hip_check(hipMemsetAsync(d_avg_, 0, model->averages.size() * N * sizeof(double), stream_), "resetAverage");
+        hip_check(hipStreamSynchronize(stream_), "resetAverage");
+    }
 }
 
 void Lattice::set_rows_per_tile(int r) {
@@ -457,6 +477,11 @@ void Lattice::GetQuantity(int q, const Region& over, double* out, double scale) 
 }
 
 void Lattice::GetDensity(int q, double* out) {
+    if (q >= model->Q && q < model->Q + (int)model->averages.size()) {
+        hip_check(hipMemcpy(out, d_avg_ + (size_t)(q - model->Q) * N, N * sizeof(double), hipMemcpyDeviceToHost),
+                  "GetDensity(average)");
+        return;
+    }
     if (q < 0 || q >= model->Q) throw std::out_of_range("density id");
     double* d = nullptr;
     hip_check(hipMalloc((void**)&d, N * sizeof(double)), "hipMalloc(density)");
@@ -469,6 +494,11 @@ void Lattice::GetDensity(int q, double* out) {
 }
 
 void Lattice::SetDensity(int q, const double* in) {
+    if (q >= model->Q && q < model->Q + (int)model->averages.size()) {
+        hip_check(hipMemcpy(d_avg_ + (size_t)(q - model->Q) * N, in, N * sizeof(double), hipMemcpyHostToDevice),
+                  "SetDensity(average)");
+        return;
+    }
     if (q < 0 || q >= model->Q) throw std::out_of_range("density id");
     double* d = nullptr;
     hip_check(hipMalloc((void**)&d, N * sizeof(double)), "hipMalloc(density)");
@@ -494,6 +524,13 @@ int Lattice::Save(const std::string& path) {
                   "Save");
         if (std::fwrite(host.data(), 1, host.size(), f) != host.size()) rc = -1;
     }
+    // then the averaged densities (FP64), as the reference's saveSolution
+    // writes every density of the snapshot
+    std::vector<double> av(N);
+    for (size_t k = 0; k < model->averages.size() && rc == 0; k++) {
+        hip_check(hipMemcpy(av.data(), d_avg_ + k * N, N * sizeof(double), hipMemcpyDeviceToHost), "Save");
+        if (std::fwrite(av.data(), sizeof(double), N, f) != N) rc = -1;
+    }
     std::fclose(f);
     return rc;
 }
@@ -508,6 +545,11 @@ int Lattice::Load(const std::string& path) {
         if (std::fread(host.data(), 1, host.size(), f) != host.size()) { rc = -1; break; }
         hip_check(hipMemcpy((char*)snaps_[snap_] + (size_t)q * pitch * S, host.data(), N * S, hipMemcpyHostToDevice),
                   "Load");
+    }
+    std::vector<double> av(N);
+    for (size_t k = 0; k < model->averages.size() && rc == 0; k++) {
+        if (std::fread(av.data(), sizeof(double), N, f) != N) { rc = -1; break; }
+        hip_check(hipMemcpy(d_avg_ + k * N, av.data(), N * sizeof(double), hipMemcpyHostToDevice), "Load");
     }
     std::fclose(f);
     return rc;

@@ -47,13 +47,17 @@ public:
     bool has_cuts() const { return d_cuts_ != nullptr; }
 
     void Init();                              // src/Lattice.cu.Rt:799-821
+    // Lattice::resetAverage (src/Lattice.cu.Rt:1360-1365): the averaged
+    // densities to zero, average_iter counted from this iteration
+    void resetAverage();
     void Iterate(int niter, int iter_type);   // src/Lattice.cu.Rt:900-989
     void clearGlobals();
     void calcGlobals();
     const std::vector<double>& globals() const { return globals_; }
 
     void GetQuantity(int q, const Region& over, double* out, double scale);
-    void GetDensity(int q, double* out);      // Get_<f> of the current snapshot
+    // Get_<f> of the current snapshot; q >= Q: averaged density q - Q
+    void GetDensity(int q, double* out);
     void SetDensity(int q, const double* in); // Set_<f>
     int Save(const std::string& path);
     int Load(const std::string& path);
@@ -92,6 +96,8 @@ private:
     int snap_ = 0, iter_ = 0;
     uint16_t* d_flags_ = nullptr;
     uint16_t* d_cuts_ = nullptr;
+    double* d_avg_ = nullptr;   // model->averages.size() FP64 planes of N
+    int reset_iter_ = 0;
     double* d_zone_ = nullptr;
     double* d_gpart_ = nullptr;
     double* d_glob_ = nullptr;

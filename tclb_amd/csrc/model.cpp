@@ -281,13 +281,19 @@ ModelDef make_d3q27() {
 // smag: the SMAG build (d3q27_cumulant_SMAG): a Smag setting and the
 // Smagorinsky viscosity of Dynamics.c.Rt:362-374.  ib: the IB build
 // (d3q27_cumulant_IB): NodeType IB, interpolated bounce-back on the cut
-// distances of the lattice (Dynamics.c.Rt:180-222).  Option builds are
-// named <model>_<options in the order of conf.mk's OPT> (src/models.R:54-66).
-ModelDef make_d3q27_cumulant(bool smag = false, bool ib = false) {
+// distances of the lattice (Dynamics.c.Rt:180-222).  avg: the AVG build
+// (d3q27_cumulant_AVG): 13 averaged densities and the quantities built on
+// them (Dynamics.R:66-90, Dynamics.c.Rt:62-99,410-414,507-518).  Option
+// builds are named <model>_<options in the order of conf.mk's OPT>
+// (src/models.R:54-66).
+ModelDef make_d3q27_cumulant(bool smag = false, bool ib = false, bool avg = false) {
     ModelDef m;
-    m.id = ib ? (smag ? MODEL_D3Q27_CUMULANT_IB_SMAG : MODEL_D3Q27_CUMULANT_IB)
-              : (smag ? MODEL_D3Q27_CUMULANT_SMAG : MODEL_D3Q27_CUMULANT);
-    m.name = std::string("d3q27_cumulant") + (ib ? "_IB" : "") + (smag ? "_SMAG" : "");
+    static const ModelId ids[8] = {MODEL_D3Q27_CUMULANT,     MODEL_D3Q27_CUMULANT_SMAG,
+                                   MODEL_D3Q27_CUMULANT_IB,  MODEL_D3Q27_CUMULANT_IB_SMAG,
+                                   MODEL_D3Q27_CUMULANT_AVG, MODEL_D3Q27_CUMULANT_AVG_SMAG,
+                                   MODEL_D3Q27_CUMULANT_AVG_IB, MODEL_D3Q27_CUMULANT_AVG_IB_SMAG};
+    m.id = ids[(avg ? 4 : 0) + (ib ? 2 : 0) + (smag ? 1 : 0)];
+    m.name = std::string("d3q27_cumulant") + (avg ? "_AVG" : "") + (ib ? "_IB" : "") + (smag ? "_SMAG" : "");
     m.Q = 27; m.dim = 3;
     const int e1[3] = {0, 1, -1};
     for (int i = 0; i < 27; i++) {
@@ -310,6 +316,12 @@ ModelDef make_d3q27_cumulant(bool smag = false, bool ib = false) {
                                 {"ZMomentum", false, "kgm/s"}};
     finish_globals(m, g);
     m.quantities = {{"P", false, "Pa"}, {"U", true, "m/s"}, {"Solid", false, "1"}};
+    if (avg) {
+        m.quantities.insert(m.quantities.end(), {{"KinE", false, "1"}, {"ReStr", true, "1"}, {"Dissipation", false, "1"},
+                                                 {"avgU", true, "m/s"}, {"varU", true, "1"}, {"averageP", false, "Pa"}});
+        m.averages = {"avgP",    "varUX",   "varUY",   "varUZ",   "varUXUY", "varUXUZ", "varUYUZ",
+                      "avgdxu2", "avgdyv2", "avgdzw2", "avgUX",   "avgUY",   "avgUZ"};
+    }
     std::vector<std::pair<std::string, std::string>> types = {
         {"Buffer", "BOUNDARY"},       {"WVelocityTurbulent", "BOUNDARY"}, {"NVelocity", "BOUNDARY"},
         {"SVelocity", "BOUNDARY"},    {"NPressure", "BOUNDARY"},          {"SPressure", "BOUNDARY"},
@@ -328,7 +340,9 @@ const std::vector<ModelDef>& registry() {
                                       make_d2q9_mrt(false), make_d2q9_mrt(true), make_d3q27(),
                                       make_d3q27_cumulant(), make_d2q9_mrt(false, 1), make_d2q9_mrt(false, 2),
                                       make_d3q27_cumulant(true), make_d3q27_cumulant(false, true),
-                                      make_d3q27_cumulant(true, true)};
+                                      make_d3q27_cumulant(true, true), make_d3q27_cumulant(false, false, true),
+                                      make_d3q27_cumulant(false, true, true), make_d3q27_cumulant(true, false, true),
+                                      make_d3q27_cumulant(true, true, true)};
     return r;
 }
 

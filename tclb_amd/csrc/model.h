@@ -32,7 +32,11 @@ enum ModelId {
     MODEL_D3Q27_CUMULANT_SMAG = 10,
     MODEL_D3Q27_CUMULANT_IB = 11,
     MODEL_D3Q27_CUMULANT_IB_SMAG = 12,
-    MODEL_COUNT = 13
+    MODEL_D3Q27_CUMULANT_AVG = 13,
+    MODEL_D3Q27_CUMULANT_AVG_IB = 14,
+    MODEL_D3Q27_CUMULANT_AVG_SMAG = 15,
+    MODEL_D3Q27_CUMULANT_AVG_IB_SMAG = 16,
+    MODEL_COUNT = 17
 };
 
 // storage format of the distributions in HBM
@@ -80,6 +84,9 @@ struct ModelDef {
     std::vector<SettingDef> zonal;    // zonal settings (incl. <Global>InObj)
     std::vector<GlobalDef> globals;   // SUM..., Objective, MAX...
     std::vector<QuantityDef> quantities;
+    // densities added with average=TRUE (not streamed, reset by <Average>):
+    // kept in FP64 planes beside the distributions, in this order
+    std::vector<std::string> averages;
     std::vector<NodeTypeDef> nodetypes;
     std::vector<GroupDef> groups;
     int zone_shift = 0, zone_max = 0, flag_bits = 16;

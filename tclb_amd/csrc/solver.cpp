@@ -438,6 +438,20 @@ struct CbVTK : Callback {
     int doit() override { return solver->write_vtk(nm, what, reg); }
 };
 
+// <Average Iterations> (src/Handlers/cbAveraging.cpp): resetAverage when
+// set up and every Iterations after
+struct CbAverage : Callback {
+    int init() override {
+        Callback::init();
+        solver->lattice->resetAverage();
+        return 0;
+    }
+    int doit() override {
+        solver->lattice->resetAverage();
+        return 0;
+    }
+};
+
 // <SaveBinary [file|filename] Iterations> (src/Handlers/cbSaveBinary.cpp):
 // the current snapshot to <file>_0.pri (Lattice::saveSolution)
 struct CbSaveBinary : Callback {
@@ -711,6 +725,7 @@ HandlerRef Solver::make_handler(XmlNode* n) {
     else if (e == "VTK") h = new CbVTK;
     else if (e == "SaveBinary") h = new CbSaveBinary;
     else if (e == "SaveCheckpoint") h = new CbSaveCheckpoint;
+    else if (e == "Average") h = new CbAverage;
     else if (e == "LoadBinary") h = new AcLoadBinary;
     else if (e == "Run") h = new NullHandler;
     else {

@@ -30,7 +30,8 @@ def test_every_declared_symbol_is_exported():
 def test_models():
     assert tclb_amd.models() == ["d2q9_SRT", "d2q9_SRT_drag", "d3q19", "d3q19_lid", "d2q9", "d2q9_drag", "d3q27",
                                  "d3q27_cumulant", "d2q9_autosym", "d2q9_autosym2", "d3q27_cumulant_SMAG",
-                                 "d3q27_cumulant_IB", "d3q27_cumulant_IB_SMAG"]
+                                 "d3q27_cumulant_IB", "d3q27_cumulant_IB_SMAG", "d3q27_cumulant_AVG",
+                                 "d3q27_cumulant_AVG_IB", "d3q27_cumulant_AVG_SMAG", "d3q27_cumulant_AVG_IB_SMAG"]
 
 
 def test_d2q9_mrt_contract():
@@ -126,7 +127,7 @@ def test_lattice_creation_fails_loudly_without_gpu():
 
 def test_unknown_model_is_an_error():
     with pytest.raises(tclb_amd.TclbxError):
-        tclb_amd.Lattice("d3q27_cumulant_AVG", 8, 8, 8)
+        tclb_amd.Lattice("d3q27_cumulant_part", 8, 8, 8)
 
 
 def test_d3q27_contract():

@@ -370,3 +370,49 @@ def test_ib_sphere_case(oracle_mod, tmp_path, model):
     assert abs(rows[-1][head.index("Drag")]) > 1e-6
     u, uo = c.lattice.quantity("U"), orc.quantity("U")
     assert np.max(np.abs(u - uo)) <= 1e-12 * np.max(np.abs(uo))
+
+
+AVG_CHANNEL = """<CLBConfig output="output/">
+  <Geometry nx="32" ny="14" nz="10">
+    <MRT><Box/></MRT>
+    <WVelocity name="Inlet"><Inlet/></WVelocity>
+    <EPressure name="Outlet"><Outlet/></EPressure>
+    <Wall mask="ALL"><OffgridSphere x="10" y="7" z="5" R="2.6"/></Wall>
+  </Geometry>
+  <Model>
+    <Param name="nu" value="0.02"/>
+    <Param name="Velocity" value="0.04" zone="Inlet"/>
+  </Model>
+  <VTK Iterations="300" what="avgU,varU,KinE,averageP,U"/>
+  <Average Iterations="100"/>
+  <Solve Iterations="300"/>
+</CLBConfig>"""
+
+
+def test_average_handler_case(oracle_mod, tmp_path):
+    """<Average Iterations=100> resets the averages when set up and every 100
+    iterations; the VTK callback listed before it writes, at iteration 300,
+    the averages of iterations 201-300, equal to the oracle's after the same
+    resets."""
+    model = "d3q27_cumulant_AVG"
+    say("case: d3q27_cumulant_AVG with <Average>")
+    c = tclb_amd.Case(model, xml=AVG_CHANNEL, name="avg.xml", output_root=str(tmp_path) + "/")
+    c.run()
+    assert c.iteration == 300
+    flags, zones, _ = G.read_case(AVG_CHANNEL, model, is_text=True)
+    nz, ny, nx = flags.shape
+    oc = cases.Case("avg", model, nx, ny, nz, flags, settings=[("nu", 0.02)],
+                    zonal=[("Velocity", zones["Inlet"], 0.04)])
+    orc = oracle_for_case(oracle_mod, oc, "f64")
+    orc.iterate(200, False)
+    orc.reset_average()
+    orc.iterate(100, False)
+    root, arr = _vti_arrays(tmp_path / "output" / "avg_VTK_P00_00000300.vti")
+    for q, ncomp in (("avgU", 3), ("varU", 3), ("KinE", 1), ("averageP", 1), ("U", 3)):
+        v, k = arr[q]
+        assert k == ncomp
+        want = orc.quantity(q)
+        got = v.reshape(want.shape)
+        assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want)), q
+    mu = arr["avgU"][0].reshape(nz, ny, nx, 3)
+    assert mu[..., 0].max() > 0.01          # the mean flow is there

@@ -164,7 +164,9 @@ def random_case(model, nx, ny, nz, seed, steps=10):
 MODELS = [("d2q9_SRT", 24, 10, 1), ("d2q9_SRT_drag", 37, 9, 1), ("d3q19", 13, 7, 6), ("d3q19_lid", 11, 9, 7),
           ("d2q9", 29, 11, 1), ("d2q9_drag", 23, 13, 1), ("d3q27", 12, 7, 5), ("d3q27_cumulant", 11, 8, 5),
           ("d2q9_autosym", 17, 11, 1), ("d2q9_autosym2", 19, 12, 1), ("d3q27_cumulant_SMAG", 9, 8, 6),
-          ("d3q27_cumulant_IB", 10, 9, 5), ("d3q27_cumulant_IB_SMAG", 9, 7, 6)]
+          ("d3q27_cumulant_IB", 10, 9, 5), ("d3q27_cumulant_IB_SMAG", 9, 7, 6), ("d3q27_cumulant_AVG", 8, 7, 6),
+          ("d3q27_cumulant_AVG_IB", 7, 9, 5), ("d3q27_cumulant_AVG_SMAG", 9, 6, 5),
+          ("d3q27_cumulant_AVG_IB_SMAG", 7, 8, 5)]
 
 
 @pytest.mark.parametrize("model,nx,ny,nz", MODELS)
@@ -174,7 +176,8 @@ def test_random_case_f64(oracle_mod, model, nx, ny, nz):
                        seed={"d2q9_SRT": 11, "d2q9_SRT_drag": 12, "d3q19": 13, "d3q19_lid": 14, "d2q9": 15,
                              "d2q9_drag": 16, "d3q27": 18, "d3q27_cumulant": 19, "d2q9_autosym": 20,
                              "d2q9_autosym2": 21, "d3q27_cumulant_SMAG": 22, "d3q27_cumulant_IB": 23,
-                             "d3q27_cumulant_IB_SMAG": 24}[model])
+                             "d3q27_cumulant_IB_SMAG": 24, "d3q27_cumulant_AVG": 25, "d3q27_cumulant_AVG_IB": 26,
+                             "d3q27_cumulant_AVG_SMAG": 27, "d3q27_cumulant_AVG_IB_SMAG": 28}[model])
     lat = case.build("f64")
     orc = oracle_for_case(oracle_mod, case, "f64")
     for n in (1, 6, 13):
@@ -218,6 +221,34 @@ def test_flags_roundtrip(model, nx, ny, nz):
     want[:, 1:3, 2:5] = patch
     assert np.array_equal(lat.get_flags(), want)
     assert np.array_equal(lat.get_flags(region=(2, 1, 0, 3, 2, nz)), patch)
+
+
+@pytest.mark.parametrize("model", ["d3q27_cumulant_AVG", "d3q27_cumulant_AVG_IB_SMAG"])
+def test_averages_match_oracle(oracle_mod, tmp_path, model):
+    """The AVG builds' averaged densities (FP64, updated in place) against
+    the oracle's after some steps, again after resetAverage, and through a
+    save/load round trip."""
+    say(f"averages {model}")
+    case = random_case(model, 9, 7, 6, seed=41)
+    lat = case.build("f64")
+    orc = oracle_for_case(oracle_mod, case, "f64")
+    for steps, reset in ((5, False), (3, True), (4, False)):
+        if reset:
+            lat.reset_average()
+            orc.reset_average()
+        lat.iterate(steps)
+        orc.iterate(steps)
+        got, want = lat.get_averages(), orc.averages()
+        for k in want:
+            assert rel_max(got[k], want[k]) <= 1e-12, (steps, k)
+        for q in ("KinE", "ReStr", "Dissipation", "avgU", "varU", "averageP"):
+            assert rel_max(lat.quantity(q), orc.quantity(q)) <= 1e-10, (steps, q)
+    p = tmp_path / "avg.pri"
+    lat.save(p)
+    other = case.build("f64")
+    other.load(p)
+    for k, v in lat.get_averages().items():
+        assert np.array_equal(other.get_averages()[k], v), k
 
 
 def test_cuts_roundtrip():

@@ -1322,3 +1322,88 @@ def test_ib_contract():
         for t in ("Wall", "MRT", "Body", "NSymmetry", "WVelocity", "Buffer"):
             assert m.nodetype(t) == b.nodetype(t)
         assert m.settings == b.settings and m.global_names == b.global_names
+
+
+# ------------------------------------------------------------ averages ---
+def test_avg_build_accumulates_the_post_collision_state(O):
+    """d3q27_cumulant_AVG on a periodic box of collision, buffer and wall
+    nodes, checked against sums kept in numpy from the states alone: each
+    step adds, on collision nodes, P and u (0 on Wall nodes) of the
+    post-collision populations (the stored snapshot) and u_i u_j, and
+    dxu^2, dyv^2, dzw^2 from the second cumulants of the pulled populations
+    (after the wall's bounce-back), taken here as central moments; the
+    quantities divide by the iterations since the last reset, and a reset
+    starts the sums again."""
+    import tclb_amd
+    m = tclb_amd.Model("d3q27_cumulant_AVG")
+    assert m.averages == ["avgP", "varUX", "varUY", "varUZ", "varUXUY", "varUXUZ", "varUYUZ", "avgdxu2",
+                          "avgdyv2", "avgdzw2", "avgUX", "avgUY", "avgUZ"]
+    assert m.quantities == ["P", "U", "Solid", "KinE", "ReStr", "Dissipation", "avgU", "varU", "averageP"]
+    nx, ny, nz = 6, 5, 4
+    rng = np.random.default_rng(360)
+    kinds = np.array([m.flag("MRT")] * 6 + [m.flag("Buffer", "MRT"), m.flag("Wall", "MRT"), m.flag("Wall")],
+                     np.uint16)
+    flags = kinds[rng.integers(0, len(kinds), nx * ny * nz)].reshape(nz, ny, nx)
+    nu, nub, F = 0.04, 0.1, np.array([2e-5, -1e-5, 3e-6])
+    L = O.OracleLattice("d3q27_cumulant_AVG", nx, ny, nz)
+    L.set_params([nu, nub, 1.0, *F, 0.8])
+    L.set_zone_table(np.zeros((1, 32)))
+    L.set_flags(flags)
+    L.init()
+    f0 = np.stack([_feqc(1.0, 0.02 * rng.standard_normal(3)) for _ in range(nx * ny * nz)], 1)
+    f0 = (f0 * (1 + 0.03 * rng.standard_normal(f0.shape))).reshape(27, nz, ny, nx)
+    L.set_f(f0)
+    coll = (flags & m.flag("MRT")) != 0
+    wall = (flags & 0x78) == m.flag("Wall")
+    bnd = (flags & 0x78) != 0
+    opp = [_opp(i) for i in range(27)]
+
+    def pulled(f):
+        g = np.empty_like(f)
+        for i in range(27):
+            g[i] = np.roll(f[i], shift=(int(EC[i][2]), int(EC[i][1]), int(EC[i][0])), axis=(0, 1, 2))
+        return np.where(wall[None], g[opp], g)          # Wall: bounce-back before the collision
+
+    def run(steps):
+        S = {k: np.zeros((nz, ny, nx)) for k in O.AVERAGES}
+        for _ in range(steps):
+            p = pulled(L.get_f())
+            m0 = p.sum(0)
+            mu = np.einsum("i...,ia->a...", p, EC) / m0
+            c2 = np.einsum("i...,ia->a...", p, EC ** 2) / m0 - mu ** 2      # c200, c020, c002
+            w0 = np.where(bnd, 1 / (3 * nub + 0.5), 1 / (3 * nu + 0.5))
+            dxu = -w0 / 2 * (2 * c2[0] - c2[1] - c2[2]) - 0.5 * (c2.sum(0) - 1)
+            dyv = dxu + 1.5 * w0 * (c2[0] - c2[1])
+            dzw = dxu + 1.5 * w0 * (c2[0] - c2[2])
+            L.iterate(1, False)
+            f = L.get_f()
+            rho = f.sum(0)
+            u = (np.einsum("i...,ia->a...", f, EC) + F[:, None, None, None] / 2) / rho
+            u = np.where(wall[None], 0.0, u)
+            add = {"avgP": (rho - 1) / 3, "avgUX": u[0], "avgUY": u[1], "avgUZ": u[2], "varUX": u[0] ** 2,
+                   "varUY": u[1] ** 2, "varUZ": u[2] ** 2, "varUXUY": u[0] * u[1], "varUXUZ": u[0] * u[2],
+                   "varUYUZ": u[1] * u[2], "avgdxu2": dxu ** 2, "avgdyv2": dyv ** 2, "avgdzw2": dzw ** 2}
+            for k, v in add.items():
+                S[k] += np.where(coll, v, 0.0)
+        return S
+
+    for steps in (7, 4):
+        if steps == 4:
+            L.reset_average()
+        S = run(steps)
+        got = L.averages()
+        for k in O.AVERAGES:
+            assert np.allclose(got[k], S[k], rtol=1e-11, atol=1e-16), k
+        n = float(steps)
+        mu = np.stack([S["avgUX"], S["avgUY"], S["avgUZ"]], -1) / n
+        var = np.stack([S["varUX"], S["varUY"], S["varUZ"]], -1) / n - mu ** 2
+        rs = np.stack([S["varUYUZ"] / n - mu[..., 1] * mu[..., 2], S["varUXUZ"] / n - mu[..., 0] * mu[..., 2],
+                       S["varUXUY"] / n - mu[..., 0] * mu[..., 1]], -1)
+        diss = 0
+        for k, (a, ax) in enumerate((("avgUX", 2), ("avgUY", 1), ("avgUZ", 0))):
+            g = np.roll(S[a], -1, ax) - np.roll(S[a], 1, ax)
+            diss = diss + nu * (S[("avgdxu2", "avgdyv2", "avgdzw2")[k]] / n - g * g / (4 * n * n))
+        want = {"avgU": mu, "varU": var, "ReStr": rs, "KinE": 0.5 * var.sum(-1), "averageP": S["avgP"] / n,
+                "Dissipation": diss}
+        for q, w in want.items():
+            assert np.allclose(L.quantity(q), w, rtol=1e-10, atol=1e-18), q
