@@ -37,8 +37,8 @@ int tclbx_version(void);
 
 /* Models compiled in: "d2q9_SRT", "d2q9_SRT_drag", "d3q19", "d3q19_lid", "d2q9",
  * "d2q9_drag", "d3q27", "d3q27_cumulant" and its option builds
- * "d3q27_cumulant_[AVG_][IB_][SMAG]" (all eight), "d2q9_autosym",
- * "d2q9_autosym2" (d2q9's autosym builds).
+ * "d3q27_cumulant_[AVG_][IB_][SMAG]" (all eight), and d2q9's option builds
+ * "d2q9_[bc_]autosym[2]", "d2q9_bc".
  * Replaces the per-model binaries CLB/<model>/main (src/makefile.main.Rt:331-349). */
 int tclbx_model_count(void);
 const char* tclbx_model_name(int i);
@@ -126,11 +126,14 @@ int tclbx_get_quantity(tclbx_lattice* h, int q, int dx, int dy, int dz, int nx, 
  * lattice, as real values (storage shift removed / applied). */
 int tclbx_get_density(tclbx_lattice* h, int q, double* out);
 int tclbx_set_density(tclbx_lattice* h, int q, const double* in);
-/* Densities declared average=TRUE (the AVG build of d3q27_cumulant,
- * models/flow/d3q27_cumulant/Dynamics.R:66-90): name of averaged density k
- * (NULL past the last); it is density Q + k of tclbx_get/set_density, kept
- * in FP64 and updated in place. */
-const char* tclbx_average_name(const char* model, int k);
+/* The densities that are not streamed (AddDensity with average=TRUE or
+ * parameter=TRUE, src/conf.R:65-100): the averages of d3q27_cumulant's AVG
+ * builds (models/flow/d3q27_cumulant/Dynamics.R:66-90), then the parameter
+ * densities BC[0], BC[1] of d2q9's bc builds (models/flow/d2q9/Dynamics.R:
+ * 23-26).  Name of extra density k (NULL past the last), *is_average = 1
+ * for an average; it is density Q + k of tclbx_get/set_density, kept in FP64
+ * and updated in place. */
+const char* tclbx_extra_density_name(const char* model, int k, int* is_average);
 /* Lattice::resetAverage (src/Lattice.cu.Rt:1360-1365), what <Average>
  * calls: the averaged densities to zero, average_iter counted from the
  * current iteration. */

@@ -260,6 +260,9 @@ NODETYPES["d3q27_cumulant_SMAG"] = NODETYPES["d3q27_cumulant"]
 NODETYPES["d3q27_cumulant_IB"] = NODETYPES["d3q27_cumulant"][:9] + [("IB", "HO_BOUNDARY")] + \
     NODETYPES["d3q27_cumulant"][9:]
 NODETYPES["d3q27_cumulant_IB_SMAG"] = NODETYPES["d3q27_cumulant_IB"]
+# d2q9's bc builds add parameter densities, no NodeTypes
+for _b in ("", "_autosym", "_autosym2"):
+    NODETYPES["d2q9_bc" + _b] = NODETYPES["d2q9" + _b]
 # the AVG builds add densities and quantities, no NodeTypes
 for _b in ("", "_IB", "_SMAG", "_IB_SMAG"):
     NODETYPES["d3q27_cumulant_AVG" + _b] = NODETYPES["d3q27_cumulant" + _b]

@@ -105,7 +105,8 @@ struct orc_lattice {
     int cur;                    /* snapshot holding the current state */
     uint16_t* flags;
     uint16_t* cuts;             /* [26][N] or NULL: no cuts */
-    double* avg;                /* AVG builds: [13][N] averaged densities, else NULL */
+    double* extra;              /* densities not streamed, [nextra][N]: AVG averages or bc's BC[0..1] */
+    int nextra, navg;           /* how many, and how many of them are averages */
     int iter, reset_iter;       /* iterations since Init; of the last average reset */
     double params[16];
     double* zone;               /* [nzones][ncols] */
@@ -159,7 +160,7 @@ orc_lattice* orc_create(int model, int nx, int ny, int nz, int storage)
     L->model = model; L->nx = nx; L->ny = ny; L->nz = nz; L->storage = storage;
     L->N = (size_t)nx * ny * nz;
     if (model == ORC_D3Q27 || model == ORC_D3Q27_CUMULANT || model == ORC_D3Q27_CUMULANT_SMAG ||
-        model >= ORC_D3Q27_CUMULANT_IB) {
+        (model >= ORC_D3Q27_CUMULANT_IB && model <= ORC_D3Q27_CUMULANT_AVG_IB_SMAG)) {
         int cum = model != ORC_D3Q27;
         cum_init_d();
         cum_init_f();
@@ -178,13 +179,16 @@ orc_lattice* orc_create(int model, int nx, int ny, int nz, int storage)
     } else {
         L->Q = 9;
         L->nglob = model == ORC_D2Q9_SRT_DRAG ? 3 : (model == ORC_D2Q9_DRAG ? 6 : 1);
-        if (model == ORC_D2Q9 || model == ORC_D2Q9_AUTOSYM || model == ORC_D2Q9_AUTOSYM2) L->nglob = 4;
+        if (model == ORC_D2Q9 || model == ORC_D2Q9_AUTOSYM || model == ORC_D2Q9_AUTOSYM2 || model >= ORC_D2Q9_BC)
+            L->nglob = 4;
         for (q = 0; q < 9; q++) {
             for (a = 0; a < 3; a++) L->e[q][a] = d2q9_e[q][a];
             L->w[q] = d2q9_w[q];
         }
     }
-    if (model >= ORC_D3Q27_CUMULANT_AVG) L->avg = (double*)calloc(13 * L->N, sizeof(double));
+    if (model >= ORC_D3Q27_CUMULANT_AVG && model <= ORC_D3Q27_CUMULANT_AVG_IB_SMAG) L->nextra = L->navg = 13;
+    if (model >= ORC_D2Q9_BC) L->nextra = 2;
+    if (L->nextra) L->extra = (double*)calloc((size_t)L->nextra * L->N, sizeof(double));
     L->buf[0] = calloc(L->N * L->Q, esize(storage));
     L->buf[1] = calloc(L->N * L->Q, esize(storage));
     L->flags = (uint16_t*)calloc(L->N, sizeof(uint16_t));
@@ -196,7 +200,7 @@ orc_lattice* orc_create(int model, int nx, int ny, int nz, int storage)
 void orc_destroy(orc_lattice* L)
 {
     if (!L) return;
-    free(L->buf[0]); free(L->buf[1]); free(L->flags); free(L->cuts); free(L->avg); free(L->zone); free(L);
+    free(L->buf[0]); free(L->buf[1]); free(L->flags); free(L->cuts); free(L->extra); free(L->zone); free(L);
 }
 
 int orc_q(const orc_lattice* L) { return L->Q; }
@@ -235,13 +239,19 @@ static int is3d(const orc_lattice* L) { return L->model == ORC_D3Q19 || L->model
 static int is_d2q9m(const orc_lattice* L)
 {
     return L->model == ORC_D2Q9 || L->model == ORC_D2Q9_DRAG || L->model == ORC_D2Q9_AUTOSYM ||
-           L->model == ORC_D2Q9_AUTOSYM2;
+           L->model == ORC_D2Q9_AUTOSYM2 || L->model >= ORC_D2Q9_BC;
 }
-static int is_autosym(const orc_lattice* L) { return L->model == ORC_D2Q9_AUTOSYM || L->model == ORC_D2Q9_AUTOSYM2; }
+static int is_autosym(const orc_lattice* L)
+{
+    return L->model == ORC_D2Q9_AUTOSYM || L->model == ORC_D2Q9_AUTOSYM2 || L->model == ORC_D2Q9_BC_AUTOSYM ||
+           L->model == ORC_D2Q9_BC_AUTOSYM2;
+}
+static int is_bc(const orc_lattice* L) { return L->model >= ORC_D2Q9_BC; }
 static int is_q27(const orc_lattice* L) { return L->model == ORC_D3Q27; }
 static int is_cum(const orc_lattice* L)
 {
-    return L->model == ORC_D3Q27_CUMULANT || L->model == ORC_D3Q27_CUMULANT_SMAG || L->model >= ORC_D3Q27_CUMULANT_IB;
+    return L->model == ORC_D3Q27_CUMULANT || L->model == ORC_D3Q27_CUMULANT_SMAG ||
+           (L->model >= ORC_D3Q27_CUMULANT_IB && L->model <= ORC_D3Q27_CUMULANT_AVG_IB_SMAG);
 }
 static int is_ib(const orc_lattice* L)
 {
@@ -281,7 +291,7 @@ void orc_init(orc_lattice* L)
     /* Lattice::Init (src/Lattice.cu.Rt:799-821): iter = 0, reset_iter kept;
      * the averaged densities start at their default 0 (src/conf.R:70) */
     L->iter = 0;
-    if (L->avg) memset(L->avg, 0, 13 * L->N * sizeof(double));
+    if (L->navg) memset(L->extra, 0, (size_t)L->navg * L->N * sizeof(double));
     for (n = 0; n < L->N; n++) {
         const double* zr = zone_row(L, L->flags[n]);
         if (is_cum(L)) {
@@ -330,11 +340,21 @@ void orc_init(orc_lattice* L)
                 if (wall) { rho = 1; ux = 0; uy = 0; }
                 d2q9m_feq_d(f, rho, ux * rho, uy * rho);
                 for (q = 0; q < 9; q++) st(L, L->buf[0], (size_t)q * L->N + n, q, f[q]);
+                if (is_bc(L)) {   /* SetEquilibrum: BC = J/rho on BOUNDARY nodes, else 0 */
+                    int bnd = (L->flags[n] & 0xF) != 0;
+                    L->extra[n] = bnd ? ux * rho / rho : 0.0;
+                    L->extra[L->N + n] = bnd ? uy * rho / rho : 0.0;
+                }
             } else {
                 float f[9], rho = 1 + (float)zr[2] * 3, ux = (float)zr[0], uy = (float)zr[1];
                 if (wall) { rho = 1; ux = 0; uy = 0; }
                 d2q9m_feq_f(f, rho, ux * rho, uy * rho);
                 for (q = 0; q < 9; q++) st(L, L->buf[0], (size_t)q * L->N + n, q, f[q]);
+                if (is_bc(L)) {
+                    int bnd = (L->flags[n] & 0xF) != 0;
+                    L->extra[n] = bnd ? (double)(ux * rho / rho) : 0.0;
+                    L->extra[L->N + n] = bnd ? (double)(uy * rho / rho) : 0.0;
+                }
             }
         } else {
             /* Init(): SetEquilibrium(Density, Velocity_x, Velocity_y), Dynamics.c:26-38 */
@@ -383,7 +403,7 @@ static void autosym_access(const orc_lattice* L, const void* b, int x, int y, ui
     int sym[2], axis, q;
     sym[0] = (nt >> 7) & 3;
     sym[1] = (nt >> 9) & 3;
-    if (L->model == ORC_D2Q9_AUTOSYM) {
+    if (L->model == ORC_D2Q9_AUTOSYM || L->model == ORC_D2Q9_BC_AUTOSYM) {
         /* resolve.symmetries after the pull: on a _plus node the densities
          * with e_axis < 0 take their mirrors, on a _minus node those with
          * e_axis > 0; X, then Y */
@@ -439,10 +459,11 @@ static void step_rows(orc_lattice* L, long double* glob, long r0, long r1)
                     for (q = 0; q < 27; q++) old[q] = ld(L, in, (size_t)q * L->N + n, q);
                 if (L->storage == ORC_F64) {
                     if (cut) cum_ib_d(fd, nt, cut, old, zr, glob);
-                    if (is_cum(L)) cum_node_d(fd, nt, L->params, zr, glob, is_smag(L), L->avg ? L->avg + n : NULL, L->N);
+                    if (is_cum(L)) cum_node_d(fd, nt, L->params, zr, glob, is_smag(L), L->navg ? L->extra + n : NULL, L->N);
                     else if (is_q27(L)) d3q27_node_d(fd, nt, L->params, zr, glob);
                     else if (is3d(L)) d3q19_node_d(fd, nt, L->params, zr, glob, lid, L->flags, x, y, z, L->nx, L->ny, L->nz);
-                    else if (is_d2q9m(L)) d2q9m_node_d(fd, nt, L->params, zr, glob, L->model == ORC_D2Q9_DRAG);
+                    else if (is_d2q9m(L)) d2q9m_node_d(fd, nt, L->params, zr, glob, L->model == ORC_D2Q9_DRAG,
+                                                       is_bc(L) ? L->extra + n : NULL, L->N);
                     else d2q9_node_d(fd, nt, L->params[0], L->params[1], zr, drag, glob);
                     for (q = 0; q < L->Q; q++) st(L, out, (size_t)q * L->N + n, q, fd[q]);
                 } else {
@@ -453,10 +474,11 @@ static void step_rows(orc_lattice* L, long double* glob, long r0, long r1)
                         for (q = 0; q < 27; q++) oldf[q] = (float)old[q];
                         cum_ib_f(ff, nt, cut, oldf, zr, glob);
                     }
-                    if (is_cum(L)) cum_node_f(ff, nt, L->params, zr, glob, is_smag(L), L->avg ? L->avg + n : NULL, L->N);
+                    if (is_cum(L)) cum_node_f(ff, nt, L->params, zr, glob, is_smag(L), L->navg ? L->extra + n : NULL, L->N);
                     else if (is_q27(L)) d3q27_node_f(ff, nt, L->params, zr, glob);
                     else if (is3d(L)) d3q19_node_f(ff, nt, L->params, zr, glob, lid, L->flags, x, y, z, L->nx, L->ny, L->nz);
-                    else if (is_d2q9m(L)) d2q9m_node_f(ff, nt, L->params, zr, glob, L->model == ORC_D2Q9_DRAG);
+                    else if (is_d2q9m(L)) d2q9m_node_f(ff, nt, L->params, zr, glob, L->model == ORC_D2Q9_DRAG,
+                                                       is_bc(L) ? L->extra + n : NULL, L->N);
                     else d2q9_node_f(ff, nt, (float)L->params[0], (float)L->params[1], zr, drag, glob);
                     for (q = 0; q < L->Q; q++) st(L, out, (size_t)q * L->N + n, q, ff[q]);
                 }
@@ -534,14 +556,19 @@ void orc_set_f(orc_lattice* L, const double* in)
 void orc_reset_average(orc_lattice* L)
 {
     L->reset_iter = L->iter;
-    if (L->avg) memset(L->avg, 0, 13 * L->N * sizeof(double));
+    if (L->navg) memset(L->extra, 0, (size_t)L->navg * L->N * sizeof(double));
 }
 
-int orc_get_averages(const orc_lattice* L, double* out)
+int orc_nextra(const orc_lattice* L) { return L->nextra; }
+
+void orc_get_extra(const orc_lattice* L, double* out)
 {
-    if (!L->avg) return -1;
-    memcpy(out, L->avg, 13 * L->N * sizeof(double));
-    return 0;
+    if (L->nextra) memcpy(out, L->extra, (size_t)L->nextra * L->N * sizeof(double));
+}
+
+void orc_set_extra(orc_lattice* L, const double* in)
+{
+    if (L->nextra) memcpy(L->extra, in, (size_t)L->nextra * L->N * sizeof(double));
 }
 
 /* the AVG quantities (models/flow/d3q27_cumulant/Dynamics.c.Rt:62-99) of
@@ -549,7 +576,7 @@ int orc_get_averages(const orc_lattice* L, double* out)
  * 10 averageP, over average_iter = iter - reset_iter iterations */
 static void avg_quantity(const orc_lattice* L, int which, int x, int y, int z, double* out)
 {
-    const double* A = L->avg;
+    const double* A = L->extra;
     const int it = L->iter - L->reset_iter;
     const double N = (double)it;
 #define AV(k, xx, yy, zz) A[(size_t)(k) * L->N + (size_t)(((xx) + L->nx) % L->nx) + \
@@ -667,8 +694,13 @@ void orc_get_quantity(const orc_lattice* L, int which, double* out)
                     for (q = 0; q < 9; q++) { d += f[q]; jx += f[q] * L->e[q][0]; jy += f[q] * L->e[q][1]; }
                     if (which == 0) out[n] = d;
                     else if (which == 1) {
-                        out[3*n] = jx / d + L->params[3] * 0.5;
-                        out[3*n+1] = jy / d + L->params[4] * 0.5;
+                        double ux = jx / d, uy = jy / d;
+                        if (is_bc(L) && (nt & 0xF) == 0) {   /* + BC/2 off the BOUNDARY nodes */
+                            ux += L->extra[n] * 0.5;
+                            uy += L->extra[L->N + n] * 0.5;
+                        }
+                        out[3*n] = ux + L->params[3] * 0.5;
+                        out[3*n+1] = uy + L->params[4] * 0.5;
                         out[3*n+2] = 0;
                     } else out[n] = 0;
                 } else {

@@ -51,7 +51,7 @@ enum { ORC_D2Q9_SRT = 0, ORC_D2Q9_SRT_DRAG = 1, ORC_D3Q19 = 2, ORC_D3Q19_LID = 3
        ORC_D2Q9_DRAG = 5, ORC_D3Q27 = 6, ORC_D3Q27_CUMULANT = 7, ORC_D2Q9_AUTOSYM = 8, ORC_D2Q9_AUTOSYM2 = 9,
        ORC_D3Q27_CUMULANT_SMAG = 10, ORC_D3Q27_CUMULANT_IB = 11, ORC_D3Q27_CUMULANT_IB_SMAG = 12,
        ORC_D3Q27_CUMULANT_AVG = 13, ORC_D3Q27_CUMULANT_AVG_IB = 14, ORC_D3Q27_CUMULANT_AVG_SMAG = 15,
-       ORC_D3Q27_CUMULANT_AVG_IB_SMAG = 16 };
+       ORC_D3Q27_CUMULANT_AVG_IB_SMAG = 16, ORC_D2Q9_BC = 17, ORC_D2Q9_BC_AUTOSYM = 18, ORC_D2Q9_BC_AUTOSYM2 = 19 };
 enum { ORC_F64 = 0, ORC_F32 = 1, ORC_F32S = 2, ORC_F16S = 3, ORC_BF16S = 4 };
 
 /* d2q9 parameter vector:   [omega, Density]
@@ -88,13 +88,16 @@ void orc_set_flags(orc_lattice* L, const uint16_t* flags);
  * lattice, src/Lattice.cu.Rt:1056-1072); without a call every link reads
  * NO_CUT (src/LatticeAccess.inc.cpp.Rt:279-283). */
 void orc_set_cuts(orc_lattice* L, const uint16_t* cuts);
-/* The AVG builds: resetAverage (src/Lattice.cu.Rt:1360-1365), and the 13
- * averaged densities [k][N] (avgP, varUX, varUY, varUZ, varUXUY, varUXUZ,
- * varUYUZ, avgdxu2, avgdyv2, avgdzw2, avgUX, avgUY, avgUZ); -1 for a model
- * without them.  Their quantities are orc_get_quantity 5 KinE, 6 ReStr,
- * 7 Dissipation, 8 avgU, 9 varU, 10 averageP. */
+/* The densities that are not streamed, [k][N] in FP64: the AVG builds' 13
+ * averages (avgP, varUX, varUY, varUZ, varUXUY, varUXUZ, varUYUZ, avgdxu2,
+ * avgdyv2, avgdzw2, avgUX, avgUY, avgUZ; quantities 5 KinE, 6 ReStr,
+ * 7 Dissipation, 8 avgU, 9 varU, 10 averageP of orc_get_quantity) or the bc
+ * builds' BC[0], BC[1].  resetAverage (src/Lattice.cu.Rt:1360-1365) zeroes
+ * the averages. */
 void orc_reset_average(orc_lattice* L);
-int orc_get_averages(const orc_lattice* L, double* out);
+int orc_nextra(const orc_lattice* L);
+void orc_get_extra(const orc_lattice* L, double* out);
+void orc_set_extra(orc_lattice* L, const double* in);
 /* InterpolatedBounceBack alone on the pulled f of one node (FP64): cut[26]
  * of its links, old[27] its populations in the snapshot being read, and
  * Drag/Lateral/Lift added to globals[2], [4], [3] on Body nodes. */

@@ -17,6 +17,7 @@ LIB = os.path.join(HERE, "liboracle.so")
 D2Q9_SRT, D2Q9_SRT_DRAG, D3Q19, D3Q19_LID, D2Q9, D2Q9_DRAG, D3Q27, D3Q27_CUMULANT = 0, 1, 2, 3, 4, 5, 6, 7
 D2Q9_AUTOSYM, D2Q9_AUTOSYM2, D3Q27_CUMULANT_SMAG, D3Q27_CUMULANT_IB, D3Q27_CUMULANT_IB_SMAG = 8, 9, 10, 11, 12
 D3Q27_CUMULANT_AVG, D3Q27_CUMULANT_AVG_IB, D3Q27_CUMULANT_AVG_SMAG, D3Q27_CUMULANT_AVG_IB_SMAG = 13, 14, 15, 16
+D2Q9_BC, D2Q9_BC_AUTOSYM, D2Q9_BC_AUTOSYM2 = 17, 18, 19
 F64, F32, F32S, F16S, BF16S = 0, 1, 2, 3, 4
 MODELS = {"d2q9_SRT": D2Q9_SRT, "d2q9_SRT_drag": D2Q9_SRT_DRAG, "d3q19": D3Q19, "d3q19_lid": D3Q19_LID,
           "d2q9": D2Q9, "d2q9_drag": D2Q9_DRAG, "d3q27": D3Q27, "d3q27_cumulant": D3Q27_CUMULANT,
@@ -24,7 +25,8 @@ MODELS = {"d2q9_SRT": D2Q9_SRT, "d2q9_SRT_drag": D2Q9_SRT_DRAG, "d3q19": D3Q19, 
           "d3q27_cumulant_IB": D3Q27_CUMULANT_IB, "d3q27_cumulant_IB_SMAG": D3Q27_CUMULANT_IB_SMAG,
           "d3q27_cumulant_AVG": D3Q27_CUMULANT_AVG, "d3q27_cumulant_AVG_IB": D3Q27_CUMULANT_AVG_IB,
           "d3q27_cumulant_AVG_SMAG": D3Q27_CUMULANT_AVG_SMAG,
-          "d3q27_cumulant_AVG_IB_SMAG": D3Q27_CUMULANT_AVG_IB_SMAG}
+          "d3q27_cumulant_AVG_IB_SMAG": D3Q27_CUMULANT_AVG_IB_SMAG, "d2q9_bc": D2Q9_BC,
+          "d2q9_bc_autosym": D2Q9_BC_AUTOSYM, "d2q9_bc_autosym2": D2Q9_BC_AUTOSYM2}
 AVERAGES = ["avgP", "varUX", "varUY", "varUZ", "varUXUY", "varUXUZ", "varUYUZ", "avgdxu2", "avgdyv2", "avgdzw2",
             "avgUX", "avgUY", "avgUZ"]
 STORAGE = {"f64": F64, "f32": F32, "f32s": F32S, "f16s": F16S, "bf16s": BF16S}
@@ -53,7 +55,9 @@ def lib():
         L.orc_set_flags.argtypes = [vp, ctypes.POINTER(ctypes.c_uint16)]
         L.orc_set_cuts.argtypes = [vp, ctypes.POINTER(ctypes.c_uint16)]
         L.orc_reset_average.argtypes = [vp]
-        L.orc_get_averages.argtypes = [vp, dp]
+        L.orc_nextra.argtypes = [vp]
+        L.orc_get_extra.argtypes = [vp, dp]
+        L.orc_set_extra.argtypes = [vp, dp]
         L.orc_cum_ib.argtypes = [dp, ctypes.c_uint16, ctypes.POINTER(ctypes.c_uint16), dp, dp, dp]
         L.orc_init.argtypes = [vp]
         L.orc_iterate.argtypes = [vp, ip, ip, dp]
@@ -143,11 +147,20 @@ class OracleLattice:
     def reset_average(self):
         lib().orc_reset_average(self.h)
 
+    def extra(self):
+        """[k, nz, ny, nx] of the densities that are not streamed"""
+        a = np.empty((lib().orc_nextra(self.h),) + self.shape)
+        lib().orc_get_extra(self.h, _dp(a))
+        return a
+
+    def set_extra(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.size == lib().orc_nextra(self.h) * self.nx * self.ny * self.nz
+        lib().orc_set_extra(self.h, _dp(a))
+
     def averages(self):
         """dict name -> [nz, ny, nx] of the AVG builds' averaged densities"""
-        a = np.empty((13,) + self.shape)
-        assert lib().orc_get_averages(self.h, _dp(a)) == 0
-        return dict(zip(AVERAGES, a))
+        return dict(zip(AVERAGES, self.extra()))
 
     def quantity(self, which):
         w = {"Rho": 0, "U": 1, "P": 2, "Fd": 3, "Solid": 4, "KinE": 5, "ReStr": 6, "Dissipation": 7, "avgU": 8,
@@ -287,7 +300,8 @@ def lattice_for_case(case, storage="f64"):
         L.set_params([vals[n] for n in D3Q19_PARAMS])
     elif case.model == "d3q19_lid":
         L.set_params([vals[n] for n in D3Q19_LID_PARAMS])
-    elif case.model in ("d2q9", "d2q9_drag", "d2q9_autosym", "d2q9_autosym2"):
+    elif case.model in ("d2q9", "d2q9_drag", "d2q9_autosym", "d2q9_autosym2", "d2q9_bc", "d2q9_bc_autosym",
+                        "d2q9_bc_autosym2"):
         L.set_params([vals[n] for n in D2Q9_PARAMS])
     elif case.model == "d3q27":
         L.set_params([vals[n] for n in D3Q27_PARAMS])
@@ -312,4 +326,9 @@ def lattice_for_case(case, storage="f64"):
     L.init()
     if case.init_f is not None:
         L.set_f(case.init_f)
+    if getattr(case, "extra", None):
+        e = L.extra()
+        for n, v in case.extra.items():
+            e[m.extras.index(n)] = v
+        L.set_extra(e)
     return L

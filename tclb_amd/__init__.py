@@ -32,7 +32,7 @@ SYMBOLS = [
     "tclbx_set_zone_setting", "tclbx_flag_overwrite", "tclbx_get_flags", "tclbx_cuts_overwrite",
     "tclbx_get_cuts", "tclbx_init",
     "tclbx_iterate", "tclbx_get_globals", "tclbx_get_iteration", "tclbx_get_quantity",
-    "tclbx_get_density", "tclbx_set_density", "tclbx_average_name", "tclbx_reset_average", "tclbx_save",
+    "tclbx_get_density", "tclbx_set_density", "tclbx_extra_density_name", "tclbx_reset_average", "tclbx_save",
     "tclbx_load", "tclbx_get_timing",
     "tclbx_bytes_per_node", "tclbx_set_block", "tclbx_set_graph_steps", "tclbx_copy_bandwidth",
     "tclbx_set_rows_per_tile", "tclbx_get_rows_per_tile",
@@ -78,8 +78,7 @@ def lib():
     L.tclbx_model_info.argtypes = [c_char_p, ip, ip, ip, ip, ip]
     for n in ["tclbx_setting_id", "tclbx_zone_setting_id", "tclbx_global_id"]:
         getattr(L, n).argtypes = [c_char_p, c_char_p]
-    for n in ["tclbx_global_name", "tclbx_setting_name", "tclbx_zone_setting_name", "tclbx_quantity_name",
-              "tclbx_average_name"]:
+    for n in ["tclbx_global_name", "tclbx_setting_name", "tclbx_zone_setting_name", "tclbx_quantity_name"]:
         getattr(L, n).argtypes = [c_char_p, c_int]
         getattr(L, n).restype = c_char_p
     L.tclbx_setting_default.argtypes = [c_char_p, c_int, dp]
@@ -100,6 +99,8 @@ def lib():
     L.tclbx_get_quantity.argtypes = [vp, c_int] + [c_int] * 6 + [dp, c_double]
     L.tclbx_get_density.argtypes = [vp, c_int, dp]
     L.tclbx_reset_average.argtypes = [vp]
+    L.tclbx_extra_density_name.argtypes = [c_char_p, c_int, ip]
+    L.tclbx_extra_density_name.restype = c_char_p
     L.tclbx_set_density.argtypes = [vp, c_int, dp]
     L.tclbx_save.argtypes = [vp, c_char_p]
     L.tclbx_load.argtypes = [vp, c_char_p]
@@ -198,9 +199,15 @@ class Model:
         self.zonal_settings = [L.tclbx_zone_setting_name(b, i).decode() for i in range(self.nzonal)]
         self.global_names = [L.tclbx_global_name(b, i).decode() for i in range(self.nglobals)]
         self.quantities = [L.tclbx_quantity_name(b, i).decode() for i in range(self.nquantities)]
-        self.averages = []     # averaged densities (density Q + k)
-        while L.tclbx_average_name(b, len(self.averages)):
-            self.averages.append(L.tclbx_average_name(b, len(self.averages)).decode())
+        # densities that are not streamed (density Q + k): averages, then parameters
+        self.extras, self.averages, self.params = [], [], []
+        isavg = ctypes.c_int()
+        while True:
+            nm = L.tclbx_extra_density_name(b, len(self.extras), ctypes.byref(isavg))
+            if not nm:
+                break
+            self.extras.append(nm.decode())
+            (self.averages if isavg.value else self.params).append(nm.decode())
 
     def setting_id(self, name):
         return lib().tclbx_setting_id(self.name.encode(), name.encode())
@@ -395,14 +402,19 @@ class Lattice:
             plane = np.ascontiguousarray(a[q])
             _check(lib().tclbx_set_density(self._h, q, _dp(plane)))
 
+    def get_extra(self, name):
+        """A density that is not streamed (an average, BC[0], ...), [nz, ny, nx]."""
+        a = np.empty(self.shape, dtype=np.float64)
+        _check(lib().tclbx_get_density(self._h, self.Q + self.spec.extras.index(name), _dp(a)))
+        return a
+
+    def set_extra(self, name, values):
+        a = np.ascontiguousarray(np.broadcast_to(values, self.shape), dtype=np.float64)
+        _check(lib().tclbx_set_density(self._h, self.Q + self.spec.extras.index(name), _dp(a)))
+
     def get_averages(self):
         """The averaged densities (AVG builds), dict name -> [nz, ny, nx]."""
-        out = {}
-        for k, n in enumerate(self.spec.averages):
-            a = np.empty(self.shape, dtype=np.float64)
-            _check(lib().tclbx_get_density(self._h, self.Q + k, _dp(a)))
-            out[n] = a
-        return out
+        return {n: self.get_extra(n) for n in self.spec.averages}
 
     def reset_average(self):
         """Lattice::resetAverage: the averages to zero, counted from now."""

@@ -64,6 +64,7 @@ class Case:
     init_f: np.ndarray = None              # optional [Q, nz, ny, nx] float64
     steps: int = 1
     cuts: np.ndarray = None                # optional uint16 [26, nz, ny, nx] cut distances
+    extra: dict = None                     # optional {extra density name: [nz, ny, nx]}, set after Init
 
     @property
     def nodes(self):
@@ -86,6 +87,8 @@ class Case:
         lat.init()
         if self.init_f is not None:
             lat.set_f(self.init_f)
+        for n, v in (self.extra or {}).items():
+            lat.set_extra(n, v)
         return lat
 
 

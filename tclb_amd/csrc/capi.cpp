@@ -241,10 +241,12 @@ int tclbx_set_density(tclbx_lattice* h, int q, const double* in) {
     if (!in) return fail("null input");
     return guard([&]() { h->lat->SetDensity(q, in); return TCLBX_OK; });
 }
-const char* tclbx_average_name(const char* model, int k) {
+const char* tclbx_extra_density_name(const char* model, int k, int* is_average) {
     const ModelDef* m = M(model);
-    if (!m || k < 0 || k >= (int)m->averages.size()) return nullptr;
-    return m->averages[k].c_str();
+    if (!m || k < 0 || k >= m->nextra()) return nullptr;
+    const int na = (int)m->averages.size();
+    if (is_average) *is_average = k < na ? 1 : 0;
+    return k < na ? m->averages[k].c_str() : m->params[k - na].c_str();
 }
 int tclbx_reset_average(tclbx_lattice* h) {
     CHECK_H(h);

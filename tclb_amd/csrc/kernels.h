@@ -22,7 +22,8 @@ struct StepArgs {
     void* fout;            // next snapshot
     const uint16_t* flags; // NodeType per node, x fastest
     const uint16_t* cuts;  // cut distances [26][node] (IB builds), or null: no cuts
-    double* avg;           // averaged densities [k][node] (AVG builds), updated in place, or null
+    double* extra;         // densities that are not streamed, [k][node], FP64, updated in place
+                           // (averages of the AVG builds, BC[0..1] of the bc builds), or null
     double avg_n;          // average_iter = iterations since the last average reset (quantities)
     const double* zone;    // zonal settings [zone][zcols]
     double* gpart;         // per-block partial Globals [nglob][nblocks][2] (sum, compensation)

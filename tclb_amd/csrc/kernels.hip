@@ -60,8 +60,19 @@ template <> struct Lat<MODEL_D2Q9_AUTOSYM> : Lat<MODEL_D2Q9> {
 template <> struct Lat<MODEL_D2Q9_AUTOSYM2> : Lat<MODEL_D2Q9> {
     static constexpr int ZSHIFT = 11;
 };
-template <int MODEL> constexpr bool is_d2q9m =
-    MODEL == MODEL_D2Q9 || MODEL == MODEL_D2Q9_DRAG || MODEL == MODEL_D2Q9_AUTOSYM || MODEL == MODEL_D2Q9_AUTOSYM2;
+// the bc builds: the same NodeType bits, plus the parameter densities BC[0], BC[1]
+template <> struct Lat<MODEL_D2Q9_BC> : Lat<MODEL_D2Q9> {};
+template <> struct Lat<MODEL_D2Q9_BC_AUTOSYM> : Lat<MODEL_D2Q9_AUTOSYM> {};
+template <> struct Lat<MODEL_D2Q9_BC_AUTOSYM2> : Lat<MODEL_D2Q9_AUTOSYM2> {};
+template <int MODEL> constexpr bool is_d2q9_bc =
+    MODEL == MODEL_D2Q9_BC || MODEL == MODEL_D2Q9_BC_AUTOSYM || MODEL == MODEL_D2Q9_BC_AUTOSYM2;
+template <int MODEL> constexpr bool is_d2q9m = MODEL == MODEL_D2Q9 || MODEL == MODEL_D2Q9_DRAG ||
+                                              MODEL == MODEL_D2Q9_AUTOSYM || MODEL == MODEL_D2Q9_AUTOSYM2 ||
+                                              is_d2q9_bc<MODEL>;
+// 1: autosym (planes through the node), 2: autosym2 (planes half a node out)
+template <int MODEL> constexpr int autosym_kind =
+    (MODEL == MODEL_D2Q9_AUTOSYM || MODEL == MODEL_D2Q9_BC_AUTOSYM) ? 1
+    : (MODEL == MODEL_D2Q9_AUTOSYM2 || MODEL == MODEL_D2Q9_BC_AUTOSYM2) ? 2 : 0;
 template <> struct Lat<MODEL_D3Q19> {
     static constexpr int Q = 19, G = 26, SUMG = 25, ZSHIFT = 6;
     __host__ __device__ static constexpr int ex(int i) {
@@ -176,6 +187,9 @@ int kernel_nglobals(int model) {
     case MODEL_D3Q27_CUMULANT_AVG_IB_SMAG: return Lat<MODEL_D3Q27_CUMULANT_AVG_IB_SMAG>::G;
     case MODEL_D2Q9_AUTOSYM: return Lat<MODEL_D2Q9_AUTOSYM>::G;
     case MODEL_D2Q9_AUTOSYM2: return Lat<MODEL_D2Q9_AUTOSYM2>::G;
+    case MODEL_D2Q9_BC: return Lat<MODEL_D2Q9_BC>::G;
+    case MODEL_D2Q9_BC_AUTOSYM: return Lat<MODEL_D2Q9_BC_AUTOSYM>::G;
+    case MODEL_D2Q9_BC_AUTOSYM2: return Lat<MODEL_D2Q9_BC_AUTOSYM2>::G;
     }
     return 0;
 }
@@ -393,14 +407,31 @@ __device__ __forceinline__ void d2q9m_symmetry(R* f) {
 // InletFluxInObj, (d2q9_drag) DragInObj, LiftInObj]; Globals
 // [PressureLoss, OutletFlux, InletFlux, (DRAG) Drag, Lift, Objective].
 // DRAG adds the momentum exchange of Solid nodes, 2 sum_i f_i e_i of the
-// incoming populations (as d2q9_SRT_drag).
-template <class R, bool GLOB, bool DRAG>
-__device__ __forceinline__ void d2q9m_node(R* f, unsigned nt, const double* par, const double* zr, double* g) {
+// incoming populations (as d2q9_SRT_drag).  BCOPT: the bc builds, with the
+// node's BC[0] at bc[0] and BC[1] at bc[NN] (null: BC = 0, nothing kept).
+template <class R, bool GLOB, bool DRAG, bool BCOPT = false>
+__device__ __forceinline__ void d2q9m_node(R* f, unsigned nt, const double* par, const double* zr, double* g,
+                                           double* bc = nullptr, size_t NN = 0) {
     constexpr int OBJ = DRAG ? 5 : 3;
-    const R Vx = (R)zr[0], Vy = (R)zr[1], P = (R)zr[2];
+    R Vx = (R)zr[0], Vy = (R)zr[1];
+    const R P = (R)zr[2];
+    if constexpr (BCOPT) {  // BC_Velocity = (BC[0], BC[1]) (Dynamics.c.Rt:28-34)
+        Vx = bc ? (R)bc[0] : R(0);
+        Vy = bc ? (R)bc[NN] : R(0);
+    }
     switch (nt & 0xFu) {
     case 9:   // Solid
     case 10: {  // Wall: FullBounceBack
+        if constexpr (BCOPT) {
+            // BC = 2 sum f e of what is reversed, 0 when the node holds
+            // exactly 1 + 3 Pressure (Dynamics.c.Rt:146-159)
+            const R temp = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8];
+            const bool keep = temp != R(1) + P * R(3);
+            if (bc) {
+                bc[0] = keep ? (double)(R(2.) * (f[1] - f[3] + f[5] - f[6] - f[7] + f[8])) : 0.0;
+                bc[NN] = keep ? (double)(R(2.) * (f[2] - f[4] + f[5] + f[6] - f[7] - f[8])) : 0.0;
+            }
+        }
         if (DRAG && GLOB && (nt & 0xFu) == 9u) {
             const R fx = R(2.0) * (f[1] - f[3] + f[5] - f[6] - f[7] + f[8]);
             const R fy = R(2.0) * (f[2] - f[4] + f[5] + f[6] - f[7] - f[8]);
@@ -453,8 +484,17 @@ __device__ __forceinline__ void d2q9m_node(R* f, unsigned nt, const double* par,
     m[6] = S3 * m[6];
     m[7] = S3 * m[7];
     m[8] = S4 * m[8];
-    Jx = Jx + (R)par[3] * rho;
-    Jy = Jy + (R)par[4] * rho;
+    if constexpr (BCOPT) {
+        // only the nodes with no BOUNDARY type take the force, G + BC
+        // (Dynamics.c.Rt:254-258)
+        if ((nt & 0xFu) == 0u) {
+            Jx = Jx + ((R)par[3] + (bc ? (R)bc[0] : R(0))) * rho;
+            Jy = Jy + ((R)par[4] + (bc ? (R)bc[NN] : R(0))) * rho;
+        }
+    } else {
+        Jx = Jx + (R)par[3] * rho;
+        Jy = Jy + (R)par[4] * rho;
+    }
     m[1] = Jx;
     m[2] = Jy;
     m[3] = m[3] + R(3.) * Jx * Jx / rho;
@@ -1565,7 +1605,7 @@ __device__ __forceinline__ void block_globals(const double* g, int sum_globals, 
 template <int MODEL, class R, class LOAD>
 __device__ __forceinline__ void d2q9_autosym(R* f, unsigned nt, LOAD load) {
     const unsigned sx = (nt >> 7) & 3u, sy = (nt >> 9) & 3u;
-    if constexpr (MODEL == MODEL_D2Q9_AUTOSYM) {
+    if constexpr (autosym_kind<MODEL> == 1) {
         const bool xp = sx == 1u, xm = sx == 2u, yp = sy == 1u, ym = sy == 2u;
         const R a1 = f[1], a3 = f[3], a5 = f[5], a6 = f[6], a7 = f[7], a8 = f[8];
         f[3] = xp ? a1 : a3; f[6] = xp ? a5 : a6; f[7] = xp ? a8 : a7;
@@ -1636,7 +1676,7 @@ __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, 
         const T v = *(const T*)(fin + (size_t)i * pbytes + src * (unsigned)sizeof(T));
         f[i] = DEV ? S::raw(v) : S::get(v, (R)L::w(i));
     }
-    if constexpr (MODEL == MODEL_D2Q9_AUTOSYM || MODEL == MODEL_D2Q9_AUTOSYM2) {
+    if constexpr (autosym_kind<MODEL> != 0) {
         d2q9_autosym<MODEL>(f, nt, [&](int fi, int ox, int oy) {
             const int sxn = ox > 0 ? xp : (ox < 0 ? xm : xc);
             const int syn = oy > 0 ? yp : (oy < 0 ? ym : y);
@@ -1651,7 +1691,8 @@ __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, 
         } else if constexpr (is_d3q19<MODEL>) {
             d3q19_node<R, GLOB, MODEL == MODEL_D3Q19_LID>(f, nt, a.par, zr, g, a.flags, xc, y, z, nx, ny, nz);
         } else if constexpr (is_d2q9m<MODEL>) {
-            d2q9m_node<R, GLOB, MODEL == MODEL_D2Q9_DRAG>(f, nt, a.par, zr, g);
+            d2q9m_node<R, GLOB, MODEL == MODEL_D2Q9_DRAG, is_d2q9_bc<MODEL>>(
+                f, nt, a.par, zr, g, a.extra ? a.extra + n : nullptr, (size_t)nx * ny * nz);
         } else if constexpr (MODEL == MODEL_D3Q27) {
             d3q27_node<R, GLOB>(f, nt, a.par, zr, g);
         } else if constexpr (is_cum<MODEL>) {
@@ -1662,7 +1703,7 @@ __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, 
                     }, zr, g);
             }
             if constexpr (is_cum_avg<MODEL>)
-                cum_node<R, GLOB, is_cum_smag<MODEL>, true>(f, nt, a.par, zr, g, a.avg ? a.avg + n : nullptr,
+                cum_node<R, GLOB, is_cum_smag<MODEL>, true>(f, nt, a.par, zr, g, a.extra ? a.extra + n : nullptr,
                                                             (size_t)nx * ny * nz);
             else
                 cum_node<R, GLOB, is_cum_smag<MODEL>>(f, nt, a.par, zr, g);
@@ -1796,6 +1837,16 @@ __global__ void __launch_bounds__(256) k_init(StepArgs a) {
         R ux = (R)zr[0], uy = (R)zr[1];
         if ((nt & 0xFu) == 10u) { rho = R(1); ux = R(0); uy = R(0); }
         d2q9m_feq<R>(f, rho, ux * rho, uy * rho);
+        if constexpr (is_d2q9_bc<MODEL>) {
+            // SetEquilibrum of the bc builds: BC = J/rho on BOUNDARY nodes,
+            // else 0 (Dynamics.c.Rt:80-87)
+            if (a.extra != nullptr) {
+                const size_t NN = (size_t)a.nx * a.ny * a.nz;
+                const bool bnd = (nt & 0xFu) != 0u;
+                a.extra[n] = bnd ? (double)(ux * rho / rho) : 0.0;
+                a.extra[NN + n] = bnd ? (double)(uy * rho / rho) : 0.0;
+            }
+        }
     } else if constexpr (MODEL == MODEL_D3Q27) {
         // Init(), models/flow/d3q27/Dynamics.c.Rt:184-190: SetEquilibrum(1 +
         // 3 Pressure, Velocity + Turbulence*ST.x, ...) with no synthetic
@@ -1844,7 +1895,7 @@ __global__ void __launch_bounds__(256) k_quantity(StepArgs a, int qid, int x0, i
         const unsigned src = (unsigned)sx + (unsigned)nx * ((unsigned)sy + (unsigned)ny * (unsigned)sz);
         f[i] = (double)S::get(fin[(size_t)i * a.pitch + src], (typename S::R)L::w(i));
     }
-    if constexpr (MODEL == MODEL_D2Q9_AUTOSYM || MODEL == MODEL_D2Q9_AUTOSYM2) {
+    if constexpr (autosym_kind<MODEL> != 0) {
         d2q9_autosym<MODEL>(f, nt, [&](int fi, int ox, int oy) {
             const unsigned src = (unsigned)((x + ox + nx) % nx) + (unsigned)nx * (unsigned)((y + oy + ny) % ny);
             return (double)S::get(fin[(size_t)fi * a.pitch + src], (typename S::R)L::w(fi));
@@ -1914,7 +1965,7 @@ __global__ void __launch_bounds__(256) k_quantity(StepArgs a, int qid, int x0, i
             out[3 * tid + 2] = wall ? 0.0 : (jz + a.par[5] / 2.) / d * scale;
         } else if (qid == 2 || !is_cum_avg<MODEL>) {
             out[tid] = (wall ? 1.0 : 0.0) * scale;
-        } else if (a.avg == nullptr) {  // a lattice that keeps no averages
+        } else if (a.extra == nullptr) {  // a lattice that keeps no averages
             const bool vec = qid == 4 || qid == 6 || qid == 7;
             for (int k = 0; k < (vec ? 3 : 1); k++) out[(vec ? 3 : 1) * tid + k] = __builtin_nan("");
         } else {
@@ -1924,7 +1975,7 @@ __global__ void __launch_bounds__(256) k_quantity(StepArgs a, int qid, int x0, i
             const size_t NN = (size_t)nx * ny * nz;
             auto A = [&](int k, int xx, int yy, int zz) {
                 xx = (xx + nx) % nx; yy = (yy + ny) % ny; zz = (zz + nz) % nz;
-                return a.avg[(size_t)k * NN + (size_t)xx + (size_t)nx * ((size_t)yy + (size_t)ny * zz)];
+                return a.extra[(size_t)k * NN + (size_t)xx + (size_t)nx * ((size_t)yy + (size_t)ny * zz)];
             };
             const double it = a.avg_n;
             const double mx = A(10, x, y, z) / it, my = A(11, x, y, z) / it, mz = A(12, x, y, z) / it;
@@ -1965,8 +2016,17 @@ __global__ void __launch_bounds__(256) k_quantity(StepArgs a, int qid, int x0, i
         if (qid == 0) {
             out[tid] = d * scale;
         } else {
-            out[3 * tid] = (jx / d + a.par[3] * 0.5) * scale;
-            out[3 * tid + 1] = (jy / d + a.par[4] * 0.5) * scale;
+            double ux = jx / d, uy = jy / d;
+            if constexpr (is_d2q9_bc<MODEL>) {
+                // + BC/2 off the BOUNDARY nodes (Dynamics.c.Rt:49-53)
+                if ((nt & 0xFu) == 0u && a.extra != nullptr) {
+                    const size_t NN = (size_t)nx * ny * nz;
+                    ux += a.extra[n] * 0.5;
+                    uy += a.extra[NN + n] * 0.5;
+                }
+            }
+            out[3 * tid] = (ux + a.par[3] * 0.5) * scale;
+            out[3 * tid + 1] = (uy + a.par[4] * 0.5) * scale;
             out[3 * tid + 2] = 0.0;
         }
     } else {
@@ -2122,7 +2182,8 @@ hipError_t quantity_m(int st, int q, const StepArgs& a, int x0, int y0, int z0, 
 
 double weight(int model, int q) {
     if (model == MODEL_D3Q27) return Lat<MODEL_D3Q27>::w(q);
-    if (model >= MODEL_D3Q27_CUMULANT_IB || model == MODEL_D3Q27_CUMULANT || model == MODEL_D3Q27_CUMULANT_SMAG)
+    if ((model >= MODEL_D3Q27_CUMULANT_IB && model <= MODEL_D3Q27_CUMULANT_AVG_IB_SMAG) ||
+        model == MODEL_D3Q27_CUMULANT || model == MODEL_D3Q27_CUMULANT_SMAG)
         return Lat<MODEL_D3Q27_CUMULANT>::w(q);
     return (model == MODEL_D3Q19 || model == MODEL_D3Q19_LID) ? Lat<MODEL_D3Q19>::w(q) : Lat<MODEL_D2Q9_SRT>::w(q);
 }
@@ -2158,6 +2219,9 @@ hipError_t launch_step(int model, int storage, bool globals, const StepArgs& a, 
         return step_m<MODEL_D3Q27_CUMULANT_AVG_IB_SMAG>(storage, globals, a, block, s);
     case MODEL_D2Q9_AUTOSYM: return step_m<MODEL_D2Q9_AUTOSYM>(storage, globals, a, block, s);
     case MODEL_D2Q9_AUTOSYM2: return step_m<MODEL_D2Q9_AUTOSYM2>(storage, globals, a, block, s);
+    case MODEL_D2Q9_BC: return step_m<MODEL_D2Q9_BC>(storage, globals, a, block, s);
+    case MODEL_D2Q9_BC_AUTOSYM: return step_m<MODEL_D2Q9_BC_AUTOSYM>(storage, globals, a, block, s);
+    case MODEL_D2Q9_BC_AUTOSYM2: return step_m<MODEL_D2Q9_BC_AUTOSYM2>(storage, globals, a, block, s);
     }
     return hipErrorInvalidValue;
 }
@@ -2182,6 +2246,9 @@ hipError_t launch_init(int model, int storage, const StepArgs& a, int block, hip
     case MODEL_D3Q27_CUMULANT_AVG_IB_SMAG: return init_m<MODEL_D3Q27_CUMULANT_IB_SMAG>(storage, a, block, s);
     case MODEL_D2Q9_AUTOSYM: return init_m<MODEL_D2Q9_AUTOSYM>(storage, a, block, s);
     case MODEL_D2Q9_AUTOSYM2: return init_m<MODEL_D2Q9_AUTOSYM2>(storage, a, block, s);
+    case MODEL_D2Q9_BC: return init_m<MODEL_D2Q9_BC>(storage, a, block, s);
+    case MODEL_D2Q9_BC_AUTOSYM: return init_m<MODEL_D2Q9_BC_AUTOSYM>(storage, a, block, s);
+    case MODEL_D2Q9_BC_AUTOSYM2: return init_m<MODEL_D2Q9_BC_AUTOSYM2>(storage, a, block, s);
     }
     return hipErrorInvalidValue;
 }
@@ -2266,6 +2333,12 @@ hipError_t launch_batch_step(int model, int storage, bool globals, const BatchCa
         return batch_m<MODEL_D2Q9_AUTOSYM>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
     case MODEL_D2Q9_AUTOSYM2:
         return batch_m<MODEL_D2Q9_AUTOSYM2>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
+    case MODEL_D2Q9_BC:
+        return batch_m<MODEL_D2Q9_BC>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
+    case MODEL_D2Q9_BC_AUTOSYM:
+        return batch_m<MODEL_D2Q9_BC_AUTOSYM>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
+    case MODEL_D2Q9_BC_AUTOSYM2:
+        return batch_m<MODEL_D2Q9_BC_AUTOSYM2>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
     }
     return hipErrorInvalidValue;
 }
@@ -2303,6 +2376,11 @@ hipError_t launch_quantity(int model, int storage, int q, const StepArgs& a, int
         return quantity_m<MODEL_D2Q9_AUTOSYM>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
     case MODEL_D2Q9_AUTOSYM2:
         return quantity_m<MODEL_D2Q9_AUTOSYM2>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
+    case MODEL_D2Q9_BC: return quantity_m<MODEL_D2Q9_BC>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
+    case MODEL_D2Q9_BC_AUTOSYM:
+        return quantity_m<MODEL_D2Q9_BC_AUTOSYM>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
+    case MODEL_D2Q9_BC_AUTOSYM2:
+        return quantity_m<MODEL_D2Q9_BC_AUTOSYM2>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
     }
     return hipErrorInvalidValue;
 }

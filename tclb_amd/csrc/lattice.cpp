@@ -67,10 +67,10 @@ Lattice::Lattice(const ModelDef* m, int nx_, int ny_, int nz_, int storage_)
     }
     hip_check(hipMalloc((void**)&d_flags_, N * sizeof(uint16_t)), "hipMalloc(flags)");
     hip_check(hipMemsetAsync(d_flags_, 0, N * sizeof(uint16_t), stream_), "hipMemset(flags)");
-    if (!m->averages.empty()) {
-        const size_t ab = m->averages.size() * N * sizeof(double);
-        hip_check(hipMalloc((void**)&d_avg_, ab), "hipMalloc(averages)");
-        hip_check(hipMemsetAsync(d_avg_, 0, ab, stream_), "hipMemset(averages)");
+    if (m->nextra() > 0) {
+        const size_t ab = (size_t)m->nextra() * N * sizeof(double);
+        hip_check(hipMalloc((void**)&d_extra_, ab), "hipMalloc(extra densities)");
+        hip_check(hipMemsetAsync(d_extra_, 0, ab, stream_), "hipMemset(extra densities)");
     }
     hip_check(hipMalloc((void**)&d_zone_, std::max<size_t>(1, zone_.size()) * sizeof(double)), "hipMalloc(zone)");
     set_block(pick_block(nx));
@@ -87,7 +87,7 @@ Lattice::~Lattice() {
     for (void* p : snaps_) if (p) (void)hipFree(p);
     if (d_flags_) (void)hipFree(d_flags_);
     if (d_cuts_) (void)hipFree(d_cuts_);
-    if (d_avg_) (void)hipFree(d_avg_);
+    if (d_extra_) (void)hipFree(d_extra_);
     if (d_zone_) (void)hipFree(d_zone_);
     if (d_gpart_) (void)hipFree(d_gpart_);
     if (d_glob_) (void)hipFree(d_glob_);
@@ -111,8 +111,8 @@ void Lattice::set_block(int b) {
 }
 
 size_t Lattice::bytes_per_node_step() const {
-    // the averaged densities are read and written in place on collision nodes
-    return (size_t)2 * model->Q * storage_bytes(storage) + sizeof(uint16_t) + 2 * model->averages.size() * sizeof(double);
+    // the extra densities are read and written in place
+    return (size_t)2 * model->Q * storage_bytes(storage) + sizeof(uint16_t) + 2 * (size_t)model->nextra() * sizeof(double);
 }
 
 void Lattice::SetSetting(int id, double v) {
@@ -231,7 +231,7 @@ StepArgs Lattice::args(int from, int to, bool glob) const {
     a.fout = snaps_[to];
     a.flags = d_flags_;
     a.cuts = d_cuts_;
-    a.avg = d_avg_;
+    a.extra = d_extra_;
     a.avg_n = (double)(iter_ - reset_iter_);
     a.zone = d_zone_;
     a.gpart = glob ? d_gpart_ : nullptr;
@@ -280,7 +280,8 @@ void model_params(const ModelDef* model, const std::vector<double>& settings, do
         const char* names[] = {"omega", "Smag", "ForceX", "ForceY", "ForceZ"};
         for (int i = 0; i < 5; i++) par[i] = S(names[i]);
     } else if (model->id == MODEL_D2Q9 || model->id == MODEL_D2Q9_DRAG || model->id == MODEL_D2Q9_AUTOSYM ||
-               model->id == MODEL_D2Q9_AUTOSYM2) {
+               model->id == MODEL_D2Q9_AUTOSYM2 || model->id == MODEL_D2Q9_BC || model->id == MODEL_D2Q9_BC_AUTOSYM ||
+               model->id == MODEL_D2Q9_BC_AUTOSYM2) {
         // [S2, S3, S4, GravitationX, GravitationY]: the relaxation factors of
         // the 2nd, 3rd and 4th order moments and the body force
         par[0] = S("S2");
@@ -347,18 +348,19 @@ void Lattice::Init() {
     snap_ = 0;
     StepArgs a = args(1, 0, false);
     hip_check(launch_init(model->id, storage, a, block_, stream_), "init kernel");
-    // the averaged densities start from their default, 0 (src/conf.R:70);
-    // reset_iter is left as it is, as Init leaves it there
-    if (d_avg_)
-        hip_check(hipMemsetAsync(d_avg_, 0, model->averages.size() * N * sizeof(double), stream_), "Init(averages)");
+    // the averaged densities start from their default, 0 (src/conf.R:70),
+    // the init kernel sets the parameter densities; reset_iter is left as
+    // it is, as Init leaves it there
+    if (!model->averages.empty())
+        hip_check(hipMemsetAsync(d_extra_, 0, model->averages.size() * N * sizeof(double), stream_), "Init(averages)");
     hip_check(hipStreamSynchronize(stream_), "Init");
     clearGlobals();
 }
 
 void Lattice::resetAverage() {
     reset_iter_ = iter_;
-    if (d_avg_) {
-        hip_check(hipMemsetAsync(d_avg_, 0, model->averages.size() * N * sizeof(double), stream_), "resetAverage");
+    if (!model->averages.empty()) {
+        hip_check(hipMemsetAsync(d_extra_, 0, model->averages.size() * N * sizeof(double), stream_), "resetAverage");
         hip_check(hipStreamSynchronize(stream_), "resetAverage");
     }
 }
@@ -477,9 +479,9 @@ void Lattice::GetQuantity(int q, const Region& over, double* out, double scale) 
 }
 
 void Lattice::GetDensity(int q, double* out) {
-    if (q >= model->Q && q < model->Q + (int)model->averages.size()) {
-        hip_check(hipMemcpy(out, d_avg_ + (size_t)(q - model->Q) * N, N * sizeof(double), hipMemcpyDeviceToHost),
-                  "GetDensity(average)");
+    if (q >= model->Q && q < model->Q + model->nextra()) {
+        hip_check(hipMemcpy(out, d_extra_ + (size_t)(q - model->Q) * N, N * sizeof(double), hipMemcpyDeviceToHost),
+                  "GetDensity(extra)");
         return;
     }
     if (q < 0 || q >= model->Q) throw std::out_of_range("density id");
@@ -494,9 +496,9 @@ void Lattice::GetDensity(int q, double* out) {
 }
 
 void Lattice::SetDensity(int q, const double* in) {
-    if (q >= model->Q && q < model->Q + (int)model->averages.size()) {
-        hip_check(hipMemcpy(d_avg_ + (size_t)(q - model->Q) * N, in, N * sizeof(double), hipMemcpyHostToDevice),
-                  "SetDensity(average)");
+    if (q >= model->Q && q < model->Q + model->nextra()) {
+        hip_check(hipMemcpy(d_extra_ + (size_t)(q - model->Q) * N, in, N * sizeof(double), hipMemcpyHostToDevice),
+                  "SetDensity(extra)");
         return;
     }
     if (q < 0 || q >= model->Q) throw std::out_of_range("density id");
@@ -524,11 +526,11 @@ int Lattice::Save(const std::string& path) {
                   "Save");
         if (std::fwrite(host.data(), 1, host.size(), f) != host.size()) rc = -1;
     }
-    // then the averaged densities (FP64), as the reference's saveSolution
+    // then the extra densities (FP64), as the reference's saveSolution
     // writes every density of the snapshot
     std::vector<double> av(N);
-    for (size_t k = 0; k < model->averages.size() && rc == 0; k++) {
-        hip_check(hipMemcpy(av.data(), d_avg_ + k * N, N * sizeof(double), hipMemcpyDeviceToHost), "Save");
+    for (size_t k = 0; k < (size_t)model->nextra() && rc == 0; k++) {
+        hip_check(hipMemcpy(av.data(), d_extra_ + k * N, N * sizeof(double), hipMemcpyDeviceToHost), "Save");
         if (std::fwrite(av.data(), sizeof(double), N, f) != N) rc = -1;
     }
     std::fclose(f);
@@ -547,9 +549,9 @@ int Lattice::Load(const std::string& path) {
                   "Load");
     }
     std::vector<double> av(N);
-    for (size_t k = 0; k < model->averages.size() && rc == 0; k++) {
+    for (size_t k = 0; k < (size_t)model->nextra() && rc == 0; k++) {
         if (std::fread(av.data(), sizeof(double), N, f) != N) { rc = -1; break; }
-        hip_check(hipMemcpy(d_avg_ + k * N, av.data(), N * sizeof(double), hipMemcpyHostToDevice), "Load");
+        hip_check(hipMemcpy(d_extra_ + k * N, av.data(), N * sizeof(double), hipMemcpyHostToDevice), "Load");
     }
     std::fclose(f);
     return rc;

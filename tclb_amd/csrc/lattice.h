@@ -56,7 +56,8 @@ public:
     const std::vector<double>& globals() const { return globals_; }
 
     void GetQuantity(int q, const Region& over, double* out, double scale);
-    // Get_<f> of the current snapshot; q >= Q: averaged density q - Q
+    // Get_<f> of the current snapshot; q >= Q: extra density q - Q (the
+    // model's averages, then its parameter densities)
     void GetDensity(int q, double* out);
     void SetDensity(int q, const double* in); // Set_<f>
     int Save(const std::string& path);
@@ -96,7 +97,7 @@ private:
     int snap_ = 0, iter_ = 0;
     uint16_t* d_flags_ = nullptr;
     uint16_t* d_cuts_ = nullptr;
-    double* d_avg_ = nullptr;   // model->averages.size() FP64 planes of N
+    double* d_extra_ = nullptr; // model->nextra() FP64 planes of N: averages, then parameters
     int reset_iter_ = 0;
     double* d_zone_ = nullptr;
     double* d_gpart_ = nullptr;

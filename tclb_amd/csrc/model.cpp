@@ -199,11 +199,20 @@ ModelDef make_d3q19(bool lid) {
 // (models/flow/d2q9/conf.mk OPT="bc*autosym", src/conf.R:373-445): NodeType
 // groups SYMX and SYMY with <name>X_plus/_minus, <name>Y_plus/_minus, name
 // Symmetry (plane through the node) or SymmetryEdge (plane half a node out),
-// acting in the streaming step.
-ModelDef make_d2q9_mrt(bool drag, int autosym = 0) {
+// acting in the streaming step.  bc: the bc builds (d2q9_bc,
+// d2q9_bc_autosym, d2q9_bc_autosym2): parameter densities BC[0], BC[1]
+// (Dynamics.R:23-26) that the velocity conditions read, Wall/Solid nodes
+// fill with the momentum they reverse, and that force the fluid nodes
+// (Dynamics.c.Rt:28-34,49-53,80-87,146-159,254-261).
+ModelDef make_d2q9_mrt(bool drag, int autosym = 0, bool bc = false) {
     ModelDef m;
-    m.id = drag ? MODEL_D2Q9_DRAG : (autosym == 1 ? MODEL_D2Q9_AUTOSYM : autosym == 2 ? MODEL_D2Q9_AUTOSYM2 : MODEL_D2Q9);
-    m.name = drag ? "d2q9_drag" : (autosym == 1 ? "d2q9_autosym" : autosym == 2 ? "d2q9_autosym2" : "d2q9");
+    static const ModelId plain[3] = {MODEL_D2Q9, MODEL_D2Q9_AUTOSYM, MODEL_D2Q9_AUTOSYM2};
+    static const ModelId withbc[3] = {MODEL_D2Q9_BC, MODEL_D2Q9_BC_AUTOSYM, MODEL_D2Q9_BC_AUTOSYM2};
+    m.id = drag ? MODEL_D2Q9_DRAG : (bc ? withbc[autosym] : plain[autosym]);
+    // option builds are named model_<options in conf.mk's order, bc*autosym>
+    m.name = drag ? "d2q9_drag"
+                  : std::string("d2q9") + (bc ? "_bc" : "") +
+                        (autosym == 1 ? "_autosym" : autosym == 2 ? "_autosym2" : "");
     m.Q = 9; m.dim = 2;
     m.ex = {0, 1, 0, -1, 0, 1, -1, -1, 1};
     m.ey = {0, 0, 1, 0, -1, 1, 1, -1, -1};
@@ -224,6 +233,7 @@ ModelDef make_d2q9_mrt(bool drag, int autosym = 0) {
     }
     finish_globals(m, g);
     m.quantities = {{"Rho", false, "kg/m3"}, {"U", true, "m/s"}};
+    if (bc) m.params = {"BC[0]", "BC[1]"};
     std::vector<std::pair<std::string, std::string>> types = {
         {"EPressure", "BOUNDARY"}, {"WPressure", "BOUNDARY"}, {"NVelocity", "BOUNDARY"},
         {"SVelocity", "BOUNDARY"}, {"WVelocity", "BOUNDARY"}, {"EVelocity", "BOUNDARY"},
@@ -342,7 +352,8 @@ const std::vector<ModelDef>& registry() {
                                       make_d3q27_cumulant(true), make_d3q27_cumulant(false, true),
                                       make_d3q27_cumulant(true, true), make_d3q27_cumulant(false, false, true),
                                       make_d3q27_cumulant(false, true, true), make_d3q27_cumulant(true, false, true),
-                                      make_d3q27_cumulant(true, true, true)};
+                                      make_d3q27_cumulant(true, true, true), make_d2q9_mrt(false, 0, true),
+                                      make_d2q9_mrt(false, 1, true), make_d2q9_mrt(false, 2, true)};
     return r;
 }
 

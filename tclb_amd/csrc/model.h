@@ -36,7 +36,10 @@ enum ModelId {
     MODEL_D3Q27_CUMULANT_AVG_IB = 14,
     MODEL_D3Q27_CUMULANT_AVG_SMAG = 15,
     MODEL_D3Q27_CUMULANT_AVG_IB_SMAG = 16,
-    MODEL_COUNT = 17
+    MODEL_D2Q9_BC = 17,
+    MODEL_D2Q9_BC_AUTOSYM = 18,
+    MODEL_D2Q9_BC_AUTOSYM2 = 19,
+    MODEL_COUNT = 20
 };
 
 // storage format of the distributions in HBM
@@ -84,9 +87,13 @@ struct ModelDef {
     std::vector<SettingDef> zonal;    // zonal settings (incl. <Global>InObj)
     std::vector<GlobalDef> globals;   // SUM..., Objective, MAX...
     std::vector<QuantityDef> quantities;
-    // densities added with average=TRUE (not streamed, reset by <Average>):
-    // kept in FP64 planes beside the distributions, in this order
+    // densities that are not streamed, kept in FP64 planes beside the
+    // distributions and updated in place (densities Q, Q+1, ...): first
+    // those added with average=TRUE (zeroed by resetAverage), then the
+    // parameter densities (parameter=TRUE, set from outside)
     std::vector<std::string> averages;
+    std::vector<std::string> params;
+    int nextra() const { return (int)(averages.size() + params.size()); }
     std::vector<NodeTypeDef> nodetypes;
     std::vector<GroupDef> groups;
     int zone_shift = 0, zone_max = 0, flag_bits = 16;

@@ -31,7 +31,8 @@ def test_models():
     assert tclb_amd.models() == ["d2q9_SRT", "d2q9_SRT_drag", "d3q19", "d3q19_lid", "d2q9", "d2q9_drag", "d3q27",
                                  "d3q27_cumulant", "d2q9_autosym", "d2q9_autosym2", "d3q27_cumulant_SMAG",
                                  "d3q27_cumulant_IB", "d3q27_cumulant_IB_SMAG", "d3q27_cumulant_AVG",
-                                 "d3q27_cumulant_AVG_IB", "d3q27_cumulant_AVG_SMAG", "d3q27_cumulant_AVG_IB_SMAG"]
+                                 "d3q27_cumulant_AVG_IB", "d3q27_cumulant_AVG_SMAG", "d3q27_cumulant_AVG_IB_SMAG",
+                                 "d2q9_bc", "d2q9_bc_autosym", "d2q9_bc_autosym2"]
 
 
 def test_d2q9_mrt_contract():

@@ -65,7 +65,8 @@ def random_case(model, nx, ny, nz, seed, steps=10):
     rng = np.random.default_rng(seed)
     m = tclb_amd.Model(model)
     zs = m.group("SETTINGZONE")[1]
-    if model in ("d2q9", "d2q9_drag", "d2q9_autosym", "d2q9_autosym2"):
+    if model in ("d2q9", "d2q9_drag", "d2q9_autosym", "d2q9_autosym2", "d2q9_bc", "d2q9_bc_autosym",
+                 "d2q9_bc_autosym2"):
         MRT = m.flag("MRT")
         kinds = [MRT] * 12 + [m.flag("Wall"), m.flag("Solid"), m.flag("Wall", "MRT"), m.flag("Solid", "MRT")]
         for b in ("EPressure", "WPressure", "NVelocity", "SVelocity", "WVelocity", "EVelocity", "NSymmetry",
@@ -151,14 +152,18 @@ def random_case(model, nx, ny, nz, seed, steps=10):
         cuts = cuts.astype(np.uint16).reshape(26, nz, ny, nx)
     if model.startswith("d3q19"):
         flags |= (rng.integers(0, 4, n) * ((flags & 0x20) > 0)).astype(np.uint16)  # slices on MRT nodes
-    if model.startswith("d2q9_autosym"):   # SYMX / SYMY planes on a third of the nodes each
+    if "autosym" in model:   # SYMX / SYMY planes on a third of the nodes each
         flags |= ((rng.integers(0, 3, n) * (rng.random(n) < 0.33)) << 7).astype(np.uint16)
         flags |= ((rng.integers(0, 3, n) * (rng.random(n) < 0.33)) << 9).astype(np.uint16)
     flags |= (rng.integers(0, 4, n) << zs).astype(np.uint16)
     flags = flags.reshape(nz, ny, nx)
     Q = len(w)
     f = w[:, None, None, None] * (1 + 0.02 * rng.standard_normal((Q, nz, ny, nx)))
-    return cases.Case("random", model, nx, ny, nz, flags, settings, zonal, init_f=f, steps=steps, cuts=cuts)
+    extra = None
+    if model.startswith("d2q9_bc"):   # BC on every node: velocities of the velocity conditions, forces elsewhere
+        extra = {"BC[0]": 1e-4 * rng.standard_normal((nz, ny, nx)), "BC[1]": 1e-4 * rng.standard_normal((nz, ny, nx))}
+    return cases.Case("random", model, nx, ny, nz, flags, settings, zonal, init_f=f, steps=steps, cuts=cuts,
+                      extra=extra)
 
 
 MODELS = [("d2q9_SRT", 24, 10, 1), ("d2q9_SRT_drag", 37, 9, 1), ("d3q19", 13, 7, 6), ("d3q19_lid", 11, 9, 7),
@@ -166,7 +171,8 @@ MODELS = [("d2q9_SRT", 24, 10, 1), ("d2q9_SRT_drag", 37, 9, 1), ("d3q19", 13, 7,
           ("d2q9_autosym", 17, 11, 1), ("d2q9_autosym2", 19, 12, 1), ("d3q27_cumulant_SMAG", 9, 8, 6),
           ("d3q27_cumulant_IB", 10, 9, 5), ("d3q27_cumulant_IB_SMAG", 9, 7, 6), ("d3q27_cumulant_AVG", 8, 7, 6),
           ("d3q27_cumulant_AVG_IB", 7, 9, 5), ("d3q27_cumulant_AVG_SMAG", 9, 6, 5),
-          ("d3q27_cumulant_AVG_IB_SMAG", 7, 8, 5)]
+          ("d3q27_cumulant_AVG_IB_SMAG", 7, 8, 5), ("d2q9_bc", 21, 12, 1), ("d2q9_bc_autosym", 18, 13, 1),
+          ("d2q9_bc_autosym2", 20, 11, 1)]
 
 
 @pytest.mark.parametrize("model,nx,ny,nz", MODELS)
@@ -177,7 +183,8 @@ def test_random_case_f64(oracle_mod, model, nx, ny, nz):
                              "d2q9_drag": 16, "d3q27": 18, "d3q27_cumulant": 19, "d2q9_autosym": 20,
                              "d2q9_autosym2": 21, "d3q27_cumulant_SMAG": 22, "d3q27_cumulant_IB": 23,
                              "d3q27_cumulant_IB_SMAG": 24, "d3q27_cumulant_AVG": 25, "d3q27_cumulant_AVG_IB": 26,
-                             "d3q27_cumulant_AVG_SMAG": 27, "d3q27_cumulant_AVG_IB_SMAG": 28}[model])
+                             "d3q27_cumulant_AVG_SMAG": 27, "d3q27_cumulant_AVG_IB_SMAG": 28, "d2q9_bc": 29,
+                             "d2q9_bc_autosym": 30, "d2q9_bc_autosym2": 31}[model])
     lat = case.build("f64")
     orc = oracle_for_case(oracle_mod, case, "f64")
     for n in (1, 6, 13):
@@ -185,6 +192,9 @@ def test_random_case_f64(oracle_mod, model, nx, ny, nz):
         g_orc = orc.iterate(n)
         assert rel_max(lat.get_f(), orc.get_f()) <= 1e-12, n
         glob_close(lat.globals, g_orc[: lat.nglobals], 1e-10)
+        ex = orc.extra() if lat.spec.extras else []
+        for k, name in enumerate(lat.spec.extras):   # averages, BC[0..1]
+            assert rel_max(lat.get_extra(name), ex[k]) <= 1e-12, (n, name)
     assert lat.iteration == 20
 
 
