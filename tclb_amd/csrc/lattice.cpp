@@ -111,8 +111,10 @@ void Lattice::set_block(int b) {
 }
 
 size_t Lattice::bytes_per_node_step() const {
-    // the extra densities are read and written in place
-    return (size_t)2 * model->Q * storage_bytes(storage) + sizeof(uint16_t) + 2 * (size_t)model->nextra() * sizeof(double);
+    // the averages are read and written in place on collision nodes; the
+    // parameter densities are read (only wall nodes write theirs)
+    return (size_t)2 * model->Q * storage_bytes(storage) + sizeof(uint16_t) +
+           (2 * model->averages.size() + model->params.size()) * sizeof(double);
 }
 
 void Lattice::SetSetting(int id, double v) {

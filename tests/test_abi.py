@@ -171,3 +171,14 @@ def test_d3q27_cumulant_contract():
     s = tclb_amd.Model("d3q27_cumulant_SMAG")
     assert s.settings == m.settings[:-1] + ["Smag", "Threshold"]
     assert s.global_names == m.global_names and s.group("SETTINGZONE") == m.group("SETTINGZONE")
+
+
+def test_modelbench_covers_every_model():
+    """python -m tclb_amd.modelbench can set up a box of every model: its
+    collision NodeType and settings exist in each model's contract."""
+    from tclb_amd import modelbench
+    for name in tclb_amd.models():
+        m = tclb_amd.Model(name)
+        assert m.flag(modelbench.COLLISION.get(name, "MRT")) != 0, name
+        for k, _ in modelbench.settings_of(name):
+            assert m.setting_id(k) >= 0, (name, k)
