@@ -57,10 +57,22 @@ void Arena::release(size_t offset) {
 }
 
 // ---------------------------------------------------------------- batch ---
-Batch::Batch(const ModelDef* m, int storage_, size_t capacity_bytes)
-    : model(m), storage(storage_), arena_(capacity_bytes) {
+namespace {
+// the arena holds distributions, flags and zone tables: a model with
+// densities that are not streamed (the AVG builds' averages, the bc builds'
+// BC) runs on a Lattice.  Checked before the arena is allocated.
+const ModelDef* batchable(const ModelDef* m, int storage) {
     if (!m) throw std::invalid_argument("unknown model");
     if (storage < 0 || storage >= ST_COUNT) throw std::invalid_argument("unknown storage format");
+    if (m->nextra() > 0)
+        throw std::invalid_argument("model " + m->name +
+                                    " keeps densities that are not streamed, which a batch does not hold; use a lattice");
+    return m;
+}
+}  // namespace
+
+Batch::Batch(const ModelDef* m, int storage_, size_t capacity_bytes)
+    : model(batchable(m, storage_)), storage(storage_), arena_(capacity_bytes) {
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     hip_check(hipEventCreate(&ev0_), "hipEventCreate");
     hip_check(hipEventCreate(&ev1_), "hipEventCreate");

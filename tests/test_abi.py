@@ -182,3 +182,12 @@ def test_modelbench_covers_every_model():
         assert m.flag(modelbench.COLLISION.get(name, "MRT")) != 0, name
         for k, _ in modelbench.settings_of(name):
             assert m.setting_id(k) >= 0, (name, k)
+
+
+@pytest.mark.parametrize("model", ["d3q27_cumulant_AVG", "d2q9_bc_autosym"])
+def test_batch_refuses_models_with_extra_densities(model):
+    """A batch's arena holds distributions, flags and zone tables only; the
+    AVG and bc builds keep per-node densities that are not streamed, so a
+    batch (and the server built on it) refuses them before allocating."""
+    with pytest.raises(tclb_amd.TclbxError, match="use a lattice"):
+        tclb_amd.Batch(model, "f64", 1 << 20)
