@@ -73,6 +73,7 @@ const ModelDef* batchable(const ModelDef* m, int storage) {
 
 Batch::Batch(const ModelDef* m, int storage_, size_t capacity_bytes)
     : model(batchable(m, storage_)), storage(storage_), arena_(capacity_bytes) {
+    graph_steps_ = default_graph_steps();
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     hip_check(hipEventCreate(&ev0_), "hipEventCreate");
     hip_check(hipEventCreate(&ev1_), "hipEventCreate");

@@ -29,8 +29,14 @@ int pick_block(int nx) {
 }
 }  // namespace
 
+int default_graph_steps() {
+    if (const char* e = std::getenv("TCLBX_GRAPH_STEPS")) return std::max(0, std::atoi(e));
+    return 32;
+}
+
 Lattice::Lattice(const ModelDef* m, int nx_, int ny_, int nz_, int storage_)
     : model(m), nx(nx_), ny(ny_), nz(nz_), storage(storage_), N((size_t)nx_ * ny_ * nz_) {
+    graph_steps_ = default_graph_steps();
     if (!m) throw std::invalid_argument("unknown model");
     if (nx <= 0 || ny <= 0 || nz <= 0) throw std::invalid_argument("lattice size must be positive");
     if (m->dim == 2 && nz != 1) throw std::invalid_argument("2D model needs nz = 1");

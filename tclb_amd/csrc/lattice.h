@@ -118,6 +118,10 @@ private:
 
 // hipError_t -> exception with a message naming the call
 void hip_check(hipError_t e, const char* what);
+// steps per captured graph of a new Lattice or Batch: 32, or the
+// environment's TCLBX_GRAPH_STEPS (0 or 1: every step launched directly,
+// for tools that do not follow graph launches)
+int default_graph_steps();
 // kernel parameter vector (StepArgs::par) of a model from its settings
 void model_params(const ModelDef* model, const std::vector<double>& settings, double* par);
 // rows walked by one block of a Globals step for a lattice of `rows` rows
