@@ -15,9 +15,15 @@ numpy over each primitive's box of nodes), what the reference computes:
   * loadSTL / transformSTL / calcCut (src/Geometry.cpp.Rt:381-686): binary
     STL meshes, side in/out by ray parity along ray_axis, side surface by
     the 26 links of each node against each triangle (GaussSolve of
-    src/unit.cpp:13-49, batched over numpy arrays);
+    src/unit.cpp:13-49, batched over numpy arrays), keeping each link's
+    nearest crossing as its cut distance;
   * loadSweep (src/Geometry.cpp.Rt:688-755) with the clamped B-spline of
     src/spline.h:9-43.
+
+Pinned in tests/test_geometry.py by hand-derived answers and closed forms
+(each test names what it pins).  Parity unpinned: the STL tie-break for a
+ray through an edge shared by two triangles, which only engine-oracle
+agreement exercises.
 """
 import math
 import xml.etree.ElementTree as ET

@@ -21,8 +21,15 @@
  *   d2q9_autosym, d2q9_autosym2
  *                  d2q9 with the symmetric access of its autosym builds
  *                  (src/conf.R:373-445, src/LatticeAccess.inc.cpp.Rt:233-470)
- *   d3q27_cumulant models/flow/d3q27_cumulant/Dynamics.c.Rt, Dynamics.R without
- *                  the AVG/IB/SMAG build options
+ *   d2q9_bc, d2q9_bc_autosym, d2q9_bc_autosym2
+ *                  d2q9's bc builds: parameter densities BC[0], BC[1]
+ *                  (models/flow/d2q9/Dynamics.R:23-26, Dynamics.c.Rt:28-34,
+ *                  49-53,80-87,146-159,254-261)
+ *   d3q27_cumulant models/flow/d3q27_cumulant/Dynamics.c.Rt, Dynamics.R, and
+ *                  its option builds: SMAG (Smagorinsky, :362-374), IB
+ *                  (interpolated bounce-back on cut distances, :180-222), AVG
+ *                  (averaged densities, Dynamics.R:66-90, Dynamics.c.Rt:62-99,
+ *                  410-414,507-518), in every combination
  *   d3q19_lid      d3q19 plus a MovingWall node type (bounce-back with Ladd's
  *                  wall-velocity term) and its force/torque Globals; the
  *                  reference has no moving wall, so this part is pinned by
@@ -38,6 +45,20 @@
  *
  * Globals are accumulated in long double (x87 80-bit) so the oracle's sums
  * are at least as accurate as any summation order of the device.
+ *
+ * Pinned in tests/test_oracle_pins.py by closed forms, independent
+ * derivations and brute force (each test names what it pins).  Parity
+ * unpinned -- nothing the reference holds fixes these, since its golden
+ * data (tests/external) is absent and its generator (R polyAlgebra) cannot
+ * run here:
+ *   - the floating-point operation order that polyAlgebra's ToC emits for
+ *     the generated models (d3q19 CollisionMRT / SetEquilibrum / Zou-He, the
+ *     d2q9 MRT, d3q27 and cumulant collisions and Zou-He); the oracle follows
+ *     the dense forms in ascending index order, within the FP64 tolerance;
+ *   - the cumulant model's Galilean-correction coefficients beyond the
+ *     identities the pins check (zero at rest, diagonal only);
+ *   - the reference float build's literal rewrite (Dynamics_sp.c,
+ *     src/makefile.main.Rt:383-386), which the reduced formats do not follow.
  */
 #ifndef TCLBX_ORACLE_H
 #define TCLBX_ORACLE_H
