@@ -21,6 +21,7 @@
 // reference's per-warp atomicAdd, src/cross.hpp:202-218).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
 #include <utility>
 
 #include "kernels.h"
@@ -322,9 +323,69 @@ __device__ constexpr double kB9i[9][9] = {
     {0, -1. / 6, 0, 1. / 6, 0, 1. / 12, -1. / 12, -1. / 12, 1. / 12},
     {1. / 9, -1. / 18, -1. / 18, -1. / 18, -1. / 18, 1. / 36, 1. / 36, 1. / 36, 1. / 36}};
 
-// f = R B^-1 over the nonzero coefficients in ascending k
+// The basis is a tensor product: per axis (1, e, 3e^2 - 1) over e = -1, 0,
+// 1, and moment (a, b) of that product is k = {0,1,3}[a] + {0,2,4}[b] for
+// a + b <= 1, k = 5 (3 x the (1,1) product), 6 = (2,1), 7 = (1,2), 8 =
+// (2,2).  One axis maps (f-, f0, f+) to (f- + f0 + f+, f+ - f-, 2(f+ + f-)
+// - f0) and back by f0 = (2 M0 - M2)/3, f+- = (M0 + M2)/6 +- M1/2.  The
+// float path transforms by these two passes (about half the operations
+// of the dense products); FP64 keeps the dense products of the reference's
+// generated code, and with them bit-for-bit parity with the oracle.
+// Rows by e_y (-1, 0, +1) of the directions by e_x (-1, 0, +1):
+__device__ constexpr int kD9rows[3][3] = {{7, 4, 8}, {3, 0, 1}, {6, 2, 5}};
+
+template <class R>
+__device__ __forceinline__ void d2q9m_moments_fact(const R* f, R* m) {
+    R A[3][3];  // A[a][row]
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        const R fm = f[kD9rows[r][0]], f0 = f[kD9rows[r][1]], fp = f[kD9rows[r][2]];
+        const R s = fp + fm;
+        A[0][r] = s + f0;
+        A[1][r] = fp - fm;
+        A[2][r] = R(2) * s - f0;
+    }
+    R T[3][3];  // T[a][b]
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const R s = A[a][2] + A[a][0];
+        T[a][0] = s + A[a][1];
+        T[a][1] = A[a][2] - A[a][0];
+        T[a][2] = R(2) * s - A[a][1];
+    }
+    m[0] = T[0][0]; m[1] = T[1][0]; m[2] = T[0][1]; m[3] = T[2][0]; m[4] = T[0][2];
+    m[5] = R(3) * T[1][1]; m[6] = T[2][1]; m[7] = T[1][2]; m[8] = T[2][2];
+}
+
+template <class R>
+__device__ __forceinline__ void d2q9m_from_moments_fact(R* f, const R* m) {
+    const R third = R(1) / R(3), sixth = R(1) / R(6);
+    const R T[3][3] = {{m[0], m[2], m[4]}, {m[1], m[5] * third, m[7]}, {m[3], m[6], m[8]}};
+    R A[3][3];  // A[a][row]
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const R c = (T[a][0] + T[a][2]) * sixth, d = T[a][1] * R(0.5);
+        A[a][1] = (R(2) * T[a][0] - T[a][2]) * third;
+        A[a][2] = c + d;
+        A[a][0] = c - d;
+    }
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        const R c = (A[0][r] + A[2][r]) * sixth, d = A[1][r] * R(0.5);
+        f[kD9rows[r][1]] = (R(2) * A[0][r] - A[2][r]) * third;
+        f[kD9rows[r][2]] = c + d;
+        f[kD9rows[r][0]] = c - d;
+    }
+}
+
+// f = R B^-1 over the nonzero coefficients in ascending k (FP64), or by the
+// two passes above (float)
 template <class R>
 __device__ __forceinline__ void d2q9m_from_moments(R* f, const R* m) {
+    if constexpr (!std::is_same<R, double>::value) {
+        d2q9m_from_moments_fact(f, m);
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < 9; i++) {
         R acc = 0;
@@ -456,13 +517,17 @@ __device__ __forceinline__ void d2q9m_node(R* f, unsigned nt, const double* par,
     }
     if ((nt & 0x10u) == 0) return;  // MRT
     R m[9];
+    if constexpr (!std::is_same<R, double>::value) {
+        d2q9m_moments_fact(f, m);
+    } else {
 #pragma unroll
-    for (int k = 0; k < 9; k++) {
-        R acc = 0;
+        for (int k = 0; k < 9; k++) {
+            R acc = 0;
 #pragma unroll
-        for (int i = 0; i < 9; i++)
-            if (kB9[i][k] != 0) acc += f[i] * R(kB9[i][k]);
-        m[k] = acc;
+            for (int i = 0; i < 9; i++)
+                if (kB9[i][k] != 0) acc += f[i] * R(kB9[i][k]);
+            m[k] = acc;
+        }
     }
     const R rho = m[0];
     R Jx = m[1], Jy = m[2];
