@@ -241,6 +241,8 @@ StepArgs Batch::case_args(const Case& c) const {
     a.rpt = tune_.rpt;
     a.zcols = (int)model->zonal.size();
     a.nzones = model->zone_max + 1;
+    const NodeTypeDef* ib = model->nodetype("IB");
+    a.ib_mask = ib ? ib->value : 0u;   // a batch holds no cuts: the branch never runs
     model_params(model, c.settings, a.par);
     return a;
 }

@@ -35,6 +35,7 @@ struct StepArgs {
     int flat;              // 1: tiles are runs of the linear node index, not row segments
     double inv_nx;         // 1/nx (flat tiles: node index -> row)
     int zcols, nzones;
+    unsigned ib_mask;      // NodeType bit of IB (the cumulant IB builds), 0 for every other model
     double par[MAX_PAR];
 };
 

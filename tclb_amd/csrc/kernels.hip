@@ -1761,12 +1761,12 @@ __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, 
         } else if constexpr (MODEL == MODEL_D3Q27) {
             d3q27_node<R, GLOB>(f, nt, a.par, zr, g);
         } else if constexpr (is_cum<MODEL>) {
-            if constexpr (is_cum_ib<MODEL>) {
-                if ((nt & 0x100u) && a.cuts != nullptr)  // NodeType IB, cuts present
-                    cum_ib<R, GLOB>(f, nt, a.cuts + n, (size_t)nx * ny * nz, [&](int i) {
-                        return S::get(*(const T*)(fin + (size_t)i * pbytes + n * (unsigned)sizeof(T)), (R)L::w(i));
-                    }, zr, g);
-            }
+            // NodeType IB with cuts present; every cumulant build carries
+            // the branch, whose mask is 0 in the builds without IB
+            if ((nt & a.ib_mask) && a.cuts != nullptr)
+                cum_ib<R, GLOB>(f, nt, a.cuts + n, (size_t)nx * ny * nz, [&](int i) {
+                    return S::get(*(const T*)(fin + (size_t)i * pbytes + n * (unsigned)sizeof(T)), (R)L::w(i));
+                }, zr, g);
             if constexpr (is_cum_avg<MODEL>)
                 cum_node<R, GLOB, is_cum_smag<MODEL>, true>(f, nt, a.par, zr, g, a.extra ? a.extra + n : nullptr,
                                                             (size_t)nx * ny * nz);
@@ -2013,12 +2013,10 @@ __global__ void __launch_bounds__(256) k_quantity(StepArgs a, int qid, int x0, i
         } else if (qid == 1) {
             int zone = (int)(nt >> L::ZSHIFT);
             if (zone >= a.nzones) zone = 0;
-            if constexpr (is_cum_ib<MODEL>) {
-                if ((nt & 0x100u) && a.cuts != nullptr)
-                    cum_ib<double, false>(f, nt, a.cuts + n, (size_t)nx * ny * nz, [&](int i) {
-                        return (double)S::get(fin[(size_t)i * a.pitch + n], (typename S::R)L::w(i));
-                    }, nullptr, nullptr);
-            }
+            if ((nt & a.ib_mask) && a.cuts != nullptr)
+                cum_ib<double, false>(f, nt, a.cuts + n, (size_t)nx * ny * nz, [&](int i) {
+                    return (double)S::get(fin[(size_t)i * a.pitch + n], (typename S::R)L::w(i));
+                }, nullptr, nullptr);
             cum_boundaries<double, false>(f, nt, a.zone + (size_t)zone * a.zcols, nullptr);
             double d = 0, jx = 0, jy = 0, jz = 0;
 #pragma unroll

@@ -252,6 +252,8 @@ StepArgs Lattice::args(int from, int to, bool glob) const {
     a.inv_nx = 1.0 / nx;
     a.zcols = zcols_;
     a.nzones = nzones_;
+    const NodeTypeDef* ib = model->nodetype("IB");
+    a.ib_mask = ib ? ib->value : 0u;
     model_params(model, settings_, a.par);
     return a;
 }
