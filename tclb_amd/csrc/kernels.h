@@ -1,4 +1,5 @@
-// kernels.h -- host-side interface of the node kernels (kernels.hip).
+// kernels.h -- host-side interface of the node kernels (kernels_impl.h,
+// launched through kernels.hip).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstddef>

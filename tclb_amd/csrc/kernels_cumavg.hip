@@ -1,0 +1,50 @@
+// kernels_cumavg.hip -- instantiates and dispatches the node kernels of one
+// family of models (see kernels_impl.h); compiled apart so the families
+// build in parallel.
+#include "kernels_impl.h"
+
+namespace tclbx {
+
+hipError_t step_cumavg(int model, int storage, bool globals, const StepArgs& a, int block, hipStream_t s) {
+    switch (model) {
+    case MODEL_D3Q27_CUMULANT_AVG: return step_m<MODEL_D3Q27_CUMULANT_AVG>(storage, globals, a, block, s);
+    case MODEL_D3Q27_CUMULANT_AVG_IB: return step_m<MODEL_D3Q27_CUMULANT_AVG_IB>(storage, globals, a, block, s);
+    case MODEL_D3Q27_CUMULANT_AVG_SMAG: return step_m<MODEL_D3Q27_CUMULANT_AVG_SMAG>(storage, globals, a, block, s);
+    case MODEL_D3Q27_CUMULANT_AVG_IB_SMAG: return step_m<MODEL_D3Q27_CUMULANT_AVG_IB_SMAG>(storage, globals, a, block, s);
+    }
+    return hipErrorNotFound;
+}
+
+hipError_t init_cumavg(int model, int storage, const StepArgs& a, int block, hipStream_t s) {
+    switch (model) {
+    case MODEL_D3Q27_CUMULANT_AVG: return init_m<MODEL_D3Q27_CUMULANT>(storage, a, block, s);
+    case MODEL_D3Q27_CUMULANT_AVG_IB: return init_m<MODEL_D3Q27_CUMULANT_IB>(storage, a, block, s);
+    case MODEL_D3Q27_CUMULANT_AVG_SMAG: return init_m<MODEL_D3Q27_CUMULANT_SMAG>(storage, a, block, s);
+    case MODEL_D3Q27_CUMULANT_AVG_IB_SMAG: return init_m<MODEL_D3Q27_CUMULANT_IB_SMAG>(storage, a, block, s);
+    }
+    return hipErrorNotFound;
+}
+
+hipError_t batch_cumavg(int model, int storage, bool globals, const BatchCase* cases, const int2* tiles, int ntiles,
+                    int phase, double* gpart, int grid, int block, hipStream_t s) {
+    switch (model) {
+    case MODEL_D3Q27_CUMULANT_AVG: return batch_m<MODEL_D3Q27_CUMULANT_AVG>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
+    case MODEL_D3Q27_CUMULANT_AVG_IB: return batch_m<MODEL_D3Q27_CUMULANT_AVG_IB>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
+    case MODEL_D3Q27_CUMULANT_AVG_SMAG: return batch_m<MODEL_D3Q27_CUMULANT_AVG_SMAG>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
+    case MODEL_D3Q27_CUMULANT_AVG_IB_SMAG: return batch_m<MODEL_D3Q27_CUMULANT_AVG_IB_SMAG>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
+    }
+    return hipErrorNotFound;
+}
+
+hipError_t quantity_cumavg(int model, int storage, int q, const StepArgs& a, int x0, int y0, int z0, int rx, int ry,
+                       int rz, double scale, double* out, hipStream_t s) {
+    switch (model) {
+    case MODEL_D3Q27_CUMULANT_AVG: return quantity_m<MODEL_D3Q27_CUMULANT_AVG>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
+    case MODEL_D3Q27_CUMULANT_AVG_IB: return quantity_m<MODEL_D3Q27_CUMULANT_AVG_IB>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
+    case MODEL_D3Q27_CUMULANT_AVG_SMAG: return quantity_m<MODEL_D3Q27_CUMULANT_AVG_SMAG>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
+    case MODEL_D3Q27_CUMULANT_AVG_IB_SMAG: return quantity_m<MODEL_D3Q27_CUMULANT_AVG_IB_SMAG>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
+    }
+    return hipErrorNotFound;
+}
+
+}  // namespace tclbx
