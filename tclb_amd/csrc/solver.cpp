@@ -4,11 +4,13 @@
 #include "solver.h"
 
 #include <sys/stat.h>
+#include <zlib.h>
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <fstream>
@@ -152,6 +154,35 @@ struct GenericContainer : GenericAction {
     int finish() override {
         unstack();
         return GenericAction::finish();
+    }
+};
+
+// <Container> (src/Handlers/acContainer.cpp): the children once; the
+// callbacks they stacked come off when the container ends
+struct AcContainer : GenericAction {
+    int init() override {
+        GenericAction::init();
+        if (execute_internal()) return -1;
+        unstack();
+        return 0;
+    }
+};
+
+// <Repeat Times> (src/Handlers/acRepeat.cpp): the children Times times.
+// Each pass restarts the stack count, so only the last pass's interval
+// callbacks come off at the end, as in the reference.
+struct AcRepeat : GenericAction {
+    int init() override {
+        GenericAction::init();
+        const XmlAttr* a = node->attr("Times");
+        if (!a) {
+            solver->note("ERROR: no Times parameter in Repeat");
+            return -1;
+        }
+        const int times = std::atoi(a->value.c_str());
+        for (int i = 0; i < times; i++)
+            if (execute_internal()) return -1;
+        return 0;
     }
 };
 
@@ -436,6 +467,22 @@ struct CbVTK : Callback {
         return 0;
     }
     int doit() override { return solver->write_vtk(nm, what, reg); }
+};
+
+// <TXT [name] [what] [gzip] Iterations> (src/Handlers/cbTXT.cpp): every
+// quantity as text, one lattice row of nx values per line
+struct CbTXT : Callback {
+    std::string nm, what;
+    bool gzip = false;
+    int init() override {
+        Callback::init();
+        nm = node->get("name", "TXT");
+        what = node->get("what", "all");
+        const XmlAttr* g = node->attr("gzip");
+        gzip = g && xml_as_bool(g->value);
+        return 0;
+    }
+    int doit() override { return solver->write_txt(nm, what, gzip); }
 };
 
 // <Average Iterations> (src/Handlers/cbAveraging.cpp): resetAverage when
@@ -723,6 +770,9 @@ HandlerRef Solver::make_handler(XmlNode* n) {
     else if (e == "Stop") h = new CbStop;
     else if (e == "Failcheck") h = new CbFailcheck;
     else if (e == "VTK") h = new CbVTK;
+    else if (e == "TXT") h = new CbTXT;
+    else if (e == "Container") h = new AcContainer;
+    else if (e == "Repeat") h = new AcRepeat;
     else if (e == "SaveBinary") h = new CbSaveBinary;
     else if (e == "SaveCheckpoint") h = new CbSaveCheckpoint;
     else if (e == "Average") h = new CbAverage;
@@ -911,6 +961,64 @@ int Solver::write_vtk(const std::string& name, const std::string& what_s, const 
     std::fprintf(fp, "</PCellData>\n</PImageData>\n</VTKFile>\n");
     std::fclose(f);
     std::fclose(fp);
+    return 0;
+}
+
+// The text output of src/vtkLattice.cpp:85-160 (txtWriteLattice): an _info
+// file with the unit scales and the size, then one <prefix>_<quantity>.txt
+// per selected quantity in the output unit, %.16lg values (a vector as its
+// three components), a newline after every nx values.  gzip writes each as
+// <file>.gz (zlib in process, where the reference pipes through gzip).
+int Solver::write_txt(const std::string& name, const std::string& what_s, bool gzip) {
+    const NameSet what(what_s);
+    const std::string base = out_iter_file(name, "");
+    const std::string info = base + "_info.txt";
+    FILE* f = std::fopen(info.c_str(), "w");
+    if (!f) {
+        note("ERROR: Cannot open file: " + info);
+        return -1;
+    }
+    const size_t n = (size_t)nx * ny * nz;
+    std::fprintf(f, "dx: %lg\n", 1 / units.alt("m"));
+    std::fprintf(f, "dt: %lg\n", 1 / units.alt("s"));
+    std::fprintf(f, "dm: %lg\n", 1 / units.alt("kg"));
+    std::fprintf(f, "dT: %lg\n", 1 / units.alt("K"));
+    std::fprintf(f, "size: %d\n", (int)n);
+    std::fprintf(f, "NX: %d\nNY: %d\nNZ: %d\n", nx, ny, nz);
+    std::fclose(f);
+    Region o;
+    o.dx = 0; o.dy = 0; o.dz = 0; o.nx = nx; o.ny = ny; o.nz = nz;
+    for (int q = 0; q < (int)model->quantities.size(); q++) {
+        const QuantityDef& Q = model->quantities[q];
+        if (!what.in(Q.name)) continue;
+        const std::string fn = base + "_" + Q.name + ".txt";
+        const int comp = Q.vector ? 3 : 1;
+        std::vector<double> tmp(n * comp);
+        lattice->GetQuantity(q, o, tmp.data(), 1 / units.alt(Q.unit));
+        std::string text;
+        char num[40];
+        for (size_t i = 0; i < n; i++) {
+            for (int c = 0; c < comp; c++) {
+                if (c) text += ' ';
+                text.append(num, (size_t)std::snprintf(num, sizeof num, "%.16lg", tmp[i * comp + c]));
+            }
+            text += (i + 1) % (size_t)nx == 0 ? '\n' : ' ';
+        }
+        bool ok;
+        if (gzip) {
+            gzFile z = gzopen((fn + ".gz").c_str(), "wb");
+            ok = z != nullptr && gzwrite(z, text.data(), (unsigned)text.size()) == (int)text.size();
+            if (z && gzclose(z) != Z_OK) ok = false;
+        } else {
+            FILE* out = std::fopen(fn.c_str(), "w");
+            ok = out != nullptr && std::fwrite(text.data(), 1, text.size(), out) == text.size();
+            if (out && std::fclose(out) != 0) ok = false;
+        }
+        if (!ok) {
+            note("ERROR: Cannot write file: " + fn + (gzip ? ".gz" : ""));
+            return -1;
+        }
+    }
     return 0;
 }
 

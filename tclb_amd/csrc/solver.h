@@ -90,6 +90,7 @@ public:
     void init_log(const std::string& path);
     void write_log(const std::string& path);
     int write_vtk(const std::string& name, const std::string& what, const GeomRegion& reg);
+    int write_txt(const std::string& name, const std::string& what, bool gzip);
     double walltime() const;
     void note(const std::string& m) { messages.push_back(m); }
     // draws a <Geometry> element; its notices join `messages`

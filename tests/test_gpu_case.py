@@ -416,3 +416,53 @@ def test_average_handler_case(oracle_mod, tmp_path):
         assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want)), q
     mu = arr["avgU"][0].reshape(nz, ny, nx, 3)
     assert mu[..., 0].max() > 0.01          # the mean flow is there
+
+
+def _read_txt(path, nx, comp):
+    """A TXT quantity file: one row of nx values (comp numbers each) per line."""
+    if str(path).endswith(".gz"):
+        import gzip
+        with gzip.open(path, "rt") as f:
+            lines = f.read().splitlines()
+    else:
+        with open(path) as f:
+            lines = f.read().splitlines()
+    rows = [np.array([float(v) for v in ln.split()]) for ln in lines]
+    assert all(r.size == nx * comp for r in rows)
+    return np.concatenate(rows)
+
+
+def test_txt_container_repeat(tmp_path):
+    """<Repeat Times="3"> runs its Solve and TXT three times; the TXT files
+    (src/vtkLattice.cpp:97-160) hold the info header and the quantities at
+    16 digits, a row of nx values per line, plain and gzipped.  A <Log>
+    stacked inside a <Container> comes off when the container ends, so the
+    Solve after it adds no rows."""
+    say("case: TXT, Container, Repeat")
+    geo = """<Geometry nx="16" ny="12" nz="4"><MRT><Box/></MRT><Wall mask="ALL"><Channel/></Wall></Geometry>
+      <Model><Param name="nu" value="0.1"/><Param name="ForceX" value="1e-5"/></Model>"""
+    xml = f"""<CLBConfig output="out/">{geo}
+      <Repeat Times="3"><Solve Iterations="10"/><TXT name="T" what="P,U"/></Repeat>
+      <Container><Log Iterations="5"/><Solve Iterations="10"/></Container>
+      <Solve Iterations="10"/>
+      <TXT name="Z" gzip="true"/></CLBConfig>"""
+    c = run_case("d3q19", xml, tmp_path, "t.xml")
+    assert c.iteration == 50
+    out = tmp_path / "out"
+    for it in (10, 20, 30):
+        assert (out / f"t_T_P00_{it:08d}_P.txt").exists() and (out / f"t_T_P00_{it:08d}_U.txt").exists()
+    assert not (out / "t_T_P00_00000010_Solid.txt").exists()
+    info = (out / "t_T_P00_00000030_info.txt").read_text().splitlines()
+    assert info[4:] == ["size: 768", "NX: 16", "NY: 12", "NZ: 4"] and info[0] == "dx: 1"
+    # the state after 30 steps, from a case that only solves 30
+    ref = run_case("d3q19", f"""<CLBConfig output="ref/">{geo}<Solve Iterations="30"/></CLBConfig>""", tmp_path, "r.xml")
+    np.testing.assert_allclose(_read_txt(out / "t_T_P00_00000030_P.txt", 16, 1),
+                               ref.lattice.quantity("P").reshape(-1), rtol=1e-15, atol=0)
+    np.testing.assert_allclose(_read_txt(out / "t_T_P00_00000030_U.txt", 16, 3),
+                               ref.lattice.quantity("U").reshape(-1), rtol=1e-15, atol=1e-300)
+    np.testing.assert_allclose(_read_txt(out / "t_Z_P00_00000050_U.txt.gz", 16, 3),
+                               c.lattice.quantity("U").reshape(-1), rtol=1e-15, atol=1e-300)
+    logs = sorted(out.glob("t_Log*.csv"))
+    assert len(logs) == 1
+    _, rows = read_log(logs[0])
+    assert [r[0] for r in rows] == [35.0, 40.0]
