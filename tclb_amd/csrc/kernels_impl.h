@@ -1829,6 +1829,15 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a) {
     step_tile<MODEL, ST, GLOB>(a, blockIdx.x, GLOB ? a.rpb : a.rpt, blockIdx.x, gridDim.x);
 }
 
+// The same step with its registers capped at 128 VGPRs, four waves per SIMD.
+// The cumulant collisions compile to 130-135 VGPRs (three waves) uncapped
+// and fit 128 without spilling, which keeps more loads in flight.  The
+// AVG_SMAG builds would spill under the cap and keep the plain kernel.
+template <int MODEL, int ST, bool GLOB>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_step_w4(StepArgs a) {
+    step_tile<MODEL, ST, GLOB>(a, blockIdx.x, GLOB ? a.rpb : a.rpt, blockIdx.x, gridDim.x);
+}
+
 // One step of a batch of lattices (a ragged arena): a persistent grid walks
 // the tile list; tile t = {case, tile within the case}.  Each case has its
 // own descriptor (sizes, pitch, snapshots, flags, zone table, parameters);
@@ -2092,8 +2101,13 @@ inline int nblocks_of(const StepArgs& a, int block) { return a.bpr * a.ny * a.nz
 template <int MODEL, int ST>
 hipError_t step_ms(bool glob, const StepArgs& a, int block, hipStream_t s) {
     dim3 grid(step_blocks(a, glob, block)), blk(block);
-    if (glob) hipLaunchKernelGGL((k_step<MODEL, ST, true>), grid, blk, 0, s, a);
-    else hipLaunchKernelGGL((k_step<MODEL, ST, false>), grid, blk, 0, s, a);
+    if (glob) {
+        hipLaunchKernelGGL((k_step<MODEL, ST, true>), grid, blk, 0, s, a);
+    } else if constexpr (is_cum<MODEL> && !(is_cum_avg<MODEL> && is_cum_smag<MODEL>)) {
+        hipLaunchKernelGGL((k_step_w4<MODEL, ST, false>), grid, blk, 0, s, a);
+    } else {
+        hipLaunchKernelGGL((k_step<MODEL, ST, false>), grid, blk, 0, s, a);
+    }
     return hipGetLastError();
 }
 
