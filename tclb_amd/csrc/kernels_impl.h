@@ -1829,13 +1829,22 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a) {
     step_tile<MODEL, ST, GLOB>(a, blockIdx.x, GLOB ? a.rpb : a.rpt, blockIdx.x, gridDim.x);
 }
 
-// The same step with its registers capped at 128 VGPRs, four waves per SIMD.
+// The same step with a floor of W waves per SIMD, which caps its registers.
 // The cumulant collisions compile to 130-135 VGPRs (three waves) uncapped
-// and fit 128 without spilling, which keeps more loads in flight.  The
-// AVG_SMAG builds would spill under the cap and keep the plain kernel.
-template <int MODEL, int ST, bool GLOB>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_step_w4(StepArgs a) {
+// and fit 128 (four waves) without spilling; the AVG_SMAG builds would
+// spill and keep the plain kernel.  More waves keep more loads in flight.
+// (The d3q19 float steps, held from seven waves to eight, ran no faster:
+// they already sit at the 19-plane pull ceiling.)
+template <int MODEL, int ST, bool GLOB, int W>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) k_step_w(StepArgs a) {
     step_tile<MODEL, ST, GLOB>(a, blockIdx.x, GLOB ? a.rpb : a.rpt, blockIdx.x, gridDim.x);
+}
+
+// waves per SIMD the plain step of MODEL in storage ST is held to (0: none)
+template <int MODEL, int ST>
+constexpr int step_waves() {
+    if constexpr (is_cum<MODEL>) return is_cum_avg<MODEL> && is_cum_smag<MODEL> ? 0 : 4;
+    return 0;
 }
 
 // One step of a batch of lattices (a ragged arena): a persistent grid walks
@@ -2123,8 +2132,8 @@ hipError_t step_ms(bool glob, const StepArgs& a, int block, hipStream_t s) {
     dim3 grid(step_blocks(a, glob, block)), blk(block);
     if (glob) {
         hipLaunchKernelGGL((k_step<MODEL, ST, true>), grid, blk, 0, s, a);
-    } else if constexpr (is_cum<MODEL> && !(is_cum_avg<MODEL> && is_cum_smag<MODEL>)) {
-        hipLaunchKernelGGL((k_step_w4<MODEL, ST, false>), grid, blk, 0, s, a);
+    } else if constexpr (step_waves<MODEL, ST>() > 0) {
+        hipLaunchKernelGGL((k_step_w<MODEL, ST, false, step_waves<MODEL, ST>()>), grid, blk, 0, s, a);
     } else {
         hipLaunchKernelGGL((k_step<MODEL, ST, false>), grid, blk, 0, s, a);
     }
