@@ -139,8 +139,11 @@ const char* tclbx_extra_density_name(const char* model, int k, int* is_average);
  * current iteration. */
 int tclbx_reset_average(tclbx_lattice* h);
 /* saveSolution / loadSolution (src/Lattice.cu.Rt:708-769): raw densities
- * of the current snapshot in the storage format; -1 if the file cannot be
- * opened or is short. */
+ * of the current snapshot in the storage format, then the extra densities
+ * (FP64), then a 64-byte footer (model, storage, size) that a reader taking
+ * only the bytes it expects ignores.  TCLBX_ERR if the file cannot be opened
+ * or read, or holds another model, storage format or size (checked before
+ * the lattice is touched); a file without the footer is checked by size. */
 int tclbx_save(tclbx_lattice* h, const char* path);
 int tclbx_load(tclbx_lattice* h, const char* path);
 

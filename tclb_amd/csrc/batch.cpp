@@ -85,8 +85,9 @@ Batch::Batch(const ModelDef* m, int storage_, size_t capacity_bytes)
 }
 
 Batch::~Batch() {
-    drop_graph();
+    // nothing may still run from the captured graph when it is destroyed
     if (stream_) (void)hipStreamSynchronize(stream_);
+    drop_graph();
     for (auto& kv : active_)
         if (kv.second.d_zone) (void)hipFree(kv.second.d_zone);
     for (void* p : {(void*)d_cases_, (void*)d_tiles_, (void*)d_gtiles_, (void*)d_first_, (void*)d_count_,
@@ -107,10 +108,8 @@ size_t Batch::CaseBytes(int nx, int ny, int nz) const {
 }
 
 int Batch::AddCase(int nx, int ny, int nz) {
-    if (nx <= 0 || ny <= 0 || nz <= 0) throw std::invalid_argument("lattice size must be positive");
-    if (model->dim == 2 && nz != 1) throw std::invalid_argument("2D model needs nz = 1");
+    check_lattice_size(model, nx, ny, nz, storage);
     const size_t N = (size_t)nx * ny * nz;
-    if (N >= (1ull << 32)) throw std::invalid_argument("lattice exceeds 2^32 nodes");
     const size_t bytes = CaseBytes(nx, ny, nz);
     const size_t off = arena_.alloc(bytes);
     if (off == std::numeric_limits<size_t>::max()) return -1;

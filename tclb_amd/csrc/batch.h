@@ -95,7 +95,7 @@ public:
     size_t bytes_per_node_step() const;
     size_t arena_used() const { return arena_.used(); }
     size_t arena_capacity() const { return arena_.capacity(); }
-    void set_graph_steps(int k) { graph_steps_ = k; drop_graph(); }
+    void set_graph_steps(int k) { graph_steps_ = k & ~1; drop_graph(); }
 
     const ModelDef* model;
     const int storage;

@@ -30,18 +30,34 @@ int pick_block(int nx) {
 }  // namespace
 
 int default_graph_steps() {
-    if (const char* e = std::getenv("TCLBX_GRAPH_STEPS")) return std::max(0, std::atoi(e));
+    // a captured loop starts and ends on snapshot 0, so its length is even:
+    // an odd value is rounded down (1 -> 0, no graph)
+    if (const char* e = std::getenv("TCLBX_GRAPH_STEPS")) return std::max(0, std::atoi(e)) & ~1;
     return 32;
+}
+
+void check_lattice_size(const ModelDef* m, int nx, int ny, int nz, int storage) {
+    if (!m) throw std::invalid_argument("unknown model");
+    if (nx <= 0 || ny <= 0 || nz <= 0) throw std::invalid_argument("lattice size must be positive");
+    if (m->dim == 2 && nz != 1) throw std::invalid_argument("2D model needs nz = 1");
+    if (storage < 0 || storage >= ST_COUNT) throw std::invalid_argument("unknown storage format");
+    const unsigned long long N = (unsigned long long)nx * ny * nz;
+    if (N >= (1ull << 32)) throw std::invalid_argument("lattice exceeds 2^32 nodes");
+    // the step kernel addresses a density plane with 32-bit byte offsets
+    const unsigned long long S = storage_bytes(storage);
+    const unsigned long long align = 256 / S;
+    const unsigned long long plane = (N + align - 1) / align * align * S;
+    if (plane > (1ull << 32))
+        throw std::invalid_argument("lattice " + std::to_string(nx) + "x" + std::to_string(ny) + "x" +
+                                    std::to_string(nz) + " in " + storage_name(storage) + " has a density plane of " +
+                                    std::to_string(plane) + " bytes; the limit is 4 GiB per plane (" +
+                                    std::to_string((1ull << 32) / S) + " nodes in this format)");
 }
 
 Lattice::Lattice(const ModelDef* m, int nx_, int ny_, int nz_, int storage_)
     : model(m), nx(nx_), ny(ny_), nz(nz_), storage(storage_), N((size_t)nx_ * ny_ * nz_) {
     graph_steps_ = default_graph_steps();
-    if (!m) throw std::invalid_argument("unknown model");
-    if (nx <= 0 || ny <= 0 || nz <= 0) throw std::invalid_argument("lattice size must be positive");
-    if (m->dim == 2 && nz != 1) throw std::invalid_argument("2D model needs nz = 1");
-    if (storage < 0 || storage >= ST_COUNT) throw std::invalid_argument("unknown storage format");
-    if (N >= (1ull << 32)) throw std::invalid_argument("lattice exceeds 2^32 nodes");
+    check_lattice_size(m, nx_, ny_, nz_, storage_);
     const int S = storage_bytes(storage);
     const size_t align = 256 / S;
     pitch = (N + align - 1) / align * align;
@@ -88,8 +104,9 @@ Lattice::Lattice(const ModelDef* m, int nx_, int ny_, int nz_, int storage_)
 }
 
 Lattice::~Lattice() {
-    drop_graph();
+    // nothing may still run from the captured graph when it is destroyed
     if (stream_) (void)hipStreamSynchronize(stream_);
+    drop_graph();
     for (void* p : snaps_) if (p) (void)hipFree(p);
     if (d_flags_) (void)hipFree(d_flags_);
     if (d_cuts_) (void)hipFree(d_cuts_);
@@ -338,7 +355,7 @@ void Lattice::run_graph_steps(int nsteps) {
             StepArgs a = args(i % 2, (i + 1) % 2, false);
             hipError_t e = launch_step(model->id, storage, false, a, block_, stream_);
             if (e != hipSuccess) {
-                hipGraph_t g;
+                hipGraph_t g = nullptr;
                 (void)hipStreamEndCapture(stream_, &g);
                 if (g) (void)hipGraphDestroy(g);
                 hip_check(e, "step kernel (capture)");
@@ -522,9 +539,28 @@ void Lattice::SetDensity(int q, const double* in) {
     hip_check(e, "SetDensity");
 }
 
-// raw dump of the current snapshot, density planes in order, in the storage
-// format (the reference's saveSolution writes its FTabs raw, no header:
-// src/Lattice.cu.Rt:708-737)
+// A checkpoint is the raw dump of the current snapshot, density planes in
+// order, in the storage format (the reference's saveSolution writes its FTabs
+// raw, no header: src/Lattice.cu.Rt:708-737), then the extra densities in
+// FP64, then a 64-byte footer naming the model, storage format and size.  A
+// reader that takes only the bytes it expects (the reference's load reads
+// block by block, :740-769) ignores the footer.  Load checks the footer, or
+// for a file without one its size, before it writes anything to the device.
+namespace {
+struct CheckpointFooter {
+    char magic[8];    // "TCLBXCK1"
+    char model[24];
+    char storage[8];
+    int32_t nx, ny, nz, q, nextra, reserved;
+};
+static_assert(sizeof(CheckpointFooter) == 64, "checkpoint footer is 64 bytes");
+constexpr char kMagic[8] = {'T', 'C', 'L', 'B', 'X', 'C', 'K', '1'};
+}  // namespace
+
+size_t Lattice::checkpoint_data_bytes() const {
+    return (size_t)model->Q * N * storage_bytes(storage) + (size_t)model->nextra() * N * sizeof(double);
+}
+
 int Lattice::Save(const std::string& path) {
     FILE* f = std::fopen(path.c_str(), "wb");
     if (!f) return -1;
@@ -543,28 +579,65 @@ int Lattice::Save(const std::string& path) {
         hip_check(hipMemcpy(av.data(), d_extra_ + k * N, N * sizeof(double), hipMemcpyDeviceToHost), "Save");
         if (std::fwrite(av.data(), sizeof(double), N, f) != N) rc = -1;
     }
-    std::fclose(f);
+    if (rc == 0) {
+        CheckpointFooter ft;
+        std::memset(&ft, 0, sizeof(ft));
+        std::memcpy(ft.magic, kMagic, sizeof(kMagic));
+        std::strncpy(ft.model, model->name.c_str(), sizeof(ft.model) - 1);
+        std::strncpy(ft.storage, storage_name(storage), sizeof(ft.storage) - 1);
+        ft.nx = nx; ft.ny = ny; ft.nz = nz;
+        ft.q = model->Q;
+        ft.nextra = model->nextra();
+        if (std::fwrite(&ft, sizeof(ft), 1, f) != 1) rc = -1;
+    }
+    if (std::fclose(f) != 0) rc = -1;
     return rc;
 }
 
 int Lattice::Load(const std::string& path) {
     FILE* f = std::fopen(path.c_str(), "rb");
     if (!f) return -1;
+    struct Closer { FILE* f; ~Closer() { std::fclose(f); } } closer{f};
+    const size_t data = checkpoint_data_bytes();
+    if (std::fseek(f, 0, SEEK_END) != 0) return -1;
+    const long long size = std::ftell(f);
+    if (size < 0) return -1;
+    const std::string what = "checkpoint " + path + ": ";
+    const std::string mine = model->name + " " + storage_name(storage) + " " + std::to_string(nx) + "x" +
+                             std::to_string(ny) + "x" + std::to_string(nz);
+    if ((size_t)size == data + sizeof(CheckpointFooter)) {
+        CheckpointFooter ft;
+        if (std::fseek(f, (long)data, SEEK_SET) != 0 || std::fread(&ft, sizeof(ft), 1, f) != 1) return -1;
+        if (std::memcmp(ft.magic, kMagic, sizeof(kMagic)) != 0)
+            throw std::runtime_error(what + "has " + std::to_string(size) + " bytes and no tclbx footer; " + mine +
+                                     " takes " + std::to_string(data));
+        ft.model[sizeof(ft.model) - 1] = 0;
+        ft.storage[sizeof(ft.storage) - 1] = 0;
+        const std::string theirs = std::string(ft.model) + " " + ft.storage + " " + std::to_string(ft.nx) + "x" +
+                                   std::to_string(ft.ny) + "x" + std::to_string(ft.nz);
+        if (theirs != mine || ft.q != model->Q || ft.nextra != model->nextra())
+            throw std::runtime_error(what + "holds " + theirs + ", the lattice is " + mine);
+    } else if ((size_t)size != data) {
+        throw std::runtime_error(what + "has " + std::to_string(size) + " bytes; a checkpoint of " + mine +
+                                 " has " + std::to_string(data) + " (+" + std::to_string(sizeof(CheckpointFooter)) +
+                                 " of footer)");
+    }
+    if (std::fseek(f, 0, SEEK_SET) != 0) return -1;
+    // plane by plane, so that host memory stays at one plane; the size is
+    // checked, so only an I/O error can stop it part way
     const size_t S = storage_bytes(storage);
     std::vector<char> host(N * S);
-    int rc = 0;
-    for (int q = 0; q < model->Q && rc == 0; q++) {
-        if (std::fread(host.data(), 1, host.size(), f) != host.size()) { rc = -1; break; }
+    for (int q = 0; q < model->Q; q++) {
+        if (std::fread(host.data(), 1, host.size(), f) != host.size()) return -1;
         hip_check(hipMemcpy((char*)snaps_[snap_] + (size_t)q * pitch * S, host.data(), N * S, hipMemcpyHostToDevice),
                   "Load");
     }
     std::vector<double> av(N);
-    for (size_t k = 0; k < (size_t)model->nextra() && rc == 0; k++) {
-        if (std::fread(av.data(), sizeof(double), N, f) != N) { rc = -1; break; }
+    for (size_t k = 0; k < (size_t)model->nextra(); k++) {
+        if (std::fread(av.data(), sizeof(double), N, f) != N) return -1;
         hip_check(hipMemcpy(d_extra_ + k * N, av.data(), N * sizeof(double), hipMemcpyHostToDevice), "Load");
     }
-    std::fclose(f);
-    return rc;
+    return 0;
 }
 
 }  // namespace tclbx

@@ -60,8 +60,12 @@ public:
     // model's averages, then its parameter densities)
     void GetDensity(int q, double* out);
     void SetDensity(int q, const double* in); // Set_<f>
+    // saveSolution / loadSolution (src/Lattice.cu.Rt:708-769): -1 when the
+    // file cannot be opened or read; Load throws when the file holds another
+    // model, storage format or size
     int Save(const std::string& path);
     int Load(const std::string& path);
+    size_t checkpoint_data_bytes() const;  // bytes of a checkpoint before its footer
 
     double last_iterate_ms() const { return last_ms_; }
     double kernel_ms_total() const { return kernel_ms_; }
@@ -70,7 +74,7 @@ public:
     size_t bytes_per_node_step() const;       // algorithmic HBM bytes
     void set_block(int b);
     int block() const { return block_; }
-    void set_graph_steps(int k) { graph_steps_ = k; drop_graph(); }
+    void set_graph_steps(int k) { graph_steps_ = k & ~1; drop_graph(); }
     // rows of the lattice walked by one block of a plain step: 1, 2 or 4,
     // or 0 to let the first long Iterate time the three and keep the fastest
     void set_rows_per_tile(int r);
@@ -119,9 +123,14 @@ private:
 // hipError_t -> exception with a message naming the call
 void hip_check(hipError_t e, const char* what);
 // steps per captured graph of a new Lattice or Batch: 32, or the
-// environment's TCLBX_GRAPH_STEPS (0 or 1: every step launched directly,
-// for tools that do not follow graph launches)
+// environment's TCLBX_GRAPH_STEPS rounded down to an even count (0 or 1:
+// every step launched directly, for tools that do not follow graph launches)
 int default_graph_steps();
+// throws std::invalid_argument unless a lattice of the model in this storage
+// format can be held: positive sizes, nz = 1 for a 2D model, fewer than 2^32
+// nodes, and each density plane within 4 GiB (the step kernel's per-lane
+// addresses are 32-bit byte offsets from the plane base)
+void check_lattice_size(const ModelDef* m, int nx, int ny, int nz, int storage);
 // kernel parameter vector (StepArgs::par) of a model from its settings
 void model_params(const ModelDef* model, const std::vector<double>& settings, double* par);
 // rows walked by one block of a Globals step for a lattice of `rows` rows

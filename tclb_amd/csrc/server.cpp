@@ -69,7 +69,9 @@ std::string Server::Error() {
 }
 
 int Server::Submit(Request r) {
-    if (r.nx <= 0 || r.ny <= 0 || r.nz <= 0) throw std::invalid_argument("lattice size must be positive");
+    // the checks Batch::AddCase makes, so that a malformed request fails
+    // here and never reaches the worker
+    check_lattice_size(model, r.nx, r.ny, r.nz, batch_->storage);
     if ((long long)r.flags.size() != (long long)r.nx * r.ny * r.nz)
         throw std::invalid_argument("flags do not match the lattice size");
     if (r.max_steps < 1) throw std::invalid_argument("step budget must be positive");
@@ -162,19 +164,39 @@ void Server::run() {
                 std::lock_guard<std::mutex> lk(mu_);
                 results_[pr.first].state = RQ_REJECTED;
                 results_[pr.first].t_finish = now();
+                results_[pr.first].error = "the request needs " + std::to_string(need) +
+                                           " bytes of the arena, which holds " +
+                                           std::to_string(batch_->arena_capacity());
                 stats_.rejected++;
                 inflight_--;
                 continue;
             }
-            const int cid = batch_->AddCase(r.nx, r.ny, r.nz);
+            // an error while admitting one request rejects that request
+            // only; an error of the device shows again at the next step,
+            // which ends the worker
+            int cid = -1;
+            try {
+                cid = batch_->AddCase(r.nx, r.ny, r.nz);
+                if (cid >= 0) {
+                    batch_->FlagOverwrite(cid, r.flags.data());
+                    for (auto& s : r.settings) batch_->SetSetting(cid, s.first, s.second);
+                    for (auto& z : r.zonal) batch_->SetZoneSetting(cid, z.id, z.zone, z.v);
+                    batch_->InitCase(cid);
+                }
+            } catch (const std::exception& e) {
+                if (cid >= 0 && batch_->HasCase(cid)) batch_->RemoveCase(cid);
+                std::lock_guard<std::mutex> lk(mu_);
+                results_[pr.first].state = RQ_REJECTED;
+                results_[pr.first].t_finish = now();
+                results_[pr.first].error = e.what();
+                stats_.rejected++;
+                inflight_--;
+                continue;
+            }
             if (cid < 0) {
                 waiting.push_back(std::move(pr));
                 continue;
             }
-            batch_->FlagOverwrite(cid, r.flags.data());
-            for (auto& s : r.settings) batch_->SetSetting(cid, s.first, s.second);
-            for (auto& z : r.zonal) batch_->SetZoneSetting(cid, z.id, z.zone, z.v);
-            batch_->InitCase(cid);
             Active a;
             a.rid = pr.first;
             a.max_steps = r.max_steps;

@@ -65,6 +65,7 @@ struct RequestResult {
     long long steps = 0;
     std::vector<double> globals;
     double t_submit = 0, t_admit = -1, t_finish = -1;  // seconds since the server started
+    std::string error;  // why a rejected request was rejected
 };
 
 class Server {

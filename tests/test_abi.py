@@ -191,3 +191,26 @@ def test_batch_refuses_models_with_extra_densities(model):
     batch (and the server built on it) refuses them before allocating."""
     with pytest.raises(tclb_amd.TclbxError, match="use a lattice"):
         tclb_amd.Batch(model, "f64", 1 << 20)
+
+
+@pytest.mark.parametrize("model,n,storage", [("d3q19", 813, "f64"), ("d3q19", 1025, "f32"), ("d3q19", 1291, "f16s"),
+                                             ("d3q27", 1025, "f32s")])
+def test_density_plane_limit(model, n, storage):
+    """The step kernel addresses a density plane with 32-bit byte offsets, so
+    a lattice whose plane passes 4 GiB is refused before anything is
+    allocated: FP64 past 2^29 nodes (813^3), FP32 past 2^30 (1025^3), 16-bit
+    formats past 2^31 (1291^3)."""
+    with pytest.raises(tclb_amd.TclbxError, match="4 GiB per plane"):
+        tclb_amd.Lattice(model, n, n, n, storage)
+
+
+def test_density_plane_limit_boundary():
+    """812^3 FP64 (535,387,328 nodes, a plane of 4,283,098,624 bytes) is
+    within the limit: creation gets past the size check (without a GPU it
+    then fails on the device, not on the size)."""
+    if gpu_available():
+        pytest.skip("would allocate 163 GB")
+    with pytest.raises(tclb_amd.TclbxError) as e:
+        tclb_amd.Lattice("d3q19", 812, 812, 812, "f64")
+    assert "4 GiB" not in str(e.value)
+

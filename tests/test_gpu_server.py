@@ -80,3 +80,93 @@ def test_admission_by_budget_and_rejection():
     assert st["completed"] == 5 and st["rejected"] == 1 and st["active"] == 0
     # some request had to wait for memory
     assert max(r["t_admit"] - r["t_submit"] for r in res) > 0
+
+
+def _stated_requests():
+    """Requests of the serving stream as stated (seed 5; sides log-uniform in
+    [48, 256], porosity 0-25%, g_x 1e-7..1e-5, tau 0.52..0.9), with short
+    step budgets so the oracle can follow them."""
+    reqs = cases.d3q19_serving_requests(12)
+    pick = [(1, 20), (2, 24), (3, 30), (4, 36), (9, 40), (6, 12)]
+    out = [(reqs[k][1], steps) for k, steps in pick]
+    sides = [d for c, _ in out for d in (c.nx, c.ny, c.nz)]
+    assert 48 <= min(sides) and max(sides) <= 256 and max(sides) >= 240
+    return out
+
+
+def test_served_requests_match_the_oracle(oracle_mod):
+    """Six requests at the serving workload's stated sizes (up to 4.2M
+    nodes, sides 49-248), served in FP64, three of them joining while the
+    first three run and each retiring at its own budget: every request's
+    Globals at its end within 1e-10 of the oracle's run of the same case
+    (the bar of the reference's harness, tools/csvdiff:7)."""
+    say("server: stated request sizes vs the oracle")
+    reqs = _stated_requests()
+    s = tclb_amd.Server("d3q19", "f64", hbm_budget=8 << 30, max_chunk=8)
+    ids = [s.submit(c, st) for c, st in reqs[:3]]
+    t0 = time.time()
+    while s.poll(ids[0])["state"] == "queued" and time.time() - t0 < 60:
+        time.sleep(0.005)
+    ids += [s.submit(c, st) for c, st in reqs[3:]]
+    assert s.wait_all(300)
+    oracle_mod.set_threads(16)
+    try:
+        for (case, steps), rid in zip(reqs, ids):
+            r = s.poll(rid)
+            assert r["state"] == "done_budget" and r["steps"] == steps, (case.name, r)
+            g = oracle_mod.lattice_for_case(case, "f64").iterate(steps)
+            say(f"  {case.name}: {steps} steps, VOLvx {r['globals'][16]:.9e} (oracle {g[16]:.9e})")
+            for i, (a, b) in enumerate(zip(r["globals"], g)):
+                scale = max(abs(a), abs(b))
+                assert scale < 1e-10 or abs(a - b) <= 1e-10 * scale, (case.name, i, a, b)
+    finally:
+        oracle_mod.set_threads(1)
+    s.close()
+
+
+def test_served_f16s_request_tracks_the_oracle(oracle_mod):
+    """One request of the stated stream (61x71x84, 8% solid) served in
+    FP16-shift storage: each volume Global stays within twice the format's
+    own error (the oracle's f16s emulation against its FP64 run) of the
+    oracle's f16s emulation, and the fluid volume is exact."""
+    say("server: f16s request vs the oracle's f16s")
+    case, steps = _stated_requests()[4]
+    s = tclb_amd.Server("d3q19", "f16s", hbm_budget=1 << 30, max_chunk=500)
+    rid = s.submit(case, steps)
+    assert s.wait_all(120)
+    r = s.poll(rid)
+    assert r["state"] == "done_budget" and r["steps"] == steps
+    oracle_mod.set_threads(16)
+    try:
+        g16 = oracle_mod.lattice_for_case(case, "f16s").iterate(steps)
+        g64 = oracle_mod.lattice_for_case(case, "f64").iterate(steps)
+    finally:
+        oracle_mod.set_threads(1)
+    names = tclb_amd.Model("d3q19").global_names
+    for name in ("VOLvx", "VOLvy", "VOLvz", "VOLpx", "VOLpy", "VOLpz", "VOLrho", "MaxV"):
+        i = names.index(name)
+        a, b, c = r["globals"][i], g16[i], g64[i]
+        say(f"  {name}: served {a:.6e} oracle f16s {b:.6e} oracle f64 {c:.6e}")
+        assert abs(a - b) <= 2 * abs(b - c) + 1e-6 * abs(c), (name, a, b, c)
+    i = names.index("VOLvolume")
+    assert r["globals"][i] == g64[i]
+    s.close()
+
+
+def test_malformed_request_fails_at_submit():
+    """A request the batch cannot hold (a 2D model with nz = 2) fails in
+    submit() with a message; the server keeps serving the valid request
+    submitted beside it."""
+    import dataclasses
+    say("server: malformed request")
+    good = cases.d2q9_cylinder(nx=64, ny=32, diameter=8, cx=16, cy=16, steps=50)
+    bad = dataclasses.replace(good, nz=2, flags=np.concatenate([good.flags, good.flags]))
+    s = tclb_amd.Server("d2q9_SRT", "f64", hbm_budget=64 << 20, max_chunk=20)
+    r0 = s.submit(good, 50)
+    with pytest.raises(tclb_amd.TclbxError, match="nz = 1"):
+        s.submit(bad, 50)
+    r1 = s.submit(good, 30)
+    assert s.wait_all(60)
+    assert s.poll(r0)["state"] == "done_budget" and s.poll(r1)["state"] == "done_budget"
+    assert s.stats()["completed"] == 2
+    s.close()

@@ -293,6 +293,97 @@ def test_d2q9_conservation_and_body_force(O):
     assert np.allclose(Jn, J + mass * np.array(G), rtol=0, atol=1e-14)
 
 
+D2_EX = np.array([0, 1, 0, -1, 0, 1, -1, -1, 1])
+D2_EY = np.array([0, 0, 1, 0, -1, 1, 1, -1, -1])
+D2_W = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+
+
+def test_d2q9_srt_feq_is_the_second_order_polynomial(O):
+    """SetEquilibrium (models/flow/d2q9_SRT/Dynamics.c:120-130) is the
+    standard second-order equilibrium w_i rho (1 + 3 e.u + 9/2 (e.u)^2 -
+    3/2 u^2), quadratic terms included (they set the momentum flux; a BGK
+    fixed point alone pins only moments 0 and 1)."""
+    rng = np.random.default_rng(41)
+    for _ in range(50):
+        rho = 0.8 + 0.4 * rng.random()
+        ux, uy = 0.2 * rng.standard_normal(2)
+        eu = D2_EX * ux + D2_EY * uy
+        want = D2_W * rho * (1 + 3 * eu + 4.5 * eu * eu - 1.5 * (ux * ux + uy * uy))
+        got = O.d2q9_feq(rho, ux, uy)
+        assert np.max(np.abs(got - want)) <= 4e-16 * rho, (rho, ux, uy)
+        # the second moment: Pi = rho/3 I + rho u u
+        Pxy = (got * D2_EX * D2_EY).sum()
+        Pxx = (got * D2_EX * D2_EX).sum()
+        assert abs(Pxy - rho * ux * uy) < 1e-15 and abs(Pxx - rho / 3 - rho * ux * ux) < 1e-15
+
+
+def _srt_channel(O, tau, g, ny, steps):
+    """x-invariant d2q9_SRT channel: Wall rows (bounce-back, no collision:
+    mask ALL) at y = 0 and ny-1, BGK fluid between, zonal GravitationX = g."""
+    L = O.OracleLattice("d2q9_SRT", 1, ny, 1)
+    L.set_params([1 / tau, 1.0])
+    L.set_zone_table(np.array([[0, 0, 0, g, 0]]))
+    fl = np.full((1, ny, 1), 8, np.uint16)
+    fl[0, 0, 0] = fl[0, ny - 1, 0] = 6
+    L.set_flags(fl)
+    L.init()
+    L.iterate(steps)
+    return L
+
+
+@pytest.mark.parametrize("tau", [0.56, 0.8, 0.5 + np.sqrt(3) / 4, 1.2])
+def test_d2q9_srt_poiseuille_with_bounce_back_slip(O, tau):
+    """BGK force-driven Poiseuille flow between bounce-back walls.  The
+    steady x-invariant solution of the lattice BGK equation is the parabola
+    with walls half-way between nodes, u = g/(2 nu)(y - 1/2)(ny - 3/2 - y),
+    nu = (tau - 1/2)/3, plus a uniform slip u_s = g (16 L - 3)/(24 nu) with
+    L = (tau - 1/2)^2: zero at L = 3/16, where bounce-back is exact for
+    Poiseuille flow (Ginzburg & d'Humieres, Phys. Rev. E 68, 066614, 2003;
+    the tau-dependent bounce-back slip of He, Zou, Luo & Dembo, J. Stat.
+    Phys. 87, 115, 1997).  U is J/rho + G/2 (Dynamics.c:107-116).  A wrong
+    viscosity, a force not shifted by G/omega, or a missing G/2 in U fails
+    this by far more than the 1e-8 bound."""
+    ny, g = 12, 1e-6
+    nu = (tau - 0.5) / 3
+    lam = (tau - 0.5) ** 2
+    L = _srt_channel(O, tau, g, ny, 80000)
+    u = L.quantity("U")[0, 1:-1, 0, 0]
+    y = np.arange(1, ny - 1)
+    want = g / (2 * nu) * (y - 0.5) * (ny - 1.5 - y) + g * (16 * lam - 3) / (24 * nu)
+    assert np.max(np.abs(u - want)) / np.max(np.abs(want)) < 1e-8, np.max(np.abs(u - want))
+    # no transverse flow, density uniform
+    assert np.max(np.abs(L.quantity("U")[0, 1:-1, 0, 1])) < 1e-14
+    rho = L.quantity("Rho")[0, 1:-1, 0]
+    assert np.ptp(rho) < 1e-12
+
+
+@pytest.mark.parametrize("g", [(1e-5, 0.0), (4e-6, -3e-6)])
+def test_d2q9_srt_drag_balances_the_body_force(O, g):
+    """d2q9_SRT_drag's Drag/Lift (momentum exchange 2 sum_i f_i e_i of the
+    populations arriving at Solid nodes) against the momentum balance of a
+    periodic box: at steady state the body force puts G sum(rho) into the
+    fluid each step (BGK nodes only), and all of it leaves through the
+    body, so Drag = G_x M and Lift = G_y M with M the fluid mass, whatever
+    the body's shape.  A factor or sign slip in the exchange, or a sum over
+    the populations after the swap, fails this."""
+    nx, ny = 24, 16
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    solid = ((xx + 0.5 - 9) ** 2 + (yy + 0.5 - 7.3) ** 2 < 3.2 ** 2) | ((xx == 15) & (yy > 3) & (yy < 9))
+    fl = np.where(solid, 5, 8).astype(np.uint16).reshape(1, ny, nx)
+    L = O.OracleLattice("d2q9_SRT_drag", nx, ny, 1)
+    L.set_params([1.0, 1.0])
+    L.set_zone_table(np.array([[0, 0, 0, g[0], g[1], 0, 0]]))
+    L.set_flags(fl)
+    L.init()
+    L.iterate(30000)
+    glob = L.iterate(1)
+    rho = L.quantity("Rho")[0]
+    mass = rho[~solid].sum()
+    assert abs(glob[0] - g[0] * mass) <= 1e-9 * abs(g[0] * mass), (glob[0], g[0] * mass)
+    scale = abs(g[0] * mass)
+    assert abs(glob[1] - g[1] * mass) <= 1e-9 * scale, (glob[1], g[1] * mass)
+
+
 def _node_after(O, model, flag, f_node, params, zone_row=None):
     """Post-boundary state of one node with NodeType `flag` on a 3x3(x3)
     lattice whose other nodes have flag 0 (stream only)."""
