@@ -40,9 +40,7 @@ def traffic_table():
         return json.load(f).get("per_launch_bytes", {})
 
 
-def cpu_baseline(n, seconds, threads):
-    """The oracle (a C restatement of the reference CPU build's node kernel,
-    FP64) on a bounded sample of the same workload."""
+def _oracle_mlups(n, seconds, threads):
     from oracle import oracle as O
     from tclb_amd import cases
 
@@ -56,21 +54,46 @@ def cpu_baseline(n, seconds, threads):
     t0 = time.perf_counter()
     L.iterate(steps)
     dt = time.perf_counter() - t0
+    return case.nodes * steps / dt / 1e6, steps, dt
+
+
+def cpu_share():
+    """Host threads this process may use: the OpenMP setting of the host
+    (the GPU box gives one GPU's process a share of its cores), else every
+    core."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    return os.cpu_count() or 1
+
+
+def cpu_baseline(n, seconds, threads):
+    """The oracle (a C restatement of the reference CPU build's node kernel,
+    FP64, gcc -O3) on a bounded sample of the same workload: OpenMP over the
+    host threads given to this process, and one thread (one MPI rank of the
+    reference, no OpenMP)."""
+    mt, steps, dt = _oracle_mlups(n, seconds, threads)
+    st, steps1, dt1 = _oracle_mlups(n, seconds / 3, 1)
     return {
-        "value": case.nodes * steps / dt / 1e6,
+        "value": round(mt, 2),
         "unit": "MLUPS",
         "cores": threads,
         "kind": "port",
-        "sample": f"same generator at {n}^3 (seed 2, 10% solid), {steps} steps, FP64, "
-                  f"OpenMP over (y,z) rows, {threads} threads, MPI ranks = 1, {dt:.1f} s",
+        "single_thread_value": round(st, 3),
+        "nproc": os.cpu_count(),
+        "sample": f"same generator at {n}^3 (seed 2, 10% solid), FP64, oracle built gcc -O3; "
+                  f"{steps} steps on {threads} OpenMP threads over (y,z) rows ({dt:.1f} s; the host's "
+                  f"OMP_NUM_THREADS share of its {os.cpu_count()} logical cores) and {steps1} steps on "
+                  f"1 thread ({dt1:.1f} s); MPI ranks = 1",
     }
 
 
 def batched_section(steps, seq_cases):
     """The 256-case d2q9 cylinder sweep (d2q9 MRT with Drag/Lift) in one
     batch (FP16-shift storage, FP32 collision, autotuned tiles) against the
-    same cases solved one at a time as FP64 lattices (a sample of
-    `seq_cases`, scaled by nodes)."""
+    same cases solved one at a time as plain lattices, in the same storage
+    format (the gain of batching) and in FP64 (a sample of `seq_cases`,
+    scaled by nodes)."""
     import tclb_amd
     from tclb_amd import cases
     model = "d2q9_drag"
@@ -83,42 +106,61 @@ def batched_section(steps, seq_cases):
     for c in sweep:
         b.add(c)
     tune = b.autotune(steps=5)
+    b.prepare()
     t0 = time.perf_counter()
     b.iterate(steps)
     wall = time.perf_counter() - t0
     b.close()
-    seq_nodes, seq_t = 0, 0.0
-    for c in sweep[:seq_cases]:
-        lat = c.build("f64")
-        t0 = time.perf_counter()
-        lat.iterate(steps)
-        seq_t += time.perf_counter() - t0
-        seq_nodes += c.nodes
-        lat.close()
-    seq_mlups = seq_nodes * steps / seq_t / 1e6
     batch_mlups = nodes * steps / wall / 1e6
-    return {
+    out = {
         "workload": f"256 d2q9 (MRT) cylinder cases (nx 128-1024, ny 64-256, Re 20-200), {nodes} nodes, "
                     f"{steps} steps timed, FP16-shift storage",
         "MLUPS": round(batch_mlups, 1),
         "solves_per_s_8000_steps": round(256 / (wall * 8000 / steps), 2),
         "tuning": tune,
-        "sequential_f64_MLUPS": round(seq_mlups, 1),
-        "sequential_f64_sample_cases": seq_cases,
-        "speedup_solves_per_s": round(batch_mlups / seq_mlups, 2),
+        "sequential_sample_cases": seq_cases,
     }
+    for st in ("f16s", "f64"):
+        seq_nodes, seq_t = 0, 0.0
+        for c in sweep[:seq_cases]:
+            lat = c.build(st)
+            lat.prepare()
+            t0 = time.perf_counter()
+            lat.iterate(steps)
+            seq_t += time.perf_counter() - t0
+            seq_nodes += c.nodes
+            lat.close()
+        seq_mlups = seq_nodes * steps / seq_t / 1e6
+        out[f"sequential_{st}_MLUPS"] = round(seq_mlups, 1)
+        out[f"speedup_vs_sequential_{st}"] = round(batch_mlups / seq_mlups, 2)
+    return out
 
 
-def serving_section(rate, window, sample):
+SERVE_KEEP = ["rate_per_s", "window_s", "arrival_span_s", "requests", "completed", "stopped_by_rule", "rejected",
+              "makespan_s", "solves_per_s", "latency_p50_s", "latency_p99_s", "queue_wait_p50_s", "MLUPS",
+              "MLUPS_while_busy", "mean_nodes", "mean_steps_run",
+              "sequential_f16s_MLUPS", "sequential_f16s_solves_per_s", "speedup_vs_sequential_f16s",
+              "sequential_f64_MLUPS", "sequential_f64_solves_per_s", "speedup_vs_sequential_f64",
+              "sequential_sample"]
+
+
+def _serve_keep(r):
+    return {k: (round(r[k], 4) if isinstance(r[k], float) else r[k]) for k in SERVE_KEEP if k in r}
+
+
+def serving_section(rate, window, sample, nsat):
+    """The d3q19 serving stream: at the stated 50 req/s (backlogged: the
+    first `nsat` requests arrive within half a second, far faster than they
+    are served, so solves/s is the server's capacity) and at a rate below
+    capacity (latency without queueing)."""
     from tclb_amd import serve
-    r = serve.run(rate=rate, window=window, storage="f16s", budget_gb=240.0, sample=sample)
-    keep = ["rate_per_s", "window_s", "requests", "completed", "stopped_by_rule", "solves_per_s",
-            "latency_p50_s", "latency_p99_s", "queue_wait_p50_s", "MLUPS", "MLUPS_while_busy",
-            "sequential_f64_MLUPS", "sequential_f64_solves_per_s", "speedup_solves_per_s"]
-    out = {k: (round(r[k], 4) if isinstance(r.get(k), float) else r.get(k)) for k in keep}
-    out["workload"] = ("Poisson stream of d3q19 porous boxes (sides log-U[48,256], porosity U[0,25]%, "
-                       "g_x U[1e-7,1e-5], tau U[0.52,0.9], budget U[1k,20k] steps or VOLvx relative change "
-                       "< 1e-6 over 500 steps), FP16-shift storage, 240 GB HBM budget")
+    out = {"workload": ("Poisson stream of d3q19 porous boxes (sides log-U[48,256], porosity U[0,25]%, "
+                        "g_x U[1e-7,1e-5], tau U[0.52,0.9], budget U[1k,20k] steps or VOLvx relative change "
+                        "< 1e-6 over 500 steps), FP16-shift storage, 240 GB HBM budget")}
+    out["saturated"] = _serve_keep(serve.saturated(nreq=nsat, rate=50.0, storage="f16s", budget_gb=240.0,
+                                                   sample=sample, log=log))
+    out["below_capacity"] = _serve_keep(serve.run(rate=rate, window=window, storage="f16s", budget_gb=240.0,
+                                                  sample=sample, log=log))
     return out
 
 
@@ -129,6 +171,7 @@ def cylinder_section(steps):
     from tclb_amd import cases
     c = cases.d2q9_cylinder(steps=steps)
     lat = c.build("f64")
+    lat.prepare()
     lat.iterate(100)
     t0 = time.perf_counter()
     for _ in range(steps // 1000):
@@ -165,6 +208,9 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--serve-rate", type=float, default=2.0)
     ap.add_argument("--serve-window", type=float, default=15.0)
+    ap.add_argument("--serve-saturated", type=int, default=24,
+                    help="requests of the 50 req/s stream played backlogged")
+    ap.add_argument("--perm-steps", type=int, default=5000, help="FP64 steps before the permeability is read")
     ap.add_argument("--cavity-n", type=int, default=640)
     ap.add_argument("--cavity-steps", type=int, default=300)
     args = ap.parse_args()
@@ -183,8 +229,14 @@ def main():
     log(f"device copy ceiling {copy_gbs:.0f} GB/s")
     traffic = traffic_table()
     results = {}
+    nu = (0.6 - 0.5) / 3     # tau = 0.6
+    g_x = 1e-6
+    permeability = None
     for fmt in args.formats.split(","):
         lat = case.build(fmt)
+        # the rows-per-tile timing and the graph capture happen here, not in
+        # the timed steps
+        lat.prepare()
         lat.iterate(args.warmup)
         t0 = time.perf_counter()
         lat.iterate(args.steps, tclb_amd.ITER_NORM | tclb_amd.ITER_LASTGLOB)
@@ -194,6 +246,9 @@ def main():
         per_launch_ms = ev_ms / args.steps
         achieved = bpn * N / (per_launch_ms * 1e-3) / 1e9
         g = lat.global_dict()
+        # VOLvx sums the fluid nodes' velocity scaled by 1.9 (the reference
+        # computes its Globals after R /= Mw, Dynamics.c.Rt:262-289)
+        ux_fluid = g["VOLvx"] / g["VOLvolume"] / 1.9
         results[fmt] = {
             "MLUPS": N * args.steps / wall / 1e6,
             "ms_per_step": wall * 1e3 / args.steps,
@@ -201,10 +256,31 @@ def main():
             "bytes_per_node": bpn,
             "achieved_GBs": achieved,
             "roofline_frac": achieved / HBM_PEAK_GBS,
-            "mean_ux_lattice": g["VOLvx"] / g["VOLvolume"] / 1.9,
+            "steps_run": args.warmup + args.steps,
+            "mean_ux_fluid": float(f"{ux_fluid:.6g}"),
         }
         log(f"{fmt}: {results[fmt]['MLUPS']:.0f} MLUPS, {achieved:.0f} GB/s "
-            f"({100 * achieved / HBM_PEAK_GBS:.1f}% of 8 TB/s), {per_launch_ms:.3f} ms/step")
+            f"({100 * achieved / HBM_PEAK_GBS:.1f}% of 8 TB/s), {per_launch_ms:.3f} ms/step "
+            f"(wall {results[fmt]['ms_per_step']:.3f} ms/step)")
+        if fmt == "f64" and args.perm_steps > 0:
+            # the stated 5,000 steps, untimed, for the permeability
+            more = args.perm_steps - (args.warmup + args.steps)
+            if more > 0:
+                lat.iterate(more, tclb_amd.ITER_NORM | tclb_amd.ITER_LASTGLOB)
+            g = lat.global_dict()
+            ux_f = g["VOLvx"] / g["VOLvolume"] / 1.9
+            ux_d = g["VOLvx"] / N / 1.9
+            permeability = {
+                "steps": lat.iteration,
+                "mean_ux_fluid": float(f"{ux_f:.6g}"),
+                "darcy_velocity": float(f"{ux_d:.6g}"),
+                "porosity": float(f"{g['VOLvolume'] / N:.6g}"),
+                "k_lattice_units": float(f"{nu * ux_d / g_x:.6g}"),
+                "note": "k = nu <u_x>/g_x with <u_x> the Darcy (superficial) velocity, sum of u_x over all nodes; "
+                        "not converged at this step count: the viscous time of the 256-node channel, "
+                        "H^2/nu, is about 2e6 steps",
+            }
+            log(f"permeability after {lat.iteration} steps: k = {permeability['k_lattice_units']}")
         lat.close()
 
     main_fmt = "f64" if "f64" in results else list(results)[0]
@@ -241,22 +317,30 @@ def main():
                     for k, v in results.items()},
         "copy_ceiling_GBs": round(copy_gbs, 1),
     }
+    if permeability:
+        out["permeability"] = permeability
     if "f16s" in results and "f64" in results:
         out["f16s_over_f64"] = round(results["f16s"]["MLUPS"] / results["f64"]["MLUPS"], 3)
     if not args.no_extra:
         log("batched sweep")
         out["batched"] = batched_section(args.batch_steps, 16)
-        log(f"batched: {out['batched']['MLUPS']} MLUPS, x{out['batched']['speedup_solves_per_s']} vs sequential FP64")
+        bt = out["batched"]
+        log(f"batched: {bt['MLUPS']} MLUPS, x{bt['speedup_vs_sequential_f16s']} vs sequential f16s, "
+            f"x{bt['speedup_vs_sequential_f64']} vs sequential FP64")
         log("serving")
-        out["serving"] = serving_section(args.serve_rate, args.serve_window, 2)
-        log(f"serving: p50 {out['serving']['latency_p50_s']} s, p99 {out['serving']['latency_p99_s']} s")
+        out["serving"] = serving_section(args.serve_rate, args.serve_window, 2, args.serve_saturated)
+        sv = out["serving"]
+        log(f"serving: saturated {sv['saturated']['solves_per_s']} solves/s "
+            f"(x{sv['saturated'].get('speedup_vs_sequential_f16s')} vs f16s, "
+            f"x{sv['saturated'].get('speedup_vs_sequential_f64')} vs FP64); below capacity p50 "
+            f"{sv['below_capacity']['latency_p50_s']} s, p99 {sv['below_capacity']['latency_p99_s']} s")
         log("cylinder 1024x128")
         out["cylinder"] = cylinder_section(5000)
         log(f"cavity {args.cavity_n}^3")
         out["cavity"] = cavity_section(args.cavity_n, args.cavity_steps)
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        log(f"CPU baseline: oracle on {args.cpu_n}^3, {threads} threads")
+        threads = cpu_share()
+        log(f"CPU baseline: oracle on {args.cpu_n}^3, {threads} threads and 1 thread")
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_seconds, threads)
     print(json.dumps(out), flush=True)
 

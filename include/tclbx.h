@@ -163,6 +163,25 @@ int tclbx_set_graph_steps(tclbx_lattice* h, int steps);
  * choice of this engine; the reference's shape is fixed (32x4 threads). */
 int tclbx_set_rows_per_tile(tclbx_lattice* h, int rows);
 int tclbx_get_rows_per_tile(const tclbx_lattice* h, int* rows);
+/* Runs now what the first long tclbx_iterate would do before its steps: the
+ * rows-per-tile timing above (trial launches that leave the state as it is)
+ * and the capture of the step loop, so that a timed tclbx_iterate runs steps
+ * only.  An engine call with no counterpart in the reference, whose launch
+ * shape is fixed and which captures no graphs. */
+int tclbx_prepare(tclbx_lattice* h);
+/* NodeType-sorted dispatch of the plain steps (on = 1): a pass over the plain
+ * collision nodes (MRT, or BGK in d2q9_SRT) on the usual tiles, then a pass
+ * over every other node, from a list sorted by NodeType class (the NodeType
+ * bits below the zone) and rebuilt when the flags change, so each wavefront
+ * takes one branch of the reference's per-node switch
+ * (models/flow/d3q19/Dynamics.c.Rt:181-210, models/flow/d2q9_SRT/Dynamics.c:
+ * 40-67).  Results are those of the single pass, bit for bit.  The
+ * environment's TCLBX_SORTED_DISPATCH=1 turns it on for new lattices.  The
+ * reference's only split is border versus interior, for MPI overlap
+ * (src/LatticeContainer.inc.cpp.Rt:201-287). */
+int tclbx_set_sorted_dispatch(tclbx_lattice* h, int on);
+/* whether it is on, and the nodes of the sorted list (0 when off) */
+int tclbx_get_sorted_dispatch(tclbx_lattice* h, int* on, long long* list_nodes);
 
 /* ---------------------------------------------------------------- batch --
  * Many independent lattices ("cases") of one model and storage format in
@@ -205,6 +224,9 @@ int tclbx_batch_get_timing(const tclbx_batch* b, double* last_ms, double* total_
 int tclbx_batch_info(const tclbx_batch* b, int* ncases, long long* nodes, long long* arena_used,
                      long long* arena_capacity);
 int tclbx_batch_set_graph_steps(tclbx_batch* b, int steps);
+/* The device tables of the current cases and the captured step loop, built
+ * now instead of in the next tclbx_batch_iterate. */
+int tclbx_batch_prepare(tclbx_batch* b);
 
 /* --------------------------------------------------------------- server --
  * Continuous-batching solve server: requests arrive from any thread; a

@@ -35,13 +35,14 @@ SYMBOLS = [
     "tclbx_get_density", "tclbx_set_density", "tclbx_extra_density_name", "tclbx_reset_average", "tclbx_save",
     "tclbx_load", "tclbx_get_timing",
     "tclbx_bytes_per_node", "tclbx_set_block", "tclbx_set_graph_steps", "tclbx_copy_bandwidth",
-    "tclbx_set_rows_per_tile", "tclbx_get_rows_per_tile",
+    "tclbx_set_rows_per_tile", "tclbx_get_rows_per_tile", "tclbx_prepare",
+    "tclbx_set_sorted_dispatch", "tclbx_get_sorted_dispatch",
     "tclbx_batch_create", "tclbx_batch_destroy", "tclbx_batch_case_bytes", "tclbx_batch_add_case",
     "tclbx_batch_remove_case", "tclbx_batch_set_setting", "tclbx_batch_set_zone_setting",
     "tclbx_batch_flag_overwrite", "tclbx_batch_init_case", "tclbx_batch_iterate", "tclbx_batch_get_globals",
     "tclbx_batch_get_iteration", "tclbx_batch_get_quantity", "tclbx_batch_get_density",
     "tclbx_batch_set_density", "tclbx_batch_autotune", "tclbx_batch_set_tuning", "tclbx_batch_get_timing",
-    "tclbx_batch_info", "tclbx_batch_set_graph_steps",
+    "tclbx_batch_info", "tclbx_batch_set_graph_steps", "tclbx_batch_prepare",
     "tclbx_server_create", "tclbx_server_destroy", "tclbx_server_submit", "tclbx_server_poll",
     "tclbx_server_wait_all", "tclbx_server_stats", "tclbx_server_set_tuning",
     "tclbx_case_load", "tclbx_case_load_string", "tclbx_case_destroy", "tclbx_case_size", "tclbx_case_units",
@@ -110,6 +111,9 @@ def lib():
     L.tclbx_set_graph_steps.argtypes = [vp, c_int]
     L.tclbx_set_rows_per_tile.argtypes = [vp, c_int]
     L.tclbx_get_rows_per_tile.argtypes = [vp, ip]
+    L.tclbx_prepare.argtypes = [vp]
+    L.tclbx_set_sorted_dispatch.argtypes = [vp, c_int]
+    L.tclbx_get_sorted_dispatch.argtypes = [vp, ip, ctypes.POINTER(ctypes.c_longlong)]
     L.tclbx_copy_bandwidth.argtypes = [ctypes.c_longlong, c_int, dp, dp]
     ll, llp = ctypes.c_longlong, ctypes.POINTER(ctypes.c_longlong)
     L.tclbx_batch_create.argtypes = [c_char_p, c_char_p, ll, ctypes.POINTER(vp)]
@@ -132,6 +136,7 @@ def lib():
     L.tclbx_batch_get_timing.argtypes = [vp, dp, dp]
     L.tclbx_batch_info.argtypes = [vp, ip, llp, llp, llp]
     L.tclbx_batch_set_graph_steps.argtypes = [vp, c_int]
+    L.tclbx_batch_prepare.argtypes = [vp]
     L.tclbx_server_create.argtypes = [c_char_p, c_char_p, ll, c_int, ctypes.POINTER(vp)]
     L.tclbx_server_destroy.argtypes = [vp]
     L.tclbx_server_submit.argtypes = [vp, c_int, c_int, c_int, u16p, c_int, ip, dp, c_int, ip, ip, dp, ll,
@@ -442,6 +447,28 @@ class Lattice:
     def set_graph_steps(self, steps):
         _check(lib().tclbx_set_graph_steps(self._h, int(steps)))
 
+    def prepare(self):
+        """Time the launch shape and capture the step loop now (no state
+        change), so that the next iterate() runs steps only."""
+        _check(lib().tclbx_prepare(self._h))
+
+    @property
+    def sorted_dispatch(self):
+        """NodeType-sorted dispatch of the plain steps (bulk collision pass,
+        then a pass over the other nodes sorted by NodeType class)."""
+        on = ctypes.c_int()
+        _check(lib().tclbx_get_sorted_dispatch(self._h, ctypes.byref(on), None))
+        return bool(on.value)
+
+    @sorted_dispatch.setter
+    def sorted_dispatch(self, on):
+        _check(lib().tclbx_set_sorted_dispatch(self._h, int(bool(on))))
+
+    def sorted_list_nodes(self):
+        n = ctypes.c_longlong()
+        _check(lib().tclbx_get_sorted_dispatch(self._h, None, ctypes.byref(n)))
+        return n.value
+
     @property
     def rows_per_tile(self):
         """Rows walked by one block of a plain step (tuned on the first long
@@ -591,6 +618,10 @@ class Batch:
 
     def set_graph_steps(self, steps):
         _check(lib().tclbx_batch_set_graph_steps(self._h, int(steps)))
+
+    def prepare(self):
+        """Build the device tables and capture the step loop now."""
+        _check(lib().tclbx_batch_prepare(self._h))
 
 
 STOP_TYPES = {"Change": 0, "PercentChange": 1, "Above": 2, "Below": 3}

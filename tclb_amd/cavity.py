@@ -33,6 +33,7 @@ def run(n=640, steps=20000, formats=("f64", "f32s", "f16s", "bf16s"), u_lid=0.05
     ref = None
     for fmt in formats:
         lat = case.build(fmt)
+        lat.prepare()
         done = 0
         t0 = time.perf_counter()
         while done < steps:

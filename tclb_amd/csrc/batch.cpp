@@ -351,6 +351,34 @@ void Batch::drop_graph() {
     graph_version_ = ~0u;
 }
 
+// graph_steps_ plain steps from phase 0, captured once per set of cases
+void Batch::capture_graph() {
+    if (graph_exec_ != nullptr && graph_version_ == version_) return;
+    drop_graph();
+    hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture");
+    for (int k = 0; k < graph_steps_; k++) {
+        const int nt = ntiles_;
+        const int grid = tune_.grid > 0 ? std::min(tune_.grid, nt) : nt;
+        hipError_t e = launch_batch_step(model->id, storage, false, d_cases_, d_tiles_, nt, k & 1, d_gpart_, grid,
+                                         tune_.block, stream_);
+        if (e != hipSuccess) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(stream_, &g);
+            if (g) (void)hipGraphDestroy(g);
+            hip_check(e, "batch step kernel (capture)");
+        }
+    }
+    hip_check(hipStreamEndCapture(stream_, &graph_), "end capture");
+    hip_check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "graph instantiate");
+    graph_version_ = version_;
+}
+
+void Batch::Prepare() {
+    if (dirty_) rebuild();
+    upload_zones();
+    if (ncases_dev_ > 0 && graph_steps_ >= 2) capture_graph();
+}
+
 void Batch::Iterate(int niter, int iter_type) {
     if (niter < 0) throw std::invalid_argument("negative iteration count");
     if (dirty_) rebuild();
@@ -363,31 +391,15 @@ void Batch::Iterate(int niter, int iter_type) {
     hip_check(hipEventRecord(ev0_, stream_), "event record");
     int i = 0;
     const int plain_end = niter - (last_glob ? 1 : 0);
-    if (!every_glob && graph_steps_ >= 2 && plain_end - i >= 2 * graph_steps_) {
+    const bool captured = graph_exec_ != nullptr && graph_version_ == version_;
+    const int lead = phase_ != 0 ? 1 : 0;
+    if (!every_glob && graph_steps_ >= 2 && plain_end - lead >= (captured ? 1 : 2) * graph_steps_) {
         if (phase_ != 0) {
             launch(phase_, false);
             phase_ ^= 1;
             i++;
         }
-        if (graph_exec_ == nullptr || graph_version_ != version_) {
-            drop_graph();
-            hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture");
-            for (int k = 0; k < graph_steps_; k++) {
-                const int nt = ntiles_;
-                const int grid = tune_.grid > 0 ? std::min(tune_.grid, nt) : nt;
-                hipError_t e = launch_batch_step(model->id, storage, false, d_cases_, d_tiles_, nt, k & 1,
-                                                 d_gpart_, grid, tune_.block, stream_);
-                if (e != hipSuccess) {
-                    hipGraph_t g = nullptr;
-                    (void)hipStreamEndCapture(stream_, &g);
-                    if (g) (void)hipGraphDestroy(g);
-                    hip_check(e, "batch step kernel (capture)");
-                }
-            }
-            hip_check(hipStreamEndCapture(stream_, &graph_), "end capture");
-            hip_check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "graph instantiate");
-            graph_version_ = version_;
-        }
+        capture_graph();
         const int reps = (plain_end - i) / graph_steps_;
         for (int r = 0; r < reps; r++) hip_check(hipGraphLaunch(graph_exec_, stream_), "graph launch");
         i += reps * graph_steps_;

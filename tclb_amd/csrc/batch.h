@@ -75,6 +75,9 @@ public:
     void GetFlags(int id, uint16_t* out);
     void InitCase(int id);  // Init() of one case
     void Iterate(int niter, int iter_type);
+    // rebuild the device tables and capture the step loop now, so that the
+    // next Iterate runs steps only
+    void Prepare();
     const std::vector<double>& Globals(int id) const;
     long long CaseIteration(int id) const;
     void GetQuantity(int id, int q, double* out, double scale);
@@ -118,6 +121,7 @@ private:
     void upload_zones();
     void launch(int phase, bool glob);
     void drop_graph();
+    void capture_graph();
     Case& get(int id);
     const Case& get(int id) const;
 

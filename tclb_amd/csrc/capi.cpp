@@ -297,6 +297,31 @@ int tclbx_set_graph_steps(tclbx_lattice* h, int steps) {
     return TCLBX_OK;
 }
 
+int tclbx_set_sorted_dispatch(tclbx_lattice* h, int on) {
+    CHECK_H(h);
+    return guard([&]() {
+        h->lat->set_sorted_dispatch(on != 0);
+        return TCLBX_OK;
+    });
+}
+
+int tclbx_get_sorted_dispatch(tclbx_lattice* h, int* on, long long* list_nodes) {
+    CHECK_H(h);
+    return guard([&]() {
+        if (on) *on = h->lat->sorted_dispatch() ? 1 : 0;
+        if (list_nodes) *list_nodes = h->lat->sorted_dispatch() ? (long long)h->lat->sorted_list_size() : 0;
+        return TCLBX_OK;
+    });
+}
+
+int tclbx_prepare(tclbx_lattice* h) {
+    CHECK_H(h);
+    return guard([&]() {
+        h->lat->Prepare();
+        return TCLBX_OK;
+    });
+}
+
 int tclbx_set_rows_per_tile(tclbx_lattice* h, int rows) {
     CHECK_H(h);
     return guard([&]() {
@@ -435,6 +460,14 @@ int tclbx_batch_info(const tclbx_batch* b, int* ncases, long long* nodes, long l
     if (arena_capacity) *arena_capacity = (long long)b->b->arena_capacity();
     return TCLBX_OK;
 }
+int tclbx_batch_prepare(tclbx_batch* b) {
+    CHECK_B(b);
+    return guard([&]() {
+        b->b->Prepare();
+        return TCLBX_OK;
+    });
+}
+
 int tclbx_batch_set_graph_steps(tclbx_batch* b, int steps) {
     CHECK_B(b);
     if (steps < 0 || (steps % 2) != 0) return fail("graph steps must be a non-negative even number");

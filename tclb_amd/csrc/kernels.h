@@ -37,6 +37,15 @@ struct StepArgs {
     double inv_nx;         // 1/nx (flat tiles: node index -> row)
     int zcols, nzones;
     unsigned ib_mask;      // NodeType bit of IB (the cumulant IB builds), 0 for every other model
+    // NodeType-sorted dispatch of a plain step: mode 0 updates every node of
+    // the tiles; mode 1 (bulk) only the nodes whose class (NodeType bits
+    // below the zone) is bulk_val, the plain collision node, so that every
+    // wavefront takes the same branch; mode 2 the list[0..list_n) of the
+    // other nodes, sorted by class, one thread each
+    int mode;
+    unsigned class_mask, bulk_val;
+    const unsigned* list;
+    unsigned list_n;
     double par[MAX_PAR];
 };
 

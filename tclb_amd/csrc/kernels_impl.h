@@ -1683,7 +1683,7 @@ __device__ __forceinline__ void d2q9_autosym(R* f, unsigned nt, LOAD load) {
 // One node of row `row` (y + ny*z) at x = xc: pull with periodic wrap, Run(),
 // store; `active` is false for the lanes past the end of the row (they load
 // a valid node but store nothing).
-template <int MODEL, int ST, bool GLOB>
+template <int MODEL, int ST, bool GLOB, int DISPATCH = 0>
 __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, int xm, int xp, bool active,
                                             double* g) {
     using L = Lat<MODEL>;
@@ -1702,23 +1702,29 @@ __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, 
     int zone = (int)(nt >> L::ZSHIFT);
     if (zone >= a.nzones) zone = 0;
     const double* zr = a.zone + (size_t)zone * a.zcols;
+    // the bulk pass leaves every node of another class to the list pass,
+    // and loads nothing for it
+    if constexpr (DISPATCH == 1) active = active && (nt & a.class_mask) == a.bulk_val;
 
     // plane base pointers are uniform (scalar registers); the per-lane part
-    // of every address is a 32-bit byte offset (a plane is < 4 GiB)
+    // of every address is a 32-bit byte offset (a plane is at most 4 GiB:
+    // check_lattice_size refuses larger ones)
     const char* fin = (const char*)a.fin;
     const size_t pbytes = (size_t)a.pitch * sizeof(T);
     // d3q19 in a shifted format collides on the stored deviations f - w
     constexpr bool DEV = is_d3q19<MODEL> && S::shifted;
     R f[Q];
+    if (DISPATCH != 1 || active) {
 #pragma unroll
-    for (int i = 0; i < Q; i++) {
-        // source node x - e_i with periodic wrap
-        const int sx = L::ex(i) > 0 ? xm : (L::ex(i) < 0 ? xp : xc);
-        const int sy = L::ey(i) > 0 ? ym : (L::ey(i) < 0 ? yp : y);
-        const int sz = L::ez(i) > 0 ? zm : (L::ez(i) < 0 ? zp : z);
-        const unsigned src = (unsigned)sx + (unsigned)nx * ((unsigned)sy + (unsigned)ny * (unsigned)sz);
-        const T v = *(const T*)(fin + (size_t)i * pbytes + src * (unsigned)sizeof(T));
-        f[i] = DEV ? S::raw(v) : S::get(v, (R)L::w(i));
+        for (int i = 0; i < Q; i++) {
+            // source node x - e_i with periodic wrap
+            const int sx = L::ex(i) > 0 ? xm : (L::ex(i) < 0 ? xp : xc);
+            const int sy = L::ey(i) > 0 ? ym : (L::ey(i) < 0 ? yp : y);
+            const int sz = L::ez(i) > 0 ? zm : (L::ez(i) < 0 ? zp : z);
+            const unsigned src = (unsigned)sx + (unsigned)nx * ((unsigned)sy + (unsigned)ny * (unsigned)sz);
+            const T v = *(const T*)(fin + (size_t)i * pbytes + src * (unsigned)sizeof(T));
+            f[i] = DEV ? S::raw(v) : S::get(v, (R)L::w(i));
+        }
     }
     if constexpr (autosym_kind<MODEL> != 0) {
         d2q9_autosym<MODEL>(f, nt, [&](int fi, int ox, int oy) {
@@ -1777,7 +1783,7 @@ __device__ __forceinline__ void split_node(unsigned n, int nx, double inv_nx, in
     x = xx;
 }
 
-template <int MODEL, int ST, bool GLOB>
+template <int MODEL, int ST, bool GLOB, int DISPATCH = 0>
 __device__ __forceinline__ void step_tile(const StepArgs& a, int tile, int rpb, int slot, int nslots) {
     using L = Lat<MODEL>;
     if (a.flat) {
@@ -1795,7 +1801,7 @@ __device__ __forceinline__ void step_tile(const StepArgs& a, int tile, int rpb, 
             int row, x;
             split_node(in ? n : N - 1, a.nx, a.inv_nx, row, x);
             const int xm = x == 0 ? a.nx - 1 : x - 1, xp = x == a.nx - 1 ? 0 : x + 1;
-            node_update<MODEL, ST, GLOB>(a, row, x, xm, xp, in, g);
+            node_update<MODEL, ST, GLOB, DISPATCH>(a, row, x, xm, xp, in, g);
         }
         if (GLOB) {
             block_globals<L::G>(g, L::SUMG, a.gpart, slot, nslots);
@@ -1815,7 +1821,7 @@ __device__ __forceinline__ void step_tile(const StepArgs& a, int tile, int rpb, 
     }
     const int nrows = a.ny * a.nz;
     for (int row = rowblk * rpb; row < (rowblk + 1) * rpb && row < nrows; row++)
-        node_update<MODEL, ST, GLOB>(a, row, xc, xm, xp, xin, g);
+        node_update<MODEL, ST, GLOB, DISPATCH>(a, row, xc, xm, xp, xin, g);
     if (GLOB) {
         block_globals<L::G>(g, L::SUMG, a.gpart, slot, nslots);
     }
@@ -1838,6 +1844,30 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a) {
 template <int MODEL, int ST, bool GLOB, int W>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) k_step_w(StepArgs a) {
     step_tile<MODEL, ST, GLOB>(a, blockIdx.x, GLOB ? a.rpb : a.rpt, blockIdx.x, gridDim.x);
+}
+
+// NodeType-sorted dispatch of a plain step.  The bulk pass walks the tiles
+// of the plain step and updates only the nodes of the plain collision class;
+// the list pass takes one node of the class-sorted list per thread, so a
+// wavefront meets at most two classes only where the list changes class.
+// W: waves per SIMD the kernel is held to (0: none).
+template <int MODEL, int ST, int W>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W > 0 ? W : 1)))
+k_step_bulk(StepArgs a) {
+    step_tile<MODEL, ST, false, 1>(a, blockIdx.x, a.rpt, 0, 0);
+}
+
+template <int MODEL, int ST, int W>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W > 0 ? W : 1)))
+k_step_list(StepArgs a) {
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = i < a.list_n;
+    const unsigned n = a.list[in ? i : a.list_n - 1];
+    int row, x;
+    split_node(n, a.nx, a.inv_nx, row, x);
+    const int xm = x == 0 ? a.nx - 1 : x - 1, xp = x == a.nx - 1 ? 0 : x + 1;
+    double g[1];
+    node_update<MODEL, ST, false>(a, row, x, xm, xp, in, g);
 }
 
 // waves per SIMD the plain step of MODEL in storage ST is held to (0: none)
@@ -2130,7 +2160,13 @@ inline int nblocks_of(const StepArgs& a, int block) { return a.bpr * a.ny * a.nz
 template <int MODEL, int ST>
 hipError_t step_ms(bool glob, const StepArgs& a, int block, hipStream_t s) {
     dim3 grid(step_blocks(a, glob, block)), blk(block);
-    if (glob) {
+    if (!glob && a.mode == 1) {
+        hipLaunchKernelGGL((k_step_bulk<MODEL, ST, step_waves<MODEL, ST>()>), grid, blk, 0, s, a);
+    } else if (!glob && a.mode == 2) {
+        if (a.list_n == 0) return hipSuccess;
+        hipLaunchKernelGGL((k_step_list<MODEL, ST, step_waves<MODEL, ST>()>),
+                           dim3((a.list_n + block - 1) / block), blk, 0, s, a);
+    } else if (glob) {
         hipLaunchKernelGGL((k_step<MODEL, ST, true>), grid, blk, 0, s, a);
     } else if constexpr (step_waves<MODEL, ST>() > 0) {
         hipLaunchKernelGGL((k_step_w<MODEL, ST, false, step_waves<MODEL, ST>()>), grid, blk, 0, s, a);

@@ -100,6 +100,13 @@ Lattice::Lattice(const ModelDef* m, int nx_, int ny_, int nz_, int storage_)
     hip_check(hipMalloc((void**)&d_glob_, MAX_GLOBALS * sizeof(double)), "hipMalloc(globals)");
     hip_check(hipMemsetAsync(d_glob_, 0, MAX_GLOBALS * sizeof(double), stream_), "hipMemset(globals)");
     globals_.assign(G, 0.0);
+    // the class of a node: its NodeType bits below the zone; the bulk class
+    // is the plain collision node (MRT, or BGK in d2q9_SRT)
+    class_mask_ = (1u << m->zone_shift) - 1u;
+    const NodeTypeDef* coll = m->nodetype("MRT");
+    if (!coll) coll = m->nodetype("BGK");
+    bulk_val_ = coll ? coll->value : 0u;
+    if (const char* e = std::getenv("TCLBX_SORTED_DISPATCH")) sorted_ = std::atoi(e) != 0 && coll != nullptr;
     hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
 }
 
@@ -114,6 +121,7 @@ Lattice::~Lattice() {
     if (d_zone_) (void)hipFree(d_zone_);
     if (d_gpart_) (void)hipFree(d_gpart_);
     if (d_glob_) (void)hipFree(d_glob_);
+    if (d_list_) (void)hipFree(d_list_);
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
     if (stream_) (void)hipStreamDestroy(stream_);
@@ -180,6 +188,7 @@ static bool region_in(const Region& r, int nx, int ny, int nz) {
 
 void Lattice::FlagOverwrite(const uint16_t* flags, const Region& over) {
     if (!region_in(over, nx, ny, nz)) throw std::out_of_range("flag region outside the lattice");
+    flags_version_++;
     if (over.dx == 0 && over.dy == 0 && over.dz == 0 && over.nx == nx && over.ny == ny && over.nz == nz) {
         hip_check(hipMemcpy(d_flags_, flags, N * sizeof(uint16_t), hipMemcpyHostToDevice), "FlagOverwrite");
         return;
@@ -271,8 +280,52 @@ StepArgs Lattice::args(int from, int to, bool glob) const {
     a.nzones = nzones_;
     const NodeTypeDef* ib = model->nodetype("IB");
     a.ib_mask = ib ? ib->value : 0u;
+    a.mode = 0;
+    a.class_mask = class_mask_;
+    a.bulk_val = bulk_val_;
+    a.list = d_list_;
+    a.list_n = (unsigned)list_n_;
     model_params(model, settings_, a.par);
     return a;
+}
+
+void Lattice::set_sorted_dispatch(bool on) {
+    if (on && !model->nodetype("MRT") && !model->nodetype("BGK"))
+        throw std::invalid_argument("model " + model->name + " has no plain collision NodeType to sort around");
+    sorted_ = on;
+    drop_graph();
+}
+
+// The nodes outside the bulk class, sorted by class and, within a class, by
+// index: a counting sort of the flags read back from the device.
+void Lattice::ensure_list() {
+    if (!sorted_ || list_version_ == flags_version_) return;
+    std::vector<uint16_t> fl(N);
+    hip_check(hipMemcpy(fl.data(), d_flags_, N * sizeof(uint16_t), hipMemcpyDeviceToHost), "read flags");
+    std::vector<size_t> start((size_t)class_mask_ + 2, 0);
+    for (size_t n = 0; n < N; n++) {
+        const unsigned c = fl[n] & class_mask_;
+        if (c != bulk_val_) start[c + 1]++;
+    }
+    for (size_t c = 1; c < start.size(); c++) start[c] += start[c - 1];
+    const size_t count = start.back();
+    std::vector<unsigned> list(std::max<size_t>(count, 1));
+    for (size_t n = 0; n < N; n++) {
+        const unsigned c = fl[n] & class_mask_;
+        if (c != bulk_val_) list[start[c]++] = (unsigned)n;
+    }
+    if (count > list_cap_) {
+        if (d_list_) (void)hipFree(d_list_);
+        d_list_ = nullptr;
+        list_cap_ = 0;
+        hip_check(hipMalloc((void**)&d_list_, count * sizeof(unsigned)), "hipMalloc(node list)");
+        list_cap_ = count;
+    }
+    if (count > 0)
+        hip_check(hipMemcpy(d_list_, list.data(), count * sizeof(unsigned), hipMemcpyHostToDevice), "upload node list");
+    list_n_ = count;
+    list_version_ = flags_version_;
+    params_version_++;  // a captured step loop holds the old list
 }
 
 void model_params(const ModelDef* model, const std::vector<double>& settings, double* par) {
@@ -330,6 +383,13 @@ int globals_rows_per_block(long long rows, int bpr) {
 
 void Lattice::launch(int from, int to, bool glob) {
     StepArgs a = args(from, to, glob);
+    if (sorted_ && !glob) {
+        a.mode = 1;
+        hip_check(launch_step(model->id, storage, false, a, block_, stream_), "step kernel (bulk)");
+        a.mode = 2;
+        hip_check(launch_step(model->id, storage, false, a, block_, stream_), "step kernel (sorted list)");
+        return;
+    }
     hip_check(launch_step(model->id, storage, glob, a, block_, stream_), "step kernel");
     if (glob) {
         hip_check(launch_reduce_globals(d_gpart_, step_blocks(a, true, block_), kernel_nglobals(model->id),
@@ -353,7 +413,15 @@ void Lattice::run_graph_steps(int nsteps) {
         hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture");
         for (int i = 0; i < graph_steps_; i++) {
             StepArgs a = args(i % 2, (i + 1) % 2, false);
-            hipError_t e = launch_step(model->id, storage, false, a, block_, stream_);
+            hipError_t e = hipSuccess;
+            if (sorted_) {
+                a.mode = 1;
+                e = launch_step(model->id, storage, false, a, block_, stream_);
+                a.mode = 2;
+                if (e == hipSuccess) e = launch_step(model->id, storage, false, a, block_, stream_);
+            } else {
+                e = launch_step(model->id, storage, false, a, block_, stream_);
+            }
             if (e != hipSuccess) {
                 hipGraph_t g = nullptr;
                 (void)hipStreamEndCapture(stream_, &g);
@@ -425,11 +493,23 @@ void Lattice::tune_rows() {
     drop_graph();
 }
 
+void Lattice::Prepare() {
+    upload_zone();
+    ensure_list();
+    if (tune_rpt_ && N >= (1u << 20)) tune_rows();
+    if (graph_steps_ >= 2) {
+        run_graph_steps(0);  // capture only
+        hip_check(hipStreamSynchronize(stream_), "Prepare");
+    }
+}
+
 void Lattice::Iterate(int niter, int iter_type) {
     if (niter < 0) throw std::invalid_argument("negative iteration count");
     upload_zone();
+    ensure_list();
     // lattices large enough for the tiling to matter are tuned once, on
     // the first Iterate that runs enough steps to repay the trial launches
+    // (or earlier, by Prepare)
     if (tune_rpt_ && niter >= 16 && N >= (1u << 20)) tune_rows();
     int last_glob = iter_type & ITER_LASTGLOB;
     if (iter_type & ITER_INTEG) last_glob = 0;
@@ -437,9 +517,13 @@ void Lattice::Iterate(int niter, int iter_type) {
     hip_check(hipEventRecord(ev0_, stream_), "event record");
     int i = 0;
     // leading plain steps until snapshot 0 is current, then whole graph
-    // replays, then the tail (the Globals step is always launched directly)
+    // replays, then the tail (the Globals step is always launched directly);
+    // a loop not yet captured is captured only when it will be replayed at
+    // least twice
     const int plain_end = niter - (last_glob ? 1 : 0);
-    if (!every_glob && graph_steps_ >= 2 && plain_end - i >= 2 * graph_steps_) {
+    const bool captured = graph_exec_ != nullptr && graph_version_ == params_version_;
+    const int lead = snap_ != 0 ? 1 : 0;
+    if (!every_glob && graph_steps_ >= 2 && plain_end - lead >= (captured ? 1 : 2) * graph_steps_) {
         if (snap_ != 0) {
             launch(snap_, 1 - snap_, false);
             snap_ = 1 - snap_;

@@ -51,6 +51,9 @@ public:
     // densities to zero, average_iter counted from this iteration
     void resetAverage();
     void Iterate(int niter, int iter_type);   // src/Lattice.cu.Rt:900-989
+    // what the first long Iterate would do before its steps: time the rows
+    // per tile (the state is not changed) and capture the step loop
+    void Prepare();
     void clearGlobals();
     void calcGlobals();
     const std::vector<double>& globals() const { return globals_; }
@@ -79,6 +82,12 @@ public:
     // or 0 to let the first long Iterate time the three and keep the fastest
     void set_rows_per_tile(int r);
     int rows_per_tile() const { return rpt_; }
+    // NodeType-sorted dispatch of the plain steps: a bulk pass over the plain
+    // collision nodes, then one over the other nodes sorted by NodeType class
+    // (Globals steps always take the single pass)
+    void set_sorted_dispatch(bool on);
+    bool sorted_dispatch() const { return sorted_; }
+    size_t sorted_list_size() { ensure_list(); return list_n_; }
 
     const ModelDef* model;
     const int nx, ny, nz, storage;
@@ -92,6 +101,7 @@ private:
     void drop_graph();
     void tune_rows();
     void run_graph_steps(int nsteps);
+    void ensure_list();
 
     std::vector<double> settings_;
     std::vector<double> zone_;  // [zone][zcols]
@@ -118,6 +128,13 @@ private:
     int rpt_ = 1;
     bool tune_rpt_ = true;
     unsigned params_version_ = 0, graph_version_ = ~0u;
+    // sorted dispatch: the nodes outside the bulk class, sorted by class,
+    // rebuilt from the flags when they change
+    bool sorted_ = false;
+    unsigned class_mask_ = 0, bulk_val_ = 0;
+    unsigned* d_list_ = nullptr;
+    size_t list_n_ = 0, list_cap_ = 0;
+    unsigned flags_version_ = 0, list_version_ = ~0u;
 };
 
 // hipError_t -> exception with a message naming the call

@@ -311,6 +311,45 @@ def test_reduced_storage(oracle_mod, model, nx, ny, nz, storage):
         orc.iterate(1)
 
 
+@pytest.mark.parametrize("storage", ["f64", "f16s"])
+@pytest.mark.parametrize("model,nx,ny,nz", MODELS)
+def test_sorted_dispatch_is_bit_identical(model, nx, ny, nz, storage):
+    """NodeType-sorted dispatch (a bulk pass over the plain collision nodes,
+    then the class-sorted list of the others) gives the single pass's
+    distributions, averages and Globals bit for bit, on random cases holding
+    every node type of the model; also after a flag change (the list is
+    rebuilt) and across graph replays."""
+    say(f"sorted dispatch {model} {storage}")
+    case = random_case(model, nx, ny, nz, seed=31)
+    a = case.build(storage)
+    b = case.build(storage)
+    a.sorted_dispatch = True
+    assert a.sorted_dispatch and not b.sorted_dispatch
+    m = tclb_amd.Model(model)
+    coll = m.flag("BGK") if model.startswith("d2q9_SRT") else m.flag("MRT")
+    shift = m.group("SETTINGZONE")[1]
+    others = int(np.count_nonzero((case.flags & ((1 << shift) - 1)) != coll))
+    assert a.sorted_list_nodes() == others
+    a.set_graph_steps(4)
+    b.set_graph_steps(0)
+    for n in (3, 1, 12):
+        a.iterate(n)
+        b.iterate(n)
+        assert np.array_equal(a.get_f(), b.get_f()), n
+        assert np.array_equal(a.globals, b.globals), n
+    flags = case.flags.copy()
+    flags.reshape(-1)[:: 5] = flags.reshape(-1)[1:: 5][: flags.reshape(-1)[:: 5].size]
+    a.set_flags(flags)
+    b.set_flags(flags)
+    for n in (5, 9):
+        a.iterate(n)
+        b.iterate(n)
+        assert np.array_equal(a.get_f(), b.get_f()), n
+        assert np.array_equal(a.globals, b.globals), n
+    for name in m.extras:
+        assert np.array_equal(a.get_extra(name), b.get_extra(name)), name
+
+
 def test_graph_replay_is_bit_identical():
     say("graph replay vs plain launches")
     case = random_case("d3q19", 16, 8, 8, seed=21)
