@@ -122,6 +122,17 @@ int tclbx_get_iteration(const tclbx_lattice* h, int* iter);
  * evaluated on the pulled populations of the current snapshot. */
 int tclbx_get_quantity(tclbx_lattice* h, int q, int dx, int dy, int dz, int nx, int ny, int nz,
                        double* out, double scale);
+/* tclbx_get_quantity without any device allocation, for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * quantity kernel writes into a device array the library reserves on each
+ * GPU for the life of the process (64 MiB: at most 8,388,608 doubles, i.e.
+ * that many nodes of a scalar quantity or a third of them of a vector one;
+ * larger regions are refused with TCLBX_ERR), then the values are copied to
+ * `out` (host memory).  Calls on one GPU share that array: calls that
+ * overlap on one GPU race. */
+int tclbx_get_quantity_noalloc(tclbx_lattice* h, int q, int dx, int dy, int dz, int nx, int ny, int nz,
+                               double* out, double scale);
 /* Get_<f> / Set_<f> (src/Lattice.cu.Rt:1133-1172): density q of the whole
  * lattice, as real values (storage shift removed / applied). */
 int tclbx_get_density(tclbx_lattice* h, int q, double* out);

@@ -32,6 +32,7 @@ SYMBOLS = [
     "tclbx_set_zone_setting", "tclbx_flag_overwrite", "tclbx_get_flags", "tclbx_cuts_overwrite",
     "tclbx_get_cuts", "tclbx_init",
     "tclbx_iterate", "tclbx_get_globals", "tclbx_get_iteration", "tclbx_get_quantity",
+    "tclbx_get_quantity_noalloc",
     "tclbx_get_density", "tclbx_set_density", "tclbx_extra_density_name", "tclbx_reset_average", "tclbx_save",
     "tclbx_load", "tclbx_get_timing",
     "tclbx_bytes_per_node", "tclbx_set_block", "tclbx_set_graph_steps", "tclbx_copy_bandwidth",
@@ -98,6 +99,7 @@ def lib():
     L.tclbx_get_globals.argtypes = [vp, dp, c_int]
     L.tclbx_get_iteration.argtypes = [vp, ip]
     L.tclbx_get_quantity.argtypes = [vp, c_int] + [c_int] * 6 + [dp, c_double]
+    L.tclbx_get_quantity_noalloc.argtypes = [vp, c_int] + [c_int] * 6 + [dp, c_double]
     L.tclbx_get_density.argtypes = [vp, c_int, dp]
     L.tclbx_reset_average.argtypes = [vp]
     L.tclbx_extra_density_name.argtypes = [c_char_p, c_int, ip]

@@ -231,6 +231,16 @@ int tclbx_get_quantity(tclbx_lattice* h, int q, int dx, int dy, int dz, int nx, 
     if (!out) return fail("null output");
     return guard([&]() { h->lat->GetQuantity(q, Region{dx, dy, dz, nx, ny, nz}, out, scale); return TCLBX_OK; });
 }
+int tclbx_get_quantity_noalloc(tclbx_lattice* h, int q, int dx, int dy, int dz, int nx, int ny, int nz,
+                               double* out, double scale) {
+    CHECK_H(h);
+    if (!out) return fail("null output");
+    return guard([&]() {
+        h->lat->GetQuantityStaged(q, Region{dx, dy, dz, nx, ny, nz}, out, scale, quantity_scratch(),
+                                  quantity_scratch_doubles());
+        return TCLBX_OK;
+    });
+}
 int tclbx_get_density(tclbx_lattice* h, int q, double* out) {
     CHECK_H(h);
     if (!out) return fail("null output");

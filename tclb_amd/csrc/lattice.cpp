@@ -574,18 +574,37 @@ void Lattice::calcGlobals() {
 void Lattice::GetQuantity(int q, const Region& over, double* out, double scale) {
     if (q < 0 || q >= (int)model->quantities.size()) throw std::out_of_range("quantity id");
     if (!region_in(over, nx, ny, nz)) throw std::out_of_range("quantity region outside the lattice");
-    upload_zone();
     const size_t n = (size_t)over.nx * over.ny * over.nz;
     if (n == 0) return;
-    const size_t bytes = n * (model->quantities[q].vector ? 3 : 1) * sizeof(double);
+    const size_t count = n * (model->quantities[q].vector ? 3 : 1);
     double* d_out = nullptr;
-    hip_check(hipMalloc((void**)&d_out, bytes), "hipMalloc(quantity)");
+    hip_check(hipMalloc((void**)&d_out, count * sizeof(double)), "hipMalloc(quantity)");
+    try {
+        GetQuantityStaged(q, over, out, scale, d_out, count);
+    } catch (...) {
+        (void)hipFree(d_out);
+        throw;
+    }
+    hip_check(hipFree(d_out), "hipFree(quantity)");
+}
+
+void Lattice::GetQuantityStaged(int q, const Region& over, double* out, double scale, double* staging,
+                                size_t capacity) {
+    if (q < 0 || q >= (int)model->quantities.size()) throw std::out_of_range("quantity id");
+    if (!region_in(over, nx, ny, nz)) throw std::out_of_range("quantity region outside the lattice");
+    const size_t n = (size_t)over.nx * over.ny * over.nz;
+    if (n == 0) return;
+    const size_t count = n * (model->quantities[q].vector ? 3 : 1);
+    if (count > capacity)
+        throw std::invalid_argument("quantity region of " + std::to_string(n) + " nodes needs " +
+                                    std::to_string(count) + " doubles of staging; " + std::to_string(capacity) +
+                                    " are available");
+    upload_zone();
     StepArgs a = args(snap_, 1 - snap_, false);
     hipError_t e = launch_quantity(model->id, storage, q, a, over.dx, over.dy, over.dz, over.nx, over.ny,
-                                   over.nz, scale, d_out, stream_);
-    if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, bytes, hipMemcpyDeviceToHost, stream_);
+                                   over.nz, scale, staging, stream_);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, staging, count * sizeof(double), hipMemcpyDeviceToHost, stream_);
     if (e == hipSuccess) e = hipStreamSynchronize(stream_);
-    (void)hipFree(d_out);
     hip_check(e, "GetQuantity");
 }
 

@@ -59,6 +59,10 @@ public:
     const std::vector<double>& globals() const { return globals_; }
 
     void GetQuantity(int q, const Region& over, double* out, double scale);
+    // the same through `capacity` doubles of device staging the caller owns:
+    // no allocation
+    void GetQuantityStaged(int q, const Region& over, double* out, double scale, double* staging,
+                           size_t capacity);
     // Get_<f> of the current snapshot; q >= Q: extra density q - Q (the
     // model's averages, then its parameter densities)
     void GetDensity(int q, double* out);
@@ -148,6 +152,10 @@ int default_graph_steps();
 // nodes, and each density plane within 4 GiB (the step kernel's per-lane
 // addresses are 32-bit byte offsets from the plane base)
 void check_lattice_size(const ModelDef* m, int nx, int ny, int nz, int storage);
+// device staging of tclbx_get_quantity_noalloc (noalloc.hip): the current
+// device's copy of a file-scope array of quantity_scratch_doubles() doubles
+double* quantity_scratch();
+size_t quantity_scratch_doubles();
 // kernel parameter vector (StepArgs::par) of a model from its settings
 void model_params(const ModelDef* model, const std::vector<double>& settings, double* par);
 // rows walked by one block of a Globals step for a lattice of `rows` rows
