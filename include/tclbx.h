@@ -151,7 +151,7 @@ const char* tclbx_extra_density_name(const char* model, int k, int* is_average);
 int tclbx_reset_average(tclbx_lattice* h);
 /* saveSolution / loadSolution (src/Lattice.cu.Rt:708-769): raw densities
  * of the current snapshot in the storage format, then the extra densities
- * (FP64), then a 64-byte footer (model, storage, size) that a reader taking
+ * (FP64), then an 80-byte footer (model, storage, size) that a reader taking
  * only the bytes it expects ignores.  TCLBX_ERR if the file cannot be opened
  * or read, or holds another model, storage format or size (checked before
  * the lattice is touched); a file without the footer is checked by size. */

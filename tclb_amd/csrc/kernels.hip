@@ -6,6 +6,9 @@
 
 namespace tclbx {
 
+// the compile unit of a family that holds the kernels of this storage format
+#define TCLBX_PICK(name, storage) ((storage) == ST_F64 ? name##_f64 : name##_f32)
+
 int kernel_nglobals(int model) {
     switch (model) {
     case MODEL_D2Q9_SRT: return Lat<MODEL_D2Q9_SRT>::G;
@@ -124,20 +127,20 @@ int step_blocks(const StepArgs& a, bool globals, int block) {
 }
 
 hipError_t launch_step(int model, int storage, bool globals, const StepArgs& a, int block, hipStream_t s) {
-    if (hipError_t e = step_2d(model, storage, globals, a, block, s); e != hipErrorNotFound) return e;
-    if (hipError_t e = step_d3q19(model, storage, globals, a, block, s); e != hipErrorNotFound) return e;
-    if (hipError_t e = step_d3q27(model, storage, globals, a, block, s); e != hipErrorNotFound) return e;
-    if (hipError_t e = step_cum(model, storage, globals, a, block, s); e != hipErrorNotFound) return e;
-    if (hipError_t e = step_cumavg(model, storage, globals, a, block, s); e != hipErrorNotFound) return e;
+    if (hipError_t e = TCLBX_PICK(step_2d, storage)(model, storage, globals, a, block, s); e != hipErrorNotFound) return e;
+    if (hipError_t e = TCLBX_PICK(step_d3q19, storage)(model, storage, globals, a, block, s); e != hipErrorNotFound) return e;
+    if (hipError_t e = TCLBX_PICK(step_d3q27, storage)(model, storage, globals, a, block, s); e != hipErrorNotFound) return e;
+    if (hipError_t e = TCLBX_PICK(step_cum, storage)(model, storage, globals, a, block, s); e != hipErrorNotFound) return e;
+    if (hipError_t e = TCLBX_PICK(step_cumavg, storage)(model, storage, globals, a, block, s); e != hipErrorNotFound) return e;
     return hipErrorInvalidValue;
 }
 
 hipError_t launch_init(int model, int storage, const StepArgs& a, int block, hipStream_t s) {
-    if (hipError_t e = init_2d(model, storage, a, block, s); e != hipErrorNotFound) return e;
-    if (hipError_t e = init_d3q19(model, storage, a, block, s); e != hipErrorNotFound) return e;
-    if (hipError_t e = init_d3q27(model, storage, a, block, s); e != hipErrorNotFound) return e;
-    if (hipError_t e = init_cum(model, storage, a, block, s); e != hipErrorNotFound) return e;
-    if (hipError_t e = init_cumavg(model, storage, a, block, s); e != hipErrorNotFound) return e;
+    if (hipError_t e = TCLBX_PICK(init_2d, storage)(model, storage, a, block, s); e != hipErrorNotFound) return e;
+    if (hipError_t e = TCLBX_PICK(init_d3q19, storage)(model, storage, a, block, s); e != hipErrorNotFound) return e;
+    if (hipError_t e = TCLBX_PICK(init_d3q27, storage)(model, storage, a, block, s); e != hipErrorNotFound) return e;
+    if (hipError_t e = TCLBX_PICK(init_cum, storage)(model, storage, a, block, s); e != hipErrorNotFound) return e;
+    if (hipError_t e = TCLBX_PICK(init_cumavg, storage)(model, storage, a, block, s); e != hipErrorNotFound) return e;
     return hipErrorInvalidValue;
 }
 
@@ -158,19 +161,19 @@ hipError_t launch_reduce_globals_cases(const double* gpart, int nslots, int nglo
 
 hipError_t launch_batch_step(int model, int storage, bool globals, const BatchCase* cases, const int2* tiles,
                              int ntiles, int phase, double* gpart, int grid, int block, hipStream_t s) {
-    if (hipError_t e = batch_2d(model, storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
+    if (hipError_t e = TCLBX_PICK(batch_2d, storage)(model, storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
         e != hipErrorNotFound)
         return e;
-    if (hipError_t e = batch_d3q19(model, storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
+    if (hipError_t e = TCLBX_PICK(batch_d3q19, storage)(model, storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
         e != hipErrorNotFound)
         return e;
-    if (hipError_t e = batch_d3q27(model, storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
+    if (hipError_t e = TCLBX_PICK(batch_d3q27, storage)(model, storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
         e != hipErrorNotFound)
         return e;
-    if (hipError_t e = batch_cum(model, storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
+    if (hipError_t e = TCLBX_PICK(batch_cum, storage)(model, storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
         e != hipErrorNotFound)
         return e;
-    if (hipError_t e = batch_cumavg(model, storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
+    if (hipError_t e = TCLBX_PICK(batch_cumavg, storage)(model, storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
         e != hipErrorNotFound)
         return e;
     return hipErrorInvalidValue;
@@ -178,19 +181,19 @@ hipError_t launch_batch_step(int model, int storage, bool globals, const BatchCa
 
 hipError_t launch_quantity(int model, int storage, int q, const StepArgs& a, int x0, int y0, int z0, int rx,
                            int ry, int rz, double scale, double* out, hipStream_t s) {
-    if (hipError_t e = quantity_2d(model, storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
+    if (hipError_t e = TCLBX_PICK(quantity_2d, storage)(model, storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
         e != hipErrorNotFound)
         return e;
-    if (hipError_t e = quantity_d3q19(model, storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
+    if (hipError_t e = TCLBX_PICK(quantity_d3q19, storage)(model, storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
         e != hipErrorNotFound)
         return e;
-    if (hipError_t e = quantity_d3q27(model, storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
+    if (hipError_t e = TCLBX_PICK(quantity_d3q27, storage)(model, storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
         e != hipErrorNotFound)
         return e;
-    if (hipError_t e = quantity_cum(model, storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
+    if (hipError_t e = TCLBX_PICK(quantity_cum, storage)(model, storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
         e != hipErrorNotFound)
         return e;
-    if (hipError_t e = quantity_cumavg(model, storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
+    if (hipError_t e = TCLBX_PICK(quantity_cumavg, storage)(model, storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
         e != hipErrorNotFound)
         return e;
     return hipErrorInvalidValue;

@@ -5,7 +5,7 @@
 
 namespace tclbx {
 
-hipError_t step_2d(int model, int storage, bool globals, const StepArgs& a, int block, hipStream_t s) {
+hipError_t TCLBX_FAM(step_2d)(int model, int storage, bool globals, const StepArgs& a, int block, hipStream_t s) {
     switch (model) {
     case MODEL_D2Q9_SRT: return step_m<MODEL_D2Q9_SRT>(storage, globals, a, block, s);
     case MODEL_D2Q9_SRT_DRAG: return step_m<MODEL_D2Q9_SRT_DRAG>(storage, globals, a, block, s);
@@ -20,7 +20,7 @@ hipError_t step_2d(int model, int storage, bool globals, const StepArgs& a, int 
     return hipErrorNotFound;
 }
 
-hipError_t init_2d(int model, int storage, const StepArgs& a, int block, hipStream_t s) {
+hipError_t TCLBX_FAM(init_2d)(int model, int storage, const StepArgs& a, int block, hipStream_t s) {
     switch (model) {
     case MODEL_D2Q9_SRT: return init_m<MODEL_D2Q9_SRT>(storage, a, block, s);
     case MODEL_D2Q9_SRT_DRAG: return init_m<MODEL_D2Q9_SRT_DRAG>(storage, a, block, s);
@@ -35,7 +35,7 @@ hipError_t init_2d(int model, int storage, const StepArgs& a, int block, hipStre
     return hipErrorNotFound;
 }
 
-hipError_t batch_2d(int model, int storage, bool globals, const BatchCase* cases, const int2* tiles, int ntiles,
+hipError_t TCLBX_FAM(batch_2d)(int model, int storage, bool globals, const BatchCase* cases, const int2* tiles, int ntiles,
                     int phase, double* gpart, int grid, int block, hipStream_t s) {
     switch (model) {
     case MODEL_D2Q9_SRT: return batch_m<MODEL_D2Q9_SRT>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
@@ -51,7 +51,7 @@ hipError_t batch_2d(int model, int storage, bool globals, const BatchCase* cases
     return hipErrorNotFound;
 }
 
-hipError_t quantity_2d(int model, int storage, int q, const StepArgs& a, int x0, int y0, int z0, int rx, int ry,
+hipError_t TCLBX_FAM(quantity_2d)(int model, int storage, int q, const StepArgs& a, int x0, int y0, int z0, int rx, int ry,
                        int rz, double scale, double* out, hipStream_t s) {
     switch (model) {
     case MODEL_D2Q9_SRT: return quantity_m<MODEL_D2Q9_SRT>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);

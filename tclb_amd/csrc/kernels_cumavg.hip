@@ -5,7 +5,7 @@
 
 namespace tclbx {
 
-hipError_t step_cumavg(int model, int storage, bool globals, const StepArgs& a, int block, hipStream_t s) {
+hipError_t TCLBX_FAM(step_cumavg)(int model, int storage, bool globals, const StepArgs& a, int block, hipStream_t s) {
     switch (model) {
     case MODEL_D3Q27_CUMULANT_AVG: return step_m<MODEL_D3Q27_CUMULANT_AVG>(storage, globals, a, block, s);
     case MODEL_D3Q27_CUMULANT_AVG_IB: return step_m<MODEL_D3Q27_CUMULANT_AVG_IB>(storage, globals, a, block, s);
@@ -15,7 +15,7 @@ hipError_t step_cumavg(int model, int storage, bool globals, const StepArgs& a, 
     return hipErrorNotFound;
 }
 
-hipError_t init_cumavg(int model, int storage, const StepArgs& a, int block, hipStream_t s) {
+hipError_t TCLBX_FAM(init_cumavg)(int model, int storage, const StepArgs& a, int block, hipStream_t s) {
     switch (model) {
     case MODEL_D3Q27_CUMULANT_AVG: return init_m<MODEL_D3Q27_CUMULANT>(storage, a, block, s);
     case MODEL_D3Q27_CUMULANT_AVG_IB: return init_m<MODEL_D3Q27_CUMULANT_IB>(storage, a, block, s);
@@ -25,7 +25,7 @@ hipError_t init_cumavg(int model, int storage, const StepArgs& a, int block, hip
     return hipErrorNotFound;
 }
 
-hipError_t batch_cumavg(int model, int storage, bool globals, const BatchCase* cases, const int2* tiles, int ntiles,
+hipError_t TCLBX_FAM(batch_cumavg)(int model, int storage, bool globals, const BatchCase* cases, const int2* tiles, int ntiles,
                     int phase, double* gpart, int grid, int block, hipStream_t s) {
     switch (model) {
     case MODEL_D3Q27_CUMULANT_AVG: return batch_m<MODEL_D3Q27_CUMULANT_AVG>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
@@ -36,7 +36,7 @@ hipError_t batch_cumavg(int model, int storage, bool globals, const BatchCase* c
     return hipErrorNotFound;
 }
 
-hipError_t quantity_cumavg(int model, int storage, int q, const StepArgs& a, int x0, int y0, int z0, int rx, int ry,
+hipError_t TCLBX_FAM(quantity_cumavg)(int model, int storage, int q, const StepArgs& a, int x0, int y0, int z0, int rx, int ry,
                        int rz, double scale, double* out, hipStream_t s) {
     switch (model) {
     case MODEL_D3Q27_CUMULANT_AVG: return quantity_m<MODEL_D3Q27_CUMULANT_AVG>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);

@@ -5,21 +5,21 @@
 
 namespace tclbx {
 
-hipError_t step_d3q27(int model, int storage, bool globals, const StepArgs& a, int block, hipStream_t s) {
+hipError_t TCLBX_FAM(step_d3q27)(int model, int storage, bool globals, const StepArgs& a, int block, hipStream_t s) {
     switch (model) {
     case MODEL_D3Q27: return step_m<MODEL_D3Q27>(storage, globals, a, block, s);
     }
     return hipErrorNotFound;
 }
 
-hipError_t init_d3q27(int model, int storage, const StepArgs& a, int block, hipStream_t s) {
+hipError_t TCLBX_FAM(init_d3q27)(int model, int storage, const StepArgs& a, int block, hipStream_t s) {
     switch (model) {
     case MODEL_D3Q27: return init_m<MODEL_D3Q27>(storage, a, block, s);
     }
     return hipErrorNotFound;
 }
 
-hipError_t batch_d3q27(int model, int storage, bool globals, const BatchCase* cases, const int2* tiles, int ntiles,
+hipError_t TCLBX_FAM(batch_d3q27)(int model, int storage, bool globals, const BatchCase* cases, const int2* tiles, int ntiles,
                     int phase, double* gpart, int grid, int block, hipStream_t s) {
     switch (model) {
     case MODEL_D3Q27: return batch_m<MODEL_D3Q27>(storage, globals, cases, tiles, ntiles, phase, gpart, grid, block, s);
@@ -27,7 +27,7 @@ hipError_t batch_d3q27(int model, int storage, bool globals, const BatchCase* ca
     return hipErrorNotFound;
 }
 
-hipError_t quantity_d3q27(int model, int storage, int q, const StepArgs& a, int x0, int y0, int z0, int rx, int ry,
+hipError_t TCLBX_FAM(quantity_d3q27)(int model, int storage, int q, const StepArgs& a, int x0, int y0, int z0, int rx, int ry,
                        int rz, double scale, double* out, hipStream_t s) {
     switch (model) {
     case MODEL_D3Q27: return quantity_m<MODEL_D3Q27>(storage, q, a, x0, y0, z0, rx, ry, rz, scale, out, s);

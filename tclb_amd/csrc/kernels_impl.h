@@ -32,6 +32,14 @@
 #include "model.h"
 #include "storage.h"
 
+// Which storage formats this compile unit instantiates: TCLBX_FP_GROUP 64 =
+// FP64 only, 32 = the float-collision formats only, unset = all.
+#ifndef TCLBX_FP_GROUP
+#define TCLBX_FP_GROUP 0
+#endif
+#define TCLBX_HAS_F64 (TCLBX_FP_GROUP != 32)
+#define TCLBX_HAS_FLOAT (TCLBX_FP_GROUP != 64)
+
 namespace tclbx {
 
 // ------------------------------------------------------------ lattices ---
@@ -2179,11 +2187,15 @@ hipError_t step_ms(bool glob, const StepArgs& a, int block, hipStream_t s) {
 template <int MODEL>
 hipError_t step_m(int st, bool glob, const StepArgs& a, int block, hipStream_t s) {
     switch (st) {
+#if TCLBX_HAS_F64
     case ST_F64: return step_ms<MODEL, ST_F64>(glob, a, block, s);
+#endif
+#if TCLBX_HAS_FLOAT
     case ST_F32: return step_ms<MODEL, ST_F32>(glob, a, block, s);
     case ST_F32S: return step_ms<MODEL, ST_F32S>(glob, a, block, s);
     case ST_F16S: return step_ms<MODEL, ST_F16S>(glob, a, block, s);
     case ST_BF16S: return step_ms<MODEL, ST_BF16S>(glob, a, block, s);
+#endif
     }
     return hipErrorInvalidValue;
 }
@@ -2197,11 +2209,15 @@ hipError_t init_ms(const StepArgs& a, int block, hipStream_t s) {
 template <int MODEL>
 hipError_t init_m(int st, const StepArgs& a, int block, hipStream_t s) {
     switch (st) {
+#if TCLBX_HAS_F64
     case ST_F64: return init_ms<MODEL, ST_F64>(a, block, s);
+#endif
+#if TCLBX_HAS_FLOAT
     case ST_F32: return init_ms<MODEL, ST_F32>(a, block, s);
     case ST_F32S: return init_ms<MODEL, ST_F32S>(a, block, s);
     case ST_F16S: return init_ms<MODEL, ST_F16S>(a, block, s);
     case ST_BF16S: return init_ms<MODEL, ST_BF16S>(a, block, s);
+#endif
     }
     return hipErrorInvalidValue;
 }
@@ -2220,11 +2236,15 @@ template <int MODEL>
 hipError_t quantity_m(int st, int q, const StepArgs& a, int x0, int y0, int z0, int rx, int ry, int rz,
                       double scale, double* out, hipStream_t s) {
     switch (st) {
+#if TCLBX_HAS_F64
     case ST_F64: return quantity_ms<MODEL, ST_F64>(q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
+#endif
+#if TCLBX_HAS_FLOAT
     case ST_F32: return quantity_ms<MODEL, ST_F32>(q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
     case ST_F32S: return quantity_ms<MODEL, ST_F32S>(q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
     case ST_F16S: return quantity_ms<MODEL, ST_F16S>(q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
     case ST_BF16S: return quantity_ms<MODEL, ST_BF16S>(q, a, x0, y0, z0, rx, ry, rz, scale, out, s);
+#endif
     }
     return hipErrorInvalidValue;
 }
@@ -2246,46 +2266,91 @@ template <int MODEL>
 hipError_t batch_m(int st, bool glob, const BatchCase* cases, const int2* tiles, int ntiles, int phase,
                    double* gpart, int grid, int block, hipStream_t s) {
     switch (st) {
+#if TCLBX_HAS_F64
     case ST_F64: return batch_ms<MODEL, ST_F64>(glob, cases, tiles, ntiles, phase, gpart, grid, block, s);
+#endif
+#if TCLBX_HAS_FLOAT
     case ST_F32: return batch_ms<MODEL, ST_F32>(glob, cases, tiles, ntiles, phase, gpart, grid, block, s);
     case ST_F32S: return batch_ms<MODEL, ST_F32S>(glob, cases, tiles, ntiles, phase, gpart, grid, block, s);
     case ST_F16S: return batch_ms<MODEL, ST_F16S>(glob, cases, tiles, ntiles, phase, gpart, grid, block, s);
     case ST_BF16S: return batch_ms<MODEL, ST_BF16S>(glob, cases, tiles, ntiles, phase, gpart, grid, block, s);
+#endif
     }
     return hipErrorInvalidValue;
 }
 
 // Dispatch of one family of models (kernels_<family>.hip); hipErrorNotFound
-// for a model of another family.
-hipError_t step_2d(int model, int storage, bool globals, const StepArgs& a, int block, hipStream_t s);
-hipError_t init_2d(int model, int storage, const StepArgs& a, int block, hipStream_t s);
-hipError_t batch_2d(int model, int storage, bool globals, const BatchCase* cases, const int2* tiles, int ntiles,
+// for a model of another family.  Each family is compiled twice: the _f64
+// dispatchers hold the FP64-storage kernels, built with every multiply and
+// add rounded separately (the arithmetic the FP64 oracle follows); the _f32
+// ones hold the float-collision kernels of the other storage formats, built
+// with multiply-adds contracted into FMAs.
+#if TCLBX_FP_GROUP == 64
+#define TCLBX_FAM(name) name##_f64
+#elif TCLBX_FP_GROUP == 32
+#define TCLBX_FAM(name) name##_f32
+#else
+#define TCLBX_FAM(name) name##_all
+#endif
+hipError_t step_2d_f64(int model, int storage, bool globals, const StepArgs& a, int block, hipStream_t s);
+hipError_t init_2d_f64(int model, int storage, const StepArgs& a, int block, hipStream_t s);
+hipError_t batch_2d_f64(int model, int storage, bool globals, const BatchCase* cases, const int2* tiles, int ntiles,
                     int phase, double* gpart, int grid, int block, hipStream_t s);
-hipError_t quantity_2d(int model, int storage, int q, const StepArgs& a, int x0, int y0, int z0, int rx, int ry,
+hipError_t quantity_2d_f64(int model, int storage, int q, const StepArgs& a, int x0, int y0, int z0, int rx, int ry,
                        int rz, double scale, double* out, hipStream_t s);
-hipError_t step_d3q19(int model, int storage, bool globals, const StepArgs& a, int block, hipStream_t s);
-hipError_t init_d3q19(int model, int storage, const StepArgs& a, int block, hipStream_t s);
-hipError_t batch_d3q19(int model, int storage, bool globals, const BatchCase* cases, const int2* tiles, int ntiles,
+hipError_t step_d3q19_f64(int model, int storage, bool globals, const StepArgs& a, int block, hipStream_t s);
+hipError_t init_d3q19_f64(int model, int storage, const StepArgs& a, int block, hipStream_t s);
+hipError_t batch_d3q19_f64(int model, int storage, bool globals, const BatchCase* cases, const int2* tiles, int ntiles,
                     int phase, double* gpart, int grid, int block, hipStream_t s);
-hipError_t quantity_d3q19(int model, int storage, int q, const StepArgs& a, int x0, int y0, int z0, int rx, int ry,
+hipError_t quantity_d3q19_f64(int model, int storage, int q, const StepArgs& a, int x0, int y0, int z0, int rx, int ry,
                        int rz, double scale, double* out, hipStream_t s);
-hipError_t step_d3q27(int model, int storage, bool globals, const StepArgs& a, int block, hipStream_t s);
-hipError_t init_d3q27(int model, int storage, const StepArgs& a, int block, hipStream_t s);
-hipError_t batch_d3q27(int model, int storage, bool globals, const BatchCase* cases, const int2* tiles, int ntiles,
+hipError_t step_d3q27_f64(int model, int storage, bool globals, const StepArgs& a, int block, hipStream_t s);
+hipError_t init_d3q27_f64(int model, int storage, const StepArgs& a, int block, hipStream_t s);
+hipError_t batch_d3q27_f64(int model, int storage, bool globals, const BatchCase* cases, const int2* tiles, int ntiles,
                     int phase, double* gpart, int grid, int block, hipStream_t s);
-hipError_t quantity_d3q27(int model, int storage, int q, const StepArgs& a, int x0, int y0, int z0, int rx, int ry,
+hipError_t quantity_d3q27_f64(int model, int storage, int q, const StepArgs& a, int x0, int y0, int z0, int rx, int ry,
                        int rz, double scale, double* out, hipStream_t s);
-hipError_t step_cum(int model, int storage, bool globals, const StepArgs& a, int block, hipStream_t s);
-hipError_t init_cum(int model, int storage, const StepArgs& a, int block, hipStream_t s);
-hipError_t batch_cum(int model, int storage, bool globals, const BatchCase* cases, const int2* tiles, int ntiles,
+hipError_t step_cum_f64(int model, int storage, bool globals, const StepArgs& a, int block, hipStream_t s);
+hipError_t init_cum_f64(int model, int storage, const StepArgs& a, int block, hipStream_t s);
+hipError_t batch_cum_f64(int model, int storage, bool globals, const BatchCase* cases, const int2* tiles, int ntiles,
                     int phase, double* gpart, int grid, int block, hipStream_t s);
-hipError_t quantity_cum(int model, int storage, int q, const StepArgs& a, int x0, int y0, int z0, int rx, int ry,
+hipError_t quantity_cum_f64(int model, int storage, int q, const StepArgs& a, int x0, int y0, int z0, int rx, int ry,
                        int rz, double scale, double* out, hipStream_t s);
-hipError_t step_cumavg(int model, int storage, bool globals, const StepArgs& a, int block, hipStream_t s);
-hipError_t init_cumavg(int model, int storage, const StepArgs& a, int block, hipStream_t s);
-hipError_t batch_cumavg(int model, int storage, bool globals, const BatchCase* cases, const int2* tiles, int ntiles,
+hipError_t step_cumavg_f64(int model, int storage, bool globals, const StepArgs& a, int block, hipStream_t s);
+hipError_t init_cumavg_f64(int model, int storage, const StepArgs& a, int block, hipStream_t s);
+hipError_t batch_cumavg_f64(int model, int storage, bool globals, const BatchCase* cases, const int2* tiles, int ntiles,
                     int phase, double* gpart, int grid, int block, hipStream_t s);
-hipError_t quantity_cumavg(int model, int storage, int q, const StepArgs& a, int x0, int y0, int z0, int rx, int ry,
+hipError_t quantity_cumavg_f64(int model, int storage, int q, const StepArgs& a, int x0, int y0, int z0, int rx, int ry,
+                       int rz, double scale, double* out, hipStream_t s);
+hipError_t step_2d_f32(int model, int storage, bool globals, const StepArgs& a, int block, hipStream_t s);
+hipError_t init_2d_f32(int model, int storage, const StepArgs& a, int block, hipStream_t s);
+hipError_t batch_2d_f32(int model, int storage, bool globals, const BatchCase* cases, const int2* tiles, int ntiles,
+                    int phase, double* gpart, int grid, int block, hipStream_t s);
+hipError_t quantity_2d_f32(int model, int storage, int q, const StepArgs& a, int x0, int y0, int z0, int rx, int ry,
+                       int rz, double scale, double* out, hipStream_t s);
+hipError_t step_d3q19_f32(int model, int storage, bool globals, const StepArgs& a, int block, hipStream_t s);
+hipError_t init_d3q19_f32(int model, int storage, const StepArgs& a, int block, hipStream_t s);
+hipError_t batch_d3q19_f32(int model, int storage, bool globals, const BatchCase* cases, const int2* tiles, int ntiles,
+                    int phase, double* gpart, int grid, int block, hipStream_t s);
+hipError_t quantity_d3q19_f32(int model, int storage, int q, const StepArgs& a, int x0, int y0, int z0, int rx, int ry,
+                       int rz, double scale, double* out, hipStream_t s);
+hipError_t step_d3q27_f32(int model, int storage, bool globals, const StepArgs& a, int block, hipStream_t s);
+hipError_t init_d3q27_f32(int model, int storage, const StepArgs& a, int block, hipStream_t s);
+hipError_t batch_d3q27_f32(int model, int storage, bool globals, const BatchCase* cases, const int2* tiles, int ntiles,
+                    int phase, double* gpart, int grid, int block, hipStream_t s);
+hipError_t quantity_d3q27_f32(int model, int storage, int q, const StepArgs& a, int x0, int y0, int z0, int rx, int ry,
+                       int rz, double scale, double* out, hipStream_t s);
+hipError_t step_cum_f32(int model, int storage, bool globals, const StepArgs& a, int block, hipStream_t s);
+hipError_t init_cum_f32(int model, int storage, const StepArgs& a, int block, hipStream_t s);
+hipError_t batch_cum_f32(int model, int storage, bool globals, const BatchCase* cases, const int2* tiles, int ntiles,
+                    int phase, double* gpart, int grid, int block, hipStream_t s);
+hipError_t quantity_cum_f32(int model, int storage, int q, const StepArgs& a, int x0, int y0, int z0, int rx, int ry,
+                       int rz, double scale, double* out, hipStream_t s);
+hipError_t step_cumavg_f32(int model, int storage, bool globals, const StepArgs& a, int block, hipStream_t s);
+hipError_t init_cumavg_f32(int model, int storage, const StepArgs& a, int block, hipStream_t s);
+hipError_t batch_cumavg_f32(int model, int storage, bool globals, const BatchCase* cases, const int2* tiles, int ntiles,
+                    int phase, double* gpart, int grid, int block, hipStream_t s);
+hipError_t quantity_cumavg_f32(int model, int storage, int q, const StepArgs& a, int x0, int y0, int z0, int rx, int ry,
                        int rz, double scale, double* out, hipStream_t s);
 
 }  // namespace tclbx

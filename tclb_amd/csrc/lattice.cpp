@@ -645,18 +645,18 @@ void Lattice::SetDensity(int q, const double* in) {
 // A checkpoint is the raw dump of the current snapshot, density planes in
 // order, in the storage format (the reference's saveSolution writes its FTabs
 // raw, no header: src/Lattice.cu.Rt:708-737), then the extra densities in
-// FP64, then a 64-byte footer naming the model, storage format and size.  A
+// FP64, then an 80-byte footer naming the model, storage format and size.  A
 // reader that takes only the bytes it expects (the reference's load reads
 // block by block, :740-769) ignores the footer.  Load checks the footer, or
 // for a file without one its size, before it writes anything to the device.
 namespace {
 struct CheckpointFooter {
     char magic[8];    // "TCLBXCK1"
-    char model[24];
+    char model[40];   // NUL-terminated
     char storage[8];
     int32_t nx, ny, nz, q, nextra, reserved;
 };
-static_assert(sizeof(CheckpointFooter) == 64, "checkpoint footer is 64 bytes");
+static_assert(sizeof(CheckpointFooter) == 80, "checkpoint footer is 80 bytes");
 constexpr char kMagic[8] = {'T', 'C', 'L', 'B', 'X', 'C', 'K', '1'};
 }  // namespace
 

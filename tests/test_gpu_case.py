@@ -131,7 +131,7 @@ def test_save_load_and_failcheck(tmp_path):
         <Model><Param name="nu" value="0.1"/><Param name="GravitationX" value="1e-5"/></Model>
         <SaveBinary file="state" Iterations="20"/><Solve Iterations="20"/></CLBConfig>""", tmp_path, "a.xml")
     pri = tmp_path / "out" / "a_state_0.pri"
-    f = np.fromfile(pri, dtype=np.float64).reshape(9, 16, 32)
+    f = np.fromfile(pri, dtype=np.float64, count=9 * 16 * 32).reshape(9, 16, 32)   # then the 80-byte footer
     assert np.array_equal(f, a.lattice.get_f().reshape(9, 16, 32))
     b = run_case("d2q9_SRT", f"""<CLBConfig output="out/">{geom}
         <Model><Param name="nu" value="0.1"/><Param name="GravitationX" value="1e-5"/></Model>
@@ -253,7 +253,7 @@ def test_save_checkpoint_keeps_the_last_and_restarts(tmp_path):
     names = sorted(p.name for p in out.iterdir())
     assert names == ["ck_checkpoint_00000040_0.pri", "ck_checkpoint_00000060_0.pri",
                      "ck_restart_00000040.xml", "ck_restart_00000060.xml"], names
-    pri = np.fromfile(out / "ck_checkpoint_00000060_0.pri", dtype=np.float64).reshape(9, 16, 32)
+    pri = np.fromfile(out / "ck_checkpoint_00000060_0.pri", dtype=np.float64, count=9 * 16 * 32).reshape(9, 16, 32)
     assert np.array_equal(pri, a.lattice.get_f().reshape(9, 16, 32))
     text = (out / "ck_restart_00000040.xml").read_text()
     assert text.startswith('<?xml version="1.0"?>')

@@ -429,7 +429,7 @@ def test_load_refuses_a_mismatched_checkpoint(tmp_path):
     p = tmp_path / "a.bin"
     a.save(p)
     data = open(p, "rb").read()
-    assert len(data) == 9 * 20 * 8 * 4 + 64 and data[-64:-56] == b"TCLBXCK1"
+    assert len(data) == 9 * 20 * 8 * 4 + 80 and data[-80:-72] == b"TCLBXCK1"
     for other in (case.build("f32"), case.build("f64"), random_case("d2q9_SRT", 16, 10, 1, seed=3).build("f32s"),
                   random_case("d2q9", 20, 8, 1, seed=3).build("f32s")):
         before = other.get_f()
@@ -444,7 +444,7 @@ def test_load_refuses_a_mismatched_checkpoint(tmp_path):
         b.load(short)
     assert np.array_equal(b.get_f(), before)
     raw = tmp_path / "raw.bin"
-    raw.write_bytes(data[:-64])
+    raw.write_bytes(data[:-80])
     b.load(raw)
     assert np.array_equal(b.get_f(), a.get_f())
 
