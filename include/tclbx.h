@@ -193,6 +193,15 @@ int tclbx_prepare(tclbx_lattice* h);
 int tclbx_set_sorted_dispatch(tclbx_lattice* h, int on);
 /* whether it is on, and the nodes of the sorted list (0 when off) */
 int tclbx_get_sorted_dispatch(tclbx_lattice* h, int* on, long long* list_nodes);
+/* Stochastic rounding of the stored values in the 16-bit formats (f16s,
+ * bf16s; TCLBX_ERR for the others): a value between two representable ones
+ * is stored as either with the probabilities that make its expectation
+ * exact, so rounding errors do not accumulate into a drift over long runs.
+ * The random bits come from a hash of (node, direction, iteration, seed): a
+ * run is reproducible.  The step loop is then launched step by step (no
+ * captured graph).  No counterpart in the reference, which rounds to
+ * nearest (src/LatticeAccess.inc.cpp.Rt:14-35). */
+int tclbx_set_stochastic_rounding(tclbx_lattice* h, int on, unsigned seed);
 
 /* ---------------------------------------------------------------- batch --
  * Many independent lattices ("cases") of one model and storage format in

@@ -77,6 +77,49 @@ uint16_t orc_f32_to_bf16(float v)
 
 float orc_bf16_to_f32(uint16_t h) { return u2f((uint32_t)h << 16); }
 
+/* ----------------------------------------------- stochastic rounding ---
+ * The engine's optional stochastic rounding of the 16-bit stored values
+ * (no counterpart in the reference, which rounds to nearest): a value x
+ * between the neighbouring representable values lo and hi (|lo| <= |x| <
+ * |hi|) is stored as hi with probability (|x| - |lo|)/(|hi| - |lo|), so its
+ * expectation is x.  The random bits come from MurmurHash3's 32-bit
+ * finalizer (fmix32, public domain) over the node index, the direction and
+ * the step, so a run is reproducible. */
+static uint32_t fmix32(uint32_t h)
+{
+    h ^= h >> 16; h *= 0x85ebca6bu;
+    h ^= h >> 13; h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+
+uint32_t orc_sr_bits(uint32_t node, uint32_t q, uint32_t step, uint32_t seed)
+{
+    return fmix32(node * 0x9e3779b1u ^ fmix32(step * 0x85ebca77u + q * 0xc2b2ae3du + seed));
+}
+
+/* x -> binary16, stochastically: the neighbours from the nearest-even
+ * conversion, the choice from the top 24 random bits */
+uint16_t orc_f32_to_f16_sr(float x, uint32_t r)
+{
+    uint16_t h = orc_f32_to_f16(x), lo, hi;
+    float hf = orc_f16_to_f32(h), p;
+    if (hf == x || (h & 0x7fffu) >= 0x7c00u) return h;
+    if (fabsf(hf) > fabsf(x)) { hi = h; lo = (uint16_t)(h - 1); }
+    else { lo = h; hi = (uint16_t)(h + 1); }
+    p = (fabsf(x) - fabsf(orc_f16_to_f32(lo))) / (fabsf(orc_f16_to_f32(hi)) - fabsf(orc_f16_to_f32(lo)));
+    return (float)(r >> 8) * 5.9604644775390625e-08f < p ? hi : lo;
+}
+
+/* x -> bfloat16, stochastically: the low 16 bits of x plus 16 random bits
+ * carry into the kept half with probability (low bits)/2^16 */
+uint16_t orc_f32_to_bf16_sr(float x, uint32_t r)
+{
+    uint32_t u = f2u(x);
+    if ((u & 0x7fffffffu) >= 0x7f800000u) return orc_f32_to_bf16(x);
+    return (uint16_t)((u + (r & 0xffffu)) >> 16);
+}
+
 /* ----------------------------------------------------- node arithmetic --- */
 #define REAL double
 #define SFX(n) n##_d
@@ -108,6 +151,8 @@ struct orc_lattice {
     double* extra;              /* densities not streamed, [nextra][N]: AVG averages or bc's BC[0..1] */
     int nextra, navg;           /* how many, and how many of them are averages */
     int iter, reset_iter;       /* iterations since Init; of the last average reset */
+    int sr;                     /* stochastic rounding of the 16-bit formats' stores in steps */
+    uint32_t sr_seed;
     double params[16];
     double* zone;               /* [nzones][ncols] */
     int nzones, ncols;
@@ -148,6 +193,26 @@ static void st(const orc_lattice* L, void* b, size_t i, int q, double v)
     case ORC_F16S: ((uint16_t*)b)[i] = orc_f32_to_f16((float)v - (float)L->w[q]); break;
     default: ((uint16_t*)b)[i] = orc_f32_to_bf16((float)v - (float)L->w[q]); break;
     }
+}
+
+/* the store of a step: stochastic in the 16-bit formats when asked */
+static void st_step(const orc_lattice* L, void* b, size_t n, int q, double v)
+{
+    size_t i = (size_t)q * L->N + n;
+    if (L->sr && L->storage == ORC_F16S)
+        ((uint16_t*)b)[i] = orc_f32_to_f16_sr((float)v - (float)L->w[q],
+                                              orc_sr_bits((uint32_t)n, (uint32_t)q, (uint32_t)L->iter, L->sr_seed));
+    else if (L->sr && L->storage == ORC_BF16S)
+        ((uint16_t*)b)[i] = orc_f32_to_bf16_sr((float)v - (float)L->w[q],
+                                               orc_sr_bits((uint32_t)n, (uint32_t)q, (uint32_t)L->iter, L->sr_seed));
+    else
+        st(L, b, i, q, v);
+}
+
+void orc_set_stochastic(orc_lattice* L, int on, uint32_t seed)
+{
+    L->sr = on;
+    L->sr_seed = seed;
 }
 
 orc_lattice* orc_create(int model, int nx, int ny, int nz, int storage)
@@ -480,7 +545,7 @@ static void step_rows(orc_lattice* L, long double* glob, long r0, long r1)
                     else if (is_d2q9m(L)) d2q9m_node_f(ff, nt, L->params, zr, glob, L->model == ORC_D2Q9_DRAG,
                                                        is_bc(L) ? L->extra + n : NULL, L->N);
                     else d2q9_node_f(ff, nt, (float)L->params[0], (float)L->params[1], zr, drag, glob);
-                    for (q = 0; q < L->Q; q++) st(L, out, (size_t)q * L->N + n, q, ff[q]);
+                    for (q = 0; q < L->Q; q++) st_step(L, out, n, q, ff[q]);
                 }
         }
 }

@@ -144,6 +144,12 @@ uint16_t orc_f32_to_f16(float v);
 float    orc_f16_to_f32(uint16_t h);
 uint16_t orc_f32_to_bf16(float v);
 float    orc_bf16_to_f32(uint16_t h);
+/* stochastic rounding of the 16-bit formats (an engine option): the random
+ * bits of (node, direction, step, seed), and the two conversions */
+uint32_t orc_sr_bits(uint32_t node, uint32_t q, uint32_t step, uint32_t seed);
+uint16_t orc_f32_to_f16_sr(float x, uint32_t r);
+uint16_t orc_f32_to_bf16_sr(float x, uint32_t r);
+void orc_set_stochastic(orc_lattice* L, int on, uint32_t seed);
 
 /* single-node kernels exported for closed-form tests (double arithmetic) */
 void orc_d2q9_feq(double feq[9], double d, double ux, double uy);

@@ -80,6 +80,14 @@ def lib():
         L.orc_cum_node.argtypes = [dp, ctypes.c_uint16, dp, dp, dp, ip]
         L.orc_d3q27_basis.argtypes = [dp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.orc_set_threads.argtypes = [ip]
+        u32 = ctypes.c_uint32
+        L.orc_sr_bits.restype = u32
+        L.orc_sr_bits.argtypes = [u32, u32, u32, u32]
+        L.orc_f32_to_f16_sr.restype = ctypes.c_uint16
+        L.orc_f32_to_f16_sr.argtypes = [ctypes.c_float, u32]
+        L.orc_f32_to_bf16_sr.restype = ctypes.c_uint16
+        L.orc_f32_to_bf16_sr.argtypes = [ctypes.c_float, u32]
+        L.orc_set_stochastic.argtypes = [vp, ctypes.c_int, u32]
         _lib = L
     return _lib
 
@@ -146,6 +154,10 @@ class OracleLattice:
 
     def reset_average(self):
         lib().orc_reset_average(self.h)
+
+    def set_stochastic(self, on=True, seed=0):
+        """Stochastic rounding of the 16-bit formats' stores in steps."""
+        lib().orc_set_stochastic(self.h, 1 if on else 0, int(seed) & 0xFFFFFFFF)
 
     def extra(self):
         """[k, nz, ny, nx] of the densities that are not streamed"""

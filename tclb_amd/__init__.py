@@ -37,7 +37,7 @@ SYMBOLS = [
     "tclbx_load", "tclbx_get_timing",
     "tclbx_bytes_per_node", "tclbx_set_block", "tclbx_set_graph_steps", "tclbx_copy_bandwidth",
     "tclbx_set_rows_per_tile", "tclbx_get_rows_per_tile", "tclbx_prepare",
-    "tclbx_set_sorted_dispatch", "tclbx_get_sorted_dispatch",
+    "tclbx_set_sorted_dispatch", "tclbx_get_sorted_dispatch", "tclbx_set_stochastic_rounding",
     "tclbx_batch_create", "tclbx_batch_destroy", "tclbx_batch_case_bytes", "tclbx_batch_add_case",
     "tclbx_batch_remove_case", "tclbx_batch_set_setting", "tclbx_batch_set_zone_setting",
     "tclbx_batch_flag_overwrite", "tclbx_batch_init_case", "tclbx_batch_iterate", "tclbx_batch_get_globals",
@@ -116,6 +116,7 @@ def lib():
     L.tclbx_prepare.argtypes = [vp]
     L.tclbx_set_sorted_dispatch.argtypes = [vp, c_int]
     L.tclbx_get_sorted_dispatch.argtypes = [vp, ip, ctypes.POINTER(ctypes.c_longlong)]
+    L.tclbx_set_stochastic_rounding.argtypes = [vp, c_int, ctypes.c_uint]
     L.tclbx_copy_bandwidth.argtypes = [ctypes.c_longlong, c_int, dp, dp]
     ll, llp = ctypes.c_longlong, ctypes.POINTER(ctypes.c_longlong)
     L.tclbx_batch_create.argtypes = [c_char_p, c_char_p, ll, ctypes.POINTER(vp)]
@@ -465,6 +466,10 @@ class Lattice:
     @sorted_dispatch.setter
     def sorted_dispatch(self, on):
         _check(lib().tclbx_set_sorted_dispatch(self._h, int(bool(on))))
+
+    def set_stochastic_rounding(self, on=True, seed=0):
+        """Stochastic rounding of the stores of the 16-bit formats."""
+        _check(lib().tclbx_set_stochastic_rounding(self._h, int(bool(on)), int(seed) & 0xFFFFFFFF))
 
     def sorted_list_nodes(self):
         n = ctypes.c_longlong()

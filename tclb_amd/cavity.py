@@ -2,14 +2,15 @@
 every storage format, errors of rho, u and the lid torque against the FP64
 run.
 
-    python -m tclb_amd.cavity --n 640 --steps 20000 --formats f64,f32s,f16s,bf16s
+    python -m tclb_amd.cavity --n 640 --steps 20000 --formats f64,f32s,f16s,bf16s,f16s+sr,bf16s+sr
 
 The case (tclb_amd.cases.d3q19_cavity): n^3 nodes, the top face a MovingWall
 at u_x = 0.05, bounce-back Wall on the other five faces, MRT at tau = 0.52
 with the reference's rates.  Each format runs `steps` steps (Globals on the
 last one); errors are taken over the MRT (fluid) nodes against the first
 format of the list: relative L2 and max-norm of u (max-norm relative to the
-lid speed) and of rho, and the relative error of MovingWallTorque.  Prints
+lid speed) and of rho, and the relative error of MovingWallTorque.  A
+format "<f16s|bf16s>+sr" runs that format with stochastic rounding.  Prints
 one JSON object.
 """
 import argparse
@@ -32,7 +33,11 @@ def run(n=640, steps=20000, formats=("f64", "f32s", "f16s", "bf16s"), u_lid=0.05
            "steps": steps, "formats": {}}
     ref = None
     for fmt in formats:
-        lat = case.build(fmt)
+        # "<storage>+sr": the 16-bit format with stochastic rounding
+        storage, sr = (fmt[:-3], True) if fmt.endswith("+sr") else (fmt, False)
+        lat = case.build(storage)
+        if sr:
+            lat.set_stochastic_rounding(True, seed=1)
         lat.prepare()
         done = 0
         t0 = time.perf_counter()

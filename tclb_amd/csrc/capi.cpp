@@ -324,6 +324,14 @@ int tclbx_get_sorted_dispatch(tclbx_lattice* h, int* on, long long* list_nodes) 
     });
 }
 
+int tclbx_set_stochastic_rounding(tclbx_lattice* h, int on, unsigned seed) {
+    CHECK_H(h);
+    return guard([&]() {
+        h->lat->set_stochastic_rounding(on != 0, seed);
+        return TCLBX_OK;
+    });
+}
+
 int tclbx_prepare(tclbx_lattice* h) {
     CHECK_H(h);
     return guard([&]() {

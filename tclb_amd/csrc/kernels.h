@@ -46,6 +46,10 @@ struct StepArgs {
     unsigned class_mask, bulk_val;
     const unsigned* list;
     unsigned list_n;
+    // stochastic rounding of the stores of a 16-bit format (0: nearest even);
+    // sr_step is the iteration the step computes, sr_seed the lattice's seed
+    int sr;
+    unsigned sr_step, sr_seed;
     double par[MAX_PAR];
 };
 

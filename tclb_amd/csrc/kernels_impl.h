@@ -1770,6 +1770,17 @@ __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, 
         }
         // streaming stores: nothing reads them back this step
         char* fout = (char*)a.fout;
+        if constexpr (S::stochastic) {
+            if (a.sr) {
+#pragma unroll
+                for (int i = 0; i < Q; i++) {
+                    const uint32_t r = sr_bits(n, (uint32_t)i, a.sr_step, a.sr_seed);
+                    __builtin_nontemporal_store(DEV ? S::unraw_sr(f[i], r) : S::put_sr(f[i], (R)L::w(i), r),
+                                                (T*)(fout + (size_t)i * pbytes + n * (unsigned)sizeof(T)));
+                }
+                return;
+            }
+        }
 #pragma unroll
         for (int i = 0; i < Q; i++)
             __builtin_nontemporal_store(DEV ? S::unraw(f[i]) : S::put(f[i], (R)L::w(i)),

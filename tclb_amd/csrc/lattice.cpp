@@ -285,8 +285,20 @@ StepArgs Lattice::args(int from, int to, bool glob) const {
     a.bulk_val = bulk_val_;
     a.list = d_list_;
     a.list_n = (unsigned)list_n_;
+    a.sr = sr_ ? 1 : 0;
+    a.sr_seed = sr_seed_;
+    a.sr_step = 0;
     model_params(model, settings_, a.par);
     return a;
+}
+
+void Lattice::set_stochastic_rounding(bool on, unsigned seed) {
+    if (on && storage != ST_F16S && storage != ST_BF16S)
+        throw std::invalid_argument(std::string("stochastic rounding applies to the 16-bit formats f16s and bf16s, not ") +
+                                    storage_name(storage));
+    sr_ = on;
+    sr_seed_ = seed;
+    drop_graph();
 }
 
 void Lattice::set_sorted_dispatch(bool on) {
@@ -381,8 +393,9 @@ int globals_rows_per_block(long long rows, int bpr) {
     return (int)std::max(1LL, std::min(16LL, rows * bpr / 8192));
 }
 
-void Lattice::launch(int from, int to, bool glob) {
+void Lattice::launch(int from, int to, bool glob, int step) {
     StepArgs a = args(from, to, glob);
+    a.sr_step = (unsigned)step;
     if (sorted_ && !glob) {
         a.mode = 1;
         hip_check(launch_step(model->id, storage, false, a, block_, stream_), "step kernel (bulk)");
@@ -497,7 +510,7 @@ void Lattice::Prepare() {
     upload_zone();
     ensure_list();
     if (tune_rpt_ && N >= (1u << 20)) tune_rows();
-    if (graph_steps_ >= 2) {
+    if (graph_steps_ >= 2 && !sr_) {
         run_graph_steps(0);  // capture only
         hip_check(hipStreamSynchronize(stream_), "Prepare");
     }
@@ -523,9 +536,9 @@ void Lattice::Iterate(int niter, int iter_type) {
     const int plain_end = niter - (last_glob ? 1 : 0);
     const bool captured = graph_exec_ != nullptr && graph_version_ == params_version_;
     const int lead = snap_ != 0 ? 1 : 0;
-    if (!every_glob && graph_steps_ >= 2 && plain_end - lead >= (captured ? 1 : 2) * graph_steps_) {
+    if (!every_glob && !sr_ && graph_steps_ >= 2 && plain_end - lead >= (captured ? 1 : 2) * graph_steps_) {
         if (snap_ != 0) {
-            launch(snap_, 1 - snap_, false);
+            launch(snap_, 1 - snap_, false, iter_ + i);
             snap_ = 1 - snap_;
             i++;
         }
@@ -539,7 +552,7 @@ void Lattice::Iterate(int niter, int iter_type) {
         const bool glob = every_glob || (last_glob && i == niter - 1);
         if (last_glob && i == niter - 1)
             hip_check(hipMemsetAsync(d_glob_, 0, MAX_GLOBALS * sizeof(double), stream_), "clear globals");
-        launch(snap_, 1 - snap_, glob);
+        launch(snap_, 1 - snap_, glob, iter_ + i);
         snap_ = 1 - snap_;
     }
     hip_check(hipEventRecord(ev1_, stream_), "event record");

@@ -92,6 +92,11 @@ public:
     void set_sorted_dispatch(bool on);
     bool sorted_dispatch() const { return sorted_; }
     size_t sorted_list_size() { ensure_list(); return list_n_; }
+    // stochastic rounding of the stores of the 16-bit formats (f16s, bf16s)
+    // in steps: random bits from (node, direction, iteration, seed); the
+    // step loop is then launched directly, since every step's launch differs
+    void set_stochastic_rounding(bool on, unsigned seed);
+    bool stochastic_rounding() const { return sr_; }
 
     const ModelDef* model;
     const int nx, ny, nz, storage;
@@ -101,7 +106,7 @@ public:
 private:
     StepArgs args(int from, int to, bool glob) const;
     void upload_zone();
-    void launch(int from, int to, bool glob);
+    void launch(int from, int to, bool glob, int step = 0);
     void drop_graph();
     void tune_rows();
     void run_graph_steps(int nsteps);
@@ -139,6 +144,8 @@ private:
     unsigned* d_list_ = nullptr;
     size_t list_n_ = 0, list_cap_ = 0;
     unsigned flags_version_ = 0, list_version_ = ~0u;
+    bool sr_ = false;
+    unsigned sr_seed_ = 0;
 };
 
 // hipError_t -> exception with a message naming the call

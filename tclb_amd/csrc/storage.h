@@ -13,12 +13,51 @@
 
 namespace tclbx {
 
+// ---- stochastic rounding of the 16-bit formats (an option of a lattice) --
+// A value x between the neighbouring representable values lo and hi (|lo| <=
+// |x| < |hi|) is stored as hi with probability (|x| - |lo|)/(|hi| - |lo|),
+// so the stored value's expectation is x: rounding errors do not add up
+// into a drift over many steps.  The random bits come from MurmurHash3's
+// 32-bit finalizer (fmix32) over (node, direction, step, seed), so a run is
+// reproducible.
+__device__ __forceinline__ uint32_t fmix32(uint32_t h) {
+    h ^= h >> 16; h *= 0x85ebca6bu;
+    h ^= h >> 13; h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+__device__ __forceinline__ uint32_t sr_bits(uint32_t node, uint32_t q, uint32_t step, uint32_t seed) {
+    return fmix32(node * 0x9e3779b1u ^ fmix32(step * 0x85ebca77u + q * 0xc2b2ae3du + seed));
+}
+// float -> half: the neighbours around the nearest-even conversion, the
+// choice from the top 24 random bits
+__device__ __forceinline__ _Float16 f16_sr(float x, uint32_t r) {
+    const _Float16 h = (_Float16)x;
+    const float hf = (float)h;
+    const uint16_t hb = __builtin_bit_cast(uint16_t, h);
+    if (hf == x || (hb & 0x7fffu) >= 0x7c00u) return h;
+    const bool above = fabsf(hf) > fabsf(x);
+    const uint16_t lob = above ? (uint16_t)(hb - 1) : hb, hib = above ? hb : (uint16_t)(hb + 1);
+    const float lo = fabsf((float)__builtin_bit_cast(_Float16, lob));
+    const float hi = fabsf((float)__builtin_bit_cast(_Float16, hib));
+    const float p = (fabsf(x) - lo) / (hi - lo);
+    return __builtin_bit_cast(_Float16, (float)(r >> 8) * 5.9604644775390625e-08f < p ? hib : lob);
+}
+// float -> bfloat16: the low 16 bits plus 16 random bits carry into the
+// kept half with probability (low bits)/2^16
+__device__ __forceinline__ __bf16 bf16_sr(float x, uint32_t r) {
+    const uint32_t u = __builtin_bit_cast(uint32_t, x);
+    if ((u & 0x7fffffffu) >= 0x7f800000u) return (__bf16)x;
+    return __builtin_bit_cast(__bf16, (uint16_t)((u + (r & 0xffffu)) >> 16));
+}
+
 template <int ST> struct Store;
 
 template <> struct Store<ST_F64> {
     using T = double;
     using R = double;
     static constexpr bool shifted = false;
+    static constexpr bool stochastic = false;
     __device__ __forceinline__ static R get(T v, R) { return v; }
     __device__ __forceinline__ static T put(R v, R) { return v; }
     __device__ __forceinline__ static R raw(T v) { return v; }
@@ -29,6 +68,7 @@ template <> struct Store<ST_F32> {
     using T = float;
     using R = float;
     static constexpr bool shifted = false;
+    static constexpr bool stochastic = false;
     __device__ __forceinline__ static R get(T v, R) { return v; }
     __device__ __forceinline__ static T put(R v, R) { return v; }
     __device__ __forceinline__ static R raw(T v) { return v; }
@@ -39,6 +79,7 @@ template <> struct Store<ST_F32S> {
     using T = float;
     using R = float;
     static constexpr bool shifted = true;
+    static constexpr bool stochastic = false;
     __device__ __forceinline__ static R get(T v, R s) { return v + s; }
     __device__ __forceinline__ static T put(R v, R s) { return v - s; }
     // the stored value itself (f - w)
@@ -55,6 +96,10 @@ template <> struct Store<ST_F16S> {
     __device__ __forceinline__ static T put(R v, R s) { return (_Float16)(v - s); }
     __device__ __forceinline__ static R raw(T v) { return (float)v; }
     __device__ __forceinline__ static T unraw(R v) { return (_Float16)v; }
+    // the same stores, rounded stochastically with random bits r
+    static constexpr bool stochastic = true;
+    __device__ __forceinline__ static T put_sr(R v, R s, uint32_t r) { return f16_sr(v - s, r); }
+    __device__ __forceinline__ static T unraw_sr(R v, uint32_t r) { return f16_sr(v, r); }
 };
 
 template <> struct Store<ST_BF16S> {
@@ -66,6 +111,9 @@ template <> struct Store<ST_BF16S> {
     __device__ __forceinline__ static T put(R v, R s) { return (__bf16)(v - s); }
     __device__ __forceinline__ static R raw(T v) { return (float)v; }
     __device__ __forceinline__ static T unraw(R v) { return (__bf16)v; }
+    static constexpr bool stochastic = true;
+    __device__ __forceinline__ static T put_sr(R v, R s, uint32_t r) { return bf16_sr(v - s, r); }
+    __device__ __forceinline__ static T unraw_sr(R v, uint32_t r) { return bf16_sr(v, r); }
 };
 
 }  // namespace tclbx

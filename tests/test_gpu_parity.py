@@ -628,6 +628,71 @@ def test_cavity_reduced_storage_tracks_f64(storage):
     assert abs(t1 - t0) <= (0.25 if storage == "bf16s" else 0.05) * abs(t0), (t0, t1)
 
 
+@pytest.mark.parametrize("storage", ["f16s", "bf16s"])
+@pytest.mark.parametrize("model,nx,ny,nz", [m for m in MODELS if m[0] in ("d2q9_SRT", "d3q19", "d2q9", "d3q27")])
+def test_stochastic_rounding_tracks_the_oracle(oracle_mod, model, nx, ny, nz, storage):
+    """Stochastic rounding of the 16-bit stores, step by step against the
+    oracle's emulation with the same seed (both draw the same random bits
+    from (node, direction, iteration, seed)): within the bound of
+    test_reduced_storage, where one unit of the format covers a rounding
+    that a float-ulp difference of the value turned the other way."""
+    say(f"stochastic rounding {model} {storage}")
+    case = random_case(model, nx, ny, nz, seed=17)
+    lat = case.build(storage)
+    lat.set_stochastic_rounding(True, seed=1234)
+    orc = oracle_for_case(oracle_mod, case, storage)
+    orc.set_stochastic(True, seed=1234)
+    w = lattice_weights(lat.Q)
+    for step in range(4):
+        lat.iterate(1)
+        orc.iterate(1)
+        a, b = lat.get_f(), orc.get_f()
+        stored = b - w[:, None, None, None]
+        bound = 32 * 2.0 ** -24 * np.abs(b) + 4 * STORAGE_ULP[storage] * np.abs(stored) + 2 * STORAGE_TINY[storage]
+        assert np.all(np.abs(a - b) <= bound), (step, np.max(np.abs(a - b) / bound))
+        lat.set_f(b)
+
+
+def test_stochastic_rounding_is_reproducible():
+    """The same seed gives the same run bit for bit; another seed another
+    run; rounding to nearest a third."""
+    say("stochastic rounding reproducible")
+    case = cases.d3q19_cavity(n=24, u_lid=0.05, tau=0.52, steps=50)
+    runs = []
+    for seed in (7, 7, 8, None):
+        lat = case.build("f16s")
+        if seed is not None:
+            lat.set_stochastic_rounding(True, seed=seed)
+        lat.iterate(50)
+        runs.append(lat.get_f())
+    assert np.array_equal(runs[0], runs[1])
+    assert not np.array_equal(runs[0], runs[2]) and not np.array_equal(runs[0], runs[3])
+    with pytest.raises(tclb_amd.TclbxError, match="16-bit"):
+        case.build("f32s").set_stochastic_rounding(True)
+
+
+@pytest.mark.parametrize("storage", ["f16s", "bf16s"])
+def test_cavity_stochastic_rounding_tracks_f64(storage):
+    """The 48^3 cavity after 2000 steps with stochastic rounding: velocity
+    relative L2 error against FP64 within the bound below (reported beside
+    the nearest-even run's)."""
+    say(f"cavity 48^3 {storage}+sr vs f64")
+    case = cases.d3q19_cavity(n=48, u_lid=0.05, tau=0.52, steps=2000)
+    ref = case.build("f64")
+    ref.iterate(2000)
+    u0 = ref.quantity("U")
+    errs = {}
+    for sr in (False, True):
+        lat = case.build(storage)
+        if sr:
+            lat.set_stochastic_rounding(True, seed=11)
+        lat.iterate(2000)
+        u1 = lat.quantity("U")
+        errs[sr] = float(np.linalg.norm(u1 - u0) / np.linalg.norm(u0))
+    say(f"  u L2: nearest {errs[False]:.3e}, stochastic {errs[True]:.3e}")
+    assert errs[True] < (2e-2 if storage == "f16s" else 0.1), errs
+
+
 @pytest.mark.parametrize("storage", ["f64", "f16s"])
 def test_rows_per_tile_and_tuning_change_nothing(storage):
     """1, 2 or 4 rows per tile, or the first-Iterate tuning that times them

@@ -1598,3 +1598,61 @@ def test_bc_build_walls_inlets_and_init(O):
     # on the inlet nodes the plain model adds G rho to J, the bc build does not
     rho = fb[:, inlet].sum(0)
     assert np.allclose((D2E[:, 0] @ fb[:, inlet]) - (D2E[:, 0] @ fa[:, inlet]), 1e-5 * rho, rtol=1e-9, atol=0)
+
+
+# ------------------------------------------- stochastic rounding (16-bit) ---
+def _sr_neighbours_f16(x):
+    """the binary16 values either side of float32 x (numpy's float16)"""
+    h = np.float16(x)
+    if float(h) == float(x):
+        return float(h), float(h)
+    a = np.nextafter(h, np.float16(np.inf if float(h) < float(x) else -np.inf))
+    lo, hi = sorted([float(h), float(a)], key=abs)
+    return lo, hi
+
+
+@pytest.mark.parametrize("x", [1.0e-3 + 3.1e-7, -2.7e-3 - 1.3e-7, 3.0e-6 + 1.1e-8, -4.5e-7, 0.15 + 1e-5, 1.0 / 3.0])
+def test_stochastic_rounding_f16_is_unbiased(O, x):
+    """The oracle's stochastic binary16 rounding (the engine's option for the
+    16-bit formats): every result is one of the two neighbours lo, hi of x
+    (normal and subnormal binary16 alike), the mean over the hash's bits is x
+    within 4.5 standard errors, and the variance is (|hi| - |x|)(|x| - |lo|),
+    the two-point distribution's.  Round-to-nearest would fail the mean."""
+    L = O.lib()
+    xf = float(np.float32(x))
+    lo, hi = _sr_neighbours_f16(xf)
+    n = 40000
+    vals = np.array([L.orc_f16_to_f32(L.orc_f32_to_f16_sr(xf, L.orc_sr_bits(k, 3, 17, 5))) for k in range(n)])
+    assert set(np.unique(vals)) <= {lo, hi}
+    var = (abs(hi) - abs(xf)) * (abs(xf) - abs(lo))
+    assert abs(vals.mean() - xf) <= 4.5 * np.sqrt(var / n) + 1e-12 * abs(xf)
+    assert abs(vals.var() - var) <= 0.05 * var
+    nearest = float(np.float16(xf))
+    assert abs(nearest - xf) > 10 * np.sqrt(var / n)   # the bias stochastic rounding removes
+
+
+@pytest.mark.parametrize("x", [1.0e-3 + 3.1e-7, -2.7e-3 - 1.3e-7, 0.15 + 1e-5])
+def test_stochastic_rounding_bf16_is_unbiased(O, x):
+    """The same for bfloat16: lo and hi are x's bits truncated and truncated
+    plus one unit in the last kept place."""
+    L = O.lib()
+    xf = np.float32(x)
+    u = int(xf.view(np.uint32))
+    lo = float(np.uint32(u & 0xFFFF0000).view(np.float32))
+    hi = float(np.uint32((u & 0xFFFF0000) + 0x10000).view(np.float32))
+    n = 40000
+    vals = np.array([L.orc_bf16_to_f32(L.orc_f32_to_bf16_sr(float(xf), L.orc_sr_bits(k, 11, 3, 9))) for k in range(n)])
+    assert set(np.unique(vals)) <= {lo, hi}
+    xv = float(xf)
+    var = (abs(hi) - abs(xv)) * (abs(xv) - abs(lo))
+    assert abs(vals.mean() - xv) <= 4.5 * np.sqrt(var / n)
+    assert abs(vals.var() - var) <= 0.05 * var
+
+
+def test_stochastic_rounding_bits_are_uniform(O):
+    """The hash of (node, direction, step, seed): uniform in mean and
+    variance, and different for neighbouring nodes, directions and steps."""
+    L = O.lib()
+    r = np.array([L.orc_sr_bits(k % 997, k // 997, 7 + k % 3, 1) for k in range(60000)], dtype=np.float64) / 2.0 ** 32
+    assert abs(r.mean() - 0.5) < 0.01 and abs(r.var() - 1 / 12) < 0.003
+    assert len({L.orc_sr_bits(5, q, s, 0) for q in range(19) for s in range(50)}) == 19 * 50
