@@ -14,6 +14,7 @@
 #ifndef TCLBX_H
 #define TCLBX_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -157,6 +158,32 @@ int tclbx_reset_average(tclbx_lattice* h);
  * the lattice is touched); a file without the footer is checked by size. */
 int tclbx_save(tclbx_lattice* h, const char* path);
 int tclbx_load(tclbx_lattice* h, const char* path);
+/* saveSolution in a chosen layout: TCLBX_LAYOUT_ENGINE (what tclbx_save
+ * writes) or TCLBX_LAYOUT_REFERENCE, the file the reference itself writes
+ * (listTabs/save, src/Lattice.cu.Rt:642-661,708-737): the snapshot's
+ * non-empty margin blocks in order (margins and their sizes from the fields'
+ * reach, src/conf.R:771-1012), each field's layer as stored values,
+ * extra densities included, no footer.  Only where a reference storage
+ * build holds the same stored values: f64 and f32 for every model, f32s and
+ * f16s (float-shift, half-shift) for the d2q9 MRT models, which declare the
+ * shift w_i; TCLBX_ERR otherwise.  tclbx_load reads both layouts; the case
+ * driver's SaveBinary and SaveCheckpoint write the reference's wherever it
+ * is available. */
+#define TCLBX_LAYOUT_ENGINE 0
+#define TCLBX_LAYOUT_REFERENCE 1
+int tclbx_save_layout(tclbx_lattice* h, const char* path, int layout);
+/* The reference layout on the host, no GPU needed: the file's size, its
+ * blocks (number 1..27 = sx + 3 sy + 9 sz + 1, side 0 below, 1 domain, 2
+ * above; values per block) in file order, and the conversion from / to the
+ * engine's planes (Q planes of nx*ny*nz stored values, then nextra planes
+ * of doubles in `extra`).  Domain-block positions the reference's put never
+ * writes are packed as zero and ignored when unpacking. */
+int tclbx_pri_bytes(const char* model, const char* storage, int nx, int ny, int nz, size_t* bytes);
+int tclbx_pri_blocks(const char* model, int nx, int ny, int nz, int* index, size_t* values, int max, int* count);
+int tclbx_pri_pack(const char* model, const char* storage, int nx, int ny, int nz, const void* planes,
+                   const double* extra, void* out);
+int tclbx_pri_unpack(const char* model, const char* storage, int nx, int ny, int nz, const void* in, void* planes,
+                     double* extra);
 
 /* Event timing of the last tclbx_iterate and the running total, in ms --
  * what MainCallback measures for MLBUps (src/main.cpp:65-157). */
@@ -326,6 +353,15 @@ const char* tclbx_case_zone_name(const tclbx_case* c, int k);
 int tclbx_case_set_output_root(tclbx_case* c, const char* dir);
 /* Creates the lattice on the GPU and runs every element of <CLBConfig>. */
 int tclbx_case_run(tclbx_case* c);
+/* The handler walk of tclbx_case_run without a lattice, host only: every
+ * element of <CLBConfig> is made into its handler and initialised as the
+ * reference's Handler does (src/Handlers.h:18-33, HandlerFactory), its
+ * units, settings, zones and geometry resolved, and each <Solve> steps
+ * through its iterations and callbacks with nothing computed and no file
+ * written.  TCLBX_ERR if a handler fails; the notices and errors, an
+ * unknown element among them ("ERROR: Unknown Handler: <name>", which the
+ * reference also passes over), are the case's messages. */
+int tclbx_case_check(tclbx_case* c);
 /* The case's lattice after tclbx_case_run, for reading Globals, quantities
  * and flags; it belongs to the case (tclbx_lattice_destroy refuses it). */
 int tclbx_case_lattice(tclbx_case* c, tclbx_lattice** out);

@@ -34,7 +34,8 @@ SYMBOLS = [
     "tclbx_iterate", "tclbx_get_globals", "tclbx_get_iteration", "tclbx_get_quantity",
     "tclbx_get_quantity_noalloc",
     "tclbx_get_density", "tclbx_set_density", "tclbx_extra_density_name", "tclbx_reset_average", "tclbx_save",
-    "tclbx_load", "tclbx_get_timing",
+    "tclbx_load", "tclbx_save_layout", "tclbx_pri_bytes", "tclbx_pri_blocks", "tclbx_pri_pack", "tclbx_pri_unpack",
+    "tclbx_get_timing",
     "tclbx_bytes_per_node", "tclbx_set_block", "tclbx_set_graph_steps", "tclbx_copy_bandwidth",
     "tclbx_set_rows_per_tile", "tclbx_get_rows_per_tile", "tclbx_prepare",
     "tclbx_set_sorted_dispatch", "tclbx_get_sorted_dispatch", "tclbx_set_stochastic_rounding",
@@ -48,7 +49,7 @@ SYMBOLS = [
     "tclbx_server_wait_all", "tclbx_server_stats", "tclbx_server_set_tuning",
     "tclbx_case_load", "tclbx_case_load_string", "tclbx_case_destroy", "tclbx_case_size", "tclbx_case_units",
     "tclbx_case_rasterize", "tclbx_case_cuts", "tclbx_case_zone_count", "tclbx_case_zone_name", "tclbx_case_set_output_root",
-    "tclbx_case_run", "tclbx_case_lattice", "tclbx_case_iteration", "tclbx_case_message_count",
+    "tclbx_case_run", "tclbx_case_check", "tclbx_case_lattice", "tclbx_case_iteration", "tclbx_case_message_count",
     "tclbx_case_message",
 ]
 
@@ -107,6 +108,12 @@ def lib():
     L.tclbx_set_density.argtypes = [vp, c_int, dp]
     L.tclbx_save.argtypes = [vp, c_char_p]
     L.tclbx_load.argtypes = [vp, c_char_p]
+    L.tclbx_save_layout.argtypes = [vp, c_char_p, c_int]
+    szp = ctypes.POINTER(ctypes.c_size_t)
+    L.tclbx_pri_bytes.argtypes = [c_char_p, c_char_p, c_int, c_int, c_int, szp]
+    L.tclbx_pri_blocks.argtypes = [c_char_p, c_int, c_int, c_int, ip, szp, c_int, ip]
+    L.tclbx_pri_pack.argtypes = [c_char_p, c_char_p, c_int, c_int, c_int, vp, dp, vp]
+    L.tclbx_pri_unpack.argtypes = [c_char_p, c_char_p, c_int, c_int, c_int, vp, vp, dp]
     L.tclbx_get_timing.argtypes = [vp, dp, dp]
     L.tclbx_bytes_per_node.argtypes = [vp, dp]
     L.tclbx_set_block.argtypes = [vp, c_int]
@@ -160,6 +167,7 @@ def lib():
     L.tclbx_case_zone_name.restype = c_char_p
     L.tclbx_case_set_output_root.argtypes = [vp, c_char_p]
     L.tclbx_case_run.argtypes = [vp]
+    L.tclbx_case_check.argtypes = [vp]
     L.tclbx_case_lattice.argtypes = [vp, ctypes.POINTER(vp)]
     L.tclbx_case_iteration.argtypes = [vp]
     L.tclbx_case_message_count.argtypes = [vp]
@@ -184,6 +192,50 @@ def copy_bandwidth(nbytes=1 << 30, reps=10):
     g, t = ctypes.c_double(), ctypes.c_double()
     _check(lib().tclbx_copy_bandwidth(int(nbytes), int(reps), ctypes.byref(g), ctypes.byref(t)))
     return g.value, t.value
+
+
+STORAGE_DTYPE = {"f64": np.float64, "f32": np.float32, "f32s": np.float32, "f16s": np.float16, "bf16s": np.uint16}
+
+
+def pri_bytes(model, storage, nx, ny, nz=1):
+    """Size of the reference's <file>_0.pri for this lattice (host only)."""
+    n = ctypes.c_size_t()
+    _check(lib().tclbx_pri_bytes(model.encode(), storage.encode(), nx, ny, nz, ctypes.byref(n)))
+    return n.value
+
+
+def pri_blocks(model, nx, ny, nz=1):
+    """[(block number 1..27, values)] of the reference's snapshot, in file order."""
+    idx = (ctypes.c_int * 27)()
+    val = (ctypes.c_size_t * 27)()
+    cnt = ctypes.c_int()
+    _check(lib().tclbx_pri_blocks(model.encode(), nx, ny, nz, idx, val, 27, ctypes.byref(cnt)))
+    return [(idx[i], val[i]) for i in range(cnt.value)]
+
+
+def pri_pack(model, storage, planes, extra=None):
+    """The reference's file bytes of a snapshot given as stored values
+    [Q, nz, ny, nx] (the storage format's dtype) and extra densities
+    [nextra, nz, ny, nx] (float64)."""
+    planes = np.ascontiguousarray(planes, dtype=STORAGE_DTYPE[storage])
+    nz, ny, nx = planes.shape[1:]
+    out = np.zeros(pri_bytes(model, storage, nx, ny, nz), dtype=np.uint8)
+    ex = None if extra is None else np.ascontiguousarray(extra, dtype=np.float64)
+    _check(lib().tclbx_pri_pack(model.encode(), storage.encode(), nx, ny, nz, planes.ctypes.data,
+                                None if ex is None else _dp(ex), out.ctypes.data))
+    return out
+
+
+def pri_unpack(model, storage, nx, ny, nz, data, q, nextra=0):
+    """(planes [Q, nz, ny, nx] stored values, extra [nextra, nz, ny, nx])
+    of the reference's file bytes."""
+    data = np.ascontiguousarray(np.frombuffer(bytes(data), dtype=np.uint8) if not isinstance(data, np.ndarray)
+                                else data.view(np.uint8))
+    planes = np.empty((q, nz, ny, nx), dtype=STORAGE_DTYPE[storage])
+    extra = np.empty((nextra, nz, ny, nx), dtype=np.float64)
+    _check(lib().tclbx_pri_unpack(model.encode(), storage.encode(), nx, ny, nz, data.ctypes.data, planes.ctypes.data,
+                                  _dp(extra) if nextra else None))
+    return planes, extra
 
 
 def models():
@@ -428,8 +480,13 @@ class Lattice:
         """Lattice::resetAverage: the averages to zero, counted from now."""
         _check(lib().tclbx_reset_average(self._h))
 
-    def save(self, path):
-        _check(lib().tclbx_save(self._h, str(path).encode()))
+    def save(self, path, layout="engine"):
+        """layout "engine" (density planes + footer) or "reference" (the
+        reference's margin blocks, tclbx_save_layout)."""
+        if layout == "engine":
+            _check(lib().tclbx_save(self._h, str(path).encode()))
+        else:
+            _check(lib().tclbx_save_layout(self._h, str(path).encode(), {"reference": 1}[layout]))
 
     def load(self, path):
         _check(lib().tclbx_load(self._h, str(path).encode()))
@@ -784,6 +841,13 @@ class Case:
 
     def run(self):
         _check(lib().tclbx_case_run(self._h))
+        return self
+
+    def check(self):
+        """tclbx_case_check: every handler made and stepped through with no
+        lattice (no GPU); raises on a failing handler.  The messages hold
+        the notices and errors."""
+        _check(lib().tclbx_case_check(self._h))
         return self
 
     @property

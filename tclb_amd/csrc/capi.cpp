@@ -7,6 +7,7 @@
 #include "batch.h"
 #include "lattice.h"
 #include "model.h"
+#include "pri.h"
 #include "server.h"
 #include "solver.h"
 
@@ -271,6 +272,78 @@ int tclbx_load(tclbx_lattice* h, const char* path) {
     CHECK_H(h);
     if (!path) return fail("null path");
     return guard([&]() { return h->lat->Load(path) == 0 ? TCLBX_OK : fail(std::string("cannot read ") + path); });
+}
+int tclbx_save_layout(tclbx_lattice* h, const char* path, int layout) {
+    CHECK_H(h);
+    if (!path) return fail("null path");
+    return guard([&]() {
+        return h->lat->Save(path, layout) == 0 ? TCLBX_OK : fail(std::string("cannot write ") + path);
+    });
+}
+
+namespace {
+int pri_args(const char* model, const char* storage, int nx, int ny, int nz, const ModelDef** m, int* st) {
+    *m = model ? model_by_name(model) : nullptr;
+    if (!*m) return fail(std::string("unknown model: ") + (model ? model : "(null)"));
+    *st = storage_by_name(storage ? storage : "f64");
+    if (*st < 0) return fail(std::string("unknown storage: ") + storage);
+    if (nx < 1 || ny < 1 || nz < 1) return fail("lattice sizes must be positive");
+    if ((*m)->dim == 2 && nz != 1) return fail("a 2D model takes nz = 1");
+    std::string why;
+    if (!pri_exact(*m, *st, &why)) return fail("no reference layout for " + (*m)->name + " in " + storage + ": " + why);
+    return TCLBX_OK;
+}
+}  // namespace
+
+int tclbx_pri_bytes(const char* model, const char* storage, int nx, int ny, int nz, size_t* bytes) {
+    return guard([&]() {
+        const ModelDef* m;
+        int st;
+        if (pri_args(model, storage, nx, ny, nz, &m, &st) != TCLBX_OK) return TCLBX_ERR;
+        if (!bytes) return fail("null output pointer");
+        *bytes = pri_bytes(m, st, nx, ny, nz);
+        return TCLBX_OK;
+    });
+}
+
+int tclbx_pri_blocks(const char* model, int nx, int ny, int nz, int* index, size_t* values, int max, int* count) {
+    return guard([&]() {
+        const ModelDef* m;
+        int st;
+        if (pri_args(model, "f64", nx, ny, nz, &m, &st) != TCLBX_OK) return TCLBX_ERR;
+        if (!count) return fail("null output pointer");
+        const PriLayout L = pri_layout(m, nx, ny, nz);
+        *count = (int)L.blocks.size();
+        for (int i = 0; i < (int)L.blocks.size() && i < max; i++) {
+            if (index) index[i] = L.blocks[i].index;
+            if (values) values[i] = L.blocks[i].elems;
+        }
+        return TCLBX_OK;
+    });
+}
+
+int tclbx_pri_pack(const char* model, const char* storage, int nx, int ny, int nz, const void* planes,
+                   const double* extra, void* out) {
+    return guard([&]() {
+        const ModelDef* m;
+        int st;
+        if (pri_args(model, storage, nx, ny, nz, &m, &st) != TCLBX_OK) return TCLBX_ERR;
+        if (!planes || !out || (m->nextra() > 0 && !extra)) return fail("null buffer");
+        pri_pack(m, st, nx, ny, nz, planes, (size_t)nx * ny * nz, extra, out);
+        return TCLBX_OK;
+    });
+}
+
+int tclbx_pri_unpack(const char* model, const char* storage, int nx, int ny, int nz, const void* in, void* planes,
+                     double* extra) {
+    return guard([&]() {
+        const ModelDef* m;
+        int st;
+        if (pri_args(model, storage, nx, ny, nz, &m, &st) != TCLBX_OK) return TCLBX_ERR;
+        if (!planes || !in || (m->nextra() > 0 && !extra)) return fail("null buffer");
+        pri_unpack(m, st, nx, ny, nz, in, planes, (size_t)nx * ny * nz, extra);
+        return TCLBX_OK;
+    });
 }
 
 int tclbx_get_timing(const tclbx_lattice* h, double* last_ms, double* total_ms) {
@@ -751,6 +824,15 @@ int tclbx_case_run(tclbx_case* c) {
     return guard([&]() {
         c->s->run();
         c->view.lat = c->s->lattice.get();
+        return TCLBX_OK;
+    });
+}
+
+int tclbx_case_check(tclbx_case* c) {
+    CHECK_C(c);
+    return guard([&]() {
+        c->s->check();
+        c->view.lat = nullptr;
         return TCLBX_OK;
     });
 }

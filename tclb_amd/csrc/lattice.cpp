@@ -1,6 +1,8 @@
 // lattice.cpp -- see lattice.h.
 #include "lattice.h"
 
+#include "pri.h"
+
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -655,13 +657,16 @@ void Lattice::SetDensity(int q, const double* in) {
     hip_check(e, "SetDensity");
 }
 
-// A checkpoint is the raw dump of the current snapshot, density planes in
-// order, in the storage format (the reference's saveSolution writes its FTabs
-// raw, no header: src/Lattice.cu.Rt:708-737), then the extra densities in
-// FP64, then an 80-byte footer naming the model, storage format and size.  A
-// reader that takes only the bytes it expects (the reference's load reads
-// block by block, :740-769) ignores the footer.  Load checks the footer, or
-// for a file without one its size, before it writes anything to the device.
+// A checkpoint in the engine's layout is the raw dump of the current
+// snapshot, density planes in order, in the storage format (the reference's
+// saveSolution writes its FTabs raw, no header: src/Lattice.cu.Rt:708-737),
+// then the extra densities in FP64, then an 80-byte footer naming the model,
+// storage format and size.  A reader that takes only the bytes it expects
+// (the reference's load reads block by block, :740-769) ignores the footer.
+// In the reference's layout it is the snapshot's margin blocks (pri.h),
+// extra densities included as stored values, no footer.  Load tells them
+// apart by the footer and then by the size, before it writes anything to
+// the device.
 namespace {
 struct CheckpointFooter {
     char magic[8];    // "TCLBXCK1"
@@ -677,35 +682,61 @@ size_t Lattice::checkpoint_data_bytes() const {
     return (size_t)model->Q * N * storage_bytes(storage) + (size_t)model->nextra() * N * sizeof(double);
 }
 
-int Lattice::Save(const std::string& path) {
+int Lattice::case_layout() const {
+    return pri_exact(model, storage, nullptr) ? LAYOUT_REFERENCE : LAYOUT_ENGINE;
+}
+
+int Lattice::Save(const std::string& path, int layout) {
+    if (layout != LAYOUT_ENGINE && layout != LAYOUT_REFERENCE) throw std::invalid_argument("unknown checkpoint layout");
+    if (layout == LAYOUT_REFERENCE) {
+        std::string why;
+        if (!pri_exact(model, storage, &why))
+            throw std::invalid_argument("no reference layout for " + model->name + " in " + storage_name(storage) +
+                                        ": " + why);
+    }
     FILE* f = std::fopen(path.c_str(), "wb");
     if (!f) return -1;
+    struct Closer { FILE* f; ~Closer() { if (f) std::fclose(f); } } closer{f};
     const size_t S = storage_bytes(storage);
-    std::vector<char> host(N * S);
     int rc = 0;
-    for (int q = 0; q < model->Q && rc == 0; q++) {
-        hip_check(hipMemcpy(host.data(), (char*)snaps_[snap_] + (size_t)q * pitch * S, N * S, hipMemcpyDeviceToHost),
-                  "Save");
-        if (std::fwrite(host.data(), 1, host.size(), f) != host.size()) rc = -1;
+    if (layout == LAYOUT_REFERENCE) {
+        std::vector<char> planes((size_t)model->Q * N * S);
+        for (int q = 0; q < model->Q; q++)
+            hip_check(hipMemcpy(planes.data() + (size_t)q * N * S, (char*)snaps_[snap_] + (size_t)q * pitch * S, N * S,
+                                hipMemcpyDeviceToHost), "Save");
+        std::vector<double> extra((size_t)model->nextra() * N);
+        if (!extra.empty())
+            hip_check(hipMemcpy(extra.data(), d_extra_, extra.size() * sizeof(double), hipMemcpyDeviceToHost), "Save");
+        std::vector<char> out(pri_bytes(model, storage, nx, ny, nz));
+        pri_pack(model, storage, nx, ny, nz, planes.data(), N, extra.data(), out.data());
+        if (std::fwrite(out.data(), 1, out.size(), f) != out.size()) rc = -1;
+    } else {
+        std::vector<char> host(N * S);
+        for (int q = 0; q < model->Q && rc == 0; q++) {
+            hip_check(hipMemcpy(host.data(), (char*)snaps_[snap_] + (size_t)q * pitch * S, N * S,
+                                hipMemcpyDeviceToHost), "Save");
+            if (std::fwrite(host.data(), 1, host.size(), f) != host.size()) rc = -1;
+        }
+        // then the extra densities (FP64), as the reference's saveSolution
+        // writes every density of the snapshot
+        std::vector<double> av(N);
+        for (size_t k = 0; k < (size_t)model->nextra() && rc == 0; k++) {
+            hip_check(hipMemcpy(av.data(), d_extra_ + k * N, N * sizeof(double), hipMemcpyDeviceToHost), "Save");
+            if (std::fwrite(av.data(), sizeof(double), N, f) != N) rc = -1;
+        }
+        if (rc == 0) {
+            CheckpointFooter ft;
+            std::memset(&ft, 0, sizeof(ft));
+            std::memcpy(ft.magic, kMagic, sizeof(kMagic));
+            std::strncpy(ft.model, model->name.c_str(), sizeof(ft.model) - 1);
+            std::strncpy(ft.storage, storage_name(storage), sizeof(ft.storage) - 1);
+            ft.nx = nx; ft.ny = ny; ft.nz = nz;
+            ft.q = model->Q;
+            ft.nextra = model->nextra();
+            if (std::fwrite(&ft, sizeof(ft), 1, f) != 1) rc = -1;
+        }
     }
-    // then the extra densities (FP64), as the reference's saveSolution
-    // writes every density of the snapshot
-    std::vector<double> av(N);
-    for (size_t k = 0; k < (size_t)model->nextra() && rc == 0; k++) {
-        hip_check(hipMemcpy(av.data(), d_extra_ + k * N, N * sizeof(double), hipMemcpyDeviceToHost), "Save");
-        if (std::fwrite(av.data(), sizeof(double), N, f) != N) rc = -1;
-    }
-    if (rc == 0) {
-        CheckpointFooter ft;
-        std::memset(&ft, 0, sizeof(ft));
-        std::memcpy(ft.magic, kMagic, sizeof(kMagic));
-        std::strncpy(ft.model, model->name.c_str(), sizeof(ft.model) - 1);
-        std::strncpy(ft.storage, storage_name(storage), sizeof(ft.storage) - 1);
-        ft.nx = nx; ft.ny = ny; ft.nz = nz;
-        ft.q = model->Q;
-        ft.nextra = model->nextra();
-        if (std::fwrite(&ft, sizeof(ft), 1, f) != 1) rc = -1;
-    }
+    closer.f = nullptr;
     if (std::fclose(f) != 0) rc = -1;
     return rc;
 }
@@ -721,27 +752,47 @@ int Lattice::Load(const std::string& path) {
     const std::string what = "checkpoint " + path + ": ";
     const std::string mine = model->name + " " + storage_name(storage) + " " + std::to_string(nx) + "x" +
                              std::to_string(ny) + "x" + std::to_string(nz);
+    const bool ref_ok = pri_exact(model, storage, nullptr);
+    const size_t ref_bytes = ref_ok ? pri_bytes(model, storage, nx, ny, nz) : 0;
+    bool footer = false;
     if ((size_t)size == data + sizeof(CheckpointFooter)) {
         CheckpointFooter ft;
         if (std::fseek(f, (long)data, SEEK_SET) != 0 || std::fread(&ft, sizeof(ft), 1, f) != 1) return -1;
-        if (std::memcmp(ft.magic, kMagic, sizeof(kMagic)) != 0)
-            throw std::runtime_error(what + "has " + std::to_string(size) + " bytes and no tclbx footer; " + mine +
-                                     " takes " + std::to_string(data));
-        ft.model[sizeof(ft.model) - 1] = 0;
-        ft.storage[sizeof(ft.storage) - 1] = 0;
-        const std::string theirs = std::string(ft.model) + " " + ft.storage + " " + std::to_string(ft.nx) + "x" +
-                                   std::to_string(ft.ny) + "x" + std::to_string(ft.nz);
-        if (theirs != mine || ft.q != model->Q || ft.nextra != model->nextra())
-            throw std::runtime_error(what + "holds " + theirs + ", the lattice is " + mine);
-    } else if ((size_t)size != data) {
+        footer = std::memcmp(ft.magic, kMagic, sizeof(kMagic)) == 0;
+        if (footer) {
+            ft.model[sizeof(ft.model) - 1] = 0;
+            ft.storage[sizeof(ft.storage) - 1] = 0;
+            const std::string theirs = std::string(ft.model) + " " + ft.storage + " " + std::to_string(ft.nx) + "x" +
+                                       std::to_string(ft.ny) + "x" + std::to_string(ft.nz);
+            if (theirs != mine || ft.q != model->Q || ft.nextra != model->nextra())
+                throw std::runtime_error(what + "holds " + theirs + ", the lattice is " + mine);
+        }
+    }
+    const bool reference = !footer && ref_ok && (size_t)size == ref_bytes;
+    if (!footer && !reference && (size_t)size != data) {
+        std::string also;
+        if (ref_ok) also = ", or " + std::to_string(ref_bytes) + " in the reference's layout";
         throw std::runtime_error(what + "has " + std::to_string(size) + " bytes; a checkpoint of " + mine +
                                  " has " + std::to_string(data) + " (+" + std::to_string(sizeof(CheckpointFooter)) +
-                                 " of footer)");
+                                 " of footer)" + also);
     }
     if (std::fseek(f, 0, SEEK_SET) != 0) return -1;
+    const size_t S = storage_bytes(storage);
+    if (reference) {
+        std::vector<char> in(ref_bytes);
+        if (std::fread(in.data(), 1, in.size(), f) != in.size()) return -1;
+        std::vector<char> planes((size_t)model->Q * N * S);
+        std::vector<double> extra((size_t)model->nextra() * N);
+        pri_unpack(model, storage, nx, ny, nz, in.data(), planes.data(), N, extra.data());
+        for (int q = 0; q < model->Q; q++)
+            hip_check(hipMemcpy((char*)snaps_[snap_] + (size_t)q * pitch * S, planes.data() + (size_t)q * N * S, N * S,
+                                hipMemcpyHostToDevice), "Load");
+        if (!extra.empty())
+            hip_check(hipMemcpy(d_extra_, extra.data(), extra.size() * sizeof(double), hipMemcpyHostToDevice), "Load");
+        return 0;
+    }
     // plane by plane, so that host memory stays at one plane; the size is
     // checked, so only an I/O error can stop it part way
-    const size_t S = storage_bytes(storage);
     std::vector<char> host(N * S);
     for (int q = 0; q < model->Q; q++) {
         if (std::fread(host.data(), 1, host.size(), f) != host.size()) return -1;

@@ -69,9 +69,15 @@ public:
     void SetDensity(int q, const double* in); // Set_<f>
     // saveSolution / loadSolution (src/Lattice.cu.Rt:708-769): -1 when the
     // file cannot be opened or read; Load throws when the file holds another
-    // model, storage format or size
-    int Save(const std::string& path);
+    // model, storage format or size.  layout: LAYOUT_ENGINE (density planes
+    // and a footer) or LAYOUT_REFERENCE (the reference's margin blocks,
+    // pri.h); Load takes either, told apart by the footer and the size.
+    enum { LAYOUT_ENGINE = 0, LAYOUT_REFERENCE = 1 };
+    int Save(const std::string& path, int layout = LAYOUT_ENGINE);
     int Load(const std::string& path);
+    // the layout the case driver's SaveBinary/SaveCheckpoint write: the
+    // reference's wherever its storage build holds the same values
+    int case_layout() const;
     size_t checkpoint_data_bytes() const;  // bytes of a checkpoint before its footer
 
     double last_iterate_ms() const { return last_ms_; }

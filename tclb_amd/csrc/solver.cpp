@@ -207,7 +207,7 @@ struct AcModel : GenericContainer {
     int init() override {
         int ret = GenericContainer::init();
         if (ret) return ret;
-        solver->lattice->Init();
+        if (solver->lattice) solver->lattice->Init();
         solver->iter = 0;
         return 0;
     }
@@ -256,7 +256,7 @@ struct AcParam : Action {
 struct AcInit : Action {
     int init() override {
         Action::init();
-        solver->lattice->Init();
+        if (solver->lattice) solver->lattice->Init();
         solver->iter = 0;
         return 0;
     }
@@ -286,7 +286,7 @@ struct AcSolve : GenericAction {
             solver->iter += steps;
             int type = solver->iter_type;
             if (steps == my_next) type |= ITER_LASTGLOB;
-            solver->lattice->Iterate(steps, type);
+            if (solver->lattice) solver->lattice->Iterate(steps, type);
             for (size_t i = 0; i < solver->hands.size(); i++) {
                 auto h = solver->hands[i];
                 if (h->now(solver->iter)) {
@@ -309,13 +309,13 @@ struct CbLog : Callback {
     int init() override {
         Callback::init();
         filename = solver->out_iter_file(node->get("name", "Log"), ".csv");
-        solver->init_log(filename);
+        if (solver->lattice) solver->init_log(filename);
         old_type = solver->iter_type;
         solver->iter_type |= ITER_LASTGLOB;
         return 0;
     }
     int doit() override {
-        solver->write_log(filename);
+        if (solver->lattice) solver->write_log(filename);
         return 0;
     }
     int finish() override {
@@ -361,6 +361,7 @@ struct CbStop : Callback {
         return 0;
     }
     int doit() override {
+        if (!solver->lattice) return 0;
         int any = 0;
         for (size_t i = 0; i < what.size(); i++) {
             const double v = solver->lattice->globals()[what[i]];
@@ -407,7 +408,7 @@ struct CbFailcheck : Callback {
         return 0;
     }
     int doit() override {
-        if (active) return 0;
+        if (active || !solver->lattice) return 0;
         active = true;
         const NameSet comps(node->get("what", "all"));
         bool nan = false;
@@ -466,7 +467,7 @@ struct CbVTK : Callback {
         }
         return 0;
     }
-    int doit() override { return solver->write_vtk(nm, what, reg); }
+    int doit() override { return solver->lattice ? solver->write_vtk(nm, what, reg) : 0; }
 };
 
 // <TXT [name] [what] [gzip] Iterations> (src/Handlers/cbTXT.cpp): every
@@ -482,7 +483,7 @@ struct CbTXT : Callback {
         gzip = g && xml_as_bool(g->value);
         return 0;
     }
-    int doit() override { return solver->write_txt(nm, what, gzip); }
+    int doit() override { return solver->lattice ? solver->write_txt(nm, what, gzip) : 0; }
 };
 
 // <Average Iterations> (src/Handlers/cbAveraging.cpp): resetAverage when
@@ -490,17 +491,19 @@ struct CbTXT : Callback {
 struct CbAverage : Callback {
     int init() override {
         Callback::init();
-        solver->lattice->resetAverage();
+        if (solver->lattice) solver->lattice->resetAverage();
         return 0;
     }
     int doit() override {
-        solver->lattice->resetAverage();
+        if (solver->lattice) solver->lattice->resetAverage();
         return 0;
     }
 };
 
 // <SaveBinary [file|filename] Iterations> (src/Handlers/cbSaveBinary.cpp):
-// the current snapshot to <file>_0.pri (Lattice::saveSolution)
+// the current snapshot to <file>_0.pri (Lattice::saveSolution), in the
+// reference's margin-block layout wherever a reference storage build holds
+// the same stored values (FP64 and FP32 always), else the engine's layout
 struct CbSaveBinary : Callback {
     std::string fn;
     int init() override {
@@ -515,9 +518,10 @@ struct CbSaveBinary : Callback {
             solver->note("ERROR: SaveBinary comp= is not supported by this engine");
             return -1;
         }
+        if (!solver->lattice) return 0;
         const std::string path = fn + "_0.pri";
         mkpath(path);
-        return solver->lattice->Save(path) == 0 ? 0 : -1;
+        return solver->lattice->Save(path, solver->lattice->case_layout()) == 0 ? 0 : -1;
     }
 };
 
@@ -545,10 +549,11 @@ struct CbSaveCheckpoint : Callback {
         return 0;
     }
     int doit() override {
+        if (!solver->lattice) return 0;
         const std::string base = solver->out_collective_file("checkpoint", "");
         const std::string restart = solver->out_collective_file("restart", ".xml");
         const std::string pri = base + "_0.pri";
-        if (solver->lattice->Save(pri) != 0) return -1;
+        if (solver->lattice->Save(pri, solver->lattice->case_layout()) != 0) return -1;
         write_restart(base, restart);
         if (keep != 0) {
             pri_.push_back(pri);
@@ -601,7 +606,8 @@ struct AcLoadBinary : Action {
             solver->note("ERROR: LoadBinary comp= is not supported by this engine");
             return -1;
         }
-        if (solver->lattice->Load(a->value + "_0.pri") != 0) fail("LoadBinary: cannot read " + a->value + "_0.pri");
+        if (solver->lattice && solver->lattice->Load(a->value + "_0.pri") != 0)
+            fail("LoadBinary: cannot read " + a->value + "_0.pri");
         return 0;
     }
 };
@@ -744,6 +750,22 @@ void Solver::run() {
     // a fresh NodeType map: the Geometry elements draw it as they run
     geometry = std::make_unique<Geometry>(*model, nx, ny, nz, units);
     lattice = std::make_unique<Lattice>(model, nx, ny, nz, storage);
+    iter = 0;
+    iter_type = ITER_NORM;
+    HandlerRef main = make_handler(config);
+    if (!main) {
+        std::string last = messages.empty() ? "" : messages.back();
+        hands.clear();
+        fail("case failed: " + last);
+    }
+    main.reset();
+    hands.clear();
+}
+
+void Solver::check() {
+    if (!config) fail("no case loaded");
+    geometry = std::make_unique<Geometry>(*model, nx, ny, nz, units);
+    lattice.reset();
     iter = 0;
     iter_type = ITER_NORM;
     HandlerRef main = make_handler(config);

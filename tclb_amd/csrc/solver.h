@@ -63,6 +63,12 @@ public:
     void rasterize();
     // the whole case: lattice on the GPU, every handler in order
     void run();
+    // the same walk without a lattice (no GPU, no output files): every
+    // handler is made and initialised, its attributes, units, settings and
+    // zones resolved and the geometry drawn, and the Solve loops step
+    // through their iterations and callbacks without computing anything.
+    // Throws like run(); the handlers' errors are in `messages`.
+    void check();
 
     // --- state the handlers use
     const ModelDef* model;

@@ -131,7 +131,11 @@ def test_save_load_and_failcheck(tmp_path):
         <Model><Param name="nu" value="0.1"/><Param name="GravitationX" value="1e-5"/></Model>
         <SaveBinary file="state" Iterations="20"/><Solve Iterations="20"/></CLBConfig>""", tmp_path, "a.xml")
     pri = tmp_path / "out" / "a_state_0.pri"
-    f = np.fromfile(pri, dtype=np.float64, count=9 * 16 * 32).reshape(9, 16, 32)   # then the 80-byte footer
+    # the reference's layout (margin blocks, no footer)
+    data = np.fromfile(pri, dtype=np.uint8)
+    assert data.size == tclb_amd.pri_bytes("d2q9_SRT", "f64", 32, 16, 1)
+    f, _ = tclb_amd.pri_unpack("d2q9_SRT", "f64", 32, 16, 1, data, 9)
+    f = f.reshape(9, 16, 32)
     assert np.array_equal(f, a.lattice.get_f().reshape(9, 16, 32))
     b = run_case("d2q9_SRT", f"""<CLBConfig output="out/">{geom}
         <Model><Param name="nu" value="0.1"/><Param name="GravitationX" value="1e-5"/></Model>
@@ -141,7 +145,7 @@ def test_save_load_and_failcheck(tmp_path):
         <Solve Iterations="50"/></CLBConfig>""", tmp_path, "a2.xml")
     assert np.array_equal(b.lattice.get_f(), a2.lattice.get_f())   # 20 + 30 steps = 50 steps
     f[3, 8, 10] = np.nan
-    f.tofile(tmp_path / "out" / "bad_0.pri")
+    tclb_amd.pri_pack("d2q9_SRT", "f64", f.reshape(9, 1, 16, 32)).tofile(tmp_path / "out" / "bad_0.pri")
     c = run_case("d2q9_SRT", f"""<CLBConfig output="out/">{geom}
         <Model><Param name="nu" value="0.1"/></Model>
         <LoadBinary file="{tmp_path}/out/bad"/>
@@ -253,8 +257,9 @@ def test_save_checkpoint_keeps_the_last_and_restarts(tmp_path):
     names = sorted(p.name for p in out.iterdir())
     assert names == ["ck_checkpoint_00000040_0.pri", "ck_checkpoint_00000060_0.pri",
                      "ck_restart_00000040.xml", "ck_restart_00000060.xml"], names
-    pri = np.fromfile(out / "ck_checkpoint_00000060_0.pri", dtype=np.float64, count=9 * 16 * 32).reshape(9, 16, 32)
-    assert np.array_equal(pri, a.lattice.get_f().reshape(9, 16, 32))
+    data = np.fromfile(out / "ck_checkpoint_00000060_0.pri", dtype=np.uint8)
+    pri, _ = tclb_amd.pri_unpack("d2q9_SRT", "f64", 32, 16, 1, data, 9)
+    assert np.array_equal(pri.reshape(9, 16, 32), a.lattice.get_f().reshape(9, 16, 32))
     text = (out / "ck_restart_00000040.xml").read_text()
     assert text.startswith('<?xml version="1.0"?>')
     root = ET.fromstring(text)
