@@ -313,7 +313,8 @@ def main():
             "frac": round(r["roofline_frac"], 4),
             "traffic": t,
         },
-        "formats": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+        "formats": {k: {kk: (round(vv, 4) if isinstance(vv, float) and kk != "mean_ux_fluid" else vv)
+                        for kk, vv in v.items()}
                     for k, v in results.items()},
         "copy_ceiling_GBs": round(copy_gbs, 1),
     }

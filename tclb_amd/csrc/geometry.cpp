@@ -530,6 +530,8 @@ void Geometry::load_stl(const GeomRegion& reg, const XmlNode& n) {
             }
 
     std::vector<signed char> lev((size_t)std::max(0LL, reg.size()), (signed char)inside_out);
+    // rays through a triangle; edge hits; resolved negatively; unresolved
+    size_t hits[4] = {0, 0, 0, 0};
     auto roff = [&reg](int x, int y, int z) {
         return (size_t)(x - reg.dx) + (size_t)reg.nx * ((size_t)(y - reg.dy) + (size_t)reg.ny * (z - reg.dz));
     };
@@ -589,10 +591,19 @@ void Geometry::load_stl(const GeomRegion& reg, const XmlNode& n) {
                 const double cs[3] = {c1, c2, c3}, dcs[3] = {dc1, dc2, dc3};
                 int topo = 0;
                 for (int k = 0; k < 3; k++) {
-                    if (cs[k] == 0) { if (dcs[k] > 0) topo++; }
-                    else if (cs[k] > 0) topo++;
+                    if (cs[k] == 0) {
+                        // the ray runs through an edge: counted as the ray
+                        // moved a little along dv would be
+                        hits[1]++;
+                        if (dcs[k] > 0) topo++;
+                        else if (dcs[k] == 0) hits[3]++;
+                        else hits[2]++;
+                    } else if (cs[k] > 0) {
+                        topo++;
+                    }
                 }
                 if (topo != 3) continue;
+                hits[0]++;
                 const double h = t.p1[a1] * c3 + t.p2[a1] * c2 + t.p3[a1] * c1;
                 for (int x1 = l1; x1 <= h; x1++) {
                     int x, y, z;
@@ -603,11 +614,21 @@ void Geometry::load_stl(const GeomRegion& reg, const XmlNode& n) {
                 }
             }
     }
-    if (inside_out != 2)
+    if (inside_out != 2) {
         for (int z = reg.dz; z < reg.dz + reg.nz; z++)
             for (int y = reg.dy; y < reg.dy + reg.ny; y++)
                 for (int x = reg.dx; x < reg.dx + reg.nx; x++)
                     if (lev[roff(x, y, z)] % 2 == 1) dot(x, y, z);
+        // the reference's report of the ray hits (loadSTL's output/notice)
+        note("STL: triangle hits: " + std::to_string(hits[0]));
+        if (hits[1] > 0) {
+            note("STL: \\_ edge hits: " + std::to_string(hits[1]));
+            note("STL:    \\_ resolved negatively: " + std::to_string(hits[2]));
+            note("STL:    \\_ resolved positively: " + std::to_string(hits[1] - hits[3] - hits[2]));
+            if (hits[3] > 0)
+                note("STL:    \\_ could not be resolved: " + std::to_string(hits[3]) + " (this can cause problems!)");
+        }
+    }
 }
 
 // ---------------------------------------------------------------- Sweep ---

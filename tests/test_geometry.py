@@ -275,6 +275,47 @@ def test_stl_box_inside_outside_closed_form(tmp_path, axis):
     assert np.array_equal(out, ~want)
 
 
+def test_stl_rays_through_edges_closed_form(tmp_path):
+    """Rays that run exactly through a mesh's edges and vertices take the
+    reference's tie-break (loadSTL, src/Geometry.cpp.Rt:612-629: a zero
+    barycentric counts as the sign it takes when the ray moves a little
+    along a fixed direction dv), so a closed mesh still gives the right
+    parity.  An octahedron |x - cx| + |y - cy'| + |z - cz| < r with its
+    vertices on whole nodes in x and z: at x, z in [512, 1024) float32's
+    spacing (6.1e-5) swallows the transform's offsets (1.4e-5, 2.0e-5), so
+    the vertices land exactly on the grid lines, and the y rays at x = cx,
+    z = cz and along the outline run through edges and vertices.  y sits
+    0.3 off the grid, so no node lies on the surface.  r is a power of two,
+    so every barycentric coordinate of a grid ray is an exact binary
+    fraction (the third, 1 - c1 - c2, included: with r = 6 its rounding
+    leaves some edge rays unresolved in the reference's arithmetic too).
+    Engine and oracle both fill exactly the closed-form nodes, and the
+    engine reports the edge hits, half resolved each way, as the reference
+    does."""
+    cx, cy, cz, r = 520, 9, 530, 8
+    yc = cy + 0.3
+    # mesh coordinates: the transform moves them by -1/2 (plus the offsets)
+    C = np.array([cx + 0.5, yc + 0.5, cz + 0.5])
+    V = [C + r * np.array(d) for d in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1))]
+    tris = [[V[i], V[j], V[k]] for i in (0, 1) for j in (2, 3) for k in (4, 5)]
+    f = write_stl(tmp_path / "oct.stl", tris)
+    size = 'nx="540" ny="20" nz="540"'
+    body = f'<Wall><STL file="{f}" dx="505" dz="515" nx="30" nz="30"/></Wall>'
+    c, xml = case("d3q19", body, size=size)
+    got = c.rasterize() == G.Packing("d3q19").value("Wall")
+    Z, Y, X = np.meshgrid(np.arange(540), np.arange(20), np.arange(540), indexing="ij")
+    want = (np.abs(X - cx) + np.abs(Y - yc) + np.abs(Z - cz)) < r
+    assert want.sum() > 100
+    assert np.array_equal(got, want)
+    b, _, _ = G.read_case(xml, "d3q19", is_text=True)
+    assert np.array_equal(c.rasterize(), b)
+    edges = [m for m in c.messages if "edge hits" in m]
+    assert edges and int(edges[0].split()[-1]) > 100, c.messages
+    neg = [m for m in c.messages if "resolved negatively" in m]
+    assert int(neg[0].split()[-1]) * 2 == int(edges[0].split()[-1])
+    assert not any("could not be resolved" in m for m in c.messages)
+
+
 @pytest.mark.parametrize("rot,shift,image", [
     # Zrot 90d: (x, y) -> (-y, x); Xrot 90d: (y, z) -> (-z, y); Yrot 90d: (z, x) -> (-x, z)
     ('Zrot="90d"', 'x="12"', lambda lo, hi: ((12 - hi[1], 12 - lo[1]), (lo[0], hi[0]), (lo[2], hi[2]))),
