@@ -261,12 +261,18 @@ int tclbx_batch_get_quantity(tclbx_batch* b, int id, int q, double* out, double 
 int tclbx_batch_get_density(tclbx_batch* b, int id, int q, double* out);
 int tclbx_batch_set_density(tclbx_batch* b, int id, int q, const double* in);
 /* time every tile shape (row segments or flat runs of the node index;
- * threads per block 64/128/256; rows per tile 1/2/4) and persistent grid
- * size on the current cases for `steps` steps each, without changing their
- * state, and keep the fastest */
+ * threads per block 64/128/256; rows per tile 1/2/4), persistent grid size
+ * and tile order on the current cases for `steps` steps each, without
+ * changing their state, and keep the fastest */
 int tclbx_batch_autotune(tclbx_batch* b, int steps, int* flat, int* block, int* rows_per_tile, int* grid,
                          double* us_per_step);
 int tclbx_batch_set_tuning(tclbx_batch* b, int flat, int block, int rows_per_tile, int grid);
+/* the order the step walks the tiles in: 0 the tile list's, 1 XCD-
+ * contiguous (the k-th of the eight XCDs takes the k-th contiguous run of
+ * tiles, so neighbouring rows are pulled through one L2); the autotuner
+ * times both.  set < 0 leaves it; *order gets the current one.  No
+ * counterpart in the reference (batching is this engine's). */
+int tclbx_batch_tile_order(tclbx_batch* b, int set, int* order);
 int tclbx_batch_get_timing(const tclbx_batch* b, double* last_ms, double* total_ms);
 int tclbx_batch_info(const tclbx_batch* b, int* ncases, long long* nodes, long long* arena_used,
                      long long* arena_capacity);

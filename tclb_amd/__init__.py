@@ -44,7 +44,7 @@ SYMBOLS = [
     "tclbx_batch_flag_overwrite", "tclbx_batch_init_case", "tclbx_batch_iterate", "tclbx_batch_get_globals",
     "tclbx_batch_get_iteration", "tclbx_batch_get_quantity", "tclbx_batch_get_density",
     "tclbx_batch_set_density", "tclbx_batch_autotune", "tclbx_batch_set_tuning", "tclbx_batch_get_timing",
-    "tclbx_batch_info", "tclbx_batch_set_graph_steps", "tclbx_batch_prepare",
+    "tclbx_batch_info", "tclbx_batch_tile_order", "tclbx_batch_set_graph_steps", "tclbx_batch_prepare",
     "tclbx_server_create", "tclbx_server_destroy", "tclbx_server_submit", "tclbx_server_poll",
     "tclbx_server_wait_all", "tclbx_server_stats", "tclbx_server_set_tuning",
     "tclbx_case_load", "tclbx_case_load_string", "tclbx_case_destroy", "tclbx_case_size", "tclbx_case_units",
@@ -145,6 +145,7 @@ def lib():
     L.tclbx_batch_set_tuning.argtypes = [vp, c_int, c_int, c_int, c_int]
     L.tclbx_batch_get_timing.argtypes = [vp, dp, dp]
     L.tclbx_batch_info.argtypes = [vp, ip, llp, llp, llp]
+    L.tclbx_batch_tile_order.argtypes = [vp, c_int, ip]
     L.tclbx_batch_set_graph_steps.argtypes = [vp, c_int]
     L.tclbx_batch_prepare.argtypes = [vp]
     L.tclbx_server_create.argtypes = [c_char_p, c_char_p, ll, c_int, ctypes.POINTER(vp)]
@@ -664,10 +665,18 @@ class Batch:
         _check(lib().tclbx_batch_autotune(self._h, int(steps), ctypes.byref(fl), ctypes.byref(b), ctypes.byref(r),
                                           ctypes.byref(g), ctypes.byref(us)))
         return {"flat": fl.value, "block": b.value, "rows_per_tile": r.value, "grid": g.value,
-                "us_per_step": us.value}
+                "order": self.tile_order(), "us_per_step": us.value}
 
-    def set_tuning(self, block=256, rows_per_tile=1, grid=0, flat=False):
+    def set_tuning(self, block=256, rows_per_tile=1, grid=0, flat=False, order=0):
         _check(lib().tclbx_batch_set_tuning(self._h, int(bool(flat)), int(block), int(rows_per_tile), int(grid)))
+        self.tile_order(order)
+
+    def tile_order(self, set=None):
+        """The tile order of the step (0: list order, 1: XCD-contiguous);
+        `set` changes it."""
+        o = ctypes.c_int()
+        _check(lib().tclbx_batch_tile_order(self._h, -1 if set is None else int(set), ctypes.byref(o)))
+        return o.value
 
     def timing(self):
         a, b = ctypes.c_double(), ctypes.c_double()

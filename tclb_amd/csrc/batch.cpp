@@ -334,8 +334,10 @@ void Batch::launch(int phase, bool glob) {
     if (ncases_dev_ == 0) return;
     const int nt = glob ? ngtiles_ : ntiles_;
     const int grid = glob ? std::min(nt, 8 * ncu_) : (tune_.grid > 0 ? std::min(tune_.grid, nt) : nt);
-    hip_check(launch_batch_step(model->id, storage, glob, d_cases_, glob ? d_gtiles_ : d_tiles_, nt, phase,
-                                d_gpart_, grid, tune_.block, stream_), "batch step kernel");
+    // the phase word: bit 0 the snapshot phase, bit 1 the tile order
+    hip_check(launch_batch_step(model->id, storage, glob, d_cases_, glob ? d_gtiles_ : d_tiles_, nt,
+                                (phase & 1) | (tune_.order ? 2 : 0), d_gpart_, grid, tune_.block, stream_),
+              "batch step kernel");
     if (glob) {
         hip_check(launch_reduce_globals_cases(d_gpart_, ngtiles_, kernel_nglobals(model->id), model->sum_globals,
                                               ncases_dev_, d_first_, d_count_, d_glob_, stream_),
@@ -359,8 +361,8 @@ void Batch::capture_graph() {
     for (int k = 0; k < graph_steps_; k++) {
         const int nt = ntiles_;
         const int grid = tune_.grid > 0 ? std::min(tune_.grid, nt) : nt;
-        hipError_t e = launch_batch_step(model->id, storage, false, d_cases_, d_tiles_, nt, k & 1, d_gpart_, grid,
-                                         tune_.block, stream_);
+        hipError_t e = launch_batch_step(model->id, storage, false, d_cases_, d_tiles_, nt,
+                                         (k & 1) | (tune_.order ? 2 : 0), d_gpart_, grid, tune_.block, stream_);
         if (e != hipSuccess) {
             hipGraph_t g = nullptr;
             (void)hipStreamEndCapture(stream_, &g);
@@ -495,6 +497,7 @@ void Batch::SetTuning(const Tuning& t) {
     if (t.block != 64 && t.block != 128 && t.block != 256) throw std::invalid_argument("block must be 64, 128 or 256");
     if (t.rpt < 1 || t.rpt > 64) throw std::invalid_argument("rows per tile must be 1..64");
     if (t.grid < 0) throw std::invalid_argument("grid must be >= 0");
+    if (t.order != 0 && t.order != 1) throw std::invalid_argument("tile order must be 0 or 1");
     tune_ = t;
     dirty_ = true;
 }
@@ -512,12 +515,14 @@ Tuning Batch::Autotune(int steps) {
     for (int flat : {0, 1})
     for (int block : {256, 128, 64})
         for (int rpt : {1, 2, 4})
-            for (int gmul : {0, 8}) {
+            for (int gmul : {0, 8})
+            for (int order : {0, 1}) {
                 Tuning t;
                 t.flat = flat;
                 t.block = block;
                 t.rpt = rpt;
                 t.grid = gmul * ncu_;
+                t.order = order;
                 tune_ = t;
                 rebuild();
                 upload_zones();

@@ -48,6 +48,7 @@ struct Tuning {
     int rpt = 1;      // rows per plain tile (nodes per thread)
     int grid = 0;     // persistent blocks (0: one block per tile)
     int flat = 0;     // 1: tiles are runs of the linear node index (no idle lanes at row ends)
+    int order = 0;    // 1: tiles walked in XCD-contiguous order (xcd_tile)
     double us_per_step = 0;
 };
 

@@ -536,6 +536,18 @@ int tclbx_batch_set_tuning(tclbx_batch* b, int flat, int block, int rows_per_til
         return TCLBX_OK;
     });
 }
+int tclbx_batch_tile_order(tclbx_batch* b, int set, int* order) {
+    CHECK_B(b);
+    return guard([&]() {
+        if (set >= 0) {
+            Tuning t = b->b->tuning();
+            t.order = set;
+            b->b->SetTuning(t);
+        }
+        if (order) *order = b->b->tuning().order;
+        return TCLBX_OK;
+    });
+}
 int tclbx_batch_get_timing(const tclbx_batch* b, double* last_ms, double* total_ms) {
     CHECK_B(b);
     if (last_ms) *last_ms = b->b->last_iterate_ms();

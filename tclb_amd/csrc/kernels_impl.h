@@ -1896,16 +1896,30 @@ constexpr int step_waves() {
     return 0;
 }
 
+// Tile of virtual block v (v = blockIdx.x + k gridDim.x) in XCD-contiguous
+// order.  The hardware deals the blocks of a launch to the eight XCDs in
+// turn, block b to XCD b % 8 (and virtual block v too while gridDim.x is a
+// multiple of 8), so this order gives XCD k the k-th contiguous run of the
+// tile list: the rows a tile pulls from its neighbours are mostly fetched
+// through the same L2.  A bijection of [0, n): the first n % 8 XCDs take
+// n/8 + 1 tiles, the others n/8.
+__device__ __forceinline__ int xcd_tile(int v, int n) {
+    const int q = n >> 3, r = n & 7, x = v & 7, i = v >> 3;
+    return x * q + (x < r ? x : r) + i;
+}
+
 // One step of a batch of lattices (a ragged arena): a persistent grid walks
 // the tile list; tile t = {case, tile within the case}.  Each case has its
-// own descriptor (sizes, pitch, snapshots, flags, zone table, parameters);
-// `phase` selects which snapshot of every case is read this step.  Globals
+// own descriptor (sizes, pitch, snapshots, flags, zone table, parameters).
+// Bit 0 of `phase` selects which snapshot of every case is read this step;
+// bit 1 walks the tiles in XCD-contiguous order (xcd_tile).  Globals
 // partials of case c occupy the slots of its tiles, [tile0, tile0 + ntiles).
 template <int MODEL, int ST, bool GLOB>
 __global__ void __launch_bounds__(256) k_batch_step(const BatchCase* __restrict__ cases,
                                                     const int2* __restrict__ tiles, int ntiles, int phase,
                                                     double* gpart) {
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int v = blockIdx.x; v < ntiles; v += gridDim.x) {
+        const int t = (phase & 2) ? xcd_tile(v, ntiles) : v;
         const int2 tl = tiles[t];
         StepArgs a = cases[tl.x].args;
         if ((phase ^ cases[tl.x].parity) & 1) {
@@ -1925,7 +1939,8 @@ template <int MODEL, int ST, bool GLOB>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
 k_batch_step_w8(const BatchCase* __restrict__ cases, const int2* __restrict__ tiles, int ntiles, int phase,
                 double* gpart) {
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int v = blockIdx.x; v < ntiles; v += gridDim.x) {
+        const int t = (phase & 2) ? xcd_tile(v, ntiles) : v;
         const int2 tl = tiles[t];
         StepArgs a = cases[tl.x].args;
         if ((phase ^ cases[tl.x].parity) & 1) {

@@ -75,6 +75,32 @@ def test_flat_tiles_match_row_tiles(flat):
         glob_close(a.globals(x), b.globals(y), 1e-13)
 
 
+@pytest.mark.parametrize("grid", [0, 17, 2048])
+@pytest.mark.parametrize("flat", [False, True])
+def test_xcd_tile_order_changes_nothing(grid, flat):
+    """Walking the tiles in XCD-contiguous order (each of the eight XCDs
+    takes a contiguous run of the tile list) updates every tile exactly
+    once, in one block per tile and in persistent grids of 17 and 2048
+    blocks: distributions and Globals bit for bit as in list order.  The
+    13 cases of random sizes make a tile count that is in general not a
+    multiple of 8."""
+    say(f"xcd tile order grid={grid} flat={flat}")
+    cs = mixed_cases("d2q9_SRT_drag", 13)
+    a = tclb_amd.Batch("d2q9_SRT_drag", "f32s", capacity_bytes=64 << 20)
+    b = tclb_amd.Batch("d2q9_SRT_drag", "f32s", capacity_bytes=64 << 20)
+    a.set_tuning(block=64, rows_per_tile=1, grid=grid, flat=flat, order=1)
+    b.set_tuning(block=64, rows_per_tile=1, grid=grid, flat=flat, order=0)
+    assert a.tile_order() == 1 and b.tile_order() == 0
+    ia = [a.add(c) for c in cs]
+    ib = [b.add(c) for c in cs]
+    for n in (5, 33):
+        a.iterate(n)
+        b.iterate(n)
+        for x, y in zip(ia, ib):
+            assert np.array_equal(a.get_f(x), b.get_f(y), equal_nan=True)
+            assert np.array_equal(a.globals(x), b.globals(y), equal_nan=True)
+
+
 def test_batch_case_alone_and_in_mix_and_oracle(oracle_mod):
     say("case alone vs in a mix vs oracle")
     cs = mixed_cases("d3q19", 5)
@@ -175,3 +201,51 @@ def test_sweep_generator_cases_run():
         assert np.all(np.isfinite(b.globals(cid)))
         assert b.globals(cid)[gd] > 0          # the flow pushes every disc downstream
         assert np.all(np.isfinite(b.quantity(cid, "U")))
+
+
+def test_full_sweep_batch_case_by_case(oracle_mod):
+    """The cylinder sweep at its stated size: all 256 cases (24.9M nodes)
+    in one f16s batch, autotuned, 200 steps.  Every case stays finite with
+    drag downstream, and a sample of eight cases (the four smallest, four
+    at random) equals, bit for bit, its own solve in a batch of one with the
+    same tile shape.  In FP64 the three smallest cases of the sweep, run
+    as one batch, match the oracle after 200 steps (populations to 1e-12,
+    Drag/Lift to 1e-10)."""
+    say("full 256-case sweep, case by case")
+    steps = 200
+    sweep = cases.d2q9_cylinder_sweep(ncases=256, steps=steps)
+    model = sweep[0].model
+    probe = tclb_amd.Batch(model, "f16s", 1 << 20)
+    cap = sum(probe.case_bytes(c.nx, c.ny) for c in sweep) + (16 << 20)
+    probe.close()
+    b = tclb_amd.Batch(model, "f16s", capacity_bytes=cap)
+    ids = [b.add(c) for c in sweep]
+    assert b.info()["nodes"] == sum(c.nodes for c in sweep)
+    t = b.autotune(steps=2)
+    b.iterate(steps)
+    gd = tclb_amd.Model(model).global_id("Drag")
+    for cid in ids:
+        g = b.globals(cid)
+        assert np.all(np.isfinite(g)) and g[gd] > 0
+    order = sorted(range(256), key=lambda k: sweep[k].nodes)
+    sample = order[:4] + list(np.random.default_rng(3).choice(order[4:], 4, replace=False))
+    for k in sample:
+        one = tclb_amd.Batch(model, "f16s", capacity_bytes=sweep[k].nodes * 64 + (4 << 20))
+        one.set_tuning(block=t["block"], rows_per_tile=t["rows_per_tile"], grid=t["grid"], flat=t["flat"],
+                       order=t["order"])
+        c1 = one.add(sweep[k])
+        one.iterate(steps)
+        assert np.array_equal(b.get_f(ids[k]), one.get_f(c1)), sweep[k].name
+        assert np.array_equal(b.globals(ids[k]), one.globals(c1)), sweep[k].name
+        one.close()
+    b.close()
+    small = [sweep[k] for k in order[:3]]
+    f = tclb_amd.Batch(model, "f64", capacity_bytes=64 << 20)
+    fid = [f.add(c) for c in small]
+    f.iterate(steps)
+    for c, cid in zip(small, fid):
+        orc = oracle_for_case(oracle_mod, c, "f64")
+        g = orc.iterate(steps)
+        a, o = f.get_f(cid), orc.get_f()
+        assert np.max(np.abs(a - o)) <= 1e-12 * np.max(np.abs(o)), c.name
+        glob_close(f.globals(cid), g[:f.nglobals], 1e-10)
