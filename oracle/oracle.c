@@ -98,17 +98,26 @@ uint32_t orc_sr_bits(uint32_t node, uint32_t q, uint32_t step, uint32_t seed)
     return fmix32(node * 0x9e3779b1u ^ fmix32(step * 0x85ebca77u + q * 0xc2b2ae3du + seed));
 }
 
-/* x -> binary16, stochastically: the neighbours from the nearest-even
- * conversion, the choice from the top 24 random bits */
+/* x -> binary16, stochastically: k = the bits of x's float significand
+ * below binary16's spacing at |x| (13 where binary16 is normal, -1 - e for
+ * a float exponent e in [-24, -15], where it is subnormal); k random bits
+ * added to them carry into the kept bits with probability (low bits)/2^k,
+ * and the truncated value converts exactly.  Below the smallest subnormal
+ * (e <= -25) the result is +-2^-24 with probability |x| 2^24, else +-0,
+ * from the top 24 random bits. */
 uint16_t orc_f32_to_f16_sr(float x, uint32_t r)
 {
-    uint16_t h = orc_f32_to_f16(x), lo, hi;
-    float hf = orc_f16_to_f32(h), p;
-    if (hf == x || (h & 0x7fffu) >= 0x7c00u) return h;
-    if (fabsf(hf) > fabsf(x)) { hi = h; lo = (uint16_t)(h - 1); }
-    else { lo = h; hi = (uint16_t)(h + 1); }
-    p = (fabsf(x) - fabsf(orc_f16_to_f32(lo))) / (fabsf(orc_f16_to_f32(hi)) - fabsf(orc_f16_to_f32(lo)));
-    return (float)(r >> 8) * 5.9604644775390625e-08f < p ? hi : lo;
+    uint32_t u = f2u(x), a = u & 0x7fffffffu, k, mask;
+    int e;
+    if (a >= 0x7f800000u) return orc_f32_to_f16(x);
+    e = (int)(a >> 23) - 127;
+    if (e <= -25) {
+        int up = (float)(r >> 8) * 5.9604644775390625e-08f < u2f(a) * 16777216.0f;
+        return (uint16_t)(((u >> 16) & 0x8000u) | (up ? 1u : 0u));
+    }
+    k = e >= -14 ? 13u : (uint32_t)(-1 - e);
+    mask = (1u << k) - 1u;
+    return orc_f32_to_f16(u2f((u + (r & mask)) & ~mask));
 }
 
 /* x -> bfloat16, stochastically: the low 16 bits of x plus 16 random bits

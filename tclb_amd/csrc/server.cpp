@@ -43,9 +43,13 @@ Server::Server(const ModelDef* m, int storage, size_t hbm_budget, int max_chunk)
     if (max_chunk < 1) throw std::invalid_argument("chunk must be at least one step");
     batch_.reset(new Batch(m, storage, hbm_budget));
     // requests have arbitrary sizes: flat tiles leave no lane idle at the
-    // end of a row
+    // end of a row; four rows per tile (a block walks four runs of 256
+    // nodes) measured 64.6k MLUPS on the d3q19 f16s serving mix against
+    // 51.6k with one, as fast as one 256^3 lattice alone (66.4k)
     Tuning t;
     t.flat = 1;
+    t.block = 256;
+    t.rpt = 4;
     batch_->SetTuning(t);
     worker_ = std::thread([this]() { loop(); });
 }

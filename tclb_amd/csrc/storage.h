@@ -29,19 +29,25 @@ __device__ __forceinline__ uint32_t fmix32(uint32_t h) {
 __device__ __forceinline__ uint32_t sr_bits(uint32_t node, uint32_t q, uint32_t step, uint32_t seed) {
     return fmix32(node * 0x9e3779b1u ^ fmix32(step * 0x85ebca77u + q * 0xc2b2ae3du + seed));
 }
-// float -> half: the neighbours around the nearest-even conversion, the
-// choice from the top 24 random bits
+// float -> half: k = the bits of x's float significand below binary16's
+// spacing at |x| (13 where binary16 is normal, -1 - e for a float exponent
+// e in [-24, -15], where it is subnormal); k random bits added to those
+// bits carry into the kept ones with probability (low bits)/2^k, and the
+// truncated value converts to binary16 exactly.  Below binary16's smallest
+// subnormal (e <= -25) x becomes +-2^-24 with probability |x| 2^24, else
+// +-0, drawn from the top 24 random bits.
 __device__ __forceinline__ _Float16 f16_sr(float x, uint32_t r) {
-    const _Float16 h = (_Float16)x;
-    const float hf = (float)h;
-    const uint16_t hb = __builtin_bit_cast(uint16_t, h);
-    if (hf == x || (hb & 0x7fffu) >= 0x7c00u) return h;
-    const bool above = fabsf(hf) > fabsf(x);
-    const uint16_t lob = above ? (uint16_t)(hb - 1) : hb, hib = above ? hb : (uint16_t)(hb + 1);
-    const float lo = fabsf((float)__builtin_bit_cast(_Float16, lob));
-    const float hi = fabsf((float)__builtin_bit_cast(_Float16, hib));
-    const float p = (fabsf(x) - lo) / (hi - lo);
-    return __builtin_bit_cast(_Float16, (float)(r >> 8) * 5.9604644775390625e-08f < p ? hib : lob);
+    const uint32_t u = __builtin_bit_cast(uint32_t, x);
+    const uint32_t a = u & 0x7fffffffu;
+    if (a >= 0x7f800000u) return (_Float16)x;
+    const int e = (int)(a >> 23) - 127;
+    if (e <= -25) {
+        const bool up = (float)(r >> 8) * 5.9604644775390625e-08f < __builtin_bit_cast(float, a) * 16777216.0f;
+        return __builtin_bit_cast(_Float16, (uint16_t)(((u >> 16) & 0x8000u) | (up ? 1u : 0u)));
+    }
+    const uint32_t k = e >= -14 ? 13u : (uint32_t)(-1 - e);
+    const uint32_t mask = (1u << k) - 1u;
+    return (_Float16)__builtin_bit_cast(float, (u + (r & mask)) & ~mask);
 }
 // float -> bfloat16: the low 16 bits plus 16 random bits carry into the
 // kept half with probability (low bits)/2^16

@@ -652,8 +652,8 @@ def test_cavity_small_parity(oracle_mod):
 @pytest.mark.parametrize("storage", ["f32s", "f16s", "bf16s"])
 def test_cavity_reduced_storage_tracks_f64(storage):
     """A 48^3 cavity after 2000 steps: every reduced format stays close to
-    the FP64 run (velocity relative L2 error below 2e-2 for the 16-bit
-    formats, 1e-4 for FP32).  The lid torque is dominated by the pressure
+    the FP64 run (velocity relative L2 error below 1e-4 for FP32, 1e-3 for
+    f16s, 2e-2 for bf16s).  The lid torque is dominated by the pressure
     on the lid (the x-lever of Fy), the least precise quantity in a shifted
     format: within 5% for FP32 and FP16, 25% for BF16 (8-bit mantissa)."""
     say(f"cavity 48^3 {storage} vs f64")
@@ -664,7 +664,10 @@ def test_cavity_reduced_storage_tracks_f64(storage):
     lat.iterate(2000)
     u0, u1 = ref.quantity("U"), lat.quantity("U")
     l2 = float(np.linalg.norm(u1 - u0) / np.linalg.norm(u0))
-    bound = 1e-4 if storage == "f32s" else 2e-2
+    # the bounds: about three times the f16s error measured on MI355X
+    # (3.2e-4); bf16s measured 1.8e-2
+    bound = {"f32s": 1e-4, "f16s": 1e-3, "bf16s": 2e-2}[storage]
+    say(f"  u L2 {l2:.3e} (bound {bound:.0e})")
     assert l2 < bound, l2
     t0, t1 = ref.global_dict()["MovingWallTorque"], lat.global_dict()["MovingWallTorque"]
     assert abs(t1 - t0) <= (0.25 if storage == "bf16s" else 0.05) * abs(t0), (t0, t1)
@@ -732,7 +735,11 @@ def test_cavity_stochastic_rounding_tracks_f64(storage):
         u1 = lat.quantity("U")
         errs[sr] = float(np.linalg.norm(u1 - u0) / np.linalg.norm(u0))
     say(f"  u L2: nearest {errs[False]:.3e}, stochastic {errs[True]:.3e}")
-    assert errs[True] < (2e-2 if storage == "f16s" else 0.1), errs
+    # about three times the measured errors (f16s 4.0e-4, bf16s 3.0e-3 with
+    # stochastic rounding; bf16s 1.8e-2 to nearest)
+    assert errs[True] < (1.2e-3 if storage == "f16s" else 1e-2), errs
+    if storage == "bf16s":
+        assert errs[True] < errs[False] / 3, errs
 
 
 @pytest.mark.parametrize("storage", ["f64", "f16s"])

@@ -1655,11 +1655,14 @@ def _sr_neighbours_f16(x):
     return lo, hi
 
 
-@pytest.mark.parametrize("x", [1.0e-3 + 3.1e-7, -2.7e-3 - 1.3e-7, 3.0e-6 + 1.1e-8, -4.5e-7, 0.15 + 1e-5, 1.0 / 3.0])
+@pytest.mark.parametrize("x", [1.0e-3 + 3.1e-7, -2.7e-3 - 1.3e-7, 3.0e-6 + 1.1e-8, -4.5e-7, 0.15 + 1e-5, 1.0 / 3.0,
+                               2.3e-8, -4.1e-8, 6.09e-5, 6.13e-5, -1234.567])
 def test_stochastic_rounding_f16_is_unbiased(O, x):
     """The oracle's stochastic binary16 rounding (the engine's option for the
     16-bit formats): every result is one of the two neighbours lo, hi of x
-    (normal and subnormal binary16 alike), the mean over the hash's bits is x
+    (normal and subnormal binary16 alike, either side of the smallest normal
+    2^-14, and below the smallest subnormal 2^-24, where lo = 0), the mean
+    over the hash's bits is x
     within 4.5 standard errors, and the variance is (|hi| - |x|)(|x| - |lo|),
     the two-point distribution's.  Round-to-nearest would fail the mean."""
     L = O.lib()
