@@ -7,7 +7,8 @@ y = 0 and 255, body force g_x = 1e-6, random Solid spheres of radius 4-12 up
 to 10% solid (splitmix64 seed 2), tau = 0.6 with s_e = 1.19, s_eps = 1.4,
 s_q = 1.2.  `value` is MLUPS (every node counted, as the reference counts
 them, src/main.cpp:101-107) with FP64 storage; the other storage formats
-are reported beside it.  Inputs are resident in HBM when the timed region
+are reported beside it ("f16s+sr": FP16-shift with stochastic rounding of
+its stores).  Inputs are resident in HBM when the timed region
 starts; every step runs the full kernel (the last one of the timed run also
 computes the Globals).
 
@@ -200,7 +201,7 @@ def main():
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--n", type=int, default=256, help="lattice side")
-    ap.add_argument("--formats", default="f64,f32s,f16s,bf16s")
+    ap.add_argument("--formats", default="f64,f32s,f16s,bf16s,f16s+sr")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-n", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
@@ -233,7 +234,11 @@ def main():
     g_x = 1e-6
     permeability = None
     for fmt in args.formats.split(","):
-        lat = case.build(fmt)
+        # "<f16s|bf16s>+sr": that format with stochastic rounding of its stores
+        storage, sr = (fmt[:-3], True) if fmt.endswith("+sr") else (fmt, False)
+        lat = case.build(storage)
+        if sr:
+            lat.set_stochastic_rounding(True, seed=1)
         # the rows-per-tile timing and the graph capture happen here, not in
         # the timed steps
         lat.prepare()
@@ -322,6 +327,8 @@ def main():
         out["permeability"] = permeability
     if "f16s" in results and "f64" in results:
         out["f16s_over_f64"] = round(results["f16s"]["MLUPS"] / results["f64"]["MLUPS"], 3)
+    if "f16s+sr" in results and "f64" in results:
+        out["f16s_sr_over_f64"] = round(results["f16s+sr"]["MLUPS"] / results["f64"]["MLUPS"], 3)
     if not args.no_extra:
         log("batched sweep")
         out["batched"] = batched_section(args.batch_steps, 16)
