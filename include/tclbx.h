@@ -224,9 +224,10 @@ int tclbx_get_sorted_dispatch(tclbx_lattice* h, int* on, long long* list_nodes);
  * bf16s; TCLBX_ERR for the others): a value between two representable ones
  * is stored as either with the probabilities that make its expectation
  * exact, so rounding errors do not accumulate into a drift over long runs.
- * The random bits come from a hash of (node, direction, iteration, seed): a
- * run is reproducible.  The step loop is then launched step by step (no
- * captured graph).  No counterpart in the reference, which rounds to
+ * The rounding is gfx950's v_cvt_sr_f16_f32 / v_cvt_sr_bf16_f32; the
+ * random bits come from a hash of (node, iteration, seed) and a xorshift32
+ * stream over the directions: a run is reproducible.  The step loop is then
+ * launched step by step (no captured graph).  No counterpart in the reference, which rounds to
  * nearest (src/LatticeAccess.inc.cpp.Rt:14-35). */
 int tclbx_set_stochastic_rounding(tclbx_lattice* h, int on, unsigned seed);
 

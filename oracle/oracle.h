@@ -145,7 +145,8 @@ float    orc_f16_to_f32(uint16_t h);
 uint16_t orc_f32_to_bf16(float v);
 float    orc_bf16_to_f32(uint16_t h);
 /* stochastic rounding of the 16-bit formats (an engine option): the random
- * bits of (node, direction, step, seed), and the two conversions */
+ * word of (node, direction, step, seed), and the two conversions (gfx950's
+ * v_cvt_sr_f16_f32 and v_cvt_sr_bf16_f32 restated) */
 uint32_t orc_sr_bits(uint32_t node, uint32_t q, uint32_t step, uint32_t seed);
 uint16_t orc_f32_to_f16_sr(float x, uint32_t r);
 uint16_t orc_f32_to_bf16_sr(float x, uint32_t r);

@@ -229,6 +229,7 @@ StepArgs Batch::case_args(const Case& c) const {
     a.nx = c.nx; a.ny = c.ny; a.nz = c.nz;
     a.flat = tune_.flat;
     a.inv_nx = 1.0 / c.nx;
+    a.inv_ny = 1.0 / c.ny;
     if (tune_.flat) {
         // a "row" of a flat tile is a run of `block` consecutive nodes
         a.bpr = 1;

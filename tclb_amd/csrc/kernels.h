@@ -35,6 +35,7 @@ struct StepArgs {
     int rpt;               // rows walked by one block of a plain step
     int flat;              // 1: tiles are runs of the linear node index, not row segments
     double inv_nx;         // 1/nx (flat tiles: node index -> row)
+    double inv_ny;         // 1/ny (flat tiles: row -> y, z)
     int zcols, nzones;
     unsigned ib_mask;      // NodeType bit of IB (the cumulant IB builds), 0 for every other model
     // NodeType-sorted dispatch of a plain step: mode 0 updates every node of
@@ -47,9 +48,9 @@ struct StepArgs {
     const unsigned* list;
     unsigned list_n;
     // stochastic rounding of the stores of a 16-bit format (0: nearest even);
-    // sr_step is the iteration the step computes, sr_seed the lattice's seed
+    // sr_key = sr_key(iteration the step computes, the lattice's seed)
     int sr;
-    unsigned sr_step, sr_seed;
+    unsigned sr_key;
     double par[MAX_PAR];
 };
 

@@ -529,10 +529,19 @@ __device__ __forceinline__ void d2q9m_node(R* f, unsigned nt, const double* par,
             g[0] += (double)(obj == 0x40u ? -loss : loss); g[OBJ] += zr[3];
         }
     }
+    // 3 a b / rho: an IEEE division in FP64 (the reference's roundings), one
+    // approximate reciprocal of rho in float (the float formats are held to a
+    // tolerance of a few float ulps, not to identical roundings)
+    R ir = R(0);
+    if constexpr (!std::is_same<R, double>::value) ir = __builtin_amdgcn_rcpf(rho);
+    const auto q3 = [&](R u, R v) {
+        if constexpr (std::is_same<R, double>::value) return R(3.) * u * v / rho;
+        else return R(3.) * u * v * ir;
+    };
     const R S2 = (R)par[0], S3 = (R)par[1], S4 = (R)par[2];
-    m[3] = S2 * (m[3] - R(3.) * Jx * Jx / rho);
-    m[4] = S2 * (m[4] - R(3.) * Jy * Jy / rho);
-    m[5] = S2 * (m[5] - R(3.) * Jx * Jy / rho);
+    m[3] = S2 * (m[3] - q3(Jx, Jx));
+    m[4] = S2 * (m[4] - q3(Jy, Jy));
+    m[5] = S2 * (m[5] - q3(Jx, Jy));
     m[6] = S3 * m[6];
     m[7] = S3 * m[7];
     m[8] = S4 * m[8];
@@ -549,9 +558,9 @@ __device__ __forceinline__ void d2q9m_node(R* f, unsigned nt, const double* par,
     }
     m[1] = Jx;
     m[2] = Jy;
-    m[3] = m[3] + R(3.) * Jx * Jx / rho;
-    m[4] = m[4] + R(3.) * Jy * Jy / rho;
-    m[5] = m[5] + R(3.) * Jx * Jy / rho;
+    m[3] = m[3] + q3(Jx, Jx);
+    m[4] = m[4] + q3(Jy, Jy);
+    m[5] = m[5] + q3(Jx, Jy);
     d2q9m_from_moments(f, m);
 }
 
@@ -1691,7 +1700,10 @@ __device__ __forceinline__ void d2q9_autosym(R* f, unsigned nt, LOAD load) {
 // One node of row `row` (y + ny*z) at x = xc: pull with periodic wrap, Run(),
 // store; `active` is false for the lanes past the end of the row (they load
 // a valid node but store nothing).
-template <int MODEL, int ST, bool GLOB, int DISPATCH = 0>
+// LANE: `row` differs between the lanes of a wave (flat tiles, the sorted
+// list), so y and z come from a double estimate of row / ny, corrected to the
+// exact quotient, instead of a per-lane integer division.
+template <int MODEL, int ST, bool GLOB, int DISPATCH = 0, bool LANE = false>
 __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, int xm, int xp, bool active,
                                             double* g) {
     using L = Lat<MODEL>;
@@ -1700,8 +1712,20 @@ __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, 
     using R = typename S::R;
     constexpr int Q = L::Q;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const int y = row % ny;
-    const int z = row / ny;
+    // a 2D lattice has nz = 1: its row is y
+    int y, z;
+    if constexpr (L::Q == 9) {
+        y = row;
+        z = 0;
+    } else if constexpr (LANE) {
+        z = (int)((double)row * a.inv_ny);
+        y = row - z * ny;
+        if (y < 0) { z--; y += ny; }
+        if (y >= ny) { z++; y -= ny; }
+    } else {
+        y = row % ny;
+        z = row / ny;
+    }
     const int ym = y == 0 ? ny - 1 : y - 1, yp = y == ny - 1 ? 0 : y + 1;
     const int zm = z == 0 ? nz - 1 : z - 1, zp = z == nz - 1 ? 0 : z + 1;
 
@@ -1772,11 +1796,14 @@ __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, 
         char* fout = (char*)a.fout;
         if constexpr (S::stochastic) {
             if (a.sr) {
+                // direction 2j rounds with word j, 2j + 1 with word j << 16
+                uint32_t w = sr_word0(n, a.sr_key);
 #pragma unroll
                 for (int i = 0; i < Q; i++) {
-                    const uint32_t r = sr_bits(n, (uint32_t)i, a.sr_step, a.sr_seed);
+                    const uint32_t r = (i & 1) ? w << 16 : w;
                     __builtin_nontemporal_store(DEV ? S::unraw_sr(f[i], r) : S::put_sr(f[i], (R)L::w(i), r),
                                                 (T*)(fout + (size_t)i * pbytes + n * (unsigned)sizeof(T)));
+                    if (i & 1) w = xorshift32(w);
                 }
                 return;
             }
@@ -1820,7 +1847,7 @@ __device__ __forceinline__ void step_tile(const StepArgs& a, int tile, int rpb, 
             int row, x;
             split_node(in ? n : N - 1, a.nx, a.inv_nx, row, x);
             const int xm = x == 0 ? a.nx - 1 : x - 1, xp = x == a.nx - 1 ? 0 : x + 1;
-            node_update<MODEL, ST, GLOB, DISPATCH>(a, row, x, xm, xp, in, g);
+            node_update<MODEL, ST, GLOB, DISPATCH, true>(a, row, x, xm, xp, in, g);
         }
         if (GLOB) {
             block_globals<L::G>(g, L::SUMG, a.gpart, slot, nslots);
@@ -1886,7 +1913,7 @@ k_step_list(StepArgs a) {
     split_node(n, a.nx, a.inv_nx, row, x);
     const int xm = x == 0 ? a.nx - 1 : x - 1, xp = x == a.nx - 1 ? 0 : x + 1;
     double g[1];
-    node_update<MODEL, ST, false>(a, row, x, xm, xp, in, g);
+    node_update<MODEL, ST, false, 0, true>(a, row, x, xm, xp, in, g);
 }
 
 // waves per SIMD the plain step of MODEL in storage ST is held to (0: none)

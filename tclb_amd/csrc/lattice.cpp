@@ -2,6 +2,7 @@
 #include "lattice.h"
 
 #include "pri.h"
+#include "storage.h"
 
 #include <algorithm>
 #include <cstdio>
@@ -278,6 +279,7 @@ StepArgs Lattice::args(int from, int to, bool glob) const {
     a.rpt = rpt_;
     a.flat = 0;
     a.inv_nx = 1.0 / nx;
+    a.inv_ny = 1.0 / ny;
     a.zcols = zcols_;
     a.nzones = nzones_;
     const NodeTypeDef* ib = model->nodetype("IB");
@@ -288,8 +290,7 @@ StepArgs Lattice::args(int from, int to, bool glob) const {
     a.list = d_list_;
     a.list_n = (unsigned)list_n_;
     a.sr = sr_ ? 1 : 0;
-    a.sr_seed = sr_seed_;
-    a.sr_step = 0;
+    a.sr_key = 0;
     model_params(model, settings_, a.par);
     return a;
 }
@@ -397,7 +398,7 @@ int globals_rows_per_block(long long rows, int bpr) {
 
 void Lattice::launch(int from, int to, bool glob, int step) {
     StepArgs a = args(from, to, glob);
-    a.sr_step = (unsigned)step;
+    a.sr_key = sr_key((unsigned)step, sr_seed_);
     if (sorted_ && !glob) {
         a.mode = 1;
         hip_check(launch_step(model->id, storage, false, a, block_, stream_), "step kernel (bulk)");

@@ -17,44 +17,44 @@ namespace tclbx {
 // A value x between the neighbouring representable values lo and hi (|lo| <=
 // |x| < |hi|) is stored as hi with probability (|x| - |lo|)/(|hi| - |lo|),
 // so the stored value's expectation is x: rounding errors do not add up
-// into a drift over many steps.  The random bits come from MurmurHash3's
-// 32-bit finalizer (fmix32) over (node, direction, step, seed), so a run is
-// reproducible.
-__device__ __forceinline__ uint32_t fmix32(uint32_t h) {
+// into a drift over many steps.  gfx950 rounds this way in one instruction
+// (v_cvt_sr_f16_f32, v_cvt_sr_bf16_f32) given 32 random bits r:
+// - binary16: the float's significand aligned to binary16's spacing at |x|
+//   (normal or subnormal), the 13 bits below the kept ones (lower bits
+//   dropped), plus the top 13 bits of r, carried into the kept bits and
+//   truncated; so below binary16's spacing/2^13 (|x| < 2^-37) x becomes 0;
+// - bfloat16: the low 16 bits of x plus the top 16 bits of r, carried and
+//   truncated.
+// The random words of a node's stores come from MurmurHash3's 32-bit
+// finalizer (fmix32) of (node, step, seed), forced odd, and Marsaglia's
+// xorshift32 after it; direction 2j takes word j, direction 2j + 1 word j
+// shifted left by 16, so the two use disjoint bits.  A run is reproducible.
+__host__ __device__ __forceinline__ uint32_t fmix32(uint32_t h) {
     h ^= h >> 16; h *= 0x85ebca6bu;
     h ^= h >> 13; h *= 0xc2b2ae35u;
     h ^= h >> 16;
     return h;
 }
-__device__ __forceinline__ uint32_t sr_bits(uint32_t node, uint32_t q, uint32_t step, uint32_t seed) {
-    return fmix32(node * 0x9e3779b1u ^ fmix32(step * 0x85ebca77u + q * 0xc2b2ae3du + seed));
+// the key of a step's random words (the same for every node; the host
+// computes it for the launch)
+__host__ __device__ __forceinline__ uint32_t sr_key(uint32_t step, uint32_t seed) {
+    return fmix32(step * 0x85ebca77u + seed);
 }
-// float -> half: k = the bits of x's float significand below binary16's
-// spacing at |x| (13 where binary16 is normal, -1 - e for a float exponent
-// e in [-24, -15], where it is subnormal); k random bits added to those
-// bits carry into the kept ones with probability (low bits)/2^k, and the
-// truncated value converts to binary16 exactly.  Below binary16's smallest
-// subnormal (e <= -25) x becomes +-2^-24 with probability |x| 2^24, else
-// +-0, drawn from the top 24 random bits.
+// the first random word of `node`'s stores in the step of `key`
+__device__ __forceinline__ uint32_t sr_word0(uint32_t node, uint32_t key) {
+    return fmix32(node * 0x9e3779b1u ^ key) | 1u;
+}
+__device__ __forceinline__ uint32_t xorshift32(uint32_t x) {
+    x ^= x << 13; x ^= x >> 17; x ^= x << 5;
+    return x;
+}
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bf162_t __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ _Float16 f16_sr(float x, uint32_t r) {
-    const uint32_t u = __builtin_bit_cast(uint32_t, x);
-    const uint32_t a = u & 0x7fffffffu;
-    if (a >= 0x7f800000u) return (_Float16)x;
-    const int e = (int)(a >> 23) - 127;
-    if (e <= -25) {
-        const bool up = (float)(r >> 8) * 5.9604644775390625e-08f < __builtin_bit_cast(float, a) * 16777216.0f;
-        return __builtin_bit_cast(_Float16, (uint16_t)(((u >> 16) & 0x8000u) | (up ? 1u : 0u)));
-    }
-    const uint32_t k = e >= -14 ? 13u : (uint32_t)(-1 - e);
-    const uint32_t mask = (1u << k) - 1u;
-    return (_Float16)__builtin_bit_cast(float, (u + (r & mask)) & ~mask);
+    return __builtin_amdgcn_cvt_sr_f16_f32(half2_t{}, x, r, false)[0];
 }
-// float -> bfloat16: the low 16 bits plus 16 random bits carry into the
-// kept half with probability (low bits)/2^16
 __device__ __forceinline__ __bf16 bf16_sr(float x, uint32_t r) {
-    const uint32_t u = __builtin_bit_cast(uint32_t, x);
-    if ((u & 0x7fffffffu) >= 0x7f800000u) return (__bf16)x;
-    return __builtin_bit_cast(__bf16, (uint16_t)((u + (r & 0xffffu)) >> 16));
+    return __builtin_amdgcn_cvt_sr_bf16_f32(bf162_t{}, x, r, false)[0];
 }
 
 template <int ST> struct Store;

@@ -1676,6 +1676,11 @@ def test_stochastic_rounding_f16_is_unbiased(O, x):
     assert abs(vals.var() - var) <= 0.05 * var
     nearest = float(np.float16(xf))
     assert abs(nearest - xf) > 10 * np.sqrt(var / n)   # the bias stochastic rounding removes
+    # over all 2^13 values of the 13 bits the rounding takes, hi comes out
+    # floor(2^13 (|x| - |lo|)/(|hi| - |lo|)) times (exact where the float's
+    # bits below binary16's spacing are 13, within 2^-13 of it elsewhere)
+    ups = sum(abs(L.orc_f16_to_f32(L.orc_f32_to_f16_sr(xf, t << 19))) == abs(hi) for t in range(8192))
+    assert ups == int(np.floor(8192 * (abs(xf) - abs(lo)) / (abs(hi) - abs(lo))))
 
 
 @pytest.mark.parametrize("x", [1.0e-3 + 3.1e-7, -2.7e-3 - 1.3e-7, 0.15 + 1e-5])
@@ -1694,12 +1699,60 @@ def test_stochastic_rounding_bf16_is_unbiased(O, x):
     var = (abs(hi) - abs(xv)) * (abs(xv) - abs(lo))
     assert abs(vals.mean() - xv) <= 4.5 * np.sqrt(var / n)
     assert abs(vals.var() - var) <= 0.05 * var
+    # over all 2^16 values of the 16 bits the rounding takes, hi comes out
+    # (|x| - |lo|)/(|hi| - |lo|) 2^16 times: the mean is x exactly
+    ups = sum(L.orc_f32_to_bf16_sr(float(xf), t << 16) != (u >> 16) for t in range(65536))
+    assert ups == round(65536 * (abs(xv) - abs(lo)) / (abs(hi) - abs(lo)))
 
 
-def test_stochastic_rounding_bits_are_uniform(O):
-    """The hash of (node, direction, step, seed): uniform in mean and
-    variance, and different for neighbouring nodes, directions and steps."""
+def test_stochastic_rounding_matches_the_gfx950_conversions(O):
+    """The oracle's two stochastic roundings against the vectors gfx950's
+    v_cvt_sr_f16_f32 and v_cvt_sr_bf16_f32 wrote
+    (tests/golden/sr_conversions.npz, made by tests/golden/make_sr_fixture.py):
+    random (x, r) pairs at every float exponent from -45 to 17, structured
+    words (0, ~0, single bits, low and high masks), zeros, binary16's largest
+    values and overflow, its subnormal limits, inf and NaN.  Bit-exact, NaN
+    compared as NaN."""
     L = O.lib()
-    r = np.array([L.orc_sr_bits(k % 997, k // 997, 7 + k % 3, 1) for k in range(60000)], dtype=np.float64) / 2.0 ** 32
-    assert abs(r.mean() - 0.5) < 0.01 and abs(r.var() - 1 / 12) < 0.003
-    assert len({L.orc_sr_bits(5, q, s, 0) for q in range(19) for s in range(50)}) == 19 * 50
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "sr_conversions.npz"))
+    x, r, h, b = d["x"], d["r"], d["f16"], d["bf16"]
+    assert len(x) > 10000
+    xf = x.view(np.float32)
+    nan = (x & 0x7FFFFFFF) > 0x7F800000
+    got_h = np.array([L.orc_f32_to_f16_sr(float(v), int(w)) for v, w in zip(xf, r)], dtype=np.uint16)
+    got_b = np.array([L.orc_f32_to_bf16_sr(float(v), int(w)) for v, w in zip(xf, r)], dtype=np.uint16)
+    assert np.array_equal(got_h[~nan], h[~nan])
+    assert np.array_equal(got_b[~nan], b[~nan])
+    assert np.all((got_h[nan] & 0x7FFF) > 0x7C00) and np.all((h[nan] & 0x7FFF) > 0x7C00)
+    assert np.all((got_b[nan] & 0x7FFF) > 0x7F80) and np.all((b[nan] & 0x7FFF) > 0x7F80)
+    # the cases the fixture holds: subnormal results, flushes to zero below
+    # 2^-37, overflow to inf, and carries into the next binade
+    e = ((x >> 23) & 0xFF).astype(int) - 127
+    assert np.any((e < -14) & (e > -25) & ((h & 0x7FFF) != 0)) and np.any((e < -37) & ((h & 0x7FFF) == 0))
+    assert np.any((h & 0x7FFF) == 0x7C00) and np.any(((h >> 10) & 31) != np.clip(e + 15, 0, 31))
+
+
+def test_stochastic_rounding_bits_are_uniform_and_independent(O):
+    """The random words of (node, direction, step, seed), in the bits the
+    roundings take (binary16 the top 13, bfloat16 the top 16): uniform in
+    mean and variance, and uncorrelated between the directions of a node,
+    between neighbouring nodes and between consecutive steps (|Pearson r|
+    below 4.5/sqrt(n) for every pair); direction 2j + 1 takes the low half
+    of direction 2j's word, so the two use disjoint bits."""
+    L = O.lib()
+    n, Q = 4000, 19
+    bound = 4.5 / np.sqrt(n)
+    for shift in (19, 16):
+        w = {st: np.array([[L.orc_sr_bits(k, q, st, 3) for q in range(Q)] for k in range(n + 1)], dtype=np.uint64)
+             for st in (7, 8)}
+        u = {st: (w[st] >> np.uint64(shift)).astype(np.float64) / 2.0 ** (32 - shift) for st in w}
+        x = u[7][:n]
+        assert np.all(np.abs(x.mean(0) - 0.5) < 4.5 * np.sqrt(1 / 12 / n))
+        assert np.all(np.abs(x.var(0) - 1 / 12) < 0.006)
+        c = np.corrcoef(x.T)
+        assert np.max(np.abs(c - np.eye(Q))) < bound                     # directions of a node
+        for q in range(Q):
+            assert abs(np.corrcoef(u[7][:n, q], u[7][1:, q])[0, 1]) < bound   # neighbouring nodes
+            assert abs(np.corrcoef(u[7][:n, q], u[8][:n, q])[0, 1]) < bound   # consecutive steps
+        for j in range(Q // 2):
+            assert np.all(w[7][:, 2 * j + 1] == (w[7][:, 2 * j] << np.uint64(16)) & np.uint64(0xFFFFFFFF))
