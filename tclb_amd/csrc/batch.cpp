@@ -243,7 +243,7 @@ StepArgs Batch::case_args(const Case& c) const {
     a.nzones = model->zone_max + 1;
     const NodeTypeDef* ib = model->nodetype("IB");
     a.ib_mask = ib ? ib->value : 0u;   // a batch holds no cuts: the branch never runs
-    model_params(model, c.settings, a.par);
+    model_params(model, c.settings, a.par, a.fpar);
     return a;
 }
 

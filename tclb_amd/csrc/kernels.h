@@ -52,6 +52,9 @@ struct StepArgs {
     int sr;
     unsigned sr_key;
     double par[MAX_PAR];
+    // float forms the float collisions take, derived from par on the host
+    // (model_params)
+    float fpar[MAX_PAR];
 };
 
 // One lattice of a batch: its step arguments (fin/fout = snapshots 0/1, rpb

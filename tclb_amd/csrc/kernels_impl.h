@@ -645,6 +645,55 @@ __device__ __forceinline__ void d3q19_from_moments(R* f, const R* r) {
     f[18] = G0 - t5 - t7 + r[14] - r[17] + r[18];
 }
 
+// f = sum_k r_k M[k][i] in float: the groups of d3q19_from_moments with
+// shared partial sums and FMAs for the integer coefficients.
+__device__ __forceinline__ void d3q19_from_moments_f(float* f, const float* r) {
+    const float bf = __builtin_fmaf(-11.f, r[1], __builtin_fmaf(-4.f, r[2], r[0]));
+    const float be = __builtin_fmaf(8.f, r[1], r[0] + r[2]);
+    const float d1x = __builtin_fmaf(-4.f, r[4], r[3]), d1y = __builtin_fmaf(-4.f, r[6], r[5]),
+                d1z = __builtin_fmaf(-4.f, r[8], r[7]);
+    const float t3 = r[3] + r[4], t5 = r[5] + r[6], t7 = r[7] + r[8];
+    const float p9 = __builtin_fmaf(2.f, r[9], -4.f * r[10]);
+    const float q9 = __builtin_fmaf(2.f, r[10], -r[9]);
+    const float e9 = r[9] + r[10];
+    const float w11 = __builtin_fmaf(-2.f, r[12], r[11]);
+    const float e11 = r[11] + r[12];
+    f[0] = __builtin_fmaf(-30.f, r[1], __builtin_fmaf(12.f, r[2], r[0]));
+    const float bp = bf + p9, bq = bf + q9, bqp = bq + w11, bqm = bq - w11;
+    f[1] = bp + d1x;
+    f[2] = bp - d1x;
+    f[3] = bqp + d1y;
+    f[4] = bqp - d1y;
+    f[5] = bqm + d1z;
+    f[6] = bqm - d1z;
+    const float be9 = be + e9;
+    const float E0 = be9 + e11, F0 = be9 - e11, G0 = __builtin_fmaf(-2.f, e9, be);
+    {   // 7..10: E0 +-t3 +-t5 with r13, r16, r17
+        const float a = t3 + t5, b = t5 - t3, u = r[13] + r[16], v = r[16] - r[13];
+        const float P = E0 - r[17], Qn = E0 + r[17];
+        f[7] = (P + u) + a;
+        f[8] = (P - u) + b;
+        f[9] = (Qn + v) - b;
+        f[10] = (Qn - v) - a;
+    }
+    {   // 11..14: F0 +-t3 +-t7 with r15, r16, r18
+        const float a = t3 + t7, b = t7 - t3, x = r[15] - r[16], w = r[15] + r[16];
+        const float P = F0 + r[18], Qn = F0 - r[18];
+        f[11] = (P + x) + a;
+        f[12] = (P - x) + b;
+        f[13] = (Qn - w) - b;
+        f[14] = (Qn + w) - a;
+    }
+    {   // 15..18: G0 +-t5 +-t7 with r14, r17, r18
+        const float a = t5 + t7, b = t7 - t5, u = r[14] + r[17], v = r[17] - r[14];
+        const float P = G0 - r[18], Qn = G0 + r[18];
+        f[15] = (P + u) + a;
+        f[16] = (P + b) - u;
+        f[17] = (Qn + v) - b;
+        f[18] = (Qn - v) - a;
+    }
+}
+
 // SetEquilibrum, Dynamics.c.Rt:212-217: f = Req %*% diag(1/Mw) %*% t(M)
 template <class R>
 __device__ __forceinline__ void d3q19_feq(R* f, R rho, R Jx, R Jy, R Jz) {
@@ -702,88 +751,105 @@ __device__ __forceinline__ void d3q19_globals(double* g, unsigned nt, R rs, R jx
 // CollisionMRT, Dynamics.c.Rt:223-290.  Rates by moment group (0-based):
 // e {1}, eps {2}, q {4,6,8}, nu {9,11,13,14,15}, pi {10,12}, m {16,17,18};
 // the reference uses gamma1 = 1-omega for e/eps/nu/pi and
-// gamma2 = 1-8(2-omega)/(8-omega) for q/m; a positive MRT_S* setting
-// replaces the rate of its group by 1 - S.
+// The float collision of d3q19's MRT (CollisionMRT, Dynamics.c.Rt:196-262):
+// the moments R = f M, relaxation of the non-conserved ones toward Req,
+// the body force added to J after relaxation, Req of the forced J added
+// back, R /= Mw (Mw = colSums(M^2)) and f = R M^T, as d3q19_mrt_dense
+// computes them in FP64, here with fewer float operations: the moments and
+// the inverse transform from shared pair sums and differences, the factor
+// 1/Mw folded into the relaxation factors (computed on the host, `fp`) and
+// into the equilibrium constants, and explicit FMAs (the float formats are
+// held to a tolerance of a few float ulps, not to identical roundings).
+// fp = [ge/2394, geps/252, gq/40, gamma1/36, gpi/72, gamma1/12, gpi/24,
+// gamma1/4, gm/8, ForceX, ForceY, ForceZ]: gamma1 = 1 - omega, gamma2 =
+// 1-8(2-omega)/(8-omega), ge, geps, gq, gpi, gm the group factors (a
+// positive MRT_S* setting gives its group 1 - S).
 // DEV: f holds the deviations f - w of a shifted storage format (the rest
 // populations w have moments rho = 1, m1 = -11, m2 = 3 and no others), and
 // the collision is evaluated on them: the linear terms of Req take rho - 1,
 // J^2/rho and rho F take the full rho, and the result is again f - w.  The
 // large constant parts cancel exactly instead of being rounded in float.
 template <class R, bool GLOB, bool LID, bool DEV = false>
-__device__ __forceinline__ void d3q19_mrt(R* f, unsigned nt, const double* par, const double* zr,
-                                          double* g) {
-    // relaxation factors of the moment groups, computed on the host
-    const R ge = (R)par[6], geps = (R)par[7], gq = (R)par[8], gpi = (R)par[9], gm = (R)par[10];
-    const R gamma1 = (R)par[11];
+__device__ __forceinline__ void d3q19_mrt(R* f, unsigned nt, const float* fp, const double* zr, double* g) {
+    static_assert(std::is_same<R, float>::value, "the FP64 collision is d3q19_mrt_dense");
+    (void)nt;
+    // pair sums and differences of opposite-in-one-axis directions
+    const float s12 = f[1] + f[2], d12 = f[1] - f[2];
+    const float s34 = f[3] + f[4], d34 = f[3] - f[4];
+    const float s56 = f[5] + f[6], d56 = f[5] - f[6];
+    const float s78 = f[7] + f[8], d78 = f[7] - f[8];
+    const float s910 = f[9] + f[10], d910 = f[9] - f[10];
+    const float s1112 = f[11] + f[12], d1112 = f[11] - f[12];
+    const float s1314 = f[13] + f[14], d1314 = f[13] - f[14];
+    const float s1516 = f[15] + f[16], d1516 = f[15] - f[16];
+    const float s1718 = f[17] + f[18], d1718 = f[17] - f[18];
+    const float E = s78 + s910, F = s1112 + s1314, G = s1516 + s1718;
+    const float A = s12 + (s34 + s56), B = E + (F + G);
+    const float Px = d78 + d910, Qx = d1112 + d1314;     // bx = Px + Qx, m16 = Px - Qx
+    const float Ry = s78 - s910, Ty = d1516 + d1718;     // by = Ry + Ty, m17 = Ty - Ry
+    const float Sz = s1112 - s1314, Uz = s1516 - s1718;  // bz = Sz + Uz, m18 = Sz - Uz
+    const float bx = Px + Qx, by = Ry + Ty, bz = Sz + Uz;
+    const float m0 = f[0] + (A + B);
+    const float m1 = __builtin_fmaf(-30.f, f[0], __builtin_fmaf(-11.f, A, 8.f * B));
+    const float m2 = __builtin_fmaf(12.f, f[0], __builtin_fmaf(-4.f, A, B));
+    const float Jx = d12 + bx, Jy = d34 + by, Jz = d56 + bz;
+    const float m4 = __builtin_fmaf(-4.f, d12, bx), m6 = __builtin_fmaf(-4.f, d34, by),
+                m8 = __builtin_fmaf(-4.f, d56, bz);
+    const float EF = E + F, c23 = s34 + s56, c2m3 = s34 - s56, EmF = E - F;
+    const float m9 = __builtin_fmaf(2.f, s12 - G, EF - c23);
+    const float m10 = __builtin_fmaf(-4.f, s12, __builtin_fmaf(2.f, c23 - G, EF));
+    const float m11 = c2m3 + EmF;
+    const float m12 = __builtin_fmaf(-2.f, c2m3, EmF);
+    const float m13 = d78 - d910, m14 = d1516 - d1718, m15 = d1112 - d1314;
+    const float m16 = Px - Qx, m17 = Ty - Ry, m18 = Sz - Uz;
 
-    // moments R = f M (Dynamics.R:1-21), by symmetry groups
-    R A = f[1] + f[2] + f[3] + f[4] + f[5] + f[6];
-    R c1 = f[1] + f[2], c2 = f[3] + f[4], c3 = f[5] + f[6];
-    R E = f[7] + f[8] + f[9] + f[10];
-    R F = f[11] + f[12] + f[13] + f[14];
-    R Gs = f[15] + f[16] + f[17] + f[18];
-    R B = E + F + Gs;
-    R m[19];
-    m[0] = f[0] + A + B;
-    m[1] = R(-30.)*f[0] - R(11.)*A + R(8.)*B;
-    m[2] = R(12.)*f[0] - R(4.)*A + B;
-    R ax = f[1] - f[2];
-    R bx = (f[7] - f[8]) + (f[9] - f[10]) + (f[11] - f[12]) + (f[13] - f[14]);
-    R ay = f[3] - f[4];
-    R by = (f[7] + f[8] - f[9] - f[10]) + (f[15] - f[16] + f[17] - f[18]);
-    R az = f[5] - f[6];
-    R bz = (f[11] + f[12] - f[13] - f[14]) + (f[15] + f[16] - f[17] - f[18]);
-    m[3] = ax + bx; m[4] = R(-4.)*ax + bx;
-    m[5] = ay + by; m[6] = R(-4.)*ay + by;
-    m[7] = az + bz; m[8] = R(-4.)*az + bz;
-    R EF = E + F;
-    m[9] = R(2.)*c1 - c2 - c3 + EF - R(2.)*Gs;
-    m[10] = R(-4.)*c1 + R(2.)*(c2 + c3) + EF - R(2.)*Gs;
-    m[11] = c2 - c3 + E - F;
-    m[12] = R(-2.)*c2 + R(2.)*c3 + E - F;
-    m[13] = f[7] - f[8] - f[9] + f[10];
-    m[14] = f[15] - f[16] - f[17] + f[18];
-    m[15] = f[11] - f[12] - f[13] + f[14];
-    m[16] = f[7] - f[8] + f[9] - f[10] - f[11] + f[12] - f[13] + f[14];
-    m[17] = -f[7] - f[8] + f[9] + f[10] + f[15] - f[16] + f[17] - f[18];
-    m[18] = f[11] + f[12] - f[13] - f[14] - f[15] - f[16] + f[17] + f[18];
+    // Req of the unforced J (d3q19_req: rho-linear terms on m0, which is
+    // rho - 1 for deviations)
+    const float rho = DEV ? 1.f + m0 : m0;
+    const float ir = __builtin_amdgcn_rcpf(rho);
+    float xx = Jx * Jx * ir, yy = Jy * Jy * ir, zz = Jz * Jz * ir;
+    float j2 = xx + yy + zz, yz = yy + zz;
+    const float n1 = m1 - __builtin_fmaf(19.f, j2, -11.f * m0);
+    const float n2 = m2 - __builtin_fmaf(-2.f, j2, 3.f * m0);
+    const float n4 = __builtin_fmaf(2.f / 3.f, Jx, m4), n6 = __builtin_fmaf(2.f / 3.f, Jy, m6),
+                n8 = __builtin_fmaf(2.f / 3.f, Jz, m8);
+    const float q9 = __builtin_fmaf(2.f, xx, -yz);
+    const float n9 = m9 - q9, n10 = m10 + q9;
+    const float n11 = m11 - (yy - zz), n12 = m12 + (yy - zz);
+    const float n13 = m13 - Jx * Jy * ir, n14 = m14 - Jy * Jz * ir, n15 = m15 - Jx * Jz * ir;
 
-    // DEV: Req - Req(1, 0), i.e. rho - 1 in the rho-linear moments
-    const R rho = DEV ? R(1) + m[0] : m[0];
-    R q[19];
-    d3q19_req_fast(q, rho, m[3], m[5], m[7], m[0]);
-    m[1] = ge * (m[1] - q[1]);
-    m[2] = geps * (m[2] - q[2]);
-    m[4] = gq * (m[4] - q[4]);
-    m[6] = gq * (m[6] - q[6]);
-    m[8] = gq * (m[8] - q[8]);
-    m[9] = gamma1 * (m[9] - q[9]);
-    m[10] = gpi * (m[10] - q[10]);
-    m[11] = gamma1 * (m[11] - q[11]);
-    m[12] = gpi * (m[12] - q[12]);
-    m[13] = gamma1 * (m[13] - q[13]);
-    m[14] = gamma1 * (m[14] - q[14]);
-    m[15] = gamma1 * (m[15] - q[15]);
-    m[16] = gm * m[16];
-    m[17] = gm * m[17];
-    m[18] = gm * m[18];
-    // forcing after relaxation (Dynamics.c.Rt:256-258)
-    m[3] += rho * (R)par[3];
-    m[5] += rho * (R)par[4];
-    m[7] += rho * (R)par[5];
-    d3q19_req_fast(q, rho, m[3], m[5], m[7], m[0]);
-    m[1] += q[1]; m[2] += q[2]; m[4] += q[4]; m[6] += q[6]; m[8] += q[8];
-#pragma unroll
-    for (int k = 9; k < 16; k++) m[k] += q[k];
-    constexpr double iv[19] = {1./19, 1./2394, 1./252, 1./10, 1./40, 1./10, 1./40, 1./10, 1./40,
-                               1./36, 1./72, 1./12, 1./24, 1./4, 1./4, 1./4, 1./8, 1./8, 1./8};
-#pragma unroll
-    for (int k = 0; k < 19; k++) m[k] = m[k] * R(iv[k]);
-    d3q19_from_moments(f, m);
+    // forcing after relaxation (Dynamics.c.Rt:256-258), then Req of the
+    // forced J, everything scaled by 1/Mw
+    const float Fx = __builtin_fmaf(rho, fp[9], Jx), Fy = __builtin_fmaf(rho, fp[10], Jy),
+                Fz = __builtin_fmaf(rho, fp[11], Jz);
+    xx = Fx * Fx * ir; yy = Fy * Fy * ir; zz = Fz * Fz * ir;
+    j2 = xx + yy + zz; yz = yy + zz;
+    const float ir4 = ir * 0.25f, yzd = yy - zz;
+    float M[19];
+    M[0] = m0 * (1.f / 19.f);
+    M[1] = __builtin_fmaf(fp[0], n1, __builtin_fmaf(19.f / 2394.f, j2, (-11.f / 2394.f) * m0));
+    M[2] = __builtin_fmaf(fp[1], n2, __builtin_fmaf(-2.f / 252.f, j2, (3.f / 252.f) * m0));
+    M[3] = Fx * 0.1f;
+    M[5] = Fy * 0.1f;
+    M[7] = Fz * 0.1f;
+    M[4] = __builtin_fmaf(fp[2], n4, (-1.f / 60.f) * Fx);
+    M[6] = __builtin_fmaf(fp[2], n6, (-1.f / 60.f) * Fy);
+    M[8] = __builtin_fmaf(fp[2], n8, (-1.f / 60.f) * Fz);
+    M[9] = __builtin_fmaf(fp[3], n9, __builtin_fmaf(1.f / 18.f, xx, (-1.f / 36.f) * yz));
+    M[10] = __builtin_fmaf(fp[4], n10, __builtin_fmaf(-1.f / 36.f, xx, (1.f / 72.f) * yz));
+    M[11] = __builtin_fmaf(fp[5], n11, (1.f / 12.f) * yzd);
+    M[12] = __builtin_fmaf(fp[6], n12, (-1.f / 24.f) * yzd);
+    M[13] = __builtin_fmaf(fp[7], n13, Fx * Fy * ir4);
+    M[14] = __builtin_fmaf(fp[7], n14, Fy * Fz * ir4);
+    M[15] = __builtin_fmaf(fp[7], n15, Fx * Fz * ir4);
+    M[16] = fp[8] * m16;
+    M[17] = fp[8] * m17;
+    M[18] = fp[8] * m18;
+    d3q19_from_moments_f(f, M);
 
     // Dynamics.c.Rt:266-289 evaluates the Globals after "R /= Mw", i.e. on
     // rho/19 and J/10 -- kept as the reference computes them
-    if (GLOB) d3q19_globals<GLOB, LID>(g, nt, DEV ? rho * R(iv[0]) : m[0], m[3], m[5], m[7], zr);
+    if (GLOB) d3q19_globals<GLOB, LID>(g, nt, DEV ? rho * (1.f / 19.f) : M[0], M[3], M[5], M[7], zr);
 }
 
 // The d3q19 moment matrix M[k][i] (moment k of direction i), the listed
@@ -963,7 +1029,7 @@ __device__ __forceinline__ void d3q19_moving_wall(R* f, const double* par, const
 template <class R, bool GLOB, bool LID, bool DEV = false>
 __device__ __forceinline__ void d3q19_node(R* f, unsigned nt, const double* par, const double* zr,
                                            double* g, const uint16_t* flags, int x, int y, int z, int nx, int ny,
-                                           int nz) {
+                                           int nz, const float* fp = nullptr) {
     const R InletVelocity = (R)par[1], InletDensity = (R)par[2];
     switch (nt & 0x1Cu) {
     case 0x14: {  // WPressureL
@@ -1016,7 +1082,7 @@ __device__ __forceinline__ void d3q19_node(R* f, unsigned nt, const double* par,
     if constexpr (!DEV) {
         if ((nt & 0x20u) == 0x20u) {
             if constexpr (sizeof(R) == 8) d3q19_mrt_dense<R, GLOB, LID>(f, nt, par, zr, g);
-            else d3q19_mrt<R, GLOB, LID>(f, nt, par, zr, g);
+            else d3q19_mrt<R, GLOB, LID>(f, nt, fp, zr, g);
         }
     }
 }
@@ -1026,7 +1092,7 @@ __device__ __forceinline__ void d3q19_node(R* f, unsigned nt, const double* par,
 template <class R, bool GLOB, bool LID>
 __device__ __forceinline__ void d3q19_node_dev(R* d, unsigned nt, const double* par, const double* zr,
                                                double* g, const uint16_t* flags, int x, int y, int z, int nx,
-                                               int ny, int nz) {
+                                               int ny, int nz, const float* fp) {
     using L = Lat<MODEL_D3Q19>;
     if ((nt & 0x1Cu) != 0u) {
 #pragma unroll
@@ -1035,7 +1101,7 @@ __device__ __forceinline__ void d3q19_node_dev(R* d, unsigned nt, const double* 
 #pragma unroll
         for (int i = 0; i < 19; i++) d[i] -= R(L::w(i));
     }
-    if ((nt & 0x20u) == 0x20u) d3q19_mrt<R, GLOB, LID, true>(d, nt, par, zr, g);
+    if ((nt & 0x20u) == 0x20u) d3q19_mrt<R, GLOB, LID, true>(d, nt, fp, zr, g);
 }
 
 // --------------------------------------------------------------- d3q27 ---
@@ -1769,9 +1835,11 @@ __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, 
 
     if (active) {
         if constexpr (DEV) {
-            d3q19_node_dev<R, GLOB, MODEL == MODEL_D3Q19_LID>(f, nt, a.par, zr, g, a.flags, xc, y, z, nx, ny, nz);
+            d3q19_node_dev<R, GLOB, MODEL == MODEL_D3Q19_LID>(f, nt, a.par, zr, g, a.flags, xc, y, z, nx, ny, nz,
+                                                              a.fpar);
         } else if constexpr (is_d3q19<MODEL>) {
-            d3q19_node<R, GLOB, MODEL == MODEL_D3Q19_LID>(f, nt, a.par, zr, g, a.flags, xc, y, z, nx, ny, nz);
+            d3q19_node<R, GLOB, MODEL == MODEL_D3Q19_LID>(f, nt, a.par, zr, g, a.flags, xc, y, z, nx, ny, nz,
+                                                          a.fpar);
         } else if constexpr (is_d2q9m<MODEL>) {
             d2q9m_node<R, GLOB, MODEL == MODEL_D2Q9_DRAG, is_d2q9_bc<MODEL>>(
                 f, nt, a.par, zr, g, a.extra ? a.extra + n : nullptr, (size_t)nx * ny * nz);

@@ -291,7 +291,7 @@ StepArgs Lattice::args(int from, int to, bool glob) const {
     a.list_n = (unsigned)list_n_;
     a.sr = sr_ ? 1 : 0;
     a.sr_key = 0;
-    model_params(model, settings_, a.par);
+    model_params(model, settings_, a.par, a.fpar);
     return a;
 }
 
@@ -343,9 +343,10 @@ void Lattice::ensure_list() {
     params_version_++;  // a captured step loop holds the old list
 }
 
-void model_params(const ModelDef* model, const std::vector<double>& settings, double* par) {
+void model_params(const ModelDef* model, const std::vector<double>& settings, double* par, float* fpar) {
     auto S = [&](const char* n) { int i = model->setting_id(n); return i >= 0 ? settings[i] : 0.0; };
     for (int i = 0; i < MAX_PAR; i++) par[i] = 0.0;
+    for (int i = 0; i < MAX_PAR; i++) fpar[i] = 0.0f;
     if (model->id == MODEL_D3Q19 || model->id == MODEL_D3Q19_LID) {
         const char* names[] = {"omega", "InletVelocity", "InletDensity", "ForceX", "ForceY", "ForceZ"};
         for (int i = 0; i < 6; i++) par[i] = S(names[i]);
@@ -366,6 +367,12 @@ void model_params(const ModelDef* model, const std::vector<double>& settings, do
         par[12] = S("MovingWallVelocityX");
         par[13] = S("MovingWallVelocityY");
         par[14] = S("MovingWallVelocityZ");
+        // the float collision's factors, each divided by its moment's
+        // colSums(M^2) (2394, 252, 40, 36, 72, 12, 24, 4, 8), and the force
+        const double scaled[9] = {par[6] / 2394, par[7] / 252, par[8] / 40, gamma1 / 36, par[9] / 72,
+                                  gamma1 / 12, par[9] / 24, gamma1 / 4, par[10] / 8};
+        for (int i = 0; i < 9; i++) fpar[i] = (float)scaled[i];
+        for (int i = 0; i < 3; i++) fpar[9 + i] = (float)par[3 + i];
     } else if (model->name.compare(0, 14, "d3q27_cumulant") == 0) {
         // [nu, nubuffer, GalileanCorrection, ForceX, ForceY, ForceZ, Omega, Smag]
         const char* names[] = {"nu", "nubuffer", "GalileanCorrection", "ForceX", "ForceY", "ForceZ", "Omega", "Smag"};

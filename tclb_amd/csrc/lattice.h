@@ -170,7 +170,7 @@ void check_lattice_size(const ModelDef* m, int nx, int ny, int nz, int storage);
 double* quantity_scratch();
 size_t quantity_scratch_doubles();
 // kernel parameter vector (StepArgs::par) of a model from its settings
-void model_params(const ModelDef* model, const std::vector<double>& settings, double* par);
+void model_params(const ModelDef* model, const std::vector<double>& settings, double* par, float* fpar);
 // rows walked by one block of a Globals step for a lattice of `rows` rows
 int globals_rows_per_block(long long rows, int bpr);
 
