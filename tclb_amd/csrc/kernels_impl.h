@@ -182,6 +182,13 @@ template <int MODEL> constexpr bool is_cum_avg = MODEL == MODEL_D3Q27_CUMULANT_A
                                                 MODEL == MODEL_D3Q27_CUMULANT_AVG_IB_SMAG;
 
 
+// A pointer into HBM as a global (address space 1) pointer.  The lattices
+// of a batch come from a case table in memory, so the compiler sees generic
+// pointers there and would use flat loads, which count against the scalar
+// memory counter as well: each wait for a case field read by a scalar load
+// would then drain the density loads in flight, one by one.
+template <class T> using gptr = __attribute__((address_space(1))) T*;
+
 // ---------------------------------------------------------------- d2q9 ---
 // models/flow/d2q9_SRT/Dynamics.c, same operations in the same order.
 template <class R>
@@ -450,6 +457,68 @@ __device__ __forceinline__ void d2q9m_symmetry(R* f) {
     else { f[2] = f[4]; f[5] = f[8]; f[6] = f[7]; }
 }
 
+// The float MRT collision (CollisionMRT, Dynamics.c.Rt:196-268) as an
+// increment, f += B^-1 (m' - m), with m = f B the moments and m' the relaxed
+// and forced ones.  The inverse transform's float constants 1/3 and 1/6 are
+// both rounded up, so the product form f = B^-1 m' scales every population
+// by about 1 + 6e-8 and creates that much mass each step (a drift that
+// dominated the float formats' error on open channels); the increment has
+// no mass and the force as its momentum, so those constants only scale the
+// non-equilibrium change.  DEV: f holds the deviations f - w, whose moments
+// are those of f except m0, which is rho - 1 (the rest populations w have
+// only the zeroth moment).  Zone columns and Globals as d2q9m_node.
+template <bool GLOB, bool DRAG, bool BCOPT, bool DEV>
+__device__ __forceinline__ void d2q9m_collide_f(float* f, unsigned nt, const double* par, const double* zr,
+                                                double* g, double* bc, size_t NN) {
+    constexpr int OBJ = DRAG ? 5 : 3;
+    float m[9];
+    d2q9m_moments_fact(f, m);
+    const float rho = DEV ? 1.f + m[0] : m[0];
+    const float Jx = m[1], Jy = m[2];
+    if (GLOB) {
+        const unsigned obj = nt & 0x60u;
+        if (obj == 0x40u || obj == 0x20u) {  // Outlet / Inlet
+            const float Usq = (Jx * Jx + Jy * Jy) / (rho * rho);
+            const float flux = Jx / rho / rho;
+            const float loss = (Jx / rho) / rho * ((rho - 1.f) / 3.f + Usq / rho / 2.f);
+            const int k = obj == 0x40u ? 1 : 2;
+            g[k] += (double)flux; g[OBJ] += zr[3 + k];
+            g[0] += (double)(obj == 0x40u ? -loss : loss); g[OBJ] += zr[3];
+        }
+    }
+    // 3 a b / rho with one approximate reciprocal (second-order terms only)
+    const float ir3 = 3.f * __builtin_amdgcn_rcpf(rho);
+    const float S2 = (float)par[0], S3 = (float)par[1], S4 = (float)par[2];
+    const float n3 = m[3] - Jx * Jx * ir3, n4 = m[4] - Jy * Jy * ir3, n5 = m[5] - Jx * Jy * ir3;
+    float Gx = (float)par[3], Gy = (float)par[4];
+    if constexpr (BCOPT) {
+        // only the nodes with no BOUNDARY type take the force, G + BC
+        // (Dynamics.c.Rt:254-258)
+        if ((nt & 0xFu) == 0u) {
+            Gx += bc ? (float)bc[0] : 0.f;
+            Gy += bc ? (float)bc[NN] : 0.f;
+        } else {
+            Gx = 0.f;
+            Gy = 0.f;
+        }
+    }
+    const float Kx = __builtin_fmaf(Gx, rho, Jx), Ky = __builtin_fmaf(Gy, rho, Jy);
+    float dm[9];
+    dm[0] = 0.f;
+    dm[1] = Kx - Jx;
+    dm[2] = Ky - Jy;
+    dm[3] = __builtin_fmaf(S2, n3, Kx * Kx * ir3) - m[3];
+    dm[4] = __builtin_fmaf(S2, n4, Ky * Ky * ir3) - m[4];
+    dm[5] = __builtin_fmaf(S2, n5, Kx * Ky * ir3) - m[5];
+    dm[6] = __builtin_fmaf(S3, m[6], -m[6]);
+    dm[7] = __builtin_fmaf(S3, m[7], -m[7]);
+    dm[8] = __builtin_fmaf(S4, m[8], -m[8]);
+    float df[9];
+    d2q9m_from_moments_fact(df, dm);
+#pragma unroll
+    for (int i = 0; i < 9; i++) f[i] += df[i];
+}
+
 // Run(), Dynamics.c.Rt:104-138 and CollisionMRT, :196-268.  Zone columns:
 // [VelocityX, VelocityY, Pressure, PressureLossInObj, OutletFluxInObj,
 // InletFluxInObj, (d2q9_drag) DragInObj, LiftInObj]; Globals
@@ -457,7 +526,7 @@ __device__ __forceinline__ void d2q9m_symmetry(R* f) {
 // DRAG adds the momentum exchange of Solid nodes, 2 sum_i f_i e_i of the
 // incoming populations (as d2q9_SRT_drag).  BCOPT: the bc builds, with the
 // node's BC[0] at bc[0] and BC[1] at bc[NN] (null: BC = 0, nothing kept).
-template <class R, bool GLOB, bool DRAG, bool BCOPT = false>
+template <class R, bool GLOB, bool DRAG, bool BCOPT = false, bool COLLIDE = true>
 __device__ __forceinline__ void d2q9m_node(R* f, unsigned nt, const double* par, const double* zr, double* g,
                                            double* bc = nullptr, size_t NN = 0) {
     constexpr int OBJ = DRAG ? 5 : 3;
@@ -502,11 +571,13 @@ __device__ __forceinline__ void d2q9m_node(R* f, unsigned nt, const double* par,
     case 8: d2q9m_symmetry<R, false>(f); break;                 // SSymmetry
     default: break;
     }
-    if ((nt & 0x10u) == 0) return;  // MRT
-    R m[9];
+    if (!COLLIDE || (nt & 0x10u) == 0) return;  // MRT
     if constexpr (!std::is_same<R, double>::value) {
-        d2q9m_moments_fact(f, m);
-    } else {
+        d2q9m_collide_f<GLOB, DRAG, BCOPT, false>(f, nt, par, zr, g, bc, NN);
+        return;
+    }
+    R m[9];
+    {
 #pragma unroll
         for (int k = 0; k < 9; k++) {
             R acc = 0;
@@ -529,15 +600,9 @@ __device__ __forceinline__ void d2q9m_node(R* f, unsigned nt, const double* par,
             g[0] += (double)(obj == 0x40u ? -loss : loss); g[OBJ] += zr[3];
         }
     }
-    // 3 a b / rho: an IEEE division in FP64 (the reference's roundings), one
-    // approximate reciprocal of rho in float (the float formats are held to a
-    // tolerance of a few float ulps, not to identical roundings)
-    R ir = R(0);
-    if constexpr (!std::is_same<R, double>::value) ir = __builtin_amdgcn_rcpf(rho);
-    const auto q3 = [&](R u, R v) {
-        if constexpr (std::is_same<R, double>::value) return R(3.) * u * v / rho;
-        else return R(3.) * u * v * ir;
-    };
+    // FP64 from here on (the float formats take d2q9m_collide_f): the
+    // reference's operations, IEEE divisions included
+    const auto q3 = [&](R u, R v) { return R(3.) * u * v / rho; };
     const R S2 = (R)par[0], S3 = (R)par[1], S4 = (R)par[2];
     m[3] = S2 * (m[3] - q3(Jx, Jx));
     m[4] = S2 * (m[4] - q3(Jy, Jy));
@@ -562,6 +627,23 @@ __device__ __forceinline__ void d2q9m_node(R* f, unsigned nt, const double* par,
     m[4] = m[4] + q3(Jy, Jy);
     m[5] = m[5] + q3(Jx, Jy);
     d2q9m_from_moments(f, m);
+}
+
+// d2q9 MRT node of a shifted storage format, on the deviations d = f - w:
+// boundary nodes add w back for their boundary condition, the collision
+// runs on d.
+template <bool GLOB, bool DRAG, bool BCOPT>
+__device__ __forceinline__ void d2q9m_node_dev(float* d, unsigned nt, const double* par, const double* zr,
+                                               double* g, double* bc, size_t NN) {
+    using L = Lat<MODEL_D2Q9_SRT>;
+    if ((nt & 0xFu) != 0u) {
+#pragma unroll
+        for (int i = 0; i < 9; i++) d[i] += (float)L::w(i);
+        d2q9m_node<float, GLOB, DRAG, BCOPT, false>(d, nt, par, zr, g, bc, NN);
+#pragma unroll
+        for (int i = 0; i < 9; i++) d[i] -= (float)L::w(i);
+    }
+    if ((nt & 0x10u) != 0u) d2q9m_collide_f<GLOB, DRAG, BCOPT, true>(d, nt, par, zr, g, bc, NN);
 }
 
 // --------------------------------------------------------------- d3q19 ---
@@ -1796,7 +1878,7 @@ __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, 
     const int zm = z == 0 ? nz - 1 : z - 1, zp = z == nz - 1 ? 0 : z + 1;
 
     const unsigned n = (unsigned)xc + (unsigned)nx * ((unsigned)y + (unsigned)ny * (unsigned)z);
-    const unsigned nt = a.flags[n];
+    const unsigned nt = *(gptr<const uint16_t>)(a.flags + n);
     int zone = (int)(nt >> L::ZSHIFT);
     if (zone >= a.nzones) zone = 0;
     const double* zr = a.zone + (size_t)zone * a.zcols;
@@ -1809,8 +1891,9 @@ __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, 
     // check_lattice_size refuses larger ones)
     const char* fin = (const char*)a.fin;
     const size_t pbytes = (size_t)a.pitch * sizeof(T);
-    // d3q19 in a shifted format collides on the stored deviations f - w
-    constexpr bool DEV = is_d3q19<MODEL> && S::shifted;
+    // d3q19 and the d2q9 MRT models in a shifted format collide on the
+    // stored deviations f - w
+    constexpr bool DEV = (is_d3q19<MODEL> || is_d2q9m<MODEL>) && S::shifted;
     R f[Q];
     if (DISPATCH != 1 || active) {
 #pragma unroll
@@ -1820,7 +1903,7 @@ __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, 
             const int sy = L::ey(i) > 0 ? ym : (L::ey(i) < 0 ? yp : y);
             const int sz = L::ez(i) > 0 ? zm : (L::ez(i) < 0 ? zp : z);
             const unsigned src = (unsigned)sx + (unsigned)nx * ((unsigned)sy + (unsigned)ny * (unsigned)sz);
-            const T v = *(const T*)(fin + (size_t)i * pbytes + src * (unsigned)sizeof(T));
+            const T v = *(gptr<const T>)(fin + (size_t)i * pbytes + src * (unsigned)sizeof(T));
             f[i] = DEV ? S::raw(v) : S::get(v, (R)L::w(i));
         }
     }
@@ -1829,17 +1912,21 @@ __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, 
             const int sxn = ox > 0 ? xp : (ox < 0 ? xm : xc);
             const int syn = oy > 0 ? yp : (oy < 0 ? ym : y);
             const unsigned src = (unsigned)sxn + (unsigned)nx * (unsigned)syn;
-            return S::get(*(const T*)(fin + (size_t)fi * pbytes + src * (unsigned)sizeof(T)), (R)L::w(fi));
+            const T v = *(gptr<const T>)(fin + (size_t)fi * pbytes + src * (unsigned)sizeof(T));
+            return DEV ? S::raw(v) : S::get(v, (R)L::w(fi));   // a mirror image has the same w
         });
     }
 
     if (active) {
-        if constexpr (DEV) {
+        if constexpr (DEV && is_d3q19<MODEL>) {
             d3q19_node_dev<R, GLOB, MODEL == MODEL_D3Q19_LID>(f, nt, a.par, zr, g, a.flags, xc, y, z, nx, ny, nz,
                                                               a.fpar);
         } else if constexpr (is_d3q19<MODEL>) {
             d3q19_node<R, GLOB, MODEL == MODEL_D3Q19_LID>(f, nt, a.par, zr, g, a.flags, xc, y, z, nx, ny, nz,
                                                           a.fpar);
+        } else if constexpr (is_d2q9m<MODEL> && DEV) {
+            d2q9m_node_dev<GLOB, MODEL == MODEL_D2Q9_DRAG, is_d2q9_bc<MODEL>>(
+                f, nt, a.par, zr, g, a.extra ? a.extra + n : nullptr, (size_t)nx * ny * nz);
         } else if constexpr (is_d2q9m<MODEL>) {
             d2q9m_node<R, GLOB, MODEL == MODEL_D2Q9_DRAG, is_d2q9_bc<MODEL>>(
                 f, nt, a.par, zr, g, a.extra ? a.extra + n : nullptr, (size_t)nx * ny * nz);
@@ -1850,7 +1937,7 @@ __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, 
             // the branch, whose mask is 0 in the builds without IB
             if ((nt & a.ib_mask) && a.cuts != nullptr)
                 cum_ib<R, GLOB>(f, nt, a.cuts + n, (size_t)nx * ny * nz, [&](int i) {
-                    return S::get(*(const T*)(fin + (size_t)i * pbytes + n * (unsigned)sizeof(T)), (R)L::w(i));
+                    return S::get(*(gptr<const T>)(fin + (size_t)i * pbytes + n * (unsigned)sizeof(T)), (R)L::w(i));
                 }, zr, g);
             if constexpr (is_cum_avg<MODEL>)
                 cum_node<R, GLOB, is_cum_smag<MODEL>, true>(f, nt, a.par, zr, g, a.extra ? a.extra + n : nullptr,
@@ -1870,7 +1957,7 @@ __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, 
                 for (int i = 0; i < Q; i++) {
                     const uint32_t r = (i & 1) ? w << 16 : w;
                     __builtin_nontemporal_store(DEV ? S::unraw_sr(f[i], r) : S::put_sr(f[i], (R)L::w(i), r),
-                                                (T*)(fout + (size_t)i * pbytes + n * (unsigned)sizeof(T)));
+                                                (gptr<T>)(fout + (size_t)i * pbytes + n * (unsigned)sizeof(T)));
                     if (i & 1) w = xorshift32(w);
                 }
                 return;
@@ -1879,7 +1966,7 @@ __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, 
 #pragma unroll
         for (int i = 0; i < Q; i++)
             __builtin_nontemporal_store(DEV ? S::unraw(f[i]) : S::put(f[i], (R)L::w(i)),
-                                        (T*)(fout + (size_t)i * pbytes + n * (unsigned)sizeof(T)));
+                                        (gptr<T>)(fout + (size_t)i * pbytes + n * (unsigned)sizeof(T)));
     }
 }
 
