@@ -204,26 +204,44 @@ __device__ __forceinline__ void d2q9_feq(R* f_eq, R d, R ux, R uy) {
     f_eq[8] = ( R(1.) + ( ( -1 + uy )*uy + ( 1 + ux - uy*R(3.) )*ux )*R(3.) )*d/R(36.);
 }
 
-// SetEquilibrium for the float path: the same polynomial with the divisions
-// by 9, 18 and 36 as products (held to a float tolerance, not to identical
-// roundings)
-__device__ __forceinline__ void d2q9_feq_fast(float* f_eq, float d, float ux, float uy) {
-    const float d9 = d * (2.f/9.f), d18 = d * (1.f/18.f), d36 = d * (1.f/36.f);
-    f_eq[0] = ( 2.f + ( -uy*uy - ux*ux )*3.f )*d9;
-    f_eq[1] = ( 2.f + ( -uy*uy + ( 1 + ux )*ux*2.f )*3.f )*d18;
-    f_eq[2] = ( 2.f + ( -ux*ux + ( 1 + uy )*uy*2.f )*3.f )*d18;
-    f_eq[3] = ( 2.f + ( -uy*uy + ( -1 + ux )*ux*2.f )*3.f )*d18;
-    f_eq[4] = ( 2.f + ( -ux*ux + ( -1 + uy )*uy*2.f )*3.f )*d18;
-    f_eq[5] = ( 1.f + ( ( 1 + uy )*uy + ( 1 + ux + uy*3.f )*ux )*3.f )*d36;
-    f_eq[6] = ( 1.f + ( ( 1 + uy )*uy + ( -1 + ux - uy*3.f )*ux )*3.f )*d36;
-    f_eq[7] = ( 1.f + ( ( -1 + uy )*uy + ( -1 + ux + uy*3.f )*ux )*3.f )*d36;
-    f_eq[8] = ( 1.f + ( ( -1 + uy )*uy + ( 1 + ux - uy*3.f )*ux )*3.f )*d36;
-}
-
 // Run(), Dynamics.c:40-67, plus (DRAG) Ladd momentum exchange on Solid nodes.
 // zone columns: Velocity, Velocity_x, Velocity_y, GravitationX,
 // GravitationY, [DragInObj, LiftInObj].
-template <class R, bool DRAG, bool GLOB>
+// The float BGK collision, f += omega (feq - f) (Dynamics.c:57-66), on the
+// deviations d = f - w, written so that the update Delta conserves mass and
+// gives momentum rho G exactly up to unbiased roundings: Delta is evaluated
+// for directions 1, 2, 5, 6, 7, 8 from feq - w = w_i (rho - 1 + 3 e.U +
+// (9/2 (e.U)^2 - 3/2 U^2)/rho), U = J + rho G/omega (the reference's
+// rho u with u = J/rho + G/omega), direction 3 and 4 from the momentum
+// balance and the rest population from the mass balance.  Evaluated
+// directly, the rounded weights 1/9, 1/36 and an approximate 1/rho create
+// mass and momentum each step.
+__device__ __forceinline__ void d2q9_bgk_dev(float* d, float omega, const double* zr) {
+    const float s = ((d[0] + d[1]) + (d[2] + d[3])) + ((d[4] + d[5]) + (d[6] + d[7])) + d[8];
+    const float rho = 1.f + s;
+    const float d57 = d[5] - d[7], d68 = d[6] - d[8];
+    const float Jx = (d[1] - d[3]) + d57 - d68, Jy = (d[2] - d[4]) + d57 + d68;
+    const float Gx = (float)zr[3], Gy = (float)zr[4];
+    const float rw = rho * __builtin_amdgcn_rcpf(omega);
+    const float Ux = __builtin_fmaf(Gx, rw, Jx), Uy = __builtin_fmaf(Gy, rw, Jy);
+    const float ir = __builtin_amdgcn_rcpf(rho), h = 4.5f * ir;
+    const float A = s - 1.5f * (Ux * Ux + Uy * Uy) * ir;   // rho - 1 - 3/2 U^2/rho
+    const float p = Ux + Uy, m = Uy - Ux;
+    const float D1 = omega * (__builtin_fmaf(Ux, __builtin_fmaf(h, Ux, 3.f), A) * (1.f / 9.f) - d[1]);
+    const float D2 = omega * (__builtin_fmaf(Uy, __builtin_fmaf(h, Uy, 3.f), A) * (1.f / 9.f) - d[2]);
+    const float D5 = omega * (__builtin_fmaf(p, __builtin_fmaf(h, p, 3.f), A) * (1.f / 36.f) - d[5]);
+    const float D7 = omega * (__builtin_fmaf(p, __builtin_fmaf(h, p, -3.f), A) * (1.f / 36.f) - d[7]);
+    const float D6 = omega * (__builtin_fmaf(m, __builtin_fmaf(h, m, 3.f), A) * (1.f / 36.f) - d[6]);
+    const float D8 = omega * (__builtin_fmaf(m, __builtin_fmaf(h, m, -3.f), A) * (1.f / 36.f) - d[8]);
+    const float E57 = D5 - D7, E68 = D6 - D8;
+    const float D3 = (D1 + E57) - (E68 + rho * Gx);
+    const float D4 = (D2 + E57) + (E68 - rho * Gy);
+    const float D0 = -(((D1 + D3) + (D2 + D4)) + ((D5 + D7) + (D6 + D8)));
+    d[0] += D0; d[1] += D1; d[2] += D2; d[3] += D3; d[4] += D4;
+    d[5] += D5; d[6] += D6; d[7] += D7; d[8] += D8;
+}
+
+template <class R, bool DRAG, bool GLOB, bool COLLIDE = true>
 __device__ __forceinline__ void d2q9_node(R* f, unsigned nt, R omega, R Density, const double* zr,
                                           double* g) {
     R uf;
@@ -275,22 +293,41 @@ __device__ __forceinline__ void d2q9_node(R* f, unsigned nt, R omega, R Density,
         break; }
     default: break;
     }
-    if (nt & 0x8u) {  // BGK
+    if (COLLIDE && (nt & 0x8u)) {  // BGK
+        if constexpr (sizeof(R) != 8) {
+            // float: on the deviations (f32 storage holds f itself)
+            using L = Lat<MODEL_D2Q9_SRT>;
+#pragma unroll
+            for (int i = 0; i < 9; i++) f[i] -= (R)L::w(i);
+            d2q9_bgk_dev(f, omega, zr);
+#pragma unroll
+            for (int i = 0; i < 9; i++) f[i] += (R)L::w(i);
+            return;
+        }
         R d = f[8]+f[7]+f[6]+f[5]+f[4]+f[3]+f[2]+f[1]+f[0];
         R f_eq[9];
-        if constexpr (sizeof(R) == 8) {
-            R ux = (( f[8]-f[7]-f[6]+f[5]-f[3]+f[1] )/d + (R)zr[3]/omega );
-            R uy = ((-f[8]-f[7]+f[6]+f[5]-f[4]+f[2] )/d + (R)zr[4]/omega );
-            d2q9_feq(f_eq, d, ux, uy);
-        } else {
-            const float id = __builtin_amdgcn_rcpf(d), iw = __builtin_amdgcn_rcpf(omega);
-            R ux = ( f[8]-f[7]-f[6]+f[5]-f[3]+f[1] )*id + (R)zr[3]*iw;
-            R uy = (-f[8]-f[7]+f[6]+f[5]-f[4]+f[2] )*id + (R)zr[4]*iw;
-            d2q9_feq_fast(f_eq, d, ux, uy);
-        }
+        R ux = (( f[8]-f[7]-f[6]+f[5]-f[3]+f[1] )/d + (R)zr[3]/omega );
+        R uy = ((-f[8]-f[7]+f[6]+f[5]-f[4]+f[2] )/d + (R)zr[4]/omega );
+        d2q9_feq(f_eq, d, ux, uy);
 #pragma unroll
         for (int i = 0; i < 9; i++) f[i] = f[i] + omega*(f_eq[i]-f[i]);
     }
+}
+
+// d2q9_SRT node of a shifted storage format, on the deviations d = f - w:
+// boundary nodes add w back for their boundary condition, BGK runs on d.
+template <bool DRAG, bool GLOB>
+__device__ __forceinline__ void d2q9_node_dev(float* d, unsigned nt, float omega, float Density, const double* zr,
+                                              double* g) {
+    using L = Lat<MODEL_D2Q9_SRT>;
+    if ((nt & 0x7u) != 0u) {
+#pragma unroll
+        for (int i = 0; i < 9; i++) d[i] += (float)L::w(i);
+        d2q9_node<float, DRAG, GLOB, false>(d, nt, omega, Density, zr, g);
+#pragma unroll
+        for (int i = 0; i < 9; i++) d[i] -= (float)L::w(i);
+    }
+    if (nt & 0x8u) d2q9_bgk_dev(d, omega, zr);
 }
 
 // ------------------------------------------------------------ d2q9 MRT ---
@@ -1891,9 +1928,9 @@ __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, 
     // check_lattice_size refuses larger ones)
     const char* fin = (const char*)a.fin;
     const size_t pbytes = (size_t)a.pitch * sizeof(T);
-    // d3q19 and the d2q9 MRT models in a shifted format collide on the
-    // stored deviations f - w
-    constexpr bool DEV = (is_d3q19<MODEL> || is_d2q9m<MODEL>) && S::shifted;
+    // d3q19 and the 2D models in a shifted format collide on the stored
+    // deviations f - w
+    constexpr bool DEV = (is_d3q19<MODEL> || L::Q == 9) && S::shifted;
     R f[Q];
     if (DISPATCH != 1 || active) {
 #pragma unroll
@@ -1944,6 +1981,8 @@ __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, 
                                                             (size_t)nx * ny * nz);
             else
                 cum_node<R, GLOB, is_cum_smag<MODEL>>(f, nt, a.par, zr, g);
+        } else if constexpr (DEV) {
+            d2q9_node_dev<MODEL == MODEL_D2Q9_SRT_DRAG, GLOB>(f, nt, (R)a.par[0], (R)a.par[1], zr, g);
         } else {
             d2q9_node<R, MODEL == MODEL_D2Q9_SRT_DRAG, GLOB>(f, nt, (R)a.par[0], (R)a.par[1], zr, g);
         }
