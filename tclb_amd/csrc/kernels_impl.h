@@ -948,9 +948,15 @@ __device__ __forceinline__ void d3q19_mrt(R* f, unsigned nt, const float* fp, co
     M[0] = m0 * (1.f / 19.f);
     M[1] = __builtin_fmaf(fp[0], n1, __builtin_fmaf(19.f / 2394.f, j2, (-11.f / 2394.f) * m0));
     M[2] = __builtin_fmaf(fp[1], n2, __builtin_fmaf(-2.f / 252.f, j2, (3.f / 252.f) * m0));
-    M[3] = Fx * 0.1f;
-    M[5] = Fy * 0.1f;
-    M[7] = Fz * 0.1f;
+    // J/10 corrected to the rounded quotient: the rounded constant 0.1 alone
+    // would add 1.5e-8 of the momentum each step
+    const auto tenth = [](float x) {
+        const float q = x * 0.1f;
+        return __builtin_fmaf(__builtin_fmaf(-10.f, q, x), 0.1f, q);
+    };
+    M[3] = tenth(Fx);
+    M[5] = tenth(Fy);
+    M[7] = tenth(Fz);
     M[4] = __builtin_fmaf(fp[2], n4, (-1.f / 60.f) * Fx);
     M[6] = __builtin_fmaf(fp[2], n6, (-1.f / 60.f) * Fy);
     M[8] = __builtin_fmaf(fp[2], n8, (-1.f / 60.f) * Fz);
@@ -1200,8 +1206,18 @@ __device__ __forceinline__ void d3q19_node(R* f, unsigned nt, const double* par,
     }
     if constexpr (!DEV) {
         if ((nt & 0x20u) == 0x20u) {
-            if constexpr (sizeof(R) == 8) d3q19_mrt_dense<R, GLOB, LID>(f, nt, par, zr, g);
-            else d3q19_mrt<R, GLOB, LID>(f, nt, fp, zr, g);
+            if constexpr (sizeof(R) == 8) {
+                d3q19_mrt_dense<R, GLOB, LID>(f, nt, par, zr, g);
+            } else {
+                // float on the deviations f - w: the rounded 1/19 then scales
+                // rho - 1 only, and adds no mass each step
+                using L = Lat<MODEL_D3Q19>;
+#pragma unroll
+                for (int i = 0; i < 19; i++) f[i] -= R(L::w(i));
+                d3q19_mrt<R, GLOB, LID, true>(f, nt, fp, zr, g);
+#pragma unroll
+                for (int i = 0; i < 19; i++) f[i] += R(L::w(i));
+            }
         }
     }
 }
@@ -1662,7 +1678,7 @@ __device__ __forceinline__ void cum_node(R* f, unsigned nt, const double* par, c
 #pragma unroll
     for (int i = 0; i < 27; i++) m[i] = f[i];
     cum_fwd(m);
-    const R rho = m[0];
+    const R rho = m[0], J0x = m[1], J0y = m[3], J0z = m[9];
     if (GLOB) {
         // getU_: 0 on Wall nodes, else (J + F/2)/rho
         const bool wall = ((nt >> 3) & 0xFu) == 10u;
@@ -1731,6 +1747,14 @@ __device__ __forceinline__ void cum_node(R* f, unsigned nt, const double* par, c
         if (o > 3) c[i] = R(0);
     }
     cum_convert<R, false>(m, c);
+    if constexpr (!std::is_same<R, double>::value) {
+        // the collision keeps rho and adds rho F to J; the float round trip
+        // through J/rho would add about 1e-8 of the momentum each step
+        m[0] = rho;
+        m[1] = __builtin_fmaf(rho, Fx, J0x);
+        m[3] = __builtin_fmaf(rho, Fy, J0y);
+        m[9] = __builtin_fmaf(rho, Fz, J0z);
+    }
     cum_inv(m);
 #pragma unroll
     for (int i = 0; i < 27; i++) f[i] = m[i];

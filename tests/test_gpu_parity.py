@@ -673,6 +673,66 @@ def test_cavity_reduced_storage_tracks_f64(storage):
     assert abs(t1 - t0) <= (0.25 if storage == "bf16s" else 0.05) * abs(t0), (t0, t1)
 
 
+def _velocities(model):
+    """e_i of each model's velocity set, in its direction order"""
+    if model.startswith("d2q9"):
+        return np.array([(0, 0, 0), (1, 0, 0), (0, 1, 0), (-1, 0, 0), (0, -1, 0), (1, 1, 0), (-1, 1, 0), (-1, -1, 0),
+                         (1, -1, 0)], float)
+    if model.startswith("d3q19"):
+        return np.array([(0, 0, 0), (1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1), (1, 1, 0),
+                         (-1, 1, 0), (1, -1, 0), (-1, -1, 0), (1, 0, 1), (-1, 0, 1), (1, 0, -1), (-1, 0, -1),
+                         (0, 1, 1), (0, -1, 1), (0, 1, -1), (0, -1, -1)], float)
+    digit = [0, 1, -1] if model.startswith("d3q27_cumulant") else [-1, 0, 1]
+    return np.array([(digit[i % 3], digit[i // 3 % 3], digit[i // 9]) for i in range(27)], float)
+
+
+@pytest.mark.parametrize("storage", ["f32", "f32s"])
+@pytest.mark.parametrize("model", ["d2q9_SRT", "d2q9", "d3q19", "d3q27", "d3q27_cumulant"])
+def test_float_collisions_conserve_mass_and_momentum(model, storage):
+    """A periodic box of collision nodes with no force, from a random state
+    (rho = 1 + 0.01 r, u = (0.02, 0, 0) + 0.01 r at every node), after 1,000
+    float steps: the total mass has moved by less than 1e-7 of itself and
+    the momentum by less than 1e-6 of itself (unbiased roundings of the
+    stored values give about 1e-8 here).
+    A float collision whose rounded constants scale every population (1/3
+    and 1/6 of a factored inverse transform, 1/19 of the moment norms,
+    rounded weights, an approximate 1/rho) adds 1e-8 to 1e-7 of the mass
+    or momentum per step, 1e-5 or more after 1,000 steps."""
+    say(f"conservation {model} {storage}")
+    from tclb_amd.modelbench import COLLISION, settings_of
+    m = tclb_amd.Model(model)
+    e = _velocities(model)
+    nx, ny, nz = (128, 96, 1) if m.Q == 9 else (40, 36, 32)
+    lat = tclb_amd.Lattice(model, nx, ny, nz, storage=storage)
+    try:
+        lat.set_flags(np.full((nz, ny, nx), m.flag(COLLISION.get(model, "MRT")), np.uint16))
+        for k, v in settings_of(model):
+            if not k.startswith(("Force", "Gravitation")):
+                lat.set_setting(k, v)
+        lat.init()
+        rng = np.random.default_rng(11)
+        rho = 1 + 0.01 * rng.uniform(-1, 1, (nz, ny, nx))
+        u = 0.01 * rng.uniform(-1, 1, (3, nz, ny, nx))
+        u[0] += 0.02
+        if m.Q == 9:
+            u[2] = 0
+        w = lattice_weights(m.Q, cumulant=model.startswith("d3q27_cumulant"))
+        eu = np.einsum("ia,a...->i...", e, u)
+        f = w[:, None, None, None] * rho * (1 + 3 * eu + 4.5 * eu ** 2 - 1.5 * (u ** 2).sum(0))
+        lat.set_f(f)
+        f0 = lat.get_f()
+        mass0, mom0 = f0.sum(), e.T @ f0.reshape(m.Q, -1).sum(1)
+        lat.iterate(1000)
+        f1 = lat.get_f()
+        assert np.all(np.isfinite(f1))
+        mass1, mom1 = f1.sum(), e.T @ f1.reshape(m.Q, -1).sum(1)
+        say(f"  mass {mass1 / mass0 - 1:.2e}  momentum x {mom1[0] / mom0[0] - 1:.2e}")
+        assert abs(mass1 / mass0 - 1) < 1e-7
+        assert np.all(np.abs(mom1 - mom0) < 1e-6 * abs(mom0[0]))
+    finally:
+        lat.close()
+
+
 @pytest.mark.parametrize("storage", ["f16s", "bf16s"])
 @pytest.mark.parametrize("model,nx,ny,nz", [m for m in MODELS if m[0] in ("d2q9_SRT", "d3q19", "d2q9", "d3q27")])
 def test_stochastic_rounding_tracks_the_oracle(oracle_mod, model, nx, ny, nz, storage):
