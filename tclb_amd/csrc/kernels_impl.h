@@ -1559,8 +1559,18 @@ __device__ __forceinline__ R cum_sum(const R* c, const R* m, std::integer_sequen
 template <class R, bool TO_CUM, int I>
 __device__ __forceinline__ void cum_one(R* m, R* c, R m0, R im0) {
     const R s = cum_sum<R, I>(c, m, std::make_integer_sequence<int, 27>{});
-    if (TO_CUM) c[I] = (m[I] - s) * im0;
-    else m[I] = fma(c[I], m0, s);
+    if constexpr (TO_CUM && !std::is_same<R, double>::value && (I == 1 || I == 3 || I == 9)) {
+        // the velocity J/rho in float, corrected to the rounded quotient:
+        // J (1/rho) rounds 1/rho once more, and with rho near 1 from below
+        // that rounding goes up more often than down, so J/rho rho would
+        // gain about 1e-8 of the momentum each step
+        const R q = (m[I] - s) * im0;
+        c[I] = fma(fma(-q, m0, m[I] - s), im0, q);
+    } else if (TO_CUM) {
+        c[I] = (m[I] - s) * im0;
+    } else {
+        m[I] = fma(c[I], m0, s);
+    }
 }
 
 // one reciprocal of m0 instead of 26 divisions (the float path is
@@ -1678,7 +1688,7 @@ __device__ __forceinline__ void cum_node(R* f, unsigned nt, const double* par, c
 #pragma unroll
     for (int i = 0; i < 27; i++) m[i] = f[i];
     cum_fwd(m);
-    const R rho = m[0], J0x = m[1], J0y = m[3], J0z = m[9];
+    const R rho = m[0];
     if (GLOB) {
         // getU_: 0 on Wall nodes, else (J + F/2)/rho
         const bool wall = ((nt >> 3) & 0xFu) == 10u;
@@ -1747,14 +1757,6 @@ __device__ __forceinline__ void cum_node(R* f, unsigned nt, const double* par, c
         if (o > 3) c[i] = R(0);
     }
     cum_convert<R, false>(m, c);
-    if constexpr (!std::is_same<R, double>::value) {
-        // the collision keeps rho and adds rho F to J; the float round trip
-        // through J/rho would add about 1e-8 of the momentum each step
-        m[0] = rho;
-        m[1] = __builtin_fmaf(rho, Fx, J0x);
-        m[3] = __builtin_fmaf(rho, Fy, J0y);
-        m[9] = __builtin_fmaf(rho, Fz, J0z);
-    }
     cum_inv(m);
 #pragma unroll
     for (int i = 0; i < 27; i++) f[i] = m[i];
@@ -2013,18 +2015,16 @@ __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, 
         // streaming stores: nothing reads them back this step
         char* fout = (char*)a.fout;
         if constexpr (S::stochastic) {
-            if (a.sr) {
-                // direction 2j rounds with word j, 2j + 1 with word j << 16
-                uint32_t w = sr_word0(n, a.sr_key);
+            // direction 2j rounds with word j, 2j + 1 with word j << 16
+            uint32_t w = sr_word0(n, a.sr_key);
 #pragma unroll
-                for (int i = 0; i < Q; i++) {
-                    const uint32_t r = (i & 1) ? w << 16 : w;
-                    __builtin_nontemporal_store(DEV ? S::unraw_sr(f[i], r) : S::put_sr(f[i], (R)L::w(i), r),
-                                                (gptr<T>)(fout + (size_t)i * pbytes + n * (unsigned)sizeof(T)));
-                    if (i & 1) w = xorshift32(w);
-                }
-                return;
+            for (int i = 0; i < Q; i++) {
+                const uint32_t r = (i & 1) ? w << 16 : w;
+                __builtin_nontemporal_store(DEV ? S::unraw_sr(f[i], r) : S::put_sr(f[i], (R)L::w(i), r),
+                                            (gptr<T>)(fout + (size_t)i * pbytes + n * (unsigned)sizeof(T)));
+                if (i & 1) w = xorshift32(w);
             }
+            return;
         }
 #pragma unroll
         for (int i = 0; i < Q; i++)
@@ -2464,8 +2464,10 @@ hipError_t step_m(int st, bool glob, const StepArgs& a, int block, hipStream_t s
 #if TCLBX_HAS_FLOAT
     case ST_F32: return step_ms<MODEL, ST_F32>(glob, a, block, s);
     case ST_F32S: return step_ms<MODEL, ST_F32S>(glob, a, block, s);
-    case ST_F16S: return step_ms<MODEL, ST_F16S>(glob, a, block, s);
-    case ST_BF16S: return step_ms<MODEL, ST_BF16S>(glob, a, block, s);
+    case ST_F16S:
+        return a.sr ? step_ms<MODEL, ST_F16S_SR>(glob, a, block, s) : step_ms<MODEL, ST_F16S>(glob, a, block, s);
+    case ST_BF16S:
+        return a.sr ? step_ms<MODEL, ST_BF16S_SR>(glob, a, block, s) : step_ms<MODEL, ST_BF16S>(glob, a, block, s);
 #endif
     }
     return hipErrorInvalidValue;

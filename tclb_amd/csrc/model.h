@@ -44,6 +44,10 @@ enum ModelId {
 
 // storage format of the distributions in HBM
 enum Storage { ST_F64 = 0, ST_F32 = 1, ST_F32S = 2, ST_F16S = 3, ST_BF16S = 4, ST_COUNT = 5 };
+// kernel instantiations of the 16-bit formats whose step stores round
+// stochastically (a lattice option), kept apart so that the plain step
+// carries none of that code; never a lattice's storage
+enum { ST_F16S_SR = 5, ST_BF16S_SR = 6 };
 
 struct SettingDef {
     std::string name;

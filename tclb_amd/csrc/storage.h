@@ -102,8 +102,9 @@ template <> struct Store<ST_F16S> {
     __device__ __forceinline__ static T put(R v, R s) { return (_Float16)(v - s); }
     __device__ __forceinline__ static R raw(T v) { return (float)v; }
     __device__ __forceinline__ static T unraw(R v) { return (_Float16)v; }
-    // the same stores, rounded stochastically with random bits r
-    static constexpr bool stochastic = true;
+    // the same stores, rounded stochastically with random bits r (used by
+    // the ST_F16S_SR instantiation)
+    static constexpr bool stochastic = false;
     __device__ __forceinline__ static T put_sr(R v, R s, uint32_t r) { return f16_sr(v - s, r); }
     __device__ __forceinline__ static T unraw_sr(R v, uint32_t r) { return f16_sr(v, r); }
 };
@@ -117,9 +118,13 @@ template <> struct Store<ST_BF16S> {
     __device__ __forceinline__ static T put(R v, R s) { return (__bf16)(v - s); }
     __device__ __forceinline__ static R raw(T v) { return (float)v; }
     __device__ __forceinline__ static T unraw(R v) { return (__bf16)v; }
-    static constexpr bool stochastic = true;
+    static constexpr bool stochastic = false;
     __device__ __forceinline__ static T put_sr(R v, R s, uint32_t r) { return bf16_sr(v - s, r); }
     __device__ __forceinline__ static T unraw_sr(R v, uint32_t r) { return bf16_sr(v, r); }
 };
+
+// the step kernels of a lattice with stochastic rounding on
+template <> struct Store<ST_F16S_SR> : Store<ST_F16S> { static constexpr bool stochastic = true; };
+template <> struct Store<ST_BF16S_SR> : Store<ST_BF16S> { static constexpr bool stochastic = true; };
 
 }  // namespace tclbx
