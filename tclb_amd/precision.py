@@ -4,8 +4,8 @@
 
 The storage formats store f - w_i (f32s, f16s, bf16s) or f (f32) and
 collide in float; how the float collision meets the stored values differs
-by model family: d3q19 collides on the stored deviations themselves, the
-2D and 27-velocity models on f = stored + w as the reference's float build
+by model family: d3q19 and the 2D models collide on the deviations f - w,
+the 27-velocity models on f = stored + w as the reference's float build
 does.  This runs one flow per model in FP64 and in each format and reports
 the relative L2 and max-norm of the velocity (over the fluid nodes,
 max-norm relative to the largest FP64 speed), the relative L2 of the

@@ -8,7 +8,9 @@ solved sequentially as single FP64 lattices, and per case the relative
 error of the drag against that FP64 solve: of the drag averaged over the
 last `window` steps (sampled every `every` steps; shedding cases oscillate,
 so the last-step value also carries the phase of the oscillation) and of
-the last step's drag.  Prints one JSON object.
+the last step's drag.  --tuning FLAT,BLOCK,ROWS_PER_TILE,GRID,ORDER runs
+that tile shape instead of the autotuner's pick (for profiled runs, whose
+timing misleads the autotuner).  Prints one JSON object.
 """
 import argparse
 import json
@@ -33,7 +35,10 @@ def _advance(it, steps, window, every, gd):
 
 
 def run(ncases=256, steps=8000, storage="f16s", seed=4, reference=True, autotune_steps=5, log=None,
-        model="d2q9_drag", window=2000, every=100):
+        model="d2q9_drag", window=2000, every=100, tuning=None):
+    """`tuning`: a tile shape {flat, block, rows_per_tile, grid, order} to
+    run instead of the autotuner's (a profiler run perturbs the autotuner's
+    timing, so its pick differs from an unprofiled run's)"""
     say = log or (lambda m: None)
     sweep = C.d2q9_cylinder_sweep(ncases=ncases, seed=seed, steps=steps, model=model)
     nodes = sum(c.nodes for c in sweep)
@@ -43,7 +48,12 @@ def run(ncases=256, steps=8000, storage="f16s", seed=4, reference=True, autotune
     b = Batch(model, storage, capacity_bytes=cap)
     ids = [b.add(c) for c in sweep]
     gd, gl = Model(model).global_id("Drag"), Model(model).global_id("Lift")
-    tune = b.autotune(steps=autotune_steps)
+    if tuning is None:
+        tune = b.autotune(steps=autotune_steps)
+    else:
+        tune = dict(tuning)
+        b.set_tuning(block=tune["block"], rows_per_tile=tune["rows_per_tile"], grid=tune["grid"],
+                     flat=bool(tune["flat"]), order=tune["order"])
     say(f"batch of {ncases} cases, {nodes} nodes, tuning {tune}")
     def bat(n, flags):
         b.iterate(n, flags)
@@ -117,9 +127,15 @@ def main():
     ap.add_argument("--model", default="d2q9_drag", help="d2q9_drag (MRT) or d2q9_SRT_drag (BGK)")
     ap.add_argument("--window", type=int, default=2000, help="steps at the end over which drag is averaged")
     ap.add_argument("--every", type=int, default=100, help="drag sampling interval in that window")
+    ap.add_argument("--tuning", default="", help="FLAT,BLOCK,ROWS_PER_TILE,GRID,ORDER: this tile shape, no autotuner")
     a = ap.parse_args()
+    tuning = None
+    if a.tuning:
+        v = [int(x) for x in a.tuning.split(",")]
+        tuning = dict(zip(("flat", "block", "rows_per_tile", "grid", "order"), v))
     r = run(a.cases, a.steps, a.storage, a.seed, not a.no_reference,
-            log=lambda m: print(m, file=sys.stderr, flush=True), model=a.model, window=a.window, every=a.every)
+            log=lambda m: print(m, file=sys.stderr, flush=True), model=a.model, window=a.window, every=a.every,
+            tuning=tuning)
     print(json.dumps(r), flush=True)
 
 
