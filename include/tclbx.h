@@ -268,6 +268,13 @@ int tclbx_batch_set_density(tclbx_batch* b, int id, int q, const double* in);
 int tclbx_batch_autotune(tclbx_batch* b, int steps, int* flat, int* block, int* rows_per_tile, int* grid,
                          double* us_per_step);
 int tclbx_batch_set_tuning(tclbx_batch* b, int flat, int block, int rows_per_tile, int grid);
+/* stochastic rounding of every case's stores (f16s, bf16s; TCLBX_ERR for
+ * the others), each case keyed by its own node index, iteration and `seed`
+ * as tclbx_set_stochastic_rounding keys a lone lattice, so a case gives the
+ * same result alone and in a batch; steps are then launched one by one.  No
+ * counterpart in the reference (batching is this engine's, and it rounds to
+ * nearest). */
+int tclbx_batch_set_stochastic_rounding(tclbx_batch* b, int on, unsigned seed);
 /* the order the step walks the tiles in: 0 the tile list's, 1 XCD-
  * contiguous (the k-th of the eight XCDs takes the k-th contiguous run of
  * tiles, so neighbouring rows are pulled through one L2); the autotuner
@@ -321,6 +328,10 @@ int tclbx_server_stats(tclbx_server* s, long long* completed, long long* rejecte
                        double* busy_s, int* active, int* queued, double* now_s);
 /* tile shape of the server's batch (flat runs of the node index by default) */
 int tclbx_server_set_tuning(tclbx_server* s, int flat, int block, int rows_per_tile, int grid);
+/* stochastic rounding of the served requests' stores, as
+ * tclbx_batch_set_stochastic_rounding; taken up by the worker before its next
+ * admission */
+int tclbx_server_set_stochastic_rounding(tclbx_server* s, int on, unsigned seed);
 
 /* Measured device copy bandwidth: `reps` copies of `bytes` (rounded down
  * to 16) between two device buffers by a plain vector-copy kernel;

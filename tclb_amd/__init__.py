@@ -39,6 +39,7 @@ SYMBOLS = [
     "tclbx_bytes_per_node", "tclbx_set_block", "tclbx_set_graph_steps", "tclbx_copy_bandwidth",
     "tclbx_set_rows_per_tile", "tclbx_get_rows_per_tile", "tclbx_prepare",
     "tclbx_set_sorted_dispatch", "tclbx_get_sorted_dispatch", "tclbx_set_stochastic_rounding",
+    "tclbx_batch_set_stochastic_rounding", "tclbx_server_set_stochastic_rounding",
     "tclbx_batch_create", "tclbx_batch_destroy", "tclbx_batch_case_bytes", "tclbx_batch_add_case",
     "tclbx_batch_remove_case", "tclbx_batch_set_setting", "tclbx_batch_set_zone_setting",
     "tclbx_batch_flag_overwrite", "tclbx_batch_init_case", "tclbx_batch_iterate", "tclbx_batch_get_globals",
@@ -124,6 +125,8 @@ def lib():
     L.tclbx_set_sorted_dispatch.argtypes = [vp, c_int]
     L.tclbx_get_sorted_dispatch.argtypes = [vp, ip, ctypes.POINTER(ctypes.c_longlong)]
     L.tclbx_set_stochastic_rounding.argtypes = [vp, c_int, ctypes.c_uint]
+    L.tclbx_batch_set_stochastic_rounding.argtypes = [vp, c_int, ctypes.c_uint]
+    L.tclbx_server_set_stochastic_rounding.argtypes = [vp, c_int, ctypes.c_uint]
     L.tclbx_copy_bandwidth.argtypes = [ctypes.c_longlong, c_int, dp, dp]
     ll, llp = ctypes.c_longlong, ctypes.POINTER(ctypes.c_longlong)
     L.tclbx_batch_create.argtypes = [c_char_p, c_char_p, ll, ctypes.POINTER(vp)]
@@ -671,6 +674,11 @@ class Batch:
         _check(lib().tclbx_batch_set_tuning(self._h, int(bool(flat)), int(block), int(rows_per_tile), int(grid)))
         self.tile_order(order)
 
+    def set_stochastic_rounding(self, on=True, seed=0):
+        """Stochastic rounding of every case's stores (16-bit formats);
+        a case is keyed as a lone Lattice with the same seed."""
+        _check(lib().tclbx_batch_set_stochastic_rounding(self._h, int(bool(on)), int(seed) & 0xFFFFFFFF))
+
     def tile_order(self, set=None):
         """The tile order of the step (0: list order, 1: XCD-contiguous);
         `set` changes it."""
@@ -776,6 +784,10 @@ class Server:
 
     def set_tuning(self, block=256, rows_per_tile=1, grid=0, flat=True):
         _check(lib().tclbx_server_set_tuning(self._h, int(bool(flat)), int(block), int(rows_per_tile), int(grid)))
+
+    def set_stochastic_rounding(self, on=True, seed=0):
+        """Stochastic rounding of the served requests' stores (16-bit formats)."""
+        _check(lib().tclbx_server_set_stochastic_rounding(self._h, int(bool(on)), int(seed) & 0xFFFFFFFF))
 
 
 class Case:

@@ -295,6 +295,8 @@ void Batch::rebuild() {
         std::memset(&bc, 0, sizeof(bc));
         bc.args = case_args(c);
         bc.parity = c.parity;
+        bc.sr_iter = (unsigned)c.iter;
+        bc.sr_seed = sr_seed_;
         descs.push_back(bc);
         const int nt = step_blocks(bc.args, false, tune_.block);
         for (int t = 0; t < nt; t++) tiles.push_back(make_int2(slot, t));
@@ -317,6 +319,7 @@ void Batch::rebuild() {
         hip_check(hipMalloc((void**)&d_glob_, (size_t)ncases_dev_ * G * sizeof(double)), "hipMalloc(globals)");
         hip_check(hipMemcpy(d_cases_, descs.data(), descs.size() * sizeof(BatchCase), hipMemcpyHostToDevice),
                   "upload cases");
+        descs_ = descs;
         hip_check(hipMemcpy(d_tiles_, tiles.data(), tiles.size() * sizeof(int2), hipMemcpyHostToDevice),
                   "upload tiles");
         hip_check(hipMemcpy(d_gtiles_, gtiles.data(), gtiles.size() * sizeof(int2), hipMemcpyHostToDevice),
@@ -331,13 +334,17 @@ void Batch::rebuild() {
     version_++;
 }
 
-void Batch::launch(int phase, bool glob) {
+void Batch::launch(int phase, bool glob, int k) {
     if (ncases_dev_ == 0) return;
     const int nt = glob ? ngtiles_ : ntiles_;
     const int grid = glob ? std::min(nt, 8 * ncu_) : (tune_.grid > 0 ? std::min(tune_.grid, nt) : nt);
-    // the phase word: bit 0 the snapshot phase, bit 1 the tile order
-    hip_check(launch_batch_step(model->id, storage, glob, d_cases_, glob ? d_gtiles_ : d_tiles_, nt,
-                                (phase & 1) | (tune_.order ? 2 : 0), d_gpart_, grid, tune_.block, stream_),
+    // the phase word: bit 0 the snapshot phase, bit 1 the tile order, the
+    // bits above the step's index k in the Iterate (stochastic rounding);
+    // with stochastic rounding on, the kernels of its instantiation
+    const int st = !sr_ ? storage : (storage == ST_F16S ? ST_F16S_SR : ST_BF16S_SR);
+    hip_check(launch_batch_step(model->id, st, glob, d_cases_, glob ? d_gtiles_ : d_tiles_, nt,
+                                (phase & 1) | (tune_.order ? 2 : 0) | (k << 2), d_gpart_, grid, tune_.block,
+                                stream_),
               "batch step kernel");
     if (glob) {
         hip_check(launch_reduce_globals_cases(d_gpart_, ngtiles_, kernel_nglobals(model->id), model->sum_globals,
@@ -376,10 +383,28 @@ void Batch::capture_graph() {
     graph_version_ = version_;
 }
 
+void Batch::SetStochasticRounding(bool on, unsigned seed) {
+    if (on && storage != ST_F16S && storage != ST_BF16S)
+        throw std::invalid_argument(std::string("stochastic rounding applies to the 16-bit formats f16s and bf16s, not ") +
+                                    storage_name(storage));
+    sr_ = on;
+    sr_seed_ = seed;
+    dirty_ = true;  // the case table carries the seed
+}
+
+// each case's iteration at the start of this Iterate into the case table
+void Batch::upload_sr_iterations() {
+    if (ncases_dev_ == 0) return;
+    for (int slot = 0; slot < ncases_dev_; slot++) descs_[slot].sr_iter = (unsigned)get(order_[slot]).iter;
+    hip_check(hipMemcpyAsync(d_cases_, descs_.data(), descs_.size() * sizeof(BatchCase), hipMemcpyHostToDevice,
+                             stream_),
+              "upload case iterations");
+}
+
 void Batch::Prepare() {
     if (dirty_) rebuild();
     upload_zones();
-    if (ncases_dev_ > 0 && graph_steps_ >= 2) capture_graph();
+    if (ncases_dev_ > 0 && graph_steps_ >= 2 && !sr_) capture_graph();
 }
 
 void Batch::Iterate(int niter, int iter_type) {
@@ -396,7 +421,8 @@ void Batch::Iterate(int niter, int iter_type) {
     const int plain_end = niter - (last_glob ? 1 : 0);
     const bool captured = graph_exec_ != nullptr && graph_version_ == version_;
     const int lead = phase_ != 0 ? 1 : 0;
-    if (!every_glob && graph_steps_ >= 2 && plain_end - lead >= (captured ? 1 : 2) * graph_steps_) {
+    if (sr_) upload_sr_iterations();
+    if (!every_glob && !sr_ && graph_steps_ >= 2 && plain_end - lead >= (captured ? 1 : 2) * graph_steps_) {
         if (phase_ != 0) {
             launch(phase_, false);
             phase_ ^= 1;
@@ -412,7 +438,7 @@ void Batch::Iterate(int niter, int iter_type) {
         if (last_glob && i == niter - 1)
             hip_check(hipMemsetAsync(d_glob_, 0, (size_t)ncases_dev_ * G * sizeof(double), stream_),
                       "clear globals");
-        launch(phase_, glob);
+        launch(phase_, glob, i);
         phase_ ^= 1;
     }
     hip_check(hipEventRecord(ev1_, stream_), "event record");

@@ -90,6 +90,12 @@ public:
     // keep the fastest
     Tuning Autotune(int steps);
     void SetTuning(const Tuning& t);
+    // stochastic rounding of every case's stores (16-bit formats only); a
+    // case is keyed by its own node index, iteration and `seed`, as a lone
+    // Lattice with the same seed keys it, so its result is the same alone and
+    // in a batch.  Steps are then launched one by one (no captured loop).
+    void SetStochasticRounding(bool on, unsigned seed);
+    bool stochastic_rounding() const { return sr_; }
     const Tuning& tuning() const { return tune_; }
     std::vector<Tuning> last_trials() const { return trials_; }
 
@@ -120,7 +126,10 @@ private:
     StepArgs case_args(const Case& c) const;
     void rebuild();
     void upload_zones();
-    void launch(int phase, bool glob);
+    // one step; k = its index in the current Iterate (stochastic rounding
+    // keys a case's step k by the case's iteration + k)
+    void launch(int phase, bool glob, int k = 0);
+    void upload_sr_iterations();
     void drop_graph();
     void capture_graph();
     Case& get(int id);
@@ -131,6 +140,9 @@ private:
     int next_id_ = 0;
     int phase_ = 0;
     bool dirty_ = true;
+    bool sr_ = false;
+    unsigned sr_seed_ = 0;
+    std::vector<BatchCase> descs_;  // host copy of the case table
     Tuning tune_;
     std::vector<Tuning> trials_;
     // device tables, rebuilt when the case set or tuning changes

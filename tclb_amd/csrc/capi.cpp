@@ -674,6 +674,20 @@ int tclbx_server_stats(tclbx_server* s, long long* completed, long long* rejecte
     if (now_s) *now_s = s->s->now();
     return TCLBX_OK;
 }
+int tclbx_batch_set_stochastic_rounding(tclbx_batch* b, int on, unsigned seed) {
+    CHECK_B(b);
+    return guard([&]() {
+        b->b->SetStochasticRounding(on != 0, seed);
+        return TCLBX_OK;
+    });
+}
+int tclbx_server_set_stochastic_rounding(tclbx_server* s, int on, unsigned seed) {
+    CHECK_S(s);
+    return guard([&]() {
+        s->s->SetStochasticRounding(on != 0, seed);
+        return TCLBX_OK;
+    });
+}
 int tclbx_server_set_tuning(tclbx_server* s, int flat, int block, int rows_per_tile, int grid) {
     CHECK_S(s);
     return guard([&]() {

@@ -63,7 +63,12 @@ struct StepArgs {
 struct BatchCase {
     StepArgs args;
     int parity;
-    int pad_[3];
+    // stochastic rounding (the SR instantiations of the batch step): the
+    // case's iteration when the current sequence of launches began, and its
+    // seed; launch k of the sequence computes iteration sr_iter + k, keyed
+    // as a lone lattice keys it
+    unsigned sr_iter, sr_seed;
+    int pad_;
 };
 
 // Blocks (tiles) of one step launch of the lattice `a` with `block`

@@ -1959,6 +1959,11 @@ __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, 
     constexpr bool DEV = (is_d3q19<MODEL> || L::Q == 9) && S::shifted;
     R f[Q];
     if (DISPATCH != 1 || active) {
+        // all Q pulls are issued before any is used: the scheduler barrier
+        // keeps the compiler from pairing each load with its conversion and
+        // waiting on it alone (which it did in the batch step's flat-tile
+        // loop, serializing the pulls)
+        T v[Q];
 #pragma unroll
         for (int i = 0; i < Q; i++) {
             // source node x - e_i with periodic wrap
@@ -1966,9 +1971,11 @@ __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, 
             const int sy = L::ey(i) > 0 ? ym : (L::ey(i) < 0 ? yp : y);
             const int sz = L::ez(i) > 0 ? zm : (L::ez(i) < 0 ? zp : z);
             const unsigned src = (unsigned)sx + (unsigned)nx * ((unsigned)sy + (unsigned)ny * (unsigned)sz);
-            const T v = *(gptr<const T>)(fin + (size_t)i * pbytes + src * (unsigned)sizeof(T));
-            f[i] = DEV ? S::raw(v) : S::get(v, (R)L::w(i));
+            v[i] = *(gptr<const T>)(fin + (size_t)i * pbytes + src * (unsigned)sizeof(T));
         }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < Q; i++) f[i] = DEV ? S::raw(v[i]) : S::get(v[i], (R)L::w(i));
     }
     if constexpr (autosym_kind<MODEL> != 0) {
         d2q9_autosym<MODEL>(f, nt, [&](int fi, int ox, int oy) {
@@ -2173,6 +2180,8 @@ __global__ void __launch_bounds__(256) k_batch_step(const BatchCase* __restrict_
             a.fout = (void*)tmp;
         }
         a.gpart = gpart;
+        if constexpr (Store<ST>::stochastic)
+            a.sr_key = sr_key(cases[tl.x].sr_iter + ((unsigned)phase >> 2), cases[tl.x].sr_seed);
         step_tile<MODEL, ST, GLOB>(a, tl.y, GLOB ? a.rpb : a.rpt, t, ntiles);
     }
 }
@@ -2194,6 +2203,8 @@ k_batch_step_w8(const BatchCase* __restrict__ cases, const int2* __restrict__ ti
             a.fout = (void*)tmp;
         }
         a.gpart = gpart;
+        if constexpr (Store<ST>::stochastic)
+            a.sr_key = sr_key(cases[tl.x].sr_iter + ((unsigned)phase >> 2), cases[tl.x].sr_seed);
         step_tile<MODEL, ST, GLOB>(a, tl.y, GLOB ? a.rpb : a.rpt, t, ntiles);
     }
 }
@@ -2547,6 +2558,8 @@ hipError_t batch_m(int st, bool glob, const BatchCase* cases, const int2* tiles,
     case ST_F32S: return batch_ms<MODEL, ST_F32S>(glob, cases, tiles, ntiles, phase, gpart, grid, block, s);
     case ST_F16S: return batch_ms<MODEL, ST_F16S>(glob, cases, tiles, ntiles, phase, gpart, grid, block, s);
     case ST_BF16S: return batch_ms<MODEL, ST_BF16S>(glob, cases, tiles, ntiles, phase, gpart, grid, block, s);
+    case ST_F16S_SR: return batch_ms<MODEL, ST_F16S_SR>(glob, cases, tiles, ntiles, phase, gpart, grid, block, s);
+    case ST_BF16S_SR: return batch_ms<MODEL, ST_BF16S_SR>(glob, cases, tiles, ntiles, phase, gpart, grid, block, s);
 #endif
     }
     return hipErrorInvalidValue;

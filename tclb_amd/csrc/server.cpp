@@ -123,6 +123,15 @@ void Server::SetTuning(const Tuning& t) {
     has_pending_tuning_ = true;
 }
 
+void Server::SetStochasticRounding(bool on, unsigned seed) {
+    if (on && batch_->storage != ST_F16S && batch_->storage != ST_BF16S)
+        throw std::invalid_argument("stochastic rounding applies to the 16-bit formats f16s and bf16s");
+    std::lock_guard<std::mutex> lk(mu_);
+    pending_sr_ = on;
+    pending_sr_seed_ = seed;
+    has_pending_sr_ = true;
+}
+
 // A device or host error ends the worker: every request not finished is
 // marked rejected and the error is kept for Error().
 void Server::loop() {
@@ -156,6 +165,10 @@ void Server::run() {
             if (has_pending_tuning_) {
                 batch_->SetTuning(pending_tuning_);
                 has_pending_tuning_ = false;
+            }
+            if (has_pending_sr_) {
+                batch_->SetStochasticRounding(pending_sr_, pending_sr_seed_);
+                has_pending_sr_ = false;
             }
         }
         // admission: in arrival order, every request that fits the arena now

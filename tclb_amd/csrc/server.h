@@ -84,6 +84,8 @@ public:
     };
     Stats stats();
     void SetTuning(const Tuning& t);
+    // stochastic rounding of the requests' stores (Batch::SetStochasticRounding)
+    void SetStochasticRounding(bool on, unsigned seed);
     double now() const;
     std::string Error();  // message of the error that stopped the worker, "" if none
 
@@ -111,6 +113,8 @@ private:
     std::string error_;
     Tuning pending_tuning_;
     bool has_pending_tuning_ = false;
+    bool has_pending_sr_ = false, pending_sr_ = false;
+    unsigned pending_sr_seed_ = 0;
     Stats stats_;
     std::chrono::steady_clock::time_point t0_;
     std::thread worker_;

@@ -177,6 +177,32 @@ def test_served_f16s_request_tracks_the_oracle(oracle_mod):
         assert abs(a - c) <= bound, (name, a, c, bound)
 
 
+def test_served_request_with_stochastic_rounding_matches_a_lattice():
+    """With stochastic rounding on, a served f16s request ends with the
+    populations and Globals of the same case run as a lone Lattice with the
+    same seed, bit for bit, whatever else shares the server's batch."""
+    say("server: stochastic rounding vs a plain lattice")
+    reqs = _stated_requests()
+    (case, steps), (other, osteps) = reqs[4], reqs[2]
+    s = tclb_amd.Server("d3q19", "f16s", hbm_budget=1 << 30, max_chunk=17)
+    s.set_stochastic_rounding(True, seed=5)
+    rid = s.submit(case, steps)
+    s.submit(other, min(osteps, 3 * steps))
+    assert s.wait_all(120)
+    r = s.poll(rid)
+    assert r["state"] == "done_budget" and r["steps"] == steps
+    s.close()
+    lat = case.build("f16s")
+    lat.set_stochastic_rounding(True, seed=5)
+    lat.iterate(steps, tclb_amd.ITER_NORM | tclb_amd.ITER_LASTGLOB)
+    assert np.array_equal(r["globals"], lat.globals)
+    plain = case.build("f16s")
+    plain.iterate(steps, tclb_amd.ITER_NORM | tclb_amd.ITER_LASTGLOB)
+    assert not np.array_equal(plain.globals, lat.globals)
+    lat.close()
+    plain.close()
+
+
 def test_malformed_request_fails_at_submit():
     """A request the batch cannot hold (a 2D model with nz = 2) fails in
     submit() with a message; the server keeps serving the valid request
