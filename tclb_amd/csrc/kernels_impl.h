@@ -1962,7 +1962,8 @@ __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, 
         // all Q pulls are issued before any is used: the scheduler barrier
         // keeps the compiler from pairing each load with its conversion and
         // waiting on it alone (which it did in the batch step's flat-tile
-        // loop, serializing the pulls)
+        // loop, serializing the pulls).  Not for the 27-velocity models,
+        // whose capped steps would then spill.
         T v[Q];
 #pragma unroll
         for (int i = 0; i < Q; i++) {
@@ -1973,7 +1974,7 @@ __device__ __forceinline__ void node_update(const StepArgs& a, int row, int xc, 
             const unsigned src = (unsigned)sx + (unsigned)nx * ((unsigned)sy + (unsigned)ny * (unsigned)sz);
             v[i] = *(gptr<const T>)(fin + (size_t)i * pbytes + src * (unsigned)sizeof(T));
         }
-        __builtin_amdgcn_sched_barrier(0);
+        if constexpr (Q < 27) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int i = 0; i < Q; i++) f[i] = DEV ? S::raw(v[i]) : S::get(v[i], (R)L::w(i));
     }
