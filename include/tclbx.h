@@ -230,6 +230,23 @@ int tclbx_get_sorted_dispatch(tclbx_lattice* h, int* on, long long* list_nodes);
  * launched step by step (no captured graph).  No counterpart in the reference, which rounds to
  * nearest (src/LatticeAccess.inc.cpp.Rt:14-35). */
 int tclbx_set_stochastic_rounding(tclbx_lattice* h, int on, unsigned seed);
+/* Temporal blocking of the plain steps of the 2D models d2q9_SRT,
+ * d2q9_SRT_drag, d2q9 and d2q9_drag (TCLBX_ERR for the others): `steps`
+ * (2-15) time steps per launch, each block stepping a 32 x 32 tile with
+ * `steps` nodes of halo in LDS and writing its interior; 0 or 1 turns it
+ * off; -1, the default of these models, lets a lattice under 2^23 nodes
+ * time single steps against 3 and 5 per launch at tclbx_prepare or at its
+ * first tclbx_iterate of 64 or more steps, and keep the fastest.  Results
+ * are those of single steps, bit for bit.  Steps with Globals or with
+ * stochastic rounding take the single step.  It pays where one step does
+ * not fill the GPU or the snapshots sit in the Infinity Cache, and loses
+ * where steps stream from HBM.  The environment's TCLBX_TEMPORAL_BLOCKING=k
+ * (k >= 0) fixes it for new lattices of these models.  No counterpart in
+ * the reference, which launches one kernel per iteration
+ * (src/Lattice.cu.Rt:1246-1297). */
+int tclbx_set_temporal_blocking(tclbx_lattice* h, int steps);
+/* the steps per launch in use (0: single steps, also before a timed choice) */
+int tclbx_get_temporal_blocking(tclbx_lattice* h, int* steps);
 
 /* ---------------------------------------------------------------- batch --
  * Many independent lattices ("cases") of one model and storage format in

@@ -39,6 +39,7 @@ SYMBOLS = [
     "tclbx_bytes_per_node", "tclbx_set_block", "tclbx_set_graph_steps", "tclbx_copy_bandwidth",
     "tclbx_set_rows_per_tile", "tclbx_get_rows_per_tile", "tclbx_prepare",
     "tclbx_set_sorted_dispatch", "tclbx_get_sorted_dispatch", "tclbx_set_stochastic_rounding",
+    "tclbx_set_temporal_blocking", "tclbx_get_temporal_blocking",
     "tclbx_batch_set_stochastic_rounding", "tclbx_server_set_stochastic_rounding",
     "tclbx_batch_create", "tclbx_batch_destroy", "tclbx_batch_case_bytes", "tclbx_batch_add_case",
     "tclbx_batch_remove_case", "tclbx_batch_set_setting", "tclbx_batch_set_zone_setting",
@@ -125,6 +126,8 @@ def lib():
     L.tclbx_set_sorted_dispatch.argtypes = [vp, c_int]
     L.tclbx_get_sorted_dispatch.argtypes = [vp, ip, ctypes.POINTER(ctypes.c_longlong)]
     L.tclbx_set_stochastic_rounding.argtypes = [vp, c_int, ctypes.c_uint]
+    L.tclbx_set_temporal_blocking.argtypes = [vp, c_int]
+    L.tclbx_get_temporal_blocking.argtypes = [vp, ip]
     L.tclbx_batch_set_stochastic_rounding.argtypes = [vp, c_int, ctypes.c_uint]
     L.tclbx_server_set_stochastic_rounding.argtypes = [vp, c_int, ctypes.c_uint]
     L.tclbx_copy_bandwidth.argtypes = [ctypes.c_longlong, c_int, dp, dp]
@@ -531,6 +534,20 @@ class Lattice:
     def set_stochastic_rounding(self, on=True, seed=0):
         """Stochastic rounding of the stores of the 16-bit formats."""
         _check(lib().tclbx_set_stochastic_rounding(self._h, int(bool(on)), int(seed) & 0xFFFFFFFF))
+
+    @property
+    def temporal_blocking(self):
+        """Plain steps per launch of the temporally blocked 2D step in use (0:
+        single steps).  Set k >= 2 to fix it, 0 to turn it off, -1 to let
+        the lattice time 1, 3 and 5 steps per launch and keep the fastest
+        (the default of d2q9_SRT, d2q9_SRT_drag, d2q9 and d2q9_drag)."""
+        k = ctypes.c_int()
+        _check(lib().tclbx_get_temporal_blocking(self._h, ctypes.byref(k)))
+        return k.value
+
+    @temporal_blocking.setter
+    def temporal_blocking(self, steps):
+        _check(lib().tclbx_set_temporal_blocking(self._h, int(steps)))
 
     def sorted_list_nodes(self):
         n = ctypes.c_longlong()

@@ -397,6 +397,20 @@ int tclbx_get_sorted_dispatch(tclbx_lattice* h, int* on, long long* list_nodes) 
     });
 }
 
+int tclbx_set_temporal_blocking(tclbx_lattice* h, int steps) {
+    CHECK_H(h);
+    return guard([&]() {
+        h->lat->set_temporal_blocking(steps);
+        return TCLBX_OK;
+    });
+}
+
+int tclbx_get_temporal_blocking(tclbx_lattice* h, int* steps) {
+    CHECK_H(h);
+    if (steps) *steps = h->lat->temporal_blocking();
+    return TCLBX_OK;
+}
+
 int tclbx_set_stochastic_rounding(tclbx_lattice* h, int on, unsigned seed) {
     CHECK_H(h);
     return guard([&]() {

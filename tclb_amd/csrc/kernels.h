@@ -16,6 +16,9 @@ namespace tclbx {
 // d3q27_cumulant and its builds: [nu, nubuffer, GalileanCorrection, ForceX,
 // ForceY, ForceZ, Omega, Smag].
 constexpr int MAX_PAR = 16;
+// side of the tiles of the temporally blocked step (nodes, one per thread);
+// it takes at most TB_TILE / 2 - 1 steps per launch
+constexpr int TB_TILE = 32;
 constexpr int MAX_GLOBALS = 32;
 
 struct StepArgs {
@@ -42,11 +45,14 @@ struct StepArgs {
     // the tiles; mode 1 (bulk) only the nodes whose class (NodeType bits
     // below the zone) is bulk_val, the plain collision node, so that every
     // wavefront takes the same branch; mode 2 the list[0..list_n) of the
-    // other nodes, sorted by class, one thread each
+    // other nodes, sorted by class, one thread each.  Mode 3: tb_k plain
+    // steps in one launch of the temporally blocked step (2D models), tiles
+    // of TB_TILE x TB_TILE nodes, tb_bx of them along x (set by the launch)
     int mode;
     unsigned class_mask, bulk_val;
     const unsigned* list;
     unsigned list_n;
+    int tb_k, tb_bx;
     // stochastic rounding of the stores of a 16-bit format (0: nearest even);
     // sr_key = sr_key(iteration the step computes, the lattice's seed)
     int sr;

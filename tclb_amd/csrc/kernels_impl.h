@@ -2142,6 +2142,98 @@ k_step_list(StepArgs a) {
     node_update<MODEL, ST, false, 0, true>(a, row, x, xm, xp, in, g);
 }
 
+// ------------------------------------------------- temporally blocked step ---
+// 2D models whose node update reads nothing but its own node's pulled
+// populations, flags and zone (no mirror nodes, no densities updated in
+// place): d2q9_SRT, d2q9_SRT_drag, d2q9 and d2q9_drag.
+template <int MODEL>
+constexpr bool tb_model = MODEL == MODEL_D2Q9_SRT || MODEL == MODEL_D2Q9_SRT_DRAG || MODEL == MODEL_D2Q9 ||
+                          MODEL == MODEL_D2Q9_DRAG;
+
+// K = a.tb_k plain steps of a 2D lattice in one launch.  Block b takes the
+// TX x TY tile whose interior is the (TX - 2K) x (TY - 2K) nodes from
+// ((b % tb_bx)(TX - 2K), (b / tb_bx)(TY - 2K)), K nodes of halo on every
+// side (periodic wrap), loads its populations of the current snapshot into
+// LDS and steps the tile K times there.  After step s the nodes at least s
+// from the tile's edge hold exact values (their pulls reached only nodes
+// that held exact values), so the last step writes the interior to the
+// next snapshot.  Every pull of a step is read before any node writes its
+// own populations (a barrier between), so one LDS copy of the tile serves
+// all K steps.  Each value written to LDS is rounded to the storage format
+// exactly as the single step rounds its stores, and the node update is the
+// single step's own: K launches of k_step and one of this give the same
+// bits.  The halo is computed again by the neighbouring blocks; the launch
+// saves K - 1 round trips through memory and K - 1 launch latencies, which
+// pays where a lattice is too small for one step to fill the GPU.
+template <int MODEL, int ST, int TX, int TY>
+__global__ void __launch_bounds__(TX * TY) k_step_tb(StepArgs a) {
+    using L = Lat<MODEL>;
+    using S = Store<ST>;
+    using T = typename S::T;
+    using R = typename S::R;
+    constexpr int Q = 9;
+    constexpr bool DEV = S::shifted;
+    static_assert(tb_model<MODEL> && L::Q == 9 && !S::stochastic, "2D models without stochastic rounding");
+    __shared__ T tile[Q][TY][TX];
+    const int K = a.tb_k;
+    const int nx = a.nx, ny = a.ny;
+    const int bx = blockIdx.x % a.tb_bx, by = blockIdx.x / a.tb_bx;
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    // this thread's tile node: unwrapped coordinates (ux, uy), lattice node (x, y)
+    const int ux = bx * (TX - 2 * K) - K + tx, uy = by * (TY - 2 * K) - K + ty;
+    int x = ux % nx, y = uy % ny;
+    if (x < 0) x += nx;
+    if (y < 0) y += ny;
+    const unsigned n = (unsigned)x + (unsigned)nx * (unsigned)y;
+    const unsigned nt = *(gptr<const uint16_t>)(a.flags + n);
+    int zone = (int)(nt >> L::ZSHIFT);
+    if (zone >= a.nzones) zone = 0;
+    const double* zr = a.zone + (size_t)zone * a.zcols;
+    const size_t pbytes = (size_t)a.pitch * sizeof(T);
+    const char* fin = (const char*)a.fin;
+#pragma unroll
+    for (int i = 0; i < Q; i++) tile[i][ty][tx] = *(gptr<const T>)(fin + (size_t)i * pbytes + n * (unsigned)sizeof(T));
+    // the interior node this thread writes at the end (each lattice node is
+    // the interior node of exactly one tile)
+    const bool writes = tx >= K && tx < TX - K && ty >= K && ty < TY - K && ux < nx && uy < ny;
+    double g[1];
+    for (int s = 1; s <= K; s++) {
+        __syncthreads();
+        R f[Q];
+#pragma unroll
+        for (int i = 0; i < Q; i++) {
+            // the pull of the tile's edge nodes is clamped into the tile:
+            // their values are never used
+            const int sx = min(max(tx - L::ex(i), 0), TX - 1);
+            const int sy = min(max(ty - L::ey(i), 0), TY - 1);
+            const T v = tile[i][sy][sx];
+            f[i] = DEV ? S::raw(v) : S::get(v, (R)L::w(i));
+        }
+        __syncthreads();
+        if constexpr (is_d2q9m<MODEL> && DEV)
+            d2q9m_node_dev<false, MODEL == MODEL_D2Q9_DRAG, false>(f, nt, a.par, zr, g, nullptr, 0);
+        else if constexpr (is_d2q9m<MODEL>)
+            d2q9m_node<R, false, MODEL == MODEL_D2Q9_DRAG, false>(f, nt, a.par, zr, g, nullptr, 0);
+        else if constexpr (DEV)
+            d2q9_node_dev<MODEL == MODEL_D2Q9_SRT_DRAG, false>(f, nt, (R)a.par[0], (R)a.par[1], zr, g);
+        else
+            d2q9_node<R, MODEL == MODEL_D2Q9_SRT_DRAG, false>(f, nt, (R)a.par[0], (R)a.par[1], zr, g);
+        if (s < K) {
+#pragma unroll
+            for (int i = 0; i < Q; i++) tile[i][ty][tx] = DEV ? S::unraw(f[i]) : S::put(f[i], (R)L::w(i));
+        } else if (writes) {
+            char* fout = (char*)a.fout;
+#pragma unroll
+            for (int i = 0; i < Q; i++)
+                __builtin_nontemporal_store(DEV ? S::unraw(f[i]) : S::put(f[i], (R)L::w(i)),
+                                            (gptr<T>)(fout + (size_t)i * pbytes + n * (unsigned)sizeof(T)));
+        }
+    }
+}
+
+// tile of the temporally blocked step: 32 x 32 nodes, one per thread
+constexpr int TB_TX = TB_TILE, TB_TY = TB_TILE;
+
 // waves per SIMD the plain step of MODEL in storage ST is held to (0: none)
 template <int MODEL, int ST>
 constexpr int step_waves() {
@@ -2451,7 +2543,18 @@ inline int nblocks_of(const StepArgs& a, int block) { return a.bpr * a.ny * a.nz
 template <int MODEL, int ST>
 hipError_t step_ms(bool glob, const StepArgs& a, int block, hipStream_t s) {
     dim3 grid(step_blocks(a, glob, block)), blk(block);
-    if (!glob && a.mode == 1) {
+    if (!glob && a.mode == 3) {
+        if constexpr (tb_model<MODEL> && !Store<ST>::stochastic) {
+            if (a.tb_k < 2 || 2 * a.tb_k >= TB_TX || 2 * a.tb_k >= TB_TY || a.nz != 1) return hipErrorInvalidValue;
+            StepArgs b = a;
+            b.tb_bx = (a.nx + TB_TX - 2 * a.tb_k - 1) / (TB_TX - 2 * a.tb_k);
+            const int by = (a.ny + TB_TY - 2 * a.tb_k - 1) / (TB_TY - 2 * a.tb_k);
+            hipLaunchKernelGGL((k_step_tb<MODEL, ST, TB_TX, TB_TY>), dim3(b.tb_bx * by), dim3(TB_TX * TB_TY), 0, s, b);
+            return hipGetLastError();
+        } else {
+            return hipErrorInvalidValue;
+        }
+    } else if (!glob && a.mode == 1) {
         hipLaunchKernelGGL((k_step_bulk<MODEL, ST, step_waves<MODEL, ST>()>), grid, blk, 0, s, a);
     } else if (!glob && a.mode == 2) {
         if (a.list_n == 0) return hipSuccess;

@@ -110,6 +110,12 @@ Lattice::Lattice(const ModelDef* m, int nx_, int ny_, int nz_, int storage_)
     if (!coll) coll = m->nodetype("BGK");
     bulk_val_ = coll ? coll->value : 0u;
     if (const char* e = std::getenv("TCLBX_SORTED_DISPATCH")) sorted_ = std::atoi(e) != 0 && coll != nullptr;
+    tb_auto_ = temporal_blocking_model(m->id);
+    if (const char* e = std::getenv("TCLBX_TEMPORAL_BLOCKING"))
+        if (tb_auto_ && std::atoi(e) >= 0) {
+            tb_k_ = std::min(TB_TILE / 2 - 1, std::atoi(e));
+            tb_auto_ = false;
+        }
     hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
 }
 
@@ -304,6 +310,78 @@ void Lattice::set_stochastic_rounding(bool on, unsigned seed) {
     drop_graph();
 }
 
+bool temporal_blocking_model(int id) {
+    return id == MODEL_D2Q9_SRT || id == MODEL_D2Q9_SRT_DRAG || id == MODEL_D2Q9 || id == MODEL_D2Q9_DRAG;
+}
+
+void Lattice::set_temporal_blocking(int k) {
+    if (k < -1 || k > TB_TILE / 2 - 1)
+        throw std::invalid_argument("temporal blocking takes -1 (timed) or 0-" + std::to_string(TB_TILE / 2 - 1) +
+                                    " steps per launch");
+    if ((k > 1 || k == -1) && !temporal_blocking_model(model->id))
+        throw std::invalid_argument("temporal blocking applies to d2q9_SRT, d2q9_SRT_drag, d2q9 and d2q9_drag, not " +
+                                    model->name);
+    tb_auto_ = k == -1;
+    tb_k_ = k == -1 ? 0 : k;
+    drop_graph();
+}
+
+// Times single steps against launches of 3 and of 5 blocked steps and
+// keeps the fastest per step.  Every trial launch reads the current
+// snapshot and writes the other one, which the next real step overwrites:
+// the state does not change.  Blocking wins where a lattice is too small
+// for one step to fill the GPU or its snapshots sit in the Infinity Cache,
+// and loses on lattices whose steps stream from HBM (the halo is computed
+// again, at lower occupancy); which k is best depends on the storage format
+// (the FP64 tile allows one block per CU) and on how the tiles divide over
+// the CUs.
+void Lattice::tune_blocking() {
+    tb_auto_ = false;
+    float best = 0;
+    int best_k = 0;
+    for (int k : {0, 3, 5}) {
+        tb_k_ = k;
+        const int steps = k > 1 ? k : 1;
+        const int launches = k > 1 ? 4 : 12;
+        auto once = [&]() {
+            if (k > 1)
+                launch_blocked(snap_, 1 - snap_);
+            else
+                launch(snap_, 1 - snap_, false);
+        };
+        once();  // warm-up
+        hip_check(hipEventRecord(ev0_, stream_), "event record");
+        for (int j = 0; j < launches; j++) once();
+        hip_check(hipEventRecord(ev1_, stream_), "event record");
+        hip_check(hipStreamSynchronize(stream_), "tune blocking");
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, ev0_, ev1_), "event elapsed");
+        const float per_step = ms / (float)(launches * steps);
+        if (best == 0 || per_step < best) {
+            best = per_step;
+            best_k = k;
+        }
+    }
+    tb_k_ = best_k;
+    drop_graph();
+}
+
+// steps of one replay of the captured loop: graph_steps_, or with temporal
+// blocking an even number (at least two) of blocked launches
+int Lattice::graph_len() const {
+    const int k = blocked_k();
+    if (k == 1) return graph_steps_;
+    return std::max(2, (graph_steps_ / k) & ~1) * k;
+}
+
+// blocked_k() plain steps in one launch, snapshot `from` -> `to`
+void Lattice::launch_blocked(int from, int to) {
+    StepArgs a = args(from, to, false);
+    a.mode = 3;
+    a.tb_k = tb_k_;
+    hip_check(launch_step(model->id, storage, false, a, block_, stream_), "temporally blocked step");
+}
+
 void Lattice::set_sorted_dispatch(bool on) {
     if (on && !model->nodetype("MRT") && !model->nodetype("BGK"))
         throw std::invalid_argument("model " + model->name + " has no plain collision NodeType to sort around");
@@ -434,10 +512,15 @@ void Lattice::run_graph_steps(int nsteps) {
     if (graph_exec_ == nullptr || graph_version_ != params_version_) {
         drop_graph();
         hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture");
-        for (int i = 0; i < graph_steps_; i++) {
+        const int k = blocked_k();
+        for (int i = 0; i < graph_len() / k; i++) {
             StepArgs a = args(i % 2, (i + 1) % 2, false);
             hipError_t e = hipSuccess;
-            if (sorted_) {
+            if (k > 1) {
+                a.mode = 3;
+                a.tb_k = k;
+                e = launch_step(model->id, storage, false, a, block_, stream_);
+            } else if (sorted_) {
                 a.mode = 1;
                 e = launch_step(model->id, storage, false, a, block_, stream_);
                 a.mode = 2;
@@ -456,7 +539,7 @@ void Lattice::run_graph_steps(int nsteps) {
         hip_check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "graph instantiate");
         graph_version_ = params_version_;
     }
-    for (int done = 0; done < nsteps; done += graph_steps_)
+    for (int done = 0; done < nsteps; done += graph_len())
         hip_check(hipGraphLaunch(graph_exec_, stream_), "graph launch");
 }
 
@@ -520,6 +603,7 @@ void Lattice::Prepare() {
     upload_zone();
     ensure_list();
     if (tune_rpt_ && N >= (1u << 20)) tune_rows();
+    if (blocking_to_tune()) tune_blocking();
     if (graph_steps_ >= 2 && !sr_) {
         run_graph_steps(0);  // capture only
         hip_check(hipStreamSynchronize(stream_), "Prepare");
@@ -532,8 +616,10 @@ void Lattice::Iterate(int niter, int iter_type) {
     ensure_list();
     // lattices large enough for the tiling to matter are tuned once, on
     // the first Iterate that runs enough steps to repay the trial launches
-    // (or earlier, by Prepare)
+    // (or earlier, by Prepare); so is the temporal blocking of the 2D
+    // lattices it may pay on
     if (tune_rpt_ && niter >= 16 && N >= (1u << 20)) tune_rows();
+    if (blocking_to_tune() && niter >= 64) tune_blocking();
     int last_glob = iter_type & ITER_LASTGLOB;
     if (iter_type & ITER_INTEG) last_glob = 0;
     const bool every_glob = (iter_type & ITER_GLOBS) != 0;
@@ -545,18 +631,28 @@ void Lattice::Iterate(int niter, int iter_type) {
     // least twice
     const int plain_end = niter - (last_glob ? 1 : 0);
     const bool captured = graph_exec_ != nullptr && graph_version_ == params_version_;
-    const int lead = snap_ != 0 ? 1 : 0;
-    if (!every_glob && !sr_ && graph_steps_ >= 2 && plain_end - lead >= (captured ? 1 : 2) * graph_steps_) {
+    const int k = every_glob ? 1 : blocked_k();
+    const int lead = snap_ != 0 ? k : 0;
+    const int glen = graph_len();
+    if (!every_glob && !sr_ && graph_steps_ >= 2 && plain_end - lead >= (captured ? 1 : 2) * glen) {
         if (snap_ != 0) {
-            launch(snap_, 1 - snap_, false, iter_ + i);
+            if (k > 1)
+                launch_blocked(snap_, 1 - snap_);
+            else
+                launch(snap_, 1 - snap_, false, iter_ + i);
             snap_ = 1 - snap_;
-            i++;
+            i += k;
         }
-        const int reps = (plain_end - i) / graph_steps_;
+        const int reps = (plain_end - i) / glen;
         if (reps > 0) {
-            run_graph_steps(reps * graph_steps_);
-            i += reps * graph_steps_;
+            run_graph_steps(reps * glen);
+            i += reps * glen;
         }
+    }
+    // the remaining plain steps: blocked launches while k are left
+    for (; k > 1 && plain_end - i >= k; i += k) {
+        launch_blocked(snap_, 1 - snap_);
+        snap_ = 1 - snap_;
     }
     for (; i < niter; i++) {
         const bool glob = every_glob || (last_glob && i == niter - 1);

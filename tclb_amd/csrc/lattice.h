@@ -103,6 +103,14 @@ public:
     // step loop is then launched directly, since every step's launch differs
     void set_stochastic_rounding(bool on, unsigned seed);
     bool stochastic_rounding() const { return sr_; }
+    // temporal blocking of the plain steps of the 2D models d2q9_SRT,
+    // d2q9_SRT_drag, d2q9 and d2q9_drag: k (2 to TB_TILE/2 - 1) steps per
+    // launch on tiles held in LDS, bit-identical to k single steps; 0 or 1:
+    // off; -1 (the default of these models): chosen by timing (tune_blocking)
+    // on lattices under 2^23 nodes.  Steps with Globals and steps with
+    // stochastic rounding take the single step.
+    void set_temporal_blocking(int k);
+    int temporal_blocking() const { return tb_k_; }
 
     const ModelDef* model;
     const int nx, ny, nz, storage;
@@ -113,8 +121,13 @@ private:
     StepArgs args(int from, int to, bool glob) const;
     void upload_zone();
     void launch(int from, int to, bool glob, int step = 0);
+    void launch_blocked(int from, int to);
+    int blocked_k() const { return tb_k_ > 1 && !sr_ ? tb_k_ : 1; }
+    int graph_len() const;
     void drop_graph();
     void tune_rows();
+    void tune_blocking();
+    bool blocking_to_tune() const { return tb_auto_ && !sr_ && N < (1u << 23); }
     void run_graph_steps(int nsteps);
     void ensure_list();
 
@@ -136,7 +149,7 @@ private:
     hipStream_t stream_ = nullptr;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     double last_ms_ = 0, kernel_ms_ = 0;
-    // captured step loop: graph_steps_ plain steps starting at snapshot 0
+    // captured step loop: graph_len() plain steps starting at snapshot 0
     hipGraphExec_t graph_exec_ = nullptr;
     hipGraph_t graph_ = nullptr;
     int graph_steps_ = 32;
@@ -152,8 +165,12 @@ private:
     unsigned flags_version_ = 0, list_version_ = ~0u;
     bool sr_ = false;
     unsigned sr_seed_ = 0;
+    int tb_k_ = 0;
+    bool tb_auto_ = false;
 };
 
+// the models the temporally blocked step covers
+bool temporal_blocking_model(int model_id);
 // hipError_t -> exception with a message naming the call
 void hip_check(hipError_t e, const char* what);
 // steps per captured graph of a new Lattice or Batch: 32, or the
