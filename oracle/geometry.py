@@ -21,9 +21,9 @@ numpy over each primitive's box of nodes), what the reference computes:
     src/spline.h:9-43.
 
 Pinned in tests/test_geometry.py by hand-derived answers and closed forms
-(each test names what it pins).  Parity unpinned: the STL tie-break for a
-ray through an edge shared by two triangles, which only engine-oracle
-agreement exercises.
+(each test names what it pins); the STL tie-break for rays through the
+edges and vertices of a mesh by a closed-form octahedron whose vertices lie
+on grid lines (test_stl_rays_through_edges_closed_form).
 """
 import math
 import xml.etree.ElementTree as ET

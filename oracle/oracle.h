@@ -47,7 +47,10 @@
  * are at least as accurate as any summation order of the device.
  *
  * Pinned in tests/test_oracle_pins.py by closed forms, independent
- * derivations and brute force (each test names what it pins).  Parity
+ * derivations and brute force (each test names what it pins); the cumulant
+ * model's Galilean-correction terms by what they are for, the U^2 defect
+ * they remove from a longitudinal wave's decay under a mean flow
+ * (test_cumulant_galilean_correction_restores_galilean_invariance).  Parity
  * unpinned -- nothing the reference holds fixes these, since its golden
  * data (tests/external) is absent and its generator (R polyAlgebra) cannot
  * run here:
@@ -55,8 +58,6 @@
  *     the generated models (d3q19 CollisionMRT / SetEquilibrum / Zou-He, the
  *     d2q9 MRT, d3q27 and cumulant collisions and Zou-He); the oracle follows
  *     the dense forms in ascending index order, within the FP64 tolerance;
- *   - the cumulant model's Galilean-correction coefficients beyond the
- *     identities the pins check (zero at rest, diagonal only);
  *   - the reference float build's literal rewrite (Dynamics_sp.c,
  *     src/makefile.main.Rt:383-386), which the reduced formats do not follow.
  */
