@@ -619,10 +619,10 @@ void Lattice::Iterate(int niter, int iter_type) {
     // (or earlier, by Prepare); so is the temporal blocking of the 2D
     // lattices it may pay on
     if (tune_rpt_ && niter >= 16 && N >= (1u << 20)) tune_rows();
-    if (blocking_to_tune() && niter >= 64) tune_blocking();
     int last_glob = iter_type & ITER_LASTGLOB;
     if (iter_type & ITER_INTEG) last_glob = 0;
     const bool every_glob = (iter_type & ITER_GLOBS) != 0;
+    if (blocking_to_tune() && !every_glob && niter >= 64) tune_blocking();
     hip_check(hipEventRecord(ev0_, stream_), "event record");
     int i = 0;
     // leading plain steps until snapshot 0 is current, then whole graph
