@@ -327,42 +327,47 @@ void Lattice::set_temporal_blocking(int k) {
 }
 
 // Times single steps against launches of 3 and of 5 blocked steps and
-// keeps the fastest per step.  Every trial launch reads the current
-// snapshot and writes the other one, which the next real step overwrites:
-// the state does not change.  Blocking wins where a lattice is too small
-// for one step to fill the GPU or its snapshots sit in the Infinity Cache,
-// and loses on lattices whose steps stream from HBM (the halo is computed
-// again, at lower occupancy); which k is best depends on the storage format
-// (the FP64 tile allows one block per CU) and on how the tiles divide over
-// the CUs.
+// keeps the fastest per step: three rounds over the candidates in turn,
+// each candidate's best round counting, so a transient (another stream's
+// work, a clock change) during one short sample does not decide.  Every
+// trial launch reads the current snapshot and writes the other one, which
+// the next real step overwrites: the state does not change.  Blocking wins
+// where a lattice is too small for one step to fill the GPU or its
+// snapshots sit in the Infinity Cache, and loses on lattices whose steps
+// stream from HBM (the halo is computed again, at lower occupancy); which
+// k is best depends on the storage format (the FP64 tile allows one block
+// per CU) and on how the tiles divide over the CUs.
 void Lattice::tune_blocking() {
     tb_auto_ = false;
-    float best = 0;
-    int best_k = 0;
-    for (int k : {0, 3, 5}) {
-        tb_k_ = k;
-        const int steps = k > 1 ? k : 1;
-        const int launches = k > 1 ? 4 : 12;
-        auto once = [&]() {
-            if (k > 1)
-                launch_blocked(snap_, 1 - snap_);
-            else
-                launch(snap_, 1 - snap_, false);
-        };
-        once();  // warm-up
-        hip_check(hipEventRecord(ev0_, stream_), "event record");
-        for (int j = 0; j < launches; j++) once();
-        hip_check(hipEventRecord(ev1_, stream_), "event record");
-        hip_check(hipStreamSynchronize(stream_), "tune blocking");
-        float ms = 0;
-        hip_check(hipEventElapsedTime(&ms, ev0_, ev1_), "event elapsed");
-        const float per_step = ms / (float)(launches * steps);
-        if (best == 0 || per_step < best) {
-            best = per_step;
-            best_k = k;
+    const int ks[3] = {0, 3, 5};
+    float best[3] = {0, 0, 0};
+    for (int round = 0; round < 3; round++) {
+        for (int c = 0; c < 3; c++) {
+            const int k = ks[c];
+            tb_k_ = k;
+            const int steps = k > 1 ? k : 1;
+            const int launches = k > 1 ? 4 : 12;
+            auto once = [&]() {
+                if (k > 1)
+                    launch_blocked(snap_, 1 - snap_);
+                else
+                    launch(snap_, 1 - snap_, false);
+            };
+            once();  // warm-up
+            hip_check(hipEventRecord(ev0_, stream_), "event record");
+            for (int j = 0; j < launches; j++) once();
+            hip_check(hipEventRecord(ev1_, stream_), "event record");
+            hip_check(hipStreamSynchronize(stream_), "tune blocking");
+            float ms = 0;
+            hip_check(hipEventElapsedTime(&ms, ev0_, ev1_), "event elapsed");
+            const float per_step = ms / (float)(launches * steps);
+            if (round == 0 || per_step < best[c]) best[c] = per_step;
         }
     }
-    tb_k_ = best_k;
+    int bi = 0;
+    for (int c = 1; c < 3; c++)
+        if (best[c] < best[bi]) bi = c;
+    tb_k_ = ks[bi];
     drop_graph();
 }
 
