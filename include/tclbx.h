@@ -237,8 +237,8 @@ int tclbx_set_stochastic_rounding(tclbx_lattice* h, int on, unsigned seed);
  * off; -1, the default of these models, lets a lattice under 2^23 nodes
  * time single steps against 3 and 5 per launch at tclbx_prepare or at its
  * first tclbx_iterate of 64 or more steps, and keep the fastest.  Results
- * are those of single steps, bit for bit.  Steps with Globals or with
- * stochastic rounding take the single step.  It pays where one step does
+ * are those of single steps, bit for bit, stochastic rounding included.
+ * Steps with Globals take the single step.  It pays where one step does
  * not fill the GPU or the snapshots sit in the Infinity Cache, and loses
  * where steps stream from HBM.  The environment's TCLBX_TEMPORAL_BLOCKING=k
  * (k >= 0) fixes it for new lattices of these models.  No counterpart in

@@ -57,6 +57,9 @@ struct StepArgs {
     // sr_key = sr_key(iteration the step computes, the lattice's seed)
     int sr;
     unsigned sr_key;
+    // a launch of several steps (the temporally blocked step) keys step s
+    // of the launch itself: sr_key(sr_step + s, sr_seed)
+    unsigned sr_step, sr_seed;
     double par[MAX_PAR];
     // float forms the float collisions take, derived from par on the host
     // (model_params)

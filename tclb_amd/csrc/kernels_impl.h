@@ -2150,6 +2150,22 @@ template <int MODEL>
 constexpr bool tb_model = MODEL == MODEL_D2Q9_SRT || MODEL == MODEL_D2Q9_SRT_DRAG || MODEL == MODEL_D2Q9 ||
                           MODEL == MODEL_D2Q9_DRAG;
 
+// The stored form of population i, rounded as the single step rounds its
+// store; with stochastic rounding direction 2j takes word j of the node's
+// stream and 2j + 1 word j << 16 (w advances after each odd direction).
+template <class S, class L, bool DEV>
+__device__ __forceinline__ typename S::T tb_stored(typename S::R v, int i, uint32_t& w) {
+    using R = typename S::R;
+    if constexpr (S::stochastic) {
+        const uint32_t r = (i & 1) ? w << 16 : w;
+        const typename S::T out = DEV ? S::unraw_sr(v, r) : S::put_sr(v, (R)L::w(i), r);
+        if (i & 1) w = xorshift32(w);
+        return out;
+    } else {
+        return DEV ? S::unraw(v) : S::put(v, (R)L::w(i));
+    }
+}
+
 // K = a.tb_k plain steps of a 2D lattice in one launch.  Block b takes the
 // TX x TY tile whose interior is the (TX - 2K) x (TY - 2K) nodes from
 // ((b % tb_bx)(TX - 2K), (b / tb_bx)(TY - 2K)), K nodes of halo on every
@@ -2161,10 +2177,12 @@ constexpr bool tb_model = MODEL == MODEL_D2Q9_SRT || MODEL == MODEL_D2Q9_SRT_DRA
 // own populations (a barrier between), so one LDS copy of the tile serves
 // all K steps.  Each value written to LDS is rounded to the storage format
 // exactly as the single step rounds its stores, and the node update is the
-// single step's own: K launches of k_step and one of this give the same
-// bits.  The halo is computed again by the neighbouring blocks; the launch
-// saves K - 1 round trips through memory and K - 1 launch latencies, which
-// pays where a lattice is too small for one step to fill the GPU.
+// single step's own (stochastic rounding included: step s draws the words
+// of its own iteration): K launches of k_step and one of this give the
+// same bits.  The halo is computed again by the neighbouring blocks; the
+// launch saves K - 1 round trips through memory and K - 1 launch
+// latencies, which pays where a lattice is too small for one step to fill
+// the GPU.
 template <int MODEL, int ST, int TX, int TY>
 __global__ void __launch_bounds__(TX * TY) k_step_tb(StepArgs a) {
     using L = Lat<MODEL>;
@@ -2173,7 +2191,7 @@ __global__ void __launch_bounds__(TX * TY) k_step_tb(StepArgs a) {
     using R = typename S::R;
     constexpr int Q = 9;
     constexpr bool DEV = S::shifted;
-    static_assert(tb_model<MODEL> && L::Q == 9 && !S::stochastic, "2D models without stochastic rounding");
+    static_assert(tb_model<MODEL> && L::Q == 9, "2D models");
     __shared__ T tile[Q][TY][TX];
     const int K = a.tb_k;
     const int nx = a.nx, ny = a.ny;
@@ -2218,14 +2236,18 @@ __global__ void __launch_bounds__(TX * TY) k_step_tb(StepArgs a) {
             d2q9_node_dev<MODEL == MODEL_D2Q9_SRT_DRAG, false>(f, nt, (R)a.par[0], (R)a.par[1], zr, g);
         else
             d2q9_node<R, MODEL == MODEL_D2Q9_SRT_DRAG, false>(f, nt, (R)a.par[0], (R)a.par[1], zr, g);
+        // stochastic rounding: the words of this node at the iteration
+        // step s computes, as the single step of that iteration draws them
+        uint32_t w = 0;
+        if constexpr (S::stochastic) w = sr_word0(n, sr_key(a.sr_step + (unsigned)(s - 1), a.sr_seed));
         if (s < K) {
 #pragma unroll
-            for (int i = 0; i < Q; i++) tile[i][ty][tx] = DEV ? S::unraw(f[i]) : S::put(f[i], (R)L::w(i));
+            for (int i = 0; i < Q; i++) tile[i][ty][tx] = tb_stored<S, L, DEV>(f[i], i, w);
         } else if (writes) {
             char* fout = (char*)a.fout;
 #pragma unroll
             for (int i = 0; i < Q; i++)
-                __builtin_nontemporal_store(DEV ? S::unraw(f[i]) : S::put(f[i], (R)L::w(i)),
+                __builtin_nontemporal_store(tb_stored<S, L, DEV>(f[i], i, w),
                                             (gptr<T>)(fout + (size_t)i * pbytes + n * (unsigned)sizeof(T)));
         }
     }
@@ -2544,7 +2566,7 @@ template <int MODEL, int ST>
 hipError_t step_ms(bool glob, const StepArgs& a, int block, hipStream_t s) {
     dim3 grid(step_blocks(a, glob, block)), blk(block);
     if (!glob && a.mode == 3) {
-        if constexpr (tb_model<MODEL> && !Store<ST>::stochastic) {
+        if constexpr (tb_model<MODEL>) {
             if (a.tb_k < 2 || 2 * a.tb_k >= TB_TX || 2 * a.tb_k >= TB_TY || a.nz != 1) return hipErrorInvalidValue;
             StepArgs b = a;
             b.tb_bx = (a.nx + TB_TX - 2 * a.tb_k - 1) / (TB_TX - 2 * a.tb_k);

@@ -327,40 +327,48 @@ void Lattice::set_temporal_blocking(int k) {
 }
 
 // Times single steps against launches of 3 and of 5 blocked steps and
-// keeps the fastest per step: three rounds over the candidates in turn,
-// each candidate's best round counting, so a transient (another stream's
-// work, a clock change) during one short sample does not decide.  Every
-// trial launch reads the current snapshot and writes the other one, which
-// the next real step overwrites: the state does not change.  Blocking wins
-// where a lattice is too small for one step to fill the GPU or its
-// snapshots sit in the Infinity Cache, and loses on lattices whose steps
-// stream from HBM (the halo is computed again, at lower occupancy); which
-// k is best depends on the storage format (the FP64 tile allows one block
-// per CU) and on how the tiles divide over the CUs.
+// keeps the fastest per step.  The GPU first runs about 2 ms of single
+// steps, so its clocks have risen before any sample: at the low clocks of
+// a GPU that has just idled, the compute-heavy FP64 blocked step slows more
+// than the latency-bound single step and lost samples it wins at speed.
+// Then three rounds over the candidates in turn, each sample about 0.5 ms
+// of steps, each candidate's best round counting.  Every trial launch reads
+// the current snapshot and writes the other one, which the next real step
+// overwrites: the state does not change.  Blocking wins where a lattice is
+// too small for one step to fill the GPU or its snapshots sit in the
+// Infinity Cache, and loses on lattices whose steps stream from HBM (the
+// halo is computed again, at lower occupancy); which k is best depends on
+// the storage format (the FP64 tile allows one block per CU) and on how the
+// tiles divide over the CUs.
 void Lattice::tune_blocking() {
     tb_auto_ = false;
+    auto timed = [&](int k, int launches) {
+        tb_k_ = k;
+        hip_check(hipEventRecord(ev0_, stream_), "event record");
+        for (int j = 0; j < launches; j++) {
+            if (k > 1)
+                launch_blocked(snap_, 1 - snap_);
+            else
+                launch(snap_, 1 - snap_, false);
+        }
+        hip_check(hipEventRecord(ev1_, stream_), "event record");
+        hip_check(hipStreamSynchronize(stream_), "tune blocking");
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, ev0_, ev1_), "event elapsed");
+        return ms;
+    };
+    // the single step's time sizes the warm-up and the samples
+    const float t1 = std::max(timed(0, 8) / 8.0f, 1e-4f);
+    auto count = [&](float ms) { return (int)std::min(16384.0f, std::max(8.0f, ms / t1)); };
+    timed(0, count(2.0f));
     const int ks[3] = {0, 3, 5};
     float best[3] = {0, 0, 0};
     for (int round = 0; round < 3; round++) {
         for (int c = 0; c < 3; c++) {
             const int k = ks[c];
-            tb_k_ = k;
             const int steps = k > 1 ? k : 1;
-            const int launches = k > 1 ? 4 : 12;
-            auto once = [&]() {
-                if (k > 1)
-                    launch_blocked(snap_, 1 - snap_);
-                else
-                    launch(snap_, 1 - snap_, false);
-            };
-            once();  // warm-up
-            hip_check(hipEventRecord(ev0_, stream_), "event record");
-            for (int j = 0; j < launches; j++) once();
-            hip_check(hipEventRecord(ev1_, stream_), "event record");
-            hip_check(hipStreamSynchronize(stream_), "tune blocking");
-            float ms = 0;
-            hip_check(hipEventElapsedTime(&ms, ev0_, ev1_), "event elapsed");
-            const float per_step = ms / (float)(launches * steps);
+            const int launches = std::max(2, count(0.5f) / steps);
+            const float per_step = timed(k, launches) / (float)(launches * steps);
             if (round == 0 || per_step < best[c]) best[c] = per_step;
         }
     }
@@ -379,11 +387,14 @@ int Lattice::graph_len() const {
     return std::max(2, (graph_steps_ / k) & ~1) * k;
 }
 
-// blocked_k() plain steps in one launch, snapshot `from` -> `to`
-void Lattice::launch_blocked(int from, int to) {
+// blocked_k() plain steps in one launch, snapshot `from` -> `to`; the first
+// computes iteration `step` (the key of stochastic rounding)
+void Lattice::launch_blocked(int from, int to, int step) {
     StepArgs a = args(from, to, false);
     a.mode = 3;
     a.tb_k = tb_k_;
+    a.sr_step = (unsigned)step;
+    a.sr_seed = sr_seed_;
     hip_check(launch_step(model->id, storage, false, a, block_, stream_), "temporally blocked step");
 }
 
@@ -642,7 +653,7 @@ void Lattice::Iterate(int niter, int iter_type) {
     if (!every_glob && !sr_ && graph_steps_ >= 2 && plain_end - lead >= (captured ? 1 : 2) * glen) {
         if (snap_ != 0) {
             if (k > 1)
-                launch_blocked(snap_, 1 - snap_);
+                launch_blocked(snap_, 1 - snap_, iter_ + i);
             else
                 launch(snap_, 1 - snap_, false, iter_ + i);
             snap_ = 1 - snap_;
@@ -656,7 +667,7 @@ void Lattice::Iterate(int niter, int iter_type) {
     }
     // the remaining plain steps: blocked launches while k are left
     for (; k > 1 && plain_end - i >= k; i += k) {
-        launch_blocked(snap_, 1 - snap_);
+        launch_blocked(snap_, 1 - snap_, iter_ + i);
         snap_ = 1 - snap_;
     }
     for (; i < niter; i++) {

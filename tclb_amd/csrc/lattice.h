@@ -107,8 +107,8 @@ public:
     // d2q9_SRT_drag, d2q9 and d2q9_drag: k (2 to TB_TILE/2 - 1) steps per
     // launch on tiles held in LDS, bit-identical to k single steps; 0 or 1:
     // off; -1 (the default of these models): chosen by timing (tune_blocking)
-    // on lattices under 2^23 nodes.  Steps with Globals and steps with
-    // stochastic rounding take the single step.
+    // on lattices under 2^23 nodes.  Steps with Globals take the single
+    // step.
     void set_temporal_blocking(int k);
     int temporal_blocking() const { return tb_k_; }
 
@@ -121,13 +121,13 @@ private:
     StepArgs args(int from, int to, bool glob) const;
     void upload_zone();
     void launch(int from, int to, bool glob, int step = 0);
-    void launch_blocked(int from, int to);
-    int blocked_k() const { return tb_k_ > 1 && !sr_ ? tb_k_ : 1; }
+    void launch_blocked(int from, int to, int step = 0);
+    int blocked_k() const { return tb_k_ > 1 ? tb_k_ : 1; }
     int graph_len() const;
     void drop_graph();
     void tune_rows();
     void tune_blocking();
-    bool blocking_to_tune() const { return tb_auto_ && !sr_ && N < (1u << 23); }
+    bool blocking_to_tune() const { return tb_auto_ && N < (1u << 23); }
     void run_graph_steps(int nsteps);
     void ensure_list();
 

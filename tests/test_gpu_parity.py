@@ -363,15 +363,23 @@ def test_temporal_blocking_is_bit_identical(model, storage):
     type and zone, lattices that are no multiple of the tile interior, one
     narrower and lower than a tile (its tiles wrap onto themselves), k = 2,
     3 and 7, replays of a captured loop of two blocked launches, leading and
-    trailing launches, a flag change.  The random BGK cases (Zou-He nodes
+    trailing launches, a flag change; the 16-bit formats also with
+    stochastic rounding (every launch direct, each step keyed by its own
+    iteration).  The random BGK cases (Zou-He nodes
     on random nodes) turn non-finite after some tens of steps on either
     path, so values compare with NaN equal to NaN, and the first
     comparisons must see a lattice at least half finite."""
-    for (nx, ny), k in (((75, 41), 3), ((20, 9), 2), ((130, 70), 7)):
-        say(f"temporal blocking {model} {storage} {nx}x{ny} k={k}")
+    runs = [((75, 41), 3, False), ((20, 9), 2, False), ((130, 70), 7, False)]
+    if storage in ("f16s", "bf16s"):
+        runs += [((75, 41), 3, True), ((130, 70), 7, True)]
+    for (nx, ny), k, sr in runs:
+        say(f"temporal blocking {model} {storage} {nx}x{ny} k={k} sr={sr}")
         case = random_case(model, nx, ny, 1, seed=41 + k)
         a = case.build(storage)
         b = case.build(storage)
+        if sr:
+            for lat in (a, b):
+                lat.set_stochastic_rounding(True, seed=77)
         a.temporal_blocking = k
         b.temporal_blocking = 0
         assert a.temporal_blocking == k and b.temporal_blocking == 0
@@ -399,10 +407,10 @@ def test_temporal_blocking_is_bit_identical(model, storage):
 def test_temporal_blocking_on_the_cylinder_channel():
     """The 1024 x 128 FP64 cylinder channel, 600 steps with 3 steps per
     launch, equals 600 single steps bit for bit (distributions, Drag and
-    Lift), and so does a lattice left to time its choice; stochastic
-    rounding takes the single step, so with it on the blocked and unblocked
-    f16s runs agree too.  Models outside the four 2D ones and step counts
-    outside -1 to 15 are refused."""
+    Lift), and so does a lattice left to time its choice; with stochastic
+    rounding, 5 f16s steps per launch equal single steps over 70 steps.
+    Models outside the four 2D ones and step counts outside -1 to 15 are
+    refused."""
     say("temporal blocking: cylinder channel 1024x128")
     case = cases.d2q9_cylinder(steps=0)
     a = case.build("f64")
