@@ -235,8 +235,9 @@ int tclbx_set_stochastic_rounding(tclbx_lattice* h, int on, unsigned seed);
  * (2-15) time steps per launch, each block stepping a 32 x 32 tile with
  * `steps` nodes of halo in LDS and writing its interior; 0 or 1 turns it
  * off; -1, the default of these models, lets a lattice under 2^23 nodes
- * time single steps against 3 and 5 per launch at tclbx_prepare or at its
- * first tclbx_iterate of 64 or more steps, and keep the fastest.  Results
+ * time single steps against 3 and 5 per launch on the first 270 plain steps
+ * of the first tclbx_iterate that runs that many (real steps of the run)
+ * and keep the fastest.  Results
  * are those of single steps, bit for bit, stochastic rounding included.
  * Steps with Globals take the single step.  It pays where one step does
  * not fill the GPU or the snapshots sit in the Infinity Cache, and loses

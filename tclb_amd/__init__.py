@@ -539,7 +539,8 @@ class Lattice:
     def temporal_blocking(self):
         """Plain steps per launch of the temporally blocked 2D step in use (0:
         single steps).  Set k >= 2 to fix it, 0 to turn it off, -1 to let
-        the lattice time 1, 3 and 5 steps per launch and keep the fastest
+        the lattice time 1, 3 and 5 steps per launch on the first 270 plain
+        steps of its first Iterate that runs that many and keep the fastest
         (the default of d2q9_SRT, d2q9_SRT_drag, d2q9 and d2q9_drag)."""
         k = ctypes.c_int()
         _check(lib().tclbx_get_temporal_blocking(self._h, ctypes.byref(k)))

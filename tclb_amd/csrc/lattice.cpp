@@ -326,49 +326,51 @@ void Lattice::set_temporal_blocking(int k) {
     drop_graph();
 }
 
-// Times single steps against launches of 3 and of 5 blocked steps and
-// keeps the fastest per step.  The GPU first runs about 2 ms of single
-// steps, so its clocks have risen before any sample: at the low clocks of
-// a GPU that has just idled, the compute-heavy FP64 blocked step slows more
-// than the latency-bound single step and lost samples it wins at speed.
-// Then three rounds over the candidates in turn, each sample about 0.5 ms
-// of steps, each candidate's best round counting.  Every trial launch reads
-// the current snapshot and writes the other one, which the next real step
-// overwrites: the state does not change.  Blocking wins where a lattice is
-// too small for one step to fill the GPU or its snapshots sit in the
-// Infinity Cache, and loses on lattices whose steps stream from HBM (the
-// halo is computed again, at lower occupancy); which k is best depends on
-// the storage format (the FP64 tile allows one block per CU) and on how the
-// tiles divide over the CUs.
-void Lattice::tune_blocking() {
+// Chooses the temporal blocking by timing real steps: the plain steps
+// [i, i + 270) of the current Iterate run in three rounds of 30 steps per
+// candidate (single steps, 3 and 5 per launch), each candidate's best
+// round counting, and the fastest per step is kept; returns the index of
+// the next step.  Every candidate gives the same bits, so the steps count
+// as the run's own.  (Trial steps that leave the state as it is, reading
+// one snapshot over and over, keep their input in the L2 and time the
+// single step at two thirds of its real cost on small lattices.)  Blocking
+// wins where a lattice is too small for one step to fill the GPU or its
+// snapshots sit in the Infinity Cache, and loses on lattices whose steps
+// stream from HBM (the halo is computed again, at lower occupancy); which
+// k is best depends on the storage format (the FP64 tile allows one block
+// per CU) and on how the tiles divide over the CUs.
+int Lattice::tune_blocking(int i) {
     tb_auto_ = false;
-    auto timed = [&](int k, int launches) {
-        tb_k_ = k;
-        hip_check(hipEventRecord(ev0_, stream_), "event record");
-        for (int j = 0; j < launches; j++) {
-            if (k > 1)
-                launch_blocked(snap_, 1 - snap_);
-            else
-                launch(snap_, 1 - snap_, false);
+    struct Events {
+        hipEvent_t e[2] = {nullptr, nullptr};
+        ~Events() {
+            for (hipEvent_t x : e)
+                if (x) (void)hipEventDestroy(x);
         }
-        hip_check(hipEventRecord(ev1_, stream_), "event record");
-        hip_check(hipStreamSynchronize(stream_), "tune blocking");
-        float ms = 0;
-        hip_check(hipEventElapsedTime(&ms, ev0_, ev1_), "event elapsed");
-        return ms;
-    };
-    // the single step's time sizes the warm-up and the samples
-    const float t1 = std::max(timed(0, 8) / 8.0f, 1e-4f);
-    auto count = [&](float ms) { return (int)std::min(16384.0f, std::max(8.0f, ms / t1)); };
-    timed(0, count(2.0f));
+    } ev;
+    hip_check(hipEventCreate(&ev.e[0]), "event create");
+    hip_check(hipEventCreate(&ev.e[1]), "event create");
     const int ks[3] = {0, 3, 5};
     float best[3] = {0, 0, 0};
-    for (int round = 0; round < 3; round++) {
+    for (int round = 0; round < TB_TUNE_ROUNDS; round++) {
         for (int c = 0; c < 3; c++) {
             const int k = ks[c];
-            const int steps = k > 1 ? k : 1;
-            const int launches = std::max(2, count(0.5f) / steps);
-            const float per_step = timed(k, launches) / (float)(launches * steps);
+            tb_k_ = k;
+            hip_check(hipEventRecord(ev.e[0], stream_), "event record");
+            for (int done = 0; done < TB_TUNE_SAMPLE;) {
+                if (k > 1)
+                    launch_blocked(snap_, 1 - snap_, iter_ + i);
+                else
+                    launch(snap_, 1 - snap_, false, iter_ + i);
+                snap_ = 1 - snap_;
+                i += k > 1 ? k : 1;
+                done += k > 1 ? k : 1;
+            }
+            hip_check(hipEventRecord(ev.e[1], stream_), "event record");
+            hip_check(hipEventSynchronize(ev.e[1]), "tune blocking");
+            float ms = 0;
+            hip_check(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]), "event elapsed");
+            const float per_step = ms / (float)TB_TUNE_SAMPLE;
             if (round == 0 || per_step < best[c]) best[c] = per_step;
         }
     }
@@ -376,7 +378,12 @@ void Lattice::tune_blocking() {
     for (int c = 1; c < 3; c++)
         if (best[c] < best[bi]) bi = c;
     tb_k_ = ks[bi];
+    if (const char* e = std::getenv("TCLBX_TUNE_LOG"); e && std::atoi(e) != 0)
+        std::fprintf(stderr, "tclbx: %s %dx%d %s, us per step: single %.3f, 3 per launch %.3f, 5 per launch %.3f -> %d\n",
+                     model->name.c_str(), nx, ny, storage_name(storage), 1e3 * best[0], 1e3 * best[1], 1e3 * best[2],
+                     tb_k_);
     drop_graph();
+    return i;
 }
 
 // steps of one replay of the captured loop: graph_steps_, or with temporal
@@ -619,7 +626,6 @@ void Lattice::Prepare() {
     upload_zone();
     ensure_list();
     if (tune_rpt_ && N >= (1u << 20)) tune_rows();
-    if (blocking_to_tune()) tune_blocking();
     if (graph_steps_ >= 2 && !sr_) {
         run_graph_steps(0);  // capture only
         hip_check(hipStreamSynchronize(stream_), "Prepare");
@@ -632,13 +638,11 @@ void Lattice::Iterate(int niter, int iter_type) {
     ensure_list();
     // lattices large enough for the tiling to matter are tuned once, on
     // the first Iterate that runs enough steps to repay the trial launches
-    // (or earlier, by Prepare); so is the temporal blocking of the 2D
-    // lattices it may pay on
+    // (or earlier, by Prepare)
     if (tune_rpt_ && niter >= 16 && N >= (1u << 20)) tune_rows();
     int last_glob = iter_type & ITER_LASTGLOB;
     if (iter_type & ITER_INTEG) last_glob = 0;
     const bool every_glob = (iter_type & ITER_GLOBS) != 0;
-    if (blocking_to_tune() && !every_glob && niter >= 64) tune_blocking();
     hip_check(hipEventRecord(ev0_, stream_), "event record");
     int i = 0;
     // leading plain steps until snapshot 0 is current, then whole graph
@@ -646,11 +650,14 @@ void Lattice::Iterate(int niter, int iter_type) {
     // a loop not yet captured is captured only when it will be replayed at
     // least twice
     const int plain_end = niter - (last_glob ? 1 : 0);
+    // the first Iterate with enough plain steps times the temporal blocking
+    // on them
+    if (blocking_to_tune() && !every_glob && plain_end >= TB_TUNE_STEPS) i = tune_blocking(i);
     const bool captured = graph_exec_ != nullptr && graph_version_ == params_version_;
     const int k = every_glob ? 1 : blocked_k();
     const int lead = snap_ != 0 ? k : 0;
     const int glen = graph_len();
-    if (!every_glob && !sr_ && graph_steps_ >= 2 && plain_end - lead >= (captured ? 1 : 2) * glen) {
+    if (!every_glob && !sr_ && graph_steps_ >= 2 && plain_end - i - lead >= (captured ? 1 : 2) * glen) {
         if (snap_ != 0) {
             if (k > 1)
                 launch_blocked(snap_, 1 - snap_, iter_ + i);

@@ -106,8 +106,9 @@ public:
     // temporal blocking of the plain steps of the 2D models d2q9_SRT,
     // d2q9_SRT_drag, d2q9 and d2q9_drag: k (2 to TB_TILE/2 - 1) steps per
     // launch on tiles held in LDS, bit-identical to k single steps; 0 or 1:
-    // off; -1 (the default of these models): chosen by timing (tune_blocking)
-    // on lattices under 2^23 nodes.  Steps with Globals take the single
+    // off; -1 (the default of these models): chosen on lattices under 2^23
+    // nodes by timing the first 270 plain steps of the first Iterate that
+    // runs that many (tune_blocking).  Steps with Globals take the single
     // step.
     void set_temporal_blocking(int k);
     int temporal_blocking() const { return tb_k_; }
@@ -126,7 +127,10 @@ private:
     int graph_len() const;
     void drop_graph();
     void tune_rows();
-    void tune_blocking();
+    int tune_blocking(int i);
+    // the timing of the temporal blocking: rounds x 3 candidates x steps
+    static constexpr int TB_TUNE_ROUNDS = 3, TB_TUNE_SAMPLE = 30;
+    static constexpr int TB_TUNE_STEPS = TB_TUNE_ROUNDS * 3 * TB_TUNE_SAMPLE;
     bool blocking_to_tune() const { return tb_auto_ && N < (1u << 23); }
     void run_graph_steps(int nsteps);
     void ensure_list();

@@ -5,9 +5,9 @@ Drag and Lift) stepped in one storage format.
 
 With --temporal-blocking K the plain steps run K per launch on LDS tiles
 (bit-identical to single steps; 0: single steps; -1, the default: the
-lattice times 1, 3 and 5 per launch in prepare() and keeps the fastest).
-`prepare()` (rows-per-tile and blocking timing, graph capture) and 512
-warm-up steps, then `steps` timed steps: wall-clock and HIP-event microseconds per step,
+lattice times 1, 3 and 5 per launch on the first 270 warm-up steps and
+keeps the fastest).  `prepare()` (rows-per-tile timing, graph capture) and
+512 warm-up steps, then `steps` timed steps: wall-clock and HIP-event microseconds per step,
 MLUPS, and the Drag and Lift of a final Globals step.  The environment
 variable TCLBX_GRAPH_STEPS sets the length of the captured step loops (0:
 every step launched directly).  Prints one JSON object.
