@@ -214,3 +214,15 @@ def test_density_plane_limit_boundary():
         tclb_amd.Lattice("d3q19", 812, 812, 812, "f64")
     assert "4 GiB" not in str(e.value)
 
+
+
+def test_temporal_blocking_entry_points_refuse_a_null_lattice():
+    """tclbx_set_temporal_blocking and tclbx_get_temporal_blocking check the
+    lattice handle before anything else (no GPU needed): an error with a
+    message, and the output left untouched."""
+    lib = tclb_amd.lib()
+    assert lib.tclbx_set_temporal_blocking(None, 3) != 0
+    assert "null lattice handle" in lib.tclbx_last_error().decode()
+    k = ctypes.c_int(-7)
+    assert lib.tclbx_get_temporal_blocking(None, ctypes.byref(k)) != 0
+    assert k.value == -7
